@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: PBF particle-updates/s on the 1,048,576-particle dam break (one MI355X).
+
+One step = one PBF substep of the whole scene (predict, grid hash, radix sort, reorder,
+4 Jacobi iterations, velocity update, vorticity confinement, XSPH, commit), i.e. one
+particle-update per particle.  Inputs are resident in HBM before the timed region; the timed
+region replays the captured substeps on the model's stream and is bracketed by stream
+synchronisation.
+
+Prints one JSON line.  Besides the throughput it carries
+  roofline     : the dominant kernel's algorithmic HBM bytes per launch / its mean launch
+                 time (HIP events around each launch on the model's stream, in a second timed
+                 window with per-kernel events), against MI355X's 8 TB/s;
+  cpu_baseline : the CPU oracle (the reference's kernels restated in C, OpenMP over
+                 particles) timed on this host on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--workload pbf-1m]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PBF particle-updates/s at 1M particles; learned vs plain at equal density error"
+
+
+def substep_bytes(name: str, n: int, nc: int, iters: int, vorticity: bool) -> float | None:
+    """Algorithmic (compulsory) HBM bytes one substep moves through kernel `name`.
+
+    Per particle, every stream that must cross HBM once (float4 16 B, float/uint 4 B);
+    neighbour re-reads are taken as cache hits and the cell table counts once (8 B/cell).
+    """
+    last_extra = 16 + 16 + (0 if vorticity else 16)  # R pos, W vel [, W pos]
+    per = {
+        "pbf_predict_hash": 16 * 2 + 16 + 4,               # R pos, vel; W predPos, key
+        "reorder_cells": 4 + 4 + 3 * (16 + 16),            # R key, idx; gather pos, vel, predPos
+        "pbf_density_lambda": iters * (16 + 4 + 4),        # R predPos; W density, lambda
+        "pbf_correct": iters * (16 + 4 + 16) + last_extra,  # R predPos, lambda; W predPos
+        "pbf_vorticity": 16 + 16 + 16,                     # R predPos, vel; W vorticity
+        "pbf_confinement": 16 * 4,                         # R predPos, vort, vel; W vel copy
+        "pbf_xsph": 16 * 4,                                # R predPos, vel copy; W vel, pos
+    }.get(name)
+    if per is None:
+        return None
+    launches = {"pbf_density_lambda": iters, "pbf_correct": iters}.get(name, 1)
+    return float(per * n + launches * nc * 8)
+
+
+def run_gpu(args, sc):
+    from realtimeparticles_amd import capi
+    prm = sc.params()
+    m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(**prm)
+    m.set_nb_jacobi_iters(sc.iters)
+    if args.block or args.xcd is not None or args.sort_bits:
+        m.set_tuning(args.block or 256, 1 if args.xcd is None else args.xcd, args.sort_bits or 0)
+    m.load_state(sc.pos, sc.vel)
+    m.sync()
+    # warmup (also captures the graph)
+    m.update(args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    m.update(args.steps)
+    m.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    err = m.density_error()
+    # second window: per-kernel HIP events on the model's stream (direct launches)
+    m.enable_profiling(True)
+    m.reset_profile()
+    prof_steps = max(2, min(args.steps, 40))
+    tp0 = time.perf_counter()
+    m.update(prof_steps)
+    m.sync()
+    tp1 = time.perf_counter()
+    prof = m.profile()
+    m.enable_profiling(False)
+    m.close()
+    return elapsed, err, prof, prof_steps, tp1 - tp0
+
+
+def run_cpu_baseline(sc, budget_s: float, max_steps: int):
+    from oracle import oracle as O
+    O.build()
+    prm = sc.params()
+    orc = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), sc.iters,
+                         gravity=sc.gravity)
+    orc.upload(sc.pos, sc.vel)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        orc.step(1)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    el = time.perf_counter() - t0
+    cores = O.num_threads()
+    return {
+        "value": sc.n * steps / el,
+        "unit": "particle-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{sc.name}: first {steps} substep(s) from the initial lattice "
+                  f"({el:.1f} s), OpenMP over particles on {cores} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="pbf-1m")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--xcd", type=int, default=None)
+    ap.add_argument("--sort-bits", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
+              f"{world}); a multi-GPU run is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from realtimeparticles_amd import scenes
+    sc = scenes.pbf_scene(args.workload)
+    nc = sc.res[0] * sc.res[1] * sc.res[2]
+
+    elapsed, err, prof, prof_steps, prof_elapsed = run_gpu(args, sc)
+    value = sc.n * args.steps / elapsed
+
+    # dominant kernel of the profiled window
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_ms) = dom
+    avg_ms = dom_ms / max(dom_launches, 1)
+    per_step = substep_bytes(dom_name, sc.n, nc, sc.iters, True)
+    launches_per_step = dom_launches / prof_steps
+    nbytes = None if per_step is None else per_step / launches_per_step
+    roofline = None
+    if nbytes is not None:
+        achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": None, "kernel": dom_name,
+                    "bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = run_cpu_baseline(sc, args.cpu_budget, 5)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{sc.name} dam break: {sc.lattice[0]}x{sc.lattice[1]}x{sc.lattice[2]} "
+                        f"jittered lattice ({sc.n} particles), box {sc.box}, grid {sc.res}, "
+                        f"h=0.1, {sc.iters} Jacobi iterations, substep dt=1/120, plain solver",
+            "particles": sc.n,
+            "substeps_per_frame": 2,
+            "params": sc.params(),
+            "density_error_after_run": {"max": float(err[0]), "mean": float(err[1]),
+                                        "max_compression": float(err[2])},
+            "kernel_ms_per_step": {k: round(v[1] / prof_steps, 5) for k, v in sorted(prof.items())},
+            "profiled_ms_per_step": round(prof_elapsed * 1e3 / prof_steps, 4),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

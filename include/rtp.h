@@ -1,0 +1,190 @@
+/*
+ * rtp.h — C ABI of the MI355X (gfx950) particle-physics backend.
+ *
+ * This library replaces the OpenCL path behind the reference's Physics::Model interface
+ * (physics/Model.hpp:81-216) for the two models on the hot path: position based fluids
+ * (physics/ocl/Fluids.{hpp,cpp}) and 3-D boids (physics/ocl/Boids.{hpp,cpp}).  Each entry
+ * point below names the reference interface it stands in for.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative rtp_status; rtp_last_error()
+ *     returns a text for the last failure on the calling thread (the reference's LOG_ERROR);
+ *   - no C++ exception crosses this boundary;
+ *   - the library owns device memory; host arrays are caller-owned and copied;
+ *   - particle arrays are float4 (x, y, z, w) laid out as 4 consecutive floats; the w lane is
+ *     carried as 0, as the reference's initialisation sets it (Fluids.cpp:383-387);
+ *   - one HIP stream per handle; a handle is not thread-safe (callers serialise per handle,
+ *     as the reference's single SDL thread does).
+ */
+#ifndef RTP_H
+#define RTP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTP_ABI_VERSION 1
+
+typedef enum {
+  RTP_OK = 0,
+  RTP_ERR_INVALID = -1,  /* bad argument / size */
+  RTP_ERR_HIP = -2,      /* HIP runtime failure */
+  RTP_ERR_NO_DEVICE = -3,
+  RTP_ERR_STATE = -4     /* call not valid in the handle's current state */
+} rtp_status;
+
+/* physics/ocl/Fluids.hpp:17-32 FluidKernelInputs == define.cl:13-25 FluidParams (44 bytes). */
+typedef struct {
+  float restDensity;
+  float relaxCFM;
+  float timeStep;
+  uint32_t dim;
+  uint32_t isArtPressureEnabled;
+  float artPressureRadius;
+  float artPressureCoeff;
+  uint32_t artPressureExp;
+  uint32_t isVorticityConfEnabled;
+  float vorticityConfCoeff;
+  float xsphViscosityCoeff;
+} rtp_fluid_params;
+
+/* physics/ocl/Boids.hpp:14-20 BoidsRuleKernelInputs == boids.cl:12-17 BoidsRuleParams. */
+typedef struct {
+  float velocityScale;
+  float alignmentScale;
+  float separationScale;
+  float cohesionScale;
+} rtp_boids_params;
+
+/* physics/Model.hpp:60-73 ModelParams, minus the GL VBO ids (the library owns the buffers,
+ * see rtp_*_device_state).  boxSize is float (the reference's BoxSize3D is size_t,
+ * utils/Geometry.hpp:42-47); integral values reproduce the reference exactly. */
+typedef struct {
+  uint32_t maxNbParticles;  /* ModelParams.maxNbParticles */
+  uint32_t currNbParticles; /* ModelParams.currNbParticles */
+  float boxSize[3];         /* ModelParams.boxSize */
+  int32_t gridRes[3];       /* ModelParams.gridRes */
+  uint32_t maxPartsInCell;  /* m_maxNbPartsInCell: 0 = model default (100 fluids
+                               Fluids.cpp:79, 3000 boids Boids.cpp:78) */
+  float gravity;            /* ABS_GRAVITY_ACC_Y (define.cl:8); 0 = reference's 9.81 */
+  int32_t device;           /* HIP device ordinal */
+} rtp_model_desc;
+
+/* Per-kernel timing record (CL::Context::runKernel profiling, Context.cpp:692-705). */
+typedef struct {
+  char name[48];
+  uint32_t launches;
+  double totalMs;
+} rtp_kernel_time;
+
+/* Which buffers rtp_fluids_read_field copies. */
+typedef enum {
+  RTP_FIELD_DENSITY = 0,   /* p_density  (float, last Jacobi iteration) */
+  RTP_FIELD_LAMBDA = 1,    /* p_constFactor (float) */
+  RTP_FIELD_PRED_POS = 2,  /* p_predPos (float4, final) */
+  RTP_FIELD_VORTICITY = 3, /* p_vort (float4; w = |vorticity|) */
+  RTP_FIELD_ACC = 4        /* boids p_acc (float4) */
+} rtp_field;
+
+/* ---- library ---- */
+int rtp_abi_version(void);
+/* Text of the last error on this thread ("" if none). */
+const char* rtp_last_error(void);
+/* Number of visible HIP devices. */
+int rtp_device_count(int* count);
+
+/* ======================= Position based fluids ======================= */
+typedef struct rtp_fluids rtp_fluids;
+
+/* Fluids::Fluids(ModelParams) (physics/ocl/Fluids.cpp:76-92) via Physics::CreateModel
+ * (physics/Model.cpp:10-24): program constants (Fluids.cpp:97-126) and buffers (:128-150). */
+int rtp_fluids_create(const rtp_model_desc* desc, rtp_fluids** out);
+/* Fluids::~Fluids / OclModel::~OclModel (physics/ocl/OclModel.hpp:23-26). */
+int rtp_fluids_destroy(rtp_fluids* h);
+/* Fluids::transferKernelInputsToGPU (Fluids.cpp:253-271). */
+int rtp_fluids_set_params(rtp_fluids* h, const rtp_fluid_params* p);
+int rtp_fluids_get_params(rtp_fluids* h, rtp_fluid_params* p);
+/* m_nbJacobiIters, set from "Nb Jacobi Iterations" (Fluids.cpp:227). */
+int rtp_fluids_set_nb_jacobi_iters(rtp_fluids* h, uint32_t iters);
+/* Model::setNbParticles (physics/Model.hpp:106). */
+int rtp_fluids_set_nb_particles(rtp_fluids* h, uint32_t n);
+int rtp_fluids_get_nb_particles(rtp_fluids* h, uint32_t* n);
+/* Fluids::initFluidsParticles (Fluids.cpp:383-395) + resetCellIDs (:214): count float4
+ * positions/velocities; entries count..maxN-1 become (inf,inf,inf,0) and 0. */
+int rtp_fluids_load_state(rtp_fluids* h, const float* pos4, const float* vel4, uint32_t count);
+/* Read back p_pos / p_vel (the GL VBO contents in the reference); either may be NULL. */
+int rtp_fluids_read_state(rtp_fluids* h, float* pos4, float* vel4, uint32_t count);
+/* Fluids::update (Fluids.cpp:400-471) repeated nsteps times, enqueued on the handle's
+ * stream; returns once enqueued (rtp_fluids_sync waits, like Context::finishTasks). */
+int rtp_fluids_update(rtp_fluids* h, uint32_t nsteps);
+/* Model::pause (physics/Model.hpp:127): while paused, update() does nothing. */
+int rtp_fluids_pause(rtp_fluids* h, int pause);
+int rtp_fluids_sync(rtp_fluids* h);
+/* p_cellID (maxN), RadixSortIndices (maxN) and c_startEndPartID (2*NC) after the last
+ * update; any pointer may be NULL. */
+int rtp_fluids_read_grid(rtp_fluids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
+/* Copy one per-particle field (see rtp_field) for the first count particles. */
+int rtp_fluids_read_field(rtp_fluids* h, int field, float* out, uint32_t count);
+/* Density error of the last update over active particles, from p_density:
+ * out[0] = max |rho/rho0 - 1|, out[1] = mean |rho/rho0 - 1|, out[2] = max(rho/rho0 - 1, 0). */
+int rtp_fluids_density_error(rtp_fluids* h, float out[3]);
+/* Device pointers of the current float4 position/velocity buffers (the role of the
+ * reference's GL VBOs, physics/ocl/Fluids.cpp:132-134); valid until the next update. */
+int rtp_fluids_device_state(rtp_fluids* h, void** pos4, void** vel4);
+/* The handle's HIP stream (hipStream_t). */
+int rtp_fluids_stream(rtp_fluids* h, void** stream);
+/* Model::enableProfiling / isProfilingEnabled (physics/Model.hpp:150-151): per-kernel
+ * HIP-event timing on the handle's stream. */
+int rtp_fluids_enable_profiling(rtp_fluids* h, int enable);
+int rtp_fluids_profile(rtp_fluids* h, rtp_kernel_time* out, int capacity, int* count);
+int rtp_fluids_reset_profile(rtp_fluids* h);
+/* Replay pairs of substeps from a captured hipGraph (no reference counterpart; the
+ * reference enqueues every kernel each frame).  Default on. */
+int rtp_fluids_use_graph(rtp_fluids* h, int enable);
+/* Launch tuning (no reference counterpart): block size of the neighbour passes (64..1024,
+ * multiple of 64), XCD-contiguous block order (0/1), sort digit width (0 = auto, 4..11). */
+int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBits);
+
+/* ============================== Boids ================================ */
+typedef struct rtp_boids rtp_boids;
+
+/* Boids::Boids(ModelParams) (physics/ocl/Boids.cpp:75-91), constants Boids.cpp:96-120. */
+int rtp_boids_create(const rtp_model_desc* desc, rtp_boids** out);
+int rtp_boids_destroy(rtp_boids* h);
+/* Boids::transferKernelInputsToGPU (Boids.cpp:212-226). */
+int rtp_boids_set_params(rtp_boids* h, const rtp_boids_params* p);
+/* The update time step; the reference hard-codes 0.1 (Boids.cpp:334). */
+int rtp_boids_set_time_step(rtp_boids* h, float dt);
+int rtp_boids_set_nb_particles(rtp_boids* h, uint32_t n);
+/* Boids::initBoidsParticles (Boids.cpp:310-318) + resetCellIDs (:271). */
+int rtp_boids_load_state(rtp_boids* h, const float* pos4, const float* vel4, uint32_t count);
+int rtp_boids_read_state(rtp_boids* h, float* pos4, float* vel4, uint32_t count);
+/* Boids::update (Boids.cpp:323-384), 3-D rules and bouncing wall, nsteps times. */
+int rtp_boids_update(rtp_boids* h, uint32_t nsteps);
+int rtp_boids_pause(rtp_boids* h, int pause);
+int rtp_boids_sync(rtp_boids* h);
+int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
+int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count);
+int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4);
+int rtp_boids_enable_profiling(rtp_boids* h, int enable);
+int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* count);
+
+/* ========================= Grid building blocks ======================== */
+/* RadixSort::sort on one key buffer (physics/utils/RadixSort.cpp:122-161): stable
+ * ascending sort of n 32-bit keys with maxKey >= every key; permOut[i] is the original
+ * position of the key now at i.  Host arrays in and out (for tests and tools). */
+int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxKey,
+                   uint32_t* keysOut, uint32_t* permOut);
+/* The grid part of Fluids::update for one batch of positions (Fluids.cpp:415-424):
+ * fillCellIDs over n, sort of all maxN keys (tail keys 2*NC+i as after resetCellIDs), cell
+ * start/end with the cap.  keys/idx are maxN long, startEnd 2*NC. */
+int rtp_grid_build(const rtp_model_desc* desc, const float* pos4, uint32_t* keys, uint32_t* idx,
+                   uint32_t* startEnd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTP_H */

@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU oracle (oracle/rtp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by the
+CPU-baseline leg of ``bench.py``.  The HIP library never routes through this module.
+
+The oracle restates the reference's OpenCL kernels in C; this module owns the numpy
+arrays that stand for the reference's OpenCL buffers (physics/ocl/Fluids.cpp:128-150,
+physics/ocl/Boids.cpp:122-139) and drives ``orc_fluids_update`` / ``orc_boids_update``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "librtp_oracle.so")
+
+_u32 = ctypes.c_uint32
+_f32 = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+class Consts(ctypes.Structure):
+    _fields_ = [
+        ("effectRadius", _f32),
+        ("effectRadiusSq", _f32),
+        ("absWall", _f32 * 3),
+        ("res", ctypes.c_int32 * 3),
+        ("cellSize", _f32),
+        ("numCells", _u32),
+        ("maxPartsInCell", _u32),
+        ("poly6Coeff", _f32),
+        ("spikyCoeff", _f32),
+        ("maxVel", _f32),
+        ("gravity", _f32),
+    ]
+
+
+class FluidParams(ctypes.Structure):
+    """physics/ocl/kernels/define.cl:13-25 (FluidParams, 44 bytes)."""
+
+    _fields_ = [
+        ("restDensity", _f32),
+        ("relaxCFM", _f32),
+        ("timeStep", _f32),
+        ("dim", _u32),
+        ("isArtPressureEnabled", _u32),
+        ("artPressureRadius", _f32),
+        ("artPressureCoeff", _f32),
+        ("artPressureExp", _u32),
+        ("isVorticityConfEnabled", _u32),
+        ("vorticityConfCoeff", _f32),
+        ("xsphViscosityCoeff", _f32),
+    ]
+
+
+class BoidsParams(ctypes.Structure):
+    """physics/ocl/kernels/boids.cl:12-17 (BoidsRuleParams)."""
+
+    _fields_ = [
+        ("velocityScale", _f32),
+        ("alignmentScale", _f32),
+        ("separationScale", _f32),
+        ("cohesionScale", _f32),
+    ]
+
+
+class _FluidsState(ctypes.Structure):
+    _fields_ = [
+        ("c", Consts),
+        ("p", FluidParams),
+        ("maxN", _u32),
+        ("n", _u32),
+        ("iters", _u32),
+        ("pos", _vp), ("vel", _vp), ("pred", _vp), ("corr", _vp), ("vort", _vp),
+        ("velIn", _vp), ("tmp", _vp),
+        ("density", _vp), ("lambda_", _vp),
+        ("key", _vp), ("idx", _vp),
+        ("se", _vp),
+    ]
+
+
+class _BoidsState(ctypes.Structure):
+    _fields_ = [
+        ("c", Consts),
+        ("p", BoidsParams),
+        ("timeStep", _f32),
+        ("maxN", _u32),
+        ("n", _u32),
+        ("pos", _vp), ("vel", _vp), ("acc", _vp), ("tmp", _vp),
+        ("key", _vp), ("idx", _vp),
+        ("se", _vp),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no fast-math, no FP contraction)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.orc_float_roundtrip.restype = _f32
+        L.orc_float_roundtrip.argtypes = [_f32]
+        L.orc_make_consts.argtypes = [P(_f32), P(ctypes.c_int32), _u32, _f32, P(Consts)]
+        L.orc_pown.restype = _f32
+        L.orc_pown.argtypes = [_f32, _u32]
+        L.orc_poly6.restype = _f32
+        L.orc_poly6.argtypes = [_vp, P(Consts)]
+        L.orc_grad_spiky.argtypes = [_vp, P(Consts), _vp]
+        L.orc_art_pressure.restype = _f32
+        L.orc_art_pressure.argtypes = [_vp, P(FluidParams), P(Consts)]
+        L.orc_cell3d.argtypes = [_vp, P(Consts), _vp]
+        L.orc_reset_cell_ids.argtypes = [_vp, _u32, P(Consts)]
+        L.orc_fill_cell_ids.argtypes = [_vp, _vp, _u32, P(Consts)]
+        L.orc_reset_start_end.argtypes = [_vp, P(Consts)]
+        L.orc_fill_start.argtypes = [_vp, _vp, _u32, P(Consts)]
+        L.orc_fill_end.argtypes = [_vp, _vp, _u32, _u32, P(Consts)]
+        L.orc_adjust_end.argtypes = [_vp, P(Consts)]
+        L.orc_radix_sort.restype = ctypes.c_int
+        L.orc_radix_sort.argtypes = [_vp, _vp, _u32]
+        L.orc_permute_f4.argtypes = [_vp, _vp, _vp, _u32]
+        for name in ("orc_fld_density",):
+            getattr(L, name).argtypes = [_vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_predict.argtypes = [_vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_boundary.argtypes = [_vp, _u32, P(Consts)]
+        L.orc_fld_lambda.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_correction.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_correct_pos.argtypes = [_vp, _vp, _u32]
+        L.orc_fld_update_vel.argtypes = [_vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_vorticity.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_confinement.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_xsph.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fld_update_pos.argtypes = [_vp, _vp, _u32]
+        L.orc_bd_rules3d.argtypes = [_vp, _vp, _vp, P(BoidsParams), _vp, _u32, P(Consts)]
+        L.orc_bd_update_vel.argtypes = [_vp, _f32, _f32, _vp, _u32]
+        L.orc_bd_update_pos_wall.argtypes = [_vp, _f32, _vp, _u32, P(Consts)]
+        L.orc_fluids_update.restype = ctypes.c_int
+        L.orc_fluids_update.argtypes = [P(_FluidsState)]
+        L.orc_boids_update.restype = ctypes.c_int
+        L.orc_boids_update.argtypes = [P(_BoidsState)]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def make_consts(box, res, max_parts_in_cell: int, gravity: float = 9.81) -> Consts:
+    c = Consts()
+    b = (_f32 * 3)(*[float(x) for x in box])
+    r = (ctypes.c_int32 * 3)(*[int(x) for x in res])
+    lib().orc_make_consts(b, r, int(max_parts_in_cell), float(gravity), ctypes.byref(c))
+    return c
+
+
+def fluid_params(**kw) -> FluidParams:
+    """FluidKernelInputs defaults (physics/ocl/Fluids.hpp:17-32), overridable by keyword."""
+    p = FluidParams(restDensity=450.0, relaxCFM=600.0, timeStep=0.010, dim=3,
+                    isArtPressureEnabled=1, artPressureRadius=0.006, artPressureCoeff=0.001,
+                    artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0004,
+                    xsphViscosityCoeff=0.0001)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+class OracleFluids:
+    """Fluids::update (physics/ocl/Fluids.cpp:400-471) on numpy-owned buffers."""
+
+    def __init__(self, box, res, max_n: int, n: int, params: FluidParams, iters: int,
+                 gravity: float = 9.81, max_parts_in_cell: int = 100):
+        self.c = make_consts(box, res, max_parts_in_cell, gravity)
+        nc = int(self.c.numCells)
+        self.max_n, self.n = int(max_n), int(n)
+        f4 = lambda: np.zeros((self.max_n, 4), np.float32)  # noqa: E731
+        self.pos, self.vel, self.pred, self.corr = f4(), f4(), f4(), f4()
+        self.vort, self.vel_in, self.tmp = f4(), f4(), f4()
+        self.density = np.zeros(self.max_n, np.float32)
+        self.lam = np.zeros(self.max_n, np.float32)
+        self.key = np.zeros(self.max_n, np.uint32)
+        self.idx = np.zeros(self.max_n, np.uint32)
+        self.se = np.zeros((nc, 2), np.uint32)
+        self.s = _FluidsState()
+        self.s.c = self.c
+        self.s.maxN, self.s.n, self.s.iters = self.max_n, self.n, int(iters)
+        self.set_params(params)
+        for name, arr in (("pos", self.pos), ("vel", self.vel), ("pred", self.pred),
+                          ("corr", self.corr), ("vort", self.vort), ("velIn", self.vel_in),
+                          ("tmp", self.tmp), ("density", self.density), ("lambda_", self.lam),
+                          ("key", self.key), ("idx", self.idx), ("se", self.se)):
+            setattr(self.s, name, ptr(arr))
+
+    def set_params(self, params: FluidParams) -> None:
+        self.p = params
+        self.s.p = params
+
+    def set_iters(self, iters: int) -> None:
+        self.s.iters = int(iters)
+
+    def upload(self, pos: np.ndarray, vel: np.ndarray) -> None:
+        """Fluids::initFluidsParticles (Fluids.cpp:383-395) + resetCellIDs (:214)."""
+        m = pos.shape[0]
+        self.pos[:] = (np.inf, np.inf, np.inf, 0.0)
+        self.vel[:] = 0.0
+        self.pos[:m] = pos
+        self.vel[:m] = vel
+        lib().orc_reset_cell_ids(ptr(self.key), self.max_n, ctypes.byref(self.c))
+
+    def step(self, nsteps: int = 1) -> None:
+        for _ in range(nsteps):
+            rc = lib().orc_fluids_update(ctypes.byref(self.s))
+            if rc:
+                raise RuntimeError(f"orc_fluids_update failed ({rc})")
+
+
+class OracleBoids:
+    """Boids::update (physics/ocl/Boids.cpp:323-384) on numpy-owned buffers."""
+
+    def __init__(self, box, res, max_n: int, n: int, params: BoidsParams, time_step: float = 0.1,
+                 max_parts_in_cell: int = 3000):
+        self.c = make_consts(box, res, max_parts_in_cell, 9.81)
+        nc = int(self.c.numCells)
+        self.max_n, self.n = int(max_n), int(n)
+        f4 = lambda: np.zeros((self.max_n, 4), np.float32)  # noqa: E731
+        self.pos, self.vel, self.acc, self.tmp = f4(), f4(), f4(), f4()
+        self.key = np.zeros(self.max_n, np.uint32)
+        self.idx = np.zeros(self.max_n, np.uint32)
+        self.se = np.zeros((nc, 2), np.uint32)
+        self.s = _BoidsState()
+        self.s.c = self.c
+        self.s.p = params
+        self.s.timeStep = float(time_step)
+        self.s.maxN, self.s.n = self.max_n, self.n
+        for name, arr in (("pos", self.pos), ("vel", self.vel), ("acc", self.acc),
+                          ("tmp", self.tmp), ("key", self.key), ("idx", self.idx), ("se", self.se)):
+            setattr(self.s, name, ptr(arr))
+
+    def upload(self, pos: np.ndarray, vel: np.ndarray) -> None:
+        """Boids::initBoidsParticles (Boids.cpp:310-318) + resetCellIDs (:271)."""
+        m = pos.shape[0]
+        self.pos[:] = (np.inf, np.inf, np.inf, 0.0)
+        self.vel[:] = (np.inf, np.inf, np.inf, 0.0)
+        self.acc[:] = 0.0
+        self.pos[:m] = pos
+        self.vel[:m] = vel
+        lib().orc_reset_cell_ids(ptr(self.key), self.max_n, ctypes.byref(self.c))
+
+    def step(self, nsteps: int = 1) -> None:
+        for _ in range(nsteps):
+            rc = lib().orc_boids_update(ctypes.byref(self.s))
+            if rc:
+                raise RuntimeError(f"orc_boids_update failed ({rc})")

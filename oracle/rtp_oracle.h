@@ -1,0 +1,159 @@
+/*
+ * CPU oracle for the RealTimeParticles physics hot path — TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of the reference's OpenCL kernels and host step orders
+ * (physics/ocl/kernels/{define,sph,grid,radixSort,fluids,boids}.cl,
+ *  physics/ocl/Fluids.cpp:97-126,400-471, physics/ocl/Boids.cpp:96-120,323-384,
+ *  physics/utils/RadixSort.cpp:122-189).  Only the test suite, the smoke check and the
+ * benchmark's CPU-baseline leg may load it; the HIP library never links or calls it.
+ *
+ * Each kernel is one C function that loops over the reference's work-items.  The
+ * arithmetic follows the OpenCL expressions as written (left-to-right, no contraction:
+ * build with -ffp-contract=off).  The OpenCL builtins whose precision the reference
+ * leaves to the vendor are pinned as follows (all within the OpenCL 1.2 error bounds):
+ *   dot(a,b)        = ((a.x*b.x + a.y*b.y) + a.z*b.z) + a.w*b.w
+ *   length/fast_length(v)       = sqrtf(dot(v,v))           (IEEE sqrt)
+ *   normalize/fast_normalize(v) = v / sqrtf(dot(v,v)), and 0 for the zero vector
+ *   pow(x, n) for integral n    = ((x*x)*x)*...  (n-1 left-to-right products; 1 for n==0)
+ *   clamp(x,lo,hi)  = fminf(fmaxf(x,lo),hi);  step(e,x) = x < e ? 0 : 1
+ *   division        = IEEE division (no -cl-fast-relaxed-math reciprocal)
+ *   denormals       = IEEE (not flushed)
+ */
+#ifndef RTP_ORACLE_H
+#define RTP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y, z, w; } orc_f4;
+typedef struct { uint32_t x, y; } orc_u2;
+
+/* physics/ocl/kernels/define.cl:13-25 — FluidParams, 44 bytes, passed by value. */
+typedef struct {
+  float restDensity;
+  float relaxCFM;
+  float timeStep;
+  uint32_t dim;
+  uint32_t isArtPressureEnabled;
+  float artPressureRadius;
+  float artPressureCoeff;
+  uint32_t artPressureExp;
+  uint32_t isVorticityConfEnabled;
+  float vorticityConfCoeff;
+  float xsphViscosityCoeff;
+} orc_fluid_params;
+
+/* physics/ocl/kernels/boids.cl:12-17 — BoidsRuleParams, 16 bytes. */
+typedef struct {
+  float velocityScale;
+  float alignmentScale;
+  float separationScale;
+  float cohesionScale;
+} orc_boids_params;
+
+/* The compile-time "-D" constants of the reference programs (Fluids.cpp:104-119,
+ * Boids.cpp:104-113) after their FloatToStr round trip (Utils.cpp:24-29). */
+typedef struct {
+  float effectRadius;     /* EFFECT_RADIUS          (fluids) */
+  float effectRadiusSq;   /* EFFECT_RADIUS_SQUARED  (boids)  */
+  float absWall[3];       /* ABS_WALL_X/Y/Z */
+  int32_t res[3];         /* GRID_RES_X/Y/Z */
+  float cellSize;         /* GRID_CELL_SIZE_XYZ */
+  uint32_t numCells;      /* GRID_NUM_CELLS */
+  uint32_t maxPartsInCell;/* NUM_MAX_PARTS_IN_CELL */
+  float poly6Coeff;       /* POLY6_COEFF */
+  float spikyCoeff;       /* SPIKY_COEFF */
+  float maxVel;           /* MAX_VEL */
+  float gravity;          /* ABS_GRAVITY_ACC_Y (define.cl:8 literal, 9.81f in the reference) */
+} orc_consts;
+
+/* FloatToStr(v, 10) followed by the OpenCL compiler's parse of the literal. */
+float orc_float_roundtrip(float v);
+/* Fluids.cpp:104-119 / Boids.cpp:104-113 with box given as floats (integers in the reference). */
+void orc_make_consts(const float box[3], const int32_t res[3], uint32_t maxPartsInCell,
+                     float gravity, orc_consts* out);
+
+/* ---- pinned builtins exposed for the pinning tests ---- */
+float orc_pown(float x, uint32_t n);
+float orc_poly6(const orc_f4* vec, const orc_consts* c);
+void orc_grad_spiky(const orc_f4* vec, const orc_consts* c, orc_f4* out);
+float orc_art_pressure(const orc_f4* vec, const orc_fluid_params* p, const orc_consts* c);
+
+/* ---- grid.cl ---- */
+void orc_cell3d(const orc_f4* pos, const orc_consts* c, uint32_t out[3]);
+void orc_reset_cell_ids(uint32_t* key, uint32_t maxN, const orc_consts* c);
+void orc_fill_cell_ids(const orc_f4* pos, uint32_t* key, uint32_t n, const orc_consts* c);
+void orc_reset_start_end(orc_u2* se, const orc_consts* c);
+void orc_fill_start(const uint32_t* key, orc_u2* se, uint32_t n, const orc_consts* c);
+void orc_fill_end(const uint32_t* key, orc_u2* se, uint32_t n, uint32_t keyLen, const orc_consts* c);
+void orc_adjust_end(orc_u2* se, const orc_consts* c);
+
+/* ---- radixSort.cl + RadixSort.cpp ---- */
+/* Returns 0, or -1 when maxN is not a multiple of 512 (RadixSort.cpp:34-35 precondition). */
+int orc_radix_sort(uint32_t* key, uint32_t* idx, uint32_t maxN);
+void orc_permute_f4(const uint32_t* idx, orc_f4* buf, orc_f4* tmp, uint32_t maxN);
+
+/* ---- fluids.cl (one function per kernel, work-items 0..n-1) ---- */
+void orc_fld_predict(const orc_f4* pos, const orc_f4* vel, const orc_fluid_params* p,
+                     orc_f4* pred, uint32_t n, const orc_consts* c);
+void orc_fld_boundary(orc_f4* pred, uint32_t n, const orc_consts* c);
+void orc_fld_density(const orc_f4* pred, const orc_u2* se, const orc_fluid_params* p,
+                     float* density, uint32_t n, const orc_consts* c);
+void orc_fld_lambda(const orc_f4* pred, const float* density, const orc_u2* se,
+                    const orc_fluid_params* p, float* lambda, uint32_t n, const orc_consts* c);
+void orc_fld_correction(const float* lambda, const orc_u2* se, const orc_f4* pred,
+                        const orc_fluid_params* p, orc_f4* corr, uint32_t n, const orc_consts* c);
+void orc_fld_correct_pos(const orc_f4* corr, orc_f4* pred, uint32_t n);
+void orc_fld_update_vel(const orc_f4* pred, const orc_f4* pos, const orc_fluid_params* p,
+                        orc_f4* vel, uint32_t n, const orc_consts* c);
+void orc_fld_vorticity(const orc_f4* pred, const orc_u2* se, const orc_f4* vel,
+                       const orc_fluid_params* p, orc_f4* vort, uint32_t n, const orc_consts* c);
+void orc_fld_confinement(const orc_f4* pred, const orc_u2* se, const orc_f4* vort,
+                         const orc_fluid_params* p, orc_f4* vel, uint32_t n, const orc_consts* c);
+void orc_fld_xsph(const orc_f4* pred, const orc_u2* se, const orc_f4* velIn,
+                  const orc_fluid_params* p, orc_f4* velOut, uint32_t n, const orc_consts* c);
+void orc_fld_update_pos(const orc_f4* pred, orc_f4* pos, uint32_t n);
+
+/* ---- boids.cl ---- */
+void orc_bd_rules3d(const orc_f4* pos, const orc_f4* vel, const orc_u2* se,
+                    const orc_boids_params* p, orc_f4* acc, uint32_t n, const orc_consts* c);
+void orc_bd_update_vel(const orc_f4* acc, float dt, float maxVel, orc_f4* vel, uint32_t n);
+void orc_bd_update_pos_wall(orc_f4* vel, float dt, orc_f4* pos, uint32_t n, const orc_consts* c);
+
+/* ---- host step drivers (caller-owned arrays, all maxN long except se = numCells) ---- */
+typedef struct {
+  orc_consts c;
+  orc_fluid_params p;
+  uint32_t maxN, n, iters;
+  orc_f4 *pos, *vel, *pred, *corr, *vort, *velIn, *tmp;
+  float *density, *lambda;
+  uint32_t *key, *idx;
+  orc_u2* se;
+} orc_fluids;
+
+typedef struct {
+  orc_consts c;
+  orc_boids_params p;
+  float timeStep;
+  uint32_t maxN, n;
+  orc_f4 *pos, *vel, *acc, *tmp;
+  uint32_t *key, *idx;
+  orc_u2* se;
+} orc_boids;
+
+/* Fluids::update (Fluids.cpp:400-471) without GL, colour or the camera-distance sort. */
+int orc_fluids_update(orc_fluids* s);
+/* Boids::update (Boids.cpp:323-384) with the bouncing wall, without GL or target. */
+int orc_boids_update(orc_boids* s);
+
+/* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,406 @@
+"""ctypes binding of librtp (include/rtp.h) with classes shaped like the reference's
+Physics::Model implementations (physics/ocl/Fluids.hpp, physics/ocl/Boids.hpp).
+
+There is no CPU fallback: if librtp.so is missing or no GPU is visible, the constructors
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "librtp.so")
+
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int32
+_f32 = ctypes.c_float
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+
+FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
+
+
+class RtpError(RuntimeError):
+    pass
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("maxNbParticles", _u32),
+        ("currNbParticles", _u32),
+        ("boxSize", _f32 * 3),
+        ("gridRes", _i32 * 3),
+        ("maxPartsInCell", _u32),
+        ("gravity", _f32),
+        ("device", _i32),
+    ]
+
+
+class FluidParams(ctypes.Structure):
+    """rtp_fluid_params == FluidKernelInputs (physics/ocl/Fluids.hpp:17-32)."""
+
+    _fields_ = [
+        ("restDensity", _f32),
+        ("relaxCFM", _f32),
+        ("timeStep", _f32),
+        ("dim", _u32),
+        ("isArtPressureEnabled", _u32),
+        ("artPressureRadius", _f32),
+        ("artPressureCoeff", _f32),
+        ("artPressureExp", _u32),
+        ("isVorticityConfEnabled", _u32),
+        ("vorticityConfCoeff", _f32),
+        ("xsphViscosityCoeff", _f32),
+    ]
+
+
+class BoidsParams(ctypes.Structure):
+    """rtp_boids_params == BoidsRuleKernelInputs (physics/ocl/Boids.hpp:14-20)."""
+
+    _fields_ = [
+        ("velocityScale", _f32),
+        ("alignmentScale", _f32),
+        ("separationScale", _f32),
+        ("cohesionScale", _f32),
+    ]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", _u32), ("totalMs", ctypes.c_double)]
+
+
+FLUID_DEFAULTS = dict(restDensity=450.0, relaxCFM=600.0, timeStep=0.010, dim=3,
+                      isArtPressureEnabled=1, artPressureRadius=0.006, artPressureCoeff=0.001,
+                      artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0004,
+                      xsphViscosityCoeff=0.0001)
+
+_SIGS = {
+    "rtp_abi_version": ([], _int),
+    "rtp_last_error": ([], ctypes.c_char_p),
+    "rtp_device_count": ([ctypes.POINTER(_int)], _int),
+    "rtp_fluids_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], _int),
+    "rtp_fluids_destroy": ([_vp], _int),
+    "rtp_fluids_set_params": ([_vp, ctypes.POINTER(FluidParams)], _int),
+    "rtp_fluids_get_params": ([_vp, ctypes.POINTER(FluidParams)], _int),
+    "rtp_fluids_set_nb_jacobi_iters": ([_vp, _u32], _int),
+    "rtp_fluids_set_nb_particles": ([_vp, _u32], _int),
+    "rtp_fluids_get_nb_particles": ([_vp, ctypes.POINTER(_u32)], _int),
+    "rtp_fluids_load_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_fluids_read_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_fluids_update": ([_vp, _u32], _int),
+    "rtp_fluids_pause": ([_vp, _int], _int),
+    "rtp_fluids_sync": ([_vp], _int),
+    "rtp_fluids_read_grid": ([_vp, _vp, _vp, _vp], _int),
+    "rtp_fluids_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_fluids_density_error": ([_vp, _vp], _int),
+    "rtp_fluids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
+    "rtp_fluids_stream": ([_vp, ctypes.POINTER(_vp)], _int),
+    "rtp_fluids_enable_profiling": ([_vp, _int], _int),
+    "rtp_fluids_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
+    "rtp_fluids_reset_profile": ([_vp], _int),
+    "rtp_fluids_use_graph": ([_vp, _int], _int),
+    "rtp_fluids_set_tuning": ([_vp, _int, _int, _int], _int),
+    "rtp_boids_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], _int),
+    "rtp_boids_destroy": ([_vp], _int),
+    "rtp_boids_set_params": ([_vp, ctypes.POINTER(BoidsParams)], _int),
+    "rtp_boids_set_time_step": ([_vp, _f32], _int),
+    "rtp_boids_set_nb_particles": ([_vp, _u32], _int),
+    "rtp_boids_load_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_boids_read_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_boids_update": ([_vp, _u32], _int),
+    "rtp_boids_pause": ([_vp, _int], _int),
+    "rtp_boids_sync": ([_vp], _int),
+    "rtp_boids_read_grid": ([_vp, _vp, _vp, _vp], _int),
+    "rtp_boids_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_boids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
+    "rtp_boids_enable_profiling": ([_vp, _int], _int),
+    "rtp_boids_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
+    "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _vp, _vp], _int),
+    "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load librtp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RtpError(f"HIP extension not built: {LIB_PATH} missing "
+                           "(run python -m realtimeparticles_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().rtp_last_error().decode(errors="replace")
+        raise RtpError(f"{what} failed ({rc}): {msg}")
+
+
+def device_count() -> int:
+    c = _int(0)
+    _check(lib().rtp_device_count(ctypes.byref(c)), "rtp_device_count")
+    return int(c.value)
+
+
+def _f4(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("expected an (n, 4) float32 array")
+    return a
+
+
+def _desc(max_n, n, box, res, gravity, max_parts_in_cell, device) -> ModelDesc:
+    d = ModelDesc()
+    d.maxNbParticles = int(max_n)
+    d.currNbParticles = int(n)
+    for a in range(3):
+        d.boxSize[a] = float(box[a])
+        d.gridRes[a] = int(res[a])
+    d.maxPartsInCell = int(max_parts_in_cell)
+    d.gravity = float(gravity)
+    d.device = int(device)
+    return d
+
+
+class Fluids:
+    """Physics::HIP::Fluids over the C ABI (position based fluids)."""
+
+    def __init__(self, max_n: int, n: int, box, res, gravity: float = 9.81,
+                 max_parts_in_cell: int = 0, device: int = 0):
+        self._h = _vp()
+        self.max_n, self.n = int(max_n), int(n)
+        self.desc = _desc(max_n, n, box, res, gravity, max_parts_in_cell, device)
+        _check(lib().rtp_fluids_create(ctypes.byref(self.desc), ctypes.byref(self._h)),
+               "rtp_fluids_create")
+        self.num_cells = int(res[0]) * int(res[1]) * int(res[2])
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtp_fluids_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, **kw) -> FluidParams:
+        p = FluidParams(**{**FLUID_DEFAULTS, **kw})
+        _check(lib().rtp_fluids_set_params(self._h, ctypes.byref(p)), "rtp_fluids_set_params")
+        return p
+
+    def params(self) -> FluidParams:
+        p = FluidParams()
+        _check(lib().rtp_fluids_get_params(self._h, ctypes.byref(p)), "rtp_fluids_get_params")
+        return p
+
+    def set_nb_jacobi_iters(self, iters: int) -> None:
+        _check(lib().rtp_fluids_set_nb_jacobi_iters(self._h, int(iters)), "set_nb_jacobi_iters")
+
+    def set_nb_particles(self, n: int) -> None:
+        _check(lib().rtp_fluids_set_nb_particles(self._h, int(n)), "set_nb_particles")
+        self.n = int(n)
+
+    def load_state(self, pos, vel) -> None:
+        pos, vel = _f4(pos), _f4(vel)
+        assert pos.shape == vel.shape
+        _check(lib().rtp_fluids_load_state(self._h, pos.ctypes.data, vel.ctypes.data,
+                                           pos.shape[0]), "rtp_fluids_load_state")
+
+    def read_state(self, count: int | None = None):
+        count = self.n if count is None else int(count)
+        pos = np.zeros((count, 4), np.float32)
+        vel = np.zeros((count, 4), np.float32)
+        _check(lib().rtp_fluids_read_state(self._h, pos.ctypes.data, vel.ctypes.data, count),
+               "rtp_fluids_read_state")
+        return pos, vel
+
+    def update(self, nsteps: int = 1) -> None:
+        _check(lib().rtp_fluids_update(self._h, int(nsteps)), "rtp_fluids_update")
+
+    def pause(self, flag: bool) -> None:
+        _check(lib().rtp_fluids_pause(self._h, int(bool(flag))), "rtp_fluids_pause")
+
+    def sync(self) -> None:
+        _check(lib().rtp_fluids_sync(self._h), "rtp_fluids_sync")
+
+    def read_grid(self):
+        keys = np.zeros(self.max_n, np.uint32)
+        idx = np.zeros(self.max_n, np.uint32)
+        se = np.zeros((self.num_cells, 2), np.uint32)
+        _check(lib().rtp_fluids_read_grid(self._h, keys.ctypes.data, idx.ctypes.data,
+                                          se.ctypes.data), "rtp_fluids_read_grid")
+        return keys, idx, se
+
+    def read_field(self, field: int, count: int | None = None) -> np.ndarray:
+        count = self.n if count is None else int(count)
+        shape = (count, 4) if field in (FIELD_PRED_POS, FIELD_VORTICITY) else (count,)
+        out = np.zeros(shape, np.float32)
+        _check(lib().rtp_fluids_read_field(self._h, int(field), out.ctypes.data, count),
+               "rtp_fluids_read_field")
+        return out
+
+    def density_error(self):
+        out = np.zeros(3, np.float32)
+        _check(lib().rtp_fluids_density_error(self._h, out.ctypes.data), "density_error")
+        return out
+
+    def device_state(self):
+        p, v = _vp(), _vp()
+        _check(lib().rtp_fluids_device_state(self._h, ctypes.byref(p), ctypes.byref(v)),
+               "device_state")
+        return p.value, v.value
+
+    def enable_profiling(self, flag: bool) -> None:
+        _check(lib().rtp_fluids_enable_profiling(self._h, int(bool(flag))), "enable_profiling")
+
+    def reset_profile(self) -> None:
+        _check(lib().rtp_fluids_reset_profile(self._h), "reset_profile")
+
+    def profile(self) -> dict:
+        cnt = _int(0)
+        _check(lib().rtp_fluids_profile(self._h, None, 0, ctypes.byref(cnt)), "profile")
+        arr = (KernelTime * max(cnt.value, 1))()
+        _check(lib().rtp_fluids_profile(self._h, arr, cnt.value, ctypes.byref(cnt)), "profile")
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].totalMs))
+                for i in range(cnt.value)}
+
+    def use_graph(self, flag: bool) -> None:
+        _check(lib().rtp_fluids_use_graph(self._h, int(bool(flag))), "use_graph")
+
+    def set_tuning(self, block: int = 256, xcd_remap: int = 1, sort_bits: int = 0) -> None:
+        _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(xcd_remap), int(sort_bits)),
+               "set_tuning")
+
+
+class Boids:
+    """Physics::HIP::Boids over the C ABI (3-D rules, bouncing wall)."""
+
+    def __init__(self, max_n: int, n: int, box, res, max_parts_in_cell: int = 0,
+                 device: int = 0):
+        self._h = _vp()
+        self.max_n, self.n = int(max_n), int(n)
+        self.desc = _desc(max_n, n, box, res, 0.0, max_parts_in_cell, device)
+        _check(lib().rtp_boids_create(ctypes.byref(self.desc), ctypes.byref(self._h)),
+               "rtp_boids_create")
+        self.num_cells = int(res[0]) * int(res[1]) * int(res[2])
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtp_boids_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, velocityScale=0.5, alignmentScale=1.6, separationScale=1.6,
+                   cohesionScale=1.45) -> None:
+        p = BoidsParams(velocityScale, alignmentScale, separationScale, cohesionScale)
+        _check(lib().rtp_boids_set_params(self._h, ctypes.byref(p)), "rtp_boids_set_params")
+
+    def set_time_step(self, dt: float) -> None:
+        _check(lib().rtp_boids_set_time_step(self._h, float(dt)), "set_time_step")
+
+    def set_nb_particles(self, n: int) -> None:
+        _check(lib().rtp_boids_set_nb_particles(self._h, int(n)), "set_nb_particles")
+        self.n = int(n)
+
+    def load_state(self, pos, vel) -> None:
+        pos, vel = _f4(pos), _f4(vel)
+        _check(lib().rtp_boids_load_state(self._h, pos.ctypes.data, vel.ctypes.data,
+                                          pos.shape[0]), "rtp_boids_load_state")
+
+    def read_state(self, count: int | None = None):
+        count = self.n if count is None else int(count)
+        pos = np.zeros((count, 4), np.float32)
+        vel = np.zeros((count, 4), np.float32)
+        _check(lib().rtp_boids_read_state(self._h, pos.ctypes.data, vel.ctypes.data, count),
+               "rtp_boids_read_state")
+        return pos, vel
+
+    def update(self, nsteps: int = 1) -> None:
+        _check(lib().rtp_boids_update(self._h, int(nsteps)), "rtp_boids_update")
+
+    def pause(self, flag: bool) -> None:
+        _check(lib().rtp_boids_pause(self._h, int(bool(flag))), "rtp_boids_pause")
+
+    def sync(self) -> None:
+        _check(lib().rtp_boids_sync(self._h), "rtp_boids_sync")
+
+    def read_grid(self):
+        keys = np.zeros(self.max_n, np.uint32)
+        idx = np.zeros(self.max_n, np.uint32)
+        se = np.zeros((self.num_cells, 2), np.uint32)
+        _check(lib().rtp_boids_read_grid(self._h, keys.ctypes.data, idx.ctypes.data,
+                                         se.ctypes.data), "rtp_boids_read_grid")
+        return keys, idx, se
+
+    def read_acc(self, count: int | None = None) -> np.ndarray:
+        count = self.n if count is None else int(count)
+        out = np.zeros((count, 4), np.float32)
+        _check(lib().rtp_boids_read_field(self._h, FIELD_ACC, out.ctypes.data, count),
+               "read_field")
+        return out
+
+    def enable_profiling(self, flag: bool) -> None:
+        _check(lib().rtp_boids_enable_profiling(self._h, int(bool(flag))), "enable_profiling")
+
+    def profile(self) -> dict:
+        cnt = _int(0)
+        _check(lib().rtp_boids_profile(self._h, None, 0, ctypes.byref(cnt)), "profile")
+        arr = (KernelTime * max(cnt.value, 1))()
+        _check(lib().rtp_boids_profile(self._h, arr, cnt.value, ctypes.byref(cnt)), "profile")
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].totalMs))
+                for i in range(cnt.value)}
+
+
+def sort_pairs(keys, max_key: int | None = None, device: int = 0):
+    """Stable sort of uint32 keys on the GPU; returns (sorted_keys, permutation)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.shape[0]
+    if max_key is None:
+        max_key = int(keys.max()) if n else 0
+    out = np.zeros(n, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    _check(lib().rtp_sort_pairs(int(device), keys.ctypes.data, n, int(max_key), out.ctypes.data,
+                                perm.ctypes.data), "rtp_sort_pairs")
+    return out, perm
+
+
+def grid_build(pos, max_n: int, box, res, max_parts_in_cell: int = 100, device: int = 0):
+    """Cell keys, sort permutation and (start, end) table of one batch of positions."""
+    pos = _f4(pos)
+    n = pos.shape[0]
+    d = _desc(max_n, n, box, res, 0.0, max_parts_in_cell, device)
+    nc = int(res[0]) * int(res[1]) * int(res[2])
+    keys = np.zeros(max_n, np.uint32)
+    idx = np.zeros(max_n, np.uint32)
+    se = np.zeros((nc, 2), np.uint32)
+    _check(lib().rtp_grid_build(ctypes.byref(d), pos.ctypes.data, keys.ctypes.data,
+                                idx.ctypes.data, se.ctypes.data), "rtp_grid_build")
+    return keys, idx, se

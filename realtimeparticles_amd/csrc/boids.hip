@@ -1,0 +1,150 @@
+// 3-D boids steering pass for gfx950.
+//
+// Reference: physics/ocl/kernels/boids.cl:46-132 (bd_applyBoidsRulesWithGrid3D),
+// :246-259 (bd_updateVel), :264-284 (bd_updatePosAndApplyWallBC), step order
+// physics/ocl/Boids.cpp:323-384.  The three kernels are fused into one pass that reads the
+// sorted (position, velocity) buffers and writes the next state into the other pair of
+// buffers, so no boid reads a neighbour's already-updated state.
+#include "rtp_device.hpp"
+
+namespace rtp {
+
+namespace {
+
+// fillCellIDs on the current positions (Boids.cpp:335) + resetStartEndCell.
+__global__ __launch_bounds__(256) void k_boids_hash(const float4* __restrict__ pos,
+                                                    uint32_t* __restrict__ key,
+                                                    uint2* __restrict__ startEnd, uint32_t n,
+                                                    GridConsts g)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n)
+    key[t] = cell1d(pos[t], g);
+  if (t < g.numCells)
+    startEnd[t] = make_uint2(1u, 0u);
+}
+
+struct V3 {
+  float x, y, z;
+};
+
+// fast_normalize pinned as v / sqrtf(dot(v, v)), 0 for the zero vector.
+__device__ __forceinline__ V3 normalize3(V3 v)
+{
+  if (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f)
+    return v;
+  const float l = __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  return V3{ v.x / l, v.y / l, v.z / l };
+}
+
+__global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
+                                                    const float4* __restrict__ velocity,
+                                                    const uint2* __restrict__ se,
+                                                    float4* __restrict__ posOut,
+                                                    float4* __restrict__ velOut,
+                                                    float4* __restrict__ accOut, uint32_t n,
+                                                    BoidsConsts c, int remap)
+{
+  const uint32_t i = logical_block(remap) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 p = position[i];
+  uint32_t cx, cy, cz;
+  cell3d(p, c.g, cx, cy, cz);
+  const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
+  int count = 0;
+  V3 avgPos{ 0.0f, 0.0f, 0.0f }, avgVel{ 0.0f, 0.0f, 0.0f }, rep{ 0.0f, 0.0f, 0.0f };
+  for (int ix = -1; ix <= 1; ++ix) {
+    const int32_t nx = (int32_t)cx + ix;
+    if (nx < 0 || nx >= rx)
+      continue;
+    for (int iy = -1; iy <= 1; ++iy) {
+      const int32_t ny = (int32_t)cy + iy;
+      if (ny < 0 || ny >= ry)
+        continue;
+      for (int iz = -1; iz <= 1; ++iz) {
+        const int32_t nz = (int32_t)cz + iz;
+        if (nz < 0 || nz >= rz)
+          continue;
+        const uint2 r = se[(nx * ry + ny) * rz + nz];
+        for (uint32_t e = r.x; e <= r.y; ++e) {
+          const float4 q = position[e];
+          const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
+          const float sq = vx * vx + vy * vy + vz * vz;
+          if (sq < c.radiusSq && sq > kFloatEps) {  // boids.cl:102-103
+            avgPos.x = avgPos.x + q.x;
+            avgPos.y = avgPos.y + q.y;
+            avgPos.z = avgPos.z + q.z;
+            const float4 ve = velocity[e];
+            const V3 nv = normalize3(V3{ ve.x, ve.y, ve.z });
+            avgVel.x = avgVel.x + nv.x;
+            avgVel.y = avgVel.y + nv.y;
+            avgVel.z = avgVel.z + nv.z;
+            rep.x = rep.x + vx / sq;
+            rep.y = rep.y + vy / sq;
+            rep.z = rep.z + vz / sq;
+            ++count;
+          }
+        }
+      }
+    }
+  }
+  V3 acc{ 0.0f, 0.0f, 0.0f };
+  if (count != 0) {  // boids.cl:115-129
+    const float fc = (float)count;
+    avgPos = V3{ avgPos.x / fc - p.x, avgPos.y / fc - p.y, avgPos.z / fc - p.z };
+    const float s = c.p.velocityScale;
+    const V3 coh = normalize3(avgPos), ali = normalize3(avgVel), sep = normalize3(rep);
+    const V3 a{ coh.x * s, coh.y * s, coh.z * s };
+    const V3 b{ ali.x * s, ali.y * s, ali.z * s };
+    const V3 d{ sep.x * s, sep.y * s, sep.z * s };
+    acc.x = (b.x * c.p.alignmentScale + d.x * c.p.separationScale) + a.x * c.p.cohesionScale;
+    acc.y = (b.y * c.p.alignmentScale + d.y * c.p.separationScale) + a.y * c.p.cohesionScale;
+    acc.z = (b.z * c.p.alignmentScale + d.z * c.p.separationScale) + a.z * c.p.cohesionScale;
+  }
+  accOut[i] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+
+  // bd_updateVel, boids.cl:246-259 (maxVelocity = velocityScale, Boids.cpp:217)
+  const float4 v = velocity[i];
+  const V3 nv{ v.x + acc.x * c.dt, v.y + acc.y * c.dt, v.z + acc.z * c.dt };
+  const float len = __builtin_sqrtf(nv.x * nv.x + nv.y * nv.y + nv.z * nv.z);
+  const float norm = clampf(len, c.vmin, c.p.velocityScale);
+  const V3 dir = normalize3(nv);
+  V3 vel{ dir.x * norm, dir.y * norm, dir.z * norm };
+
+  // bd_updatePosAndApplyWallBC, boids.cl:264-284
+  const V3 np{ p.x + vel.x * c.dt, p.y + vel.y * c.dt, p.z + vel.z * c.dt };
+  const V3 cp{ clampf(np.x, -c.g.absWall[0], c.g.absWall[0]),
+               clampf(np.y, -c.g.absWall[1], c.g.absWall[1]),
+               clampf(np.z, -c.g.absWall[2], c.g.absWall[2]) };
+  if (!(cp.x == np.x && cp.y == np.y && cp.z == np.z))
+    vel = V3{ vel.x * -0.5f, vel.y * -0.5f, vel.z * -0.5f };
+  posOut[i] = make_float4(cp.x, cp.y, cp.z, 0.0f);
+  velOut[i] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+}  // namespace
+
+void launch_boids_hash(const float4* pos, uint32_t* key, uint2* startEnd, uint32_t n,
+                       const GridConsts& g, hipStream_t s)
+{
+  const uint32_t m = n > g.numCells ? n : g.numCells;
+  if (m == 0)
+    return;
+  hipLaunchKernelGGL(k_boids_hash, dim3(blocks_for(m, 256)), dim3(256), 0, s, pos, key, startEnd,
+                     n, g);
+}
+
+void launch_boids_step(const float4* pos, const float4* vel, const uint2* startEnd,
+                       float4* posOut, float4* velOut, float4* acc, uint32_t n,
+                       const BoidsConsts& c, const Launch& L, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_boids_step, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos, vel,
+                     startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+}
+
+}  // namespace rtp

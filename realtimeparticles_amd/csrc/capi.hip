@@ -1,0 +1,1167 @@
+// C ABI of librtp: model handles, device buffers, the per-substep launch sequence,
+// hipGraph replay and per-kernel profiling.
+//
+// The launch sequence of one PBF substep (Fluids.cpp:400-471 without rendering):
+//   k_predict -> radix sort (passes x {upsweep, scan, downsweep}) -> k_reorder
+//   -> iters x {k_density_lambda, k_correct} -> [k_vorticity, k_confinement, k_xsph]
+// and of one boids step (Boids.cpp:323-384):
+//   k_boids_hash -> radix sort -> k_reorder -> k_boids_step
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rtp_internal.hpp"
+
+namespace rtp {
+
+namespace {
+thread_local std::string g_lastError;
+}
+
+int fail(int status, const std::string& msg)
+{
+  g_lastError = msg;
+  return status;
+}
+
+namespace {
+
+// ------------------------------------------------------------------------- host constants
+
+// utils/Utils.cpp:24-29 FloatToStr(v, 10) followed by the OpenCL compiler's parse.
+float float_roundtrip(float v)
+{
+  char buf[128];
+  std::snprintf(buf, sizeof(buf), "%.10f", (double)v);
+  return std::strtof(buf, nullptr);
+}
+
+constexpr float kPiF = 3.1415927f;  // utils/Math.hpp:69
+
+int make_grid(const rtp_model_desc* d, uint32_t defaultCap, GridConsts* g)
+{
+  for (int a = 0; a < 3; ++a) {
+    if (!(d->boxSize[a] > 0.0f) || d->gridRes[a] <= 0)
+      return fail(RTP_ERR_INVALID, "boxSize and gridRes must be positive");
+    g->absWall[a] = float_roundtrip(d->boxSize[a] / 2.0f);  // Fluids.cpp:108-110
+    g->res[a] = d->gridRes[a];
+  }
+  const uint64_t nc = (uint64_t)d->gridRes[0] * d->gridRes[1] * d->gridRes[2];
+  if (nc == 0 || nc > (1u << 30))
+    return fail(RTP_ERR_INVALID, "grid has too many cells");
+  g->cellSize = float_roundtrip(d->boxSize[0] / (float)d->gridRes[0]);  // Fluids.cpp:114
+  g->numCells = (uint32_t)nc;
+  g->cap = d->maxPartsInCell ? d->maxPartsInCell : defaultCap;
+  return RTP_OK;
+}
+
+// Largest cell key fillCellIDs can produce (each 3-D index is at most res when a position
+// sits exactly on the +wall) — the sort only needs that many bits.
+uint32_t max_active_key(const GridConsts& g)
+{
+  const uint64_t ry = (uint64_t)g.res[1], rz = (uint64_t)g.res[2];
+  const uint64_t m = (uint64_t)g.res[0] * rz * ry + ry * rz + rz;
+  return (uint32_t)std::min<uint64_t>(m, 0xffffffffu);
+}
+
+// Smallest float that is >= h*h as a real number.
+float cull_bound(float h)
+{
+  const double hh = (double)h * (double)h;
+  float t = (float)hh;
+  if ((double)t < hh)
+    t = std::nextafter(t, INFINITY);
+  return t;
+}
+
+// --------------------------------------------------------------------------------- buffers
+
+template <class T>
+int dev_alloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0)
+    count = 1;
+  RTP_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count));
+  return RTP_OK;
+}
+
+template <class T>
+void dev_free(T*& p)
+{
+  if (p)
+    (void)hipFree(p);
+  p = nullptr;
+}
+
+// ------------------------------------------------------------------------------- profiling
+
+struct Profiler {
+  bool enabled = false;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<std::string> names;
+  std::vector<uint32_t> launches;
+  std::vector<double> totals;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  int id_of(const char* name)
+  {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name)
+        return (int)i;
+    names.emplace_back(name);
+    launches.push_back(0);
+    totals.push_back(0.0);
+    return (int)names.size() - 1;
+  }
+  hipEvent_t take()
+  {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  // Wrap one launch.  Call begin() before and end() after enqueueing it.
+  int beginId = -1;
+  hipEvent_t beginEv = nullptr;
+  void begin(const char* name, hipStream_t s)
+  {
+    if (!enabled)
+      return;
+    beginId = id_of(name);
+    beginEv = take();
+    (void)hipEventRecord(beginEv, s);
+  }
+  void end(hipStream_t s)
+  {
+    if (!enabled || beginId < 0)
+      return;
+    hipEvent_t e = take();
+    (void)hipEventRecord(e, s);
+    pending.push_back(Pending{ beginId, beginEv, e });
+    beginId = -1;
+  }
+  int collect()
+  {
+    for (auto& p : pending) {
+      RTP_HIP_TRY(hipEventSynchronize(p.b));
+      float ms = 0.0f;
+      RTP_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+      launches[p.id] += 1;
+      totals[p.id] += ms;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+    return RTP_OK;
+  }
+  void reset()
+  {
+    (void)collect();
+    for (auto& l : launches)
+      l = 0;
+    for (auto& t : totals)
+      t = 0.0;
+  }
+  int report(rtp_kernel_time* out, int capacity, int* count)
+  {
+    const int rc = collect();
+    if (rc)
+      return rc;
+    const int n = (int)names.size();
+    if (count)
+      *count = n;
+    for (int i = 0; i < n && i < capacity && out; ++i) {
+      std::memset(out[i].name, 0, sizeof(out[i].name));
+      std::strncpy(out[i].name, names[i].c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = launches[i];
+      out[i].totalMs = totals[i];
+    }
+    return RTP_OK;
+  }
+  void destroy()
+  {
+    (void)collect();
+    for (auto e : pool)
+      (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+#define RTP_PROF(prof, name, stream, launch) \
+  do {                                       \
+    (prof).begin(name, stream);              \
+    launch;                                  \
+    (prof).end(stream);                      \
+  } while (0)
+
+// ------------------------------------------------------------------------- shared grid state
+
+// Buffers of the uniform grid that both models own: the reference's p_cellID,
+// RadixSortIndices and c_startEndPartID (plus the sort's ping-pong partners and counts).
+struct GridState {
+  GridConsts g{};
+  uint32_t maxN = 0;
+  uint32_t* key[2] = { nullptr, nullptr };
+  uint32_t* idx[2] = { nullptr, nullptr };
+  uint2* startEnd = nullptr;
+  uint32_t* counts = nullptr;
+  int keyCur = 0;   // key[keyCur] is p_cellID
+  int idxCur = 0;   // idx[idxCur] holds the last permutation over [0, sortedN)
+  uint32_t sortedN = 0;
+  uint32_t dirtyEnd = 0;  // keys in [n, dirtyEnd) are not the reset tail 2*NC+i
+  int sortBits = 0;
+
+  int alloc(uint32_t maxN_)
+  {
+    maxN = maxN_;
+    const size_t subtiles = (maxN + (size_t)kSortItems * 64 - 1) / ((size_t)kSortItems * 64);
+    int rc = 0;
+    if ((rc = dev_alloc(&key[0], maxN)) || (rc = dev_alloc(&key[1], maxN)) ||
+        (rc = dev_alloc(&idx[0], maxN)) || (rc = dev_alloc(&idx[1], maxN)) ||
+        (rc = dev_alloc(&startEnd, g.numCells)) ||
+        (rc = dev_alloc(&counts, (size_t)2048 * std::max<size_t>(subtiles, 1))))
+      return rc;
+    return RTP_OK;
+  }
+  void release()
+  {
+    dev_free(key[0]);
+    dev_free(key[1]);
+    dev_free(idx[0]);
+    dev_free(idx[1]);
+    dev_free(startEnd);
+    dev_free(counts);
+  }
+  // resetCellIDs (grid.cl:65-71) into both key buffers so their tails always agree.
+  void reset_keys(hipStream_t s)
+  {
+    launch_reset_cell_ids(key[0], maxN, g.numCells, s);
+    launch_reset_cell_ids(key[1], maxN, g.numCells, s);
+    keyCur = 0;
+    idxCur = 0;
+    sortedN = 0;
+    dirtyEnd = 0;
+  }
+  uint32_t sort_range(uint32_t n) const { return std::max(n, dirtyEnd); }
+  uint32_t max_key(uint32_t n) const
+  {
+    uint32_t m = max_active_key(g);
+    const uint32_t sn = sort_range(n);
+    if (sn > n) {  // stale keys may be anything fillCellIDs wrote or a reset tail value
+      const uint64_t tail = (uint64_t)g.numCells * 2 + sn - 1;
+      m = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(m, tail), 0xffffffffu);
+    }
+    return m;
+  }
+  // RadixSort::sort on p_cellID over [0, sort_range(n)); the rest is already in order.
+  int sort(uint32_t n, hipStream_t s, Profiler& prof)
+  {
+    const uint32_t sn = sort_range(n);
+    const SortPlan plan = make_sort_plan(sn, max_key(n), sortBits);
+    bool inB = false;
+    int rc = 0;
+    RTP_PROF(prof, "radix_sort", s,
+             rc = sort_pairs(plan, key[keyCur], key[1 - keyCur], idx[0], idx[1], counts, &inB, s));
+    if (rc)
+      return rc;
+    if (inB)
+      keyCur = 1 - keyCur;
+    idxCur = inB ? 1 : 0;
+    sortedN = sn;
+    return RTP_OK;
+  }
+  // Copy p_cellID, the permutation (identity past the sorted range) and the cell table.
+  int read(uint32_t* keys, uint32_t* perm, uint32_t* se, hipStream_t s)
+  {
+    if (keys)
+      RTP_HIP_TRY(hipMemcpyAsync(keys, key[keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToHost, s));
+    if (perm && sortedN)
+      RTP_HIP_TRY(hipMemcpyAsync(perm, idx[idxCur], sizeof(uint32_t) * sortedN, hipMemcpyDeviceToHost, s));
+    if (se)
+      RTP_HIP_TRY(hipMemcpyAsync(se, startEnd, sizeof(uint2) * g.numCells, hipMemcpyDeviceToHost, s));
+    RTP_HIP_TRY(hipStreamSynchronize(s));
+    if (perm)
+      for (uint32_t i = sortedN; i < maxN; ++i)
+        perm[i] = i;
+    return RTP_OK;
+  }
+};
+
+int check_device(int device)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(RTP_ERR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count)
+    return fail(RTP_ERR_INVALID, "device ordinal out of range");
+  RTP_HIP_TRY(hipSetDevice(device));
+  return RTP_OK;
+}
+
+int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, float4 fill,
+              hipStream_t s)
+{
+  std::vector<float4> h(maxN, fill);
+  for (uint32_t i = 0; i < count; ++i)
+    h[i] = make_float4(src[4 * i + 0], src[4 * i + 1], src[4 * i + 2], 0.0f);
+  RTP_HIP_TRY(hipMemcpyAsync(dst, h.data(), sizeof(float4) * maxN, hipMemcpyHostToDevice, s));
+  RTP_HIP_TRY(hipStreamSynchronize(s));
+  return RTP_OK;
+}
+
+}  // namespace
+}  // namespace rtp
+
+using namespace rtp;
+
+// ================================================================================ fluids
+
+struct rtp_fluids {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  GridState grid;
+  PbfConsts c{};
+  uint32_t n = 0;
+  uint32_t iters = 2;  // m_nbJacobiIters default, Fluids.cpp:81
+  bool paused = false;
+  float4* pos[2] = { nullptr, nullptr };
+  float4* vel[2] = { nullptr, nullptr };
+  float4* pred[2] = { nullptr, nullptr };
+  float4* vort = nullptr;
+  float4* velTmp = nullptr;
+  float* density = nullptr;
+  float* lambda = nullptr;
+  float* metric = nullptr;
+  int cur = 0;        // pos[cur]/vel[cur] are p_pos/p_vel
+  int predFinal = 1;  // pred[predFinal] is p_predPos after the last update
+  Launch L;
+  Profiler prof;
+  bool useGraph = true;
+  bool graphValid = false;
+  int graphCur = -1, graphKeyCur = -1, graphPred = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+
+  void drop_graph()
+  {
+    if (graphExec)
+      (void)hipGraphExecDestroy(graphExec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphValid = false;
+  }
+
+  // The host-side constant folding of Fluids::createProgram (Fluids.cpp:97-126) and of the
+  // kernels' constant subexpressions.
+  void refresh_consts()
+  {
+    const float h = c.h;
+    c.hh = h * h;
+    c.cullSq = cull_bound(h);
+    const rtp_fluid_params& p = c.p;
+    const float dq = p.artPressureRadius * h;
+    const float t = dq < h ? 1.0f : 0.0f;  // 1 - step(h, dq)
+    const float d = h * h - dq * dq;
+    c.wdq = (t * c.poly6) * ((d * d) * d);
+    c.rho0sq = p.restDensity * p.restDensity;
+    c.dtEps = p.timeStep + 0.00000001f;
+    c.gdt = -c.gravity * p.timeStep;
+    for (int a = 0; a < 3; ++a) {
+      c.lo[a] = -c.g.absWall[a] + 0.01f;
+      c.hi[a] = c.g.absWall[a] - 0.1f;
+    }
+    drop_graph();
+  }
+
+  int substep()
+  {
+    hipStream_t s = stream;
+    GridState& G = grid;
+    // p_predPos is pred[predFinal]; predict overwrites its active range only, so entries
+    // past n keep the previous step's values, as in the reference's single buffer.
+    const int P = predFinal;
+    RTP_PROF(prof, "pbf_predict_hash", s,
+             launch_pbf_predict(pos[cur], vel[cur], pred[P], G.key[G.keyCur], G.startEnd, n, c, s));
+    int rc = G.sort(n, s, prof);
+    if (rc)
+      return rc;
+    ReorderArgs a{};
+    a.key = G.key[G.keyCur];
+    a.idx = G.idx[G.idxCur];
+    a.sortN = G.sortedN;
+    a.n = n;
+    a.maxN = G.maxN;
+    a.srcA = pos[cur];
+    a.dstA = pos[1 - cur];
+    a.srcB = vel[cur];
+    a.dstB = vel[1 - cur];
+    a.srcC = pred[P];
+    a.dstC = pred[1 - P];
+    a.clampC = iters > 0;
+    for (int k = 0; k < 3; ++k) {
+      a.lo[k] = c.lo[k];
+      a.hi[k] = c.hi[k];
+    }
+    a.startEnd = G.startEnd;
+    a.g = c.g;
+    RTP_PROF(prof, "reorder_cells", s, launch_reorder(a, s));
+    cur = 1 - cur;
+    const bool vortOn = c.p.isVorticityConfEnabled != 0;
+    int in = 1 - P;
+    for (uint32_t k = 0; k < iters; ++k) {
+      const int last = (k + 1 == iters);
+      RTP_PROF(prof, "pbf_density_lambda", s,
+               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s));
+      RTP_PROF(prof, "pbf_correct", s,
+               launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
+                                  (last && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s));
+      in = 1 - in;
+    }
+    if (iters == 0)
+      RTP_PROF(prof, "pbf_update_vel", s,
+               launch_pbf_update_vel(pred[in], pos[cur], vel[cur], vortOn ? nullptr : pos[cur], n,
+                                     c, s));
+    // Permuted entries in [n, sortedN) live in the gather target; keep them in p_predPos.
+    if (in != 1 - P && G.sortedN > n)
+      RTP_PROF(prof, "copy_tail", s, launch_copy_f4(pred[in], pred[1 - P], n, G.sortedN, s));
+    predFinal = in;
+    if (vortOn) {
+      RTP_PROF(prof, "pbf_vorticity", s,
+               launch_pbf_vorticity(pred[in], G.startEnd, vel[cur], vort, n, c, L, s));
+      RTP_PROF(prof, "pbf_confinement", s,
+               launch_pbf_confinement(pred[in], G.startEnd, vort, vel[cur], velTmp, n, c, L, s));
+      RTP_PROF(prof, "pbf_xsph", s,
+               launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s));
+    }
+    return RTP_OK;
+  }
+
+  int capture_pair()
+  {
+    drop_graph();
+    graphCur = cur;
+    graphKeyCur = grid.keyCur;
+    graphPred = predFinal;
+    RTP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = substep();
+    if (!rc)
+      rc = substep();
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &gr);
+    if (rc)
+      return rc;
+    if (e != hipSuccess)
+      return fail(RTP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    graph = gr;
+    RTP_HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+    graphValid = true;
+    // Capturing does not run the work; the buffer indices advanced twice and are back.
+    return RTP_OK;
+  }
+
+  int update(uint32_t nsteps)
+  {
+    if (paused || nsteps == 0)
+      return RTP_OK;
+    RTP_HIP_TRY(hipSetDevice(device));
+    uint32_t done = 0;
+    if (useGraph && !prof.enabled && nsteps >= 2 && n > 0) {
+      if (!graphValid || graphCur != cur || graphKeyCur != grid.keyCur || graphPred != predFinal) {
+        const int rc = capture_pair();
+        if (rc)
+          return rc;
+      }
+      for (; done + 2 <= nsteps; done += 2)
+        RTP_HIP_TRY(hipGraphLaunch(graphExec, stream));
+    }
+    for (; done < nsteps; ++done) {
+      const int rc = substep();
+      if (rc)
+        return rc;
+    }
+    RTP_HIP_TRY(hipGetLastError());
+    return RTP_OK;
+  }
+
+  void release()
+  {
+    if (stream)
+      (void)hipStreamSynchronize(stream);
+    drop_graph();
+    prof.destroy();
+    grid.release();
+    for (int k = 0; k < 2; ++k) {
+      dev_free(pos[k]);
+      dev_free(vel[k]);
+      dev_free(pred[k]);
+    }
+    dev_free(vort);
+    dev_free(velTmp);
+    dev_free(density);
+    dev_free(lambda);
+    dev_free(metric);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+// ================================================================================= boids
+
+struct rtp_boids {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  GridState grid;
+  BoidsConsts c{};
+  uint32_t n = 0;
+  bool paused = false;
+  float4* pos[2] = { nullptr, nullptr };
+  float4* vel[2] = { nullptr, nullptr };
+  float4* acc = nullptr;
+  int cur = 0;
+  Launch L;
+  Profiler prof;
+
+  int step()
+  {
+    hipStream_t s = stream;
+    GridState& G = grid;
+    RTP_PROF(prof, "boids_hash", s,
+             launch_boids_hash(pos[cur], G.key[G.keyCur], G.startEnd, n, c.g, s));
+    int rc = G.sort(n, s, prof);
+    if (rc)
+      return rc;
+    ReorderArgs a{};
+    a.key = G.key[G.keyCur];
+    a.idx = G.idx[G.idxCur];
+    a.sortN = G.sortedN;
+    a.n = n;
+    a.maxN = G.maxN;
+    a.srcA = pos[cur];
+    a.dstA = pos[1 - cur];
+    a.srcB = vel[cur];
+    a.dstB = vel[1 - cur];
+    a.startEnd = G.startEnd;
+    a.g = c.g;
+    RTP_PROF(prof, "reorder_cells", s, launch_reorder(a, s));
+    // Sorted state now lives in [1 - cur]; the fused step writes the next state to [cur].
+    RTP_PROF(prof, "boids_step", s,
+             launch_boids_step(pos[1 - cur], vel[1 - cur], G.startEnd, pos[cur], vel[cur], acc, n,
+                               c, L, s));
+    // The tail [n, sortedN) was only permuted; carry it over unchanged.
+    if (G.sortedN > n) {
+      RTP_HIP_TRY(hipMemcpyAsync(pos[cur] + n, pos[1 - cur] + n, sizeof(float4) * (G.sortedN - n),
+                                 hipMemcpyDeviceToDevice, s));
+      RTP_HIP_TRY(hipMemcpyAsync(vel[cur] + n, vel[1 - cur] + n, sizeof(float4) * (G.sortedN - n),
+                                 hipMemcpyDeviceToDevice, s));
+    }
+    return RTP_OK;
+  }
+
+  void release()
+  {
+    if (stream)
+      (void)hipStreamSynchronize(stream);
+    prof.destroy();
+    grid.release();
+    for (int k = 0; k < 2; ++k) {
+      dev_free(pos[k]);
+      dev_free(vel[k]);
+    }
+    dev_free(acc);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+// ================================================================================= C ABI
+
+#define RTP_REQUIRE(cond, msg)               \
+  do {                                       \
+    if (!(cond))                             \
+      return fail(RTP_ERR_INVALID, msg);     \
+  } while (0)
+
+extern "C" {
+
+int rtp_abi_version(void) { return RTP_ABI_VERSION; }
+
+const char* rtp_last_error(void) { return g_lastError.c_str(); }
+
+int rtp_device_count(int* count)
+{
+  RTP_REQUIRE(count, "count is NULL");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess)
+    c = 0;
+  *count = c;
+  return RTP_OK;
+}
+
+// ------------------------------------------------------------------------------- fluids
+
+int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
+{
+  RTP_REQUIRE(d && out, "NULL argument");
+  *out = nullptr;
+  RTP_REQUIRE(d->maxNbParticles > 0, "maxNbParticles must be positive");
+  RTP_REQUIRE(d->currNbParticles <= d->maxNbParticles, "currNbParticles exceeds maxNbParticles");
+  int rc = check_device(d->device);
+  if (rc)
+    return rc;
+  auto* h = new rtp_fluids();
+  h->device = d->device;
+  if ((rc = make_grid(d, 100u, &h->grid.g))) {
+    delete h;
+    return rc;
+  }
+  PbfConsts& c = h->c;
+  c.g = h->grid.g;
+  const float effectRadius = d->boxSize[0] / (float)d->gridRes[0];           // Fluids.cpp:104
+  c.h = float_roundtrip(effectRadius);
+  c.poly6 = float_roundtrip(315.0f / (64.0f * kPiF * std::pow(effectRadius, 9.f)));  // :117
+  c.spiky = float_roundtrip(15.0f / (kPiF * std::pow(effectRadius, 6.f)));           // :118
+  c.maxVel = float_roundtrip(30.0f);                                                  // :119
+  c.gravity = d->gravity > 0.0f ? d->gravity : 9.81f;                                 // define.cl:8
+  c.p = rtp_fluid_params{ 450.0f, 600.0f, 0.010f, 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
+  h->n = d->currNbParticles;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(RTP_ERR_HIP, "hipStreamCreate failed");
+  }
+  const uint32_t maxN = d->maxNbParticles;
+  if ((rc = h->grid.alloc(maxN)) || (rc = dev_alloc(&h->pos[0], maxN)) ||
+      (rc = dev_alloc(&h->pos[1], maxN)) || (rc = dev_alloc(&h->vel[0], maxN)) ||
+      (rc = dev_alloc(&h->vel[1], maxN)) || (rc = dev_alloc(&h->pred[0], maxN)) ||
+      (rc = dev_alloc(&h->pred[1], maxN)) || (rc = dev_alloc(&h->vort, maxN)) ||
+      (rc = dev_alloc(&h->velTmp, maxN)) || (rc = dev_alloc(&h->density, maxN)) ||
+      (rc = dev_alloc(&h->lambda, maxN)) || (rc = dev_alloc(&h->metric, 4))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  h->refresh_consts();
+  // Zero state and the reset key tail, like the constructor's reset() (Fluids.cpp:91).
+  std::vector<float> zeros(4, 0.0f);
+  if ((rc = rtp_fluids_load_state(h, zeros.data(), zeros.data(), 0))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return RTP_OK;
+}
+
+int rtp_fluids_destroy(rtp_fluids* h)
+{
+  if (!h)
+    return RTP_OK;
+  (void)hipSetDevice(h->device);
+  h->release();
+  delete h;
+  return RTP_OK;
+}
+
+int rtp_fluids_set_params(rtp_fluids* h, const rtp_fluid_params* p)
+{
+  RTP_REQUIRE(h && p, "NULL argument");
+  RTP_REQUIRE(p->restDensity != 0.0f, "restDensity must be non-zero");
+  h->c.p = *p;
+  h->refresh_consts();
+  return RTP_OK;
+}
+
+int rtp_fluids_get_params(rtp_fluids* h, rtp_fluid_params* p)
+{
+  RTP_REQUIRE(h && p, "NULL argument");
+  *p = h->c.p;
+  return RTP_OK;
+}
+
+int rtp_fluids_set_nb_jacobi_iters(rtp_fluids* h, uint32_t iters)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(iters <= 1000, "iteration count out of range");
+  h->iters = iters;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_set_nb_particles(rtp_fluids* h, uint32_t n)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(n <= h->grid.maxN, "nb particles exceeds maxNbParticles");
+  // Keys in [n, old n) keep what fillCellIDs wrote; they stay in the sort like the
+  // reference's whole-buffer sort keeps them (RadixSort.cpp:134-161).
+  h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->n);
+  if (h->grid.dirtyEnd <= n)
+    h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->grid.sortedN);
+  h->n = n;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_get_nb_particles(rtp_fluids* h, uint32_t* n)
+{
+  RTP_REQUIRE(h && n, "NULL argument");
+  *n = h->n;
+  return RTP_OK;
+}
+
+int rtp_fluids_load_state(rtp_fluids* h, const float* pos4, const float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_REQUIRE(count == 0 || (pos4 && vel4), "NULL state arrays");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  const float inf = INFINITY;
+  const uint32_t maxN = h->grid.maxN;
+  int rc = 0;
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = upload_f4(h->pos[k], pos4, count, maxN, make_float4(inf, inf, inf, 0.0f), h->stream)) ||
+        (rc = upload_f4(h->vel[k], vel4, count, maxN, make_float4(0.0f, 0.0f, 0.0f, 0.0f), h->stream)) ||
+        (rc = upload_f4(h->pred[k], pos4, count, maxN, make_float4(inf, inf, inf, 0.0f), h->stream)))
+      return rc;
+  }
+  RTP_HIP_TRY(hipMemsetAsync(h->density, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->lambda, 0, sizeof(float) * maxN, h->stream));
+  h->cur = 0;
+  h->predFinal = 1;
+  h->grid.reset_keys(h->stream);
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_read_state(rtp_fluids* h, float* pos4, float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (pos4)
+    RTP_HIP_TRY(hipMemcpyAsync(pos4, h->pos[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  if (vel4)
+    RTP_HIP_TRY(hipMemcpyAsync(vel4, h->vel[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_fluids_update(rtp_fluids* h, uint32_t nsteps)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return h->update(nsteps);
+}
+
+int rtp_fluids_pause(rtp_fluids* h, int pause)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->paused = pause != 0;
+  return RTP_OK;
+}
+
+int rtp_fluids_sync(rtp_fluids* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_fluids_read_grid(rtp_fluids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  return h->grid.read(cellIDs, sortIdx, startEnd, h->stream);
+}
+
+int rtp_fluids_read_field(rtp_fluids* h, int field, float* out, uint32_t count)
+{
+  RTP_REQUIRE(h && out, "NULL argument");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const void* src = nullptr;
+  size_t elem = sizeof(float);
+  switch (field) {
+  case RTP_FIELD_DENSITY: src = h->density; break;
+  case RTP_FIELD_LAMBDA: src = h->lambda; break;
+  case RTP_FIELD_PRED_POS: src = h->pred[h->predFinal]; elem = sizeof(float4); break;
+  case RTP_FIELD_VORTICITY: src = h->vort; elem = sizeof(float4); break;
+  default: return fail(RTP_ERR_INVALID, "unknown field for fluids");
+  }
+  RTP_HIP_TRY(hipMemcpyAsync(out, src, elem * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_fluids_density_error(rtp_fluids* h, float out[3])
+{
+  RTP_REQUIRE(h && out, "NULL argument");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  launch_density_error(h->density, h->n, h->c.p.restDensity, h->metric, h->stream);
+  float m[3] = { 0.0f, 0.0f, 0.0f };
+  RTP_HIP_TRY(hipMemcpyAsync(m, h->metric, sizeof(m), hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  out[0] = m[0];
+  out[1] = h->n ? m[1] / (float)h->n : 0.0f;
+  out[2] = m[2];
+  return RTP_OK;
+}
+
+int rtp_fluids_device_state(rtp_fluids* h, void** pos4, void** vel4)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  if (pos4)
+    *pos4 = h->pos[h->cur];
+  if (vel4)
+    *vel4 = h->vel[h->cur];
+  return RTP_OK;
+}
+
+int rtp_fluids_stream(rtp_fluids* h, void** stream)
+{
+  RTP_REQUIRE(h && stream, "NULL argument");
+  *stream = h->stream;
+  return RTP_OK;
+}
+
+int rtp_fluids_enable_profiling(rtp_fluids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->prof.enabled = enable != 0;
+  return RTP_OK;
+}
+
+int rtp_fluids_profile(rtp_fluids* h, rtp_kernel_time* out, int capacity, int* count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return h->prof.report(out, capacity, count);
+}
+
+int rtp_fluids_reset_profile(rtp_fluids* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->prof.reset();
+  return RTP_OK;
+}
+
+int rtp_fluids_use_graph(rtp_fluids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->useGraph = enable != 0;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBits)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(blockSize >= 64 && blockSize <= 256 && blockSize % 64 == 0,
+              "blockSize must be 64, 128, 192 or 256");
+  RTP_REQUIRE(sortBits == 0 || (sortBits >= 4 && sortBits <= 11), "sortBits must be 0 or 4..11");
+  h->L.block = blockSize;
+  h->L.xcdRemap = xcdRemap ? 1 : 0;
+  h->grid.sortBits = sortBits;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+// -------------------------------------------------------------------------------- boids
+
+int rtp_boids_create(const rtp_model_desc* d, rtp_boids** out)
+{
+  RTP_REQUIRE(d && out, "NULL argument");
+  *out = nullptr;
+  RTP_REQUIRE(d->maxNbParticles > 0, "maxNbParticles must be positive");
+  RTP_REQUIRE(d->currNbParticles <= d->maxNbParticles, "currNbParticles exceeds maxNbParticles");
+  int rc = check_device(d->device);
+  if (rc)
+    return rc;
+  auto* h = new rtp_boids();
+  h->device = d->device;
+  if ((rc = make_grid(d, 3000u, &h->grid.g))) {
+    delete h;
+    return rc;
+  }
+  h->c.g = h->grid.g;
+  // Boids.cpp:104: 1.0f * box.x * box.x / (res.x * res.x)
+  h->c.radiusSq = float_roundtrip(1.0f * d->boxSize[0] * d->boxSize[0] /
+                                  (float)(d->gridRes[0] * d->gridRes[0]));
+  h->c.p = rtp_boids_params{ 0.5f, 1.6f, 1.6f, 1.45f };  // Boids.cpp:44-69 defaults
+  h->c.dt = 0.1f;                                        // Boids.cpp:334
+  h->c.vmin = 0.2f * h->c.p.velocityScale;
+  h->n = d->currNbParticles;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(RTP_ERR_HIP, "hipStreamCreate failed");
+  }
+  const uint32_t maxN = d->maxNbParticles;
+  if ((rc = h->grid.alloc(maxN)) || (rc = dev_alloc(&h->pos[0], maxN)) ||
+      (rc = dev_alloc(&h->pos[1], maxN)) || (rc = dev_alloc(&h->vel[0], maxN)) ||
+      (rc = dev_alloc(&h->vel[1], maxN)) || (rc = dev_alloc(&h->acc, maxN))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  std::vector<float> zeros(4, 0.0f);
+  if ((rc = rtp_boids_load_state(h, zeros.data(), zeros.data(), 0))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return RTP_OK;
+}
+
+int rtp_boids_destroy(rtp_boids* h)
+{
+  if (!h)
+    return RTP_OK;
+  (void)hipSetDevice(h->device);
+  h->release();
+  delete h;
+  return RTP_OK;
+}
+
+int rtp_boids_set_params(rtp_boids* h, const rtp_boids_params* p)
+{
+  RTP_REQUIRE(h && p, "NULL argument");
+  h->c.p = *p;
+  h->c.vmin = 0.2f * p->velocityScale;
+  return RTP_OK;
+}
+
+int rtp_boids_set_time_step(rtp_boids* h, float dt)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->c.dt = dt;
+  return RTP_OK;
+}
+
+int rtp_boids_set_nb_particles(rtp_boids* h, uint32_t n)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(n <= h->grid.maxN, "nb particles exceeds maxNbParticles");
+  h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->n);
+  if (h->grid.dirtyEnd <= n)
+    h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->grid.sortedN);
+  h->n = n;
+  return RTP_OK;
+}
+
+int rtp_boids_load_state(rtp_boids* h, const float* pos4, const float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_REQUIRE(count == 0 || (pos4 && vel4), "NULL state arrays");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  const float inf = INFINITY;
+  int rc = 0;
+  for (int k = 0; k < 2; ++k) {
+    // Boids.cpp:310-318: inactive boids sit at +inf and their velocity buffer gets the
+    // same values as the positions.
+    if ((rc = upload_f4(h->pos[k], pos4, count, h->grid.maxN, make_float4(inf, inf, inf, 0.0f), h->stream)) ||
+        (rc = upload_f4(h->vel[k], vel4, count, h->grid.maxN, make_float4(inf, inf, inf, 0.0f), h->stream)))
+      return rc;
+  }
+  RTP_HIP_TRY(hipMemsetAsync(h->acc, 0, sizeof(float4) * h->grid.maxN, h->stream));
+  h->cur = 0;
+  h->grid.reset_keys(h->stream);
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_read_state(rtp_boids* h, float* pos4, float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (pos4)
+    RTP_HIP_TRY(hipMemcpyAsync(pos4, h->pos[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  if (vel4)
+    RTP_HIP_TRY(hipMemcpyAsync(vel4, h->vel[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_update(rtp_boids* h, uint32_t nsteps)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  if (h->paused)
+    return RTP_OK;
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const int rc = h->step();
+    if (rc)
+      return rc;
+  }
+  RTP_HIP_TRY(hipGetLastError());
+  return RTP_OK;
+}
+
+int rtp_boids_pause(rtp_boids* h, int pause)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->paused = pause != 0;
+  return RTP_OK;
+}
+
+int rtp_boids_sync(rtp_boids* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  return h->grid.read(cellIDs, sortIdx, startEnd, h->stream);
+}
+
+int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
+{
+  RTP_REQUIRE(h && out, "NULL argument");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_REQUIRE(field == RTP_FIELD_ACC, "unknown field for boids");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipMemcpyAsync(out, h->acc, sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  if (pos4)
+    *pos4 = h->pos[h->cur];
+  if (vel4)
+    *vel4 = h->vel[h->cur];
+  return RTP_OK;
+}
+
+int rtp_boids_enable_profiling(rtp_boids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->prof.enabled = enable != 0;
+  return RTP_OK;
+}
+
+int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return h->prof.report(out, capacity, count);
+}
+
+// ------------------------------------------------------------------------ building blocks
+
+int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxKey,
+                   uint32_t* keysOut, uint32_t* permOut)
+{
+  RTP_REQUIRE(n == 0 || (keysIn && keysOut && permOut), "NULL argument");
+  if (n == 0)
+    return RTP_OK;
+  int rc = check_device(device);
+  if (rc)
+    return rc;
+  hipStream_t s = nullptr;
+  RTP_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  uint32_t *ka = nullptr, *kb = nullptr, *ia = nullptr, *ib = nullptr, *cnt = nullptr;
+  const SortPlan plan = make_sort_plan(n, maxKey, 0);
+  bool inB = false;
+  rc = dev_alloc(&ka, n);
+  if (!rc) rc = dev_alloc(&kb, n);
+  if (!rc) rc = dev_alloc(&ia, n);
+  if (!rc) rc = dev_alloc(&ib, n);
+  if (!rc) rc = dev_alloc(&cnt, (size_t)2048 * std::max<uint32_t>(plan.subtiles, 1));
+  if (!rc && hipMemcpyAsync(ka, keysIn, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = fail(RTP_ERR_HIP, "hipMemcpy keys");
+  if (!rc)
+    rc = sort_pairs(plan, ka, kb, ia, ib, cnt, &inB, s);
+  if (!rc && hipGetLastError() != hipSuccess)
+    rc = fail(RTP_ERR_HIP, "sort launch failed");
+  if (!rc && (hipMemcpyAsync(keysOut, inB ? kb : ka, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipMemcpyAsync(permOut, inB ? ib : ia, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess))
+    rc = fail(RTP_ERR_HIP, "hipMemcpy results");
+  if (!rc && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(RTP_ERR_HIP, "sort failed");
+  (void)hipStreamSynchronize(s);
+  dev_free(ka);
+  dev_free(kb);
+  dev_free(ia);
+  dev_free(ib);
+  dev_free(cnt);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+int rtp_grid_build(const rtp_model_desc* d, const float* pos4, uint32_t* keys, uint32_t* idx,
+                   uint32_t* startEnd)
+{
+  RTP_REQUIRE(d && (d->currNbParticles == 0 || pos4), "NULL argument");
+  RTP_REQUIRE(d->maxNbParticles > 0 && d->currNbParticles <= d->maxNbParticles,
+              "particle counts out of range");
+  int rc = check_device(d->device);
+  if (rc)
+    return rc;
+  GridState G;
+  if ((rc = make_grid(d, 100u, &G.g)))
+    return rc;
+  const uint32_t n = d->currNbParticles, maxN = d->maxNbParticles;
+  hipStream_t s = nullptr;
+  RTP_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  float4* pos = nullptr;
+  Profiler prof;
+  rc = G.alloc(maxN);
+  if (!rc)
+    rc = dev_alloc(&pos, maxN);
+  if (!rc)
+    rc = upload_f4(pos, pos4, n, maxN, make_float4(INFINITY, INFINITY, INFINITY, 0.0f), s);
+  if (!rc) {
+    G.reset_keys(s);
+    launch_fill_keys_and_reset_cells(pos, n, G.key[G.keyCur], G.startEnd, G.g, s);
+    rc = G.sort(n, s, prof);
+  }
+  if (!rc) {
+    ReorderArgs a{};
+    a.key = G.key[G.keyCur];
+    a.idx = G.idx[G.idxCur];
+    a.sortN = G.sortedN;
+    a.n = n;
+    a.maxN = maxN;
+    a.startEnd = G.startEnd;
+    a.g = G.g;
+    launch_reorder(a, s);
+    if (hipGetLastError() != hipSuccess)
+      rc = fail(RTP_ERR_HIP, "grid launch failed");
+  }
+  if (!rc)
+    rc = G.read(keys, idx, startEnd, s);
+  (void)hipStreamSynchronize(s);
+  G.release();
+  dev_free(pos);
+  prof.destroy();
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+}  // extern "C"

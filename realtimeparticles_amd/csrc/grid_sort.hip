@@ -1,0 +1,361 @@
+// Uniform-grid kernels for gfx950: cell hashing, a stable LSD radix sort of
+// (cellID, index) pairs and the cell start/end table.
+//
+// Reference behaviour reproduced here (bit-exact):
+//   fillCellIDs / resetCellIDs      physics/ocl/kernels/grid.cl:65-86
+//   RadixSort::sort                 physics/utils/RadixSort.cpp:122-189, radixSort.cl:19-188
+//   reset/fillStart/fillEnd/adjust  physics/ocl/kernels/grid.cl:91-152
+//
+// The sort is a different algorithm from the reference's (512 work-items scanning
+// contiguous chunks serially): every pass is a reduce-then-scan over wave-owned subtiles.
+//   upsweep   : each wave histograms the digits of its subtile in LDS
+//   scan      : one exclusive scan over the digit-major count table
+//   downsweep : each wave ranks its keys with ballot-based match (one ballot per digit bit,
+//               mbcnt for the rank among lower lanes), adds its subtile's digit base and
+//               scatters; a wave-private LDS counter per digit carries the running rank
+//               from one 64-key round to the next, so no workgroup barrier sits in the loop.
+// A stable sort has exactly one result, so it equals the reference's for any digit width.
+#include "rtp_device.hpp"
+
+namespace rtp {
+
+namespace {
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
+{
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// fillCellIDs over n particles plus resetStartEndCell over the grid.
+__global__ __launch_bounds__(256) void k_fill_keys_reset(const float4* __restrict__ pos, uint32_t n,
+                                                         uint32_t* __restrict__ key,
+                                                         uint2* __restrict__ startEnd, GridConsts g)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n)
+    key[t] = cell1d(pos[t], g);
+  if (t < g.numCells)
+    startEnd[t] = make_uint2(1u, 0u);
+}
+
+__global__ __launch_bounds__(256) void k_reset_cell_ids(uint32_t* __restrict__ key, uint32_t maxN,
+                                                        uint32_t numCells)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < maxN)
+    key[i] = numCells * 2u + i;
+}
+
+__global__ __launch_bounds__(256) void k_fill_identity(uint32_t* __restrict__ idx, uint32_t begin,
+                                                       uint32_t end)
+{
+  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end)
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_copy_f4(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                 uint32_t begin, uint32_t end)
+{
+  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < end)
+    dst[i] = src[i];
+}
+
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(256) void k_sort_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
+                                                      int shift, uint32_t S,
+                                                      uint32_t* __restrict__ counts)
+{
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t hist[4][RADIX];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t s = blockIdx.x * 4u + (uint32_t)w;
+  for (int d = lane; d < RADIX; d += 64)
+    hist[w][d] = 0;
+  __syncthreads();
+  if (s < S) {
+    const uint32_t base = s * (uint32_t)(ITEMS * 64);
+#pragma unroll 8
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t i = base + (uint32_t)(k * 64 + lane);
+      if (i < n)
+        atomicAdd(&hist[w][(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  if (s < S)
+    for (int d = lane; d < RADIX; d += 64)
+      counts[(size_t)d * S + s] = hist[w][d];
+}
+
+// Exclusive scan of m words with one 1024-thread block; each round covers 16 words per
+// thread (4 x uint4 loads when aligned).
+__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ data, uint32_t m)
+{
+  __shared__ uint32_t waveTot[16];
+  __shared__ uint32_t carryS;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  if (t == 0)
+    carryS = 0;
+  __syncthreads();
+  constexpr uint32_t PER = 16, ROUND = 1024 * PER;
+  for (uint32_t r0 = 0; r0 < m; r0 += ROUND) {
+    const uint32_t b = r0 + t * PER;
+    uint32_t v[PER];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k)
+      v[k] = (b + k < m) ? data[b + k] : 0u;
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k)
+      tot += v[k];
+    // inclusive wave scan of tot
+    uint32_t inc = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, 64);
+      if ((int)lane >= d)
+        inc += o;
+    }
+    if (lane == 63)
+      waveTot[w] = inc;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (uint32_t k = 0; k < w; ++k)
+      wbase += waveTot[k];
+    const uint32_t carry = carryS;
+    uint32_t run = carry + wbase + inc - tot;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      if (b + k < m)
+        data[b + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (t == 1023)
+      carryS = run;  // thread 1023 holds the round's inclusive total
+    __syncthreads();
+  }
+}
+
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(256) void k_sort_downsweep(
+  const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ idxIn, uint32_t n, int shift,
+  uint32_t S, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ keysOut,
+  uint32_t* __restrict__ idxOut)
+{
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t run[4][RADIX];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t s = blockIdx.x * 4u + (uint32_t)w;
+  if (s < S)
+    for (int d = lane; d < RADIX; d += 64)
+      run[w][d] = offsets[(size_t)d * S + s];
+  __syncthreads();
+  if (s >= S)
+    return;
+  const uint32_t base = s * (uint32_t)(ITEMS * 64);
+  uint32_t kr[ITEMS], vr[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t i = base + (uint32_t)(k * 64 + lane);
+    const bool valid = i < n;
+    kr[k] = valid ? keysIn[i] : 0u;
+    vr[k] = valid ? (idxIn ? idxIn[i] : i) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t i = base + (uint32_t)(k * 64 + lane);
+    const bool valid = i < n;
+    const uint32_t d = (kr[k] >> shift) & (uint32_t)(RADIX - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < BITS; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t m = __ballot(set);
+      peers &= set ? m : ~m;
+    }
+    const uint32_t rank = mbcnt64(peers);
+    const uint32_t start = run[w][d];
+    if (valid && rank == 0)
+      run[w][d] = start + (uint32_t)__popcll(peers);
+    if (valid) {
+      keysOut[start + rank] = kr[k];
+      idxOut[start + rank] = vr[k];
+    }
+  }
+}
+
+// Gather by idx + cell start/end (with the cap folded in: the thread that closes a run
+// knows the run's start from a probe CAP+1 entries back, and a binary search when the run
+// is longer than that).
+__global__ __launch_bounds__(256) void k_reorder(ReorderArgs a)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.sortN)
+    return;
+  const uint32_t j = a.idx[i];
+  if (a.dstA)
+    a.dstA[i] = a.srcA[j];
+  if (a.dstB)
+    a.dstB[i] = a.srcB[j];
+  if (a.dstC) {
+    float4 p = a.srcC[j];
+    if (a.clampC && i < a.n) {  // fld_applyBoundaryCondition, fluids.cl:435-439
+      p.x = clampf(p.x, a.lo[0], a.hi[0]);
+      p.y = clampf(p.y, a.lo[1], a.hi[1]);
+      p.z = clampf(p.z, a.lo[2], a.hi[2]);
+      p.w = 0.0f;
+    }
+    a.dstC[i] = p;
+  }
+  if (i >= a.n)
+    return;
+  const uint32_t k = a.key[i];
+  if (k >= a.g.numCells)  // grid.cl:107,128: only keys of real cells are recorded
+    return;
+  uint32_t* se = reinterpret_cast<uint32_t*>(a.startEnd);
+  if (i > 0 && k != a.key[i - 1])  // fillStartCell, grid.cl:101-117 (never for i == 0)
+    se[2u * k] = i;
+  // fillEndCell, grid.cl:122-138: key[i+1] is read even for the last active particle.
+  const bool isEnd = (i + 1u >= a.maxN) || (k != a.key[i + 1u]);
+  if (!isEnd)
+    return;
+  // adjustEndCell, grid.cl:143-152, with start = run start, or 1 (the reset value) when
+  // the run starts at 0.
+  const uint32_t cap = a.g.cap;
+  uint32_t endv = i;
+  if (i >= cap + 1u && a.key[i - cap - 1u] == k) {
+    uint32_t lo = 0, hi = i - cap - 1u;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.key[mid] < k)
+        lo = mid + 1u;
+      else
+        hi = mid;
+    }
+    const uint32_t startEff = lo == 0 ? 1u : lo;
+    if (i > startEff)
+      endv = startEff + min(i - startEff, cap);
+  }
+  se[2u * k + 1u] = endv;
+}
+
+template <int BITS>
+int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
+             uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
+{
+  const uint32_t blocks = (p.subtiles + 3u) / 4u;
+  hipLaunchKernelGGL((k_sort_upsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, p.n,
+                     shift, p.subtiles, counts);
+  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, counts,
+                     (uint32_t)((size_t)(1u << BITS) * p.subtiles));
+  hipLaunchKernelGGL((k_sort_downsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, iin,
+                     p.n, shift, p.subtiles, counts, kout, iout);
+  return RTP_OK;
+}
+
+int dispatch_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
+                  uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
+{
+  switch (p.bits) {
+  case 1: return run_pass<1>(p, shift, kin, iin, kout, iout, counts, s);
+  case 2: return run_pass<2>(p, shift, kin, iin, kout, iout, counts, s);
+  case 3: return run_pass<3>(p, shift, kin, iin, kout, iout, counts, s);
+  case 4: return run_pass<4>(p, shift, kin, iin, kout, iout, counts, s);
+  case 5: return run_pass<5>(p, shift, kin, iin, kout, iout, counts, s);
+  case 6: return run_pass<6>(p, shift, kin, iin, kout, iout, counts, s);
+  case 7: return run_pass<7>(p, shift, kin, iin, kout, iout, counts, s);
+  case 8: return run_pass<8>(p, shift, kin, iin, kout, iout, counts, s);
+  case 9: return run_pass<9>(p, shift, kin, iin, kout, iout, counts, s);
+  case 10: return run_pass<10>(p, shift, kin, iin, kout, iout, counts, s);
+  case 11: return run_pass<11>(p, shift, kin, iin, kout, iout, counts, s);
+  default: return fail(RTP_ERR_INVALID, "sort digit width out of range");
+  }
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+}  // namespace
+
+SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
+{
+  SortPlan p;
+  p.n = n;
+  int keyBits = 0;
+  while (keyBits < 32 && (maxKey >> keyBits) != 0u)
+    ++keyBits;
+  if (keyBits == 0)
+    keyBits = 1;
+  const int maxBits = (bitsOverride >= 1 && bitsOverride <= 11) ? bitsOverride : 9;
+  p.passes = (keyBits + maxBits - 1) / maxBits;
+  p.bits = (keyBits + p.passes - 1) / p.passes;
+  p.subtiles = (n + (uint32_t)(kSortItems * 64) - 1u) / (uint32_t)(kSortItems * 64);
+  p.countWords = ((size_t)1u << p.bits) * p.subtiles;
+  return p;
+}
+
+int sort_pairs(const SortPlan& p, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA, uint32_t* idxB,
+               uint32_t* counts, bool* resultInB, hipStream_t s)
+{
+  *resultInB = false;
+  if (p.n == 0)
+    return RTP_OK;
+  uint32_t *kin = keyA, *kout = keyB, *iin = idxA, *iout = idxB;
+  bool inB = false;
+  for (int pass = 0; pass < p.passes; ++pass) {
+    const int rc = dispatch_pass(p, pass * p.bits, kin, pass == 0 ? nullptr : iin, kout, iout,
+                                 counts, s);
+    if (rc)
+      return rc;
+    std::swap(kin, kout);
+    std::swap(iin, iout);
+    inB = !inB;
+  }
+  *resultInB = inB;
+  return RTP_OK;
+}
+
+void launch_fill_keys_and_reset_cells(const float4* pos, uint32_t n, uint32_t* key,
+                                      uint2* startEnd, const GridConsts& g, hipStream_t s)
+{
+  const uint32_t m = n > g.numCells ? n : g.numCells;
+  if (m == 0)
+    return;
+  hipLaunchKernelGGL(k_fill_keys_reset, dim3(blocks_for(m, 256)), dim3(256), 0, s, pos, n, key,
+                     startEnd, g);
+}
+
+void launch_reset_cell_ids(uint32_t* key, uint32_t maxN, uint32_t numCells, hipStream_t s)
+{
+  if (maxN == 0)
+    return;
+  hipLaunchKernelGGL(k_reset_cell_ids, dim3(blocks_for(maxN, 256)), dim3(256), 0, s, key, maxN,
+                     numCells);
+}
+
+void launch_fill_identity(uint32_t* idx, uint32_t begin, uint32_t end, hipStream_t s)
+{
+  if (end <= begin)
+    return;
+  hipLaunchKernelGGL(k_fill_identity, dim3(blocks_for(end - begin, 256)), dim3(256), 0, s, idx,
+                     begin, end);
+}
+
+void launch_copy_f4(float4* dst, const float4* src, uint32_t begin, uint32_t end, hipStream_t s)
+{
+  if (end <= begin)
+    return;
+  hipLaunchKernelGGL(k_copy_f4, dim3(blocks_for(end - begin, 256)), dim3(256), 0, s, dst, src,
+                     begin, end);
+}
+
+void launch_reorder(const ReorderArgs& a, hipStream_t s)
+{
+  if (a.sortN == 0)
+    return;
+  hipLaunchKernelGGL(k_reorder, dim3(blocks_for(a.sortN, 256)), dim3(256), 0, s, a);
+}
+
+}  // namespace rtp

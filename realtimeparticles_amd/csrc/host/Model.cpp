@@ -1,0 +1,362 @@
+// Physics::Model and Physics::HIP::{Fluids,Boids} on top of the C ABI (rtp.h).
+#include "Model.hpp"
+
+#include <cmath>
+#include <limits>
+
+namespace {
+
+void check(int rc, const char* what)
+{
+  if (rc != RTP_OK)
+    throw std::runtime_error(std::string(what) + ": " + rtp_last_error());
+}
+
+struct F3 {
+  float x, y, z;
+};
+
+// utils/Geometry.cpp:198-229 GenerateBoxGrid (uniform): start + i * (end - start) / res.
+std::vector<F3> box_grid(std::array<int, 3> res, F3 s, F3 e)
+{
+  std::vector<F3> v;
+  v.reserve((size_t)res[0] * res[1] * res[2]);
+  const F3 sp{ (e.x - s.x) / res[0], (e.y - s.y) / res[1], (e.z - s.z) / res[2] };
+  for (int ix = 0; ix < res[0]; ++ix)
+    for (int iy = 0; iy < res[1]; ++iy)
+      for (int iz = 0; iz < res[2]; ++iz)
+        v.push_back(F3{ s.x + ix * sp.x, s.y + iy * sp.y, s.z + iz * sp.z });
+  return v;
+}
+
+// utils/Geometry.cpp:242-272 GenerateSphereGrid: shells of a sphere inscribed in [s, e].
+std::vector<F3> sphere_grid(std::array<int, 3> res, F3 s, F3 e)
+{
+  const float PI_F = 3.1415927f;
+  const F3 vec{ e.x - s.x, e.y - s.y, e.z - s.z };
+  const F3 c{ s.x + vec.x / 2.0f, s.y + vec.y / 2.0f, s.z + vec.z / 2.0f };
+  const float radius = std::sqrt(vec.x * vec.x + vec.y * vec.y + vec.z * vec.z) / 2.0f;
+  const float dphi = PI_F / res[0], dtheta = 2.0f * PI_F / res[1], dr = radius / res[2];
+  std::vector<F3> v;
+  v.reserve((size_t)res[0] * res[1] * res[2]);
+  for (int ip = 0; ip < res[0]; ++ip)
+    for (int it = 0; it < res[1]; ++it)
+      for (int ir = 0; ir < res[2]; ++ir) {
+        const float r = (ir + 1) * dr;
+        v.push_back(F3{ c.x + r * std::cos(it * dtheta) * std::sin(ip * dphi),
+                        c.y + r * std::sin(it * dtheta) * std::sin(ip * dphi),
+                        c.z + r * std::cos(ip * dphi) });
+      }
+  return v;
+}
+
+std::vector<float> to_float4(const std::vector<F3>& v)
+{
+  std::vector<float> out(v.size() * 4, 0.0f);
+  for (size_t i = 0; i < v.size(); ++i) {
+    out[4 * i + 0] = v[i].x;
+    out[4 * i + 1] = v[i].y;
+    out[4 * i + 2] = v[i].z;
+  }
+  return out;
+}
+
+rtp_model_desc make_desc(const Physics::ModelParams& p)
+{
+  rtp_model_desc d{};
+  d.maxNbParticles = (uint32_t)p.maxNbParticles;
+  d.currNbParticles = (uint32_t)p.currNbParticles;
+  d.boxSize[0] = p.boxSize.x;
+  d.boxSize[1] = p.boxSize.y;
+  d.boxSize[2] = p.boxSize.z;
+  d.gridRes[0] = p.gridRes.x;
+  d.gridRes[1] = p.gridRes.y;
+  d.gridRes[2] = p.gridRes.z;
+  d.maxPartsInCell = 0;
+  d.gravity = p.gravity;
+  d.device = p.device;
+  return d;
+}
+
+}  // namespace
+
+namespace Utils {
+std::array<int, 3> GetNbParticlesSubdiv3D(NbParticles nb)
+{
+  switch (nb) {
+  case P512: return { 8, 8, 8 };
+  case P1K: return { 16, 8, 8 };
+  case P4K: return { 16, 16, 16 };
+  case P8K: return { 32, 16, 16 };
+  case P16K: return { 32, 32, 16 };
+  case P32K: return { 32, 32, 32 };
+  case P65K: return { 64, 32, 32 };
+  case P130K: return { 64, 64, 32 };
+  }
+  return { 0, 0, 0 };
+}
+}  // namespace Utils
+
+namespace Physics {
+
+std::unique_ptr<Model> CreateModel(ModelType type, ModelParams params)
+{
+  switch ((int)type) {
+  case ModelType::BOIDS: return std::make_unique<HIP::Boids>(params);
+  case ModelType::FLUIDS: return std::make_unique<HIP::Fluids>(params);
+  default: return nullptr;
+  }
+}
+
+Model::Model(const ModelParams& params)
+    : m_maxNbParticles(params.maxNbParticles)
+    , m_currNbParticles(params.currNbParticles)
+    , m_boxSize(params.boxSize)
+    , m_gridRes(params.gridRes)
+    , m_nbCells((size_t)params.gridRes.x * params.gridRes.y * params.gridRes.z)
+    , m_dimension(params.dimension)
+    , m_case(params.pCase)
+    , m_device(params.device)
+{
+}
+
+void Model::setNbParticles(size_t nb)
+{
+  m_currNbParticles = nb;
+  if (m_init)
+    applyNbParticles();
+}
+
+void Model::pause(bool p)
+{
+  m_pause = p;
+  if (m_init)
+    applyPause();
+}
+
+void Model::enableProfiling(bool enable)
+{
+  m_profiling = enable;
+  if (m_init)
+    applyProfiling();
+}
+
+namespace HIP {
+
+// ------------------------------------------------------------------------------ Fluids
+
+Fluids::Fluids(const ModelParams& params) : Model(params)
+{
+  const rtp_model_desc d = make_desc(params);
+  check(rtp_fluids_create(&d, &m_h), "rtp_fluids_create");
+  check(rtp_fluids_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_fluids_set_nb_jacobi_iters");
+  m_init = true;
+  if (m_case != Utils::CASE_INVALID)
+    reset();
+}
+
+Fluids::~Fluids() { rtp_fluids_destroy(m_h); }
+
+void Fluids::update()
+{
+  if (!m_init)
+    return;
+  check(rtp_fluids_update(m_h, 1), "rtp_fluids_update");
+  check(rtp_fluids_sync(m_h), "rtp_fluids_sync");  // a frame ends synchronised (Context.cpp:746)
+}
+
+// Fluids::reset / initFluidsParticles (Fluids.cpp:196-216, 335-398), 3-D cases.
+void Fluids::reset()
+{
+  if (!m_init)
+    return;
+  const rtp_fluid_params defaults{ 450.0f, 600.0f, 0.010f, 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
+  check(rtp_fluids_set_params(m_h, &defaults), "rtp_fluids_set_params");
+  m_nbJacobiIters = 2;
+  check(rtp_fluids_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_fluids_set_nb_jacobi_iters");
+  const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
+  std::vector<F3> verts;
+  switch (m_case) {
+  case Utils::FLUIDS_DAM:
+    m_currNbParticles = Utils::P130K;
+    verts = box_grid(Utils::GetNbParticlesSubdiv3D(Utils::P130K), F3{ b.x / -2.0f, b.y / -2.0f, b.z / -2.0f },
+                     F3{ b.x / 2.0f, 0.0f, 0.0f });
+    break;
+  case Utils::FLUIDS_BOMB:
+    m_currNbParticles = Utils::P65K;
+    verts = sphere_grid(Utils::GetNbParticlesSubdiv3D(Utils::P65K), F3{ b.x / -6.0f, b.y / -6.0f, b.z / -6.0f },
+                        F3{ b.x / 6.0f, b.y / 6.0f, b.z / 6.0f });
+    break;
+  case Utils::FLUIDS_DROP: {
+    m_currNbParticles = Utils::P4K + Utils::P65K;
+    verts = box_grid(Utils::GetNbParticlesSubdiv3D(Utils::P4K),
+                     F3{ b.x / -10.0f, 2.0f * b.y / 10.0f, b.z / -10.0f },
+                     F3{ b.x / 10.0f, 4.0f * b.y / 10.0f, b.z / 10.0f });
+    const auto bottom = box_grid({ 64, 16, 64 }, F3{ b.x / -2.0f, b.y / -2.0f, b.z / -2.0f },
+                                 F3{ b.x / 2.0f, b.y / -2.55f, b.z / 2.0f });
+    verts.insert(verts.end(), bottom.begin(), bottom.end());
+    break;
+  }
+  default:
+    throw std::runtime_error("Fluids::reset: unknown case");
+  }
+  if (m_currNbParticles > m_maxNbParticles)
+    throw std::runtime_error("Fluids::reset: case needs more particles than maxNbParticles");
+  const std::vector<float> pos = to_float4(verts);
+  const std::vector<float> vel(pos.size(), 0.0f);
+  check(rtp_fluids_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_fluids_set_nb_particles");
+  check(rtp_fluids_load_state(m_h, pos.data(), vel.data(), (uint32_t)verts.size()), "rtp_fluids_load_state");
+}
+
+void Fluids::setFluidParams(const rtp_fluid_params& p) { check(rtp_fluids_set_params(m_h, &p), "rtp_fluids_set_params"); }
+
+rtp_fluid_params Fluids::fluidParams() const
+{
+  rtp_fluid_params p{};
+  check(rtp_fluids_get_params(m_h, &p), "rtp_fluids_get_params");
+  return p;
+}
+
+void Fluids::setNbJacobiIters(uint32_t iters)
+{
+  check(rtp_fluids_set_nb_jacobi_iters(m_h, iters), "rtp_fluids_set_nb_jacobi_iters");
+  m_nbJacobiIters = iters;
+}
+
+void Fluids::loadState(const float* pos4, const float* vel4, size_t count)
+{
+  check(rtp_fluids_load_state(m_h, pos4, vel4, (uint32_t)count), "rtp_fluids_load_state");
+}
+
+void Fluids::readState(float* pos4, float* vel4, size_t count) const
+{
+  check(rtp_fluids_read_state(m_h, pos4, vel4, (uint32_t)count), "rtp_fluids_read_state");
+}
+
+void* Fluids::devicePositions() const
+{
+  void* p = nullptr;
+  check(rtp_fluids_device_state(m_h, &p, nullptr), "rtp_fluids_device_state");
+  return p;
+}
+
+std::vector<rtp_kernel_time> Fluids::kernelTimes() const
+{
+  int count = 0;
+  check(rtp_fluids_profile(m_h, nullptr, 0, &count), "rtp_fluids_profile");
+  std::vector<rtp_kernel_time> out((size_t)count);
+  check(rtp_fluids_profile(m_h, out.data(), count, &count), "rtp_fluids_profile");
+  return out;
+}
+
+std::array<float, 3> Fluids::densityError() const
+{
+  std::array<float, 3> e{};
+  check(rtp_fluids_density_error(m_h, e.data()), "rtp_fluids_density_error");
+  return e;
+}
+
+void Fluids::applyNbParticles()
+{
+  check(rtp_fluids_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_fluids_set_nb_particles");
+}
+
+void Fluids::applyPause() { check(rtp_fluids_pause(m_h, m_pause ? 1 : 0), "rtp_fluids_pause"); }
+
+void Fluids::applyProfiling()
+{
+  check(rtp_fluids_enable_profiling(m_h, m_profiling ? 1 : 0), "rtp_fluids_enable_profiling");
+}
+
+// ------------------------------------------------------------------------------- Boids
+
+Boids::Boids(const ModelParams& params) : Model(params)
+{
+  const rtp_model_desc d = make_desc(params);
+  check(rtp_boids_create(&d, &m_h), "rtp_boids_create");
+  m_init = true;
+  if (m_case != Utils::CASE_INVALID)
+    reset();
+}
+
+Boids::~Boids() { rtp_boids_destroy(m_h); }
+
+void Boids::update()
+{
+  if (!m_init)
+    return;
+  if (m_boundary != Boundary::BouncingWall)
+    throw std::runtime_error("Boids::update: only the bouncing wall is provided");
+  check(rtp_boids_update(m_h, 1), "rtp_boids_update");
+  check(rtp_boids_sync(m_h), "rtp_boids_sync");
+}
+
+// Boids::reset / initBoidsParticles (Boids.cpp:228-321), 3-D: a sphere lattice, with the
+// velocity buffer initialised from the positions.
+void Boids::reset()
+{
+  if (!m_init)
+    return;
+  const rtp_boids_params defaults{ 0.5f, 1.6f, 1.6f, 1.45f };
+  check(rtp_boids_set_params(m_h, &defaults), "rtp_boids_set_params");
+  switch (m_case) {
+  case Utils::BOIDS_SMALL: m_currNbParticles = Utils::P512; break;
+  case Utils::BOIDS_MEDIUM: m_currNbParticles = Utils::P16K; break;
+  case Utils::BOIDS_LARGE: m_currNbParticles = Utils::P65K; break;
+  case Utils::BOIDS_XLARGE: m_currNbParticles = Utils::P130K; break;
+  default: throw std::runtime_error("Boids::reset: unknown case");
+  }
+  if (m_currNbParticles > m_maxNbParticles)
+    throw std::runtime_error("Boids::reset: case needs more particles than maxNbParticles");
+  const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
+  const auto verts = sphere_grid(Utils::GetNbParticlesSubdiv3D((Utils::NbParticles)m_currNbParticles),
+                                 F3{ b.x / -6.0f, b.y / -6.0f, b.z / -6.0f }, F3{ b.x / 6.0f, b.y / 6.0f, b.z / 6.0f });
+  const std::vector<float> pos = to_float4(verts);
+  check(rtp_boids_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_boids_set_nb_particles");
+  check(rtp_boids_load_state(m_h, pos.data(), pos.data(), (uint32_t)verts.size()), "rtp_boids_load_state");
+}
+
+void Boids::setBoidsParams(const rtp_boids_params& p) { check(rtp_boids_set_params(m_h, &p), "rtp_boids_set_params"); }
+
+void Boids::setTimeStep(float dt) { check(rtp_boids_set_time_step(m_h, dt), "rtp_boids_set_time_step"); }
+
+void Boids::loadState(const float* pos4, const float* vel4, size_t count)
+{
+  check(rtp_boids_load_state(m_h, pos4, vel4, (uint32_t)count), "rtp_boids_load_state");
+}
+
+void Boids::readState(float* pos4, float* vel4, size_t count) const
+{
+  check(rtp_boids_read_state(m_h, pos4, vel4, (uint32_t)count), "rtp_boids_read_state");
+}
+
+void* Boids::devicePositions() const
+{
+  void* p = nullptr;
+  check(rtp_boids_device_state(m_h, &p, nullptr), "rtp_boids_device_state");
+  return p;
+}
+
+std::vector<rtp_kernel_time> Boids::kernelTimes() const
+{
+  int count = 0;
+  check(rtp_boids_profile(m_h, nullptr, 0, &count), "rtp_boids_profile");
+  std::vector<rtp_kernel_time> out((size_t)count);
+  check(rtp_boids_profile(m_h, out.data(), count, &count), "rtp_boids_profile");
+  return out;
+}
+
+void Boids::applyNbParticles()
+{
+  check(rtp_boids_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_boids_set_nb_particles");
+}
+
+void Boids::applyPause() { check(rtp_boids_pause(m_h, m_pause ? 1 : 0), "rtp_boids_pause"); }
+
+void Boids::applyProfiling()
+{
+  check(rtp_boids_enable_profiling(m_h, m_profiling ? 1 : 0), "rtp_boids_enable_profiling");
+}
+
+}  // namespace HIP
+}  // namespace Physics

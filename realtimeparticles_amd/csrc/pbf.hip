@@ -1,0 +1,454 @@
+// Position based fluids solver kernels for gfx950.
+//
+// Reference: physics/ocl/kernels/fluids.cl and sph.cl, step order physics/ocl/Fluids.cpp:400-471.
+// The reference runs 5 kernels per Jacobi iteration and 6 after them; here the work of one
+// substep is fused into 2 neighbour passes per iteration plus 3 after them:
+//
+//   predict  : fld_predictPosition + fillCellIDs + resetStartEndCell          (streaming)
+//   [sort + reorder: grid_sort.hip; the reorder applies the first boundary clamp]
+//   A_k      : fld_computeDensity + fld_computeConstraintFactor               (neighbours)
+//   B_k      : fld_computeConstraintCorrection + fld_correctPosition, then the next
+//              iteration's fld_applyBoundaryCondition, or for the last iteration
+//              fld_updateVel (+ fld_updatePosition when vorticity is off)    (neighbours)
+//   C        : fld_computeVorticity (w lane carries |vorticity|)              (neighbours)
+//   D        : fld_applyVorticityConfinement -> velocity copy                  (neighbours)
+//   E        : fld_applyXsphViscosityCorrection + fld_updatePosition           (neighbours)
+//
+// Fusion keeps each particle's summation order: every pass walks the 27 cells in the
+// reference's order (x outer, z inner, periodic wrap) and each cell from start to end.
+// Candidates whose squared distance proves r >= h contribute +-0 in the reference and
+// are skipped; candidates that are very far (d2 >= kFarD2, where the reference's
+// zero-weighted poly6 overflows to NaN) or NaN take the reference's exact expressions.
+#include "rtp_device.hpp"
+
+namespace rtp {
+
+namespace {
+
+constexpr float kFarD2 = 1.0e12f;
+
+// Visit the candidates of particle position p in the reference's order.  f is called for
+// every candidate that is not provably a +-0 contribution.
+template <class F>
+__device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
+                                       const uint2* __restrict__ se, const PbfConsts& c, F&& f)
+{
+  uint32_t cx, cy, cz;
+  cell3d(p, c.g, cx, cy, cz);
+  const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
+  for (int ix = -1; ix <= 1; ++ix) {
+    const int32_t nx = wrap((int32_t)cx + ix, rx);
+    for (int iy = -1; iy <= 1; ++iy) {
+      const int32_t row = (nx * ry + wrap((int32_t)cy + iy, ry)) * rz;
+      for (int iz = -1; iz <= 1; ++iz) {
+        const uint2 r = se[row + wrap((int32_t)cz + iz, rz)];
+        for (uint32_t e = r.x; e <= r.y; ++e) {
+          const float4 q = pred[e];
+          const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (!(d2 >= c.cullSq && d2 < kFarD2))
+            f(e, dx, dy, dz, d2);
+        }
+      }
+    }
+  }
+}
+
+// sph.cl:26-34 for r > FLOAT_EPS with (1 - step(h, r)) == t.
+struct Grad {
+  float x, y, z;
+};
+__device__ __forceinline__ Grad grad_spiky(float dx, float dy, float dz, float r, float t,
+                                           const PbfConsts& c)
+{
+  const float p2 = (c.h - r) * (c.h - r);
+  Grad g;
+  g.x = (((dx * t) * c.spiky) * -3.0f) * p2 / r;
+  g.y = (((dy * t) * c.spiky) * -3.0f) * p2 / r;
+  g.z = (((dz * t) * c.spiky) * -3.0f) * p2 / r;
+  return g;
+}
+
+// sph.cl:10-14 with (1 - step(h, r)) == t.
+__device__ __forceinline__ float poly6(float r, float t, const PbfConsts& c)
+{
+  return (t * c.poly6) * cube(c.hh - r * r);
+}
+
+// fluids.cl:51-57 given poly6(vec).
+__device__ __forceinline__ float art_pressure(float w, const PbfConsts& c)
+{
+  if (c.p.isArtPressureEnabled == 0)
+    return 0.0f;
+  return -c.p.artPressureCoeff * pown(w / c.wdq, c.p.artPressureExp);
+}
+
+__device__ __forceinline__ uint32_t particle_index(int remap)
+{
+  return logical_block(remap) * blockDim.x + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
+                                                 const float4* __restrict__ vel,
+                                                 float4* __restrict__ pred,
+                                                 uint32_t* __restrict__ key,
+                                                 uint2* __restrict__ startEnd, uint32_t n,
+                                                 PbfConsts c)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    // fluids.cl:71-73: newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt
+    const float4 p = pos[t], v = vel[t];
+    const float dt = c.p.timeStep;
+    const float zdt = 0.0f * dt;
+    const float vx = v.x + zdt, vy = v.y + c.gdt, vz = v.z + zdt;
+    float4 q;
+    q.x = p.x + vx * dt;
+    q.y = p.y + vy * dt;
+    q.z = p.z + vz * dt;
+    q.w = 0.0f;
+    pred[t] = q;
+    key[t] = cell1d(q, c.g);  // grid.cl:76-86 on predPos
+  }
+  if (t < c.g.numCells)
+    startEnd[t] = make_uint2(1u, 0u);  // grid.cl:91-96
+}
+
+// fld_computeDensity (fluids.cl:80-126) + fld_computeConstraintFactor (:131-193).
+__global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict__ pred,
+                                                        const uint2* __restrict__ se,
+                                                        float* __restrict__ density,
+                                                        float* __restrict__ lambda, uint32_t n,
+                                                        PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i >= n)
+    return;
+  const float4 p = pred[i];
+  float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
+  walk27(p, pred, se, c, [&](uint32_t, float dx, float dy, float dz, float d2) {
+    const float r = __builtin_sqrtf(d2);
+    const float t = r < c.h ? 1.0f : 0.0f;
+    dens = dens + poly6(r, t, c);
+    if (r > kFloatEps) {
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      sgx = sgx + g.x;
+      sgy = sgy + g.y;
+      sgz = sgz + g.z;
+      ssq = ssq + (g.x * g.x + g.y * g.y + g.z * g.z);
+    }
+  });
+  density[i] = dens;
+  const float C = dens / c.p.restDensity - 1.0f;
+  float sq = ssq + (sgx * sgx + sgy * sgy + sgz * sgz);
+  sq = sq / c.rho0sq;
+  lambda[i] = -C / (sq + c.p.relaxCFM);
+}
+
+// fld_computeConstraintCorrection (fluids.cl:198-247) + fld_correctPosition (:252-258),
+// then fld_applyBoundaryCondition of the next iteration, or fld_updateVel (:263-273) and,
+// without vorticity, fld_updatePosition (:444-450).
+__global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred,
+                                                 const float* __restrict__ lambda,
+                                                 const uint2* __restrict__ se,
+                                                 float4* __restrict__ predOut,
+                                                 const float4* __restrict__ pos,
+                                                 float4* __restrict__ vel,
+                                                 float4* __restrict__ posOut, uint32_t n,
+                                                 int last, PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i >= n)
+    return;
+  const float4 p = pred[i];
+  const float li = lambda[i];
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
+    const float r = __builtin_sqrtf(d2);
+    const float t = r < c.h ? 1.0f : 0.0f;
+    const float s = li + lambda[e] + art_pressure(poly6(r, t, c), c);
+    if (r > kFloatEps) {
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      cx = cx + s * g.x;
+      cy = cy + s * g.y;
+      cz = cz + s * g.z;
+    } else {  // gradSpiky returns the zero vector
+      cx = cx + s * 0.0f;
+      cy = cy + s * 0.0f;
+      cz = cz + s * 0.0f;
+    }
+  });
+  float4 q;
+  q.x = p.x + cx / c.p.restDensity;
+  q.y = p.y + cy / c.p.restDensity;
+  q.z = p.z + cz / c.p.restDensity;
+  q.w = 0.0f;
+  if (!last) {
+    q.x = clampf(q.x, c.lo[0], c.hi[0]);
+    q.y = clampf(q.y, c.lo[1], c.hi[1]);
+    q.z = clampf(q.z, c.lo[2], c.hi[2]);
+    predOut[i] = q;
+    return;
+  }
+  predOut[i] = q;
+  const float4 o = pos[i];
+  float4 v;
+  v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
+  v.y = clampf((q.y - o.y) / c.dtEps, -c.maxVel, c.maxVel);
+  v.z = clampf((q.z - o.z) / c.dtEps, -c.maxVel, c.maxVel);
+  v.w = 0.0f;
+  vel[i] = v;
+  if (posOut)
+    posOut[i] = q;
+}
+
+// fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
+__global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ pred,
+                                                    const float4* __restrict__ pos,
+                                                    float4* __restrict__ vel,
+                                                    float4* __restrict__ posOut, uint32_t n,
+                                                    PbfConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 q = pred[i], o = pos[i];
+  float4 v;
+  v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
+  v.y = clampf((q.y - o.y) / c.dtEps, -c.maxVel, c.maxVel);
+  v.z = clampf((q.z - o.z) / c.dtEps, -c.maxVel, c.maxVel);
+  v.w = 0.0f;
+  vel[i] = v;
+  if (posOut)
+    posOut[i] = make_float4(q.x, q.y, q.z, 0.0f);
+}
+
+// fld_computeVorticity (fluids.cl:278-324).  The w lane of the output carries
+// fast_length(vort) for the confinement pass.
+__global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pred,
+                                                   const uint2* __restrict__ se,
+                                                   const float4* __restrict__ vel,
+                                                   float4* __restrict__ vort, uint32_t n,
+                                                   PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i >= n)
+    return;
+  const float4 p = pred[i];
+  const float4 vi = vel[i];
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
+    const float r = __builtin_sqrtf(d2);
+    if (r > kFloatEps) {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      const float4 ve = vel[e];
+      const float ax = ve.x - vi.x, ay = ve.y - vi.y, az = ve.z - vi.z;
+      wx = wx + (ay * g.z - az * g.y);
+      wy = wy + (az * g.x - ax * g.z);
+      wz = wz + (ax * g.y - ay * g.x);
+    }
+  });
+  vort[i] = make_float4(wx, wy, wz, __builtin_sqrtf(wx * wx + wy * wy + wz * wz));
+}
+
+// fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
+// velOut (the copy fld_applyXsphViscosityCorrection reads, Fluids.cpp:451).
+__global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ pred,
+                                                     const uint2* __restrict__ se,
+                                                     const float4* __restrict__ vort,
+                                                     const float4* __restrict__ vel,
+                                                     float4* __restrict__ velOut, uint32_t n,
+                                                     PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i >= n)
+    return;
+  const float4 p = pred[i];
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
+    const float r = __builtin_sqrtf(d2);
+    if (r > kFloatEps) {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      const float l = vort[e].w;
+      nx = nx + l * g.x;
+      ny = ny + l * g.y;
+      nz = nz + l * g.z;
+    }
+  });
+  // normalize(n), 0 for the zero vector
+  if (!(nx == 0.0f && ny == 0.0f && nz == 0.0f)) {
+    const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx / len;
+    ny = ny / len;
+    nz = nz / len;
+  }
+  const float4 w = vort[i];
+  const float crx = ny * w.z - nz * w.y;
+  const float cry = nz * w.x - nx * w.z;
+  const float crz = nx * w.y - ny * w.x;
+  const float k = c.p.vorticityConfCoeff, dt = c.p.timeStep;
+  const float4 v = vel[i];
+  velOut[i] = make_float4(v.x + (k * crx) * dt, v.y + (k * cry) * dt, v.z + (k * crz) * dt, 0.0f);
+}
+
+// fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
+__global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
+                                              const uint2* __restrict__ se,
+                                              const float4* __restrict__ velIn,
+                                              float4* __restrict__ velOut,
+                                              float4* __restrict__ posOut, uint32_t n,
+                                              PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i >= n)
+    return;
+  const float4 p = pred[i];
+  const float4 vi = velIn[i];
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  walk27(p, pred, se, c, [&](uint32_t e, float, float, float, float d2) {
+    const float r = __builtin_sqrtf(d2);
+    const float t = r < c.h ? 1.0f : 0.0f;
+    const float w = poly6(r, t, c);
+    const float4 ve = velIn[e];
+    sx = sx + (ve.x - vi.x) * w;
+    sy = sy + (ve.y - vi.y) * w;
+    sz = sz + (ve.z - vi.z) * w;
+  });
+  const float k = c.p.xsphViscosityCoeff;
+  velOut[i] = make_float4(vi.x + k * sx, vi.y + k * sy, vi.z + k * sz, 0.0f);
+  posOut[i] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+__global__ __launch_bounds__(64) void k_zero3(float* __restrict__ out)
+{
+  if (threadIdx.x < 3)
+    out[threadIdx.x] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_density_error(const float* __restrict__ density, uint32_t n,
+                                                       float rho0, float* __restrict__ out)
+{
+  __shared__ float smax[4], ssum[4], scmp[4];
+  float mx = 0.0f, sm = 0.0f, cm = 0.0f;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float e = density[i] / rho0 - 1.0f;
+    const float a = fabsf(e);
+    mx = fmaxf(mx, a);
+    sm += a;
+    cm = fmaxf(cm, e);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    sm += __shfl_xor(sm, o, 64);
+    cm = fmaxf(cm, __shfl_xor(cm, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smax[w] = mx;
+    ssum[w] = sm;
+    scmp[w] = cm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      mx = fmaxf(mx, smax[k]);
+      sm += ssum[k];
+      cm = fmaxf(cm, scmp[k]);
+    }
+    // non-negative floats order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(mx));
+    atomicAdd(out + 1, sm);
+    atomicMax(reinterpret_cast<unsigned int*>(out + 2), __float_as_uint(cm));
+  }
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+}  // namespace
+
+void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint32_t* key,
+                        uint2* startEnd, uint32_t n, const PbfConsts& c, hipStream_t s)
+{
+  const uint32_t m = n > c.g.numCells ? n : c.g.numCells;
+  if (m == 0)
+    return;
+  hipLaunchKernelGGL(k_predict, dim3(blocks_for(m, 256)), dim3(256), 0, s, pos, vel, pred, key,
+                     startEnd, n, c);
+}
+
+void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
+                               float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
+                               hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_density_lambda, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
+                     startEnd, density, lambda, n, c, L.xcdRemap);
+}
+
+void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
+                        float4* predOut, const float4* pos, float4* vel, float4* posOut,
+                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_correct, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred, lambda,
+                     startEnd, predOut, pos, vel, posOut, n, last, c, L.xcdRemap);
+}
+
+void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
+                           uint32_t n, const PbfConsts& c, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_update_vel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, pos, vel,
+                     posOut, n, c);
+}
+
+void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
+                          float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
+                          hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_vorticity, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
+                     startEnd, vel, vort, n, c, L.xcdRemap);
+}
+
+void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
+                            const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
+                            const Launch& L, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_confinement, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
+                     startEnd, vort, vel, velOut, n, c, L.xcdRemap);
+}
+
+void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
+                     float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
+                     const Launch& L, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred, startEnd,
+                     velIn, velOut, posOut, n, c, L.xcdRemap);
+}
+
+void launch_density_error(const float* density, uint32_t n, float rho0, float* scratch,
+                          hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, scratch);
+  if (n == 0)
+    return;
+  uint32_t blocks = blocks_for(n, 256);
+  if (blocks > 2048)
+    blocks = 2048;
+  hipLaunchKernelGGL(k_density_error, dim3(blocks), dim3(256), 0, s, density, n, rho0, scratch);
+}
+
+}  // namespace rtp

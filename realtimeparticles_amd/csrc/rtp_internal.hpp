@@ -1,0 +1,138 @@
+// Internal declarations shared by the HIP translation units of librtp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "rtp.h"
+
+namespace rtp {
+
+// Records a failure for rtp_last_error() and returns the status.
+int fail(int status, const std::string& msg);
+
+#define RTP_HIP_TRY(expr)                                                                     \
+  do {                                                                                        \
+    hipError_t err_ = (expr);                                                                 \
+    if (err_ != hipSuccess)                                                                   \
+      return ::rtp::fail(RTP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(err_));   \
+  } while (0)
+
+// The reference's compile-time "-D" constants of grid.cl (Fluids.cpp:107-116,
+// Boids.cpp:104-113), passed to kernels by value instead of being baked into the source.
+struct GridConsts {
+  float absWall[3];   // ABS_WALL_X/Y/Z
+  float cellSize;     // GRID_CELL_SIZE_XYZ
+  int32_t res[3];     // GRID_RES_X/Y/Z
+  uint32_t numCells;  // GRID_NUM_CELLS
+  uint32_t cap;       // NUM_MAX_PARTS_IN_CELL
+};
+
+// Everything the PBF passes read besides buffers: FluidParams (define.cl:13-25), the
+// remaining program constants (Fluids.cpp:104-119) and values the host folds once.
+struct PbfConsts {
+  GridConsts g;
+  rtp_fluid_params p;
+  float h;         // EFFECT_RADIUS
+  float hh;        // EFFECT_RADIUS * EFFECT_RADIUS (rounded once, as the kernel's expression)
+  float cullSq;    // smallest float >= h*h (exact); d2 >= cullSq implies sqrt(d2) >= h
+  float poly6;     // POLY6_COEFF
+  float spiky;     // SPIKY_COEFF
+  float wdq;       // poly6L(artPressureRadius * h, h) (fluids.cl:56)
+  float maxVel;    // MAX_VEL
+  float gravity;   // ABS_GRAVITY_ACC_Y
+  float rho0sq;    // restDensity * restDensity (fluids.cl:190)
+  float dtEps;     // timeStep + FLOAT_EPS (fluids.cl:272)
+  float gdt;       // -ABS_GRAVITY_ACC_Y * timeStep (fluids.cl:71)
+  float lo[3];     // fld_applyBoundaryCondition lower bounds (fluids.cl:437)
+  float hi[3];     // upper bounds (fluids.cl:438)
+};
+
+struct BoidsConsts {
+  GridConsts g;
+  rtp_boids_params p;
+  float radiusSq;  // EFFECT_RADIUS_SQUARED
+  float dt;        // update time step
+  float vmin;      // 0.2f * maxVelocity (boids.cl:256)
+};
+
+// Launch shape shared by the per-particle kernels.
+struct Launch {
+  int block = 256;     // threads per block
+  int xcdRemap = 1;    // consecutive particle blocks on one XCD
+};
+
+// ---------------------------------------------------------------------------- grid / sort
+// Stable LSD radix sort of (key, index) pairs on the device.
+struct SortPlan {
+  uint32_t n = 0;          // keys to sort
+  int passes = 0;
+  int bits = 0;            // digit width
+  uint32_t subtiles = 0;   // wave-owned subtiles of kSortItems*64 keys
+  size_t countWords = 0;   // radix * subtiles
+};
+constexpr int kSortItems = 32;  // keys per lane in one subtile
+SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride);
+// keys/idx ping-pong through (keyA, idxA) <-> (keyB, idxB); returns whether the sorted
+// result ended in the B buffers.  idxIn == nullptr means the identity permutation.
+int sort_pairs(const SortPlan& plan, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA,
+               uint32_t* idxB, uint32_t* counts, bool* resultInB, hipStream_t s);
+
+void launch_fill_keys_and_reset_cells(const float4* pos, uint32_t n, uint32_t* key,
+                                      uint2* startEnd, const GridConsts& g, hipStream_t s);
+void launch_reset_cell_ids(uint32_t* key, uint32_t maxN, uint32_t numCells, hipStream_t s);
+void launch_fill_identity(uint32_t* idx, uint32_t begin, uint32_t end, hipStream_t s);
+// Gather float4 payloads by idx over [0, sortN) and fill cell start/end over [0, n);
+// payload C of the active range [0, n) is clamped to the PBF walls when clampC is set.
+struct ReorderArgs {
+  const uint32_t* key;
+  const uint32_t* idx;
+  uint32_t sortN, n, maxN;
+  const float4* srcA; float4* dstA;
+  const float4* srcB; float4* dstB;
+  const float4* srcC; float4* dstC;
+  int clampC;
+  float lo[3], hi[3];
+  uint2* startEnd;
+  GridConsts g;
+};
+void launch_reorder(const ReorderArgs& a, hipStream_t s);
+void launch_copy_f4(float4* dst, const float4* src, uint32_t begin, uint32_t end, hipStream_t s);
+
+// ---------------------------------------------------------------------------------- PBF
+void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint32_t* key,
+                        uint2* startEnd, uint32_t n, const PbfConsts& c, hipStream_t s);
+void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
+                               float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
+                               hipStream_t s);
+// Constraint correction + correctPosition; then either the next iteration's boundary clamp
+// (last == 0) or updateVel (+ updatePosition when vorticity is off) (last != 0).
+void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
+                        float4* predOut, const float4* pos, float4* vel, float4* posOut,
+                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s);
+void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
+                           uint32_t n, const PbfConsts& c, hipStream_t s);
+void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
+                          float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
+                          hipStream_t s);
+void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
+                            const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
+                            const Launch& L, hipStream_t s);
+void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
+                     float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
+                     const Launch& L, hipStream_t s);
+// Reduction of |rho/rho0 - 1| over n particles into out[0..2] (max, sum, max compression);
+// scratch holds 3 floats (zeroed by the launcher's own kernel).
+void launch_density_error(const float* density, uint32_t n, float rho0, float* scratch,
+                          hipStream_t s);
+
+// -------------------------------------------------------------------------------- boids
+void launch_boids_hash(const float4* pos, uint32_t* key, uint2* startEnd, uint32_t n,
+                       const GridConsts& g, hipStream_t s);
+void launch_boids_step(const float4* pos, const float4* vel, const uint2* startEnd,
+                       float4* posOut, float4* velOut, float4* acc, uint32_t n,
+                       const BoidsConsts& c, const Launch& L, hipStream_t s);
+
+}  // namespace rtp

@@ -1,0 +1,146 @@
+"""Seeded synthetic scenes for the PBF dam break and the 3-D boids flock.
+
+Every workload in the benchmark and the test-suite is generated here, so the HIP path and
+the CPU oracle always start from the same bytes.  Random numbers come from SplitMix64
+(counter-based, so any slice of a stream can be generated independently) mapped to 24-bit
+uniform floats in [0, 1).
+
+Scene sizes:
+  boids-8k      8,192 boids in a 2^3 box, 20^3 grid (cell = vision radius 0.1)
+  pbf-65k       64x32x32 jittered lattice, 4x2x2 box, 40x20x20 grid, h = 0.1
+  pbf-1m        128x128x64 lattice, 8x8x4 box, 80x80x40 grid
+  pbf-4m        256x128x128 lattice, 16x8x8 box, 160x80x80 grid
+Each PBF lattice fills 80 % of every box edge from the (-x,-y,-z) corner, spacing 0.5 h,
+jitter U[-0.05 h, +0.05 h] per axis, velocity 0 (the dam break of
+physics/ocl/Fluids.cpp:341-345 with the lattice of utils/Geometry.cpp:198-229).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
+    """Outputs start..start+count-1 of the SplitMix64 stream seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform24(seed: int, start: int, count: int) -> np.ndarray:
+    """Uniform float32 in [0, 1) with 24 random bits (exact in float32)."""
+    z = splitmix64(seed, start, count) >> np.uint64(40)
+    return (z.astype(np.float64) * (1.0 / 16777216.0)).astype(np.float32)
+
+
+def lattice_rest_density(h: float) -> float:
+    """Closed-form poly6 density of a simple-cubic lattice of spacing h/2, unit mass.
+
+    Neighbours with r < h sit at |offset|^2 in {0, 1, 2, 3} (h/2)^2 with multiplicities
+    1, 6, 12, 8, so rho0 = 315/(64 pi h^3) * (1 + 6(3/4)^3 + 12(1/2)^3 + 8(1/4)^3)
+    = 315 * 5.15625 / (64 pi h^3).
+    """
+    s = 1.0 + 6 * 0.75 ** 3 + 12 * 0.5 ** 3 + 8 * 0.25 ** 3
+    return 315.0 * s / (64.0 * math.pi * h ** 3)
+
+
+@dataclasses.dataclass
+class FluidScene:
+    name: str
+    box: tuple
+    res: tuple
+    lattice: tuple
+    n: int
+    max_n: int
+    pos: np.ndarray  # (n, 4) float32
+    vel: np.ndarray  # (n, 4) float32
+    rest_density: float
+    gravity: float = 9.8
+    iters: int = 4
+    time_step: float = 1.0 / 120.0
+
+    def params(self) -> dict:
+        """FluidKernelInputs values for this scene (dam-break parameter set)."""
+        return dict(restDensity=float(np.float32(self.rest_density)), relaxCFM=600.0,
+                    timeStep=float(np.float32(self.time_step)), dim=3,
+                    isArtPressureEnabled=1, artPressureRadius=0.2, artPressureCoeff=0.1,
+                    artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0002,
+                    xsphViscosityCoeff=0.01)
+
+
+PBF_SIZES = {
+    # name: (lattice, box, res)
+    "pbf-65k": ((64, 32, 32), (4, 2, 2), (40, 20, 20)),
+    "pbf-1m": ((128, 128, 64), (8, 8, 4), (80, 80, 40)),
+    "pbf-4m": ((256, 128, 128), (16, 8, 8), (160, 80, 80)),
+}
+
+
+def dam_break(lattice, box, res, seed: int = 1, name: str = "pbf", jitter: float = 0.05,
+              extra: int = 512) -> FluidScene:
+    """Jittered lattice in the (-x,-y,-z) corner; spacing h/2, jitter +-jitter*h."""
+    lx, ly, lz = (int(v) for v in lattice)
+    h = float(box[0]) / float(res[0])
+    s = 0.5 * h
+    n = lx * ly * lz
+    ix, iy, iz = np.meshgrid(np.arange(lx), np.arange(ly), np.arange(lz), indexing="ij")
+    base = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64)
+    corner = -0.5 * np.asarray(box, np.float64)
+    u = uniform24(seed, 0, 3 * n).reshape(n, 3).astype(np.float64)
+    xyz = corner + (base + 0.5) * s + (2.0 * u - 1.0) * (jitter * h)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = xyz.astype(np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    return FluidScene(name=name, box=tuple(float(b) for b in box), res=tuple(int(r) for r in res),
+                      lattice=(lx, ly, lz), n=n, max_n=n + extra, pos=pos, vel=vel,
+                      rest_density=lattice_rest_density(h))
+
+
+def pbf_scene(name: str, seed: int = 1) -> FluidScene:
+    lattice, box, res = PBF_SIZES[name]
+    return dam_break(lattice, box, res, seed=seed, name=name)
+
+
+@dataclasses.dataclass
+class BoidsScene:
+    name: str
+    box: tuple
+    res: tuple
+    n: int
+    max_n: int
+    pos: np.ndarray
+    vel: np.ndarray
+    velocity_scale: float = 2.0
+    alignment: float = 1.0
+    cohesion: float = 1.0
+    separation: float = 1.5
+    time_step: float = 0.01
+
+
+def boids_flock(n: int = 8192, seed: int = 7, box: float = 2.0, res: int = 20,
+                extra: int = 512) -> BoidsScene:
+    """Positions U[-box/2, box/2)^3; velocity = uniform unit direction x U[0.5, 1.0]."""
+    u = uniform24(seed, 0, 6 * n).reshape(n, 6).astype(np.float64)
+    half = 0.5 * box
+    xyz = -half + u[:, :3] * box
+    zc = 2.0 * u[:, 3] - 1.0
+    phi = 2.0 * math.pi * u[:, 4]
+    rxy = np.sqrt(np.maximum(0.0, 1.0 - zc * zc))
+    d = np.stack([rxy * np.cos(phi), rxy * np.sin(phi), zc], axis=1)
+    speed = 0.5 + 0.5 * u[:, 5]
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    pos[:, :3] = xyz.astype(np.float32)
+    vel[:, :3] = (d * speed[:, None]).astype(np.float32)
+    return BoidsScene(name=f"boids-{n}", box=(box, box, box), res=(res, res, res), n=n,
+                      max_n=n + extra, pos=pos, vel=vel)

@@ -1,0 +1,58 @@
+// Drives the C++ Physics::Model interface the way the reference app does
+// (app/ParticleSystemApp.cpp:275-315 CreateModel, :377 update) and dumps the state before and
+// after `steps` updates so a test can replay the same steps on the CPU oracle.
+//
+//   model_demo fluids|boids <out.bin> <steps>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "Model.hpp"
+
+static void put(FILE* f, const void* p, size_t n) { std::fwrite(p, 1, n, f); }
+
+int main(int argc, char** argv)
+{
+  if (argc < 4) {
+    std::fprintf(stderr, "usage: model_demo fluids|boids out.bin steps\n");
+    return 2;
+  }
+  const bool fluids = std::strcmp(argv[1], "fluids") == 0;
+  const int steps = std::atoi(argv[3]);
+  Physics::ModelParams p;
+  p.maxNbParticles = Utils::P130K;            // app: largest entry of ALL_NB_PARTICLES
+  p.boxSize = Geometry::BOX_SIZE_3D;
+  p.gridRes = Geometry::GRID_RES_3D;
+  p.dimension = Geometry::Dimension::dim3D;
+  p.pCase = fluids ? Utils::FLUIDS_DAM : Utils::BOIDS_LARGE;
+  try {
+    auto model = Physics::CreateModel(fluids ? Physics::FLUIDS : Physics::BOIDS, p);
+    const size_t n = model->nbParticles();
+    std::vector<float> pos0(4 * n), vel0(4 * n), pos1(4 * n), vel1(4 * n);
+    model->readState(pos0.data(), vel0.data(), n);
+    model->enableProfiling(true);
+    for (int s = 0; s < steps; ++s)
+      model->update();
+    model->readState(pos1.data(), vel1.data(), n);
+    size_t launches = 0;
+    for (const auto& k : model->kernelTimes())
+      launches += k.launches;
+    FILE* f = std::fopen(argv[2], "wb");
+    if (!f)
+      return 3;
+    const unsigned int hdr[3] = { (unsigned int)n, (unsigned int)steps, (unsigned int)model->maxNbParticles() };
+    put(f, hdr, sizeof(hdr));
+    put(f, pos0.data(), pos0.size() * 4);
+    put(f, vel0.data(), vel0.size() * 4);
+    put(f, pos1.data(), pos1.size() * 4);
+    put(f, vel1.data(), vel1.size() * 4);
+    std::fclose(f);
+    std::printf("model_demo %s: %zu particles, %d updates, %zu profiled launches\n", argv[1], n,
+                steps, launches);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "model_demo: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
