@@ -141,6 +141,12 @@ int rtp_fluids_stream(rtp_fluids* h, void** stream);
 int rtp_fluids_enable_profiling(rtp_fluids* h, int enable);
 int rtp_fluids_profile(rtp_fluids* h, rtp_kernel_time* out, int capacity, int* count);
 int rtp_fluids_reset_profile(rtp_fluids* h);
+/* Device-side checkpoint of the model state (p_pos, p_vel, p_predPos, p_cellID and the
+ * particle count; no reference counterpart — the reference re-creates its initial lattice in
+ * reset(), Fluids.cpp:196-216).  restore enqueues device copies on the handle's stream, so
+ * restore + update can be timed back to back without host transfers. */
+int rtp_fluids_save_checkpoint(rtp_fluids* h);
+int rtp_fluids_restore_checkpoint(rtp_fluids* h);
 /* Replay pairs of substeps from a captured hipGraph (no reference counterpart; the
  * reference enqueues every kernel each frame).  Default on. */
 int rtp_fluids_use_graph(rtp_fluids* h, int enable);

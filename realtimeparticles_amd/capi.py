@@ -101,6 +101,8 @@ _SIGS = {
     "rtp_fluids_enable_profiling": ([_vp, _int], _int),
     "rtp_fluids_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
     "rtp_fluids_reset_profile": ([_vp], _int),
+    "rtp_fluids_save_checkpoint": ([_vp], _int),
+    "rtp_fluids_restore_checkpoint": ([_vp], _int),
     "rtp_fluids_use_graph": ([_vp, _int], _int),
     "rtp_fluids_set_tuning": ([_vp, _int, _int, _int], _int),
     "rtp_boids_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], _int),
@@ -287,10 +289,22 @@ class Fluids:
         return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].totalMs))
                 for i in range(cnt.value)}
 
+    def save_checkpoint(self) -> None:
+        _check(lib().rtp_fluids_save_checkpoint(self._h), "save_checkpoint")
+
+    def restore_checkpoint(self) -> None:
+        _check(lib().rtp_fluids_restore_checkpoint(self._h), "restore_checkpoint")
+        self.n = self.read_nb_particles()
+
+    def read_nb_particles(self) -> int:
+        v = _u32(0)
+        _check(lib().rtp_fluids_get_nb_particles(self._h, ctypes.byref(v)), "get_nb_particles")
+        return int(v.value)
+
     def use_graph(self, flag: bool) -> None:
         _check(lib().rtp_fluids_use_graph(self._h, int(bool(flag))), "use_graph")
 
-    def set_tuning(self, block: int = 256, xcd_remap: int = 1, sort_bits: int = 0) -> None:
+    def set_tuning(self, block: int = 256, xcd_remap: int = 0, sort_bits: int = 0) -> None:
         _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(xcd_remap), int(sort_bits)),
                "set_tuning")
 

@@ -229,12 +229,11 @@ struct GridState {
   int alloc(uint32_t maxN_)
   {
     maxN = maxN_;
-    const size_t subtiles = (maxN + (size_t)kSortItems * 64 - 1) / ((size_t)kSortItems * 64);
     int rc = 0;
     if ((rc = dev_alloc(&key[0], maxN)) || (rc = dev_alloc(&key[1], maxN)) ||
         (rc = dev_alloc(&idx[0], maxN)) || (rc = dev_alloc(&idx[1], maxN)) ||
         (rc = dev_alloc(&startEnd, g.numCells)) ||
-        (rc = dev_alloc(&counts, (size_t)2048 * std::max<size_t>(subtiles, 1))))
+        (rc = dev_alloc(&counts, count_capacity(maxN))))
       return rc;
     return RTP_OK;
   }
@@ -356,6 +355,13 @@ struct rtp_fluids {
   int graphCur = -1, graphKeyCur = -1, graphPred = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
+  // checkpoint (rtp_fluids_save_checkpoint)
+  float4* ckPos = nullptr;
+  float4* ckVel = nullptr;
+  float4* ckPred = nullptr;
+  uint32_t* ckKey = nullptr;
+  bool ckValid = false;
+  uint32_t ckN = 0, ckDirty = 0, ckSorted = 0;
 
   void drop_graph()
   {
@@ -517,6 +523,10 @@ struct rtp_fluids {
     dev_free(density);
     dev_free(lambda);
     dev_free(metric);
+    dev_free(ckPos);
+    dev_free(ckVel);
+    dev_free(ckPred);
+    dev_free(ckKey);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -862,6 +872,59 @@ int rtp_fluids_reset_profile(rtp_fluids* h)
   return RTP_OK;
 }
 
+int rtp_fluids_save_checkpoint(rtp_fluids* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const uint32_t maxN = h->grid.maxN;
+  int rc = 0;
+  if (!h->ckPos && ((rc = dev_alloc(&h->ckPos, maxN)) || (rc = dev_alloc(&h->ckVel, maxN)) ||
+                    (rc = dev_alloc(&h->ckPred, maxN)) || (rc = dev_alloc(&h->ckKey, maxN))))
+    return rc;
+  RTP_HIP_TRY(hipMemcpyAsync(h->ckPos, h->pos[h->cur], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(h->ckVel, h->vel[h->cur], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(h->ckPred, h->pred[h->predFinal], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(h->ckKey, h->grid.key[h->grid.keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->ckN = h->n;
+  h->ckDirty = h->grid.dirtyEnd;
+  h->ckSorted = h->grid.sortedN;
+  h->ckValid = true;
+  return RTP_OK;
+}
+
+int rtp_fluids_restore_checkpoint(rtp_fluids* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  if (!h->ckValid)
+    return fail(RTP_ERR_STATE, "no checkpoint saved");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const uint32_t maxN = h->grid.maxN;
+  // Outside the sorted range both ping-pong partners of a buffer hold the same data.  When
+  // that range is unchanged since the save, the current buffers alone need the copy;
+  // otherwise both partners get it.
+  const bool same = h->grid.dirtyEnd == h->ckDirty && h->grid.sortedN == h->ckSorted && h->n == h->ckN;
+  for (int k = 0; k < 2; ++k) {
+    const bool current = k == 0;
+    if (!current && same)
+      break;
+    float4* pos = current ? h->pos[h->cur] : h->pos[1 - h->cur];
+    float4* vel = current ? h->vel[h->cur] : h->vel[1 - h->cur];
+    float4* pred = current ? h->pred[h->predFinal] : h->pred[1 - h->predFinal];
+    uint32_t* key = current ? h->grid.key[h->grid.keyCur] : h->grid.key[1 - h->grid.keyCur];
+    RTP_HIP_TRY(hipMemcpyAsync(pos, h->ckPos, sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    RTP_HIP_TRY(hipMemcpyAsync(vel, h->ckVel, sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    RTP_HIP_TRY(hipMemcpyAsync(pred, h->ckPred, sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    RTP_HIP_TRY(hipMemcpyAsync(key, h->ckKey, sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  }
+  h->n = h->ckN;
+  h->grid.dirtyEnd = h->ckDirty;
+  h->grid.sortedN = h->ckSorted;
+  if (!same)
+    h->drop_graph();
+  return RTP_OK;
+}
+
 int rtp_fluids_use_graph(rtp_fluids* h, int enable)
 {
   RTP_REQUIRE(h, "NULL handle");
@@ -1092,7 +1155,7 @@ int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxK
   if (!rc) rc = dev_alloc(&kb, n);
   if (!rc) rc = dev_alloc(&ia, n);
   if (!rc) rc = dev_alloc(&ib, n);
-  if (!rc) rc = dev_alloc(&cnt, (size_t)2048 * std::max<uint32_t>(plan.subtiles, 1));
+  if (!rc) rc = dev_alloc(&cnt, count_capacity(n));
   if (!rc && hipMemcpyAsync(ka, keysIn, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess)
     rc = fail(RTP_ERR_HIP, "hipMemcpy keys");
   if (!rc)

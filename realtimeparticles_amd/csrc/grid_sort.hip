@@ -15,6 +15,8 @@
 //               scatters; a wave-private LDS counter per digit carries the running rank
 //               from one 64-key round to the next, so no workgroup barrier sits in the loop.
 // A stable sort has exactly one result, so it equals the reference's for any digit width.
+#include <algorithm>
+
 #include "rtp_device.hpp"
 
 namespace rtp {
@@ -89,53 +91,113 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(const uint32_t* __restrict
       counts[(size_t)d * S + s] = hist[w][d];
 }
 
-// Exclusive scan of m words with one 1024-thread block; each round covers 16 words per
-// thread (4 x uint4 loads when aligned).
-__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ data, uint32_t m)
+// Exclusive scan of the digit-major count table (m words, m % 4 == 0) in three launches:
+// per-segment sums, a scan of the segment sums, and a rescan of every segment from its base.
+// A thread owns 16 consecutive words (four 16-byte loads), a block owns kScanSeg words.
+constexpr uint32_t kScanPer = 16;
+constexpr uint32_t kScanSeg = 256 * kScanPer;
+
+__device__ __forceinline__ void load16(const uint32_t* __restrict__ data, uint32_t b, uint32_t m,
+                                       uint32_t v[kScanPer])
 {
-  __shared__ uint32_t waveTot[16];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPer; k += 4) {
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    if (b + k < m)
+      q = *reinterpret_cast<const uint4*>(data + b + k);
+    v[k] = q.x;
+    v[k + 1] = q.y;
+    v[k + 2] = q.z;
+    v[k + 3] = q.w;
+  }
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns the prefix and
+// writes the block total to *total.
+__device__ __forceinline__ uint32_t block_exclusive_256(uint32_t v, uint32_t* total)
+{
+  __shared__ uint32_t waveTot[4];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if ((int)lane >= d)
+      inc += o;
+  }
+  if (lane == 63)
+    waveTot[w] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t k = 0; k < w; ++k)
+    base += waveTot[k];
+  *total = waveTot[0] + waveTot[1] + waveTot[2] + waveTot[3];
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ data, uint32_t m,
+                                                     uint32_t* __restrict__ partial)
+{
+  const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  load16(data, b, m, v);
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPer; ++k)
+    sum += v[k];
+  uint32_t total = 0;
+  (void)block_exclusive_256(sum, &total);
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of nseg segment sums in place.
+__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t nseg)
+{
   __shared__ uint32_t carryS;
-  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-  if (t == 0)
+  if (threadIdx.x == 0)
     carryS = 0;
   __syncthreads();
-  constexpr uint32_t PER = 16, ROUND = 1024 * PER;
-  for (uint32_t r0 = 0; r0 < m; r0 += ROUND) {
-    const uint32_t b = r0 + t * PER;
-    uint32_t v[PER];
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k)
-      v[k] = (b + k < m) ? data[b + k] : 0u;
-    uint32_t tot = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k)
-      tot += v[k];
-    // inclusive wave scan of tot
-    uint32_t inc = tot;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t o = __shfl_up(inc, d, 64);
-      if ((int)lane >= d)
-        inc += o;
-    }
-    if (lane == 63)
-      waveTot[w] = inc;
-    __syncthreads();
-    uint32_t wbase = 0;
-    for (uint32_t k = 0; k < w; ++k)
-      wbase += waveTot[k];
+  for (uint32_t r0 = 0; r0 < nseg; r0 += 256) {
+    const uint32_t i = r0 + threadIdx.x;
+    const uint32_t v = i < nseg ? partial[i] : 0u;
+    uint32_t total = 0;
+    const uint32_t ex = block_exclusive_256(v, &total);
     const uint32_t carry = carryS;
-    uint32_t run = carry + wbase + inc - tot;
+    if (i < nseg)
+      partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      carryS = carry + total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ data, uint32_t m,
+                                                    const uint32_t* __restrict__ partial)
+{
+  const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  load16(data, b, m, v);
+  uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-      if (b + k < m)
-        data[b + k] = run;
-      run += v[k];
-    }
-    __syncthreads();
-    if (t == 1023)
-      carryS = run;  // thread 1023 holds the round's inclusive total
-    __syncthreads();
+  for (uint32_t k = 0; k < kScanPer; ++k)
+    sum += v[k];
+  uint32_t total = 0;
+  uint32_t run = partial[blockIdx.x] + block_exclusive_256(sum, &total);
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPer; k += 4) {
+    uint4 q;
+    q.x = run;
+    run += v[k];
+    q.y = run;
+    run += v[k + 1];
+    q.z = run;
+    run += v[k + 2];
+    q.w = run;
+    run += v[k + 3];
+    if (b + k < m)
+      *reinterpret_cast<uint4*>(data + b + k) = q;
   }
 }
 
@@ -249,8 +311,12 @@ int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* 
   const uint32_t blocks = (p.subtiles + 3u) / 4u;
   hipLaunchKernelGGL((k_sort_upsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, p.n,
                      shift, p.subtiles, counts);
-  hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, counts,
-                     (uint32_t)((size_t)(1u << BITS) * p.subtiles));
+  const uint32_t m = (uint32_t)((size_t)(1u << BITS) * p.subtiles);
+  const uint32_t nseg = (m + kScanSeg - 1u) / kScanSeg;
+  uint32_t* partial = counts + (size_t)2048 * p.subtiles;  // past the largest table for n
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, counts, m, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, partial, nseg);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, counts, m, partial);
   hipLaunchKernelGGL((k_sort_downsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, iin,
                      p.n, shift, p.subtiles, counts, kout, iout);
   return RTP_OK;
@@ -260,9 +326,6 @@ int dispatch_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint3
                   uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
 {
   switch (p.bits) {
-  case 1: return run_pass<1>(p, shift, kin, iin, kout, iout, counts, s);
-  case 2: return run_pass<2>(p, shift, kin, iin, kout, iout, counts, s);
-  case 3: return run_pass<3>(p, shift, kin, iin, kout, iout, counts, s);
   case 4: return run_pass<4>(p, shift, kin, iin, kout, iout, counts, s);
   case 5: return run_pass<5>(p, shift, kin, iin, kout, iout, counts, s);
   case 6: return run_pass<6>(p, shift, kin, iin, kout, iout, counts, s);
@@ -279,6 +342,13 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
 }  // namespace
 
+size_t count_capacity(uint32_t maxN)
+{
+  const size_t subtiles = std::max<size_t>((maxN + (size_t)kSortItems * 64 - 1) / ((size_t)kSortItems * 64), 1);
+  const size_t table = (size_t)2048 * subtiles;
+  return table + table / kScanSeg + 64;
+}
+
 SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
 {
   SortPlan p;
@@ -288,9 +358,11 @@ SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
     ++keyBits;
   if (keyBits == 0)
     keyBits = 1;
-  const int maxBits = (bitsOverride >= 1 && bitsOverride <= 11) ? bitsOverride : 9;
+  const int maxBits = (bitsOverride >= 4 && bitsOverride <= 11) ? bitsOverride : 9;
   p.passes = (keyBits + maxBits - 1) / maxBits;
   p.bits = (keyBits + p.passes - 1) / p.passes;
+  if (p.bits < 4)  // the count table scan works on whole 16-byte groups
+    p.bits = 4;
   p.subtiles = (n + (uint32_t)(kSortItems * 64) - 1u) / (uint32_t)(kSortItems * 64);
   p.countWords = ((size_t)1u << p.bits) * p.subtiles;
   return p;

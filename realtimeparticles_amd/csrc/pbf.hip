@@ -27,30 +27,113 @@ namespace {
 
 constexpr float kFarD2 = 1.0e12f;
 
-// Visit the candidates of particle position p in the reference's order.  f is called for
-// every candidate that is not provably a +-0 contribution.
+// Candidates that survive the distance test are queued per lane in LDS and handed to the
+// pass body afterwards, so the expensive body runs densely over each lane's survivors
+// instead of once per candidate slot of the whole wave.  Survivors keep their order.
+constexpr uint32_t kQueue = 24;     // queued candidates per lane
+constexpr uint32_t kMaxBlock = 256; // threads per block the LDS queue is laid out for
+
+struct LaneQueue {
+  uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * kMaxBlock]
+  uint32_t count;
+};
+
+template <class F>
+__device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4* __restrict__ pred,
+                                      F& f)
+{
+  for (uint32_t k = 0; k < q.count; ++k) {
+    const uint32_t e = q.slot[k * kMaxBlock];
+    const float4 o = pred[e];
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    f(e, dx, dy, dz, d2);
+  }
+  q.count = 0;
+}
+
+// Queue candidate e unless its contribution is provably +-0 (r >= h from the squared
+// distance); very far (d2 >= kFarD2) and NaN distances are queued for the exact expressions.
+template <class F>
+__device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4 o, uint32_t e,
+                                      const float4* __restrict__ pred, const PbfConsts& c, F& f)
+{
+  const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  if (!(d2 >= c.cullSq && d2 < kFarD2)) {
+    q.slot[q.count * kMaxBlock] = e;
+    if (++q.count == kQueue)
+      drain(q, p, pred, f);
+  }
+}
+
+// Candidates lo..hi (inclusive) in index order, four loads in flight at a time.
+template <class F>
+__device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& q, const float4 p,
+                                           const float4* __restrict__ pred, const PbfConsts& c,
+                                           F& f)
+{
+  uint32_t e = lo;
+  for (; e + 3u <= hi; e += 4u) {
+    const float4 o0 = pred[e], o1 = pred[e + 1u], o2 = pred[e + 2u], o3 = pred[e + 3u];
+    offer(q, p, o0, e, pred, c, f);
+    offer(q, p, o1, e + 1u, pred, c, f);
+    offer(q, p, o2, e + 2u, pred, c, f);
+    offer(q, p, o3, e + 3u, pred, c, f);
+  }
+  for (; e <= hi; ++e)
+    offer(q, p, pred[e], e, pred, c, f);
+}
+
+// Visit the candidates of particle position p in the reference's order: 27 cells, x
+// outer and z inner with periodic wrap (fluids.cl:101-123), each cell from start to end.
+// The three cells of one z-column are scanned as a single index range whenever the cell
+// table shows their non-empty ranges are back to back (always, unless a cell was capped or
+// the column wraps) -- the same candidates in the same order with one loop instead of three.
+// The queue is drained after every x-slab of nine cells.
 template <class F>
 __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
                                        const uint2* __restrict__ se, const PbfConsts& c, F&& f)
 {
+  __shared__ uint32_t queue[kQueue * kMaxBlock];
+  LaneQueue q{ &queue[threadIdx.x], 0u };
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
+  const int32_t z0 = wrap((int32_t)cz - 1, rz), z1 = wrap((int32_t)cz, rz),
+                z2 = wrap((int32_t)cz + 1, rz);
   for (int ix = -1; ix <= 1; ++ix) {
     const int32_t nx = wrap((int32_t)cx + ix, rx);
     for (int iy = -1; iy <= 1; ++iy) {
       const int32_t row = (nx * ry + wrap((int32_t)cy + iy, ry)) * rz;
-      for (int iz = -1; iz <= 1; ++iz) {
-        const uint2 r = se[row + wrap((int32_t)cz + iz, rz)];
-        for (uint32_t e = r.x; e <= r.y; ++e) {
-          const float4 q = pred[e];
-          const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (!(d2 >= c.cullSq && d2 < kFarD2))
-            f(e, dx, dy, dz, d2);
+      const uint2 a = se[row + z0], b = se[row + z1], d = se[row + z2];
+      // Merge when every non-empty range starts right after the previous non-empty one.
+      uint32_t lo = 1u, hi = 0u;
+      bool ok = true;
+      const uint2 cells[3] = { a, b, d };
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const uint2 r = cells[k];
+        if (r.x > r.y)
+          continue;
+        if (lo > hi) {
+          lo = r.x;
+          hi = r.y;
+        } else if (r.x == hi + 1u) {
+          hi = r.y;
+        } else {
+          ok = false;
         }
       }
+      if (ok) {
+        scan_range(lo, hi, q, p, pred, c, f);
+      } else {
+        scan_range(a.x, a.y, q, p, pred, c, f);
+        scan_range(b.x, b.y, q, p, pred, c, f);
+        scan_range(d.x, d.y, q, p, pred, c, f);
+      }
     }
+    drain(q, p, pred, f);
   }
 }
 

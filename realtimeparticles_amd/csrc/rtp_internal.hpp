@@ -61,7 +61,7 @@ struct BoidsConsts {
 // Launch shape shared by the per-particle kernels.
 struct Launch {
   int block = 256;     // threads per block
-  int xcdRemap = 1;    // consecutive particle blocks on one XCD
+  int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD (slower on the dam break)
 };
 
 // ---------------------------------------------------------------------------- grid / sort
@@ -74,6 +74,9 @@ struct SortPlan {
   size_t countWords = 0;   // radix * subtiles
 };
 constexpr int kSortItems = 32;  // keys per lane in one subtile
+// Words of the counts buffer a sort of up to maxN keys needs (any digit width): the
+// digit-major count table (at most 2048 digits x subtiles) followed by its scan's segment sums.
+size_t count_capacity(uint32_t maxN);
 SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride);
 // keys/idx ping-pong through (keyA, idxA) <-> (keyB, idxB); returns whether the sorted
 // result ended in the B buffers.  idxIn == nullptr means the identity permutation.

@@ -69,13 +69,29 @@ class FluidScene:
     iters: int = 4
     time_step: float = 1.0 / 120.0
 
-    def params(self) -> dict:
-        """FluidKernelInputs values for this scene (dam-break parameter set)."""
-        return dict(restDensity=float(np.float32(self.rest_density)), relaxCFM=600.0,
-                    timeStep=float(np.float32(self.time_step)), dim=3,
-                    isArtPressureEnabled=1, artPressureRadius=0.2, artPressureCoeff=0.1,
-                    artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0002,
-                    xsphViscosityCoeff=0.01)
+    def params(self, mapping: str = "literal") -> dict:
+        """FluidKernelInputs values for this scene.
+
+        "literal": the dam-break parameter set mapped one to one (rest density = lattice
+        density, relaxation 600, s_corr k=0.1 n=4 dq=0.2h, XSPH c=0.01, vorticity 2e-4).
+        With the reference's kernels (unit mass, XSPH summing (v_j - v_i) W without a
+        1/rho_j factor) these values blow the lattice apart within the first substep and
+        collapse the fluid into point clumps after a few more.
+        "stable": a variant for diagnostics only -- XSPH and vorticity coefficients divided
+        by rho0 (the missing 1/rho_j) and the reference's own s_corr coefficient 0.001.
+        """
+        rho0 = float(np.float32(self.rest_density))
+        p = dict(restDensity=rho0, relaxCFM=600.0,
+                 timeStep=float(np.float32(self.time_step)), dim=3,
+                 isArtPressureEnabled=1, artPressureRadius=0.2, artPressureCoeff=0.1,
+                 artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0002,
+                 xsphViscosityCoeff=0.01)
+        if mapping == "stable":
+            p.update(artPressureCoeff=0.001, vorticityConfCoeff=float(np.float32(0.0002 / rho0)),
+                     xsphViscosityCoeff=float(np.float32(0.01 / rho0)))
+        elif mapping != "literal":
+            raise ValueError(f"unknown parameter mapping {mapping!r}")
+        return p
 
 
 PBF_SIZES = {

@@ -147,3 +147,31 @@ def test_errors_are_reported():
     with pytest.raises(capi.RtpError):
         gpu.load_state(np.zeros((sc.max_n + 1, 4), np.float32), np.zeros((sc.max_n + 1, 4), np.float32))
     gpu.close()
+
+
+def test_checkpoint_restore_replays_identically():
+    sc = small_scene(seed=9)
+    gpu, _ = make_pbf_pair(sc, iters=3)
+    gpu.update(3)
+    gpu.save_checkpoint()
+    gpu.update(1)
+    gpu.update(2)
+    pa, va = gpu.read_state()
+    ka, ia, sa = gpu.read_grid()
+    for rep in range(2):
+        gpu.restore_checkpoint()
+        gpu.update(3)
+        pb, vb = gpu.read_state()
+        _eq(pb, pa, f"pos after restore {rep}")
+        _eq(vb, va, f"vel after restore {rep}")
+        kb, ib, sb = gpu.read_grid()
+        _eq(kb, ka, f"keys after restore {rep}")
+        _eq(sb, sa, f"startEnd after restore {rep}")
+    # restoring across a particle-count change puts the count back as well
+    gpu.restore_checkpoint()
+    gpu.set_nb_particles(sc.n - 100)
+    gpu.restore_checkpoint()
+    assert gpu.read_nb_particles() == sc.n
+    gpu.update(3)
+    _eq(gpu.read_state()[0], pa, "pos after restore over a count change")
+    gpu.close()
