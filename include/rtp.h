@@ -154,6 +154,49 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable);
  * multiple of 64), XCD-contiguous block order (0/1), sort digit width (0 = auto, 4..11). */
 int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBits);
 
+/* ========================= Learned warm-start ========================= */
+/* No reference counterpart: an 8 -> 64 -> 64 -> 1 MLP (GELU, BF16 weights, FP32
+ * accumulation on the matrix cores) that predicts a particle's initial lambda.  Inputs per
+ * particle, computed in the first density pass of a substep: C = rho/rho0 - 1, that
+ * pass's lambda, neighbour count (r < h, self included), sum |grad C|^2, the particle's own
+ * |grad_i C|^2, speed, and the distances to the two nearest walls of the boundary clamp
+ * box.  They depend on the substep's input state only. */
+typedef struct rtp_mlp rtp_mlp;
+
+/* Network with deterministic N(0, 1/fan_in) weights from seed, zero biases. */
+int rtp_mlp_create(int device, uint32_t seed, rtp_mlp** out);
+int rtp_mlp_destroy(rtp_mlp* h);
+/* Length of the fp32 parameter vector: W1[8][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3. */
+int rtp_mlp_param_count(void);
+int rtp_mlp_set_params(rtp_mlp* h, const float* params, uint32_t count);
+int rtp_mlp_get_params(rtp_mlp* h, float* params, uint32_t count);
+/* Inputs are standardised as (x - mu) / sigma; the output is multiplied by outScale. */
+int rtp_mlp_set_normalization(rtp_mlp* h, const float* mu, const float* sigma, float outScale);
+/* out[i] = network(feat[8 i .. 8 i + 7]) for n rows (host arrays). */
+int rtp_mlp_forward(rtp_mlp* h, const float* feat, uint32_t n, float* out);
+/* Gradient of the mean squared error on target / outScale over all n rows (parameter
+ * layout above) and that loss; the parameters are not changed. */
+int rtp_mlp_gradients(rtp_mlp* h, const float* feat, const float* target, uint32_t n, float* grad,
+                      float* loss);
+/* Adam (lr, betas 0.9/0.999) over shuffled mini-batches on the GPU; epochLoss[e] (optional)
+ * receives each epoch's mean squared error. */
+int rtp_mlp_train(rtp_mlp* h, const float* feat, const float* target, uint32_t n, uint32_t epochs,
+                  uint32_t batch, float lr, uint32_t seed, float* epochLoss);
+/* Use the network for the first of learnedIters Jacobi iterations of every substep (NULL
+ * switches back to the plain solver).  After those iterations a density pass measures
+ * the mean |rho/rho0 - 1|; when it exceeds tau, fallbackIters plain iterations follow. */
+int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters,
+                              uint32_t fallbackIters, float tau);
+/* Run that post-correction density pass after plain substeps too (diagnostics). */
+int rtp_fluids_set_residual_pass(rtp_fluids* h, int enable);
+/* max, mean and max-compression |rho/rho0 - 1| of the last post-correction density pass
+ * (the same bits for the same state on every run). */
+int rtp_fluids_residual_error(rtp_fluids* h, float out[3]);
+/* One substep with teacherIters plain iterations that records, for every active particle
+ * in the substep's sorted order, the network inputs (n x 8) and the sum of the lambdas of
+ * all iterations (the training target).  The model state advances by that substep. */
+int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat, float* target);
+
 /* ============================== Boids ================================ */
 typedef struct rtp_boids rtp_boids;
 

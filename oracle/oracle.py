@@ -145,6 +145,7 @@ def lib():
         L.orc_fld_confinement.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
         L.orc_fld_xsph.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
         L.orc_fld_update_pos.argtypes = [_vp, _vp, _u32]
+        L.orc_fld_features.argtypes = [_vp, _vp, _vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
         L.orc_bd_rules3d.argtypes = [_vp, _vp, _vp, P(BoidsParams), _vp, _u32, P(Consts)]
         L.orc_bd_update_vel.argtypes = [_vp, _f32, _f32, _vp, _u32]
         L.orc_bd_update_pos_wall.argtypes = [_vp, _f32, _vp, _u32, P(Consts)]

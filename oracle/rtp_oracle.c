@@ -554,6 +554,57 @@ void orc_fld_update_pos(const orc_f4* pred, orc_f4* pos, uint32_t n)
   memcpy(pos, pred, sizeof(orc_f4) * n);
 }
 
+/* Inputs of the learned warm-start (defined by this project, no reference counterpart).
+ * The sums follow fld_computeDensity / fld_computeConstraintFactor (fluids.cl:80-193). */
+void orc_fld_features(const orc_f4* pred, const float* density, const orc_u2* se,
+                      const float* lambda, const orc_f4* vel, const orc_fluid_params* p,
+                      float* feat, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+  const float lo[3] = { -c->absWall[0] + 0.01f, -c->absWall[1] + 0.01f, -c->absWall[2] + 0.01f };
+  const float hi[3] = { c->absWall[0] - 0.1f, c->absWall[1] - 0.1f, c->absWall[2] - 0.1f };
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    orc_f4 sumGradCi = { 0.0f, 0.0f, 0.0f, 0.0f };
+    float sumSqGradC = 0.0f, count = 0.0f;
+    ORC_FLD_FOR_NEIGHBOURS(c, pos, e)
+    {
+      const orc_f4 vec = sub4(pos, pred[e]);
+      if (len4(vec) < h)
+        count += 1.0f;
+      const orc_f4 grad = grad_spiky(vec, h, c);
+      sumGradCi = add4(sumGradCi, grad);
+      sumSqGradC += dot4(grad, grad);
+    }
+    const float selfSq = dot4(sumGradCi, sumGradCi);
+    sumSqGradC += selfSq;
+    sumSqGradC /= p->restDensity * p->restDensity;
+    const float w[6] = { pos.x - lo[0], hi[0] - pos.x, pos.y - lo[1], hi[1] - pos.y,
+                         pos.z - lo[2], hi[2] - pos.z };
+    float m0 = 3.0e38f, m1 = 3.0e38f;
+    for (int k = 0; k < 6; ++k) {
+      const float d = fmaxf(w[k], 0.0f);
+      if (d < m0) {
+        m1 = m0;
+        m0 = d;
+      } else if (d < m1) {
+        m1 = d;
+      }
+    }
+    const orc_f4 v = vel[i];
+    float* f = feat + (size_t)i * 8u;
+    f[0] = density[i] / p->restDensity - 1.0f;
+    f[1] = lambda[i];
+    f[2] = count;
+    f[3] = sumSqGradC;
+    f[4] = selfSq / (p->restDensity * p->restDensity);
+    f[5] = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    f[6] = m0;
+    f[7] = m1;
+  }
+}
+
 /* ------------------------------------------------------------------------------------
  * boids.cl
  * ---------------------------------------------------------------------------------- */

@@ -117,6 +117,16 @@ void orc_fld_xsph(const orc_f4* pred, const orc_u2* se, const orc_f4* velIn,
                   const orc_fluid_params* p, orc_f4* velOut, uint32_t n, const orc_consts* c);
 void orc_fld_update_pos(const orc_f4* pred, orc_f4* pos, uint32_t n);
 
+/* ---- learned warm-start inputs (no reference counterpart) ----
+ * For each particle, from the density pass of the first Jacobi iteration on pred:
+ * C, lambda[i] (that pass's own), neighbour count (r < h incl. self), sum |grad C|^2
+ * (lambda's denominator without relaxCFM), the particle's own |grad_i C|^2, |vel[i]|, and
+ * the two smallest distances to the walls of the boundary clamp box (negative distances
+ * count as 0).  feat is n x 8. */
+void orc_fld_features(const orc_f4* pred, const float* density, const orc_u2* se,
+                      const float* lambda, const orc_f4* vel, const orc_fluid_params* p,
+                      float* feat, uint32_t n, const orc_consts* c);
+
 /* ---- boids.cl ---- */
 void orc_bd_rules3d(const orc_f4* pos, const orc_f4* vel, const orc_u2* se,
                     const orc_boids_params* p, orc_f4* acc, uint32_t n, const orc_consts* c);

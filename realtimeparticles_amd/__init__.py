@@ -15,6 +15,7 @@ from .capi import (  # noqa: F401
     BoidsParams,
     FluidParams,
     Fluids,
+    Mlp,
     RtpError,
     device_count,
     grid_build,
@@ -23,6 +24,6 @@ from .capi import (  # noqa: F401
 )
 
 __all__ = [
-    "Boids", "BoidsParams", "FluidParams", "Fluids", "RtpError", "device_count", "grid_build",
+    "Boids", "BoidsParams", "FluidParams", "Fluids", "Mlp", "RtpError", "device_count", "grid_build",
     "lib", "sort_pairs", "scenes",
 ]

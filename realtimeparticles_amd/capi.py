@@ -105,6 +105,19 @@ _SIGS = {
     "rtp_fluids_restore_checkpoint": ([_vp], _int),
     "rtp_fluids_use_graph": ([_vp, _int], _int),
     "rtp_fluids_set_tuning": ([_vp, _int, _int, _int], _int),
+    "rtp_mlp_create": ([_int, _u32, ctypes.POINTER(_vp)], _int),
+    "rtp_mlp_destroy": ([_vp], _int),
+    "rtp_mlp_param_count": ([], _int),
+    "rtp_mlp_set_params": ([_vp, _vp, _u32], _int),
+    "rtp_mlp_get_params": ([_vp, _vp, _u32], _int),
+    "rtp_mlp_set_normalization": ([_vp, _vp, _vp, _f32], _int),
+    "rtp_mlp_forward": ([_vp, _vp, _u32, _vp], _int),
+    "rtp_mlp_gradients": ([_vp, _vp, _vp, _u32, _vp, _vp], _int),
+    "rtp_mlp_train": ([_vp, _vp, _vp, _u32, _u32, _u32, _f32, _u32, _vp], _int),
+    "rtp_fluids_set_warm_start": ([_vp, _vp, _u32, _u32, _f32], _int),
+    "rtp_fluids_set_residual_pass": ([_vp, _int], _int),
+    "rtp_fluids_residual_error": ([_vp, _vp], _int),
+    "rtp_fluids_record_teacher": ([_vp, _u32, _vp, _vp], _int),
     "rtp_boids_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], _int),
     "rtp_boids_destroy": ([_vp], _int),
     "rtp_boids_set_params": ([_vp, ctypes.POINTER(BoidsParams)], _int),
@@ -304,9 +317,95 @@ class Fluids:
     def use_graph(self, flag: bool) -> None:
         _check(lib().rtp_fluids_use_graph(self._h, int(bool(flag))), "use_graph")
 
+    def set_warm_start(self, mlp: "Mlp | None", learned_iters: int = 1, fallback_iters: int = 0,
+                       tau: float = 0.0) -> None:
+        _check(lib().rtp_fluids_set_warm_start(self._h, mlp.handle if mlp else None,
+                                               int(learned_iters) if mlp else 0,
+                                               int(fallback_iters), float(tau)), "set_warm_start")
+
+    def set_residual_pass(self, flag: bool) -> None:
+        _check(lib().rtp_fluids_set_residual_pass(self._h, int(bool(flag))), "set_residual_pass")
+
+    def residual_error(self) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        _check(lib().rtp_fluids_residual_error(self._h, out.ctypes.data), "residual_error")
+        return out
+
+    def record_teacher(self, teacher_iters: int = 20):
+        feat = np.zeros((self.n, 8), np.float32)
+        target = np.zeros(self.n, np.float32)
+        _check(lib().rtp_fluids_record_teacher(self._h, int(teacher_iters), feat.ctypes.data,
+                                               target.ctypes.data), "record_teacher")
+        return feat, target
+
     def set_tuning(self, block: int = 256, xcd_remap: int = 0, sort_bits: int = 0) -> None:
         _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(xcd_remap), int(sort_bits)),
                "set_tuning")
+
+
+class Mlp:
+    """The learned warm-start network (8 -> 64 -> 64 -> 1, BF16 matrix-core kernels)."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self._h = _vp()
+        _check(lib().rtp_mlp_create(int(device), int(seed), ctypes.byref(self._h)), "rtp_mlp_create")
+        self.n_params = int(lib().rtp_mlp_param_count())
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtp_mlp_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().rtp_mlp_get_params(self._h, p.ctypes.data, self.n_params), "get_params")
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().rtp_mlp_set_params(self._h, p.ctypes.data, p.size), "set_params")
+
+    def set_normalization(self, mu, sigma, out_scale: float) -> None:
+        mu = np.ascontiguousarray(mu, np.float32)
+        sigma = np.ascontiguousarray(sigma, np.float32)
+        _check(lib().rtp_mlp_set_normalization(self._h, mu.ctypes.data, sigma.ctypes.data,
+                                               float(out_scale)), "set_normalization")
+
+    def forward(self, feat) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, np.float32)
+        out = np.zeros(feat.shape[0], np.float32)
+        _check(lib().rtp_mlp_forward(self._h, feat.ctypes.data, feat.shape[0], out.ctypes.data),
+               "rtp_mlp_forward")
+        return out
+
+    def gradients(self, feat, target):
+        feat = np.ascontiguousarray(feat, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().rtp_mlp_gradients(self._h, feat.ctypes.data, target.ctypes.data,
+                                       feat.shape[0], g.ctypes.data, loss.ctypes.data),
+               "rtp_mlp_gradients")
+        return g, float(loss[0])
+
+    def train(self, feat, target, epochs: int, batch: int, lr: float, seed: int = 0) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        hist = np.zeros(epochs, np.float32)
+        _check(lib().rtp_mlp_train(self._h, feat.ctypes.data, target.ctypes.data, feat.shape[0],
+                                   int(epochs), int(batch), float(lr), int(seed),
+                                   hist.ctypes.data), "rtp_mlp_train")
+        return hist
 
 
 class Boids:

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rtp_internal.hpp"
+#include "rtp_mlp.hpp"
 
 namespace rtp {
 
@@ -328,6 +329,58 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
 
 using namespace rtp;
 
+// =================================================================================== MLP
+
+struct rtp_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float* p = nullptr;     // fp32 master parameters
+  float* grad = nullptr;
+  float* m1 = nullptr;    // Adam moments
+  float* m2 = nullptr;
+  float* loss = nullptr;
+  short* w1t = nullptr;
+  short* w2t = nullptr;
+  short* w2 = nullptr;
+  MlpDev dev{};
+  int step = 0;
+  float* dFeat = nullptr;  // training / inference staging
+  float* dTarget = nullptr;
+  float* dOut = nullptr;
+  uint32_t* dPerm = nullptr;
+  uint32_t cap = 0;
+
+  int reserve(uint32_t n)
+  {
+    if (n <= cap)
+      return RTP_OK;
+    dev_free(dFeat);
+    dev_free(dTarget);
+    dev_free(dOut);
+    dev_free(dPerm);
+    int rc = 0;
+    if ((rc = dev_alloc(&dFeat, (size_t)n * 8)) || (rc = dev_alloc(&dTarget, n)) ||
+        (rc = dev_alloc(&dOut, n)) || (rc = dev_alloc(&dPerm, n)))
+      return rc;
+    cap = n;
+    return RTP_OK;
+  }
+  void release()
+  {
+    if (stream)
+      (void)hipStreamSynchronize(stream);
+    for (float** b : { &p, &grad, &m1, &m2, &loss, &dFeat, &dTarget, &dOut })
+      dev_free(*b);
+    dev_free(w1t);
+    dev_free(w2t);
+    dev_free(w2);
+    dev_free(dPerm);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
 // ================================================================================ fluids
 
 struct rtp_fluids {
@@ -355,6 +408,16 @@ struct rtp_fluids {
   int graphCur = -1, graphKeyCur = -1, graphPred = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
+  // learned warm-start, teacher recording and the post-correction residual pass
+  float* feat = nullptr;      // [maxN][8]
+  float* lamAcc = nullptr;
+  float* resDensity = nullptr;
+  float* resMetric = nullptr; // max, sum, max compression of the residual pass + partials
+  rtp_mlp* mlp = nullptr;
+  uint32_t learnedIters = 0, fallbackIters = 0;
+  float tau = 0.0f;
+  bool residualPass = false;
+  uint32_t teacherIters = 0;  // > 0 while rtp_fluids_record_teacher runs its substep
   // checkpoint (rtp_fluids_save_checkpoint)
   float4* ckPos = nullptr;
   float4* ckVel = nullptr;
@@ -403,11 +466,15 @@ struct rtp_fluids {
     // p_predPos is pred[predFinal]; predict overwrites its active range only, so entries
     // past n keep the previous step's values, as in the reference's single buffer.
     const int P = predFinal;
+    const bool learned = mlp && learnedIters > 0 && teacherIters == 0;
+    const bool teacher = teacherIters > 0;
+    const bool wantFeatures = learned || teacher;
     RTP_PROF(prof, "pbf_predict_hash", s,
              launch_pbf_predict(pos[cur], vel[cur], pred[P], G.key[G.keyCur], G.startEnd, n, c, s));
     int rc = G.sort(n, s, prof);
     if (rc)
       return rc;
+    const uint32_t iters = teacher ? teacherIters : (learned ? learnedIters : this->iters);
     ReorderArgs a{};
     a.key = G.key[G.keyCur];
     a.idx = G.idx[G.idxCur];
@@ -430,17 +497,51 @@ struct rtp_fluids {
     RTP_PROF(prof, "reorder_cells", s, launch_reorder(a, s));
     cur = 1 - cur;
     const bool vortOn = c.p.isVorticityConfEnabled != 0;
+    LearnArgs la;
+    la.feat = feat;
+    la.vel = vel[cur];
+    la.lamAcc = lamAcc;
+    la.gate = resMetric + 1;
+    la.tau = tau * (float)n;
     int in = 1 - P;
     for (uint32_t k = 0; k < iters; ++k) {
-      const int last = (k + 1 == iters);
+      const bool lastIt = (k + 1 == iters);
+      int mode = 0;
+      if (k == 0 && wantFeatures)
+        mode |= kFeatures;
+      if (teacher) {
+        mode |= kAccum;
+        la.accumInit = k == 0;
+      }
       RTP_PROF(prof, "pbf_density_lambda", s,
-               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s));
+               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la));
+      if (learned && k == 0)
+        RTP_PROF(prof, "mlp_warm_start", s, launch_mlp_forward(feat, n, mlp->dev, lambda, s));
+      const int last = !lastIt ? kNextIter : (learned ? kLastOnly : kLastVel);
       RTP_PROF(prof, "pbf_correct", s,
                launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
-                                  (last && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s));
+                                  (last == kLastVel && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s));
       in = 1 - in;
     }
-    if (iters == 0)
+    if (learned || residualPass) {
+      // post-correction density: the residual the learned step is judged by
+      RTP_PROF(prof, "pbf_residual", s,
+               launch_pbf_density_lambda(pred[in], G.startEnd, resDensity, nullptr, n, c, L, s, kDensOnly, la));
+      launch_density_error(resDensity, n, c.p.restDensity, resMetric, s);
+    }
+    if (learned) {
+      for (uint32_t f = 0; f < fallbackIters; ++f) {
+        const int mode = kGated | (f == 0 ? kClampIn : 0);
+        const int last = (f + 1 == fallbackIters) ? kLastOnly : kNextIter;
+        RTP_PROF(prof, "pbf_fallback", s,
+                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la));
+        RTP_PROF(prof, "pbf_fallback", s,
+                 launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
+                                    nullptr, n, last, c, L, s, mode, la));
+        in = 1 - in;
+      }
+    }
+    if (iters == 0 || learned)
       RTP_PROF(prof, "pbf_update_vel", s,
                launch_pbf_update_vel(pred[in], pos[cur], vel[cur], vortOn ? nullptr : pos[cur], n,
                                      c, s));
@@ -527,6 +628,10 @@ struct rtp_fluids {
     dev_free(ckVel);
     dev_free(ckPred);
     dev_free(ckKey);
+    dev_free(feat);
+    dev_free(lamAcc);
+    dev_free(resDensity);
+    dev_free(resMetric);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -663,7 +768,10 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->vel[1], maxN)) || (rc = dev_alloc(&h->pred[0], maxN)) ||
       (rc = dev_alloc(&h->pred[1], maxN)) || (rc = dev_alloc(&h->vort, maxN)) ||
       (rc = dev_alloc(&h->velTmp, maxN)) || (rc = dev_alloc(&h->density, maxN)) ||
-      (rc = dev_alloc(&h->lambda, maxN)) || (rc = dev_alloc(&h->metric, 4))) {
+      (rc = dev_alloc(&h->lambda, maxN)) || (rc = dev_alloc(&h->metric, kMetricFloats)) ||
+      (rc = dev_alloc(&h->feat, (size_t)maxN * 8)) ||
+      (rc = dev_alloc(&h->lamAcc, maxN)) || (rc = dev_alloc(&h->resDensity, maxN)) ||
+      (rc = dev_alloc(&h->resMetric, kMetricFloats))) {
     h->release();
     delete h;
     return rc;
@@ -754,6 +862,7 @@ int rtp_fluids_load_state(rtp_fluids* h, const float* pos4, const float* vel4, u
   }
   RTP_HIP_TRY(hipMemsetAsync(h->density, 0, sizeof(float) * maxN, h->stream));
   RTP_HIP_TRY(hipMemsetAsync(h->lambda, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->resMetric, 0, sizeof(float) * 4, h->stream));
   h->cur = 0;
   h->predFinal = 1;
   h->grid.reset_keys(h->stream);
@@ -1133,6 +1242,268 @@ int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* cou
 {
   RTP_REQUIRE(h, "NULL handle");
   return h->prof.report(out, capacity, count);
+}
+
+// ----------------------------------------------------------------- learned warm-start
+
+static void mlp_refresh_dev(rtp_mlp* h)
+{
+  h->dev.p = h->p;
+  h->dev.w1t = h->w1t;
+  h->dev.w2t = h->w2t;
+  h->dev.w2 = h->w2;
+  h->dev.w1tMut = h->w1t;
+  h->dev.w2tMut = h->w2t;
+  h->dev.w2Mut = h->w2;
+}
+
+int rtp_mlp_create(int device, uint32_t seed, rtp_mlp** out)
+{
+  RTP_REQUIRE(out, "NULL argument");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc)
+    return rc;
+  auto* h = new rtp_mlp();
+  h->device = device;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(RTP_ERR_HIP, "hipStreamCreate failed");
+  }
+  if ((rc = dev_alloc(&h->p, kMlpParams)) || (rc = dev_alloc(&h->grad, kMlpParams)) ||
+      (rc = dev_alloc(&h->m1, kMlpParams)) || (rc = dev_alloc(&h->m2, kMlpParams)) ||
+      (rc = dev_alloc(&h->loss, 4)) || (rc = dev_alloc(&h->w1t, 64 * 16)) ||
+      (rc = dev_alloc(&h->w2t, 64 * 64)) || (rc = dev_alloc(&h->w2, 64 * 64))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  mlp_refresh_dev(h);
+  for (int k = 0; k < 8; ++k) {
+    h->dev.mu[k] = 0.0f;
+    h->dev.invSigma[k] = 1.0f;
+  }
+  h->dev.outScale = 1.0f;
+  // Deterministic initial weights: N(0, 1/fan_in) from SplitMix64 + Box-Muller, zero biases.
+  std::vector<float> p(kMlpParams, 0.0f);
+  uint64_t st = 0x9E3779B97F4A7C15ull * (seed + 1ull);
+  auto next = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto normal = [&]() { return std::sqrt(-2.0 * std::log(next())) * std::cos(6.283185307179586 * next()); };
+  for (int i = 0; i < 8 * 64; ++i)
+    p[kMlpW1 + i] = (float)(normal() / std::sqrt(8.0));
+  for (int i = 0; i < 64 * 64; ++i)
+    p[kMlpW2 + i] = (float)(normal() / std::sqrt(64.0));
+  for (int i = 0; i < 64; ++i)
+    p[kMlpW3 + i] = (float)(normal() / std::sqrt(64.0));
+  if ((rc = rtp_mlp_set_params(h, p.data(), kMlpParams))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return RTP_OK;
+}
+
+int rtp_mlp_destroy(rtp_mlp* h)
+{
+  if (!h)
+    return RTP_OK;
+  (void)hipSetDevice(h->device);
+  h->release();
+  delete h;
+  return RTP_OK;
+}
+
+int rtp_mlp_param_count(void) { return kMlpParams; }
+
+int rtp_mlp_set_params(rtp_mlp* h, const float* params, uint32_t count)
+{
+  RTP_REQUIRE(h && params, "NULL argument");
+  RTP_REQUIRE(count == (uint32_t)kMlpParams, "parameter count must be rtp_mlp_param_count()");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipMemcpyAsync(h->p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, h->stream));
+  launch_zero(h->grad, kMlpParams, h->stream);
+  launch_zero(h->m1, kMlpParams, h->stream);
+  launch_zero(h->m2, kMlpParams, h->stream);
+  launch_mlp_pack(h->p, h->dev, h->stream);
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->step = 0;
+  return RTP_OK;
+}
+
+int rtp_mlp_get_params(rtp_mlp* h, float* params, uint32_t count)
+{
+  RTP_REQUIRE(h && params, "NULL argument");
+  RTP_REQUIRE(count == (uint32_t)kMlpParams, "parameter count must be rtp_mlp_param_count()");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipMemcpyAsync(params, h->p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_mlp_set_normalization(rtp_mlp* h, const float* mu, const float* sigma, float outScale)
+{
+  RTP_REQUIRE(h && mu && sigma, "NULL argument");
+  RTP_REQUIRE(outScale != 0.0f, "outScale must be non-zero");
+  for (int k = 0; k < 8; ++k) {
+    RTP_REQUIRE(sigma[k] > 0.0f, "sigma must be positive");
+    h->dev.mu[k] = mu[k];
+    h->dev.invSigma[k] = 1.0f / sigma[k];
+  }
+  h->dev.outScale = outScale;
+  return RTP_OK;
+}
+
+int rtp_mlp_forward(rtp_mlp* h, const float* feat, uint32_t n, float* out)
+{
+  RTP_REQUIRE(h && (n == 0 || (feat && out)), "NULL argument");
+  if (n == 0)
+    return RTP_OK;
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  int rc = h->reserve(n);
+  if (rc)
+    return rc;
+  RTP_HIP_TRY(hipMemcpyAsync(h->dFeat, feat, sizeof(float) * 8 * n, hipMemcpyHostToDevice, h->stream));
+  launch_mlp_forward(h->dFeat, n, h->dev, h->dOut, h->stream);
+  RTP_HIP_TRY(hipGetLastError());
+  RTP_HIP_TRY(hipMemcpyAsync(out, h->dOut, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_mlp_gradients(rtp_mlp* h, const float* feat, const float* target, uint32_t n, float* grad,
+                      float* loss)
+{
+  RTP_REQUIRE(h && feat && target && grad && loss && n > 0, "NULL argument or empty batch");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  int rc = h->reserve(n);
+  if (rc)
+    return rc;
+  RTP_HIP_TRY(hipMemcpyAsync(h->dFeat, feat, sizeof(float) * 8 * n, hipMemcpyHostToDevice, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(h->dTarget, target, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  launch_zero(h->grad, kMlpParams, h->stream);
+  launch_zero(h->loss, 1, h->stream);
+  launch_mlp_train(h->dFeat, h->dTarget, nullptr, n, h->dev, h->grad, h->loss, h->stream);
+  RTP_HIP_TRY(hipGetLastError());
+  RTP_HIP_TRY(hipMemcpyAsync(grad, h->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(loss, h->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  launch_zero(h->grad, kMlpParams, h->stream);
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  *loss /= (float)n;
+  return RTP_OK;
+}
+
+static uint64_t gcd64(uint64_t a, uint64_t b)
+{
+  while (b) {
+    const uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int rtp_mlp_train(rtp_mlp* h, const float* feat, const float* target, uint32_t n, uint32_t epochs,
+                  uint32_t batch, float lr, uint32_t seed, float* epochLoss)
+{
+  RTP_REQUIRE(h && feat && target && n > 0 && batch > 0, "NULL argument or empty data");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  int rc = h->reserve(n);
+  if (rc)
+    return rc;
+  RTP_HIP_TRY(hipMemcpyAsync(h->dFeat, feat, sizeof(float) * 8 * n, hipMemcpyHostToDevice, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(h->dTarget, target, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  uint64_t st = 0xD1B54A32D192ED03ull * (seed + 1ull);
+  for (uint32_t ep = 0; ep < epochs; ++ep) {
+    // a fresh affine permutation i -> (a i + c) mod n per epoch (a coprime with n)
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    uint64_t a = (st >> 33) % n;
+    if (a == 0)
+      a = 1;
+    while (gcd64(a, n) != 1)
+      a = (a + 1) % n == 0 ? 1 : (a + 1);
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    const uint32_t cc = (uint32_t)((st >> 33) % n);
+    launch_affine_perm(h->dPerm, n, (uint32_t)a, cc, h->stream);
+    launch_zero(h->loss, 1, h->stream);
+    for (uint32_t b0 = 0; b0 < n; b0 += batch) {
+      const uint32_t rows = std::min(batch, n - b0);
+      launch_mlp_train(h->dFeat, h->dTarget, h->dPerm + b0, rows, h->dev, h->grad, h->loss, h->stream);
+      launch_mlp_adam(h->p, h->grad, h->m1, h->m2, lr, ++h->step, h->dev, h->stream);
+      launch_mlp_pack(h->p, h->dev, h->stream);
+    }
+    RTP_HIP_TRY(hipGetLastError());
+    if (epochLoss) {
+      float l = 0.0f;
+      RTP_HIP_TRY(hipMemcpyAsync(&l, h->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+      RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+      epochLoss[ep] = l / (float)n;
+    }
+  }
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters,
+                              uint32_t fallbackIters, float tau)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(!mlp || learnedIters > 0, "learnedIters must be positive with a network");
+  RTP_REQUIRE(learnedIters <= 1000 && fallbackIters <= 1000, "iteration count out of range");
+  if (mlp && mlp->device != h->device)
+    return fail(RTP_ERR_INVALID, "network and model live on different devices");
+  h->mlp = mlp;
+  h->learnedIters = mlp ? learnedIters : 0;
+  h->fallbackIters = mlp ? fallbackIters : 0;
+  h->tau = tau;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_set_residual_pass(rtp_fluids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->residualPass = enable != 0;
+  h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_residual_error(rtp_fluids* h, float out[3])
+{
+  RTP_REQUIRE(h && out, "NULL argument");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  float m[3] = { 0.0f, 0.0f, 0.0f };
+  RTP_HIP_TRY(hipMemcpyAsync(m, h->resMetric, sizeof(m), hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  out[0] = m[0];
+  out[1] = h->n ? m[1] / (float)h->n : 0.0f;
+  out[2] = m[2];
+  return RTP_OK;
+}
+
+int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat, float* target)
+{
+  RTP_REQUIRE(h && feat && target, "NULL argument");
+  RTP_REQUIRE(teacherIters > 0 && teacherIters <= 1000, "teacherIters out of range");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (h->paused)
+    return fail(RTP_ERR_STATE, "model is paused");
+  h->teacherIters = teacherIters;
+  const int rc = h->substep();
+  h->teacherIters = 0;
+  if (rc)
+    return rc;
+  RTP_HIP_TRY(hipGetLastError());
+  RTP_HIP_TRY(hipMemcpyAsync(feat, h->feat, sizeof(float) * 8 * h->n, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipMemcpyAsync(target, h->lamAcc, sizeof(float) * h->n, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
 }
 
 // ------------------------------------------------------------------------ building blocks

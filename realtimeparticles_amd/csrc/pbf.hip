@@ -33,18 +33,34 @@ constexpr float kFarD2 = 1.0e12f;
 constexpr uint32_t kQueue = 24;     // queued candidates per lane
 constexpr uint32_t kMaxBlock = 256; // threads per block the LDS queue is laid out for
 
+// fld_applyBoundaryCondition (fluids.cl:435-439) applied to a loaded position.
+__device__ __forceinline__ float4 clamp_pos(float4 q, const PbfConsts& c)
+{
+  return make_float4(clampf(q.x, c.lo[0], c.hi[0]), clampf(q.y, c.lo[1], c.hi[1]),
+                     clampf(q.z, c.lo[2], c.hi[2]), 0.0f);
+}
+
+// Position e; with CL the boundary clamp is applied on the way in, which gives a pass the
+// positions the reference clamps in place at the start of a Jacobi iteration.
+template <bool CL>
+__device__ __forceinline__ float4 ldp(const float4* __restrict__ pred, uint32_t e, const PbfConsts& c)
+{
+  const float4 q = pred[e];
+  return CL ? clamp_pos(q, c) : q;
+}
+
 struct LaneQueue {
   uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * kMaxBlock]
   uint32_t count;
 };
 
-template <class F>
+template <bool CL, class F>
 __device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4* __restrict__ pred,
-                                      F& f)
+                                      const PbfConsts& c, F& f)
 {
   for (uint32_t k = 0; k < q.count; ++k) {
     const uint32_t e = q.slot[k * kMaxBlock];
-    const float4 o = pred[e];
+    const float4 o = ldp<CL>(pred, e, c);
     const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     f(e, dx, dy, dz, d2);
@@ -54,7 +70,7 @@ __device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4
 
 // Queue candidate e unless its contribution is provably +-0 (r >= h from the squared
 // distance); very far (d2 >= kFarD2) and NaN distances are queued for the exact expressions.
-template <class F>
+template <bool CL, class F>
 __device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4 o, uint32_t e,
                                       const float4* __restrict__ pred, const PbfConsts& c, F& f)
 {
@@ -63,26 +79,27 @@ __device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4
   if (!(d2 >= c.cullSq && d2 < kFarD2)) {
     q.slot[q.count * kMaxBlock] = e;
     if (++q.count == kQueue)
-      drain(q, p, pred, f);
+      drain<CL>(q, p, pred, c, f);
   }
 }
 
 // Candidates lo..hi (inclusive) in index order, four loads in flight at a time.
-template <class F>
+template <bool CL, class F>
 __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& q, const float4 p,
                                            const float4* __restrict__ pred, const PbfConsts& c,
                                            F& f)
 {
   uint32_t e = lo;
   for (; e + 3u <= hi; e += 4u) {
-    const float4 o0 = pred[e], o1 = pred[e + 1u], o2 = pred[e + 2u], o3 = pred[e + 3u];
-    offer(q, p, o0, e, pred, c, f);
-    offer(q, p, o1, e + 1u, pred, c, f);
-    offer(q, p, o2, e + 2u, pred, c, f);
-    offer(q, p, o3, e + 3u, pred, c, f);
+    const float4 o0 = ldp<CL>(pred, e, c), o1 = ldp<CL>(pred, e + 1u, c),
+                 o2 = ldp<CL>(pred, e + 2u, c), o3 = ldp<CL>(pred, e + 3u, c);
+    offer<CL>(q, p, o0, e, pred, c, f);
+    offer<CL>(q, p, o1, e + 1u, pred, c, f);
+    offer<CL>(q, p, o2, e + 2u, pred, c, f);
+    offer<CL>(q, p, o3, e + 3u, pred, c, f);
   }
   for (; e <= hi; ++e)
-    offer(q, p, pred[e], e, pred, c, f);
+    offer<CL>(q, p, ldp<CL>(pred, e, c), e, pred, c, f);
 }
 
 // Visit the candidates of particle position p in the reference's order: 27 cells, x
@@ -91,7 +108,7 @@ __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& 
 // table shows their non-empty ranges are back to back (always, unless a cell was capped or
 // the column wraps) -- the same candidates in the same order with one loop instead of three.
 // The queue is drained after every x-slab of nine cells.
-template <class F>
+template <bool CL = false, class F>
 __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
                                        const uint2* __restrict__ se, const PbfConsts& c, F&& f)
 {
@@ -126,14 +143,14 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
         }
       }
       if (ok) {
-        scan_range(lo, hi, q, p, pred, c, f);
+        scan_range<CL>(lo, hi, q, p, pred, c, f);
       } else {
-        scan_range(a.x, a.y, q, p, pred, c, f);
-        scan_range(b.x, b.y, q, p, pred, c, f);
-        scan_range(d.x, d.y, q, p, pred, c, f);
+        scan_range<CL>(a.x, a.y, q, p, pred, c, f);
+        scan_range<CL>(b.x, b.y, q, p, pred, c, f);
+        scan_range<CL>(d.x, d.y, q, p, pred, c, f);
       }
     }
-    drain(q, p, pred, f);
+    drain<CL>(q, p, pred, c, f);
   }
 }
 
@@ -200,22 +217,34 @@ __global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
 }
 
 // fld_computeDensity (fluids.cl:80-126) + fld_computeConstraintFactor (:131-193).
+// MODE bits (the plain solver uses 0):
+//   kFeatures : also write the 8 warm-start features of the particle
+//   kAccum    : add lambda into la.lamAcc (teacher runs), or start it when la.accumInit
+//   kDensOnly : density only, into `density` (the post-correction residual pass)
+//   kClampIn  : clamp positions on load (first fallback iteration)
+//   kGated    : do nothing unless *la.gate > la.tau (fallback iterations)
+template <int MODE>
 __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict__ pred,
                                                         const uint2* __restrict__ se,
                                                         float* __restrict__ density,
                                                         float* __restrict__ lambda, uint32_t n,
-                                                        PbfConsts c, int remap)
+                                                        PbfConsts c, int remap, LearnArgs la)
 {
   const uint32_t i = particle_index(remap);
   if (i >= n)
     return;
-  const float4 p = pred[i];
-  float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
-  walk27(p, pred, se, c, [&](uint32_t, float dx, float dy, float dz, float d2) {
+  if ((MODE & kGated) && !(la.gate[0] > la.tau))
+    return;
+  constexpr bool CL = (MODE & kClampIn) != 0;
+  const float4 p = ldp<CL>(pred, i, c);
+  float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f, count = 0.0f;
+  walk27<CL>(p, pred, se, c, [&](uint32_t, float dx, float dy, float dz, float d2) {
     const float r = __builtin_sqrtf(d2);
     const float t = r < c.h ? 1.0f : 0.0f;
     dens = dens + poly6(r, t, c);
-    if (r > kFloatEps) {
+    if (MODE & kFeatures)
+      count += t;
+    if (!(MODE & kDensOnly) && r > kFloatEps) {
       const Grad g = grad_spiky(dx, dy, dz, r, t, c);
       sgx = sgx + g.x;
       sgy = sgy + g.y;
@@ -224,15 +253,52 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
     }
   });
   density[i] = dens;
+  if (MODE & kDensOnly)
+    return;
   const float C = dens / c.p.restDensity - 1.0f;
   float sq = ssq + (sgx * sgx + sgy * sgy + sgz * sgz);
   sq = sq / c.rho0sq;
-  lambda[i] = -C / (sq + c.p.relaxCFM);
+  const float lam = -C / (sq + c.p.relaxCFM);
+  lambda[i] = lam;
+  if (MODE & kAccum)
+    la.lamAcc[i] = la.accumInit ? lam : la.lamAcc[i] + lam;
+  if (MODE & kFeatures) {
+    // constraint, plain lambda, neighbour count, sum |grad C|^2, the particle's own
+    // |grad C|^2 (near zero inside the fluid, large at its surface), speed and the
+    // distances to the two nearest walls of the clamp box
+    const float4 v = la.vel[i];
+    float w[6] = { p.x - c.lo[0], c.hi[0] - p.x, p.y - c.lo[1], c.hi[1] - p.y, p.z - c.lo[2], c.hi[2] - p.z };
+    float m0 = 3.0e38f, m1 = 3.0e38f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float d = fmaxf(w[k], 0.0f);
+      if (d < m0) {
+        m1 = m0;
+        m0 = d;
+      } else if (d < m1) {
+        m1 = d;
+      }
+    }
+    float* f = la.feat + (size_t)i * 8u;
+    f[0] = C;
+    f[1] = lam;
+    f[2] = count;
+    f[3] = sq;
+    f[4] = (sgx * sgx + sgy * sgy + sgz * sgz) / c.rho0sq;
+    f[5] = __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    f[6] = m0;
+    f[7] = m1;
+  }
 }
 
 // fld_computeConstraintCorrection (fluids.cl:198-247) + fld_correctPosition (:252-258),
-// then fld_applyBoundaryCondition of the next iteration, or fld_updateVel (:263-273) and,
-// without vorticity, fld_updatePosition (:444-450).
+// then one of (`last`):
+//   kNextIter : fld_applyBoundaryCondition of the next iteration
+//   kLastVel  : fld_updateVel (:263-273) and, without vorticity, fld_updatePosition (:444-450)
+//   kLastOnly : nothing more (the learned step updates the velocity after its fallback)
+// MODE bits kClampIn / kGated as for k_density_lambda; a gated pass that does not run
+// copies its input positions through, so the buffer sequence does not depend on the gate.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred,
                                                  const float* __restrict__ lambda,
                                                  const uint2* __restrict__ se,
@@ -240,15 +306,20 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
                                                  const float4* __restrict__ pos,
                                                  float4* __restrict__ vel,
                                                  float4* __restrict__ posOut, uint32_t n,
-                                                 int last, PbfConsts c, int remap)
+                                                 int last, PbfConsts c, int remap, LearnArgs la)
 {
   const uint32_t i = particle_index(remap);
   if (i >= n)
     return;
-  const float4 p = pred[i];
+  if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
+    predOut[i] = pred[i];
+    return;
+  }
+  constexpr bool CL = (MODE & kClampIn) != 0;
+  const float4 p = ldp<CL>(pred, i, c);
   const float li = lambda[i];
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
+  walk27<CL>(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
     const float r = __builtin_sqrtf(d2);
     const float t = r < c.h ? 1.0f : 0.0f;
     const float s = li + lambda[e] + art_pressure(poly6(r, t, c), c);
@@ -268,7 +339,7 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
   q.y = p.y + cy / c.p.restDensity;
   q.z = p.z + cz / c.p.restDensity;
   q.w = 0.0f;
-  if (!last) {
+  if (last == kNextIter) {
     q.x = clampf(q.x, c.lo[0], c.hi[0]);
     q.y = clampf(q.y, c.lo[1], c.hi[1]);
     q.z = clampf(q.z, c.lo[2], c.hi[2]);
@@ -276,6 +347,8 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
     return;
   }
   predOut[i] = q;
+  if (last == kLastOnly)
+    return;
   const float4 o = pos[i];
   float4 v;
   v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
@@ -412,18 +485,11 @@ __global__ __launch_bounds__(64) void k_zero3(float* __restrict__ out)
     out[threadIdx.x] = 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_density_error(const float* __restrict__ density, uint32_t n,
-                                                       float rho0, float* __restrict__ out)
+// Block-wide (max, sum, max) of three per-thread values, in a fixed order: the same
+// inputs give the same bits on every run.  Thread 0 holds the result.
+__device__ inline void block_reduce3(float& mx, float& sm, float& cm)
 {
   __shared__ float smax[4], ssum[4], scmp[4];
-  float mx = 0.0f, sm = 0.0f, cm = 0.0f;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float e = density[i] / rho0 - 1.0f;
-    const float a = fabsf(e);
-    mx = fmaxf(mx, a);
-    sm += a;
-    cm = fmaxf(cm, e);
-  }
   for (int o = 32; o > 0; o >>= 1) {
     mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     sm += __shfl_xor(sm, o, 64);
@@ -442,10 +508,45 @@ __global__ __launch_bounds__(256) void k_density_error(const float* __restrict__
       sm += ssum[k];
       cm = fmaxf(cm, scmp[k]);
     }
-    // non-negative floats order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(mx));
-    atomicAdd(out + 1, sm);
-    atomicMax(reinterpret_cast<unsigned int*>(out + 2), __float_as_uint(cm));
+  }
+}
+
+// Density error |rho/rho0 - 1| (max, sum) and the largest compression, in two passes with
+// a fixed grid of at most kErrBlocks blocks so the float sum does not depend on timing.
+constexpr uint32_t kErrBlocks = 1024;
+__global__ __launch_bounds__(256) void k_density_error(const float* __restrict__ density, uint32_t n,
+                                                       float rho0, float* __restrict__ partial)
+{
+  float mx = 0.0f, sm = 0.0f, cm = 0.0f;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float e = density[i] / rho0 - 1.0f;
+    const float a = fabsf(e);
+    mx = fmaxf(mx, a);
+    sm += a;
+    cm = fmaxf(cm, e);
+  }
+  block_reduce3(mx, sm, cm);
+  if (threadIdx.x == 0) {
+    partial[3u * blockIdx.x] = mx;
+    partial[3u * blockIdx.x + 1u] = sm;
+    partial[3u * blockIdx.x + 2u] = cm;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_density_error_final(const float* __restrict__ partial,
+                                                             uint32_t blocks, float* __restrict__ out)
+{
+  float mx = 0.0f, sm = 0.0f, cm = 0.0f;
+  for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+    mx = fmaxf(mx, partial[3u * b]);
+    sm += partial[3u * b + 1u];
+    cm = fmaxf(cm, partial[3u * b + 2u]);
+  }
+  block_reduce3(mx, sm, cm);
+  if (threadIdx.x == 0) {
+    out[0] = mx;
+    out[1] = sm;
+    out[2] = cm;
   }
 }
 
@@ -465,22 +566,49 @@ void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint
 
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
-                               hipStream_t s)
+                               hipStream_t s, int mode, const LearnArgs& la)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_density_lambda, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
-                     startEnd, density, lambda, n, c, L.xcdRemap);
+  const dim3 g(blocks_for(n, L.block)), b(L.block);
+#define RTP_DL(M)                                                                              \
+  case M:                                                                                    \
+    hipLaunchKernelGGL(k_density_lambda<M>, g, b, 0, s, pred, startEnd, density, lambda, n, c, \
+                       L.xcdRemap, la);                                                      \
+    break;
+  switch (mode) {
+    RTP_DL(0)
+    RTP_DL(kFeatures)
+    RTP_DL(kAccum)
+    RTP_DL(kFeatures | kAccum)
+    RTP_DL(kDensOnly)
+    RTP_DL(kGated)
+    RTP_DL(kGated | kClampIn)
+  default: break;
+  }
+#undef RTP_DL
 }
 
 void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
                         float4* predOut, const float4* pos, float4* vel, float4* posOut,
-                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s)
+                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s,
+                        int mode, const LearnArgs& la)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_correct, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred, lambda,
-                     startEnd, predOut, pos, vel, posOut, n, last, c, L.xcdRemap);
+  const dim3 g(blocks_for(n, L.block)), b(L.block);
+#define RTP_CO(M)                                                                              \
+  case M:                                                                                    \
+    hipLaunchKernelGGL(k_correct<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel,    \
+                       posOut, n, last, c, L.xcdRemap, la);                                  \
+    break;
+  switch (mode) {
+    RTP_CO(0)
+    RTP_CO(kGated)
+    RTP_CO(kGated | kClampIn)
+  default: break;
+  }
+#undef RTP_CO
 }
 
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
@@ -522,16 +650,19 @@ void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* ve
                      velIn, velOut, posOut, n, c, L.xcdRemap);
 }
 
-void launch_density_error(const float* density, uint32_t n, float rho0, float* scratch,
+void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,
                           hipStream_t s)
 {
-  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, scratch);
-  if (n == 0)
+  if (n == 0) {
+    hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, metric);
     return;
+  }
   uint32_t blocks = blocks_for(n, 256);
-  if (blocks > 2048)
-    blocks = 2048;
-  hipLaunchKernelGGL(k_density_error, dim3(blocks), dim3(256), 0, s, density, n, rho0, scratch);
+  if (blocks > kErrBlocks)
+    blocks = kErrBlocks;
+  float* partial = metric + 4;
+  hipLaunchKernelGGL(k_density_error, dim3(blocks), dim3(256), 0, s, density, n, rho0, partial);
+  hipLaunchKernelGGL(k_density_error_final, dim3(1), dim3(256), 0, s, partial, blocks, metric);
 }
 
 }  // namespace rtp

@@ -107,14 +107,27 @@ void launch_copy_f4(float4* dst, const float4* src, uint32_t begin, uint32_t end
 // ---------------------------------------------------------------------------------- PBF
 void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint32_t* key,
                         uint2* startEnd, uint32_t n, const PbfConsts& c, hipStream_t s);
+// Optional buffers of the warm-start, teacher and fallback variants of the PBF passes.
+struct LearnArgs {
+  float* feat = nullptr;             // [n][8] features (kFeatures)
+  const float4* vel = nullptr;       // sorted velocities (speed feature)
+  float* lamAcc = nullptr;           // accumulated lambda (kAccum)
+  int accumInit = 0;
+  const float* gate = nullptr;       // summed residual density error (kGated)
+  float tau = 0.0f;                  // threshold on *gate (the mean threshold times n)
+};
+enum PassMode { kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16 };
+enum LastMode { kNextIter = 0, kLastVel = 1, kLastOnly = 2 };
+
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
-                               hipStream_t s);
-// Constraint correction + correctPosition; then either the next iteration's boundary clamp
-// (last == 0) or updateVel (+ updatePosition when vorticity is off) (last != 0).
+                               hipStream_t s, int mode = 0, const LearnArgs& la = LearnArgs());
+// Constraint correction + correctPosition, then per `last` (LastMode) the next iteration's
+// boundary clamp, or updateVel (+ updatePosition when posOut), or nothing.
 void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
                         float4* predOut, const float4* pos, float4* vel, float4* posOut,
-                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s);
+                        uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s,
+                        int mode = 0, const LearnArgs& la = LearnArgs());
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
                            uint32_t n, const PbfConsts& c, hipStream_t s);
 void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
@@ -126,9 +139,11 @@ void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const flo
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
                      float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
                      const Launch& L, hipStream_t s);
-// Reduction of |rho/rho0 - 1| over n particles into out[0..2] (max, sum, max compression);
-// scratch holds 3 floats (zeroed by the launcher's own kernel).
-void launch_density_error(const float* density, uint32_t n, float rho0, float* scratch,
+// Reduction of |rho/rho0 - 1| over n particles into metric[0..2] (max, sum, max
+// compression), deterministic for a given n; metric holds kMetricFloats floats, the ones
+// past metric[3] being per-block partials.
+constexpr uint32_t kMetricFloats = 4 + 3 * 1024;
+void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,
                           hipStream_t s);
 
 // -------------------------------------------------------------------------------- boids
