@@ -183,19 +183,80 @@ int rtp_mlp_gradients(rtp_mlp* h, const float* feat, const float* target, uint32
 int rtp_mlp_train(rtp_mlp* h, const float* feat, const float* target, uint32_t n, uint32_t epochs,
                   uint32_t batch, float lr, uint32_t seed, float* epochLoss);
 /* Use the network for the first of learnedIters Jacobi iterations of every substep (NULL
- * switches back to the plain solver).  After those iterations a density pass measures
- * the mean |rho/rho0 - 1|; when it exceeds tau, fallbackIters plain iterations follow. */
+ * switches back to the plain solver).  With fallbackIters > 0, a density pass after those
+ * iterations measures the mean |rho/rho0 - 1|; when it exceeds tau, fallbackIters plain
+ * iterations follow. */
 int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters,
                               uint32_t fallbackIters, float tau);
-/* Run that post-correction density pass after plain substeps too (diagnostics). */
+/* Run a post-correction density pass in every substep, plain or learned, after any
+ * fallback iterations (diagnostics: the error the substep leaves). */
 int rtp_fluids_set_residual_pass(rtp_fluids* h, int enable);
-/* max, mean and max-compression |rho/rho0 - 1| of the last post-correction density pass
- * (the same bits for the same state on every run). */
+/* max, mean and max-compression |rho/rho0 - 1| of the last post-correction density pass,
+ * the gate's or the diagnostic one (the same bits for the same state on every run). */
 int rtp_fluids_residual_error(rtp_fluids* h, float out[3]);
 /* One substep with teacherIters plain iterations that records, for every active particle
  * in the substep's sorted order, the network inputs (n x 8) and the sum of the lambdas of
  * all iterations (the training target).  The model state advances by that substep. */
 int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat, float* target);
+
+/* ======================= Launch tuning (no reference counterpart) ==================
+ * The reference fixes its launch shapes (local size 512 for the sort, RadixSort.cpp:23-60,
+ * and the runtime's choice elsewhere).  Here the per-particle block size, the XCD-aware
+ * block mapping and the sort's digit width are chosen per workload by timing it. */
+typedef struct {
+  int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
+  int32_t xcdRemap;   /* 1: consecutive particle blocks on one XCD (single models only) */
+  int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
+  float msPerStep;    /* measured mean substep time of this choice */
+} rtp_tuning;
+
+/* Times fixed-input substeps (the current state restored before each) for every candidate
+ * launch shape, keeps the fastest and restores the state.  trialSteps substeps per
+ * candidate (after one untimed); best (optional) receives the choice. */
+int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best);
+
+/* ================= Batched fluids (no reference counterpart) =================
+ * Many independent PBF scenes of uneven sizes advanced together: every pass of a substep
+ * is one launch over all scenes (per-scene particle offsets, key ranges and constants), the
+ * grid one sort over all of them.  Each scene evolves exactly as an rtp_fluids model of its
+ * own whose particles are all active, with the batch's iteration count. */
+typedef struct rtp_batch rtp_batch;
+
+typedef struct {
+  uint32_t nbParticles;     /* active particles of the scene (all of its slots) */
+  float boxSize[3];         /* as rtp_model_desc */
+  int32_t gridRes[3];
+  uint32_t maxPartsInCell;  /* 0 = 100 */
+  float gravity;            /* 0 = 9.81 */
+} rtp_batch_scene;
+
+int rtp_batch_create(int device, const rtp_batch_scene* scenes, uint32_t count, rtp_batch** out);
+int rtp_batch_destroy(rtp_batch* h);
+/* FluidKernelInputs of one scene, or of every scene with scene == UINT32_MAX.  The
+ * vorticity switch (which decides the passes of a substep) must agree across scenes. */
+int rtp_batch_set_params(rtp_batch* h, uint32_t scene, const rtp_fluid_params* p);
+int rtp_batch_set_nb_jacobi_iters(rtp_batch* h, uint32_t iters);
+/* Positions and velocities of one scene, nbParticles float4 each (w ignored). */
+int rtp_batch_load_state(rtp_batch* h, uint32_t scene, const float* pos4, const float* vel4);
+int rtp_batch_read_state(rtp_batch* h, uint32_t scene, float* pos4, float* vel4);
+/* One scene's field (rtp_field, fluids fields), nbParticles entries. */
+int rtp_batch_read_field(rtp_batch* h, uint32_t scene, int field, float* out);
+/* One scene's grid in scene-local terms, as rtp_fluids_read_grid of a model of its own:
+ * cell IDs and sort indices (nbParticles each), start/end (2 x its cell count). */
+int rtp_batch_read_grid(rtp_batch* h, uint32_t scene, uint32_t* keys, uint32_t* idx,
+                        uint32_t* startEnd);
+int rtp_batch_update(rtp_batch* h, uint32_t nsteps);
+int rtp_batch_sync(rtp_batch* h);
+int rtp_batch_save_checkpoint(rtp_batch* h);
+int rtp_batch_restore_checkpoint(rtp_batch* h);
+int rtp_batch_use_graph(rtp_batch* h, int enable);
+/* xcdRemap must be 0 for a batch. */
+int rtp_batch_set_tuning(rtp_batch* h, int blockSize, int sortBits);
+int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
+int rtp_batch_enable_profiling(rtp_batch* h, int enable);
+int rtp_batch_profile(rtp_batch* h, rtp_kernel_time* out, int capacity, int* count);
+/* Total particles and cell-table entries of the batch. */
+int rtp_batch_sizes(rtp_batch* h, uint32_t* particles, uint32_t* cells);
 
 /* ============================== Boids ================================ */
 typedef struct rtp_boids rtp_boids;

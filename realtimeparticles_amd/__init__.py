@@ -11,6 +11,7 @@ from .capi import (  # noqa: F401
     FIELD_LAMBDA,
     FIELD_PRED_POS,
     FIELD_VORTICITY,
+    Batch,
     Boids,
     BoidsParams,
     FluidParams,
@@ -24,6 +25,6 @@ from .capi import (  # noqa: F401
 )
 
 __all__ = [
-    "Boids", "BoidsParams", "FluidParams", "Fluids", "Mlp", "RtpError", "device_count", "grid_build",
+    "Batch", "Boids", "BoidsParams", "FluidParams", "Fluids", "Mlp", "RtpError", "device_count", "grid_build",
     "lib", "sort_pairs", "scenes",
 ]

@@ -72,6 +72,24 @@ class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", _u32), ("totalMs", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("blockSize", _i32), ("xcdRemap", _i32), ("sortBits", _i32), ("msPerStep", _f32)]
+
+    def as_dict(self) -> dict:
+        return {"block": int(self.blockSize), "xcd_remap": int(self.xcdRemap),
+                "sort_bits": int(self.sortBits), "ms_per_step": float(self.msPerStep)}
+
+
+class BatchScene(ctypes.Structure):
+    _fields_ = [
+        ("nbParticles", _u32),
+        ("boxSize", _f32 * 3),
+        ("gridRes", _i32 * 3),
+        ("maxPartsInCell", _u32),
+        ("gravity", _f32),
+    ]
+
+
 FLUID_DEFAULTS = dict(restDensity=450.0, relaxCFM=600.0, timeStep=0.010, dim=3,
                       isArtPressureEnabled=1, artPressureRadius=0.006, artPressureCoeff=0.001,
                       artPressureExp=4, isVorticityConfEnabled=1, vorticityConfCoeff=0.0004,
@@ -133,6 +151,25 @@ _SIGS = {
     "rtp_boids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_boids_enable_profiling": ([_vp, _int], _int),
     "rtp_boids_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
+    "rtp_fluids_autotune": ([_vp, _u32, ctypes.POINTER(Tuning)], _int),
+    "rtp_batch_create": ([_int, _vp, _u32, ctypes.POINTER(_vp)], _int),
+    "rtp_batch_destroy": ([_vp], _int),
+    "rtp_batch_set_params": ([_vp, _u32, ctypes.POINTER(FluidParams)], _int),
+    "rtp_batch_set_nb_jacobi_iters": ([_vp, _u32], _int),
+    "rtp_batch_load_state": ([_vp, _u32, _vp, _vp], _int),
+    "rtp_batch_read_state": ([_vp, _u32, _vp, _vp], _int),
+    "rtp_batch_read_field": ([_vp, _u32, _int, _vp], _int),
+    "rtp_batch_read_grid": ([_vp, _u32, _vp, _vp, _vp], _int),
+    "rtp_batch_update": ([_vp, _u32], _int),
+    "rtp_batch_sync": ([_vp], _int),
+    "rtp_batch_save_checkpoint": ([_vp], _int),
+    "rtp_batch_restore_checkpoint": ([_vp], _int),
+    "rtp_batch_use_graph": ([_vp, _int], _int),
+    "rtp_batch_set_tuning": ([_vp, _int, _int], _int),
+    "rtp_batch_autotune": ([_vp, _u32, ctypes.POINTER(Tuning)], _int),
+    "rtp_batch_enable_profiling": ([_vp, _int], _int),
+    "rtp_batch_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
+    "rtp_batch_sizes": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _vp, _vp], _int),
     "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
 }
@@ -341,6 +378,135 @@ class Fluids:
     def set_tuning(self, block: int = 256, xcd_remap: int = 0, sort_bits: int = 0) -> None:
         _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(xcd_remap), int(sort_bits)),
                "set_tuning")
+
+    def autotune(self, trial_steps: int = 5) -> dict:
+        t = Tuning()
+        _check(lib().rtp_fluids_autotune(self._h, int(trial_steps), ctypes.byref(t)), "autotune")
+        return t.as_dict()
+
+
+ALL_SCENES = 0xFFFFFFFF
+
+
+class Batch:
+    """Many independent PBF scenes advanced by shared launches (rtp_batch_*).
+
+    scenes: sequence of objects with n, box, res, gravity (scenes.FluidScene) or of
+    (n, box, res, gravity) tuples; every particle of a scene is active.
+    """
+
+    def __init__(self, scenes, max_parts_in_cell: int = 0, device: int = 0):
+        self._h = _vp()
+        arr = (BatchScene * len(scenes))()
+        self.sizes = []
+        self.cells = []
+        for k, sc in enumerate(scenes):
+            n, box, res, g = ((sc.n, sc.box, sc.res, sc.gravity) if hasattr(sc, "n") else sc)
+            arr[k].nbParticles = int(n)
+            for a in range(3):
+                arr[k].boxSize[a] = float(box[a])
+                arr[k].gridRes[a] = int(res[a])
+            arr[k].maxPartsInCell = int(max_parts_in_cell)
+            arr[k].gravity = float(g)
+            self.sizes.append(int(n))
+            self.cells.append(int(res[0]) * int(res[1]) * int(res[2]))
+        _check(lib().rtp_batch_create(int(device), arr, len(scenes), ctypes.byref(self._h)),
+               "rtp_batch_create")
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtp_batch_destroy(self._h)
+            self._h = _vp()
+
+    def _n(self, scene: int) -> int:
+        if not 0 <= int(scene) < len(self.sizes):
+            raise RtpError(f"scene index {scene} out of range (batch of {len(self.sizes)})")
+        return self.sizes[int(scene)]
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, scene: int | None = None, **kw) -> FluidParams:
+        p = FluidParams(**{**FLUID_DEFAULTS, **kw})
+        _check(lib().rtp_batch_set_params(self._h, ALL_SCENES if scene is None else int(scene),
+                                          ctypes.byref(p)), "rtp_batch_set_params")
+        return p
+
+    def set_nb_jacobi_iters(self, iters: int) -> None:
+        _check(lib().rtp_batch_set_nb_jacobi_iters(self._h, int(iters)), "set_nb_jacobi_iters")
+
+    def load_state(self, scene: int, pos, vel) -> None:
+        pos, vel = _f4(pos), _f4(vel)
+        if pos.shape != (self._n(scene), 4) or vel.shape != pos.shape:
+            raise ValueError("state must be (nbParticles, 4) for the scene")
+        _check(lib().rtp_batch_load_state(self._h, int(scene), pos.ctypes.data, vel.ctypes.data),
+               "rtp_batch_load_state")
+
+    def read_state(self, scene: int):
+        n = self._n(scene)
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        _check(lib().rtp_batch_read_state(self._h, int(scene), pos.ctypes.data, vel.ctypes.data),
+               "rtp_batch_read_state")
+        return pos, vel
+
+    def read_field(self, scene: int, field: int) -> np.ndarray:
+        n = self._n(scene)
+        out = np.zeros((n, 4) if field in (FIELD_PRED_POS, FIELD_VORTICITY) else (n,), np.float32)
+        _check(lib().rtp_batch_read_field(self._h, int(scene), int(field), out.ctypes.data),
+               "rtp_batch_read_field")
+        return out
+
+    def read_grid(self, scene: int):
+        n = self._n(scene)
+        keys = np.zeros(n, np.uint32)
+        idx = np.zeros(n, np.uint32)
+        se = np.zeros((self.cells[scene], 2), np.uint32)
+        _check(lib().rtp_batch_read_grid(self._h, int(scene), keys.ctypes.data, idx.ctypes.data,
+                                         se.ctypes.data), "rtp_batch_read_grid")
+        return keys, idx, se
+
+    def update(self, nsteps: int = 1) -> None:
+        _check(lib().rtp_batch_update(self._h, int(nsteps)), "rtp_batch_update")
+
+    def sync(self) -> None:
+        _check(lib().rtp_batch_sync(self._h), "rtp_batch_sync")
+
+    def save_checkpoint(self) -> None:
+        _check(lib().rtp_batch_save_checkpoint(self._h), "save_checkpoint")
+
+    def restore_checkpoint(self) -> None:
+        _check(lib().rtp_batch_restore_checkpoint(self._h), "restore_checkpoint")
+
+    def use_graph(self, flag: bool) -> None:
+        _check(lib().rtp_batch_use_graph(self._h, int(bool(flag))), "use_graph")
+
+    def set_tuning(self, block: int = 256, sort_bits: int = 0) -> None:
+        _check(lib().rtp_batch_set_tuning(self._h, int(block), int(sort_bits)), "set_tuning")
+
+    def autotune(self, trial_steps: int = 5) -> dict:
+        t = Tuning()
+        _check(lib().rtp_batch_autotune(self._h, int(trial_steps), ctypes.byref(t)), "autotune")
+        return t.as_dict()
+
+    def enable_profiling(self, flag: bool) -> None:
+        _check(lib().rtp_batch_enable_profiling(self._h, int(bool(flag))), "enable_profiling")
+
+    def profile(self) -> dict:
+        cnt = _int(0)
+        _check(lib().rtp_batch_profile(self._h, None, 0, ctypes.byref(cnt)), "profile")
+        arr = (KernelTime * max(cnt.value, 1))()
+        _check(lib().rtp_batch_profile(self._h, arr, cnt.value, ctypes.byref(cnt)), "profile")
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].totalMs))
+                for i in range(cnt.value)}
+
+    def totals(self):
+        p, c = _u32(0), _u32(0)
+        _check(lib().rtp_batch_sizes(self._h, ctypes.byref(p), ctypes.byref(c)), "sizes")
+        return int(p.value), int(c.value)
 
 
 class Mlp:

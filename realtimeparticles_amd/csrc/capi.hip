@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "rtp_host.hpp"
 #include "rtp_internal.hpp"
 #include "rtp_mlp.hpp"
 
@@ -32,183 +33,6 @@ int fail(int status, const std::string& msg)
 }
 
 namespace {
-
-// ------------------------------------------------------------------------- host constants
-
-// utils/Utils.cpp:24-29 FloatToStr(v, 10) followed by the OpenCL compiler's parse.
-float float_roundtrip(float v)
-{
-  char buf[128];
-  std::snprintf(buf, sizeof(buf), "%.10f", (double)v);
-  return std::strtof(buf, nullptr);
-}
-
-constexpr float kPiF = 3.1415927f;  // utils/Math.hpp:69
-
-int make_grid(const rtp_model_desc* d, uint32_t defaultCap, GridConsts* g)
-{
-  for (int a = 0; a < 3; ++a) {
-    if (!(d->boxSize[a] > 0.0f) || d->gridRes[a] <= 0)
-      return fail(RTP_ERR_INVALID, "boxSize and gridRes must be positive");
-    g->absWall[a] = float_roundtrip(d->boxSize[a] / 2.0f);  // Fluids.cpp:108-110
-    g->res[a] = d->gridRes[a];
-  }
-  const uint64_t nc = (uint64_t)d->gridRes[0] * d->gridRes[1] * d->gridRes[2];
-  if (nc == 0 || nc > (1u << 30))
-    return fail(RTP_ERR_INVALID, "grid has too many cells");
-  g->cellSize = float_roundtrip(d->boxSize[0] / (float)d->gridRes[0]);  // Fluids.cpp:114
-  g->numCells = (uint32_t)nc;
-  g->cap = d->maxPartsInCell ? d->maxPartsInCell : defaultCap;
-  return RTP_OK;
-}
-
-// Largest cell key fillCellIDs can produce (each 3-D index is at most res when a position
-// sits exactly on the +wall) — the sort only needs that many bits.
-uint32_t max_active_key(const GridConsts& g)
-{
-  const uint64_t ry = (uint64_t)g.res[1], rz = (uint64_t)g.res[2];
-  const uint64_t m = (uint64_t)g.res[0] * rz * ry + ry * rz + rz;
-  return (uint32_t)std::min<uint64_t>(m, 0xffffffffu);
-}
-
-// Smallest float that is >= h*h as a real number.
-float cull_bound(float h)
-{
-  const double hh = (double)h * (double)h;
-  float t = (float)hh;
-  if ((double)t < hh)
-    t = std::nextafter(t, INFINITY);
-  return t;
-}
-
-// --------------------------------------------------------------------------------- buffers
-
-template <class T>
-int dev_alloc(T** p, size_t count)
-{
-  *p = nullptr;
-  if (count == 0)
-    count = 1;
-  RTP_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count));
-  return RTP_OK;
-}
-
-template <class T>
-void dev_free(T*& p)
-{
-  if (p)
-    (void)hipFree(p);
-  p = nullptr;
-}
-
-// ------------------------------------------------------------------------------- profiling
-
-struct Profiler {
-  bool enabled = false;
-  struct Pending {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<std::string> names;
-  std::vector<uint32_t> launches;
-  std::vector<double> totals;
-  std::vector<Pending> pending;
-  std::vector<hipEvent_t> pool;
-
-  int id_of(const char* name)
-  {
-    for (size_t i = 0; i < names.size(); ++i)
-      if (names[i] == name)
-        return (int)i;
-    names.emplace_back(name);
-    launches.push_back(0);
-    totals.push_back(0.0);
-    return (int)names.size() - 1;
-  }
-  hipEvent_t take()
-  {
-    if (!pool.empty()) {
-      hipEvent_t e = pool.back();
-      pool.pop_back();
-      return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-  }
-  // Wrap one launch.  Call begin() before and end() after enqueueing it.
-  int beginId = -1;
-  hipEvent_t beginEv = nullptr;
-  void begin(const char* name, hipStream_t s)
-  {
-    if (!enabled)
-      return;
-    beginId = id_of(name);
-    beginEv = take();
-    (void)hipEventRecord(beginEv, s);
-  }
-  void end(hipStream_t s)
-  {
-    if (!enabled || beginId < 0)
-      return;
-    hipEvent_t e = take();
-    (void)hipEventRecord(e, s);
-    pending.push_back(Pending{ beginId, beginEv, e });
-    beginId = -1;
-  }
-  int collect()
-  {
-    for (auto& p : pending) {
-      RTP_HIP_TRY(hipEventSynchronize(p.b));
-      float ms = 0.0f;
-      RTP_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
-      launches[p.id] += 1;
-      totals[p.id] += ms;
-      pool.push_back(p.a);
-      pool.push_back(p.b);
-    }
-    pending.clear();
-    return RTP_OK;
-  }
-  void reset()
-  {
-    (void)collect();
-    for (auto& l : launches)
-      l = 0;
-    for (auto& t : totals)
-      t = 0.0;
-  }
-  int report(rtp_kernel_time* out, int capacity, int* count)
-  {
-    const int rc = collect();
-    if (rc)
-      return rc;
-    const int n = (int)names.size();
-    if (count)
-      *count = n;
-    for (int i = 0; i < n && i < capacity && out; ++i) {
-      std::memset(out[i].name, 0, sizeof(out[i].name));
-      std::strncpy(out[i].name, names[i].c_str(), sizeof(out[i].name) - 1);
-      out[i].launches = launches[i];
-      out[i].totalMs = totals[i];
-    }
-    return RTP_OK;
-  }
-  void destroy()
-  {
-    (void)collect();
-    for (auto e : pool)
-      (void)hipEventDestroy(e);
-    pool.clear();
-  }
-};
-
-#define RTP_PROF(prof, name, stream, launch) \
-  do {                                       \
-    (prof).begin(name, stream);              \
-    launch;                                  \
-    (prof).end(stream);                      \
-  } while (0)
 
 // ------------------------------------------------------------------------- shared grid state
 
@@ -301,28 +125,6 @@ struct GridState {
     return RTP_OK;
   }
 };
-
-int check_device(int device)
-{
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
-    return fail(RTP_ERR_NO_DEVICE, "no HIP device visible");
-  if (device < 0 || device >= count)
-    return fail(RTP_ERR_INVALID, "device ordinal out of range");
-  RTP_HIP_TRY(hipSetDevice(device));
-  return RTP_OK;
-}
-
-int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, float4 fill,
-              hipStream_t s)
-{
-  std::vector<float4> h(maxN, fill);
-  for (uint32_t i = 0; i < count; ++i)
-    h[i] = make_float4(src[4 * i + 0], src[4 * i + 1], src[4 * i + 2], 0.0f);
-  RTP_HIP_TRY(hipMemcpyAsync(dst, h.data(), sizeof(float4) * maxN, hipMemcpyHostToDevice, s));
-  RTP_HIP_TRY(hipStreamSynchronize(s));
-  return RTP_OK;
-}
 
 }  // namespace
 }  // namespace rtp
@@ -437,25 +239,9 @@ struct rtp_fluids {
     graphValid = false;
   }
 
-  // The host-side constant folding of Fluids::createProgram (Fluids.cpp:97-126) and of the
-  // kernels' constant subexpressions.
   void refresh_consts()
   {
-    const float h = c.h;
-    c.hh = h * h;
-    c.cullSq = cull_bound(h);
-    const rtp_fluid_params& p = c.p;
-    const float dq = p.artPressureRadius * h;
-    const float t = dq < h ? 1.0f : 0.0f;  // 1 - step(h, dq)
-    const float d = h * h - dq * dq;
-    c.wdq = (t * c.poly6) * ((d * d) * d);
-    c.rho0sq = p.restDensity * p.restDensity;
-    c.dtEps = p.timeStep + 0.00000001f;
-    c.gdt = -c.gravity * p.timeStep;
-    for (int a = 0; a < 3; ++a) {
-      c.lo[a] = -c.g.absWall[a] + 0.01f;
-      c.hi[a] = c.g.absWall[a] - 0.1f;
-    }
+    refresh_pbf_consts(c);
     drop_graph();
   }
 
@@ -523,13 +309,15 @@ struct rtp_fluids {
                                   (last == kLastVel && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s));
       in = 1 - in;
     }
-    if (learned || residualPass) {
-      // post-correction density: the residual the learned step is judged by
+    // post-correction density: the residual the fallback gate and the diagnostics read
+    auto residual = [&]() {
       RTP_PROF(prof, "pbf_residual", s,
                launch_pbf_density_lambda(pred[in], G.startEnd, resDensity, nullptr, n, c, L, s, kDensOnly, la));
       launch_density_error(resDensity, n, c.p.restDensity, resMetric, s);
-    }
-    if (learned) {
+    };
+    const bool gated = learned && fallbackIters > 0;
+    if (gated) {
+      residual();
       for (uint32_t f = 0; f < fallbackIters; ++f) {
         const int mode = kGated | (f == 0 ? kClampIn : 0);
         const int last = (f + 1 == fallbackIters) ? kLastOnly : kNextIter;
@@ -541,6 +329,8 @@ struct rtp_fluids {
         in = 1 - in;
       }
     }
+    if (residualPass)  // after any fallback iterations: the error the substep leaves
+      residual();
     if (iters == 0 || learned)
       RTP_PROF(prof, "pbf_update_vel", s,
                launch_pbf_update_vel(pred[in], pos[cur], vel[cur], vortOn ? nullptr : pos[cur], n,
@@ -709,12 +499,6 @@ struct rtp_boids {
 
 // ================================================================================= C ABI
 
-#define RTP_REQUIRE(cond, msg)               \
-  do {                                       \
-    if (!(cond))                             \
-      return fail(RTP_ERR_INVALID, msg);     \
-  } while (0)
-
 extern "C" {
 
 int rtp_abi_version(void) { return RTP_ABI_VERSION; }
@@ -748,15 +532,7 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
     delete h;
     return rc;
   }
-  PbfConsts& c = h->c;
-  c.g = h->grid.g;
-  const float effectRadius = d->boxSize[0] / (float)d->gridRes[0];           // Fluids.cpp:104
-  c.h = float_roundtrip(effectRadius);
-  c.poly6 = float_roundtrip(315.0f / (64.0f * kPiF * std::pow(effectRadius, 9.f)));  // :117
-  c.spiky = float_roundtrip(15.0f / (kPiF * std::pow(effectRadius, 6.f)));           // :118
-  c.maxVel = float_roundtrip(30.0f);                                                  // :119
-  c.gravity = d->gravity > 0.0f ? d->gravity : 9.81f;                                 // define.cl:8
-  c.p = rtp_fluid_params{ 450.0f, 600.0f, 0.010f, 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
+  init_pbf_consts(h->c, h->grid.g, d->boxSize[0], d->gridRes[0], d->gravity);
   h->n = d->currNbParticles;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
@@ -1052,6 +828,67 @@ int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBi
   h->L.xcdRemap = xcdRemap ? 1 : 0;
   h->grid.sortBits = sortBits;
   h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(trialSteps > 0 && trialSteps <= 1000, "trialSteps must be 1..1000");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  int rc = rtp_fluids_save_checkpoint(h);
+  if (rc)
+    return rc;
+  const bool profWas = h->prof.enabled;
+  h->prof.enabled = false;
+  const bool pausedWas = h->paused;
+  h->paused = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RTP_HIP_TRY(hipEventCreate(&e0));
+  RTP_HIP_TRY(hipEventCreate(&e1));
+  rtp_tuning top{ h->L.block, h->L.xcdRemap, h->grid.sortBits, 1e30f };
+  const int blocks[] = { 64, 128, 192, 256 };
+  const int bitsChoices[] = { 7, 8, 9, 10, 11 };
+  for (int b : blocks) {
+    for (int x = 0; x < 2 && !rc; ++x) {
+      for (int bits : bitsChoices) {
+        if ((rc = rtp_fluids_set_tuning(h, b, x, bits)))
+          break;
+        if ((rc = rtp_fluids_restore_checkpoint(h)) || (rc = h->update(1)))
+          break;
+        (void)hipEventRecord(e0, h->stream);
+        for (uint32_t t = 0; t < trialSteps && !rc; ++t)
+          if (!(rc = rtp_fluids_restore_checkpoint(h)))
+            rc = h->update(1);
+        (void)hipEventRecord(e1, h->stream);
+        if (rc)
+          break;
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = fail(RTP_ERR_HIP, "autotune timing failed");
+          break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)trialSteps;
+        if (ms < top.msPerStep)
+          top = rtp_tuning{ b, x, bits, ms };
+      }
+    }
+    if (rc)
+      break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->prof.enabled = profWas;
+  h->paused = pausedWas;
+  if (rc)
+    return rc;
+  if ((rc = rtp_fluids_set_tuning(h, top.blockSize, top.xcdRemap, top.sortBits)) ||
+      (rc = rtp_fluids_restore_checkpoint(h)))
+    return rc;
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  if (best)
+    *best = top;
   return RTP_OK;
 }
 

@@ -252,12 +252,13 @@ __global__ __launch_bounds__(256) void k_sort_downsweep(
 // Gather by idx + cell start/end (with the cap folded in: the thread that closes a run
 // knows the run's start from a probe CAP+1 entries back, and a binary search when the run
 // is longer than that).
-__global__ __launch_bounds__(256) void k_reorder(ReorderArgs a)
+// Sorted slot i: gather the payloads of the element that landed there and record the cell
+// boundaries.  kbase and ibase are the key and index bases of a batched scene (0 for a
+// model of its own): keys are compared as stored, cells and gather sources are local.
+__device__ __forceinline__ void reorder_body(uint32_t i, const ReorderArgs& a, uint32_t kbase,
+                                             uint32_t ibase)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.sortN)
-    return;
-  const uint32_t j = a.idx[i];
+  const uint32_t j = a.idx[i] - ibase;
   if (a.dstA)
     a.dstA[i] = a.srcA[j];
   if (a.dstB)
@@ -274,25 +275,26 @@ __global__ __launch_bounds__(256) void k_reorder(ReorderArgs a)
   }
   if (i >= a.n)
     return;
-  const uint32_t k = a.key[i];
+  const uint32_t kr = a.key[i];
+  const uint32_t k = kr - kbase;
   if (k >= a.g.numCells)  // grid.cl:107,128: only keys of real cells are recorded
     return;
   uint32_t* se = reinterpret_cast<uint32_t*>(a.startEnd);
-  if (i > 0 && k != a.key[i - 1])  // fillStartCell, grid.cl:101-117 (never for i == 0)
+  if (i > 0 && kr != a.key[i - 1])  // fillStartCell, grid.cl:101-117 (never for i == 0)
     se[2u * k] = i;
   // fillEndCell, grid.cl:122-138: key[i+1] is read even for the last active particle.
-  const bool isEnd = (i + 1u >= a.maxN) || (k != a.key[i + 1u]);
+  const bool isEnd = (i + 1u >= a.maxN) || (kr != a.key[i + 1u]);
   if (!isEnd)
     return;
   // adjustEndCell, grid.cl:143-152, with start = run start, or 1 (the reset value) when
   // the run starts at 0.
   const uint32_t cap = a.g.cap;
   uint32_t endv = i;
-  if (i >= cap + 1u && a.key[i - cap - 1u] == k) {
+  if (i >= cap + 1u && a.key[i - cap - 1u] == kr) {
     uint32_t lo = 0, hi = i - cap - 1u;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (a.key[mid] < k)
+      if (a.key[mid] < kr)
         lo = mid + 1u;
       else
         hi = mid;
@@ -302,6 +304,43 @@ __global__ __launch_bounds__(256) void k_reorder(ReorderArgs a)
       endv = startEff + min(i - startEff, cap);
   }
   se[2u * k + 1u] = endv;
+}
+
+__global__ __launch_bounds__(256) void k_reorder(ReorderArgs a)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.sortN)
+    reorder_body(i, a, 0u, 0u);
+}
+
+// One launch over a batch: every scene's slice is a complete, fully active model
+// (sortN = n = maxN = the scene's count) with its own grid, bounds and clamp.
+__global__ __launch_bounds__(256) void k_reorder_batch(ReorderArgs a, BatchView bv)
+{
+  const uint2 blk = bv.blocks[blockIdx.x];
+  const SceneSlot sc = bv.scenes[blk.x];
+  const uint32_t i = blk.y + threadIdx.x;
+  if (i >= sc.n)
+    return;
+  const PbfConsts& c = bv.consts[blk.x];
+  const uint32_t o = sc.pbase;
+  ReorderArgs l = a;
+  l.key = a.key + o;
+  l.idx = a.idx + o;
+  l.sortN = l.n = l.maxN = sc.n;
+  l.srcA = a.srcA ? a.srcA + o : nullptr;  // gather sources are indexed scene-locally
+  l.dstA = a.dstA ? a.dstA + o : nullptr;
+  l.srcB = a.srcB ? a.srcB + o : nullptr;
+  l.dstB = a.dstB ? a.dstB + o : nullptr;
+  l.srcC = a.srcC ? a.srcC + o : nullptr;
+  l.dstC = a.dstC ? a.dstC + o : nullptr;
+  for (int d = 0; d < 3; ++d) {
+    l.lo[d] = c.lo[d];
+    l.hi[d] = c.hi[d];
+  }
+  l.startEnd = a.startEnd + sc.kbase;
+  l.g = c.g;
+  reorder_body(i, l, sc.kbase, o);
 }
 
 template <int BITS>
@@ -421,6 +460,12 @@ void launch_copy_f4(float4* dst, const float4* src, uint32_t begin, uint32_t end
     return;
   hipLaunchKernelGGL(k_copy_f4, dim3(blocks_for(end - begin, 256)), dim3(256), 0, s, dst, src,
                      begin, end);
+}
+
+void launch_reorder_batch(const BatchView& bv, const ReorderArgs& a, hipStream_t s)
+{
+  if (bv.numBlocks)
+    hipLaunchKernelGGL(k_reorder_batch, dim3(bv.numBlocks), dim3(bv.block), 0, s, a, bv);
 }
 
 void launch_reorder(const ReorderArgs& a, hipStream_t s)

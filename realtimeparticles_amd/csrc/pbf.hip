@@ -190,6 +190,26 @@ __device__ __forceinline__ uint32_t particle_index(int remap)
 
 // ---------------------------------------------------------------------------- kernels
 
+// fluids.cl:71-73 (newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt) and
+// fillCellIDs (grid.cl:76-86) on predPos, the key offset by the scene's key base.
+__device__ __forceinline__ void predict_body(uint32_t t, const float4* __restrict__ pos,
+                                             const float4* __restrict__ vel,
+                                             float4* __restrict__ pred, uint32_t* __restrict__ key,
+                                             uint32_t kbase, const PbfConsts& c)
+{
+  const float4 p = pos[t], v = vel[t];
+  const float dt = c.p.timeStep;
+  const float zdt = 0.0f * dt;
+  const float vx = v.x + zdt, vy = v.y + c.gdt, vz = v.z + zdt;
+  float4 q;
+  q.x = p.x + vx * dt;
+  q.y = p.y + vy * dt;
+  q.z = p.z + vz * dt;
+  q.w = 0.0f;
+  pred[t] = q;
+  key[t] = kbase + cell1d(q, c.g);
+}
+
 __global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
                                                  float4* __restrict__ pred,
@@ -198,20 +218,8 @@ __global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
                                                  PbfConsts c)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) {
-    // fluids.cl:71-73: newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt
-    const float4 p = pos[t], v = vel[t];
-    const float dt = c.p.timeStep;
-    const float zdt = 0.0f * dt;
-    const float vx = v.x + zdt, vy = v.y + c.gdt, vz = v.z + zdt;
-    float4 q;
-    q.x = p.x + vx * dt;
-    q.y = p.y + vy * dt;
-    q.z = p.z + vz * dt;
-    q.w = 0.0f;
-    pred[t] = q;
-    key[t] = cell1d(q, c.g);  // grid.cl:76-86 on predPos
-  }
+  if (t < n)
+    predict_body(t, pos, vel, pred, key, 0u, c);
   if (t < c.g.numCells)
     startEnd[t] = make_uint2(1u, 0u);  // grid.cl:91-96
 }
@@ -224,15 +232,12 @@ __global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
 //   kClampIn  : clamp positions on load (first fallback iteration)
 //   kGated    : do nothing unless *la.gate > la.tau (fallback iterations)
 template <int MODE>
-__global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict__ pred,
-                                                        const uint2* __restrict__ se,
-                                                        float* __restrict__ density,
-                                                        float* __restrict__ lambda, uint32_t n,
-                                                        PbfConsts c, int remap, LearnArgs la)
+__device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __restrict__ pred,
+                                                    const uint2* __restrict__ se,
+                                                    float* __restrict__ density,
+                                                    float* __restrict__ lambda,
+                                                    const PbfConsts& c, const LearnArgs& la)
 {
-  const uint32_t i = particle_index(remap);
-  if (i >= n)
-    return;
   if ((MODE & kGated) && !(la.gate[0] > la.tau))
     return;
   constexpr bool CL = (MODE & kClampIn) != 0;
@@ -291,6 +296,18 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict__ pred,
+                                                        const uint2* __restrict__ se,
+                                                        float* __restrict__ density,
+                                                        float* __restrict__ lambda, uint32_t n,
+                                                        PbfConsts c, int remap, LearnArgs la)
+{
+  const uint32_t i = particle_index(remap);
+  if (i < n)
+    density_lambda_body<MODE>(i, pred, se, density, lambda, c, la);
+}
+
 // fld_computeConstraintCorrection (fluids.cl:198-247) + fld_correctPosition (:252-258),
 // then one of (`last`):
 //   kNextIter : fld_applyBoundaryCondition of the next iteration
@@ -299,18 +316,14 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
 // MODE bits kClampIn / kGated as for k_density_lambda; a gated pass that does not run
 // copies its input positions through, so the buffer sequence does not depend on the gate.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred,
-                                                 const float* __restrict__ lambda,
-                                                 const uint2* __restrict__ se,
-                                                 float4* __restrict__ predOut,
-                                                 const float4* __restrict__ pos,
-                                                 float4* __restrict__ vel,
-                                                 float4* __restrict__ posOut, uint32_t n,
-                                                 int last, PbfConsts c, int remap, LearnArgs la)
+__device__ __forceinline__ void correct_body(uint32_t i, const float4* __restrict__ pred,
+                                             const float* __restrict__ lambda,
+                                             const uint2* __restrict__ se,
+                                             float4* __restrict__ predOut,
+                                             const float4* __restrict__ pos,
+                                             float4* __restrict__ vel, float4* __restrict__ posOut,
+                                             int last, const PbfConsts& c, const LearnArgs& la)
 {
-  const uint32_t i = particle_index(remap);
-  if (i >= n)
-    return;
   if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
     predOut[i] = pred[i];
     return;
@@ -360,16 +373,27 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
     posOut[i] = q;
 }
 
-// fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
-__global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ pred,
-                                                    const float4* __restrict__ pos,
-                                                    float4* __restrict__ vel,
-                                                    float4* __restrict__ posOut, uint32_t n,
-                                                    PbfConsts c)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred,
+                                                 const float* __restrict__ lambda,
+                                                 const uint2* __restrict__ se,
+                                                 float4* __restrict__ predOut,
+                                                 const float4* __restrict__ pos,
+                                                 float4* __restrict__ vel,
+                                                 float4* __restrict__ posOut, uint32_t n,
+                                                 int last, PbfConsts c, int remap, LearnArgs la)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n)
-    return;
+  const uint32_t i = particle_index(remap);
+  if (i < n)
+    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la);
+}
+
+// fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
+__device__ __forceinline__ void update_vel_body(uint32_t i, const float4* __restrict__ pred,
+                                                const float4* __restrict__ pos,
+                                                float4* __restrict__ vel, float4* __restrict__ posOut,
+                                                const PbfConsts& c)
+{
   const float4 q = pred[i], o = pos[i];
   float4 v;
   v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
@@ -381,17 +405,24 @@ __global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ p
     posOut[i] = make_float4(q.x, q.y, q.z, 0.0f);
 }
 
+__global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ pred,
+                                                    const float4* __restrict__ pos,
+                                                    float4* __restrict__ vel,
+                                                    float4* __restrict__ posOut, uint32_t n,
+                                                    PbfConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    update_vel_body(i, pred, pos, vel, posOut, c);
+}
+
 // fld_computeVorticity (fluids.cl:278-324).  The w lane of the output carries
 // fast_length(vort) for the confinement pass.
-__global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pred,
-                                                   const uint2* __restrict__ se,
-                                                   const float4* __restrict__ vel,
-                                                   float4* __restrict__ vort, uint32_t n,
-                                                   PbfConsts c, int remap)
+__device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restrict__ pred,
+                                               const uint2* __restrict__ se,
+                                               const float4* __restrict__ vel,
+                                               float4* __restrict__ vort, const PbfConsts& c)
 {
-  const uint32_t i = particle_index(remap);
-  if (i >= n)
-    return;
   const float4 p = pred[i];
   const float4 vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
@@ -410,18 +441,25 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
   vort[i] = make_float4(wx, wy, wz, __builtin_sqrtf(wx * wx + wy * wy + wz * wz));
 }
 
-// fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
-// velOut (the copy fld_applyXsphViscosityCorrection reads, Fluids.cpp:451).
-__global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ pred,
-                                                     const uint2* __restrict__ se,
-                                                     const float4* __restrict__ vort,
-                                                     const float4* __restrict__ vel,
-                                                     float4* __restrict__ velOut, uint32_t n,
-                                                     PbfConsts c, int remap)
+__global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pred,
+                                                   const uint2* __restrict__ se,
+                                                   const float4* __restrict__ vel,
+                                                   float4* __restrict__ vort, uint32_t n,
+                                                   PbfConsts c, int remap)
 {
   const uint32_t i = particle_index(remap);
-  if (i >= n)
-    return;
+  if (i < n)
+    vorticity_body(i, pred, se, vel, vort, c);
+}
+
+// fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
+// velOut (the copy fld_applyXsphViscosityCorrection reads, Fluids.cpp:451).
+__device__ __forceinline__ void confinement_body(uint32_t i, const float4* __restrict__ pred,
+                                                 const uint2* __restrict__ se,
+                                                 const float4* __restrict__ vort,
+                                                 const float4* __restrict__ vel,
+                                                 float4* __restrict__ velOut, const PbfConsts& c)
+{
   const float4 p = pred[i];
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
   walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
@@ -451,17 +489,25 @@ __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ 
   velOut[i] = make_float4(v.x + (k * crx) * dt, v.y + (k * cry) * dt, v.z + (k * crz) * dt, 0.0f);
 }
 
-// fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
-__global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
-                                              const uint2* __restrict__ se,
-                                              const float4* __restrict__ velIn,
-                                              float4* __restrict__ velOut,
-                                              float4* __restrict__ posOut, uint32_t n,
-                                              PbfConsts c, int remap)
+__global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ pred,
+                                                     const uint2* __restrict__ se,
+                                                     const float4* __restrict__ vort,
+                                                     const float4* __restrict__ vel,
+                                                     float4* __restrict__ velOut, uint32_t n,
+                                                     PbfConsts c, int remap)
 {
   const uint32_t i = particle_index(remap);
-  if (i >= n)
-    return;
+  if (i < n)
+    confinement_body(i, pred, se, vort, vel, velOut, c);
+}
+
+// fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
+__device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__ pred,
+                                          const uint2* __restrict__ se,
+                                          const float4* __restrict__ velIn,
+                                          float4* __restrict__ velOut, float4* __restrict__ posOut,
+                                          const PbfConsts& c)
+{
   const float4 p = pred[i];
   const float4 vi = velIn[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -477,6 +523,145 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
   const float k = c.p.xsphViscosityCoeff;
   velOut[i] = make_float4(vi.x + k * sx, vi.y + k * sy, vi.z + k * sz, 0.0f);
   posOut[i] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
+                                              const uint2* __restrict__ se,
+                                              const float4* __restrict__ velIn,
+                                              float4* __restrict__ velOut,
+                                              float4* __restrict__ posOut, uint32_t n,
+                                              PbfConsts c, int remap)
+{
+  const uint32_t i = particle_index(remap);
+  if (i < n)
+    xsph_body(i, pred, se, velIn, velOut, posOut, c);
+}
+
+// ------------------------------------------------------------------- batched wrappers
+// One launch covers every scene of a batch: block b works on scene blocks[b].x, particles
+// blocks[b].y + threadIdx.x of it.  All particle pointers are shifted to the scene's first
+// particle and the cell table to its key base, so each body runs exactly as in a model of
+// its own (same indices, same summation order).
+
+struct BatchSite {
+  uint32_t i;       // scene-local particle
+  uint32_t scene;
+  SceneSlot slot;
+};
+
+__device__ __forceinline__ bool batch_site(const BatchView& bv, BatchSite& s)
+{
+  const uint2 blk = bv.blocks[blockIdx.x];
+  s.scene = blk.x;
+  s.slot = bv.scenes[blk.x];
+  s.i = blk.y + threadIdx.x;
+  return s.i < s.slot.n;
+}
+
+__global__ __launch_bounds__(256) void k_fill_start_end(uint2* __restrict__ se, uint32_t count)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count)
+    se[t] = make_uint2(1u, 0u);  // grid.cl:91-96
+}
+
+__global__ __launch_bounds__(256) void k_predict_batch(BatchView bv, const float4* __restrict__ pos,
+                                                       const float4* __restrict__ vel,
+                                                       float4* __restrict__ pred,
+                                                       uint32_t* __restrict__ key)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  predict_body(b.i, pos + o, vel + o, pred + o, key + o, b.slot.kbase, c);
+}
+
+__global__ __launch_bounds__(256) void k_density_lambda_batch(BatchView bv,
+                                                              const float4* __restrict__ pred,
+                                                              const uint2* __restrict__ se,
+                                                              float* __restrict__ density,
+                                                              float* __restrict__ lambda)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  density_lambda_body<0>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs());
+}
+
+__global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float4* __restrict__ pred,
+                                                       const float* __restrict__ lambda,
+                                                       const uint2* __restrict__ se,
+                                                       float4* __restrict__ predOut,
+                                                       const float4* __restrict__ pos,
+                                                       float4* __restrict__ vel,
+                                                       float4* __restrict__ posOut, int last)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
+                  posOut ? posOut + o : nullptr, last, c, LearnArgs());
+}
+
+__global__ __launch_bounds__(256) void k_update_vel_batch(BatchView bv, const float4* __restrict__ pred,
+                                                          const float4* __restrict__ pos,
+                                                          float4* __restrict__ vel,
+                                                          float4* __restrict__ posOut)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  update_vel_body(b.i, pred + o, pos + o, vel + o, posOut ? posOut + o : nullptr, c);
+}
+
+__global__ __launch_bounds__(256) void k_vorticity_batch(BatchView bv, const float4* __restrict__ pred,
+                                                         const uint2* __restrict__ se,
+                                                         const float4* __restrict__ vel,
+                                                         float4* __restrict__ vort)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  vorticity_body(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c);
+}
+
+__global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
+                                                           const float4* __restrict__ pred,
+                                                           const uint2* __restrict__ se,
+                                                           const float4* __restrict__ vort,
+                                                           const float4* __restrict__ vel,
+                                                           float4* __restrict__ velOut)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  confinement_body(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c);
+}
+
+__global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* __restrict__ pred,
+                                                    const uint2* __restrict__ se,
+                                                    const float4* __restrict__ velIn,
+                                                    float4* __restrict__ velOut,
+                                                    float4* __restrict__ posOut)
+{
+  BatchSite b;
+  if (!batch_site(bv, b))
+    return;
+  const PbfConsts c = bv.consts[b.scene];
+  const uint32_t o = b.slot.pbase;
+  xsph_body(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c);
 }
 
 __global__ __launch_bounds__(64) void k_zero3(float* __restrict__ out)
@@ -664,5 +849,66 @@ void launch_density_error(const float* density, uint32_t n, float rho0, float* m
   hipLaunchKernelGGL(k_density_error, dim3(blocks), dim3(256), 0, s, density, n, rho0, partial);
   hipLaunchKernelGGL(k_density_error_final, dim3(1), dim3(256), 0, s, partial, blocks, metric);
 }
+
+
+// ------------------------------------------------------------------- batched launchers
+
+void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s)
+{
+  if (count)
+    hipLaunchKernelGGL(k_fill_start_end, dim3(blocks_for(count, 256)), dim3(256), 0, s, startEnd, count);
+}
+
+#define RTP_BATCH_LAUNCH(kernel, ...)                                                          \
+  do {                                                                                       \
+    if (bv.numBlocks)                                                                        \
+      hipLaunchKernelGGL(kernel, dim3(bv.numBlocks), dim3(bv.block), 0, s, bv, __VA_ARGS__); \
+  } while (0)
+
+void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,
+                              float4* pred, uint32_t* key, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_predict_batch, pos, vel, pred, key);
+}
+
+void launch_pbf_density_lambda_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                     float* density, float* lambda, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_density_lambda_batch, pred, startEnd, density, lambda);
+}
+
+void launch_pbf_correct_batch(const BatchView& bv, const float4* pred, const float* lambda,
+                              const uint2* startEnd, float4* predOut, const float4* pos,
+                              float4* vel, float4* posOut, int last, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_correct_batch, pred, lambda, startEnd, predOut, pos, vel, posOut, last);
+}
+
+void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const float4* pos,
+                                 float4* vel, float4* posOut, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_update_vel_batch, pred, pos, vel, posOut);
+}
+
+void launch_pbf_vorticity_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                const float4* vel, float4* vort, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_vorticity_batch, pred, startEnd, vel, vort);
+}
+
+void launch_pbf_confinement_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                  const float4* vort, const float4* vel, float4* velOut,
+                                  hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_confinement_batch, pred, startEnd, vort, vel, velOut);
+}
+
+void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s)
+{
+  RTP_BATCH_LAUNCH(k_xsph_batch, pred, startEnd, velIn, velOut, posOut);
+}
+
+#undef RTP_BATCH_LAUNCH
 
 }  // namespace rtp

@@ -1,0 +1,261 @@
+// Host-side helpers shared by the model handles of the C ABI: constant folding as the
+// reference's host does it, grid set-up, device buffers and per-launch HIP-event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rtp_internal.hpp"
+
+#define RTP_REQUIRE(cond, msg)                    \
+  do {                                            \
+    if (!(cond))                                  \
+      return ::rtp::fail(RTP_ERR_INVALID, msg);   \
+  } while (0)
+
+namespace rtp {
+namespace {
+
+// ------------------------------------------------------------------------- host constants
+
+// utils/Utils.cpp:24-29 FloatToStr(v, 10) followed by the OpenCL compiler's parse.
+float float_roundtrip(float v)
+{
+  char buf[128];
+  std::snprintf(buf, sizeof(buf), "%.10f", (double)v);
+  return std::strtof(buf, nullptr);
+}
+
+constexpr float kPiF = 3.1415927f;  // utils/Math.hpp:69
+
+int make_grid(const rtp_model_desc* d, uint32_t defaultCap, GridConsts* g)
+{
+  for (int a = 0; a < 3; ++a) {
+    if (!(d->boxSize[a] > 0.0f) || d->gridRes[a] <= 0)
+      return fail(RTP_ERR_INVALID, "boxSize and gridRes must be positive");
+    g->absWall[a] = float_roundtrip(d->boxSize[a] / 2.0f);  // Fluids.cpp:108-110
+    g->res[a] = d->gridRes[a];
+  }
+  const uint64_t nc = (uint64_t)d->gridRes[0] * d->gridRes[1] * d->gridRes[2];
+  if (nc == 0 || nc > (1u << 30))
+    return fail(RTP_ERR_INVALID, "grid has too many cells");
+  g->cellSize = float_roundtrip(d->boxSize[0] / (float)d->gridRes[0]);  // Fluids.cpp:114
+  g->numCells = (uint32_t)nc;
+  g->cap = d->maxPartsInCell ? d->maxPartsInCell : defaultCap;
+  return RTP_OK;
+}
+
+// Largest cell key fillCellIDs can produce (each 3-D index is at most res when a position
+// sits exactly on the +wall) — the sort only needs that many bits.
+uint32_t max_active_key(const GridConsts& g)
+{
+  const uint64_t ry = (uint64_t)g.res[1], rz = (uint64_t)g.res[2];
+  const uint64_t m = (uint64_t)g.res[0] * rz * ry + ry * rz + rz;
+  return (uint32_t)std::min<uint64_t>(m, 0xffffffffu);
+}
+
+// Smallest float that is >= h*h as a real number.
+float cull_bound(float h)
+{
+  const double hh = (double)h * (double)h;
+  float t = (float)hh;
+  if ((double)t < hh)
+    t = std::nextafter(t, INFINITY);
+  return t;
+}
+
+// --------------------------------------------------------------------------------- buffers
+
+template <class T>
+int dev_alloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0)
+    count = 1;
+  RTP_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count));
+  return RTP_OK;
+}
+
+template <class T>
+void dev_free(T*& p)
+{
+  if (p)
+    (void)hipFree(p);
+  p = nullptr;
+}
+
+// ------------------------------------------------------------------------------- profiling
+
+struct Profiler {
+  bool enabled = false;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<std::string> names;
+  std::vector<uint32_t> launches;
+  std::vector<double> totals;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  int id_of(const char* name)
+  {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name)
+        return (int)i;
+    names.emplace_back(name);
+    launches.push_back(0);
+    totals.push_back(0.0);
+    return (int)names.size() - 1;
+  }
+  hipEvent_t take()
+  {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  // Wrap one launch.  Call begin() before and end() after enqueueing it.
+  int beginId = -1;
+  hipEvent_t beginEv = nullptr;
+  void begin(const char* name, hipStream_t s)
+  {
+    if (!enabled)
+      return;
+    beginId = id_of(name);
+    beginEv = take();
+    (void)hipEventRecord(beginEv, s);
+  }
+  void end(hipStream_t s)
+  {
+    if (!enabled || beginId < 0)
+      return;
+    hipEvent_t e = take();
+    (void)hipEventRecord(e, s);
+    pending.push_back(Pending{ beginId, beginEv, e });
+    beginId = -1;
+  }
+  int collect()
+  {
+    for (auto& p : pending) {
+      RTP_HIP_TRY(hipEventSynchronize(p.b));
+      float ms = 0.0f;
+      RTP_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+      launches[p.id] += 1;
+      totals[p.id] += ms;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+    return RTP_OK;
+  }
+  void reset()
+  {
+    (void)collect();
+    for (auto& l : launches)
+      l = 0;
+    for (auto& t : totals)
+      t = 0.0;
+  }
+  int report(rtp_kernel_time* out, int capacity, int* count)
+  {
+    const int rc = collect();
+    if (rc)
+      return rc;
+    const int n = (int)names.size();
+    if (count)
+      *count = n;
+    for (int i = 0; i < n && i < capacity && out; ++i) {
+      std::memset(out[i].name, 0, sizeof(out[i].name));
+      std::strncpy(out[i].name, names[i].c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = launches[i];
+      out[i].totalMs = totals[i];
+    }
+    return RTP_OK;
+  }
+  void destroy()
+  {
+    (void)collect();
+    for (auto e : pool)
+      (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+#define RTP_PROF(prof, name, stream, launch) \
+  do {                                       \
+    (prof).begin(name, stream);              \
+    launch;                                  \
+    (prof).end(stream);                      \
+  } while (0)
+
+int check_device(int device)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(RTP_ERR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count)
+    return fail(RTP_ERR_INVALID, "device ordinal out of range");
+  RTP_HIP_TRY(hipSetDevice(device));
+  return RTP_OK;
+}
+
+int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, float4 fill,
+              hipStream_t s)
+{
+  std::vector<float4> h(maxN, fill);
+  for (uint32_t i = 0; i < count; ++i)
+    h[i] = make_float4(src[4 * i + 0], src[4 * i + 1], src[4 * i + 2], 0.0f);
+  RTP_HIP_TRY(hipMemcpyAsync(dst, h.data(), sizeof(float4) * maxN, hipMemcpyHostToDevice, s));
+  RTP_HIP_TRY(hipStreamSynchronize(s));
+  return RTP_OK;
+}
+
+// Fluids::createProgram constants (Fluids.cpp:97-126) of a scene with box edge boxX split
+// into resX cells; the rest of PbfConsts follows from the parameters (refresh_pbf_consts).
+void init_pbf_consts(PbfConsts& c, const GridConsts& g, float boxX, int32_t resX, float gravity)
+{
+  c.g = g;
+  const float effectRadius = boxX / (float)resX;                                      // :104
+  c.h = float_roundtrip(effectRadius);
+  c.poly6 = float_roundtrip(315.0f / (64.0f * kPiF * std::pow(effectRadius, 9.f)));  // :117
+  c.spiky = float_roundtrip(15.0f / (kPiF * std::pow(effectRadius, 6.f)));           // :118
+  c.maxVel = float_roundtrip(30.0f);                                                  // :119
+  c.gravity = gravity > 0.0f ? gravity : 9.81f;                                       // define.cl:8
+  c.p = rtp_fluid_params{ 450.0f, 600.0f, 0.010f, 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
+}
+
+// The host-side constant folding of Fluids::createProgram (Fluids.cpp:97-126) and of the
+// kernels' constant subexpressions.
+void refresh_pbf_consts(PbfConsts& c)
+{
+  const float h = c.h;
+  c.hh = h * h;
+  c.cullSq = cull_bound(h);
+  const rtp_fluid_params& p = c.p;
+  const float dq = p.artPressureRadius * h;
+  const float t = dq < h ? 1.0f : 0.0f;  // 1 - step(h, dq)
+  const float d = h * h - dq * dq;
+  c.wdq = (t * c.poly6) * ((d * d) * d);
+  c.rho0sq = p.restDensity * p.restDensity;
+  c.dtEps = p.timeStep + 0.00000001f;
+  c.gdt = -c.gravity * p.timeStep;
+  for (int a = 0; a < 3; ++a) {
+    c.lo[a] = -c.g.absWall[a] + 0.01f;
+    c.hi[a] = c.g.absWall[a] - 0.1f;
+  }
+}
+
+}  // namespace
+}  // namespace rtp

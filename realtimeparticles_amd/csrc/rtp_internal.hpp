@@ -119,6 +119,43 @@ struct LearnArgs {
 enum PassMode { kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16 };
 enum LastMode { kNextIter = 0, kLastVel = 1, kLastOnly = 2 };
 
+// ------------------------------------------------------------------------------ batches
+// Many independent PBF scenes advanced by the same launches.  Scene s owns particles
+// [pbase, pbase + n) of every particle array and the key range [kbase, kbase + keySpan) of
+// the cell-key space and of the cell table (keySpan = the largest key its hash can produce
+// + 1), so one global sort of the keys leaves every scene's particles in its own range, in
+// its own order.  Cell tables and neighbour indices inside a scene are scene-local, as in a
+// model of its own.
+struct SceneSlot {
+  uint32_t pbase, n, kbase, keySpan;
+};
+// blocks[b] = (scene, first scene-local particle) of block b of a batched launch.
+struct BatchView {
+  const SceneSlot* scenes;
+  const uint2* blocks;
+  const PbfConsts* consts;
+  uint32_t numBlocks, block;
+};
+void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s);
+void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,
+                              float4* pred, uint32_t* key, hipStream_t s);
+// ReorderArgs with whole-batch pointers; per-scene counts, bounds and grid come from bv.
+void launch_reorder_batch(const BatchView& bv, const ReorderArgs& a, hipStream_t s);
+void launch_pbf_density_lambda_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                     float* density, float* lambda, hipStream_t s);
+void launch_pbf_correct_batch(const BatchView& bv, const float4* pred, const float* lambda,
+                              const uint2* startEnd, float4* predOut, const float4* pos,
+                              float4* vel, float4* posOut, int last, hipStream_t s);
+void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const float4* pos,
+                                 float4* vel, float4* posOut, hipStream_t s);
+void launch_pbf_vorticity_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                const float4* vel, float4* vort, hipStream_t s);
+void launch_pbf_confinement_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                                  const float4* vort, const float4* vel, float4* velOut,
+                                  hipStream_t s);
+void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
+                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s);
+
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
                                hipStream_t s, int mode = 0, const LearnArgs& la = LearnArgs());

@@ -1,0 +1,176 @@
+"""Summarise rocprofv3 CSV output of bench.py runs into a short markdown table and a JSON
+file of HBM traffic per kernel launch.
+
+Inputs are the output directories of up to four runs of the same command:
+
+  --trace DIR   rocprofv3 --kernel-trace --stats           (run_kernel_stats.csv)
+  --fetch DIR   rocprofv3 --pmc FETCH_SIZE                 (run_counter_collection.csv)
+  --write DIR   rocprofv3 --pmc WRITE_SIZE
+  --sq DIR      rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES ...
+
+FETCH_SIZE and WRITE_SIZE are KiB per dispatch.  Each is scaled by what the same counter
+reads for the runtime's device-to-device copy kernel (__amd_rocclr_copyBuffer), whose
+bytes are known: the checkpoint restore in front of every fixed-input step copies
+positions, velocities and predicted positions (16 B each) and the cell keys (4 B) of
+max_n particles, i.e. 52 * max_n bytes read and written in four dispatches.
+
+    python -m realtimeparticles_amd.profile_summary --trace D1 --fetch D2 --write D3 \\
+        --max-n 1049088 --command "..." --md out.md --json out.json
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import os
+import re
+from collections import defaultdict
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def short_name(name: str) -> str:
+    """'void rtp::(anonymous namespace)::k_correct<2>(float4 const*, ...)' -> 'k_correct<2>'."""
+    s = name.strip('"')
+    depth, cut = 0, len(s)
+    for i, ch in enumerate(s):  # first '(' outside template brackets, after the name
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0 and i > 0 and s[i - 1] != ":":
+            cut = i
+            break
+    s = s[:cut]
+    s = re.sub(r"^void\s+", "", s)
+    s = s.replace("(anonymous namespace)::", "")
+    s = re.sub(r"^rtp::", "", s)
+    return s
+
+
+def base_name(name: str) -> str:
+    return short_name(name).split("<")[0]
+
+
+def read_csv(path: str):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def find(dirpath: str, suffix: str) -> str:
+    for root, _, files in os.walk(dirpath):
+        for fn in files:
+            if fn.endswith(suffix):
+                return os.path.join(root, fn)
+    raise FileNotFoundError(f"no *{suffix} under {dirpath}")
+
+
+def kernel_stats(trace_dir: str):
+    rows = read_csv(find(trace_dir, "kernel_stats.csv"))
+    out = {}
+    for r in rows:
+        out[short_name(r["Name"])] = {"calls": int(r["Calls"]), "total_ns": float(r["TotalDurationNs"]),
+                                      "avg_ns": float(r["AverageNs"]), "min_ns": float(r["MinNs"]),
+                                      "max_ns": float(r["MaxNs"]), "percent": float(r["Percentage"])}
+    return out
+
+
+def counters(pmc_dir: str, counter: str):
+    """{short kernel name: [value per dispatch]} and the dispatch metadata."""
+    rows = read_csv(find(pmc_dir, "counter_collection.csv"))
+    vals = defaultdict(list)
+    meta = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        k = short_name(r["Kernel_Name"])
+        vals[k].append(float(r["Counter_Value"]))
+        meta[k] = {"vgpr": int(r["VGPR_Count"]), "sgpr": int(r["SGPR_Count"]),
+                   "lds": int(r["LDS_Block_Size"]), "workgroup": int(r["Workgroup_Size"])}
+    return vals, meta
+
+
+def copy_factor(vals, max_n: int) -> float | None:
+    """Counter bytes per real byte for the copy kernel (four dispatches per restore)."""
+    v = vals.get(COPY_KERNEL)
+    if not v or len(v) % 4:
+        return None
+    restores = len(v) // 4
+    return (sum(v) * 1024.0) / (restores * 52.0 * max_n)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--sq")
+    ap.add_argument("--max-n", type=int, required=True)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--title", default="rocprofv3 summary")
+    ap.add_argument("--md", required=True)
+    ap.add_argument("--json")
+    a = ap.parse_args(argv)
+
+    stats = kernel_stats(a.trace) if a.trace else {}
+    fetch, meta = counters(a.fetch, "FETCH_SIZE") if a.fetch else ({}, {})
+    write, meta_w = counters(a.write, "WRITE_SIZE") if a.write else ({}, {})
+    meta = meta or meta_w
+    ff = copy_factor(fetch, a.max_n) if fetch else None
+    fw = copy_factor(write, a.max_n) if write else None
+    sq = {}
+    if a.sq:
+        rows = read_csv(find(a.sq, "counter_collection.csv"))
+        acc = defaultdict(lambda: defaultdict(float))
+        cnt = defaultdict(lambda: defaultdict(int))
+        for r in rows:
+            k = short_name(r["Kernel_Name"])
+            acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            cnt[k][r["Counter_Name"]] += 1
+        for k, d in acc.items():
+            sq[k] = {c: d[c] / cnt[k][c] for c in d}
+
+    names = sorted(set(stats) | set(fetch) | set(write),
+                   key=lambda k: -stats.get(k, {}).get("total_ns", 0.0))
+    traffic = {}
+    lines = [f"# {a.title}", "", "Commands (run from the repository root on one MI355X; "
+             "`cd /tmp && export TMPDIR=/tmp` first for rocprofv3):", "", "```", a.command.strip(), "```", ""]
+    if ff is not None or fw is not None:
+        lines += [f"Copy-kernel scale (counter bytes per real byte, {COPY_KERNEL} of the checkpoint "
+                  f"restores, 52 B x {a.max_n} per restore): FETCH_SIZE {ff if ff is None else round(ff, 3)}, "
+                  f"WRITE_SIZE {fw if fw is None else round(fw, 3)}.", ""]
+    lines += ["| kernel | calls | avg us | total ms | % | FETCH MB/launch (scaled) | WRITE MB/launch (scaled) | VGPR | LDS B |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for k in names:
+        st = stats.get(k, {})
+        fv, wv = fetch.get(k), write.get(k)
+        fb = (sum(fv) / len(fv) * 1024.0 / ff) if fv and ff else None
+        wb = (sum(wv) / len(wv) * 1024.0 / fw) if wv and fw else None
+        m = meta.get(k, {})
+        if fb is not None or wb is not None:
+            traffic[k] = {"fetch_bytes": fb, "write_bytes": wb,
+                          "traffic_bytes": (fb or 0.0) + (wb or 0.0), "base": base_name(k),
+                          "avg_ns": st.get("avg_ns")}
+        lines.append("| {} | {} | {} | {} | {} | {} | {} | {} | {} |".format(
+            k, st.get("calls", ""), "" if not st else f"{st['avg_ns'] / 1e3:.2f}",
+            "" if not st else f"{st['total_ns'] / 1e6:.3f}", "" if not st else f"{st['percent']:.1f}",
+            "" if fb is None else f"{fb / 1e6:.2f}", "" if wb is None else f"{wb / 1e6:.2f}",
+            m.get("vgpr", ""), m.get("lds", "")))
+    if sq:
+        lines += ["", "SQ counters (mean per dispatch):", "",
+                  "| kernel | " + " | ".join(sorted(next(iter(sq.values())).keys())) + " |",
+                  "|---|" + "---|" * len(next(iter(sq.values())))]
+        for k in names:
+            if k in sq:
+                lines.append(f"| {k} | " + " | ".join(f"{sq[k][c]:.4g}" for c in sorted(sq[k])) + " |")
+    os.makedirs(os.path.dirname(os.path.abspath(a.md)), exist_ok=True)
+    with open(a.md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump({"command": a.command.strip(), "max_n": a.max_n, "fetch_scale": ff,
+                       "write_scale": fw, "kernels": traffic}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

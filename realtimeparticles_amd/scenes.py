@@ -127,6 +127,33 @@ def pbf_scene(name: str, seed: int = 1) -> FluidScene:
     return dam_break(lattice, box, res, seed=seed, name=name)
 
 
+def batch_scenes(count: int = 256, seed: int = 11, n_lo: int = 2048, n_hi: int = 65536,
+                 h: float = 0.1) -> list[FluidScene]:
+    """Independent dam breaks of uneven sizes for the batched solver.
+
+    Per scene k (draws k*8 .. k*8+3 of the SplitMix64 stream of `seed`):
+      target count N = exp(U[ln n_lo, ln n_hi]) (log-uniform),
+      box aspect ratios x:z and y:z in U[1, 4], |gravity| in U[5, 15];
+    the lattice (spacing h/2) is N^(1/3)-scaled to those ratios (each side at least 4),
+    the box is the smallest whole number of h-cells per axis that the lattice fills to at
+    most 80 %, and the jitter stream is seed * 1000003 + k.  Every particle is active.
+    """
+    u = uniform24(seed, 0, 8 * count).reshape(count, 8).astype(np.float64)
+    out = []
+    for k in range(count):
+        target = math.exp(math.log(n_lo) + u[k, 0] * (math.log(n_hi) - math.log(n_lo)))
+        ax, ay = 1.0 + 3.0 * u[k, 1], 1.0 + 3.0 * u[k, 2]
+        g = 5.0 + 10.0 * u[k, 3]
+        lz = max(4, int(round((target / (ax * ay)) ** (1.0 / 3.0))))
+        lat = (max(4, int(round(ax * lz))), max(4, int(round(ay * lz))), lz)
+        res = tuple(max(4, int(math.ceil(l * 0.5 / 0.8))) for l in lat)
+        box = tuple(float(np.float32(r * h)) for r in res)
+        sc = dam_break(lat, box, res, seed=seed * 1000003 + k, name=f"batch-{k}", extra=0)
+        sc.gravity = float(np.float32(g))
+        out.append(sc)
+    return out
+
+
 @dataclasses.dataclass
 class BoidsScene:
     name: str

@@ -91,3 +91,39 @@ def oracle_grid(pos: np.ndarray, max_n: int, box, res, cap: int):
     L.orc_fill_end(key.ctypes.data, se.ctypes.data, n, max_n, ctypes.byref(c))
     L.orc_adjust_end(se.ctypes.data, ctypes.byref(c))
     return key, idx, se
+
+
+def oracle_teacher(sc, prm, iters, return_grid=False):
+    """Features and summed lambda of one teacher substep, restated step by step on the CPU
+    (and the substep's grid: keys, sort index, start/end, with return_grid)."""
+    import ctypes
+    L = O.lib()
+    f = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), iters, gravity=sc.gravity)
+    f.upload(sc.pos, sc.vel)
+    c, p, n, pt = ctypes.byref(f.c), ctypes.byref(f.p), sc.n, O.ptr
+    L.orc_fld_predict(pt(f.pos), pt(f.vel), p, pt(f.pred), n, c)
+    L.orc_fill_cell_ids(pt(f.pred), pt(f.key), n, c)
+    assert L.orc_radix_sort(pt(f.key), pt(f.idx), f.max_n) == 0
+    for b in (f.pos, f.vel, f.pred):
+        L.orc_permute_f4(pt(f.idx), pt(b), pt(f.tmp), f.max_n)
+    L.orc_reset_start_end(pt(f.se), c)
+    L.orc_fill_start(pt(f.key), pt(f.se), n, c)
+    L.orc_fill_end(pt(f.key), pt(f.se), n, f.max_n, c)
+    L.orc_adjust_end(pt(f.se), c)
+    feat = np.zeros((n, 8), np.float32)
+    acc = None
+    for k in range(iters):
+        L.orc_fld_boundary(pt(f.pred), n, c)
+        L.orc_fld_density(pt(f.pred), pt(f.se), p, pt(f.density), n, c)
+        L.orc_fld_lambda(pt(f.pred), pt(f.density), pt(f.se), p, pt(f.lam), n, c)
+        if k == 0:
+            L.orc_fld_features(pt(f.pred), pt(f.density), pt(f.se), pt(f.lam), pt(f.vel), p,
+                               pt(feat), n, c)
+            acc = f.lam[:n].copy()
+        else:
+            acc = (acc + f.lam[:n]).astype(np.float32)
+        L.orc_fld_correction(pt(f.lam), pt(f.se), pt(f.pred), p, pt(f.corr), n, c)
+        L.orc_fld_correct_pos(pt(f.corr), pt(f.pred), n)
+    if return_grid:
+        return feat, acc, (f.key.copy(), f.idx.copy(), f.se.copy())
+    return feat, acc

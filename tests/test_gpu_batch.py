@@ -1,0 +1,175 @@
+"""GPU tests of the batched PBF solver (rtp_batch_*).
+
+Every scene of a batch must evolve exactly as a model of its own with all particles active
+(rtp_fluids with maxNbParticles == nbParticles), which the single-model tests in turn tie to
+the oracle bit for bit.  Compared per scene after every substep: positions, velocities,
+density, lambda, predicted positions, cell IDs, sort index and the cell table.
+"""
+import numpy as np
+import pytest
+
+from helpers import _eq, compare_pbf, log, make_pbf_pair
+from realtimeparticles_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_scene(b: capi.Batch, k: int, m: capi.Fluids, what: str) -> None:
+    bp, bv = b.read_state(k)
+    mp, mv = m.read_state()
+    _eq(bp[:, :3], mp[:, :3], f"{what} pos")
+    _eq(bv[:, :3], mv[:, :3], f"{what} vel")
+    _eq(b.read_field(k, capi.FIELD_DENSITY), m.read_field(capi.FIELD_DENSITY), f"{what} density")
+    _eq(b.read_field(k, capi.FIELD_LAMBDA), m.read_field(capi.FIELD_LAMBDA), f"{what} lambda")
+    _eq(b.read_field(k, capi.FIELD_PRED_POS), m.read_field(capi.FIELD_PRED_POS), f"{what} predPos")
+    bk, bi, bse = b.read_grid(k)
+    mk, mi, mse = m.read_grid()
+    _eq(bk, mk, f"{what} cell IDs")
+    _eq(bi, mi, f"{what} sort index")
+    _eq(bse, mse, f"{what} start/end")
+
+
+def scene_params(k: int, mapping: str, sc: scenes.FluidScene) -> dict:
+    p = sc.params(mapping)
+    # per-scene variation of what the batch keeps per scene
+    p["artPressureCoeff"] = float(np.float32(p["artPressureCoeff"] * (1.0 + 0.1 * (k % 3))))
+    p["isArtPressureEnabled"] = int(k % 4 != 3)
+    return p
+
+
+def separate_models(scs, iters, mapping):
+    out = []
+    for k, sc in enumerate(scs):
+        m = capi.Fluids(sc.n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+        m.set_params(**scene_params(k, mapping, sc))
+        m.set_nb_jacobi_iters(iters)
+        m.load_state(sc.pos, sc.vel)
+        out.append(m)
+    return out
+
+
+def make_batch(scs, iters, mapping):
+    b = capi.Batch(scs)
+    for k, sc in enumerate(scs):
+        b.set_params(k, **scene_params(k, mapping, sc))
+        b.load_state(k, sc.pos, sc.vel)
+    b.set_nb_jacobi_iters(iters)
+    return b
+
+
+@pytest.mark.parametrize("iters,mapping", [(3, "stable"), (2, "literal"), (0, "stable")])
+def test_batch_scenes_equal_separate_models(iters, mapping):
+    scs = scenes.batch_scenes(count=10, seed=5, n_lo=300, n_hi=9000)
+    b = make_batch(scs, iters, mapping)
+    ms = separate_models(scs, iters, mapping)
+    for step in range(1, 4):
+        log(f"batch iters={iters} {mapping} step {step}")
+        b.update(1)
+        for k, m in enumerate(ms):
+            m.update(1)
+            compare_scene(b, k, m, f"scene {k} step {step}")
+    b.close()
+    for m in ms:
+        m.close()
+
+
+def test_batch_scene_against_oracle():
+    sc = scenes.batch_scenes(count=3, seed=9, n_lo=2000, n_hi=4000)[2]
+    b = capi.Batch([sc])
+    b.set_params(0, **sc.params())
+    b.set_nb_jacobi_iters(4)
+    b.load_state(0, sc.pos, sc.vel)
+    # the oracle sorts whole 512-key groups (the reference's requirement); a model whose
+    # maxN exceeds n evolves exactly as one with maxN == n
+    sc.max_n = (sc.n + 511) // 512 * 512
+    _, orc = make_pbf_pair(sc, iters=4)
+    for step in range(1, 4):
+        b.update(1)
+        orc.step(1)
+        pos, vel = b.read_state(0)
+        _eq(pos[:, :3], orc.pos[:sc.n, :3], f"pos step {step}")
+        _eq(vel[:, :3], orc.vel[:sc.n, :3], f"vel step {step}")
+        _eq(b.read_field(0, capi.FIELD_DENSITY), orc.density[:sc.n], f"density step {step}")
+        k, i, se = b.read_grid(0)
+        _eq(k, orc.key[:sc.n], f"keys step {step}")
+        _eq(i, orc.idx[:sc.n], f"idx step {step}")
+        _eq(se, orc.se, f"start/end step {step}")
+    b.close()
+
+
+def test_batch_graph_replay_and_tuning_do_not_change_bits():
+    scs = scenes.batch_scenes(count=24, seed=3, n_lo=500, n_hi=6000)
+    b = make_batch(scs, 3, "stable")
+    b.save_checkpoint()
+    log("graph replay of 4 substeps")
+    b.update(4)
+    ref = [b.read_state(k) for k in range(len(scs))]
+    for block, bits in ((64, 7), (192, 11), (128, 0)):
+        b.restore_checkpoint()
+        b.set_tuning(block, bits)
+        b.use_graph(False)
+        log(f"direct launches, block {block}, sort bits {bits}")
+        for _ in range(4):
+            b.update(1)
+        for k in range(len(scs)):
+            p, v = b.read_state(k)
+            _eq(p, ref[k][0], f"scene {k} pos (block {block}, bits {bits})")
+            _eq(v, ref[k][1], f"scene {k} vel (block {block}, bits {bits})")
+    b.use_graph(True)
+    b.restore_checkpoint()
+    best = b.autotune(trial_steps=2)
+    log(f"autotuned: {best}")
+    assert best["block"] in (64, 128, 192, 256) and 7 <= best["sort_bits"] <= 11
+    b.update(4)
+    for k in range(len(scs)):
+        _eq(b.read_state(k)[0], ref[k][0], f"scene {k} pos after autotune")
+    b.close()
+
+
+def test_batch_256_scenes_sample_equals_separate_models():
+    scs = scenes.batch_scenes()  # 256 scenes, 2,048..65,536 particles each
+    b = make_batch(scs, 4, "literal")
+    total, cells = b.totals()
+    log(f"256 scenes: {total} particles, {cells} cell keys")
+    assert total == sum(sc.n for sc in scs)
+    b.update(2)
+    pick = [0, 77, 128, 255]
+    for k in pick:
+        sc = scs[k]
+        m = capi.Fluids(sc.n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+        m.set_params(**scene_params(k, "literal", sc))
+        m.set_nb_jacobi_iters(4)
+        m.load_state(sc.pos, sc.vel)
+        m.update(2)
+        compare_scene(b, k, m, f"scene {k} ({sc.n} particles)")
+        m.close()
+    b.close()
+
+
+def test_fluids_autotune_keeps_results():
+    sc = scenes.dam_break((32, 16, 16), (4, 2, 2), (40, 20, 20), seed=4)
+    gpu, orc = make_pbf_pair(sc, iters=3)
+    best = gpu.autotune(trial_steps=2)
+    log(f"fluids autotune: {best}")
+    assert best["ms_per_step"] > 0
+    for step in range(1, 3):
+        gpu.update(1)
+        orc.step(1)
+        compare_pbf(gpu, orc, sc.n, step)
+    gpu.close()
+
+
+def test_batch_argument_errors():
+    scs = scenes.batch_scenes(count=2, seed=1, n_lo=300, n_hi=600)
+    b = capi.Batch(scs)
+    with pytest.raises(capi.RtpError):
+        b.read_state(2)
+    with pytest.raises(capi.RtpError):
+        b.read_grid(0)  # before the first substep
+    with pytest.raises(capi.RtpError):
+        b.set_params(1, isVorticityConfEnabled=0)  # must agree across scenes
+    with pytest.raises(capi.RtpError):
+        b.set_tuning(100, 0)
+    with pytest.raises(capi.RtpError):
+        b.restore_checkpoint()
+    b.close()

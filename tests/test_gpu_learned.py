@@ -6,14 +6,11 @@
 * the fallback gate, and what the trained network gains over the same iteration budget
   without it.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
-from helpers import _eq, log
+from helpers import _eq, log, oracle_teacher
 from oracle import mlp_oracle as MO
-from oracle import oracle as O
 from realtimeparticles_amd import capi, learned, scenes
 
 pytestmark = pytest.mark.gpu
@@ -79,38 +76,6 @@ def test_mlp_training_reduces_loss():
     mlp.close()
 
 
-def oracle_teacher(sc, prm, iters):
-    """Features and summed lambda of one teacher substep, restated step by step on the CPU."""
-    L = O.lib()
-    f = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), iters, gravity=sc.gravity)
-    f.upload(sc.pos, sc.vel)
-    c, p, n, pt = ctypes.byref(f.c), ctypes.byref(f.p), sc.n, O.ptr
-    L.orc_fld_predict(pt(f.pos), pt(f.vel), p, pt(f.pred), n, c)
-    L.orc_fill_cell_ids(pt(f.pred), pt(f.key), n, c)
-    assert L.orc_radix_sort(pt(f.key), pt(f.idx), f.max_n) == 0
-    for b in (f.pos, f.vel, f.pred):
-        L.orc_permute_f4(pt(f.idx), pt(b), pt(f.tmp), f.max_n)
-    L.orc_reset_start_end(pt(f.se), c)
-    L.orc_fill_start(pt(f.key), pt(f.se), n, c)
-    L.orc_fill_end(pt(f.key), pt(f.se), n, f.max_n, c)
-    L.orc_adjust_end(pt(f.se), c)
-    feat = np.zeros((n, 8), np.float32)
-    acc = None
-    for k in range(iters):
-        L.orc_fld_boundary(pt(f.pred), n, c)
-        L.orc_fld_density(pt(f.pred), pt(f.se), p, pt(f.density), n, c)
-        L.orc_fld_lambda(pt(f.pred), pt(f.density), pt(f.se), p, pt(f.lam), n, c)
-        if k == 0:
-            L.orc_fld_features(pt(f.pred), pt(f.density), pt(f.se), pt(f.lam), pt(f.vel), p,
-                               pt(feat), n, c)
-            acc = f.lam[:n].copy()
-        else:
-            acc = (acc + f.lam[:n]).astype(np.float32)
-        L.orc_fld_correction(pt(f.lam), pt(f.se), pt(f.pred), p, pt(f.corr), n, c)
-        L.orc_fld_correct_pos(pt(f.corr), pt(f.pred), n)
-    return feat, acc
-
-
 def small_scene(seed):
     return scenes.dam_break((24, 16, 16), (4, 2, 2), (40, 20, 20), seed=seed, name="pbf-small")
 
@@ -137,26 +102,32 @@ def test_fallback_gate():
     mlp.set_params(p * 0.1)
     mlp.set_normalization(np.zeros(8, np.float32), np.ones(8, np.float32), 1e-4)
 
-    def run(fb, tau):
+    def run(fb, tau, diag=False):
         m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
         m.set_params(**prm)
         m.load_state(sc.pos, sc.vel)
         m.set_warm_start(mlp, 1, fb, tau)
+        m.set_residual_pass(diag)
         m.update(1)
         out = m.read_state()[0], m.residual_error()
         m.close()
         return out
 
-    p0, r0 = run(0, 0.0)
+    p0, r0 = run(0, 0.0, diag=True)
     p1, r1 = run(3, 1e30)   # gate closed: identical to no fallback
     _eq(p1, p0, "gate closed")
     p2, r2 = run(3, -1.0)   # gate open: three more plain iterations
     assert not np.array_equal(p2, p0)
-    log(f"residual without fallback {r0}, gate closed {r1}, gate open {r2}")
-    # the residual is measured before the fallback, by a deterministic reduction
+    # the gate's pass precedes the fallback and matches the diagnostic pass bit for bit
+    # (a deterministic reduction)
     assert np.array_equal(r1, r0) and np.array_equal(r2, r0)
-    # a threshold just below / at the measured mean decides the gate
-    p3, _ = run(3, float(np.nextafter(r0[1], np.float32(0))) * 0.999)
+    # with the diagnostic pass on, the error left after the fallback iterations is read
+    p5, r5 = run(3, -1.0, diag=True)
+    _eq(p5, p2, "diagnostic pass changes nothing")
+    log(f"residual without fallback {r0}, after three fallback iterations {r5}")
+    assert r5[1] < r0[1]
+    # a threshold just below / above the measured mean decides the gate
+    p3, _ = run(3, float(r0[1]) * 0.999)
     _eq(p3, p2, "gate open at tau < mean")
     p4, _ = run(3, float(r0[1]) * 1.001)
     _eq(p4, p0, "gate closed at tau > mean")
