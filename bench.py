@@ -9,7 +9,9 @@ of the named dam break instead of the point clumps the dam-break parameters coll
 fluid into after a few substeps.  Inputs are resident in HBM before the timed region, which
 is bracketed by stream synchronisation.  A continuing simulation from the same input is
 timed as well and reported under config.continuing_run, and the learned warm-start is
-compared with plain iteration counts at equal density error under config.learned_vs_plain.
+compared with plain iteration counts at equal density error under config.learned_vs_plain,
+and 256 independent scenes advanced as one batch are compared with the same scenes run one
+after another under config.batched_scenes.
 
 Prints one JSON line.  Besides the throughput it carries
   roofline     : the dominant kernel's algorithmic HBM bytes per launch / its mean launch
@@ -65,8 +67,8 @@ def run_gpu(args, sc, mapping="literal", profile=True):
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
     m.set_params(**prm)
     m.set_nb_jacobi_iters(sc.iters)
-    if args.block or args.xcd or args.sort_bits:
-        m.set_tuning(args.block or 256, args.xcd or 0, args.sort_bits or 0)
+    if args.block or args.flags or args.sort_bits:
+        m.set_tuning(args.block or 256, args.flags, args.sort_bits or 0)
     m.load_state(sc.pos, sc.vel)
     m.save_checkpoint()
     m.sync()
@@ -136,6 +138,74 @@ def run_cpu_baseline(sc, budget_s: float, max_steps: int):
 
 def _err(e):
     return {"max": float(e[0]), "mean": float(e[1]), "max_compression": float(e[2])}
+
+
+def run_batched(args):
+    """256 independent dam breaks (2,048..65,536 particles, log-uniform; box aspect ratios
+    in [1, 4]; |g| in [5, 15]; the dam-break coefficients) advanced by one batch, against
+    the same scenes run one after another as separate models.
+
+    Both time fixed-input substeps (restore every scene's input, one substep).  The batch is
+    autotuned first (block size x sort digit width).  The sequential run synchronises after
+    every scene, so scenes do not overlap.
+    """
+    from realtimeparticles_amd import capi, scenes
+    scs = scenes.batch_scenes()
+    total = sum(sc.n for sc in scs)
+    b = capi.Batch(scs)
+    for k, sc in enumerate(scs):
+        b.set_params(k, **sc.params())
+        b.load_state(k, sc.pos, sc.vel)
+    b.set_nb_jacobi_iters(4)
+    b.save_checkpoint()
+    tuning = b.autotune(trial_steps=3)
+
+    def batch_steps(k):
+        for _ in range(k):
+            b.restore_checkpoint()
+            b.update(1)
+
+    batch_steps(3)
+    b.sync()
+    t0 = time.perf_counter()
+    batch_steps(args.batch_steps)
+    b.sync()
+    t_batch = (time.perf_counter() - t0) / args.batch_steps
+    b.close()
+
+    models = []
+    for sc in scs:
+        m = capi.Fluids(sc.n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+        m.set_params(**sc.params())
+        m.set_nb_jacobi_iters(4)
+        m.load_state(sc.pos, sc.vel)
+        m.save_checkpoint()
+        models.append(m)
+
+    def seq_steps(k):
+        for _ in range(k):
+            for m in models:
+                m.restore_checkpoint()
+                m.update(1)
+                m.sync()
+
+    seq_steps(1)
+    t0 = time.perf_counter()
+    seq_steps(args.batch_seq_steps)
+    t_seq = (time.perf_counter() - t0) / args.batch_seq_steps
+    for m in models:
+        m.close()
+    return {
+        "workload": f"{len(scs)} dam-break scenes, {total} particles "
+                    f"({min(sc.n for sc in scs)}..{max(sc.n for sc in scs)} per scene), "
+                    "4 Jacobi iterations, substep dt=1/120",
+        "scenes": len(scs), "particles": total,
+        "batch": {"particle_updates_per_s": total / t_batch, "ms_per_step": round(t_batch * 1e3, 4),
+                  "steps": args.batch_steps, "tuning": tuning},
+        "sequential": {"particle_updates_per_s": total / t_seq, "ms_per_step": round(t_seq * 1e3, 4),
+                       "steps": args.batch_seq_steps},
+        "speedup": round(t_seq / t_batch, 3),
+    }
 
 
 def run_learned(args, sc):
@@ -209,12 +279,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="pbf-1m")
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--xcd", type=int, default=0)
+    ap.add_argument("--flags", type=int, default=0,
+                    help="launch flags: 1 XCD-contiguous blocks, 2 LDS wave tiles")
     ap.add_argument("--sort-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-seq-steps", type=int, default=3)
     ap.add_argument("--learn-seeds", default="101,102")
     ap.add_argument("--learn-epochs", type=int, default=200)
     ap.add_argument("--learn-steps", type=int, default=50)
@@ -262,6 +336,10 @@ def main():
     if not args.no_learned:
         learned_cmp = run_learned(args, sc)
 
+    batched = None
+    if not args.no_batch:
+        batched = run_batched(args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = run_cpu_baseline(sc, args.cpu_budget, 5)
@@ -295,6 +373,7 @@ def main():
                 "density_error_at_end": _err(g["continuing_density_error"])},
             "stable_variant": stable,
             "learned_vs_plain": learned_cmp,
+            "batched_scenes": batched,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -150,9 +150,10 @@ int rtp_fluids_restore_checkpoint(rtp_fluids* h);
 /* Replay pairs of substeps from a captured hipGraph (no reference counterpart; the
  * reference enqueues every kernel each frame).  Default on. */
 int rtp_fluids_use_graph(rtp_fluids* h, int enable);
-/* Launch tuning (no reference counterpart): block size of the neighbour passes (64..1024,
- * multiple of 64), XCD-contiguous block order (0/1), sort digit width (0 = auto, 4..11). */
-int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBits);
+/* Launch tuning (no reference counterpart): block size of the per-particle passes (64,
+ * 128, 192 or 256), RTP_LAUNCH_* flags, sort digit width (0 = auto, 4..11).  Results do not
+ * depend on any of them. */
+int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int flags, int sortBits);
 
 /* ========================= Learned warm-start ========================= */
 /* No reference counterpart: an 8 -> 64 -> 64 -> 1 MLP (GELU, BF16 weights, FP32
@@ -205,10 +206,12 @@ int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat,
  * block mapping and the sort's digit width are chosen per workload by timing it. */
 typedef struct {
   int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
-  int32_t xcdRemap;   /* 1: consecutive particle blocks on one XCD (single models only) */
+  int32_t flags;      /* RTP_LAUNCH_* bits (single models; 0 for a batch) */
   int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
   float msPerStep;    /* measured mean substep time of this choice */
 } rtp_tuning;
+#define RTP_LAUNCH_XCD_MAP 1   /* consecutive particle blocks on one XCD (shared L2) */
+#define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
 
 /* Times fixed-input substeps (the current state restored before each) for every candidate
  * launch shape, keeps the fastest and restores the state.  trialSteps substeps per
@@ -250,7 +253,7 @@ int rtp_batch_sync(rtp_batch* h);
 int rtp_batch_save_checkpoint(rtp_batch* h);
 int rtp_batch_restore_checkpoint(rtp_batch* h);
 int rtp_batch_use_graph(rtp_batch* h, int enable);
-/* xcdRemap must be 0 for a batch. */
+/* Block size and sort digit width as rtp_fluids_set_tuning (no launch flags). */
 int rtp_batch_set_tuning(rtp_batch* h, int blockSize, int sortBits);
 int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
 int rtp_batch_enable_profiling(rtp_batch* h, int enable);

@@ -21,6 +21,7 @@ _vp = ctypes.c_void_p
 _int = ctypes.c_int
 
 FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
+LAUNCH_XCD_MAP, LAUNCH_LDS_TILES = 1, 2
 
 
 class RtpError(RuntimeError):
@@ -73,10 +74,10 @@ class KernelTime(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("blockSize", _i32), ("xcdRemap", _i32), ("sortBits", _i32), ("msPerStep", _f32)]
+    _fields_ = [("blockSize", _i32), ("flags", _i32), ("sortBits", _i32), ("msPerStep", _f32)]
 
     def as_dict(self) -> dict:
-        return {"block": int(self.blockSize), "xcd_remap": int(self.xcdRemap),
+        return {"block": int(self.blockSize), "flags": int(self.flags),
                 "sort_bits": int(self.sortBits), "ms_per_step": float(self.msPerStep)}
 
 
@@ -375,8 +376,9 @@ class Fluids:
                                                target.ctypes.data), "record_teacher")
         return feat, target
 
-    def set_tuning(self, block: int = 256, xcd_remap: int = 0, sort_bits: int = 0) -> None:
-        _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(xcd_remap), int(sort_bits)),
+    def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0) -> None:
+        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES."""
+        _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(flags), int(sort_bits)),
                "set_tuning")
 
     def autotune(self, trial_steps: int = 5) -> dict:

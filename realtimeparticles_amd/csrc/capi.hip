@@ -818,14 +818,16 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable)
   return RTP_OK;
 }
 
-int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int xcdRemap, int sortBits)
+int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int flags, int sortBits)
 {
   RTP_REQUIRE(h, "NULL handle");
   RTP_REQUIRE(blockSize >= 64 && blockSize <= 256 && blockSize % 64 == 0,
               "blockSize must be 64, 128, 192 or 256");
   RTP_REQUIRE(sortBits == 0 || (sortBits >= 4 && sortBits <= 11), "sortBits must be 0 or 4..11");
   h->L.block = blockSize;
-  h->L.xcdRemap = xcdRemap ? 1 : 0;
+  RTP_REQUIRE(flags >= 0 && flags <= 3, "flags must be a combination of XCD mapping (1) and LDS tiles (2)");
+  h->L.xcdRemap = flags & 1;
+  h->L.tile = (flags >> 1) & 1;
   h->grid.sortBits = sortBits;
   h->drop_graph();
   return RTP_OK;
@@ -846,11 +848,11 @@ int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best)
   hipEvent_t e0 = nullptr, e1 = nullptr;
   RTP_HIP_TRY(hipEventCreate(&e0));
   RTP_HIP_TRY(hipEventCreate(&e1));
-  rtp_tuning top{ h->L.block, h->L.xcdRemap, h->grid.sortBits, 1e30f };
+  rtp_tuning top{ h->L.block, h->L.xcdRemap | (h->L.tile << 1), h->grid.sortBits, 1e30f };
   const int blocks[] = { 64, 128, 192, 256 };
   const int bitsChoices[] = { 7, 8, 9, 10, 11 };
   for (int b : blocks) {
-    for (int x = 0; x < 2 && !rc; ++x) {
+    for (int x = 0; x < 4 && !rc; ++x) {
       for (int bits : bitsChoices) {
         if ((rc = rtp_fluids_set_tuning(h, b, x, bits)))
           break;
@@ -883,7 +885,7 @@ int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best)
   h->paused = pausedWas;
   if (rc)
     return rc;
-  if ((rc = rtp_fluids_set_tuning(h, top.blockSize, top.xcdRemap, top.sortBits)) ||
+  if ((rc = rtp_fluids_set_tuning(h, top.blockSize, top.flags, top.sortBits)) ||
       (rc = rtp_fluids_restore_checkpoint(h)))
     return rc;
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));

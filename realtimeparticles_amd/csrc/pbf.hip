@@ -52,6 +52,7 @@ __device__ __forceinline__ float4 ldp(const float4* __restrict__ pred, uint32_t 
 struct LaneQueue {
   uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * kMaxBlock]
   uint32_t count;
+  uint32_t cap;    // entries per lane
 };
 
 template <bool CL, class F>
@@ -78,7 +79,7 @@ __device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4
   const float d2 = dx * dx + dy * dy + dz * dz;
   if (!(d2 >= c.cullSq && d2 < kFarD2)) {
     q.slot[q.count * kMaxBlock] = e;
-    if (++q.count == kQueue)
+    if (++q.count == q.cap)
       drain<CL>(q, p, pred, c, f);
   }
 }
@@ -102,6 +103,171 @@ __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& 
     offer<CL>(q, p, ldp<CL>(pred, e, c), e, pred, c, f);
 }
 
+// ------------------------------------------------------------------ wave tiles in LDS
+// A wave's 64 consecutive particles (sorted by cell) mostly share their neighbour cells.
+// Before a neighbour pass each wave stages into LDS the candidate positions of up to
+// kTileCols cell columns (x, y) around its lanes' cells, each over the z-range its lanes
+// can reach, and the walk reads a range from LDS whenever that range lies inside its
+// column's staged span -- otherwise, as without a tile, from global memory.  The
+// candidates, their order and every result are the same either way.
+constexpr uint32_t kTileCols = 16;
+constexpr uint32_t kTileCap = 768;   // staged candidates per wave
+constexpr uint32_t kTileQueue = 12;  // survivor-queue depth of the tiled passes
+constexpr uint32_t kWaves = kMaxBlock / 64;
+
+// What a neighbour walk works with besides the particle arrays: its lane's survivor queue
+// in LDS and, optionally, the wave's tile.  The kernel owns (and sizes) both.
+struct WalkCtx {
+  uint32_t* qslot = nullptr;               // this lane's queue column
+  uint32_t qcap = 0;
+  bool on = false;                         // a tile is staged
+  int32_t x0 = 0, y0 = 0, nx = 0, ny = 0;  // staged columns: [x0, x0+nx) x [y0, y0+ny)
+  const uint32_t* lo = nullptr;            // per column: first staged global index
+  const uint32_t* hi = nullptr;            //             last (hi < lo: not staged)
+  const uint32_t* off = nullptr;           //             offset in x, y, z
+  const float* x = nullptr;
+  const float* y = nullptr;
+  const float* z = nullptr;
+};
+
+// LDS a block's waves stage their tiles in (positions as three float arrays).
+struct TileLds {
+  float x[kWaves][kTileCap], y[kWaves][kTileCap], z[kWaves][kTileCap];
+  uint32_t meta[kWaves][3][kTileCols];
+};
+
+__device__ __forceinline__ int32_t wave_min(int32_t v)
+{
+  for (int o = 32; o > 0; o >>= 1)
+    v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ int32_t wave_max(int32_t v)
+{
+  for (int o = 32; o > 0; o >>= 1)
+    v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Stage the wave's tile.  Every lane of the wave must call this (with its own position p,
+// clamped on load when CL, as the pass will see it); positions are staged the same way.
+template <bool CL>
+__device__ __forceinline__ void build_tile(const float4 p, const float4* __restrict__ pred,
+                                           const uint2* __restrict__ se, const PbfConsts& c,
+                                           TileLds& lds, WalkCtx& T)
+{
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t ux, uy, uz;
+  cell3d(p, c.g, ux, uy, uz);
+  const int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
+  const int32_t xmin = wave_min(cx), xmax = wave_max(cx), ymin = wave_min(cy), ymax = wave_max(cy);
+  if (xmax - xmin > 1 || ymax - ymin > 1)
+    return;
+  const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
+  T.x0 = xmin - 1;
+  T.y0 = ymin - 1;
+  T.nx = xmax - xmin + 3;
+  T.ny = ymax - ymin + 3;
+  const int32_t ncol = T.nx * T.ny;  // <= kTileCols
+  // z-range of every column: the cells cz-1..cz+1 of the lanes whose 3x3 columns include it
+  int32_t myLo = 1, myHi = 0;
+  for (int32_t k = 0; k < ncol; ++k) {
+    const int32_t colx = T.x0 + k / T.ny, coly = T.y0 + k % T.ny;
+    const bool in = abs(cx - colx) <= 1 && abs(cy - coly) <= 1;
+    const int32_t zl = wave_min(in ? cz - 1 : 0x7fffffff);
+    const int32_t zh = wave_max(in ? cz + 1 : -0x7fffffff);
+    if ((int32_t)lane == k) {
+      myLo = max(zl, 0);
+      myHi = min(zh, rz - 1);
+      if (colx < 0 || colx >= rx || coly < 0 || coly >= ry)
+        myHi = myLo - 1;  // outside the grid: the walk reaches such columns only by wrapping
+    }
+  }
+  // lane k < ncol: index span of its column's cells myLo..myHi
+  uint32_t first = 1u, last = 0u;
+  if ((int32_t)lane < ncol && myLo <= myHi) {
+    const int32_t colx = T.x0 + (int32_t)lane / T.ny, coly = T.y0 + (int32_t)lane % T.ny;
+    const uint2* col = se + (colx * ry + coly) * rz;
+    first = 0xffffffffu;
+    for (int32_t z = myLo; z <= myHi; ++z) {
+      const uint2 r = col[z];
+      if (r.x <= r.y) {
+        first = min(first, r.x);
+        last = max(last, r.y);
+      }
+    }
+    if (first == 0xffffffffu) {
+      first = 1u;
+      last = 0u;
+    }
+  }
+  uint32_t size = first <= last ? last - first + 1u : 0u;
+  // exclusive prefix of the sizes over the lanes
+  uint32_t incl = size;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if ((int)lane >= o)
+      incl += t;
+  }
+  const uint32_t off = incl - size;
+  if (off + size > kTileCap) {  // does not fit: this column reads global memory
+    size = 0u;
+    first = 1u;
+    last = 0u;
+  }
+  if ((int32_t)lane < ncol) {
+    lds.meta[w][0][lane] = first;
+    lds.meta[w][1][lane] = last;
+    lds.meta[w][2][lane] = off;
+  }
+  for (int32_t k = 0; k < ncol; ++k) {
+    const uint32_t g0 = __shfl(first, k, 64), sz = __shfl(size, k, 64), o = __shfl(off, k, 64);
+    for (uint32_t t = lane; t < sz; t += 64u) {
+      const float4 q = ldp<CL>(pred, g0 + t, c);
+      lds.x[w][o + t] = q.x;
+      lds.y[w][o + t] = q.y;
+      lds.z[w][o + t] = q.z;
+    }
+  }
+  T.on = true;
+  T.lo = lds.meta[w][0];
+  T.hi = lds.meta[w][1];
+  T.off = lds.meta[w][2];
+  T.x = lds.x[w];
+  T.y = lds.y[w];
+  T.z = lds.z[w];
+}
+
+// Candidates lo..hi read from the tile; the staged entry of candidate e is e - shift.
+template <bool CL, class F>
+__device__ __forceinline__ void scan_tile(uint32_t lo, uint32_t hi, uint32_t shift,
+                                          const WalkCtx& T, LaneQueue& q, const float4 p,
+                                          const float4* __restrict__ pred, const PbfConsts& c,
+                                          F& f)
+{
+  for (uint32_t e = lo; e <= hi; ++e) {
+    const uint32_t t = e - shift;
+    offer<CL>(q, p, make_float4(T.x[t], T.y[t], T.z[t], 0.0f), e, pred, c, f);
+  }
+}
+
+// One index range of column (nx, ny): from the tile when staged, else from global memory.
+template <bool CL, class F>
+__device__ __forceinline__ void scan_any(uint32_t lo, uint32_t hi, const WalkCtx& T, int32_t k,
+                                         LaneQueue& q, const float4 p,
+                                         const float4* __restrict__ pred, const PbfConsts& c,
+                                         F& f)
+{
+  if (lo > hi)
+    return;
+  if (k >= 0 && lo >= T.lo[k] && hi <= T.hi[k]) {
+    scan_tile<CL>(lo, hi, T.lo[k] - T.off[k], T, q, p, pred, c, f);
+    return;
+  }
+  scan_range<CL>(lo, hi, q, p, pred, c, f);
+}
+
 // Visit the candidates of particle position p in the reference's order: 27 cells, x
 // outer and z inner with periodic wrap (fluids.cl:101-123), each cell from start to end.
 // The three cells of one z-column are scanned as a single index range whenever the cell
@@ -110,10 +276,10 @@ __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& 
 // The queue is drained after every x-slab of nine cells.
 template <bool CL = false, class F>
 __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
-                                       const uint2* __restrict__ se, const PbfConsts& c, F&& f)
+                                       const uint2* __restrict__ se, const PbfConsts& c, F&& f,
+                                       const WalkCtx& T)
 {
-  __shared__ uint32_t queue[kQueue * kMaxBlock];
-  LaneQueue q{ &queue[threadIdx.x], 0u };
+  LaneQueue q{ T.qslot, 0u, T.qcap };
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
@@ -142,12 +308,18 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
           ok = false;
         }
       }
+      int32_t k = -1;  // staged column of (nx, ny), if any
+      if (T.on) {
+        const int32_t dxc = nx - T.x0, dyc = wrap((int32_t)cy + iy, ry) - T.y0;
+        if (dxc >= 0 && dxc < T.nx && dyc >= 0 && dyc < T.ny)
+          k = dxc * T.ny + dyc;
+      }
       if (ok) {
-        scan_range<CL>(lo, hi, q, p, pred, c, f);
+        scan_any<CL>(lo, hi, T, k, q, p, pred, c, f);
       } else {
-        scan_range<CL>(a.x, a.y, q, p, pred, c, f);
-        scan_range<CL>(b.x, b.y, q, p, pred, c, f);
-        scan_range<CL>(d.x, d.y, q, p, pred, c, f);
+        scan_any<CL>(a.x, a.y, T, k, q, p, pred, c, f);
+        scan_any<CL>(b.x, b.y, T, k, q, p, pred, c, f);
+        scan_any<CL>(d.x, d.y, T, k, q, p, pred, c, f);
       }
     }
     drain<CL>(q, p, pred, c, f);
@@ -189,6 +361,13 @@ __device__ __forceinline__ uint32_t particle_index(int remap)
 }
 
 // ---------------------------------------------------------------------------- kernels
+
+// The survivor queue of an untiled neighbour pass.
+#define RTP_PLAIN_CTX                                        \
+  __shared__ uint32_t rtpQueue[kQueue * kMaxBlock];          \
+  WalkCtx W;                                                 \
+  W.qslot = &rtpQueue[threadIdx.x];                          \
+  W.qcap = kQueue;
 
 // fluids.cl:71-73 (newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt) and
 // fillCellIDs (grid.cl:76-86) on predPos, the key offset by the scene's key base.
@@ -236,7 +415,8 @@ __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __
                                                     const uint2* __restrict__ se,
                                                     float* __restrict__ density,
                                                     float* __restrict__ lambda,
-                                                    const PbfConsts& c, const LearnArgs& la)
+                                                    const PbfConsts& c, const LearnArgs& la,
+                                                    const WalkCtx& T)
 {
   if ((MODE & kGated) && !(la.gate[0] > la.tau))
     return;
@@ -256,7 +436,7 @@ __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __
       sgz = sgz + g.z;
       ssq = ssq + (g.x * g.x + g.y * g.y + g.z * g.z);
     }
-  });
+  }, T);
   density[i] = dens;
   if (MODE & kDensOnly)
     return;
@@ -304,8 +484,9 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
                                                         PbfConsts c, int remap, LearnArgs la)
 {
   const uint32_t i = particle_index(remap);
+  RTP_PLAIN_CTX
   if (i < n)
-    density_lambda_body<MODE>(i, pred, se, density, lambda, c, la);
+    density_lambda_body<MODE>(i, pred, se, density, lambda, c, la, W);
 }
 
 // fld_computeConstraintCorrection (fluids.cl:198-247) + fld_correctPosition (:252-258),
@@ -322,7 +503,8 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
                                              float4* __restrict__ predOut,
                                              const float4* __restrict__ pos,
                                              float4* __restrict__ vel, float4* __restrict__ posOut,
-                                             int last, const PbfConsts& c, const LearnArgs& la)
+                                             int last, const PbfConsts& c, const LearnArgs& la,
+                                             const WalkCtx& T)
 {
   if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
     predOut[i] = pred[i];
@@ -346,7 +528,7 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
       cy = cy + s * 0.0f;
       cz = cz + s * 0.0f;
     }
-  });
+  }, T);
   float4 q;
   q.x = p.x + cx / c.p.restDensity;
   q.y = p.y + cy / c.p.restDensity;
@@ -384,8 +566,9 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
                                                  int last, PbfConsts c, int remap, LearnArgs la)
 {
   const uint32_t i = particle_index(remap);
+  RTP_PLAIN_CTX
   if (i < n)
-    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la);
+    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W);
 }
 
 // fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
@@ -421,7 +604,8 @@ __global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ p
 __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restrict__ pred,
                                                const uint2* __restrict__ se,
                                                const float4* __restrict__ vel,
-                                               float4* __restrict__ vort, const PbfConsts& c)
+                                               float4* __restrict__ vort, const PbfConsts& c,
+                                               const WalkCtx& T)
 {
   const float4 p = pred[i];
   const float4 vi = vel[i];
@@ -437,7 +621,7 @@ __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restr
       wy = wy + (az * g.x - ax * g.z);
       wz = wz + (ax * g.y - ay * g.x);
     }
-  });
+  }, T);
   vort[i] = make_float4(wx, wy, wz, __builtin_sqrtf(wx * wx + wy * wy + wz * wz));
 }
 
@@ -448,8 +632,9 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
                                                    PbfConsts c, int remap)
 {
   const uint32_t i = particle_index(remap);
+  RTP_PLAIN_CTX
   if (i < n)
-    vorticity_body(i, pred, se, vel, vort, c);
+    vorticity_body(i, pred, se, vel, vort, c, W);
 }
 
 // fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
@@ -458,7 +643,8 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
                                                  const uint2* __restrict__ se,
                                                  const float4* __restrict__ vort,
                                                  const float4* __restrict__ vel,
-                                                 float4* __restrict__ velOut, const PbfConsts& c)
+                                                 float4* __restrict__ velOut, const PbfConsts& c,
+                                                 const WalkCtx& T)
 {
   const float4 p = pred[i];
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
@@ -472,7 +658,7 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
       ny = ny + l * g.y;
       nz = nz + l * g.z;
     }
-  });
+  }, T);
   // normalize(n), 0 for the zero vector
   if (!(nx == 0.0f && ny == 0.0f && nz == 0.0f)) {
     const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
@@ -497,8 +683,9 @@ __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ 
                                                      PbfConsts c, int remap)
 {
   const uint32_t i = particle_index(remap);
+  RTP_PLAIN_CTX
   if (i < n)
-    confinement_body(i, pred, se, vort, vel, velOut, c);
+    confinement_body(i, pred, se, vort, vel, velOut, c, W);
 }
 
 // fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
@@ -506,7 +693,7 @@ __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__
                                           const uint2* __restrict__ se,
                                           const float4* __restrict__ velIn,
                                           float4* __restrict__ velOut, float4* __restrict__ posOut,
-                                          const PbfConsts& c)
+                                          const PbfConsts& c, const WalkCtx& T)
 {
   const float4 p = pred[i];
   const float4 vi = velIn[i];
@@ -519,7 +706,7 @@ __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__
     sx = sx + (ve.x - vi.x) * w;
     sy = sy + (ve.y - vi.y) * w;
     sz = sz + (ve.z - vi.z) * w;
-  });
+  }, T);
   const float k = c.p.xsphViscosityCoeff;
   velOut[i] = make_float4(vi.x + k * sx, vi.y + k * sy, vi.z + k * sz, 0.0f);
   posOut[i] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -533,9 +720,89 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
                                               PbfConsts c, int remap)
 {
   const uint32_t i = particle_index(remap);
+  RTP_PLAIN_CTX
   if (i < n)
-    xsph_body(i, pred, se, velIn, velOut, posOut, c);
+    xsph_body(i, pred, se, velIn, velOut, posOut, c, W);
 }
+
+// --------------------------------------------------------------- LDS-tiled wrappers
+// Same passes with the wave tiles staged first.  Every lane takes part in staging (lanes
+// past n stand in for the last particle), then only real particles run the pass.
+#define RTP_TILED_PROLOGUE(CLV)                                                \
+  __shared__ TileLds tl;                                                       \
+  __shared__ uint32_t rtpQueue[kTileQueue * kMaxBlock];                        \
+  const uint32_t i0 = particle_index(remap);                                   \
+  const bool active = i0 < n;                                                  \
+  const uint32_t i = active ? i0 : n - 1u;                                     \
+  WalkCtx T;                                                                   \
+  T.qslot = &rtpQueue[threadIdx.x];                                            \
+  T.qcap = kTileQueue;                                                         \
+  build_tile<CLV>(ldp<CLV>(pred, i, c), pred, se, c, tl, T);                   \
+  __syncthreads();
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_density_lambda_t(const float4* __restrict__ pred,
+                                                          const uint2* __restrict__ se,
+                                                          float* __restrict__ density,
+                                                          float* __restrict__ lambda, uint32_t n,
+                                                          PbfConsts c, int remap, LearnArgs la)
+{
+  RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
+  if (active)
+    density_lambda_body<MODE>(i, pred, se, density, lambda, c, la, T);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_correct_t(const float4* __restrict__ pred,
+                                                   const float* __restrict__ lambda,
+                                                   const uint2* __restrict__ se,
+                                                   float4* __restrict__ predOut,
+                                                   const float4* __restrict__ pos,
+                                                   float4* __restrict__ vel,
+                                                   float4* __restrict__ posOut, uint32_t n,
+                                                   int last, PbfConsts c, int remap, LearnArgs la)
+{
+  RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
+  if (active)
+    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, T);
+}
+
+__global__ __launch_bounds__(256) void k_vorticity_t(const float4* __restrict__ pred,
+                                                     const uint2* __restrict__ se,
+                                                     const float4* __restrict__ vel,
+                                                     float4* __restrict__ vort, uint32_t n,
+                                                     PbfConsts c, int remap)
+{
+  RTP_TILED_PROLOGUE(false)
+  if (active)
+    vorticity_body(i, pred, se, vel, vort, c, T);
+}
+
+__global__ __launch_bounds__(256) void k_confinement_t(const float4* __restrict__ pred,
+                                                       const uint2* __restrict__ se,
+                                                       const float4* __restrict__ vort,
+                                                       const float4* __restrict__ vel,
+                                                       float4* __restrict__ velOut, uint32_t n,
+                                                       PbfConsts c, int remap)
+{
+  RTP_TILED_PROLOGUE(false)
+  if (active)
+    confinement_body(i, pred, se, vort, vel, velOut, c, T);
+}
+
+__global__ __launch_bounds__(256) void k_xsph_t(const float4* __restrict__ pred,
+                                                const uint2* __restrict__ se,
+                                                const float4* __restrict__ velIn,
+                                                float4* __restrict__ velOut,
+                                                float4* __restrict__ posOut, uint32_t n,
+                                                PbfConsts c, int remap)
+{
+  RTP_TILED_PROLOGUE(false)
+  if (active)
+    xsph_body(i, pred, se, velIn, velOut, posOut, c, T);
+}
+
+#undef RTP_TILED_PROLOGUE
 
 // ------------------------------------------------------------------- batched wrappers
 // One launch covers every scene of a batch: block b works on scene blocks[b].x, particles
@@ -589,7 +856,8 @@ __global__ __launch_bounds__(256) void k_density_lambda_batch(BatchView bv,
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  density_lambda_body<0>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs());
+  RTP_PLAIN_CTX
+  density_lambda_body<0>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs(), W);
 }
 
 __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float4* __restrict__ pred,
@@ -605,8 +873,9 @@ __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
+  RTP_PLAIN_CTX
   correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
-                  posOut ? posOut + o : nullptr, last, c, LearnArgs());
+                  posOut ? posOut + o : nullptr, last, c, LearnArgs(), W);
 }
 
 __global__ __launch_bounds__(256) void k_update_vel_batch(BatchView bv, const float4* __restrict__ pred,
@@ -632,7 +901,8 @@ __global__ __launch_bounds__(256) void k_vorticity_batch(BatchView bv, const flo
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  vorticity_body(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c);
+  RTP_PLAIN_CTX
+  vorticity_body(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c, W);
 }
 
 __global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
@@ -647,7 +917,8 @@ __global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  confinement_body(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c);
+  RTP_PLAIN_CTX
+  confinement_body(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c, W);
 }
 
 __global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* __restrict__ pred,
@@ -661,7 +932,8 @@ __global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* 
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  xsph_body(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c);
+  RTP_PLAIN_CTX
+  xsph_body(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c, W);
 }
 
 __global__ __launch_bounds__(64) void k_zero3(float* __restrict__ out)
@@ -758,8 +1030,12 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
   const dim3 g(blocks_for(n, L.block)), b(L.block);
 #define RTP_DL(M)                                                                              \
   case M:                                                                                    \
-    hipLaunchKernelGGL(k_density_lambda<M>, g, b, 0, s, pred, startEnd, density, lambda, n, c, \
-                       L.xcdRemap, la);                                                      \
+    if (L.tile)                                                                              \
+      hipLaunchKernelGGL(k_density_lambda_t<M>, g, b, 0, s, pred, startEnd, density, lambda, n, \
+                         c, L.xcdRemap, la);                                                 \
+    else                                                                                     \
+      hipLaunchKernelGGL(k_density_lambda<M>, g, b, 0, s, pred, startEnd, density, lambda, n, \
+                         c, L.xcdRemap, la);                                                 \
     break;
   switch (mode) {
     RTP_DL(0)
@@ -784,8 +1060,12 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
   const dim3 g(blocks_for(n, L.block)), b(L.block);
 #define RTP_CO(M)                                                                              \
   case M:                                                                                    \
-    hipLaunchKernelGGL(k_correct<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel,    \
-                       posOut, n, last, c, L.xcdRemap, la);                                  \
+    if (L.tile)                                                                              \
+      hipLaunchKernelGGL(k_correct_t<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel, \
+                         posOut, n, last, c, L.xcdRemap, la);                                \
+    else                                                                                     \
+      hipLaunchKernelGGL(k_correct<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel,  \
+                         posOut, n, last, c, L.xcdRemap, la);                                \
     break;
   switch (mode) {
     RTP_CO(0)
@@ -811,8 +1091,8 @@ void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_vorticity, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
-                     startEnd, vel, vort, n, c, L.xcdRemap);
+  hipLaunchKernelGGL(L.tile ? k_vorticity_t : k_vorticity, dim3(blocks_for(n, L.block)),
+                     dim3(L.block), 0, s, pred, startEnd, vel, vort, n, c, L.xcdRemap);
 }
 
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
@@ -821,8 +1101,8 @@ void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const flo
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_confinement, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred,
-                     startEnd, vort, vel, velOut, n, c, L.xcdRemap);
+  hipLaunchKernelGGL(L.tile ? k_confinement_t : k_confinement, dim3(blocks_for(n, L.block)),
+                     dim3(L.block), 0, s, pred, startEnd, vort, vel, velOut, n, c, L.xcdRemap);
 }
 
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
@@ -831,8 +1111,8 @@ void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* ve
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pred, startEnd,
-                     velIn, velOut, posOut, n, c, L.xcdRemap);
+  hipLaunchKernelGGL(L.tile ? k_xsph_t : k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block), 0,
+                     s, pred, startEnd, velIn, velOut, posOut, n, c, L.xcdRemap);
 }
 
 void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,

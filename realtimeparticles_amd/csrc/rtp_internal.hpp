@@ -61,7 +61,8 @@ struct BoidsConsts {
 // Launch shape shared by the per-particle kernels.
 struct Launch {
   int block = 256;     // threads per block
-  int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD (slower on the dam break)
+  int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD
+  int tile = 0;        // 1: neighbour passes stage their candidates in LDS wave tiles
 };
 
 // ---------------------------------------------------------------------------- grid / sort
