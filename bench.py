@@ -59,6 +59,27 @@ def substep_bytes(name: str, n: int, nc: int, iters: int, vorticity: bool) -> fl
     return float(per * n + launches * nc * 8)
 
 
+# profiler names of the model -> kernel names in rocprofv3 output
+KERNEL_OF = {"pbf_predict_hash": "k_predict", "reorder_cells": "k_reorder",
+             "pbf_density_lambda": "k_density_lambda", "pbf_correct": "k_correct",
+             "pbf_vorticity": "k_vorticity", "pbf_confinement": "k_confinement",
+             "pbf_xsph": "k_xsph"}
+
+
+def measured_traffic(workload: str, name: str):
+    """HBM bytes per launch of the kernel behind profiler entry `name`, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    base = KERNEL_OF.get(name)
+    if base is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rows = [v for v in json.load(f)["kernels"].values() if v.get("base") == base]
+    if not rows:
+        return None
+    return sum(r["traffic_bytes"] for r in rows) / len(rows)
+
+
 def run_gpu(args, sc, mapping="literal", profile=True):
     """Times the fixed-input step (restore the dam-break input, one substep) and, for
     reference, a continuing simulation from that input."""
@@ -97,6 +118,9 @@ def run_gpu(args, sc, mapping="literal", profile=True):
         out["prof"] = m.profile()
         m.enable_profiling(False)
 
+    if args.no_continuing:
+        m.close()
+        return out
     # continuing simulation (graph replay) from the same input
     m.restore_checkpoint()
     m.update(args.warmup)
@@ -284,6 +308,8 @@ def main():
     ap.add_argument("--sort-bits", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-continuing", action="store_true",
+                    help="skip the continuing-run figure (profiling runs time the step alone)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -318,9 +344,11 @@ def main():
     roofline = None
     if nbytes is not None:
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(args.workload, dom_name)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                    "traffic": None, "kernel": dom_name,
+                    "traffic": None if traffic is None else round(traffic),
+                    "kernel": dom_name,
                     "bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 5)}
 
     # diagnostic only: the same workload with the "stable" coefficient mapping
@@ -328,9 +356,10 @@ def main():
     if not args.no_variants:
         gs = run_gpu(args, sc, mapping="stable", profile=False)
         stable = {"params": "stable mapping (see realtimeparticles_amd/scenes.py)",
-                  "fixed_input_particle_updates_per_s": sc.n * args.steps / gs["elapsed"],
-                  "continuing_particle_updates_per_s": sc.n * args.steps / gs["continuing_elapsed"],
-                  "continuing_density_error_at_end": _err(gs["continuing_density_error"])}
+                  "fixed_input_particle_updates_per_s": sc.n * args.steps / gs["elapsed"]}
+        if "continuing_elapsed" in gs:
+            stable["continuing_particle_updates_per_s"] = sc.n * args.steps / gs["continuing_elapsed"]
+            stable["continuing_density_error_at_end"] = _err(gs["continuing_density_error"])
 
     learned_cmp = None
     if not args.no_learned:
@@ -367,7 +396,7 @@ def main():
             "step": "restore the dam-break input on the device, then one substep",
             "density_error_after_step": _err(g["density_error"]),
             "kernel_ms_per_step": {k: round(v[1] / prof_steps, 5) for k, v in sorted(prof.items())},
-            "continuing_run": {
+            "continuing_run": None if args.no_continuing else {
                 "particle_updates_per_s": sc.n * args.steps / g["continuing_elapsed"],
                 "substeps": args.steps, "after_warmup_substeps": args.warmup,
                 "density_error_at_end": _err(g["continuing_density_error"])},

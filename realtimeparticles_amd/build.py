@@ -57,7 +57,9 @@ def _run(cmd, verbose):
     return r.stdout
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
+def build(verbose: bool = False, jobs: int = 4, extra_flags=()) -> str:
+    """Compile every source and link librtp.so; extra_flags go to every hipcc compile
+    (e.g. -DRTP_QUEUE_DEPTH=16 to size the neighbour passes' survivor queue)."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     cc = hipcc()
     inc = ["-I", INCLUDE, "-I", CSRC]
@@ -65,7 +67,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-        cmds.append([cc, *HIPCC_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([cc, *HIPCC_FLAGS, *extra_flags, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
@@ -81,4 +83,4 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, extra_flags=[a for a in sys.argv[1:] if a.startswith("-D")]))

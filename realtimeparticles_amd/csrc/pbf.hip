@@ -30,7 +30,10 @@ constexpr float kFarD2 = 1.0e12f;
 // Candidates that survive the distance test are queued per lane in LDS and handed to the
 // pass body afterwards, so the expensive body runs densely over each lane's survivors
 // instead of once per candidate slot of the whole wave.  Survivors keep their order.
-constexpr uint32_t kQueue = 24;     // queued candidates per lane
+#ifndef RTP_QUEUE_DEPTH
+#define RTP_QUEUE_DEPTH 24
+#endif
+constexpr uint32_t kQueue = RTP_QUEUE_DEPTH;  // queued candidates per lane
 constexpr uint32_t kMaxBlock = 256; // threads per block the LDS queue is laid out for
 
 // fld_applyBoundaryCondition (fluids.cl:435-439) applied to a loaded position.
@@ -111,8 +114,14 @@ __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& 
 // column's staged span -- otherwise, as without a tile, from global memory.  The
 // candidates, their order and every result are the same either way.
 constexpr uint32_t kTileCols = 16;
-constexpr uint32_t kTileCap = 768;   // staged candidates per wave
-constexpr uint32_t kTileQueue = 12;  // survivor-queue depth of the tiled passes
+#ifndef RTP_TILE_CAP
+#define RTP_TILE_CAP 768
+#endif
+#ifndef RTP_TILE_QUEUE
+#define RTP_TILE_QUEUE 12
+#endif
+constexpr uint32_t kTileCap = RTP_TILE_CAP;      // staged candidates per wave
+constexpr uint32_t kTileQueue = RTP_TILE_QUEUE;  // survivor-queue depth of the tiled passes
 constexpr uint32_t kWaves = kMaxBlock / 64;
 
 // What a neighbour walk works with besides the particle arrays: its lane's survivor queue
@@ -246,7 +255,19 @@ __device__ __forceinline__ void scan_tile(uint32_t lo, uint32_t hi, uint32_t shi
                                           const float4* __restrict__ pred, const PbfConsts& c,
                                           F& f)
 {
-  for (uint32_t e = lo; e <= hi; ++e) {
+  uint32_t e = lo;
+  for (; e + 3u <= hi; e += 4u) {
+    const uint32_t t = e - shift;
+    const float4 o0 = make_float4(T.x[t], T.y[t], T.z[t], 0.0f);
+    const float4 o1 = make_float4(T.x[t + 1u], T.y[t + 1u], T.z[t + 1u], 0.0f);
+    const float4 o2 = make_float4(T.x[t + 2u], T.y[t + 2u], T.z[t + 2u], 0.0f);
+    const float4 o3 = make_float4(T.x[t + 3u], T.y[t + 3u], T.z[t + 3u], 0.0f);
+    offer<CL>(q, p, o0, e, pred, c, f);
+    offer<CL>(q, p, o1, e + 1u, pred, c, f);
+    offer<CL>(q, p, o2, e + 2u, pred, c, f);
+    offer<CL>(q, p, o3, e + 3u, pred, c, f);
+  }
+  for (; e <= hi; ++e) {
     const uint32_t t = e - shift;
     offer<CL>(q, p, make_float4(T.x[t], T.y[t], T.z[t], 0.0f), e, pred, c, f);
   }
