@@ -88,12 +88,14 @@ def run_gpu(args, sc, mapping="literal", profile=True):
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
     m.set_params(**prm)
     m.set_nb_jacobi_iters(sc.iters)
-    if args.block or args.flags or args.sort_bits:
-        m.set_tuning(args.block or 256, args.flags, args.sort_bits or 0)
     m.load_state(sc.pos, sc.vel)
     m.save_checkpoint()
-    m.sync()
     out = {}
+    if args.block or args.flags or args.sort_bits or args.queue:
+        m.set_tuning(args.block or 256, args.flags, args.sort_bits or 0, args.queue or 0)
+    elif not args.no_tune:
+        out["tuning"] = m.autotune(trial_steps=3)  # restores the input afterwards
+    m.sync()
 
     def fixed_steps(k):
         for _ in range(k):
@@ -257,10 +259,11 @@ def run_learned(args, sc):
     runs = [(f"plain-{k}", None, k, 0, 0.0) for k in plain]
     runs += [("learned-1", mlp, 1, 0, 0.0), ("learned-2", mlp, 2, 0, 0.0)]
     res = {v.name: v for v in learned.evaluate(sc, "stable", runs, steps=args.learn_steps,
-                                                warmup=5)}
+                                                warmup=5, tune=not args.no_tune)}
     tau = res["plain-4"].residual_mean
     res.update({v.name: v for v in learned.evaluate(
-        sc, "stable", [("learned-1+fallback", mlp, 1, 3, tau)], steps=args.learn_steps, warmup=5)})
+        sc, "stable", [("learned-1+fallback", mlp, 1, 3, tau)], steps=args.learn_steps, warmup=5,
+        tune=not args.no_tune)})
     mlp.close()
 
     def cheapest_plain_at(err, key):
@@ -306,6 +309,9 @@ def main():
     ap.add_argument("--flags", type=int, default=0,
                     help="launch flags: 1 XCD-contiguous blocks, 2 LDS wave tiles")
     ap.add_argument("--sort-bits", type=int, default=0)
+    ap.add_argument("--queue", type=int, default=0, help="survivor-queue depth (0 = 24)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the default launch shape instead of the autotuned one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-continuing", action="store_true",
@@ -394,6 +400,7 @@ def main():
             "substeps_per_frame": 2,
             "params": sc.params(),
             "step": "restore the dam-break input on the device, then one substep",
+            "launch_tuning": g.get("tuning"),
             "density_error_after_step": _err(g["density_error"]),
             "kernel_ms_per_step": {k: round(v[1] / prof_steps, 5) for k, v in sorted(prof.items())},
             "continuing_run": None if args.no_continuing else {

@@ -80,6 +80,22 @@ typedef struct {
   double totalMs;
 } rtp_kernel_time;
 
+/* Launch tuning (no reference counterpart).  The reference fixes its launch shapes (local
+ * size 512 for the sort, RadixSort.cpp:23-60, and the runtime's choice elsewhere).  Here
+ * the block size, launch flags, sort digit width and survivor-queue depth are chosen per
+ * workload by timing it (rtp_fluids_autotune, rtp_batch_autotune). */
+typedef struct {
+  int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
+  int32_t flags;      /* RTP_LAUNCH_* bits (single models; 0 for a batch) */
+  int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
+  int32_t queueDepth; /* neighbour survivors queued per lane in LDS, 8..64 (0 = default 24);
+                         queueDepth x blockSize x 4 bytes must stay within 64 KiB, 24 KiB
+                         with RTP_LAUNCH_LDS_TILES */
+  float msPerStep;    /* measured mean substep time of this choice (autotune output) */
+} rtp_tuning;
+#define RTP_LAUNCH_XCD_MAP 1   /* consecutive particle blocks on one XCD (shared L2) */
+#define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
+
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {
   RTP_FIELD_DENSITY = 0,   /* p_density  (float, last Jacobi iteration) */
@@ -150,10 +166,9 @@ int rtp_fluids_restore_checkpoint(rtp_fluids* h);
 /* Replay pairs of substeps from a captured hipGraph (no reference counterpart; the
  * reference enqueues every kernel each frame).  Default on. */
 int rtp_fluids_use_graph(rtp_fluids* h, int enable);
-/* Launch tuning (no reference counterpart): block size of the per-particle passes (64,
- * 128, 192 or 256), RTP_LAUNCH_* flags, sort digit width (0 = auto, 4..11).  Results do not
- * depend on any of them. */
-int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int flags, int sortBits);
+/* Launch tuning (no reference counterpart), see rtp_tuning (msPerStep is ignored).  Results
+ * do not depend on any of it. */
+int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t);
 
 /* ========================= Learned warm-start ========================= */
 /* No reference counterpart: an 8 -> 64 -> 64 -> 1 MLP (GELU, BF16 weights, FP32
@@ -200,19 +215,7 @@ int rtp_fluids_residual_error(rtp_fluids* h, float out[3]);
  * all iterations (the training target).  The model state advances by that substep. */
 int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat, float* target);
 
-/* ======================= Launch tuning (no reference counterpart) ==================
- * The reference fixes its launch shapes (local size 512 for the sort, RadixSort.cpp:23-60,
- * and the runtime's choice elsewhere).  Here the per-particle block size, the XCD-aware
- * block mapping and the sort's digit width are chosen per workload by timing it. */
-typedef struct {
-  int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
-  int32_t flags;      /* RTP_LAUNCH_* bits (single models; 0 for a batch) */
-  int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
-  float msPerStep;    /* measured mean substep time of this choice */
-} rtp_tuning;
-#define RTP_LAUNCH_XCD_MAP 1   /* consecutive particle blocks on one XCD (shared L2) */
-#define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
-
+/* ======================= Launch tuning (no reference counterpart) ================== */
 /* Times fixed-input substeps (the current state restored before each) for every candidate
  * launch shape, keeps the fastest and restores the state.  trialSteps substeps per
  * candidate (after one untimed); best (optional) receives the choice. */
@@ -253,8 +256,8 @@ int rtp_batch_sync(rtp_batch* h);
 int rtp_batch_save_checkpoint(rtp_batch* h);
 int rtp_batch_restore_checkpoint(rtp_batch* h);
 int rtp_batch_use_graph(rtp_batch* h, int enable);
-/* Block size and sort digit width as rtp_fluids_set_tuning (no launch flags). */
-int rtp_batch_set_tuning(rtp_batch* h, int blockSize, int sortBits);
+/* As rtp_fluids_set_tuning; flags must be 0. */
+int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t);
 int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
 int rtp_batch_enable_profiling(rtp_batch* h, int enable);
 int rtp_batch_profile(rtp_batch* h, rtp_kernel_time* out, int capacity, int* count);

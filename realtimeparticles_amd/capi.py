@@ -74,11 +74,19 @@ class KernelTime(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("blockSize", _i32), ("flags", _i32), ("sortBits", _i32), ("msPerStep", _f32)]
+    _fields_ = [("blockSize", _i32), ("flags", _i32), ("sortBits", _i32), ("queueDepth", _i32),
+                ("msPerStep", _f32)]
 
     def as_dict(self) -> dict:
         return {"block": int(self.blockSize), "flags": int(self.flags),
-                "sort_bits": int(self.sortBits), "ms_per_step": float(self.msPerStep)}
+                "sort_bits": int(self.sortBits), "queue": int(self.queueDepth),
+                "ms_per_step": float(self.msPerStep)}
+
+
+def _tuning(block, flags, sort_bits, queue) -> Tuning:
+    t = Tuning()
+    t.blockSize, t.flags, t.sortBits, t.queueDepth = int(block), int(flags), int(sort_bits), int(queue)
+    return t
 
 
 class BatchScene(ctypes.Structure):
@@ -123,7 +131,7 @@ _SIGS = {
     "rtp_fluids_save_checkpoint": ([_vp], _int),
     "rtp_fluids_restore_checkpoint": ([_vp], _int),
     "rtp_fluids_use_graph": ([_vp, _int], _int),
-    "rtp_fluids_set_tuning": ([_vp, _int, _int, _int], _int),
+    "rtp_fluids_set_tuning": ([_vp, ctypes.POINTER(Tuning)], _int),
     "rtp_mlp_create": ([_int, _u32, ctypes.POINTER(_vp)], _int),
     "rtp_mlp_destroy": ([_vp], _int),
     "rtp_mlp_param_count": ([], _int),
@@ -166,7 +174,7 @@ _SIGS = {
     "rtp_batch_save_checkpoint": ([_vp], _int),
     "rtp_batch_restore_checkpoint": ([_vp], _int),
     "rtp_batch_use_graph": ([_vp, _int], _int),
-    "rtp_batch_set_tuning": ([_vp, _int, _int], _int),
+    "rtp_batch_set_tuning": ([_vp, ctypes.POINTER(Tuning)], _int),
     "rtp_batch_autotune": ([_vp, _u32, ctypes.POINTER(Tuning)], _int),
     "rtp_batch_enable_profiling": ([_vp, _int], _int),
     "rtp_batch_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
@@ -376,10 +384,10 @@ class Fluids:
                                                target.ctypes.data), "record_teacher")
         return feat, target
 
-    def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0) -> None:
-        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES."""
-        _check(lib().rtp_fluids_set_tuning(self._h, int(block), int(flags), int(sort_bits)),
-               "set_tuning")
+    def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0, queue: int = 0) -> None:
+        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES; queue: survivor-queue depth (0 = 24)."""
+        t = _tuning(block, flags, sort_bits, queue)
+        _check(lib().rtp_fluids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, trial_steps: int = 5) -> dict:
         t = Tuning()
@@ -486,8 +494,9 @@ class Batch:
     def use_graph(self, flag: bool) -> None:
         _check(lib().rtp_batch_use_graph(self._h, int(bool(flag))), "use_graph")
 
-    def set_tuning(self, block: int = 256, sort_bits: int = 0) -> None:
-        _check(lib().rtp_batch_set_tuning(self._h, int(block), int(sort_bits)), "set_tuning")
+    def set_tuning(self, block: int = 256, sort_bits: int = 0, queue: int = 0) -> None:
+        t = _tuning(block, 0, sort_bits, queue)
+        _check(lib().rtp_batch_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, trial_steps: int = 5) -> dict:
         t = Tuning()

@@ -20,8 +20,6 @@ struct SceneHost {
   SceneSlot slot;
 };
 
-constexpr int kBlockSizes[] = { 64, 128, 192, 256 };
-
 }  // namespace
 }  // namespace rtp
 
@@ -51,6 +49,7 @@ struct rtp_batch {
   uint32_t numBlocks = 0;
   int block = 256;
   int sortBits = 0;
+  int queue = 24;
   bool constsDirty = true;
   int cur = 0;
   int predFinal = 1;
@@ -112,6 +111,7 @@ struct rtp_batch {
     bv.consts = dConsts;
     bv.numBlocks = numBlocks;
     bv.block = (uint32_t)block;
+    bv.qdepth = (uint32_t)queue;
     return bv;
   }
 
@@ -541,16 +541,17 @@ int rtp_batch_use_graph(rtp_batch* h, int enable)
   return RTP_OK;
 }
 
-int rtp_batch_set_tuning(rtp_batch* h, int blockSize, int sortBits)
+int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t)
 {
-  RTP_REQUIRE(h, "NULL handle");
-  RTP_REQUIRE(blockSize >= 64 && blockSize <= 256 && blockSize % 64 == 0,
-              "blockSize must be 64, 128, 192 or 256");
-  RTP_REQUIRE(sortBits == 0 || (sortBits >= 4 && sortBits <= 11), "sortBits must be 0 or 4..11");
+  RTP_REQUIRE(h && t, "NULL argument");
+  int rc = check_tuning(*t, 0);
+  if (rc)
+    return rc;
   RTP_HIP_TRY(hipSetDevice(h->device));
-  h->sortBits = sortBits;
-  if (h->block != blockSize) {
-    h->block = blockSize;
+  h->sortBits = t->sortBits;
+  h->queue = t->queueDepth ? t->queueDepth : 24;
+  if (h->block != t->blockSize) {
+    h->block = t->blockSize;
     return h->build_blocks();
   }
   h->drop_graph();
@@ -567,43 +568,24 @@ int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best)
     return rc;
   const bool profWas = h->prof.enabled;
   h->prof.enabled = false;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  RTP_HIP_TRY(hipEventCreate(&e0));
-  RTP_HIP_TRY(hipEventCreate(&e1));
-  rtp_tuning top{ h->block, 0, h->sortBits, 1e30f };
-  const int bitsChoices[] = { 7, 8, 9, 10, 11 };
-  for (int b : kBlockSizes) {
-    for (int bits : bitsChoices) {
-      if ((rc = rtp_batch_set_tuning(h, b, bits)))
-        break;
-      if ((rc = rtp_batch_restore_checkpoint(h)) || (rc = h->update(1)))
-        break;
-      (void)hipEventRecord(e0, h->stream);
-      for (uint32_t t = 0; t < trialSteps && !rc; ++t)
-        if (!(rc = rtp_batch_restore_checkpoint(h)))
-          rc = h->update(1);
-      (void)hipEventRecord(e1, h->stream);
-      if (rc)
-        break;
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        rc = fail(RTP_ERR_HIP, "autotune timing failed");
-        break;
-      }
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= (float)trialSteps;
-      if (ms < top.msPerStep)
-        top = rtp_tuning{ b, 0, bits, ms };
-    }
-    if (rc)
-      break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  rtp_tuning start{};
+  start.blockSize = h->block;
+  start.sortBits = h->sortBits;
+  start.queueDepth = h->queue;
+  auto run = [&](const rtp_tuning& t) { return rtp_batch_set_tuning(h, &t); };
+  auto steps = [&](uint32_t k) {
+    int r = 0;
+    for (uint32_t s = 0; s < k && !r; ++s)
+      if (!(r = rtp_batch_restore_checkpoint(h)))
+        r = h->update(1);
+    return r;
+  };
+  rtp_tuning top{};
+  rc = tune_search(h->stream, trialSteps, start, false, run, steps, &top);
   h->prof.enabled = profWas;
   if (rc)
     return rc;
-  if ((rc = rtp_batch_set_tuning(h, top.blockSize, top.sortBits)) || (rc = rtp_batch_restore_checkpoint(h)))
+  if ((rc = run(top)) || (rc = rtp_batch_restore_checkpoint(h)))
     return rc;
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   if (best)

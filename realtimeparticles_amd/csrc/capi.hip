@@ -818,19 +818,29 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable)
   return RTP_OK;
 }
 
-int rtp_fluids_set_tuning(rtp_fluids* h, int blockSize, int flags, int sortBits)
+int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
 {
-  RTP_REQUIRE(h, "NULL handle");
-  RTP_REQUIRE(blockSize >= 64 && blockSize <= 256 && blockSize % 64 == 0,
-              "blockSize must be 64, 128, 192 or 256");
-  RTP_REQUIRE(sortBits == 0 || (sortBits >= 4 && sortBits <= 11), "sortBits must be 0 or 4..11");
-  h->L.block = blockSize;
-  RTP_REQUIRE(flags >= 0 && flags <= 3, "flags must be a combination of XCD mapping (1) and LDS tiles (2)");
-  h->L.xcdRemap = flags & 1;
-  h->L.tile = (flags >> 1) & 1;
-  h->grid.sortBits = sortBits;
+  RTP_REQUIRE(h && t, "NULL argument");
+  const int rc = check_tuning(*t, 3);
+  if (rc)
+    return rc;
+  h->L.block = t->blockSize;
+  h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
+  h->L.tile = (t->flags & RTP_LAUNCH_LDS_TILES) ? 1 : 0;
+  h->L.queue = t->queueDepth ? t->queueDepth : 24;
+  h->grid.sortBits = t->sortBits;
   h->drop_graph();
   return RTP_OK;
+}
+
+rtp_tuning current_tuning(const rtp_fluids* h)
+{
+  rtp_tuning t{};
+  t.blockSize = h->L.block;
+  t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0);
+  t.sortBits = h->grid.sortBits;
+  t.queueDepth = h->L.queue;
+  return t;
 }
 
 int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best)
@@ -845,48 +855,22 @@ int rtp_fluids_autotune(rtp_fluids* h, uint32_t trialSteps, rtp_tuning* best)
   h->prof.enabled = false;
   const bool pausedWas = h->paused;
   h->paused = false;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  RTP_HIP_TRY(hipEventCreate(&e0));
-  RTP_HIP_TRY(hipEventCreate(&e1));
-  rtp_tuning top{ h->L.block, h->L.xcdRemap | (h->L.tile << 1), h->grid.sortBits, 1e30f };
-  const int blocks[] = { 64, 128, 192, 256 };
-  const int bitsChoices[] = { 7, 8, 9, 10, 11 };
-  for (int b : blocks) {
-    for (int x = 0; x < 4 && !rc; ++x) {
-      for (int bits : bitsChoices) {
-        if ((rc = rtp_fluids_set_tuning(h, b, x, bits)))
-          break;
-        if ((rc = rtp_fluids_restore_checkpoint(h)) || (rc = h->update(1)))
-          break;
-        (void)hipEventRecord(e0, h->stream);
-        for (uint32_t t = 0; t < trialSteps && !rc; ++t)
-          if (!(rc = rtp_fluids_restore_checkpoint(h)))
-            rc = h->update(1);
-        (void)hipEventRecord(e1, h->stream);
-        if (rc)
-          break;
-        if (hipEventSynchronize(e1) != hipSuccess) {
-          rc = fail(RTP_ERR_HIP, "autotune timing failed");
-          break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)trialSteps;
-        if (ms < top.msPerStep)
-          top = rtp_tuning{ b, x, bits, ms };
-      }
-    }
-    if (rc)
-      break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  const rtp_tuning start = current_tuning(h);
+  auto run = [&](const rtp_tuning& t) { return rtp_fluids_set_tuning(h, &t); };
+  auto steps = [&](uint32_t k) {
+    int r = 0;
+    for (uint32_t s = 0; s < k && !r; ++s)
+      if (!(r = rtp_fluids_restore_checkpoint(h)))
+        r = h->update(1);
+    return r;
+  };
+  rtp_tuning top{};
+  rc = tune_search(h->stream, trialSteps, start, true, run, steps, &top);
   h->prof.enabled = profWas;
   h->paused = pausedWas;
   if (rc)
     return rc;
-  if ((rc = rtp_fluids_set_tuning(h, top.blockSize, top.flags, top.sortBits)) ||
-      (rc = rtp_fluids_restore_checkpoint(h)))
+  if ((rc = run(top)) || (rc = rtp_fluids_restore_checkpoint(h)))
     return rc;
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   if (best)

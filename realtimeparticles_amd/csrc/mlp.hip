@@ -35,16 +35,22 @@ __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32
 constexpr float kGeluA = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluB = 0.044715f;
 
-__device__ __forceinline__ float gelu(float z)
+// tanh-form GELU, 0.5 z (1 + tanh u) with u = sqrt(2/pi) (z + 0.044715 z^3), evaluated as
+// z * sigmoid(2u) = z / (1 + exp(-2u)) with the hardware exponential and reciprocal (about
+// 1e-6 relative; the network is trained with these same functions).
+__device__ __forceinline__ float sigmoid2u(float z)
 {
-  const float t = tanhf(kGeluA * (z + kGeluB * z * z * z));
-  return 0.5f * z * (1.0f + t);
+  const float u = kGeluA * (z + kGeluB * (z * z) * z);
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
+__device__ __forceinline__ float gelu(float z) { return z * sigmoid2u(z); }
+
+// d/dz [z sigmoid(2u)] = s + z s (1 - s) 2 u'(z),  u' = sqrt(2/pi) (1 + 3 * 0.044715 z^2)
 __device__ __forceinline__ float gelu_grad(float z)
 {
-  const float t = tanhf(kGeluA * (z + kGeluB * z * z * z));
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluA * (1.0f + 3.0f * kGeluB * z * z);
+  const float sg = sigmoid2u(z);
+  return sg + z * sg * (1.0f - sg) * (2.0f * kGeluA) * (1.0f + 3.0f * kGeluB * z * z);
 }
 
 // Four consecutive bf16 (8-byte aligned) as one operand register.
@@ -81,14 +87,19 @@ __device__ __forceinline__ bool load_inputs(const float* __restrict__ feat, uint
 __device__ __forceinline__ void layer1(const MlpDev& m, short (*X)[16], short (*H1)[64], float (*Z1)[64])
 {
   const uint32_t l = threadIdx.x, lr = l % 16u, lq = l / 16u;
+  float b1[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+    b1[nt] = m.p[kMlpB1 + nt * 16 + lr];
   for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
       f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
       acc = mfma(ld4(&X[mt * 16 + lr][4 * lq]), ld4(m.w1t + (nt * 16 + lr) * 16 + 4 * lq), acc);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = mt * 16 + 4 * lq + j, col = nt * 16 + lr;
-        const float z = acc[j] + m.p[kMlpB1 + col];
+        const float z = acc[j] + b1[nt];
         if (Z1)
           Z1[row][col] = z;
         H1[row][col] = f2bf(gelu(z));
@@ -117,32 +128,66 @@ __device__ __forceinline__ float sum16(float v)
   return v;
 }
 
+// Inference: each wave keeps its lanes' weight operands (W1, W2 fragments, biases, head)
+// in registers and runs 64-row tiles in a grid-stride loop.
 __global__ __launch_bounds__(64) void k_mlp_forward(const float* __restrict__ feat, uint32_t n, MlpDev m,
                                                     float* __restrict__ out)
 {
   __shared__ WaveLds s;
-  const uint32_t row0 = blockIdx.x * 64u;
-  (void)load_inputs(feat, row0, n, m, s.X, nullptr);
-  __syncthreads();
-  layer1(m, s.X, s.H1, nullptr);
-  __syncthreads();
   const uint32_t l = threadIdx.x, lr = l % 16u, lq = l / 16u;
-  for (int mt = 0; mt < 4; ++mt) {
-    float y[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-    for (int nt = 0; nt < 4; ++nt) {
-      const f32x4 acc = layer2_tile(m, s.H1, mt, nt);
-      const int col = nt * 16 + lr;
+  bf16x4 w1f[4], w2f[4][4];
+  float b1[4], b2[4], w3[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        y[j] += gelu(acc[j] + m.p[kMlpB2 + col]) * m.p[kMlpW3 + col];
-    }
+  for (int nt = 0; nt < 4; ++nt) {
+    w1f[nt] = ld4(m.w1t + (nt * 16 + lr) * 16 + 4 * lq);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float v = sum16(y[j]);
-      const uint32_t r = row0 + mt * 16 + 4 * lq + j;
-      if (lr == 0 && r < n)
-        out[r] = (v + m.p[kMlpB3]) * m.outScale;
+    for (int kt = 0; kt < 4; ++kt)
+      w2f[nt][kt] = ld4(m.w2t + (nt * 16 + lr) * 64 + kt * 16 + 4 * lq);
+    b1[nt] = m.p[kMlpB1 + nt * 16 + lr];
+    b2[nt] = m.p[kMlpB2 + nt * 16 + lr];
+    w3[nt] = m.p[kMlpW3 + nt * 16 + lr];
+  }
+  const float b3 = m.p[kMlpB3];
+  for (uint32_t row0 = blockIdx.x * 64u; row0 < n; row0 += gridDim.x * 64u) {
+    (void)load_inputs(feat, row0, n, m, s.X, nullptr);
+    __syncthreads();
+    for (int mt = 0; mt < 4; ++mt) {
+      const bf16x4 xa = ld4(&s.X[mt * 16 + lr][4 * lq]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+        acc = mfma(xa, w1f[nt], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          s.H1[mt * 16 + 4 * lq + j][nt * 16 + lr] = f2bf(gelu(acc[j] + b1[nt]));
+      }
     }
+    __syncthreads();
+    for (int mt = 0; mt < 4; ++mt) {
+      bf16x4 ha[4];
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt)
+        ha[kt] = ld4(&s.H1[mt * 16 + lr][kt * 16 + 4 * lq]);
+      float y[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+          acc = mfma(ha[kt], w2f[nt][kt], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          y[j] += gelu(acc[j] + b2[nt]) * w3[nt];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float v = sum16(y[j]);
+        const uint32_t r = row0 + mt * 16 + 4 * lq + j;
+        if (lr == 0 && r < n)
+          out[r] = (v + b3) * m.outScale;
+      }
+    }
+    __syncthreads();  // the next tile overwrites X and H1
   }
 }
 
@@ -349,7 +394,8 @@ void launch_mlp_forward(const float* feat, uint32_t n, const MlpDev& m, float* o
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for(n, 64)), dim3(64), 0, s, feat, n, m, out);
+  const uint32_t tiles = blocks_for(n, 64);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(tiles < 4096u ? tiles : 4096u), dim3(64), 0, s, feat, n, m, out);
 }
 
 void launch_mlp_train(const float* feat, const float* target, const uint32_t* perm, uint32_t n,

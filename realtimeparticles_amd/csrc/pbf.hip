@@ -29,12 +29,10 @@ constexpr float kFarD2 = 1.0e12f;
 
 // Candidates that survive the distance test are queued per lane in LDS and handed to the
 // pass body afterwards, so the expensive body runs densely over each lane's survivors
-// instead of once per candidate slot of the whole wave.  Survivors keep their order.
-#ifndef RTP_QUEUE_DEPTH
-#define RTP_QUEUE_DEPTH 24
-#endif
-constexpr uint32_t kQueue = RTP_QUEUE_DEPTH;  // queued candidates per lane
-constexpr uint32_t kMaxBlock = 256; // threads per block the LDS queue is laid out for
+// instead of once per candidate slot of the whole wave.  Survivors keep their order.  The
+// queue is the kernel's dynamic LDS, depth x block size words; its depth is a launch
+// parameter (deeper queues densify more, shallower ones leave room for more waves).
+constexpr uint32_t kMaxBlock = 256; // threads per block
 
 // fld_applyBoundaryCondition (fluids.cl:435-439) applied to a loaded position.
 __device__ __forceinline__ float4 clamp_pos(float4 q, const PbfConsts& c)
@@ -53,9 +51,10 @@ __device__ __forceinline__ float4 ldp(const float4* __restrict__ pred, uint32_t 
 }
 
 struct LaneQueue {
-  uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * kMaxBlock]
+  uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * stride]
   uint32_t count;
   uint32_t cap;    // entries per lane
+  uint32_t stride; // threads per block
 };
 
 template <bool CL, class F>
@@ -63,7 +62,7 @@ __device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4
                                       const PbfConsts& c, F& f)
 {
   for (uint32_t k = 0; k < q.count; ++k) {
-    const uint32_t e = q.slot[k * kMaxBlock];
+    const uint32_t e = q.slot[k * q.stride];
     const float4 o = ldp<CL>(pred, e, c);
     const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
@@ -81,7 +80,7 @@ __device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4
   const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
   if (!(d2 >= c.cullSq && d2 < kFarD2)) {
-    q.slot[q.count * kMaxBlock] = e;
+    q.slot[q.count * q.stride] = e;
     if (++q.count == q.cap)
       drain<CL>(q, p, pred, c, f);
   }
@@ -117,11 +116,7 @@ constexpr uint32_t kTileCols = 16;
 #ifndef RTP_TILE_CAP
 #define RTP_TILE_CAP 768
 #endif
-#ifndef RTP_TILE_QUEUE
-#define RTP_TILE_QUEUE 12
-#endif
 constexpr uint32_t kTileCap = RTP_TILE_CAP;      // staged candidates per wave
-constexpr uint32_t kTileQueue = RTP_TILE_QUEUE;  // survivor-queue depth of the tiled passes
 constexpr uint32_t kWaves = kMaxBlock / 64;
 
 // What a neighbour walk works with besides the particle arrays: its lane's survivor queue
@@ -300,7 +295,7 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
                                        const uint2* __restrict__ se, const PbfConsts& c, F&& f,
                                        const WalkCtx& T)
 {
-  LaneQueue q{ T.qslot, 0u, T.qcap };
+  LaneQueue q{ T.qslot, 0u, T.qcap, blockDim.x };
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
@@ -384,11 +379,11 @@ __device__ __forceinline__ uint32_t particle_index(int remap)
 // ---------------------------------------------------------------------------- kernels
 
 // The survivor queue of an untiled neighbour pass.
-#define RTP_PLAIN_CTX                                        \
-  __shared__ uint32_t rtpQueue[kQueue * kMaxBlock];          \
+#define RTP_PLAIN_CTX(DEPTH)                                 \
+  extern __shared__ uint32_t rtpQueue[];                     \
   WalkCtx W;                                                 \
   W.qslot = &rtpQueue[threadIdx.x];                          \
-  W.qcap = kQueue;
+  W.qcap = (DEPTH);
 
 // fluids.cl:71-73 (newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt) and
 // fillCellIDs (grid.cl:76-86) on predPos, the key offset by the scene's key base.
@@ -502,10 +497,10 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
                                                         const uint2* __restrict__ se,
                                                         float* __restrict__ density,
                                                         float* __restrict__ lambda, uint32_t n,
-                                                        PbfConsts c, int remap, LearnArgs la)
+                                                        PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
 {
   const uint32_t i = particle_index(remap);
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(qdepth)
   if (i < n)
     density_lambda_body<MODE>(i, pred, se, density, lambda, c, la, W);
 }
@@ -584,10 +579,10 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
                                                  const float4* __restrict__ pos,
                                                  float4* __restrict__ vel,
                                                  float4* __restrict__ posOut, uint32_t n,
-                                                 int last, PbfConsts c, int remap, LearnArgs la)
+                                                 int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
 {
   const uint32_t i = particle_index(remap);
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(qdepth)
   if (i < n)
     correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W);
 }
@@ -650,10 +645,10 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
                                                    const uint2* __restrict__ se,
                                                    const float4* __restrict__ vel,
                                                    float4* __restrict__ vort, uint32_t n,
-                                                   PbfConsts c, int remap)
+                                                   PbfConsts c, int remap, uint32_t qdepth)
 {
   const uint32_t i = particle_index(remap);
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(qdepth)
   if (i < n)
     vorticity_body(i, pred, se, vel, vort, c, W);
 }
@@ -701,10 +696,10 @@ __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ 
                                                      const float4* __restrict__ vort,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ velOut, uint32_t n,
-                                                     PbfConsts c, int remap)
+                                                     PbfConsts c, int remap, uint32_t qdepth)
 {
   const uint32_t i = particle_index(remap);
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(qdepth)
   if (i < n)
     confinement_body(i, pred, se, vort, vel, velOut, c, W);
 }
@@ -738,10 +733,10 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
                                               const float4* __restrict__ velIn,
                                               float4* __restrict__ velOut,
                                               float4* __restrict__ posOut, uint32_t n,
-                                              PbfConsts c, int remap)
+                                              PbfConsts c, int remap, uint32_t qdepth)
 {
   const uint32_t i = particle_index(remap);
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(qdepth)
   if (i < n)
     xsph_body(i, pred, se, velIn, velOut, posOut, c, W);
 }
@@ -751,13 +746,13 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
 // past n stand in for the last particle), then only real particles run the pass.
 #define RTP_TILED_PROLOGUE(CLV)                                                \
   __shared__ TileLds tl;                                                       \
-  __shared__ uint32_t rtpQueue[kTileQueue * kMaxBlock];                        \
+  extern __shared__ uint32_t rtpQueue[];                                       \
   const uint32_t i0 = particle_index(remap);                                   \
   const bool active = i0 < n;                                                  \
   const uint32_t i = active ? i0 : n - 1u;                                     \
   WalkCtx T;                                                                   \
   T.qslot = &rtpQueue[threadIdx.x];                                            \
-  T.qcap = kTileQueue;                                                         \
+  T.qcap = qdepth;                                                             \
   build_tile<CLV>(ldp<CLV>(pred, i, c), pred, se, c, tl, T);                   \
   __syncthreads();
 
@@ -766,7 +761,7 @@ __global__ __launch_bounds__(256) void k_density_lambda_t(const float4* __restri
                                                           const uint2* __restrict__ se,
                                                           float* __restrict__ density,
                                                           float* __restrict__ lambda, uint32_t n,
-                                                          PbfConsts c, int remap, LearnArgs la)
+                                                          PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
 {
   RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
   if (active)
@@ -781,7 +776,7 @@ __global__ __launch_bounds__(256) void k_correct_t(const float4* __restrict__ pr
                                                    const float4* __restrict__ pos,
                                                    float4* __restrict__ vel,
                                                    float4* __restrict__ posOut, uint32_t n,
-                                                   int last, PbfConsts c, int remap, LearnArgs la)
+                                                   int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
 {
   RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
   if (active)
@@ -792,7 +787,7 @@ __global__ __launch_bounds__(256) void k_vorticity_t(const float4* __restrict__ 
                                                      const uint2* __restrict__ se,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ vort, uint32_t n,
-                                                     PbfConsts c, int remap)
+                                                     PbfConsts c, int remap, uint32_t qdepth)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -804,7 +799,7 @@ __global__ __launch_bounds__(256) void k_confinement_t(const float4* __restrict_
                                                        const float4* __restrict__ vort,
                                                        const float4* __restrict__ vel,
                                                        float4* __restrict__ velOut, uint32_t n,
-                                                       PbfConsts c, int remap)
+                                                       PbfConsts c, int remap, uint32_t qdepth)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -816,7 +811,7 @@ __global__ __launch_bounds__(256) void k_xsph_t(const float4* __restrict__ pred,
                                                 const float4* __restrict__ velIn,
                                                 float4* __restrict__ velOut,
                                                 float4* __restrict__ posOut, uint32_t n,
-                                                PbfConsts c, int remap)
+                                                PbfConsts c, int remap, uint32_t qdepth)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -877,7 +872,7 @@ __global__ __launch_bounds__(256) void k_density_lambda_batch(BatchView bv,
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(bv.qdepth)
   density_lambda_body<0>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs(), W);
 }
 
@@ -894,7 +889,7 @@ __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(bv.qdepth)
   correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
                   posOut ? posOut + o : nullptr, last, c, LearnArgs(), W);
 }
@@ -922,7 +917,7 @@ __global__ __launch_bounds__(256) void k_vorticity_batch(BatchView bv, const flo
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(bv.qdepth)
   vorticity_body(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c, W);
 }
 
@@ -938,7 +933,7 @@ __global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(bv.qdepth)
   confinement_body(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c, W);
 }
 
@@ -953,7 +948,7 @@ __global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* 
     return;
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
-  RTP_PLAIN_CTX
+  RTP_PLAIN_CTX(bv.qdepth)
   xsph_body(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c, W);
 }
 
@@ -1030,6 +1025,9 @@ __global__ __launch_bounds__(256) void k_density_error_final(const float* __rest
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
+// Dynamic LDS of a neighbour pass: the survivor queue, depth x block size words.
+inline uint32_t queue_bytes(const Launch& L) { return (uint32_t)(L.queue * L.block) * 4u; }
+
 }  // namespace
 
 void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint32_t* key,
@@ -1049,14 +1047,15 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
   if (n == 0)
     return;
   const dim3 g(blocks_for(n, L.block)), b(L.block);
+  const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_DL(M)                                                                              \
   case M:                                                                                    \
     if (L.tile)                                                                              \
-      hipLaunchKernelGGL(k_density_lambda_t<M>, g, b, 0, s, pred, startEnd, density, lambda, n, \
-                         c, L.xcdRemap, la);                                                 \
+      hipLaunchKernelGGL(k_density_lambda_t<M>, g, b, lds, s, pred, startEnd, density, lambda, \
+                         n, c, L.xcdRemap, la, q);                                           \
     else                                                                                     \
-      hipLaunchKernelGGL(k_density_lambda<M>, g, b, 0, s, pred, startEnd, density, lambda, n, \
-                         c, L.xcdRemap, la);                                                 \
+      hipLaunchKernelGGL(k_density_lambda<M>, g, b, lds, s, pred, startEnd, density, lambda, \
+                         n, c, L.xcdRemap, la, q);                                           \
     break;
   switch (mode) {
     RTP_DL(0)
@@ -1079,14 +1078,15 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
   if (n == 0)
     return;
   const dim3 g(blocks_for(n, L.block)), b(L.block);
+  const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_CO(M)                                                                              \
   case M:                                                                                    \
     if (L.tile)                                                                              \
-      hipLaunchKernelGGL(k_correct_t<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel, \
-                         posOut, n, last, c, L.xcdRemap, la);                                \
+      hipLaunchKernelGGL(k_correct_t<M>, g, b, lds, s, pred, lambda, startEnd, predOut, pos,   \
+                         vel, posOut, n, last, c, L.xcdRemap, la, q);                        \
     else                                                                                     \
-      hipLaunchKernelGGL(k_correct<M>, g, b, 0, s, pred, lambda, startEnd, predOut, pos, vel,  \
-                         posOut, n, last, c, L.xcdRemap, la);                                \
+      hipLaunchKernelGGL(k_correct<M>, g, b, lds, s, pred, lambda, startEnd, predOut, pos,     \
+                         vel, posOut, n, last, c, L.xcdRemap, la, q);                        \
     break;
   switch (mode) {
     RTP_CO(0)
@@ -1113,7 +1113,8 @@ void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float
   if (n == 0)
     return;
   hipLaunchKernelGGL(L.tile ? k_vorticity_t : k_vorticity, dim3(blocks_for(n, L.block)),
-                     dim3(L.block), 0, s, pred, startEnd, vel, vort, n, c, L.xcdRemap);
+                     dim3(L.block), queue_bytes(L), s, pred, startEnd, vel, vort, n, c,
+                     L.xcdRemap, (uint32_t)L.queue);
 }
 
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
@@ -1123,7 +1124,8 @@ void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const flo
   if (n == 0)
     return;
   hipLaunchKernelGGL(L.tile ? k_confinement_t : k_confinement, dim3(blocks_for(n, L.block)),
-                     dim3(L.block), 0, s, pred, startEnd, vort, vel, velOut, n, c, L.xcdRemap);
+                     dim3(L.block), queue_bytes(L), s, pred, startEnd, vort, vel, velOut, n, c,
+                     L.xcdRemap, (uint32_t)L.queue);
 }
 
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
@@ -1132,8 +1134,9 @@ void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* ve
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(L.tile ? k_xsph_t : k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block), 0,
-                     s, pred, startEnd, velIn, velOut, posOut, n, c, L.xcdRemap);
+  hipLaunchKernelGGL(L.tile ? k_xsph_t : k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block),
+                     queue_bytes(L), s, pred, startEnd, velIn, velOut, posOut, n, c, L.xcdRemap,
+                     (uint32_t)L.queue);
 }
 
 void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,
@@ -1160,16 +1163,23 @@ void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s)
     hipLaunchKernelGGL(k_fill_start_end, dim3(blocks_for(count, 256)), dim3(256), 0, s, startEnd, count);
 }
 
-#define RTP_BATCH_LAUNCH(kernel, ...)                                                          \
+#define RTP_BATCH_STREAM(kernel, ...)                                                          \
   do {                                                                                       \
     if (bv.numBlocks)                                                                        \
       hipLaunchKernelGGL(kernel, dim3(bv.numBlocks), dim3(bv.block), 0, s, bv, __VA_ARGS__); \
   } while (0)
 
+#define RTP_BATCH_LAUNCH(kernel, ...)                                                          \
+  do {                                                                                       \
+    if (bv.numBlocks)                                                                        \
+      hipLaunchKernelGGL(kernel, dim3(bv.numBlocks), dim3(bv.block),                         \
+                         bv.qdepth * bv.block * 4u, s, bv, __VA_ARGS__);                     \
+  } while (0)
+
 void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,
                               float4* pred, uint32_t* key, hipStream_t s)
 {
-  RTP_BATCH_LAUNCH(k_predict_batch, pos, vel, pred, key);
+  RTP_BATCH_STREAM(k_predict_batch, pos, vel, pred, key);
 }
 
 void launch_pbf_density_lambda_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
@@ -1188,7 +1198,7 @@ void launch_pbf_correct_batch(const BatchView& bv, const float4* pred, const flo
 void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const float4* pos,
                                  float4* vel, float4* posOut, hipStream_t s)
 {
-  RTP_BATCH_LAUNCH(k_update_vel_batch, pred, pos, vel, posOut);
+  RTP_BATCH_STREAM(k_update_vel_batch, pred, pos, vel, posOut);
 }
 
 void launch_pbf_vorticity_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
@@ -1211,5 +1221,6 @@ void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2*
 }
 
 #undef RTP_BATCH_LAUNCH
+#undef RTP_BATCH_STREAM
 
 }  // namespace rtp

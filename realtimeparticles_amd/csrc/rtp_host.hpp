@@ -222,6 +222,89 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
   return RTP_OK;
 }
 
+// ----------------------------------------------------------------------------- tuning
+
+// Validates a launch tuning; allowedFlags masks the RTP_LAUNCH_* bits the caller supports.
+int check_tuning(const rtp_tuning& t, int allowedFlags)
+{
+  RTP_REQUIRE(t.blockSize >= 64 && t.blockSize <= 256 && t.blockSize % 64 == 0,
+              "blockSize must be 64, 128, 192 or 256");
+  RTP_REQUIRE((t.flags & ~allowedFlags) == 0, "launch flags not supported here");
+  RTP_REQUIRE(t.sortBits == 0 || (t.sortBits >= 4 && t.sortBits <= 11), "sortBits must be 0 or 4..11");
+  const int q = t.queueDepth ? t.queueDepth : 24;
+  RTP_REQUIRE(q >= 8 && q <= 64, "queueDepth must be 0 or 8..64");
+  const int limit = (t.flags & RTP_LAUNCH_LDS_TILES) ? 24 * 1024 : 64 * 1024;
+  RTP_REQUIRE(q * t.blockSize * 4 <= limit, "queueDepth x blockSize exceeds the LDS budget");
+  return RTP_OK;
+}
+
+// Times fixed-input substeps for candidate tunings and returns the fastest in *best.
+// set(t) applies a tuning, steps(k) runs k fixed-input substeps.  The sort's digit width
+// only affects the sort, so it is chosen first; then every valid combination of block size,
+// launch flags (single models) and queue depth is timed with it.
+template <class Set, class Steps>
+int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, bool singleModel,
+                Set set, Steps steps, rtp_tuning* best)
+{
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RTP_HIP_TRY(hipEventCreate(&e0));
+  RTP_HIP_TRY(hipEventCreate(&e1));
+  int rc = RTP_OK;
+  auto time_one = [&](const rtp_tuning& t, float* ms) -> int {
+    int r = set(t);
+    if (r || (r = steps(1)))
+      return r;
+    (void)hipEventRecord(e0, s);
+    if ((r = steps(trialSteps)))
+      return r;
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+      return fail(RTP_ERR_HIP, "autotune timing failed");
+    *ms /= (float)trialSteps;
+    return RTP_OK;
+  };
+  rtp_tuning top = start;
+  top.msPerStep = 1e30f;
+  for (int bits : { 7, 8, 9, 10, 11 }) {
+    rtp_tuning t = start;
+    t.sortBits = bits;
+    float ms = 0.0f;
+    if (check_tuning(t, singleModel ? 3 : 0))
+      continue;
+    if ((rc = time_one(t, &ms)))
+      break;
+    if (ms < top.msPerStep) {
+      top = t;
+      top.msPerStep = ms;
+    }
+  }
+  const int flagChoices = singleModel ? 4 : 1;
+  for (int b = 64; b <= 256 && !rc; b += 64)
+    for (int f = 0; f < flagChoices && !rc; ++f)
+      for (int q : { 16, 24, 32, 48, 64 }) {
+        rtp_tuning t = top;
+        t.blockSize = b;
+        t.flags = f;
+        t.queueDepth = q;
+        if (check_tuning(t, singleModel ? 3 : 0))
+          continue;
+        float ms = 0.0f;
+        if ((rc = time_one(t, &ms)))
+          break;
+        if (ms < top.msPerStep) {
+          top = t;
+          top.msPerStep = ms;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc)
+    return rc;
+  fail(RTP_OK, "");  // candidates rejected by check_tuning leave no error behind
+  *best = top;
+  return RTP_OK;
+}
+
 // Fluids::createProgram constants (Fluids.cpp:97-126) of a scene with box edge boxX split
 // into resX cells; the rest of PbfConsts follows from the parameters (refresh_pbf_consts).
 void init_pbf_consts(PbfConsts& c, const GridConsts& g, float boxX, int32_t resX, float gravity)

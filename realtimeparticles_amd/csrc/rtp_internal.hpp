@@ -63,6 +63,7 @@ struct Launch {
   int block = 256;     // threads per block
   int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD
   int tile = 0;        // 1: neighbour passes stage their candidates in LDS wave tiles
+  int queue = 24;      // survivor-queue depth of the neighbour passes (entries per lane)
 };
 
 // ---------------------------------------------------------------------------- grid / sort
@@ -136,6 +137,7 @@ struct BatchView {
   const uint2* blocks;
   const PbfConsts* consts;
   uint32_t numBlocks, block;
+  uint32_t qdepth;  // survivor-queue depth of the neighbour passes
 };
 void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s);
 void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,

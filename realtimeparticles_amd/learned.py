@@ -88,10 +88,12 @@ class Variant:
     residual_compression: float
 
 
-def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup: int = 3):
+def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup: int = 3,
+             tune: bool = False):
     """Times fixed-input substeps (restore the input, one substep) for each run spec, then
     runs one more with the post-correction density pass to read its error.  The timed
-    substeps carry that pass only where the fallback gate needs it.
+    substeps carry that pass only where the fallback gate needs it.  With `tune`, the
+    launch shape is autotuned once on the plain solver and used for every run.
 
     runs: list of (name, mlp or None, iterations, fallback_iters, tau).
     """
@@ -100,6 +102,9 @@ def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup:
     m.set_params(**sc.params(mapping))
     m.load_state(sc.pos, sc.vel)
     m.save_checkpoint()
+    if tune:
+        m.set_nb_jacobi_iters(sc.iters)
+        m.autotune(trial_steps=3)
     for name, mlp, iters, fb, tau in runs:
         if mlp is None:
             m.set_warm_start(None)

@@ -104,9 +104,9 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
     log("graph replay of 4 substeps")
     b.update(4)
     ref = [b.read_state(k) for k in range(len(scs))]
-    for block, bits in ((64, 7), (192, 11), (128, 0)):
+    for block, bits, queue in ((64, 7, 64), (192, 11, 16), (128, 0, 48)):
         b.restore_checkpoint()
-        b.set_tuning(block, bits)
+        b.set_tuning(block, bits, queue)
         b.use_graph(False)
         log(f"direct launches, block {block}, sort bits {bits}")
         for _ in range(4):
@@ -120,6 +120,7 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
     best = b.autotune(trial_steps=2)
     log(f"autotuned: {best}")
     assert best["block"] in (64, 128, 192, 256) and 7 <= best["sort_bits"] <= 11
+    assert best["queue"] in (16, 24, 32, 48, 64)
     b.update(4)
     for k in range(len(scs)):
         _eq(b.read_state(k)[0], ref[k][0], f"scene {k} pos after autotune")
@@ -170,6 +171,8 @@ def test_batch_argument_errors():
         b.set_params(1, isVorticityConfEnabled=0)  # must agree across scenes
     with pytest.raises(capi.RtpError):
         b.set_tuning(100, 0)
+    with pytest.raises(capi.RtpError):
+        b.set_tuning(256, 0, 128)  # queue past the LDS budget
     with pytest.raises(capi.RtpError):
         b.restore_checkpoint()
     b.close()

@@ -177,17 +177,20 @@ def test_checkpoint_restore_replays_identically():
     gpu.close()
 
 
-@pytest.mark.parametrize("block,flags", [(256, 2), (128, 2), (64, 3), (192, 1), (256, 3)])
+@pytest.mark.parametrize("block,flags,queue", [(256, 2, 16), (128, 2, 24), (64, 3, 48),
+                                               (192, 1, 64), (256, 3, 16), (256, 0, 64),
+                                               (128, 0, 8)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
-def test_launch_shapes_do_not_change_bits(block, flags, mapping):
-    """XCD-contiguous block order and LDS wave tiles change where candidates are read
-    from, never which candidates or in which order."""
+def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
+    """XCD-contiguous block order, LDS wave tiles and the survivor-queue depth change where
+    candidates are read from and when the neighbour bodies run, never which candidates or
+    in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
     prm = sc.params(mapping)
     gpu, orc = make_pbf_pair(sc, iters=3, **{k: v for k, v in prm.items()})
-    gpu.set_tuning(block, flags, 0)
+    gpu.set_tuning(block, flags, 0, queue)
     for step in range(1, 5):
-        log(f"block {block} flags {flags} {mapping} step {step}")
+        log(f"block {block} flags {flags} queue {queue} {mapping} step {step}")
         gpu.update(1)
         orc.step(1)
         compare_pbf(gpu, orc, sc.n, step)
