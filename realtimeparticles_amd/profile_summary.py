@@ -9,10 +9,11 @@ Inputs are the output directories of up to four runs of the same command:
   --sq DIR      rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES ...
 
 FETCH_SIZE and WRITE_SIZE are KiB per dispatch.  Each is scaled by what the same counter
-reads for the runtime's device-to-device copy kernel (__amd_rocclr_copyBuffer), whose
-bytes are known: the checkpoint restore in front of every fixed-input step copies
-positions, velocities and predicted positions (16 B each) and the cell keys (4 B) of
-max_n particles, i.e. 52 * max_n bytes read and written in four dispatches.
+reads for the runtime's copy kernel (__amd_rocclr_copyBuffer) on copies of known size: the
+model's copies are whole float4 buffers (16 B x max_n: positions, velocities, predicted
+positions) or key buffers (4 B x max_n).  The copy dispatches are split at the largest gap
+of their counter values into those two size classes, and the scale is the counter total
+over the byte total.
 
     python -m realtimeparticles_amd.profile_summary --trace D1 --fetch D2 --write D3 \\
         --max-n 1049088 --command "..." --md out.md --json out.json
@@ -91,12 +92,25 @@ def counters(pmc_dir: str, counter: str):
 
 
 def copy_factor(vals, max_n: int) -> float | None:
-    """Counter bytes per real byte for the copy kernel (four dispatches per restore)."""
-    v = vals.get(COPY_KERNEL)
-    if not v or len(v) % 4:
+    """Counter bytes per real byte of the copy kernel, from its whole-buffer copies."""
+    import math
+    v = [x for x in vals.get(COPY_KERNEL, []) if x > 0]
+    if not v:
         return None
-    restores = len(v) // 4
-    return (sum(v) * 1024.0) / (restores * 52.0 * max_n)
+    top = max(v)
+    v = sorted(x for x in v if x >= top / 8.0)  # whole-buffer copies only
+    if len(v) < 2:
+        return None
+    logs = [math.log(x) for x in v]
+    gaps = [logs[i + 1] - logs[i] for i in range(len(v) - 1)]
+    cut = max(range(len(gaps)), key=gaps.__getitem__) + 1
+    small, large = v[:cut], v[cut:]
+    # the two classes must be the 4-byte and 16-byte copies: their counters differ by ~4x
+    ratio = (sum(large) / len(large)) / (sum(small) / len(small))
+    if not 3.0 < ratio < 5.0:
+        return None
+    real = (len(small) * 4.0 + len(large) * 16.0) * max_n
+    return (sum(v) * 1024.0) / real
 
 
 def main(argv=None):
@@ -136,9 +150,10 @@ def main(argv=None):
     lines = [f"# {a.title}", "", "Commands (run from the repository root on one MI355X; "
              "`cd /tmp && export TMPDIR=/tmp` first for rocprofv3):", "", "```", a.command.strip(), "```", ""]
     if ff is not None or fw is not None:
-        lines += [f"Copy-kernel scale (counter bytes per real byte, {COPY_KERNEL} of the checkpoint "
-                  f"restores, 52 B x {a.max_n} per restore): FETCH_SIZE {ff if ff is None else round(ff, 3)}, "
-                  f"WRITE_SIZE {fw if fw is None else round(fw, 3)}.", ""]
+        lines += [f"Copy-kernel scale (counter bytes per real byte of {COPY_KERNEL} on the model's "
+                  f"whole-buffer copies, 16 B and 4 B x {a.max_n}): FETCH_SIZE "
+                  f"{ff if ff is None else round(ff, 3)}, WRITE_SIZE {fw if fw is None else round(fw, 3)}. "
+                  "Traffic columns are the counters divided by these scales.", ""]
     lines += ["| kernel | calls | avg us | total ms | % | FETCH MB/launch (scaled) | WRITE MB/launch (scaled) | VGPR | LDS B |",
               "|---|---|---|---|---|---|---|---|---|"]
     for k in names:
