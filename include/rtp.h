@@ -95,6 +95,8 @@ typedef struct {
 } rtp_tuning;
 #define RTP_LAUNCH_XCD_MAP 1   /* consecutive particle blocks on one XCD (shared L2) */
 #define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
+#define RTP_LAUNCH_RESORT 4    /* after every correction, order the next passes' threads by the
+                                  cells their particles now sit in (one extra key sort) */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {

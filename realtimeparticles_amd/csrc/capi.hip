@@ -213,6 +213,8 @@ struct rtp_fluids {
   // learned warm-start, teacher recording and the post-correction residual pass
   float* feat = nullptr;      // [maxN][8]
   float* lamAcc = nullptr;
+  uint32_t* okey[2] = { nullptr, nullptr };  // cells of the corrected positions (thread order)
+  uint32_t* oidx[2] = { nullptr, nullptr };
   float* resDensity = nullptr;
   float* resMetric = nullptr; // max, sum, max compression of the residual pass + partials
   rtp_mlp* mlp = nullptr;
@@ -290,6 +292,24 @@ struct rtp_fluids {
     la.gate = resMetric + 1;
     la.tau = tau * (float)n;
     int in = 1 - P;
+    // Thread order of the neighbour passes: the grid's order for the first pass; with
+    // L.resort, after every correction the particles sorted by the cells they now sit in.
+    PassOrder po;
+    const bool resort = L.resort != 0;
+    PassOrder pc;  // of the correction passes: also write those cells
+    pc.keyOut = resort ? okey[0] : nullptr;
+    auto order_threads = [&]() -> int {
+      if (!resort)
+        return RTP_OK;
+      bool inB = false;
+      int r = RTP_OK;
+      const SortPlan plan = make_sort_plan(n, max_active_key(c.g), G.sortBits);
+      RTP_PROF(prof, "thread_order", s,
+               r = sort_pairs(plan, okey[0], okey[1], oidx[0], oidx[1], G.counts, &inB, s));
+      po.order = inB ? oidx[1] : oidx[0];
+      pc.order = po.order;
+      return r;
+    };
     for (uint32_t k = 0; k < iters; ++k) {
       const bool lastIt = (k + 1 == iters);
       int mode = 0;
@@ -300,19 +320,22 @@ struct rtp_fluids {
         la.accumInit = k == 0;
       }
       RTP_PROF(prof, "pbf_density_lambda", s,
-               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la));
+               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la, po));
       if (learned && k == 0)
         RTP_PROF(prof, "mlp_warm_start", s, launch_mlp_forward(feat, n, mlp->dev, lambda, s));
       const int last = !lastIt ? kNextIter : (learned ? kLastOnly : kLastVel);
       RTP_PROF(prof, "pbf_correct", s,
                launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
-                                  (last == kLastVel && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s));
+                                  (last == kLastVel && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s,
+                                  0, LearnArgs(), pc));
       in = 1 - in;
+      if ((rc = order_threads()))
+        return rc;
     }
     // post-correction density: the residual the fallback gate and the diagnostics read
     auto residual = [&]() {
       RTP_PROF(prof, "pbf_residual", s,
-               launch_pbf_density_lambda(pred[in], G.startEnd, resDensity, nullptr, n, c, L, s, kDensOnly, la));
+               launch_pbf_density_lambda(pred[in], G.startEnd, resDensity, nullptr, n, c, L, s, kDensOnly, la, po));
       launch_density_error(resDensity, n, c.p.restDensity, resMetric, s);
     };
     const bool gated = learned && fallbackIters > 0;
@@ -322,11 +345,13 @@ struct rtp_fluids {
         const int mode = kGated | (f == 0 ? kClampIn : 0);
         const int last = (f + 1 == fallbackIters) ? kLastOnly : kNextIter;
         RTP_PROF(prof, "pbf_fallback", s,
-                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la));
+                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la, po));
         RTP_PROF(prof, "pbf_fallback", s,
                  launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
-                                    nullptr, n, last, c, L, s, mode, la));
+                                    nullptr, n, last, c, L, s, mode, la, pc));
         in = 1 - in;
+        if ((rc = order_threads()))
+          return rc;
       }
     }
     if (residualPass)  // after any fallback iterations: the error the substep leaves
@@ -341,11 +366,11 @@ struct rtp_fluids {
     predFinal = in;
     if (vortOn) {
       RTP_PROF(prof, "pbf_vorticity", s,
-               launch_pbf_vorticity(pred[in], G.startEnd, vel[cur], vort, n, c, L, s));
+               launch_pbf_vorticity(pred[in], G.startEnd, vel[cur], vort, n, c, L, s, po));
       RTP_PROF(prof, "pbf_confinement", s,
-               launch_pbf_confinement(pred[in], G.startEnd, vort, vel[cur], velTmp, n, c, L, s));
+               launch_pbf_confinement(pred[in], G.startEnd, vort, vel[cur], velTmp, n, c, L, s, po));
       RTP_PROF(prof, "pbf_xsph", s,
-               launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s));
+               launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s, po));
     }
     return RTP_OK;
   }
@@ -422,6 +447,10 @@ struct rtp_fluids {
     dev_free(lamAcc);
     dev_free(resDensity);
     dev_free(resMetric);
+    for (int k = 0; k < 2; ++k) {
+      dev_free(okey[k]);
+      dev_free(oidx[k]);
+    }
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -547,7 +576,9 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->lambda, maxN)) || (rc = dev_alloc(&h->metric, kMetricFloats)) ||
       (rc = dev_alloc(&h->feat, (size_t)maxN * 8)) ||
       (rc = dev_alloc(&h->lamAcc, maxN)) || (rc = dev_alloc(&h->resDensity, maxN)) ||
-      (rc = dev_alloc(&h->resMetric, kMetricFloats))) {
+      (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey[0], maxN)) ||
+      (rc = dev_alloc(&h->okey[1], maxN)) || (rc = dev_alloc(&h->oidx[0], maxN)) ||
+      (rc = dev_alloc(&h->oidx[1], maxN))) {
     h->release();
     delete h;
     return rc;
@@ -821,12 +852,13 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable)
 int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  const int rc = check_tuning(*t, 3);
+  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT);
   if (rc)
     return rc;
   h->L.block = t->blockSize;
   h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
   h->L.tile = (t->flags & RTP_LAUNCH_LDS_TILES) ? 1 : 0;
+  h->L.resort = (t->flags & RTP_LAUNCH_RESORT) ? 1 : 0;
   h->L.queue = t->queueDepth ? t->queueDepth : 24;
   h->grid.sortBits = t->sortBits;
   h->drop_graph();
@@ -837,7 +869,8 @@ rtp_tuning current_tuning(const rtp_fluids* h)
 {
   rtp_tuning t{};
   t.blockSize = h->L.block;
-  t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0);
+  t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0) |
+            (h->L.resort ? RTP_LAUNCH_RESORT : 0);
   t.sortBits = h->grid.sortBits;
   t.queueDepth = h->L.queue;
   return t;

@@ -497,12 +497,13 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
                                                         const uint2* __restrict__ se,
                                                         float* __restrict__ density,
                                                         float* __restrict__ lambda, uint32_t n,
-                                                        PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
+                                                        PbfConsts c, int remap, LearnArgs la, uint32_t qdepth,
+                     PassOrder po)
 {
-  const uint32_t i = particle_index(remap);
+  const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
-  if (i < n)
-    density_lambda_body<MODE>(i, pred, se, density, lambda, c, la, W);
+  if (t < n)
+    density_lambda_body<MODE>(po.particle(t), pred, se, density, lambda, c, la, W);
 }
 
 // fld_computeConstraintCorrection (fluids.cl:198-247) + fld_correctPosition (:252-258),
@@ -520,10 +521,13 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
                                              const float4* __restrict__ pos,
                                              float4* __restrict__ vel, float4* __restrict__ posOut,
                                              int last, const PbfConsts& c, const LearnArgs& la,
-                                             const WalkCtx& T)
+                                             const WalkCtx& T, uint32_t* __restrict__ keyOut)
 {
   if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
-    predOut[i] = pred[i];
+    const float4 q = pred[i];
+    predOut[i] = q;
+    if (keyOut)
+      keyOut[i] = cell1d(q, c.g);
     return;
   }
   constexpr bool CL = (MODE & kClampIn) != 0;
@@ -554,10 +558,12 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
     q.x = clampf(q.x, c.lo[0], c.hi[0]);
     q.y = clampf(q.y, c.lo[1], c.hi[1]);
     q.z = clampf(q.z, c.lo[2], c.hi[2]);
-    predOut[i] = q;
-    return;
   }
   predOut[i] = q;
+  if (keyOut)  // the cell the next pass's walk starts from, for its thread order
+    keyOut[i] = cell1d(q, c.g);
+  if (last == kNextIter)
+    return;
   if (last == kLastOnly)
     return;
   const float4 o = pos[i];
@@ -579,12 +585,14 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
                                                  const float4* __restrict__ pos,
                                                  float4* __restrict__ vel,
                                                  float4* __restrict__ posOut, uint32_t n,
-                                                 int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
+                                                 int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth,
+                     PassOrder po)
 {
-  const uint32_t i = particle_index(remap);
+  const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
-  if (i < n)
-    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W);
+  if (t < n)
+    correct_body<MODE>(po.particle(t), pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W,
+                       po.keyOut);
 }
 
 // fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
@@ -645,12 +653,13 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
                                                    const uint2* __restrict__ se,
                                                    const float4* __restrict__ vel,
                                                    float4* __restrict__ vort, uint32_t n,
-                                                   PbfConsts c, int remap, uint32_t qdepth)
+                                                   PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
-  const uint32_t i = particle_index(remap);
+  const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
-  if (i < n)
-    vorticity_body(i, pred, se, vel, vort, c, W);
+  if (t < n)
+    vorticity_body(po.particle(t), pred, se, vel, vort, c, W);
 }
 
 // fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
@@ -696,12 +705,13 @@ __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ 
                                                      const float4* __restrict__ vort,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ velOut, uint32_t n,
-                                                     PbfConsts c, int remap, uint32_t qdepth)
+                                                     PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
-  const uint32_t i = particle_index(remap);
+  const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
-  if (i < n)
-    confinement_body(i, pred, se, vort, vel, velOut, c, W);
+  if (t < n)
+    confinement_body(po.particle(t), pred, se, vort, vel, velOut, c, W);
 }
 
 // fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
@@ -733,12 +743,13 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
                                               const float4* __restrict__ velIn,
                                               float4* __restrict__ velOut,
                                               float4* __restrict__ posOut, uint32_t n,
-                                              PbfConsts c, int remap, uint32_t qdepth)
+                                              PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
-  const uint32_t i = particle_index(remap);
+  const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
-  if (i < n)
-    xsph_body(i, pred, se, velIn, velOut, posOut, c, W);
+  if (t < n)
+    xsph_body(po.particle(t), pred, se, velIn, velOut, posOut, c, W);
 }
 
 // --------------------------------------------------------------- LDS-tiled wrappers
@@ -747,9 +758,9 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
 #define RTP_TILED_PROLOGUE(CLV)                                                \
   __shared__ TileLds tl;                                                       \
   extern __shared__ uint32_t rtpQueue[];                                       \
-  const uint32_t i0 = particle_index(remap);                                   \
-  const bool active = i0 < n;                                                  \
-  const uint32_t i = active ? i0 : n - 1u;                                     \
+  const uint32_t t0 = particle_index(remap);                                   \
+  const bool active = t0 < n;                                                  \
+  const uint32_t i = po.particle(active ? t0 : n - 1u);                        \
   WalkCtx T;                                                                   \
   T.qslot = &rtpQueue[threadIdx.x];                                            \
   T.qcap = qdepth;                                                             \
@@ -761,7 +772,8 @@ __global__ __launch_bounds__(256) void k_density_lambda_t(const float4* __restri
                                                           const uint2* __restrict__ se,
                                                           float* __restrict__ density,
                                                           float* __restrict__ lambda, uint32_t n,
-                                                          PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
+                                                          PbfConsts c, int remap, LearnArgs la, uint32_t qdepth,
+                     PassOrder po)
 {
   RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
   if (active)
@@ -776,18 +788,20 @@ __global__ __launch_bounds__(256) void k_correct_t(const float4* __restrict__ pr
                                                    const float4* __restrict__ pos,
                                                    float4* __restrict__ vel,
                                                    float4* __restrict__ posOut, uint32_t n,
-                                                   int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth)
+                                                   int last, PbfConsts c, int remap, LearnArgs la, uint32_t qdepth,
+                     PassOrder po)
 {
   RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
   if (active)
-    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, T);
+    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, T, po.keyOut);
 }
 
 __global__ __launch_bounds__(256) void k_vorticity_t(const float4* __restrict__ pred,
                                                      const uint2* __restrict__ se,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ vort, uint32_t n,
-                                                     PbfConsts c, int remap, uint32_t qdepth)
+                                                     PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -799,7 +813,8 @@ __global__ __launch_bounds__(256) void k_confinement_t(const float4* __restrict_
                                                        const float4* __restrict__ vort,
                                                        const float4* __restrict__ vel,
                                                        float4* __restrict__ velOut, uint32_t n,
-                                                       PbfConsts c, int remap, uint32_t qdepth)
+                                                       PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -811,7 +826,8 @@ __global__ __launch_bounds__(256) void k_xsph_t(const float4* __restrict__ pred,
                                                 const float4* __restrict__ velIn,
                                                 float4* __restrict__ velOut,
                                                 float4* __restrict__ posOut, uint32_t n,
-                                                PbfConsts c, int remap, uint32_t qdepth)
+                                                PbfConsts c, int remap, uint32_t qdepth,
+                     PassOrder po)
 {
   RTP_TILED_PROLOGUE(false)
   if (active)
@@ -891,7 +907,7 @@ __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
   correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
-                  posOut ? posOut + o : nullptr, last, c, LearnArgs(), W);
+                  posOut ? posOut + o : nullptr, last, c, LearnArgs(), W, nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_update_vel_batch(BatchView bv, const float4* __restrict__ pred,
@@ -1042,7 +1058,7 @@ void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint
 
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
-                               hipStream_t s, int mode, const LearnArgs& la)
+                               hipStream_t s, int mode, const LearnArgs& la, const PassOrder& po)
 {
   if (n == 0)
     return;
@@ -1052,10 +1068,10 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
   case M:                                                                                    \
     if (L.tile)                                                                              \
       hipLaunchKernelGGL(k_density_lambda_t<M>, g, b, lds, s, pred, startEnd, density, lambda, \
-                         n, c, L.xcdRemap, la, q);                                           \
+                         n, c, L.xcdRemap, la, q, po);                                       \
     else                                                                                     \
       hipLaunchKernelGGL(k_density_lambda<M>, g, b, lds, s, pred, startEnd, density, lambda, \
-                         n, c, L.xcdRemap, la, q);                                           \
+                         n, c, L.xcdRemap, la, q, po);                                       \
     break;
   switch (mode) {
     RTP_DL(0)
@@ -1073,7 +1089,7 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
 void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
                         float4* predOut, const float4* pos, float4* vel, float4* posOut,
                         uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s,
-                        int mode, const LearnArgs& la)
+                        int mode, const LearnArgs& la, const PassOrder& po)
 {
   if (n == 0)
     return;
@@ -1083,10 +1099,10 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
   case M:                                                                                    \
     if (L.tile)                                                                              \
       hipLaunchKernelGGL(k_correct_t<M>, g, b, lds, s, pred, lambda, startEnd, predOut, pos,   \
-                         vel, posOut, n, last, c, L.xcdRemap, la, q);                        \
+                         vel, posOut, n, last, c, L.xcdRemap, la, q, po);                    \
     else                                                                                     \
       hipLaunchKernelGGL(k_correct<M>, g, b, lds, s, pred, lambda, startEnd, predOut, pos,     \
-                         vel, posOut, n, last, c, L.xcdRemap, la, q);                        \
+                         vel, posOut, n, last, c, L.xcdRemap, la, q, po);                    \
     break;
   switch (mode) {
     RTP_CO(0)
@@ -1108,35 +1124,35 @@ void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, f
 
 void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
                           float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
-                          hipStream_t s)
+                          hipStream_t s, const PassOrder& po)
 {
   if (n == 0)
     return;
   hipLaunchKernelGGL(L.tile ? k_vorticity_t : k_vorticity, dim3(blocks_for(n, L.block)),
                      dim3(L.block), queue_bytes(L), s, pred, startEnd, vel, vort, n, c,
-                     L.xcdRemap, (uint32_t)L.queue);
+                     L.xcdRemap, (uint32_t)L.queue, po);
 }
 
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
                             const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
-                            const Launch& L, hipStream_t s)
+                            const Launch& L, hipStream_t s, const PassOrder& po)
 {
   if (n == 0)
     return;
   hipLaunchKernelGGL(L.tile ? k_confinement_t : k_confinement, dim3(blocks_for(n, L.block)),
                      dim3(L.block), queue_bytes(L), s, pred, startEnd, vort, vel, velOut, n, c,
-                     L.xcdRemap, (uint32_t)L.queue);
+                     L.xcdRemap, (uint32_t)L.queue, po);
 }
 
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
                      float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
-                     const Launch& L, hipStream_t s)
+                     const Launch& L, hipStream_t s, const PassOrder& po)
 {
   if (n == 0)
     return;
   hipLaunchKernelGGL(L.tile ? k_xsph_t : k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block),
                      queue_bytes(L), s, pred, startEnd, velIn, velOut, posOut, n, c, L.xcdRemap,
-                     (uint32_t)L.queue);
+                     (uint32_t)L.queue, po);
 }
 
 void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,

@@ -269,7 +269,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
     rtp_tuning t = start;
     t.sortBits = bits;
     float ms = 0.0f;
-    if (check_tuning(t, singleModel ? 3 : 0))
+    if (check_tuning(t, singleModel ? 7 : 0))
       continue;
     if ((rc = time_one(t, &ms)))
       break;
@@ -278,7 +278,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
       top.msPerStep = ms;
     }
   }
-  const int flagChoices = singleModel ? 4 : 1;
+  const int flagChoices = singleModel ? 8 : 1;
   for (int b = 64; b <= 256 && !rc; b += 64)
     for (int f = 0; f < flagChoices && !rc; ++f)
       for (int q : { 16, 24, 32, 48, 64 }) {
@@ -286,7 +286,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
         t.blockSize = b;
         t.flags = f;
         t.queueDepth = q;
-        if (check_tuning(t, singleModel ? 3 : 0))
+        if (check_tuning(t, singleModel ? 7 : 0))
           continue;
         float ms = 0.0f;
         if ((rc = time_one(t, &ms)))

@@ -64,6 +64,7 @@ struct Launch {
   int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD
   int tile = 0;        // 1: neighbour passes stage their candidates in LDS wave tiles
   int queue = 24;      // survivor-queue depth of the neighbour passes (entries per lane)
+  int resort = 0;      // 1: order each pass's threads by the particles' current cells
 };
 
 // ---------------------------------------------------------------------------- grid / sort
@@ -119,6 +120,14 @@ struct LearnArgs {
   float tau = 0.0f;                  // threshold on *gate (the mean threshold times n)
 };
 enum PassMode { kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16 };
+// Which particle each thread of a neighbour pass handles, and (correction passes) where to
+// write the cell key of every output position so the next pass can order its threads by
+// the cells they will walk.  Results do not depend on the order.
+struct PassOrder {
+  const uint32_t* order = nullptr;  // thread t handles particle order[t]; identity when null
+  uint32_t* keyOut = nullptr;
+  __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
+};
 enum LastMode { kNextIter = 0, kLastVel = 1, kLastOnly = 2 };
 
 // ------------------------------------------------------------------------------ batches
@@ -161,24 +170,29 @@ void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2*
 
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
-                               hipStream_t s, int mode = 0, const LearnArgs& la = LearnArgs());
+                               hipStream_t s, int mode = 0, const LearnArgs& la = LearnArgs(),
+                        const PassOrder& po = PassOrder());
 // Constraint correction + correctPosition, then per `last` (LastMode) the next iteration's
 // boundary clamp, or updateVel (+ updatePosition when posOut), or nothing.
 void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* startEnd,
                         float4* predOut, const float4* pos, float4* vel, float4* posOut,
                         uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s,
-                        int mode = 0, const LearnArgs& la = LearnArgs());
+                        int mode = 0, const LearnArgs& la = LearnArgs(),
+                        const PassOrder& po = PassOrder());
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
                            uint32_t n, const PbfConsts& c, hipStream_t s);
 void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
                           float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
-                          hipStream_t s);
+                          hipStream_t s,
+                        const PassOrder& po = PassOrder());
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
                             const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
-                            const Launch& L, hipStream_t s);
+                            const Launch& L, hipStream_t s,
+                        const PassOrder& po = PassOrder());
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
                      float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
-                     const Launch& L, hipStream_t s);
+                     const Launch& L, hipStream_t s,
+                        const PassOrder& po = PassOrder());
 // Reduction of |rho/rho0 - 1| over n particles into metric[0..2] (max, sum, max
 // compression), deterministic for a given n; metric holds kMetricFloats floats, the ones
 // past metric[3] being per-block partials.

@@ -158,3 +158,30 @@ def test_trained_network_beats_same_budget_without_it():
     again = learned.evaluate(sc, "stable", [("learned-1", mlp, 1, 0, 0.0)], steps=2, warmup=0)
     assert again[0].residual_mean == r["learned-1"].residual_mean
     mlp.close()
+
+
+def test_thread_order_keeps_learned_bits():
+    """The learned substep (features, network, fallback, residual pass) gives the same bits
+    with the neighbour passes' threads re-ordered by current cell."""
+    sc = small_scene(23)
+    p, _, _, _ = rand_net(6, 1)
+    mlp = capi.Mlp()
+    mlp.set_params(p * 0.1)
+    mlp.set_normalization(np.zeros(8, np.float32), np.ones(8, np.float32), 1e-4)
+    out = []
+    for flags in (0, capi.LAUNCH_RESORT):
+        m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+        m.set_params(**sc.params("stable"))
+        m.load_state(sc.pos, sc.vel)
+        m.set_tuning(128, flags, 0, 32)
+        m.set_warm_start(mlp, 2, 2, -1.0)
+        m.set_residual_pass(True)
+        m.update(3)
+        out.append((m.read_state(), m.read_field(capi.FIELD_LAMBDA), m.residual_error()))
+        m.close()
+    _eq(out[1][0][0], out[0][0][0], "positions")
+    _eq(out[1][0][1], out[0][0][1], "velocities")
+    _eq(out[1][1], out[0][1], "lambda")
+    _eq(out[1][2], out[0][2], "residual")
+    mlp.close()
+

@@ -179,12 +179,13 @@ def test_checkpoint_restore_replays_identically():
 
 @pytest.mark.parametrize("block,flags,queue", [(256, 2, 16), (128, 2, 24), (64, 3, 48),
                                                (192, 1, 64), (256, 3, 16), (256, 0, 64),
-                                               (128, 0, 8)])
+                                               (128, 0, 8), (64, 4, 64), (128, 6, 32),
+                                               (256, 7, 16), (192, 5, 40)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
 def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
-    """XCD-contiguous block order, LDS wave tiles and the survivor-queue depth change where
-    candidates are read from and when the neighbour bodies run, never which candidates or
-    in which order."""
+    """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell and
+    the survivor-queue depth change where candidates are read from, which thread handles a
+    particle and when the neighbour bodies run -- never which candidates or in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
     prm = sc.params(mapping)
     gpu, orc = make_pbf_pair(sc, iters=3, **{k: v for k, v in prm.items()})
@@ -196,3 +197,35 @@ def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
         compare_pbf(gpu, orc, sc.n, step)
     gpu.close()
 
+
+
+@pytest.mark.parametrize("block,queue", [(256, 8), (64, 64), (128, 48)])
+def test_overfull_cell_keeps_bits(block, queue):
+    """20,000 particles in one cell (the cap iterates 101 of them) and 50 in the next cell
+    of its column: the walks through them are no longer back-to-back ranges, and survivor
+    queues of every depth drain in the reference's order."""
+    sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=8, extra=0)
+    rng = np.random.default_rng(5)
+    k = 20000
+    dense = np.zeros((k, 4), np.float32)
+    # inside cell (8, 8, 8) of the 40 x 20 x 20 grid (cell size 0.1, walls at -2/-1/-1)
+    dense[:, 0] = -2.0 + 0.8 + 0.1 * rng.random(k, dtype=np.float32) * 0.98 + 0.001
+    dense[:, 1] = -1.0 + 0.8 + 0.1 * rng.random(k, dtype=np.float32) * 0.98 + 0.001
+    dense[:, 2] = -1.0 + 0.8 + 0.1 * rng.random(k, dtype=np.float32) * 0.98 + 0.001
+    # and 50 in the next cell of the same column, (8, 8, 9): ranges after the capped cell
+    # start 20,000 indices past its start
+    nxt = np.zeros((50, 4), np.float32)
+    nxt[:, :3] = dense[:50, :3]
+    nxt[:, 2] += np.float32(0.1)
+    sc.pos = np.concatenate([sc.pos, dense, nxt])
+    sc.vel = np.zeros_like(sc.pos)
+    sc.n = sc.pos.shape[0]
+    sc.max_n = (sc.n + 511) // 512 * 512 + 512
+    gpu, orc = make_pbf_pair(sc, iters=2, **sc.params("stable"))
+    gpu.set_tuning(block, 0, 0, queue)
+    for step in range(1, 3):
+        log(f"overfull cell, block {block}, queue {queue}, step {step}")
+        gpu.update(1)
+        orc.step(1)
+        compare_pbf(gpu, orc, sc.n, step)
+    gpu.close()
