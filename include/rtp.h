@@ -86,7 +86,7 @@ typedef struct {
  * workload by timing it (rtp_fluids_autotune, rtp_batch_autotune). */
 typedef struct {
   int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
-  int32_t flags;      /* RTP_LAUNCH_* bits (single models; 0 for a batch) */
+  int32_t flags;      /* RTP_LAUNCH_* bits (a batch takes RTP_LAUNCH_RESORT only) */
   int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
   int32_t queueDepth; /* neighbour survivors queued per lane in LDS, 8..64 (0 = default 24);
                          queueDepth x blockSize x 4 bytes must stay within 64 KiB, 24 KiB
@@ -96,7 +96,7 @@ typedef struct {
 #define RTP_LAUNCH_XCD_MAP 1   /* consecutive particle blocks on one XCD (shared L2) */
 #define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
 #define RTP_LAUNCH_RESORT 4    /* after every correction, order the next passes' threads by the
-                                  cells their particles now sit in (one extra key sort) */
+                                  cells their particles now sit in (a counting sort) */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {
@@ -258,7 +258,7 @@ int rtp_batch_sync(rtp_batch* h);
 int rtp_batch_save_checkpoint(rtp_batch* h);
 int rtp_batch_restore_checkpoint(rtp_batch* h);
 int rtp_batch_use_graph(rtp_batch* h, int enable);
-/* As rtp_fluids_set_tuning; flags must be 0. */
+/* As rtp_fluids_set_tuning; of the flags only RTP_LAUNCH_RESORT applies to a batch. */
 int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t);
 int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
 int rtp_batch_enable_profiling(rtp_batch* h, int enable);

@@ -494,8 +494,9 @@ class Batch:
     def use_graph(self, flag: bool) -> None:
         _check(lib().rtp_batch_use_graph(self._h, int(bool(flag))), "use_graph")
 
-    def set_tuning(self, block: int = 256, sort_bits: int = 0, queue: int = 0) -> None:
-        t = _tuning(block, 0, sort_bits, queue)
+    def set_tuning(self, block: int = 256, sort_bits: int = 0, queue: int = 0, flags: int = 0) -> None:
+        """flags: LAUNCH_RESORT or 0."""
+        t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_batch_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, trial_steps: int = 5) -> dict:

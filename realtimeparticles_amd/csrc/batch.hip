@@ -50,6 +50,12 @@ struct rtp_batch {
   int block = 256;
   int sortBits = 0;
   int queue = 24;
+  bool resort = false;  // order the neighbour passes' threads by current cell
+  uint32_t* okey = nullptr;
+  uint32_t* orank = nullptr;
+  uint32_t* oorder = nullptr;
+  uint32_t* ocount = nullptr;
+  uint32_t* opart = nullptr;
   bool constsDirty = true;
   int cur = 0;
   int predFinal = 1;
@@ -98,6 +104,11 @@ struct rtp_batch {
     dev_free(dBlocks);
     dev_free(ckPos);
     dev_free(ckVel);
+    dev_free(okey);
+    dev_free(orank);
+    dev_free(oorder);
+    dev_free(ocount);
+    dev_free(opart);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -177,24 +188,35 @@ struct rtp_batch {
     RTP_PROF(prof, "reorder_cells", s, launch_reorder_batch(bv, a, s));
     cur = 1 - cur;
     int in = 1;
+    // neighbour passes: threads in the grid's order first, then (resort) by current cell
+    BatchView nb = bv;
+    BatchView cb = bv;
+    cb.keyOut = resort ? okey : nullptr;
+    const uint32_t m16 = (keySpace + 15u) & ~15u;
     for (uint32_t k = 0; k < iters; ++k) {
       const int last = (k + 1 == iters) ? kLastVel : kNextIter;
       RTP_PROF(prof, "pbf_density_lambda", s,
-               launch_pbf_density_lambda_batch(bv, pred[in], startEnd, density, lambda, s));
+               launch_pbf_density_lambda_batch(nb, pred[in], startEnd, density, lambda, s));
       RTP_PROF(prof, "pbf_correct", s,
-               launch_pbf_correct_batch(bv, pred[in], lambda, startEnd, pred[1 - in], pos[cur], vel[cur],
+               launch_pbf_correct_batch(cb, pred[in], lambda, startEnd, pred[1 - in], pos[cur], vel[cur],
                                         (last == kLastVel && !vortOn) ? pos[cur] : nullptr, last, s));
       in = 1 - in;
+      if (resort) {
+        RTP_PROF(prof, "thread_order", s,
+                 launch_counting_order(ocount, m16, okey, orank, total, opart, oorder, s));
+        nb.order = oorder;
+        cb.order = oorder;
+      }
     }
     if (iters == 0)
       RTP_PROF(prof, "pbf_update_vel", s,
                launch_pbf_update_vel_batch(bv, pred[in], pos[cur], vel[cur], vortOn ? nullptr : pos[cur], s));
     predFinal = in;
     if (vortOn) {
-      RTP_PROF(prof, "pbf_vorticity", s, launch_pbf_vorticity_batch(bv, pred[in], startEnd, vel[cur], vort, s));
+      RTP_PROF(prof, "pbf_vorticity", s, launch_pbf_vorticity_batch(nb, pred[in], startEnd, vel[cur], vort, s));
       RTP_PROF(prof, "pbf_confinement", s,
-               launch_pbf_confinement_batch(bv, pred[in], startEnd, vort, vel[cur], velTmp, s));
-      RTP_PROF(prof, "pbf_xsph", s, launch_pbf_xsph_batch(bv, pred[in], startEnd, velTmp, vel[cur], pos[cur], s));
+               launch_pbf_confinement_batch(nb, pred[in], startEnd, vort, vel[cur], velTmp, s));
+      RTP_PROF(prof, "pbf_xsph", s, launch_pbf_xsph_batch(nb, pred[in], startEnd, velTmp, vel[cur], pos[cur], s));
     }
     stepped = true;
     return RTP_OK;
@@ -337,7 +359,10 @@ int rtp_batch_create(int device, const rtp_batch_scene* sc, uint32_t count, rtp_
       (rc = dev_alloc(&h->counts, count_capacity(N))) ||
       (rc = dev_alloc(&h->startEnd, h->keySpace)) ||
       (rc = dev_alloc(&h->dSlots, count)) || (rc = dev_alloc(&h->dConsts, count)) ||
-      (rc = dev_alloc(&h->dBlocks, maxBlocks))) {
+      (rc = dev_alloc(&h->dBlocks, maxBlocks)) || (rc = dev_alloc(&h->okey, N)) ||
+      (rc = dev_alloc(&h->orank, N)) || (rc = dev_alloc(&h->oorder, N)) ||
+      (rc = dev_alloc(&h->ocount, (h->keySpace + 15u) & ~15u)) ||
+      (rc = dev_alloc(&h->opart, scan_partials_needed((h->keySpace + 15u) & ~15u)))) {
     h->release();
     delete h;
     return rc;
@@ -544,11 +569,12 @@ int rtp_batch_use_graph(rtp_batch* h, int enable)
 int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  int rc = check_tuning(*t, 0);
+  int rc = check_tuning(*t, RTP_LAUNCH_RESORT);
   if (rc)
     return rc;
   RTP_HIP_TRY(hipSetDevice(h->device));
   h->sortBits = t->sortBits;
+  h->resort = (t->flags & RTP_LAUNCH_RESORT) != 0;
   h->queue = t->queueDepth ? t->queueDepth : 24;
   if (h->block != t->blockSize) {
     h->block = t->blockSize;
@@ -572,6 +598,7 @@ int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best)
   start.blockSize = h->block;
   start.sortBits = h->sortBits;
   start.queueDepth = h->queue;
+  start.flags = h->resort ? RTP_LAUNCH_RESORT : 0;
   auto run = [&](const rtp_tuning& t) { return rtp_batch_set_tuning(h, &t); };
   auto steps = [&](uint32_t k) {
     int r = 0;

@@ -213,8 +213,12 @@ struct rtp_fluids {
   // learned warm-start, teacher recording and the post-correction residual pass
   float* feat = nullptr;      // [maxN][8]
   float* lamAcc = nullptr;
-  uint32_t* okey[2] = { nullptr, nullptr };  // cells of the corrected positions (thread order)
-  uint32_t* oidx[2] = { nullptr, nullptr };
+  // thread order of the neighbour passes (counting sort of the corrected positions' cells)
+  uint32_t* okey = nullptr;
+  uint32_t* orank = nullptr;
+  uint32_t* oorder = nullptr;
+  uint32_t* ocount = nullptr;
+  uint32_t* opart = nullptr;
   float* resDensity = nullptr;
   float* resMetric = nullptr; // max, sum, max compression of the residual pass + partials
   rtp_mlp* mlp = nullptr;
@@ -296,19 +300,17 @@ struct rtp_fluids {
     // L.resort, after every correction the particles sorted by the cells they now sit in.
     PassOrder po;
     const bool resort = L.resort != 0;
-    PassOrder pc;  // of the correction passes: also write those cells
-    pc.keyOut = resort ? okey[0] : nullptr;
+    PassOrder pc;  // of the correction passes: also write the cells of their outputs
+    const uint32_t m16 = order_cells(c.g);
+    pc.keyOut = resort ? okey : nullptr;
     auto order_threads = [&]() -> int {
       if (!resort)
         return RTP_OK;
-      bool inB = false;
-      int r = RTP_OK;
-      const SortPlan plan = make_sort_plan(n, max_active_key(c.g), G.sortBits);
       RTP_PROF(prof, "thread_order", s,
-               r = sort_pairs(plan, okey[0], okey[1], oidx[0], oidx[1], G.counts, &inB, s));
-      po.order = inB ? oidx[1] : oidx[0];
-      pc.order = po.order;
-      return r;
+               launch_counting_order(ocount, m16, okey, orank, n, opart, oorder, s));
+      po.order = oorder;
+      pc.order = oorder;
+      return RTP_OK;
     };
     for (uint32_t k = 0; k < iters; ++k) {
       const bool lastIt = (k + 1 == iters);
@@ -447,10 +449,11 @@ struct rtp_fluids {
     dev_free(lamAcc);
     dev_free(resDensity);
     dev_free(resMetric);
-    for (int k = 0; k < 2; ++k) {
-      dev_free(okey[k]);
-      dev_free(oidx[k]);
-    }
+    dev_free(okey);
+    dev_free(orank);
+    dev_free(oorder);
+    dev_free(ocount);
+    dev_free(opart);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -576,9 +579,10 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->lambda, maxN)) || (rc = dev_alloc(&h->metric, kMetricFloats)) ||
       (rc = dev_alloc(&h->feat, (size_t)maxN * 8)) ||
       (rc = dev_alloc(&h->lamAcc, maxN)) || (rc = dev_alloc(&h->resDensity, maxN)) ||
-      (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey[0], maxN)) ||
-      (rc = dev_alloc(&h->okey[1], maxN)) || (rc = dev_alloc(&h->oidx[0], maxN)) ||
-      (rc = dev_alloc(&h->oidx[1], maxN))) {
+      (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey, maxN)) ||
+      (rc = dev_alloc(&h->orank, maxN)) || (rc = dev_alloc(&h->oorder, maxN)) ||
+      (rc = dev_alloc(&h->ocount, order_cells(h->grid.g))) ||
+      (rc = dev_alloc(&h->opart, scan_partials_needed(order_cells(h->grid.g))))) {
     h->release();
     delete h;
     return rc;

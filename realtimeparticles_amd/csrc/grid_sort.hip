@@ -377,9 +377,60 @@ int dispatch_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint3
   }
 }
 
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+// Counting-sort ranks: each particle's place among the particles of its cell.
+__global__ __launch_bounds__(256) void k_count_ranks(const uint32_t* __restrict__ key, uint32_t n,
+                                                     uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ rank)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    rank[i] = atomicAdd(count + key[i], 1u);
+}
+
+// Particle i goes to slot offset[key[i]] + rank[i] of the thread order.
+__global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ rank,
+                                                       const uint32_t* __restrict__ offset, uint32_t n,
+                                                       uint32_t* __restrict__ order)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    order[offset[key[i]] + rank[i]] = i;
+}
+
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
 }  // namespace
+
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s)
+{
+  if (n)
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, uint32_t* rank,
+                           uint32_t n, uint32_t* partial, uint32_t* order, hipStream_t s)
+{
+  if (n == 0 || m16 == 0)
+    return;
+  const uint32_t nseg = (m16 + kScanSeg - 1u) / kScanSeg;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(m16, 256)), dim3(256), 0, s, count, m16, 0u);
+  hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, n, count, rank);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, partial, nseg);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, count, m16, partial);
+  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, rank, count, n,
+                     order);
+}
+
+uint32_t scan_partials_needed(uint32_t m16) { return (m16 + kScanSeg - 1u) / kScanSeg; }
 
 size_t count_capacity(uint32_t maxN)
 {

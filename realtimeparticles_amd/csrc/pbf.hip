@@ -513,6 +513,15 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
 //   kLastOnly : nothing more (the learned step updates the velocity after its fallback)
 // MODE bits kClampIn / kGated as for k_density_lambda; a gated pass that does not run
 // copies its input positions through, so the buffer sequence does not depend on the gate.
+// The cell the next pass's walk starts from, the key of the next thread order.
+__device__ __forceinline__ void record_cell(uint32_t i, const float4 q, const PbfConsts& c,
+                                            const PassOrder& po)
+{
+  if (!po.keyOut)
+    return;
+  po.keyOut[i] = po.keyBase + cell1d(q, c.g);
+}
+
 template <int MODE>
 __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restrict__ pred,
                                              const float* __restrict__ lambda,
@@ -521,13 +530,12 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
                                              const float4* __restrict__ pos,
                                              float4* __restrict__ vel, float4* __restrict__ posOut,
                                              int last, const PbfConsts& c, const LearnArgs& la,
-                                             const WalkCtx& T, uint32_t* __restrict__ keyOut)
+                                             const WalkCtx& T, const PassOrder& po)
 {
   if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
     const float4 q = pred[i];
     predOut[i] = q;
-    if (keyOut)
-      keyOut[i] = cell1d(q, c.g);
+    record_cell(i, q, c, po);
     return;
   }
   constexpr bool CL = (MODE & kClampIn) != 0;
@@ -560,8 +568,7 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
     q.z = clampf(q.z, c.lo[2], c.hi[2]);
   }
   predOut[i] = q;
-  if (keyOut)  // the cell the next pass's walk starts from, for its thread order
-    keyOut[i] = cell1d(q, c.g);
+  record_cell(i, q, c, po);
   if (last == kNextIter)
     return;
   if (last == kLastOnly)
@@ -591,8 +598,7 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
   if (t < n)
-    correct_body<MODE>(po.particle(t), pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W,
-                       po.keyOut);
+    correct_body<MODE>(po.particle(t), pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W, po);
 }
 
 // fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
@@ -793,7 +799,7 @@ __global__ __launch_bounds__(256) void k_correct_t(const float4* __restrict__ pr
 {
   RTP_TILED_PROLOGUE((MODE & kClampIn) != 0)
   if (active)
-    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, T, po.keyOut);
+    correct_body<MODE>(i, pred, lambda, se, predOut, pos, vel, posOut, last, c, la, T, po);
 }
 
 __global__ __launch_bounds__(256) void k_vorticity_t(const float4* __restrict__ pred,
@@ -853,8 +859,22 @@ __device__ __forceinline__ bool batch_site(const BatchView& bv, BatchSite& s)
   const uint2 blk = bv.blocks[blockIdx.x];
   s.scene = blk.x;
   s.slot = bv.scenes[blk.x];
-  s.i = blk.y + threadIdx.x;
-  return s.i < s.slot.n;
+  const uint32_t t = blk.y + threadIdx.x;
+  if (t >= s.slot.n)
+    return false;
+  s.i = bv.order ? bv.order[s.slot.pbase + t] - s.slot.pbase : t;
+  return true;
+}
+
+// The correction pass's key output of a batched scene: batch-wide keys.
+__device__ __forceinline__ PassOrder batch_order(const BatchView& bv, const BatchSite& b)
+{
+  PassOrder po;
+  if (bv.keyOut) {
+    po.keyOut = bv.keyOut + b.slot.pbase;
+    po.keyBase = b.slot.kbase;
+  }
+  return po;
 }
 
 __global__ __launch_bounds__(256) void k_fill_start_end(uint2* __restrict__ se, uint32_t count)
@@ -907,7 +927,7 @@ __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
   correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
-                  posOut ? posOut + o : nullptr, last, c, LearnArgs(), W, nullptr);
+                  posOut ? posOut + o : nullptr, last, c, LearnArgs(), W, batch_order(bv, b));
 }
 
 __global__ __launch_bounds__(256) void k_update_vel_batch(BatchView bv, const float4* __restrict__ pred,

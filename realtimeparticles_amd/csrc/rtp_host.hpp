@@ -61,6 +61,13 @@ uint32_t max_active_key(const GridConsts& g)
   return (uint32_t)std::min<uint64_t>(m, 0xffffffffu);
 }
 
+// Entries of the per-cell count table of the thread order: every key the hash can produce,
+// rounded up to the scan's 16-word groups.
+uint32_t order_cells(const GridConsts& g)
+{
+  return (max_active_key(g) + 1u + 15u) & ~15u;
+}
+
 // Smallest float that is >= h*h as a real number.
 float cull_bound(float h)
 {
@@ -269,7 +276,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
     rtp_tuning t = start;
     t.sortBits = bits;
     float ms = 0.0f;
-    if (check_tuning(t, singleModel ? 7 : 0))
+    if (check_tuning(t, singleModel ? 7 : RTP_LAUNCH_RESORT))
       continue;
     if ((rc = time_one(t, &ms)))
       break;
@@ -278,15 +285,17 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
       top.msPerStep = ms;
     }
   }
-  const int flagChoices = singleModel ? 8 : 1;
+  // single models: every flag combination; batches: with and without re-ordering
+  const int flagChoices = singleModel ? 8 : 2;
   for (int b = 64; b <= 256 && !rc; b += 64)
-    for (int f = 0; f < flagChoices && !rc; ++f)
-      for (int q : { 16, 24, 32, 48, 64 }) {
+    for (int fi = 0; fi < flagChoices && !rc; ++fi)
+      for (int q : { 16, 32, 48, 64 }) {
+        const int f = singleModel ? fi : (fi ? RTP_LAUNCH_RESORT : 0);
         rtp_tuning t = top;
         t.blockSize = b;
         t.flags = f;
         t.queueDepth = q;
-        if (check_tuning(t, singleModel ? 7 : 0))
+        if (check_tuning(t, singleModel ? 7 : RTP_LAUNCH_RESORT))
           continue;
         float ms = 0.0f;
         if ((rc = time_one(t, &ms)))

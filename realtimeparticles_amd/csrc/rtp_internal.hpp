@@ -64,7 +64,8 @@ struct Launch {
   int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD
   int tile = 0;        // 1: neighbour passes stage their candidates in LDS wave tiles
   int queue = 24;      // survivor-queue depth of the neighbour passes (entries per lane)
-  int resort = 0;      // 1: order each pass's threads by the particles' current cells
+  int resort = 0;      // 1: after every correction, order the neighbour passes' threads by
+                       // the cells their particles now sit in
 };
 
 // ---------------------------------------------------------------------------- grid / sort
@@ -125,9 +126,18 @@ enum PassMode { kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated =
 // the cells they will walk.  Results do not depend on the order.
 struct PassOrder {
   const uint32_t* order = nullptr;  // thread t handles particle order[t]; identity when null
-  uint32_t* keyOut = nullptr;
+  uint32_t* keyOut = nullptr;       // correction passes: cell key of each output position
+  uint32_t keyBase = 0;             //   plus this (a batched scene's key base)
   __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
 };
+// Counting sort of particles by cell for the thread order: count[] (m16 entries, a
+// multiple of 16 covering every key) is cleared, counts the particles of each cell (ranks
+// into rank[]) and becomes its exclusive scan; then order[scan[key[i]] + rank[i]] = i.
+// partial holds scan_partials_needed(m16) words.
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, uint32_t* rank,
+                           uint32_t n, uint32_t* partial, uint32_t* order, hipStream_t s);
+uint32_t scan_partials_needed(uint32_t m16);
 enum LastMode { kNextIter = 0, kLastVel = 1, kLastOnly = 2 };
 
 // ------------------------------------------------------------------------------ batches
@@ -146,7 +156,9 @@ struct BatchView {
   const uint2* blocks;
   const PbfConsts* consts;
   uint32_t numBlocks, block;
-  uint32_t qdepth;  // survivor-queue depth of the neighbour passes
+  uint32_t qdepth;                  // survivor-queue depth of the neighbour passes
+  const uint32_t* order = nullptr;  // batch-wide thread order (scene slices stay in place)
+  uint32_t* keyOut = nullptr;       // correction passes: batch-wide key of each output
 };
 void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s);
 void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,

@@ -104,11 +104,12 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
     log("graph replay of 4 substeps")
     b.update(4)
     ref = [b.read_state(k) for k in range(len(scs))]
-    for block, bits, queue in ((64, 7, 64), (192, 11, 16), (128, 0, 48)):
+    for block, bits, queue, flags in ((64, 7, 64, 0), (192, 11, 16, capi.LAUNCH_RESORT),
+                                      (128, 0, 48, capi.LAUNCH_RESORT)):
         b.restore_checkpoint()
-        b.set_tuning(block, bits, queue)
+        b.set_tuning(block, bits, queue, flags)
         b.use_graph(False)
-        log(f"direct launches, block {block}, sort bits {bits}")
+        log(f"direct launches, block {block}, sort bits {bits}, queue {queue}, flags {flags}")
         for _ in range(4):
             b.update(1)
         for k in range(len(scs)):
@@ -173,6 +174,8 @@ def test_batch_argument_errors():
         b.set_tuning(100, 0)
     with pytest.raises(capi.RtpError):
         b.set_tuning(256, 0, 128)  # queue past the LDS budget
+    with pytest.raises(capi.RtpError):
+        b.set_tuning(256, 0, 24, capi.LAUNCH_LDS_TILES)  # single models only
     with pytest.raises(capi.RtpError):
         b.restore_checkpoint()
     b.close()
