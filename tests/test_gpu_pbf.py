@@ -229,3 +229,39 @@ def test_overfull_cell_keeps_bits(block, queue):
         orc.step(1)
         compare_pbf(gpu, orc, sc.n, step)
     gpu.close()
+
+
+@pytest.mark.parametrize("mapping", ["literal", "stable"])
+def test_pbf_65k_long_run(mapping):
+    """The 64x32x32 dam break (4 iterations) for 300 frames (600 substeps): the first 30
+    substeps against the oracle on every field, then the rest on the GPU alone -- graph
+    replay at a tuned launch shape against direct launches at the default shape bit for bit
+    over the whole run, finite state with w = 0, and max |rho/rho0 - 1| recorded."""
+    sc = scenes.pbf_scene("pbf-65k")
+    gpu, orc = make_pbf_pair(sc, iters=4, **sc.params(mapping))
+    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP, 0, 32)
+    gpu.save_checkpoint()
+    for step in range(1, 31):
+        gpu.update(1)
+        orc.step(1)
+        if step % 10 == 0:
+            log(f"pbf-65k {mapping} step {step} against the oracle")
+            compare_pbf(gpu, orc, sc.n, step)
+    errs = []
+    for block in range(30, 600, 30):
+        gpu.update(30)
+        errs.append(float(gpu.density_error()[0]))
+    a_pos, a_vel = gpu.read_state()
+    log(f"{mapping}: max |rho/rho0 - 1| every 30 substeps: {np.round(errs, 3).tolist()}")
+    assert np.isfinite(a_pos).all() and np.isfinite(a_vel).all()
+    assert np.all(a_pos[:, 3] == 0.0) and np.all(a_vel[:, 3] == 0.0)
+    assert np.isfinite(errs).all()
+    gpu.restore_checkpoint()
+    gpu.use_graph(False)
+    gpu.set_tuning(256, 0, 0, 24)
+    for _ in range(600):
+        gpu.update(1)
+    b_pos, b_vel = gpu.read_state()
+    _eq(b_pos, a_pos, "600 substeps, graph + tuned launches vs direct default launches (pos)")
+    _eq(b_vel, a_vel, "600 substeps, graph + tuned launches vs direct default launches (vel)")
+    gpu.close()
