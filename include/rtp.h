@@ -88,7 +88,7 @@ typedef struct {
   int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
   int32_t flags;      /* RTP_LAUNCH_* bits (a batch takes RTP_LAUNCH_RESORT only) */
   int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
-  int32_t queueDepth; /* neighbour survivors queued per lane in LDS, 8..64 (0 = default 24);
+  int32_t queueDepth; /* neighbour survivors queued per lane in LDS, 8..64 (0 = default 16);
                          queueDepth x blockSize x 4 bytes must stay within 64 KiB, 24 KiB
                          with RTP_LAUNCH_LDS_TILES */
   float msPerStep;    /* measured mean substep time of this choice (autotune output) */
@@ -301,6 +301,19 @@ int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxK
  * start/end with the cap.  keys/idx are maxN long, startEnd 2*NC. */
 int rtp_grid_build(const rtp_model_desc* desc, const float* pos4, uint32_t* keys, uint32_t* idx,
                    uint32_t* startEnd);
+
+/* ============================ Self-checks ============================== */
+/* No counterpart in the reference.  The neighbour passes evaluate correctly rounded square
+ * roots and divisions in short forms valid over the operand ranges they meet; these run
+ * those forms on the device against the compiler's general correctly rounded sequences and
+ * count the inputs whose results differ in any bit (NaN results compare equal).
+ * rtp_selfcheck_sqrt: every bit pattern in [lo, hi) (pass x >= 0 patterns).
+ * rtp_selfcheck_division: divisors 2^dExp * (1 + s/2^23) for s in [dSigLo, dSigHi), numerators
+ * +-2^nExp * (1 + t/2^23) for t = 0, nSigStep, 2*nSigStep, ... and +-0.  firstBad: smallest
+ * failing bit pattern (sqrt) or divisor significand (division), ~0 if none. */
+int rtp_selfcheck_sqrt(int device, uint32_t lo, uint32_t hi, uint64_t* mismatches, uint32_t* firstBad);
+int rtp_selfcheck_division(int device, int32_t nExp, int32_t dExp, uint32_t dSigLo, uint32_t dSigHi,
+                           uint32_t nSigStep, uint64_t* mismatches, uint32_t* firstBad);
 
 #ifdef __cplusplus
 }

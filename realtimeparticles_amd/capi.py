@@ -181,6 +181,10 @@ _SIGS = {
     "rtp_batch_sizes": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _vp, _vp], _int),
     "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
+    "rtp_selfcheck_sqrt": ([_int, _u32, _u32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_u32)],
+                           _int),
+    "rtp_selfcheck_division": ([_int, ctypes.c_int32, ctypes.c_int32, _u32, _u32, _u32,
+                                ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_u32)], _int),
 }
 
 _lib = None
@@ -385,7 +389,7 @@ class Fluids:
         return feat, target
 
     def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0, queue: int = 0) -> None:
-        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT; queue: survivor-queue depth (0 = 24)."""
+        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT; queue: survivor-queue depth (0 = 16)."""
         t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_fluids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
@@ -695,3 +699,22 @@ def grid_build(pos, max_n: int, box, res, max_parts_in_cell: int = 100, device: 
     _check(lib().rtp_grid_build(ctypes.byref(d), pos.ctypes.data, keys.ctypes.data,
                                 idx.ctypes.data, se.ctypes.data), "rtp_grid_build")
     return keys, idx, se
+
+
+def selfcheck_sqrt(lo: int, hi: int, device: int = 0):
+    """(mismatches, first failing bit pattern or None) of the short square root over the bit
+    patterns [lo, hi)."""
+    bad, first = ctypes.c_uint64(0), _u32(0)
+    _check(lib().rtp_selfcheck_sqrt(int(device), int(lo), int(hi), ctypes.byref(bad),
+                                    ctypes.byref(first)), "rtp_selfcheck_sqrt")
+    return int(bad.value), (None if bad.value == 0 else int(first.value))
+
+
+def selfcheck_division(n_exp: int, d_exp: int, d_sig_lo: int = 0, d_sig_hi: int = 1 << 23,
+                       n_sig_step: int = 1, device: int = 0):
+    """(mismatches, first failing divisor significand or None) of the short division."""
+    bad, first = ctypes.c_uint64(0), _u32(0)
+    _check(lib().rtp_selfcheck_division(int(device), int(n_exp), int(d_exp), int(d_sig_lo),
+                                        int(d_sig_hi), int(n_sig_step), ctypes.byref(bad),
+                                        ctypes.byref(first)), "rtp_selfcheck_division")
+    return int(bad.value), (None if bad.value == 0 else int(first.value))

@@ -49,7 +49,7 @@ struct rtp_batch {
   uint32_t numBlocks = 0;
   int block = 256;
   int sortBits = 0;
-  int queue = 24;
+  int queue = kDefaultQueue;
   bool resort = false;  // order the neighbour passes' threads by current cell
   uint32_t* okey = nullptr;
   uint32_t* orank = nullptr;
@@ -575,7 +575,7 @@ int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t)
   RTP_HIP_TRY(hipSetDevice(h->device));
   h->sortBits = t->sortBits;
   h->resort = (t->flags & RTP_LAUNCH_RESORT) != 0;
-  h->queue = t->queueDepth ? t->queueDepth : 24;
+  h->queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   if (h->block != t->blockSize) {
     h->block = t->blockSize;
     return h->build_blocks();

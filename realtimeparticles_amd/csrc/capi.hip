@@ -863,7 +863,7 @@ int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
   h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
   h->L.tile = (t->flags & RTP_LAUNCH_LDS_TILES) ? 1 : 0;
   h->L.resort = (t->flags & RTP_LAUNCH_RESORT) ? 1 : 0;
-  h->L.queue = t->queueDepth ? t->queueDepth : 24;
+  h->L.queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   h->grid.sortBits = t->sortBits;
   h->drop_graph();
   return RTP_OK;

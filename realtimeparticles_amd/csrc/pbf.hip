@@ -19,13 +19,12 @@
 // Candidates whose squared distance proves r >= h contribute +-0 in the reference and
 // are skipped; candidates that are very far (d2 >= kFarD2, where the reference's
 // zero-weighted poly6 overflows to NaN) or NaN take the reference's exact expressions.
+#include "rtp_crmath.hpp"
 #include "rtp_device.hpp"
 
 namespace rtp {
 
 namespace {
-
-constexpr float kFarD2 = 1.0e12f;
 
 // Candidates that survive the distance test are queued per lane in LDS and handed to the
 // pass body afterwards, so the expensive body runs densely over each lane's survivors
@@ -50,59 +49,110 @@ __device__ __forceinline__ float4 ldp(const float4* __restrict__ pred, uint32_t 
   return CL ? clamp_pos(q, c) : q;
 }
 
+// The same for a neighbour, whose w lane (always 0) no distance reads: 12 bytes instead of 16.
+template <bool CL>
+__device__ __forceinline__ float4 ldn(const float4* __restrict__ pred, uint32_t e, const PbfConsts& c)
+{
+  const float* f = reinterpret_cast<const float*>(pred + e);
+  const float4 q = make_float4(f[0], f[1], f[2], 0.0f);
+  return CL ? clamp_pos(q, c) : q;
+}
+
+// A lane's survivor queue: entries slot[0], slot[stride], ... up to cap entries, the next
+// free one at wp.  Candidates are written unconditionally and kept by advancing wp, and the
+// queue is drained whenever fewer than four entries are free (wp > room4), so a group of
+// four candidates always fits.
 struct LaneQueue {
-  uint32_t* slot;  // this lane's column of the block's queue; entry k at slot[k * stride]
-  uint32_t count;
-  uint32_t cap;    // entries per lane
-  uint32_t stride; // threads per block
+  uint32_t* slot;   // this lane's first entry
+  uint32_t* wp;     // next free entry
+  uint32_t* room4;  // last position of wp that leaves four free entries
+  uint32_t stride;  // threads per block
 };
 
-template <bool CL, class F>
+// A pass's per-neighbour work: ld(e) loads what the body needs of neighbour e besides its
+// position (issued one survivor ahead), f(e, x, dx, dy, dz, d2) is the body.
+template <class L, class F>
+struct Body {
+  L& ld;
+  F& f;
+};
+
+// Process the lane's queued survivors in order, loading survivor k+1 while computing k.
+template <bool CL, class B>
 __device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4* __restrict__ pred,
-                                      const PbfConsts& c, F& f)
+                                      const PbfConsts& c, B& b)
 {
-  for (uint32_t k = 0; k < q.count; ++k) {
-    const uint32_t e = q.slot[k * q.stride];
-    const float4 o = ldp<CL>(pred, e, c);
-    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    f(e, dx, dy, dz, d2);
+  const uint32_t* rp = q.slot;
+  if (rp != q.wp) {
+    uint32_t e = *rp;
+    float4 o = ldn<CL>(pred, e, c);
+    auto x = b.ld(e);
+    for (;;) {
+      rp += q.stride;
+      const bool more = rp != q.wp;
+      const uint32_t en = *(more ? rp : q.slot);  // in bounds either way
+      const float4 on = ldn<CL>(pred, en, c);
+      const auto xn = b.ld(en);
+      const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      b.f(e, x, dx, dy, dz, d2);
+      if (!more)
+        break;
+      e = en;
+      o = on;
+      x = xn;
+    }
   }
-  q.count = 0;
+  q.wp = q.slot;
 }
 
 // Queue candidate e unless its contribution is provably +-0 (r >= h from the squared
-// distance); very far (d2 >= kFarD2) and NaN distances are queued for the exact expressions.
-template <bool CL, class F>
+// distance); very far (d2 >= kFarD2) and NaN distances are queued for the exact
+// expressions.  The test uses the fused squared distance against bounds widened by 2^-20
+// relative, so it never skips what the unfused one keeps; the few extra candidates it
+// keeps contribute exact zeros, which leave every sum (started at +0) unchanged.
+template <bool CL>
 __device__ __forceinline__ void offer(LaneQueue& q, const float4 p, const float4 o, uint32_t e,
-                                      const float4* __restrict__ pred, const PbfConsts& c, F& f)
+                                      const PbfConsts& c)
 {
   const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
-  if (!(d2 >= c.cullSq && d2 < kFarD2)) {
-    q.slot[q.count * q.stride] = e;
-    if (++q.count == q.cap)
-      drain<CL>(q, p, pred, c, f);
-  }
+  const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  const bool keep = !(d2 >= c.cullFused && d2 < c.farFused);
+  *q.wp = e;
+  q.wp += keep ? q.stride : 0u;
+}
+
+// When any lane of the wave is short of room, every active lane drains: a drain costs the
+// wave one body per entry of its fullest lane, so draining the others with it is free
+// progress.  Drain timing does not change any lane's order of survivors.
+template <bool CL, class B>
+__device__ __forceinline__ void make_room(LaneQueue& q, const float4 p,
+                                          const float4* __restrict__ pred, const PbfConsts& c, B& b)
+{
+  if (__builtin_amdgcn_ballot_w64(q.wp > q.room4) != 0)
+    drain<CL>(q, p, pred, c, b);
 }
 
 // Candidates lo..hi (inclusive) in index order, four loads in flight at a time.
-template <bool CL, class F>
+template <bool CL, class B>
 __device__ __forceinline__ void scan_range(uint32_t lo, uint32_t hi, LaneQueue& q, const float4 p,
                                            const float4* __restrict__ pred, const PbfConsts& c,
-                                           F& f)
+                                           B& b)
 {
   uint32_t e = lo;
   for (; e + 3u <= hi; e += 4u) {
-    const float4 o0 = ldp<CL>(pred, e, c), o1 = ldp<CL>(pred, e + 1u, c),
-                 o2 = ldp<CL>(pred, e + 2u, c), o3 = ldp<CL>(pred, e + 3u, c);
-    offer<CL>(q, p, o0, e, pred, c, f);
-    offer<CL>(q, p, o1, e + 1u, pred, c, f);
-    offer<CL>(q, p, o2, e + 2u, pred, c, f);
-    offer<CL>(q, p, o3, e + 3u, pred, c, f);
+    const float4* g = pred + e;
+    const float4 o0 = ldn<CL>(g, 0u, c), o1 = ldn<CL>(g, 1u, c), o2 = ldn<CL>(g, 2u, c),
+                 o3 = ldn<CL>(g, 3u, c);
+    offer<CL>(q, p, o0, e, c);
+    offer<CL>(q, p, o1, e + 1u, c);
+    offer<CL>(q, p, o2, e + 2u, c);
+    offer<CL>(q, p, o3, e + 3u, c);
+    make_room<CL>(q, p, pred, c, b);
   }
   for (; e <= hi; ++e)
-    offer<CL>(q, p, ldp<CL>(pred, e, c), e, pred, c, f);
+    offer<CL>(q, p, ldn<CL>(pred, e, c), e, c);
+  make_room<CL>(q, p, pred, c, b);
 }
 
 // ------------------------------------------------------------------ wave tiles in LDS
@@ -244,11 +294,11 @@ __device__ __forceinline__ void build_tile(const float4 p, const float4* __restr
 }
 
 // Candidates lo..hi read from the tile; the staged entry of candidate e is e - shift.
-template <bool CL, class F>
+template <bool CL, class B>
 __device__ __forceinline__ void scan_tile(uint32_t lo, uint32_t hi, uint32_t shift,
                                           const WalkCtx& T, LaneQueue& q, const float4 p,
                                           const float4* __restrict__ pred, const PbfConsts& c,
-                                          F& f)
+                                          B& b)
 {
   uint32_t e = lo;
   for (; e + 3u <= hi; e += 4u) {
@@ -257,31 +307,33 @@ __device__ __forceinline__ void scan_tile(uint32_t lo, uint32_t hi, uint32_t shi
     const float4 o1 = make_float4(T.x[t + 1u], T.y[t + 1u], T.z[t + 1u], 0.0f);
     const float4 o2 = make_float4(T.x[t + 2u], T.y[t + 2u], T.z[t + 2u], 0.0f);
     const float4 o3 = make_float4(T.x[t + 3u], T.y[t + 3u], T.z[t + 3u], 0.0f);
-    offer<CL>(q, p, o0, e, pred, c, f);
-    offer<CL>(q, p, o1, e + 1u, pred, c, f);
-    offer<CL>(q, p, o2, e + 2u, pred, c, f);
-    offer<CL>(q, p, o3, e + 3u, pred, c, f);
+    offer<CL>(q, p, o0, e, c);
+    offer<CL>(q, p, o1, e + 1u, c);
+    offer<CL>(q, p, o2, e + 2u, c);
+    offer<CL>(q, p, o3, e + 3u, c);
+    make_room<CL>(q, p, pred, c, b);
   }
   for (; e <= hi; ++e) {
     const uint32_t t = e - shift;
-    offer<CL>(q, p, make_float4(T.x[t], T.y[t], T.z[t], 0.0f), e, pred, c, f);
+    offer<CL>(q, p, make_float4(T.x[t], T.y[t], T.z[t], 0.0f), e, c);
   }
+  make_room<CL>(q, p, pred, c, b);
 }
 
 // One index range of column (nx, ny): from the tile when staged, else from global memory.
-template <bool CL, class F>
+template <bool CL, class B>
 __device__ __forceinline__ void scan_any(uint32_t lo, uint32_t hi, const WalkCtx& T, int32_t k,
                                          LaneQueue& q, const float4 p,
                                          const float4* __restrict__ pred, const PbfConsts& c,
-                                         F& f)
+                                         B& b)
 {
   if (lo > hi)
     return;
   if (k >= 0 && lo >= T.lo[k] && hi <= T.hi[k]) {
-    scan_tile<CL>(lo, hi, T.lo[k] - T.off[k], T, q, p, pred, c, f);
+    scan_tile<CL>(lo, hi, T.lo[k] - T.off[k], T, q, p, pred, c, b);
     return;
   }
-  scan_range<CL>(lo, hi, q, p, pred, c, f);
+  scan_range<CL>(lo, hi, q, p, pred, c, b);
 }
 
 // Visit the candidates of particle position p in the reference's order: 27 cells, x
@@ -290,12 +342,13 @@ __device__ __forceinline__ void scan_any(uint32_t lo, uint32_t hi, const WalkCtx
 // table shows their non-empty ranges are back to back (always, unless a cell was capped or
 // the column wraps) -- the same candidates in the same order with one loop instead of three.
 // The queue is drained after every x-slab of nine cells.
-template <bool CL = false, class F>
+template <bool CL = false, class L, class F>
 __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
-                                       const uint2* __restrict__ se, const PbfConsts& c, F&& f,
-                                       const WalkCtx& T)
+                                       const uint2* __restrict__ se, const PbfConsts& c, L&& ld,
+                                       F&& f, const WalkCtx& T)
 {
-  LaneQueue q{ T.qslot, 0u, T.qcap, blockDim.x };
+  Body<L, F> b{ ld, f };
+  LaneQueue q{ T.qslot, T.qslot, T.qslot + (T.qcap - 4u) * blockDim.x, blockDim.x };
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
@@ -305,11 +358,11 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
     const int32_t nx = wrap((int32_t)cx + ix, rx);
     for (int iy = -1; iy <= 1; ++iy) {
       const int32_t row = (nx * ry + wrap((int32_t)cy + iy, ry)) * rz;
-      const uint2 a = se[row + z0], b = se[row + z1], d = se[row + z2];
+      const uint2 ca = se[row + z0], cb = se[row + z1], cd = se[row + z2];
       // Merge when every non-empty range starts right after the previous non-empty one.
       uint32_t lo = 1u, hi = 0u;
       bool ok = true;
-      const uint2 cells[3] = { a, b, d };
+      const uint2 cells[3] = { ca, cb, cd };
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const uint2 r = cells[k];
@@ -331,14 +384,14 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
           k = dxc * T.ny + dyc;
       }
       if (ok) {
-        scan_any<CL>(lo, hi, T, k, q, p, pred, c, f);
+        scan_any<CL>(lo, hi, T, k, q, p, pred, c, b);
       } else {
-        scan_any<CL>(a.x, a.y, T, k, q, p, pred, c, f);
-        scan_any<CL>(b.x, b.y, T, k, q, p, pred, c, f);
-        scan_any<CL>(d.x, d.y, T, k, q, p, pred, c, f);
+        scan_any<CL>(ca.x, ca.y, T, k, q, p, pred, c, b);
+        scan_any<CL>(cb.x, cb.y, T, k, q, p, pred, c, b);
+        scan_any<CL>(cd.x, cd.y, T, k, q, p, pred, c, b);
       }
     }
-    drain<CL>(q, p, pred, c, f);
+    drain<CL>(q, p, pred, c, b);
   }
 }
 
@@ -369,6 +422,56 @@ __device__ __forceinline__ float art_pressure(float w, const PbfConsts& c)
   if (c.p.isArtPressureEnabled == 0)
     return 0.0f;
   return -c.p.artPressureCoeff * pown(w / c.wdq, c.p.artPressureExp);
+}
+
+// The same three functions for a neighbour with r < h (t == 1, so the factors t are exact
+// identities and left out) with the short correctly rounded forms; shortMath's bits say
+// whether the operand ranges of those forms hold for this parameter set.
+__device__ __forceinline__ float poly6_in(float r, const PbfConsts& c)
+{
+  return c.poly6 * cube(c.hh - r * r);
+}
+
+// gradSpiky for kFloatEps < r < h, as grad_spiky(dx, dy, dz, r, 1, c); yr = rcp_refined(r).
+// Lanes with a nonzero numerator under 2^-90 take the general division.
+__device__ __forceinline__ Grad grad_spiky_in(float dx, float dy, float dz, float r, float yr,
+                                              const PbfConsts& c)
+{
+  const float p2 = (c.h - r) * (c.h - r);
+  const float nx = ((dx * c.spiky) * -3.0f) * p2;
+  const float ny = ((dy * c.spiky) * -3.0f) * p2;
+  const float nz = ((dz * c.spiky) * -3.0f) * p2;
+  Grad g;
+  g.x = div_short(nx, r, yr);
+  g.y = div_short(ny, r, yr);
+  g.z = div_short(nz, r, yr);
+  const bool general = tiny_numerator3(nx, ny, nz);
+  if (__builtin_amdgcn_ballot_w64(general) != 0) {
+    rare_path();
+    if (general) {
+      g.x = nx / r;
+      g.y = ny / r;
+      g.z = nz / r;
+    }
+  }
+  return g;
+}
+
+// pown for the exponent of the dam-break parameters without the loop.
+__device__ __forceinline__ float pown_art(float x, uint32_t n)
+{
+  if (n == 4u)
+    return ((x * x) * x) * x;
+  return pown(x, n);
+}
+
+// art_pressure(w, c) for w = poly6_in(r): ywdq = rcp_refined(c.wdq) when kShortArtDiv.
+__device__ __forceinline__ float art_pressure_in(float w, float ywdq, const PbfConsts& c)
+{
+  if (c.p.isArtPressureEnabled == 0)
+    return 0.0f;
+  const float q = (c.shortMath & kShortArtDiv) ? div_short(w, c.wdq, ywdq) : w / c.wdq;
+  return -c.p.artPressureCoeff * pown_art(q, c.p.artPressureExp);
 }
 
 __device__ __forceinline__ uint32_t particle_index(int remap)
@@ -439,8 +542,28 @@ __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __
   constexpr bool CL = (MODE & kClampIn) != 0;
   const float4 p = ldp<CL>(pred, i, c);
   float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f, count = 0.0f;
-  walk27<CL>(p, pred, se, c, [&](uint32_t, float dx, float dy, float dz, float d2) {
-    const float r = __builtin_sqrtf(d2);
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  walk27<CL>(p, pred, se, c, [](uint32_t) { return 0; },
+             [&](uint32_t, int, float dx, float dy, float dz, float d2) {
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    if (fast && r < c.h) {
+      // gradient zero (and nothing added) for r <= kFloatEps; adding +0 changes no sum
+      dens = dens + poly6_in(r, c);
+      if (MODE & kFeatures)
+        count += 1.0f;
+      if (!(MODE & kDensOnly)) {
+        Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+        const bool on = r > kFloatEps;
+        g.x = on ? g.x : 0.0f;
+        g.y = on ? g.y : 0.0f;
+        g.z = on ? g.z : 0.0f;
+        sgx = sgx + g.x;
+        sgy = sgy + g.y;
+        sgz = sgz + g.z;
+        ssq = ssq + (g.x * g.x + g.y * g.y + g.z * g.z);
+      }
+      return;
+    }
     const float t = r < c.h ? 1.0f : 0.0f;
     dens = dens + poly6(r, t, c);
     if (MODE & kFeatures)
@@ -542,10 +665,23 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
   const float4 p = ldp<CL>(pred, i, c);
   const float li = lambda[i];
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  walk27<CL>(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
-    const float r = __builtin_sqrtf(d2);
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  const float ywdq = (c.shortMath & kShortArtDiv) ? rcp_refined(c.wdq) : 0.0f;
+  walk27<CL>(p, pred, se, c, [&](uint32_t e) { return lambda[e]; },
+             [&](uint32_t, float le, float dx, float dy, float dz, float d2) {
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    if (fast && r < c.h) {
+      const float s = li + le + art_pressure_in(poly6_in(r, c), ywdq, c);
+      // r <= kFloatEps: gradSpiky is the zero vector
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const bool on = r > kFloatEps;
+      cx = cx + s * (on ? g.x : 0.0f);
+      cy = cy + s * (on ? g.y : 0.0f);
+      cz = cz + s * (on ? g.z : 0.0f);
+      return;
+    }
     const float t = r < c.h ? 1.0f : 0.0f;
-    const float s = li + lambda[e] + art_pressure(poly6(r, t, c), c);
+    const float s = li + le + art_pressure(poly6(r, t, c), c);
     if (r > kFloatEps) {
       const Grad g = grad_spiky(dx, dy, dz, r, t, c);
       cx = cx + s * g.x;
@@ -640,12 +776,23 @@ __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restr
   const float4 p = pred[i];
   const float4 vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
-    const float r = __builtin_sqrtf(d2);
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  walk27(p, pred, se, c, [&](uint32_t e) { return vel[e]; },
+         [&](uint32_t, const float4 ve, float dx, float dy, float dz, float d2) {
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    if (fast && r < c.h) {
+      // nothing is added for r <= kFloatEps
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const float ax = ve.x - vi.x, ay = ve.y - vi.y, az = ve.z - vi.z;
+      const bool on = r > kFloatEps;
+      wx = on ? wx + (ay * g.z - az * g.y) : wx;
+      wy = on ? wy + (az * g.x - ax * g.z) : wy;
+      wz = on ? wz + (ax * g.y - ay * g.x) : wz;
+      return;
+    }
     if (r > kFloatEps) {
       const float t = r < c.h ? 1.0f : 0.0f;
       const Grad g = grad_spiky(dx, dy, dz, r, t, c);
-      const float4 ve = vel[e];
       const float ax = ve.x - vi.x, ay = ve.y - vi.y, az = ve.z - vi.z;
       wx = wx + (ay * g.z - az * g.y);
       wy = wy + (az * g.x - ax * g.z);
@@ -679,12 +826,22 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
 {
   const float4 p = pred[i];
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-  walk27(p, pred, se, c, [&](uint32_t e, float dx, float dy, float dz, float d2) {
-    const float r = __builtin_sqrtf(d2);
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  walk27(p, pred, se, c, [&](uint32_t e) { return vort[e].w; },
+         [&](uint32_t, float l, float dx, float dy, float dz, float d2) {
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    if (fast && r < c.h) {
+      // nothing is added for r <= kFloatEps
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const bool on = r > kFloatEps;
+      nx = on ? nx + l * g.x : nx;
+      ny = on ? ny + l * g.y : ny;
+      nz = on ? nz + l * g.z : nz;
+      return;
+    }
     if (r > kFloatEps) {
       const float t = r < c.h ? 1.0f : 0.0f;
       const Grad g = grad_spiky(dx, dy, dz, r, t, c);
-      const float l = vort[e].w;
       nx = nx + l * g.x;
       ny = ny + l * g.y;
       nz = nz + l * g.z;
@@ -730,11 +887,12 @@ __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__
   const float4 p = pred[i];
   const float4 vi = velIn[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  walk27(p, pred, se, c, [&](uint32_t e, float, float, float, float d2) {
-    const float r = __builtin_sqrtf(d2);
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  walk27(p, pred, se, c, [&](uint32_t e) { return velIn[e]; },
+         [&](uint32_t, const float4 ve, float, float, float, float d2) {
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     const float t = r < c.h ? 1.0f : 0.0f;
-    const float w = poly6(r, t, c);
-    const float4 ve = velIn[e];
+    const float w = (fast && r < c.h) ? poly6_in(r, c) : poly6(r, t, c);
     sx = sx + (ve.x - vi.x) * w;
     sy = sy + (ve.y - vi.y) * w;
     sz = sz + (ve.z - vi.z) * w;

@@ -68,14 +68,27 @@ uint32_t order_cells(const GridConsts& g)
   return (max_active_key(g) + 1u + 15u) & ~15u;
 }
 
+// Smallest float >= v, largest float <= v.
+float float_above(double v)
+{
+  float t = (float)v;
+  if ((double)t < v)
+    t = std::nextafter(t, INFINITY);
+  return t;
+}
+
+float float_below(double v)
+{
+  float t = (float)v;
+  if ((double)t > v)
+    t = std::nextafter(t, -INFINITY);
+  return t;
+}
+
 // Smallest float that is >= h*h as a real number.
 float cull_bound(float h)
 {
-  const double hh = (double)h * (double)h;
-  float t = (float)hh;
-  if ((double)t < hh)
-    t = std::nextafter(t, INFINITY);
-  return t;
+  return float_above((double)h * (double)h);
 }
 
 // --------------------------------------------------------------------------------- buffers
@@ -238,7 +251,7 @@ int check_tuning(const rtp_tuning& t, int allowedFlags)
               "blockSize must be 64, 128, 192 or 256");
   RTP_REQUIRE((t.flags & ~allowedFlags) == 0, "launch flags not supported here");
   RTP_REQUIRE(t.sortBits == 0 || (t.sortBits >= 4 && t.sortBits <= 11), "sortBits must be 0 or 4..11");
-  const int q = t.queueDepth ? t.queueDepth : 24;
+  const int q = t.queueDepth ? t.queueDepth : kDefaultQueue;
   RTP_REQUIRE(q >= 8 && q <= 64, "queueDepth must be 0 or 8..64");
   const int limit = (t.flags & RTP_LAUNCH_LDS_TILES) ? 24 * 1024 : 64 * 1024;
   RTP_REQUIRE(q * t.blockSize * 4 <= limit, "queueDepth x blockSize exceeds the LDS budget");
@@ -289,7 +302,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
   const int flagChoices = singleModel ? 8 : 2;
   for (int b = 64; b <= 256 && !rc; b += 64)
     for (int fi = 0; fi < flagChoices && !rc; ++fi)
-      for (int q : { 16, 32, 48, 64 }) {
+      for (int q : { 8, 12, 16, 24, 32 }) {
         const int f = singleModel ? fi : (fi ? RTP_LAUNCH_RESORT : 0);
         rtp_tuning t = top;
         t.blockSize = b;
@@ -346,6 +359,19 @@ void refresh_pbf_consts(PbfConsts& c)
   for (int a = 0; a < 3; ++a) {
     c.lo[a] = -c.g.absWall[a] + 0.01f;
     c.hi[a] = c.g.absWall[a] - 0.1f;
+  }
+  c.cullFused = float_above((double)c.cullSq * (1.0 + 0x1p-20));
+  c.farFused = float_below((double)kFarD2 * (1.0 - 0x1p-20));
+  // The short forms' operand ranges hold for every neighbour with r < h when
+  // 2^-7 <= h <= 2^4: r in (1e-8, h), gradient numerators |n| <= 45/(pi h^3) <= 2^25
+  // (tiny nonzero ones are checked per neighbour), so quotients in [2^-94, 2^52]; poly6
+  // values in [1.5 h^-3 2^-72, 1.6 h^-3], within [2^-84, 2^22], against a divisor wdq in
+  // [2^-30, 2^30], so quotients in [2^-114, 2^52].
+  c.shortMath = 0;
+  if (h >= 0x1p-7f && h <= 0x1p4f) {
+    c.shortMath |= kShortSqrtGrad;
+    if (c.wdq >= 0x1p-30f && c.wdq <= 0x1p30f)
+      c.shortMath |= kShortArtDiv;
   }
 }
 

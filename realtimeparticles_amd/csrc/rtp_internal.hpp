@@ -48,7 +48,19 @@ struct PbfConsts {
   float gdt;       // -ABS_GRAVITY_ACC_Y * timeStep (fluids.cl:71)
   float lo[3];     // fld_applyBoundaryCondition lower bounds (fluids.cl:437)
   float hi[3];     // upper bounds (fluids.cl:438)
+  // Candidate test on the fused squared distance fma(dz,dz, fma(dy,dy, dx*dx)), which is
+  // within 4 ulp of the reference's unfused one: skipping at fused >= cullFused (cullSq
+  // plus 2^-20 relative) and < farFused (kFarD2 less 2^-20 relative) skips only what the
+  // unfused test would.
+  float cullFused;
+  float farFused;
+  int32_t shortMath;  // kShortSqrtGrad | kShortArtDiv: which short correctly rounded forms apply
 };
+
+// Squared distances at and above this take the reference's exact expressions (its
+// zero-weighted poly6 overflows to NaN well above it).
+constexpr float kFarD2 = 1.0e12f;
+enum ShortMath { kShortSqrtGrad = 1, kShortArtDiv = 2 };
 
 struct BoidsConsts {
   GridConsts g;
@@ -58,12 +70,15 @@ struct BoidsConsts {
   float vmin;      // 0.2f * maxVelocity (boids.cl:256)
 };
 
+// Survivor-queue depth of the neighbour passes when a tuning gives 0.
+constexpr int kDefaultQueue = 16;
+
 // Launch shape shared by the per-particle kernels.
 struct Launch {
   int block = 256;     // threads per block
   int xcdRemap = 0;    // 1: consecutive particle blocks on one XCD
   int tile = 0;        // 1: neighbour passes stage their candidates in LDS wave tiles
-  int queue = 24;      // survivor-queue depth of the neighbour passes (entries per lane)
+  int queue = kDefaultQueue;  // survivor-queue depth of the neighbour passes (entries per lane)
   int resort = 0;      // 1: after every correction, order the neighbour passes' threads by
                        // the cells their particles now sit in
 };

@@ -6,7 +6,8 @@ Inputs are the output directories of up to four runs of the same command:
   --trace DIR   rocprofv3 --kernel-trace --stats           (run_kernel_stats.csv)
   --fetch DIR   rocprofv3 --pmc FETCH_SIZE                 (run_counter_collection.csv)
   --write DIR   rocprofv3 --pmc WRITE_SIZE
-  --sq DIR      rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES ...
+  --sq DIR      rocprofv3 --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES ...; may be given
+                more than once (e.g. a second pass with TA_BUSY_avr GRBM_GUI_ACTIVE)
 
 FETCH_SIZE and WRITE_SIZE are KiB per dispatch.  Each is scaled by what the same counter
 reads for the runtime's copy kernel (__amd_rocclr_copyBuffer) on copies of known size: the
@@ -118,7 +119,7 @@ def main(argv=None):
     ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
-    ap.add_argument("--sq")
+    ap.add_argument("--sq", action="append", default=[])
     ap.add_argument("--max-n", type=int, required=True)
     ap.add_argument("--command", default="")
     ap.add_argument("--title", default="rocprofv3 summary")
@@ -132,9 +133,9 @@ def main(argv=None):
     meta = meta or meta_w
     ff = copy_factor(fetch, a.max_n) if fetch else None
     fw = copy_factor(write, a.max_n) if write else None
-    sq = {}
-    if a.sq:
-        rows = read_csv(find(a.sq, "counter_collection.csv"))
+    sq = defaultdict(dict)
+    for sqdir in a.sq:
+        rows = read_csv(find(sqdir, "counter_collection.csv"))
         acc = defaultdict(lambda: defaultdict(float))
         cnt = defaultdict(lambda: defaultdict(int))
         for r in rows:
@@ -142,7 +143,7 @@ def main(argv=None):
             acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
             cnt[k][r["Counter_Name"]] += 1
         for k, d in acc.items():
-            sq[k] = {c: d[c] / cnt[k][c] for c in d}
+            sq[k].update({c: d[c] / cnt[k][c] for c in d})
 
     names = sorted(set(stats) | set(fetch) | set(write),
                    key=lambda k: -stats.get(k, {}).get("total_ns", 0.0))
@@ -172,12 +173,13 @@ def main(argv=None):
             "" if fb is None else f"{fb / 1e6:.2f}", "" if wb is None else f"{wb / 1e6:.2f}",
             m.get("vgpr", ""), m.get("lds", "")))
     if sq:
-        lines += ["", "SQ counters (mean per dispatch):", "",
-                  "| kernel | " + " | ".join(sorted(next(iter(sq.values())).keys())) + " |",
-                  "|---|" + "---|" * len(next(iter(sq.values())))]
+        cols = sorted({c for d in sq.values() for c in d})
+        lines += ["", "Counters (mean per dispatch, summed over the GPU's instances):", "",
+                  "| kernel | " + " | ".join(cols) + " |", "|---|" + "---|" * len(cols)]
         for k in names:
             if k in sq:
-                lines.append(f"| {k} | " + " | ".join(f"{sq[k][c]:.4g}" for c in sorted(sq[k])) + " |")
+                lines.append(f"| {k} | " + " | ".join(
+                    f"{sq[k][c]:.4g}" if c in sq[k] else "" for c in cols) + " |")
     os.makedirs(os.path.dirname(os.path.abspath(a.md)), exist_ok=True)
     with open(a.md, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -121,7 +121,7 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
     best = b.autotune(trial_steps=2)
     log(f"autotuned: {best}")
     assert best["block"] in (64, 128, 192, 256) and 7 <= best["sort_bits"] <= 11
-    assert best["queue"] in (16, 24, 32, 48, 64)
+    assert best["queue"] in (8, 12, 16, 24, 32)
     b.update(4)
     for k in range(len(scs)):
         _eq(b.read_state(k)[0], ref[k][0], f"scene {k} pos after autotune")
