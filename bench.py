@@ -172,8 +172,10 @@ def run_batched(args):
     the same scenes run one after another as separate models.
 
     Both time fixed-input substeps (restore every scene's input, one substep).  The batch is
-    autotuned first (block size x sort digit width).  The sequential run synchronises after
-    every scene, so scenes do not overlap.
+    autotuned first (block size x sort digit width x thread order x queue depth), or takes
+    the shape given by --block/--flags/--sort-bits/--queue (of the flags only the thread
+    order applies to a batch), or the default shape with --no-tune.  The sequential run
+    synchronises after every scene, so scenes do not overlap.
     """
     from realtimeparticles_amd import capi, scenes
     scs = scenes.batch_scenes()
@@ -184,7 +186,15 @@ def run_batched(args):
         b.load_state(k, sc.pos, sc.vel)
     b.set_nb_jacobi_iters(4)
     b.save_checkpoint()
-    tuning = b.autotune(trial_steps=3)
+    if args.block or args.flags or args.sort_bits or args.queue:
+        b.set_tuning(args.block or 256, args.sort_bits or 0, args.queue or 0,
+                     args.flags & capi.LAUNCH_RESORT)
+        tuning = {"block": args.block or 256, "flags": args.flags & capi.LAUNCH_RESORT,
+                  "sort_bits": args.sort_bits or 0, "queue": args.queue or 0}
+    elif args.no_tune:
+        tuning = None
+    else:
+        tuning = b.autotune(trial_steps=3)
 
     def batch_steps(k):
         for _ in range(k):
@@ -309,7 +319,7 @@ def main():
     ap.add_argument("--flags", type=int, default=0,
                     help="launch flags: 1 XCD-contiguous blocks, 2 LDS wave tiles")
     ap.add_argument("--sort-bits", type=int, default=0)
-    ap.add_argument("--queue", type=int, default=0, help="survivor-queue depth (0 = 24)")
+    ap.add_argument("--queue", type=int, default=0, help="survivor-queue depth (0 = 16)")
     ap.add_argument("--no-tune", action="store_true",
                     help="time the default launch shape instead of the autotuned one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
