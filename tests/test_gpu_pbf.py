@@ -265,3 +265,46 @@ def test_pbf_65k_long_run(mapping):
     _eq(b_pos, a_pos, "600 substeps, graph + tuned launches vs direct default launches (pos)")
     _eq(b_vel, a_vel, "600 substeps, graph + tuned launches vs direct default launches (vel)")
     gpu.close()
+
+
+def test_short_math_general_lanes_keep_bits():
+    """Neighbour pairs that send single lanes down the general correctly rounded forms inside
+    the short ones: squared distances in (0, 2^-96) (pairs 1e-16 and 1e-20 apart, next to
+    the x = 0 plane) and gradient numerators in (0, 2^-90) (pairs 0.05 apart whose x or z
+    differ by 1e-33 or 1e-35), placed among ordinary lattice particles so that their waves
+    also run the short forms.  Every field against the oracle, bit for bit."""
+    sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=12, extra=0)
+    special = np.array([
+        [1e-16, 0.3, 0.3, 0], [2e-16, 0.3, 0.3, 0],          # d2 = 1e-32
+        [1e-20, 0.3, 0.5, 0], [2e-20, 0.3, 0.5, 0],          # d2 = 1e-40 (denormal)
+        [1e-33, 0.4, 0.4, 0], [0.0, 0.45, 0.4, 0],           # dx = 1e-33
+        [0.6, 0.4, 1e-35, 0], [0.6, 0.45, 0.0, 0],           # dz = 1e-35
+        [0.62, 0.41, 0.01, 0], [0.63, 0.42, 0.02, 0],        # ordinary neighbours of the above
+        [1e-30, 0.35, 0.31, 0], [-1e-30, 0.33, 0.29, 0],
+    ], np.float32)
+    sc.pos = np.concatenate([sc.pos, special])
+    sc.vel = np.zeros_like(sc.pos)
+    sc.n = sc.pos.shape[0]
+    sc.max_n = (sc.n + 511) // 512 * 512 + 512
+    for mapping in ("stable", "literal"):
+        gpu, orc = make_pbf_pair(sc, iters=3, **sc.params(mapping))
+        for step in range(1, 4):
+            log(f"general lanes, {mapping}, step {step}")
+            gpu.update(1)
+            orc.step(1)
+            compare_pbf(gpu, orc, sc.n, step)
+        gpu.close()
+
+
+@pytest.mark.parametrize("box", [(0.2, 0.1, 0.1), (800.0, 400.0, 400.0)])
+def test_general_forms_outside_short_range(box):
+    """Effect radius h = 0.005 and h = 20, outside [2^-7, 2^4] where the short correctly
+    rounded forms are enabled: every body takes the general sequences."""
+    sc = scenes.dam_break((16, 16, 16), box, (40, 20, 20), seed=13)
+    gpu, orc = make_pbf_pair(sc, iters=2, **sc.params("stable"))
+    for step in range(1, 4):
+        log(f"h = {box[0] / 40}, step {step}")
+        gpu.update(1)
+        orc.step(1)
+        compare_pbf(gpu, orc, sc.n, step)
+    gpu.close()
