@@ -97,6 +97,11 @@ typedef struct {
 #define RTP_LAUNCH_LDS_TILES 2 /* neighbour passes stage candidates in LDS wave tiles */
 #define RTP_LAUNCH_RESORT 4    /* after every correction, order the next passes' threads by the
                                   cells their particles now sit in (a counting sort) */
+#define RTP_LAUNCH_NB_LIST 8   /* the density pass of each Jacobi iteration records every
+                                  particle's neighbour list (up to 64) and the correction pass
+                                  replays it; likewise vorticity for confinement and XSPH
+                                  (256 bytes of device memory per particle; ignored with
+                                  RTP_LAUNCH_LDS_TILES) */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {

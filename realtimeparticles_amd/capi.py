@@ -21,7 +21,7 @@ _vp = ctypes.c_void_p
 _int = ctypes.c_int
 
 FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
-LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT = 1, 2, 4
+LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
 
 
 class RtpError(RuntimeError):
@@ -389,7 +389,8 @@ class Fluids:
         return feat, target
 
     def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0, queue: int = 0) -> None:
-        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT; queue: survivor-queue depth (0 = 16)."""
+        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT | LAUNCH_NB_LIST; queue:
+        survivor-queue depth (0 = 16)."""
         t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_fluids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 

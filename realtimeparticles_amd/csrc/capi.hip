@@ -219,6 +219,9 @@ struct rtp_fluids {
   uint32_t* oorder = nullptr;
   uint32_t* ocount = nullptr;
   uint32_t* opart = nullptr;
+  // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled
+  uint4* nbList = nullptr;      // kNbCap / 4 x maxN granules
+  uint32_t* nbCount = nullptr;  // maxN
   float* resDensity = nullptr;
   float* resMetric = nullptr; // max, sum, max compression of the residual pass + partials
   rtp_mlp* mlp = nullptr;
@@ -301,6 +304,13 @@ struct rtp_fluids {
     PassOrder po;
     const bool resort = L.resort != 0;
     PassOrder pc;  // of the correction passes: also write the cells of their outputs
+    // survivor lists: recorded by each plain density pass (and by vorticity), replayed by
+    // the correction pass of the same iteration (and by confinement and XSPH)
+    const bool lists = L.nbList != 0 && L.tile == 0 && nbList != nullptr;
+    if (lists) {
+      po.nb = NbList{ nbList, nbCount, n };
+      pc.nb = po.nb;
+    }
     const uint32_t m16 = order_cells(c.g);
     pc.keyOut = resort ? okey : nullptr;
     auto order_threads = [&]() -> int {
@@ -321,15 +331,19 @@ struct rtp_fluids {
         mode |= kAccum;
         la.accumInit = k == 0;
       }
+      // the plain and feature-writing density passes record; the teacher's accumulating
+      // ones do not
+      const bool replay = lists && (mode & ~kFeatures) == 0;
       RTP_PROF(prof, "pbf_density_lambda", s,
-               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la, po));
+               launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s,
+                                         replay ? (mode | kNbRecord) : mode, la, po));
       if (learned && k == 0)
         RTP_PROF(prof, "mlp_warm_start", s, launch_mlp_forward(feat, n, mlp->dev, lambda, s));
       const int last = !lastIt ? kNextIter : (learned ? kLastOnly : kLastVel);
       RTP_PROF(prof, "pbf_correct", s,
                launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
                                   (last == kLastVel && !vortOn) ? pos[cur] : nullptr, n, last, c, L, s,
-                                  0, LearnArgs(), pc));
+                                  replay ? kNbReplay : 0, LearnArgs(), pc));
       in = 1 - in;
       if ((rc = order_threads()))
         return rc;
@@ -368,11 +382,14 @@ struct rtp_fluids {
     predFinal = in;
     if (vortOn) {
       RTP_PROF(prof, "pbf_vorticity", s,
-               launch_pbf_vorticity(pred[in], G.startEnd, vel[cur], vort, n, c, L, s, po));
+               launch_pbf_vorticity(pred[in], G.startEnd, vel[cur], vort, n, c, L, s, po,
+                                    lists ? kNbRecord : 0));
       RTP_PROF(prof, "pbf_confinement", s,
-               launch_pbf_confinement(pred[in], G.startEnd, vort, vel[cur], velTmp, n, c, L, s, po));
+               launch_pbf_confinement(pred[in], G.startEnd, vort, vel[cur], velTmp, n, c, L, s, po,
+                                      lists ? kNbReplay : 0));
       RTP_PROF(prof, "pbf_xsph", s,
-               launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s, po));
+               launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s, po,
+                               lists ? kNbReplay : 0));
     }
     return RTP_OK;
   }
@@ -454,6 +471,8 @@ struct rtp_fluids {
     dev_free(oorder);
     dev_free(ocount);
     dev_free(opart);
+    dev_free(nbList);
+    dev_free(nbCount);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -856,13 +875,23 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable)
 int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT);
+  int rc = check_tuning(*t, kFluidsLaunchFlags);
   if (rc)
     return rc;
+  if ((t->flags & RTP_LAUNCH_NB_LIST) && !h->nbList) {
+    RTP_HIP_TRY(hipSetDevice(h->device));
+    if ((rc = dev_alloc(&h->nbList, (size_t)(kNbCap / 4u) * h->grid.maxN)) ||
+        (rc = dev_alloc(&h->nbCount, h->grid.maxN))) {
+      dev_free(h->nbList);
+      h->nbList = nullptr;
+      return rc;
+    }
+  }
   h->L.block = t->blockSize;
   h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
   h->L.tile = (t->flags & RTP_LAUNCH_LDS_TILES) ? 1 : 0;
   h->L.resort = (t->flags & RTP_LAUNCH_RESORT) ? 1 : 0;
+  h->L.nbList = (t->flags & RTP_LAUNCH_NB_LIST) ? 1 : 0;
   h->L.queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   h->grid.sortBits = t->sortBits;
   h->drop_graph();
@@ -874,7 +903,7 @@ rtp_tuning current_tuning(const rtp_fluids* h)
   rtp_tuning t{};
   t.blockSize = h->L.block;
   t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0) |
-            (h->L.resort ? RTP_LAUNCH_RESORT : 0);
+            (h->L.resort ? RTP_LAUNCH_RESORT : 0) | (h->L.nbList ? RTP_LAUNCH_NB_LIST : 0);
   t.sortBits = h->grid.sortBits;
   t.queueDepth = h->L.queue;
   return t;

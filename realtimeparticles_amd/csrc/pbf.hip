@@ -182,6 +182,8 @@ struct WalkCtx {
   const float* x = nullptr;
   const float* y = nullptr;
   const float* z = nullptr;
+  NbList nb;                               // survivor lists of a kNbRecord / kNbReplay pass
+  uint32_t slot = 0;                       // this thread's column in them
 };
 
 // LDS a block's waves stage their tiles in (positions as three float arrays).
@@ -342,10 +344,10 @@ __device__ __forceinline__ void scan_any(uint32_t lo, uint32_t hi, const WalkCtx
 // table shows their non-empty ranges are back to back (always, unless a cell was capped or
 // the column wraps) -- the same candidates in the same order with one loop instead of three.
 // The queue is drained after every x-slab of nine cells.
-template <bool CL = false, class L, class F>
-__device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
-                                       const uint2* __restrict__ se, const PbfConsts& c, L&& ld,
-                                       F&& f, const WalkCtx& T)
+template <bool CL, class L, class F>
+__device__ __forceinline__ void walk_cells(const float4 p, const float4* __restrict__ pred,
+                                           const uint2* __restrict__ se, const PbfConsts& c,
+                                           L& ld, F& f, const WalkCtx& T)
 {
   Body<L, F> b{ ld, f };
   LaneQueue q{ T.qslot, T.qslot, T.qslot + (T.qcap - 4u) * blockDim.x, blockDim.x };
@@ -392,6 +394,85 @@ __device__ __forceinline__ void walk27(const float4 p, const float4* __restrict_
       }
     }
     drain<CL>(q, p, pred, c, b);
+  }
+}
+
+// Run a recorded survivor list: the same body calls, in the same order, as the walk that
+// recorded it.  Entry k + 1's granule, position and pass data are loaded while entry k is
+// computed (a new granule every fourth entry).
+template <bool CL, class B>
+__device__ __forceinline__ void replay(const uint4* __restrict__ gp, uint32_t stride, uint32_t cnt,
+                                       const float4 p, const float4* __restrict__ pred,
+                                       const PbfConsts& c, B& b)
+{
+  if (cnt == 0)
+    return;
+  uint4 v = *gp;
+  uint32_t e = v.x;
+  float4 o = ldn<CL>(pred, e, c);
+  auto x = b.ld(e);
+  for (uint32_t k = 1;; ++k) {
+    const bool more = k < cnt;
+    const uint32_t j = k & 3u;
+    if (more && j == 0u) {
+      gp += stride;
+      v = *gp;
+    }
+    const uint32_t en = !more ? e : (j == 0u ? v.x : (j == 1u ? v.y : (j == 2u ? v.z : v.w)));
+    const float4 on = ldn<CL>(pred, en, c);
+    const auto xn = b.ld(en);
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    b.f(e, x, dx, dy, dz, d2);
+    if (!more)
+      break;
+    e = en;
+    o = on;
+    x = xn;
+  }
+}
+
+// The neighbour walk of one particle: walk_cells, or (NB == kNbRecord) walk_cells recording
+// every survivor the body runs on into the thread's list, or (NB == kNbReplay) the thread's
+// recorded list when it holds all of them.
+template <bool CL = false, int NB = 0, class L, class F>
+__device__ __forceinline__ void walk27(const float4 p, const float4* __restrict__ pred,
+                                       const uint2* __restrict__ se, const PbfConsts& c, L&& ld,
+                                       F&& f, const WalkCtx& T)
+{
+  if constexpr (NB == kNbReplay) {
+    const uint32_t cnt = T.nb.count[T.slot];
+    if (cnt <= kNbCap) {
+      Body<L, F> b{ ld, f };
+      replay<CL>(T.nb.list + T.slot, T.nb.stride, cnt, p, pred, c, b);
+      return;
+    }
+    walk_cells<CL>(p, pred, se, c, ld, f, T);
+  } else if constexpr (NB == kNbRecord) {
+    uint4* gp = T.nb.list + T.slot;
+    uint4* const end = gp + (size_t)(kNbCap / 4u) * T.nb.stride;
+    const uint32_t stride = T.nb.stride;
+    uint32_t cnt = 0, r0 = 0, r1 = 0, r2 = 0, r3 = 0;  // the last four survivors, oldest first
+    auto fr = [&](uint32_t e, auto x, float dx, float dy, float dz, float d2) {
+      r0 = r1;
+      r1 = r2;
+      r2 = r3;
+      r3 = e;
+      ++cnt;
+      if ((cnt & 3u) == 0u && gp != end) {
+        *gp = make_uint4(r0, r1, r2, r3);
+        gp += stride;
+      }
+      f(e, x, dx, dy, dz, d2);
+    };
+    walk_cells<CL>(p, pred, se, c, ld, fr, T);
+    const uint32_t m = cnt & 3u;  // survivors of the last, partial granule
+    if (m != 0u && gp != end)
+      *gp = m == 1u ? make_uint4(r3, 0u, 0u, 0u)
+                    : (m == 2u ? make_uint4(r2, r3, 0u, 0u) : make_uint4(r1, r2, r3, 0u));
+    T.nb.count[T.slot] = cnt;
+  } else {
+    walk_cells<CL>(p, pred, se, c, ld, f, T);
   }
 }
 
@@ -487,6 +568,10 @@ __device__ __forceinline__ uint32_t particle_index(int remap)
   WalkCtx W;                                                 \
   W.qslot = &rtpQueue[threadIdx.x];                          \
   W.qcap = (DEPTH);
+// ... and the survivor lists of thread t (a plain pass's t and po).
+#define RTP_NB_CTX \
+  W.nb = po.nb;    \
+  W.slot = t;
 
 // fluids.cl:71-73 (newVel = vel + GRAVITY_ACC * dt; predPos = pos + newVel * dt) and
 // fillCellIDs (grid.cl:76-86) on predPos, the key offset by the scene's key base.
@@ -543,7 +628,7 @@ __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __
   const float4 p = ldp<CL>(pred, i, c);
   float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f, count = 0.0f;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
-  walk27<CL>(p, pred, se, c, [](uint32_t) { return 0; },
+  walk27<CL, (MODE & (kNbRecord | kNbReplay))>(p, pred, se, c, [](uint32_t) { return 0; },
              [&](uint32_t, int, float dx, float dy, float dz, float d2) {
     const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     if (fast && r < c.h) {
@@ -625,6 +710,7 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
 {
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
+  RTP_NB_CTX
   if (t < n)
     density_lambda_body<MODE>(po.particle(t), pred, se, density, lambda, c, la, W);
 }
@@ -667,7 +753,7 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
   const float ywdq = (c.shortMath & kShortArtDiv) ? rcp_refined(c.wdq) : 0.0f;
-  walk27<CL>(p, pred, se, c, [&](uint32_t e) { return lambda[e]; },
+  walk27<CL, (MODE & (kNbRecord | kNbReplay))>(p, pred, se, c, [&](uint32_t e) { return lambda[e]; },
              [&](uint32_t, float le, float dx, float dy, float dz, float d2) {
     const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     if (fast && r < c.h) {
@@ -733,6 +819,7 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
 {
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
+  RTP_NB_CTX
   if (t < n)
     correct_body<MODE>(po.particle(t), pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W, po);
 }
@@ -767,6 +854,7 @@ __global__ __launch_bounds__(256) void k_update_vel(const float4* __restrict__ p
 
 // fld_computeVorticity (fluids.cl:278-324).  The w lane of the output carries
 // fast_length(vort) for the confinement pass.
+template <int NB = 0>
 __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restrict__ pred,
                                                const uint2* __restrict__ se,
                                                const float4* __restrict__ vel,
@@ -777,7 +865,7 @@ __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restr
   const float4 vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
-  walk27(p, pred, se, c, [&](uint32_t e) { return vel[e]; },
+  walk27<false, NB>(p, pred, se, c, [&](uint32_t e) { return vel[e]; },
          [&](uint32_t, const float4 ve, float dx, float dy, float dz, float d2) {
     const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     if (fast && r < c.h) {
@@ -802,6 +890,7 @@ __device__ __forceinline__ void vorticity_body(uint32_t i, const float4* __restr
   vort[i] = make_float4(wx, wy, wz, __builtin_sqrtf(wx * wx + wy * wy + wz * wz));
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pred,
                                                    const uint2* __restrict__ se,
                                                    const float4* __restrict__ vel,
@@ -811,12 +900,14 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
 {
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
+  RTP_NB_CTX
   if (t < n)
-    vorticity_body(po.particle(t), pred, se, vel, vort, c, W);
+    vorticity_body<NB>(po.particle(t), pred, se, vel, vort, c, W);
 }
 
 // fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
 // velOut (the copy fld_applyXsphViscosityCorrection reads, Fluids.cpp:451).
+template <int NB = 0>
 __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __restrict__ pred,
                                                  const uint2* __restrict__ se,
                                                  const float4* __restrict__ vort,
@@ -827,7 +918,7 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
   const float4 p = pred[i];
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
-  walk27(p, pred, se, c, [&](uint32_t e) { return vort[e].w; },
+  walk27<false, NB>(p, pred, se, c, [&](uint32_t e) { return vort[e].w; },
          [&](uint32_t, float l, float dx, float dy, float dz, float d2) {
     const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     if (fast && r < c.h) {
@@ -863,6 +954,7 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
   velOut[i] = make_float4(v.x + (k * crx) * dt, v.y + (k * cry) * dt, v.z + (k * crz) * dt, 0.0f);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ pred,
                                                      const uint2* __restrict__ se,
                                                      const float4* __restrict__ vort,
@@ -873,11 +965,13 @@ __global__ __launch_bounds__(256) void k_confinement(const float4* __restrict__ 
 {
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
+  RTP_NB_CTX
   if (t < n)
-    confinement_body(po.particle(t), pred, se, vort, vel, velOut, c, W);
+    confinement_body<NB>(po.particle(t), pred, se, vort, vel, velOut, c, W);
 }
 
 // fld_applyXsphViscosityCorrection (fluids.cl:383-430) + fld_updatePosition (:444-450).
+template <int NB = 0>
 __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__ pred,
                                           const uint2* __restrict__ se,
                                           const float4* __restrict__ velIn,
@@ -888,7 +982,7 @@ __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__
   const float4 vi = velIn[i];
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
-  walk27(p, pred, se, c, [&](uint32_t e) { return velIn[e]; },
+  walk27<false, NB>(p, pred, se, c, [&](uint32_t e) { return velIn[e]; },
          [&](uint32_t, const float4 ve, float, float, float, float d2) {
     const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
     const float t = r < c.h ? 1.0f : 0.0f;
@@ -902,6 +996,7 @@ __device__ __forceinline__ void xsph_body(uint32_t i, const float4* __restrict__
   posOut[i] = make_float4(p.x, p.y, p.z, 0.0f);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
                                               const uint2* __restrict__ se,
                                               const float4* __restrict__ velIn,
@@ -912,8 +1007,9 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
 {
   const uint32_t t = particle_index(remap);
   RTP_PLAIN_CTX(qdepth)
+  RTP_NB_CTX
   if (t < n)
-    xsph_body(po.particle(t), pred, se, velIn, velOut, posOut, c, W);
+    xsph_body<NB>(po.particle(t), pred, se, velIn, velOut, posOut, c, W);
 }
 
 // --------------------------------------------------------------- LDS-tiled wrappers
@@ -1240,6 +1336,8 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
 {
   if (n == 0)
     return;
+  if (L.tile)  // survivor lists are kept by the untiled kernels only
+    mode &= ~(kNbRecord | kNbReplay);
   const dim3 g(blocks_for(n, L.block)), b(L.block);
   const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_DL(M)                                                                              \
@@ -1259,6 +1357,8 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
     RTP_DL(kDensOnly)
     RTP_DL(kGated)
     RTP_DL(kGated | kClampIn)
+    RTP_DL(kNbRecord)
+    RTP_DL(kFeatures | kNbRecord)
   default: break;
   }
 #undef RTP_DL
@@ -1271,6 +1371,8 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
 {
   if (n == 0)
     return;
+  if (L.tile)
+    mode &= ~(kNbRecord | kNbReplay);
   const dim3 g(blocks_for(n, L.block)), b(L.block);
   const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_CO(M)                                                                              \
@@ -1286,6 +1388,7 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
     RTP_CO(0)
     RTP_CO(kGated)
     RTP_CO(kGated | kClampIn)
+    RTP_CO(kNbReplay)
   default: break;
   }
 #undef RTP_CO
@@ -1300,38 +1403,45 @@ void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, f
                      posOut, n, c);
 }
 
+// The untiled kernel for survivor-list mode nb (0, kNbRecord, kNbReplay).
+#define RTP_BY_NB(KERNEL)                                                                      \
+  [](int nb) {                                                                               \
+    return nb == kNbRecord ? KERNEL<kNbRecord> : (nb == kNbReplay ? KERNEL<kNbReplay> : KERNEL<0>); \
+  }
+
 void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
                           float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
-                          hipStream_t s, const PassOrder& po)
+                          hipStream_t s, const PassOrder& po, int nb)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(L.tile ? k_vorticity_t : k_vorticity, dim3(blocks_for(n, L.block)),
-                     dim3(L.block), queue_bytes(L), s, pred, startEnd, vel, vort, n, c,
-                     L.xcdRemap, (uint32_t)L.queue, po);
+  hipLaunchKernelGGL(L.tile ? k_vorticity_t : RTP_BY_NB(k_vorticity)(nb),
+                     dim3(blocks_for(n, L.block)), dim3(L.block), queue_bytes(L), s, pred,
+                     startEnd, vel, vort, n, c, L.xcdRemap, (uint32_t)L.queue, po);
 }
 
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
                             const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
-                            const Launch& L, hipStream_t s, const PassOrder& po)
+                            const Launch& L, hipStream_t s, const PassOrder& po, int nb)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(L.tile ? k_confinement_t : k_confinement, dim3(blocks_for(n, L.block)),
-                     dim3(L.block), queue_bytes(L), s, pred, startEnd, vort, vel, velOut, n, c,
-                     L.xcdRemap, (uint32_t)L.queue, po);
+  hipLaunchKernelGGL(L.tile ? k_confinement_t : RTP_BY_NB(k_confinement)(nb),
+                     dim3(blocks_for(n, L.block)), dim3(L.block), queue_bytes(L), s, pred,
+                     startEnd, vort, vel, velOut, n, c, L.xcdRemap, (uint32_t)L.queue, po);
 }
 
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
                      float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
-                     const Launch& L, hipStream_t s, const PassOrder& po)
+                     const Launch& L, hipStream_t s, const PassOrder& po, int nb)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(L.tile ? k_xsph_t : k_xsph, dim3(blocks_for(n, L.block)), dim3(L.block),
-                     queue_bytes(L), s, pred, startEnd, velIn, velOut, posOut, n, c, L.xcdRemap,
-                     (uint32_t)L.queue, po);
+  hipLaunchKernelGGL(L.tile ? k_xsph_t : RTP_BY_NB(k_xsph)(nb), dim3(blocks_for(n, L.block)),
+                     dim3(L.block), queue_bytes(L), s, pred, startEnd, velIn, velOut, posOut, n,
+                     c, L.xcdRemap, (uint32_t)L.queue, po);
 }
+#undef RTP_BY_NB
 
 void launch_density_error(const float* density, uint32_t n, float rho0, float* metric,
                           hipStream_t s)

@@ -244,6 +244,10 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
 
 // ----------------------------------------------------------------------------- tuning
 
+// Launch flags a single model takes.
+constexpr int kFluidsLaunchFlags =
+    RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
+
 // Validates a launch tuning; allowedFlags masks the RTP_LAUNCH_* bits the caller supports.
 int check_tuning(const rtp_tuning& t, int allowedFlags)
 {
@@ -289,7 +293,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
     rtp_tuning t = start;
     t.sortBits = bits;
     float ms = 0.0f;
-    if (check_tuning(t, singleModel ? 7 : RTP_LAUNCH_RESORT))
+    if (check_tuning(t, singleModel ? kFluidsLaunchFlags : RTP_LAUNCH_RESORT))
       continue;
     if ((rc = time_one(t, &ms)))
       break;
@@ -299,7 +303,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
     }
   }
   // single models: every flag combination; batches: with and without re-ordering
-  const int flagChoices = singleModel ? 8 : 2;
+  const int flagChoices = singleModel ? kFluidsLaunchFlags + 1 : 2;
   for (int b = 64; b <= 256 && !rc; b += 64)
     for (int fi = 0; fi < flagChoices && !rc; ++fi)
       for (int q : { 8, 12, 16, 24, 32 }) {
@@ -308,7 +312,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
         t.blockSize = b;
         t.flags = f;
         t.queueDepth = q;
-        if (check_tuning(t, singleModel ? 7 : RTP_LAUNCH_RESORT))
+        if (check_tuning(t, singleModel ? kFluidsLaunchFlags : RTP_LAUNCH_RESORT))
           continue;
         float ms = 0.0f;
         if ((rc = time_one(t, &ms)))

@@ -81,6 +81,8 @@ struct Launch {
   int queue = kDefaultQueue;  // survivor-queue depth of the neighbour passes (entries per lane)
   int resort = 0;      // 1: after every correction, order the neighbour passes' threads by
                        // the cells their particles now sit in
+  int nbList = 0;      // 1: the second pass of each pair over the same positions replays the
+                       // first one's survivor lists
 };
 
 // ---------------------------------------------------------------------------- grid / sort
@@ -135,14 +137,34 @@ struct LearnArgs {
   const float* gate = nullptr;       // summed residual density error (kGated)
   float tau = 0.0f;                  // threshold on *gate (the mean threshold times n)
 };
-enum PassMode { kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16 };
+// kNbRecord / kNbReplay: the pass records its threads' survivor lists / replays them
+// (NbList below).
+enum PassMode {
+  kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16, kNbRecord = 32, kNbReplay = 64
+};
+// Survivor lists.  Two neighbour passes over the same positions, the same cell table and
+// the same thread order visit the same candidates and run their bodies on the same
+// survivors in the same order.  The first records, for its thread t, the index of every
+// neighbour its body runs on -- the k-th (k < kNbCap) as component k % 4 of the 16-byte
+// granule list[(k / 4) * stride + t], so each thread stores four at a time and a wave's
+// stores and loads of one granule row are contiguous -- and their number to count[t].  The
+// second runs its body over the recorded indices instead of walking the cells, or walks
+// when count[t] > kNbCap.
+constexpr uint32_t kNbCap = 64;
+struct NbList {
+  uint4* list = nullptr;      // kNbCap / 4 x stride granules
+  uint32_t* count = nullptr;  // stride words
+  uint32_t stride = 0;        // threads of the pass
+};
 // Which particle each thread of a neighbour pass handles, and (correction passes) where to
 // write the cell key of every output position so the next pass can order its threads by
-// the cells they will walk.  Results do not depend on the order.
+// the cells they will walk.  Results do not depend on the order.  nb: the survivor lists
+// a kNbRecord / kNbReplay pass writes / reads.
 struct PassOrder {
   const uint32_t* order = nullptr;  // thread t handles particle order[t]; identity when null
   uint32_t* keyOut = nullptr;       // correction passes: cell key of each output position
   uint32_t keyBase = 0;             //   plus this (a batched scene's key base)
+  NbList nb;
   __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
 };
 // Counting sort of particles by cell for the thread order: count[] (m16 entries, a
@@ -208,18 +230,18 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
                         const PassOrder& po = PassOrder());
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
                            uint32_t n, const PbfConsts& c, hipStream_t s);
+// nb: 0, kNbRecord or kNbReplay (the post passes share their positions and thread order).
 void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float4* vel,
                           float4* vort, uint32_t n, const PbfConsts& c, const Launch& L,
-                          hipStream_t s,
-                        const PassOrder& po = PassOrder());
+                          hipStream_t s, const PassOrder& po = PassOrder(), int nb = 0);
 void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const float4* vort,
                             const float4* vel, float4* velOut, uint32_t n, const PbfConsts& c,
-                            const Launch& L, hipStream_t s,
-                        const PassOrder& po = PassOrder());
+                            const Launch& L, hipStream_t s, const PassOrder& po = PassOrder(),
+                            int nb = 0);
 void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* velIn,
                      float4* velOut, float4* posOut, uint32_t n, const PbfConsts& c,
-                     const Launch& L, hipStream_t s,
-                        const PassOrder& po = PassOrder());
+                     const Launch& L, hipStream_t s, const PassOrder& po = PassOrder(),
+                     int nb = 0);
 // Reduction of |rho/rho0 - 1| over n particles into metric[0..2] (max, sum, max
 // compression), deterministic for a given n; metric holds kMetricFloats floats, the ones
 // past metric[3] being per-block partials.

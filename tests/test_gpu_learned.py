@@ -162,14 +162,15 @@ def test_trained_network_beats_same_budget_without_it():
 
 def test_thread_order_keeps_learned_bits():
     """The learned substep (features, network, fallback, residual pass) gives the same bits
-    with the neighbour passes' threads re-ordered by current cell."""
+    with the neighbour passes' threads re-ordered by current cell and with survivor lists
+    (the second learned iteration records and replays them)."""
     sc = small_scene(23)
     p, _, _, _ = rand_net(6, 1)
     mlp = capi.Mlp()
     mlp.set_params(p * 0.1)
     mlp.set_normalization(np.zeros(8, np.float32), np.ones(8, np.float32), 1e-4)
     out = []
-    for flags in (0, capi.LAUNCH_RESORT):
+    for flags in (0, capi.LAUNCH_RESORT, capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST):
         m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
         m.set_params(**sc.params("stable"))
         m.load_state(sc.pos, sc.vel)
@@ -179,9 +180,10 @@ def test_thread_order_keeps_learned_bits():
         m.update(3)
         out.append((m.read_state(), m.read_field(capi.FIELD_LAMBDA), m.residual_error()))
         m.close()
-    _eq(out[1][0][0], out[0][0][0], "positions")
-    _eq(out[1][0][1], out[0][0][1], "velocities")
-    _eq(out[1][1], out[0][1], "lambda")
-    _eq(out[1][2], out[0][2], "residual")
+    for o in out[1:]:
+        _eq(o[0][0], out[0][0][0], "positions")
+        _eq(o[0][1], out[0][0][1], "velocities")
+        _eq(o[1], out[0][1], "lambda")
+        _eq(o[2], out[0][2], "residual")
     mlp.close()
 

@@ -180,12 +180,14 @@ def test_checkpoint_restore_replays_identically():
 @pytest.mark.parametrize("block,flags,queue", [(256, 2, 16), (128, 2, 24), (64, 3, 48),
                                                (192, 1, 64), (256, 3, 16), (256, 0, 64),
                                                (128, 0, 8), (64, 4, 64), (128, 6, 32),
-                                               (256, 7, 16), (192, 5, 40)])
+                                               (256, 7, 16), (192, 5, 40), (64, 13, 16),
+                                               (256, 8, 8), (128, 12, 24), (64, 15, 12)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
 def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
-    """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell and
-    the survivor-queue depth change where candidates are read from, which thread handles a
-    particle and when the neighbour bodies run -- never which candidates or in which order."""
+    """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell, the
+    survivor-queue depth and replayed survivor lists change where candidates are read from,
+    which thread handles a particle and when the neighbour bodies run -- never which
+    candidates or in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
     prm = sc.params(mapping)
     gpu, orc = make_pbf_pair(sc, iters=3, **{k: v for k, v in prm.items()})
@@ -199,11 +201,13 @@ def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
 
 
 
-@pytest.mark.parametrize("block,queue", [(256, 8), (64, 64), (128, 48)])
-def test_overfull_cell_keeps_bits(block, queue):
+@pytest.mark.parametrize("block,queue,flags", [(256, 8, 0), (64, 64, 0), (128, 48, 0),
+                                               (64, 16, 13), (256, 12, 8)])
+def test_overfull_cell_keeps_bits(block, queue, flags):
     """20,000 particles in one cell (the cap iterates 101 of them) and 50 in the next cell
-    of its column: the walks through them are no longer back-to-back ranges, and survivor
-    queues of every depth drain in the reference's order."""
+    of its column: the walks through them are no longer back-to-back ranges, survivor
+    queues of every depth drain in the reference's order, and particles with more
+    survivors than a recorded list holds walk again instead of replaying."""
     sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=8, extra=0)
     rng = np.random.default_rng(5)
     k = 20000
@@ -222,9 +226,9 @@ def test_overfull_cell_keeps_bits(block, queue):
     sc.n = sc.pos.shape[0]
     sc.max_n = (sc.n + 511) // 512 * 512 + 512
     gpu, orc = make_pbf_pair(sc, iters=2, **sc.params("stable"))
-    gpu.set_tuning(block, 0, 0, queue)
+    gpu.set_tuning(block, flags, 0, queue)
     for step in range(1, 3):
-        log(f"overfull cell, block {block}, queue {queue}, step {step}")
+        log(f"overfull cell, block {block}, queue {queue}, flags {flags}, step {step}")
         gpu.update(1)
         orc.step(1)
         compare_pbf(gpu, orc, sc.n, step)
