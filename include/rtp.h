@@ -86,7 +86,8 @@ typedef struct {
  * workload by timing it (rtp_fluids_autotune, rtp_batch_autotune). */
 typedef struct {
   int32_t blockSize;  /* threads per block of the per-particle passes: 64, 128, 192 or 256 */
-  int32_t flags;      /* RTP_LAUNCH_* bits (a batch takes RTP_LAUNCH_RESORT only) */
+  int32_t flags;      /* RTP_LAUNCH_* bits (a batch takes RTP_LAUNCH_RESORT and
+                         RTP_LAUNCH_NB_LIST) */
   int32_t sortBits;   /* largest radix digit width, 4..11 (0 = default 9) */
   int32_t queueDepth; /* neighbour survivors queued per lane in LDS, 8..64 (0 = default 16);
                          queueDepth x blockSize x 4 bytes must stay within 64 KiB, 24 KiB
@@ -263,7 +264,8 @@ int rtp_batch_sync(rtp_batch* h);
 int rtp_batch_save_checkpoint(rtp_batch* h);
 int rtp_batch_restore_checkpoint(rtp_batch* h);
 int rtp_batch_use_graph(rtp_batch* h, int enable);
-/* As rtp_fluids_set_tuning; of the flags only RTP_LAUNCH_RESORT applies to a batch. */
+/* As rtp_fluids_set_tuning; of the flags RTP_LAUNCH_RESORT and RTP_LAUNCH_NB_LIST apply to a
+ * batch (the lists take 256 bytes per particle plus 64 KiB per scene). */
 int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t);
 int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
 int rtp_batch_enable_profiling(rtp_batch* h, int enable);

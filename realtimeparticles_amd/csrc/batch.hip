@@ -56,6 +56,12 @@ struct rtp_batch {
   uint32_t* oorder = nullptr;
   uint32_t* ocount = nullptr;
   uint32_t* opart = nullptr;
+  // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled, for nbThreads launch
+  // threads (every block size's launch fits)
+  bool lists = false;
+  uint32_t nbThreads = 0;
+  uint4* nbList = nullptr;
+  uint32_t* nbCount = nullptr;
   bool constsDirty = true;
   int cur = 0;
   int predFinal = 1;
@@ -109,6 +115,8 @@ struct rtp_batch {
     dev_free(oorder);
     dev_free(ocount);
     dev_free(opart);
+    dev_free(nbList);
+    dev_free(nbCount);
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -188,18 +196,24 @@ struct rtp_batch {
     RTP_PROF(prof, "reorder_cells", s, launch_reorder_batch(bv, a, s));
     cur = 1 - cur;
     int in = 1;
-    // neighbour passes: threads in the grid's order first, then (resort) by current cell
+    // neighbour passes: threads in the grid's order first, then (resort) by current cell;
+    // with lists the density / vorticity passes record survivors and the correction,
+    // confinement and XSPH passes replay them
     BatchView nb = bv;
-    BatchView cb = bv;
+    if (lists)
+      nb.nb = NbList{ nbList, nbCount, bv.numBlocks * bv.block };
+    BatchView cb = nb;
     cb.keyOut = resort ? okey : nullptr;
+    const int rec = lists ? kNbRecord : 0, rep = lists ? kNbReplay : 0;
     const uint32_t m16 = (keySpace + 15u) & ~15u;
     for (uint32_t k = 0; k < iters; ++k) {
       const int last = (k + 1 == iters) ? kLastVel : kNextIter;
       RTP_PROF(prof, "pbf_density_lambda", s,
-               launch_pbf_density_lambda_batch(nb, pred[in], startEnd, density, lambda, s));
+               launch_pbf_density_lambda_batch(nb, pred[in], startEnd, density, lambda, s, rec));
       RTP_PROF(prof, "pbf_correct", s,
                launch_pbf_correct_batch(cb, pred[in], lambda, startEnd, pred[1 - in], pos[cur], vel[cur],
-                                        (last == kLastVel && !vortOn) ? pos[cur] : nullptr, last, s));
+                                        (last == kLastVel && !vortOn) ? pos[cur] : nullptr, last, s,
+                                        rep));
       in = 1 - in;
       if (resort) {
         RTP_PROF(prof, "thread_order", s,
@@ -213,10 +227,12 @@ struct rtp_batch {
                launch_pbf_update_vel_batch(bv, pred[in], pos[cur], vel[cur], vortOn ? nullptr : pos[cur], s));
     predFinal = in;
     if (vortOn) {
-      RTP_PROF(prof, "pbf_vorticity", s, launch_pbf_vorticity_batch(nb, pred[in], startEnd, vel[cur], vort, s));
+      RTP_PROF(prof, "pbf_vorticity", s,
+               launch_pbf_vorticity_batch(nb, pred[in], startEnd, vel[cur], vort, s, rec));
       RTP_PROF(prof, "pbf_confinement", s,
-               launch_pbf_confinement_batch(nb, pred[in], startEnd, vort, vel[cur], velTmp, s));
-      RTP_PROF(prof, "pbf_xsph", s, launch_pbf_xsph_batch(nb, pred[in], startEnd, velTmp, vel[cur], pos[cur], s));
+               launch_pbf_confinement_batch(nb, pred[in], startEnd, vort, vel[cur], velTmp, s, rep));
+      RTP_PROF(prof, "pbf_xsph", s,
+               launch_pbf_xsph_batch(nb, pred[in], startEnd, velTmp, vel[cur], pos[cur], s, rep));
     }
     stepped = true;
     return RTP_OK;
@@ -349,6 +365,8 @@ int rtp_batch_create(int device, const rtp_batch_scene* sc, uint32_t count, rtp_
   for (auto& S : h->scenes)
     maxBlocks += (S.slot.n + 63u) / 64u;
   const uint32_t N = h->total;
+  // ceil(n / b) * b < n + 256 for every block size b <= 256
+  h->nbThreads = N + 255u * (uint32_t)h->scenes.size();
   if ((rc = dev_alloc(&h->pos[0], N)) || (rc = dev_alloc(&h->pos[1], N)) ||
       (rc = dev_alloc(&h->vel[0], N)) || (rc = dev_alloc(&h->vel[1], N)) ||
       (rc = dev_alloc(&h->pred[0], N)) || (rc = dev_alloc(&h->pred[1], N)) ||
@@ -569,12 +587,21 @@ int rtp_batch_use_graph(rtp_batch* h, int enable)
 int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  int rc = check_tuning(*t, RTP_LAUNCH_RESORT);
+  int rc = check_tuning(*t, kBatchLaunchFlags);
   if (rc)
     return rc;
   RTP_HIP_TRY(hipSetDevice(h->device));
+  if ((t->flags & RTP_LAUNCH_NB_LIST) && !h->nbList) {
+    if ((rc = dev_alloc(&h->nbList, (size_t)(kNbCap / 4u) * h->nbThreads)) ||
+        (rc = dev_alloc(&h->nbCount, h->nbThreads))) {
+      dev_free(h->nbList);
+      h->nbList = nullptr;
+      return rc;
+    }
+  }
   h->sortBits = t->sortBits;
   h->resort = (t->flags & RTP_LAUNCH_RESORT) != 0;
+  h->lists = (t->flags & RTP_LAUNCH_NB_LIST) != 0;
   h->queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   if (h->block != t->blockSize) {
     h->block = t->blockSize;
@@ -598,7 +625,7 @@ int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best)
   start.blockSize = h->block;
   start.sortBits = h->sortBits;
   start.queueDepth = h->queue;
-  start.flags = h->resort ? RTP_LAUNCH_RESORT : 0;
+  start.flags = (h->resort ? RTP_LAUNCH_RESORT : 0) | (h->lists ? RTP_LAUNCH_NB_LIST : 0);
   auto run = [&](const rtp_tuning& t) { return rtp_batch_set_tuning(h, &t); };
   auto steps = [&](uint32_t k) {
     int r = 0;

@@ -1102,6 +1102,11 @@ __global__ __launch_bounds__(256) void k_xsph_t(const float4* __restrict__ pred,
 // particle and the cell table to its key base, so each body runs exactly as in a model of
 // its own (same indices, same summation order).
 
+// A batched pass's survivor lists: slot = the thread's launch-wide id.
+#define RTP_BATCH_NB_CTX \
+  W.nb = bv.nb;          \
+  W.slot = blockIdx.x * blockDim.x + threadIdx.x;
+
 struct BatchSite {
   uint32_t i;       // scene-local particle
   uint32_t scene;
@@ -1151,6 +1156,7 @@ __global__ __launch_bounds__(256) void k_predict_batch(BatchView bv, const float
   predict_body(b.i, pos + o, vel + o, pred + o, key + o, b.slot.kbase, c);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_density_lambda_batch(BatchView bv,
                                                               const float4* __restrict__ pred,
                                                               const uint2* __restrict__ se,
@@ -1163,9 +1169,11 @@ __global__ __launch_bounds__(256) void k_density_lambda_batch(BatchView bv,
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
-  density_lambda_body<0>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs(), W);
+  RTP_BATCH_NB_CTX
+  density_lambda_body<NB>(b.i, pred + o, se + b.slot.kbase, density + o, lambda + o, c, LearnArgs(), W);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float4* __restrict__ pred,
                                                        const float* __restrict__ lambda,
                                                        const uint2* __restrict__ se,
@@ -1180,7 +1188,8 @@ __global__ __launch_bounds__(256) void k_correct_batch(BatchView bv, const float
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
-  correct_body<0>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
+  RTP_BATCH_NB_CTX
+  correct_body<NB>(b.i, pred + o, lambda + o, se + b.slot.kbase, predOut + o, pos + o, vel + o,
                   posOut ? posOut + o : nullptr, last, c, LearnArgs(), W, batch_order(bv, b));
 }
 
@@ -1197,6 +1206,7 @@ __global__ __launch_bounds__(256) void k_update_vel_batch(BatchView bv, const fl
   update_vel_body(b.i, pred + o, pos + o, vel + o, posOut ? posOut + o : nullptr, c);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_vorticity_batch(BatchView bv, const float4* __restrict__ pred,
                                                          const uint2* __restrict__ se,
                                                          const float4* __restrict__ vel,
@@ -1208,9 +1218,11 @@ __global__ __launch_bounds__(256) void k_vorticity_batch(BatchView bv, const flo
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
-  vorticity_body(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c, W);
+  RTP_BATCH_NB_CTX
+  vorticity_body<NB>(b.i, pred + o, se + b.slot.kbase, vel + o, vort + o, c, W);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
                                                            const float4* __restrict__ pred,
                                                            const uint2* __restrict__ se,
@@ -1224,9 +1236,11 @@ __global__ __launch_bounds__(256) void k_confinement_batch(BatchView bv,
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
-  confinement_body(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c, W);
+  RTP_BATCH_NB_CTX
+  confinement_body<NB>(b.i, pred + o, se + b.slot.kbase, vort + o, vel + o, velOut + o, c, W);
 }
 
+template <int NB>
 __global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* __restrict__ pred,
                                                     const uint2* __restrict__ se,
                                                     const float4* __restrict__ velIn,
@@ -1239,7 +1253,8 @@ __global__ __launch_bounds__(256) void k_xsph_batch(BatchView bv, const float4* 
   const PbfConsts c = bv.consts[b.scene];
   const uint32_t o = b.slot.pbase;
   RTP_PLAIN_CTX(bv.qdepth)
-  xsph_body(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c, W);
+  RTP_BATCH_NB_CTX
+  xsph_body<NB>(b.i, pred + o, se + b.slot.kbase, velIn + o, velOut + o, posOut + o, c, W);
 }
 
 __global__ __launch_bounds__(64) void k_zero3(float* __restrict__ out)
@@ -1486,17 +1501,22 @@ void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const floa
   RTP_BATCH_STREAM(k_predict_batch, pos, vel, pred, key);
 }
 
+// The batched kernel instantiation for survivor-list mode nb.
+#define RTP_BATCH_NB(KERNEL) \
+  (nb == kNbRecord ? KERNEL<kNbRecord> : (nb == kNbReplay ? KERNEL<kNbReplay> : KERNEL<0>))
+
 void launch_pbf_density_lambda_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                                     float* density, float* lambda, hipStream_t s)
+                                     float* density, float* lambda, hipStream_t s, int nb)
 {
-  RTP_BATCH_LAUNCH(k_density_lambda_batch, pred, startEnd, density, lambda);
+  RTP_BATCH_LAUNCH(RTP_BATCH_NB(k_density_lambda_batch), pred, startEnd, density, lambda);
 }
 
 void launch_pbf_correct_batch(const BatchView& bv, const float4* pred, const float* lambda,
                               const uint2* startEnd, float4* predOut, const float4* pos,
-                              float4* vel, float4* posOut, int last, hipStream_t s)
+                              float4* vel, float4* posOut, int last, hipStream_t s, int nb)
 {
-  RTP_BATCH_LAUNCH(k_correct_batch, pred, lambda, startEnd, predOut, pos, vel, posOut, last);
+  RTP_BATCH_LAUNCH(RTP_BATCH_NB(k_correct_batch), pred, lambda, startEnd, predOut, pos, vel, posOut,
+                   last);
 }
 
 void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const float4* pos,
@@ -1506,24 +1526,25 @@ void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const 
 }
 
 void launch_pbf_vorticity_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                                const float4* vel, float4* vort, hipStream_t s)
+                                const float4* vel, float4* vort, hipStream_t s, int nb)
 {
-  RTP_BATCH_LAUNCH(k_vorticity_batch, pred, startEnd, vel, vort);
+  RTP_BATCH_LAUNCH(RTP_BATCH_NB(k_vorticity_batch), pred, startEnd, vel, vort);
 }
 
 void launch_pbf_confinement_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
                                   const float4* vort, const float4* vel, float4* velOut,
-                                  hipStream_t s)
+                                  hipStream_t s, int nb)
 {
-  RTP_BATCH_LAUNCH(k_confinement_batch, pred, startEnd, vort, vel, velOut);
+  RTP_BATCH_LAUNCH(RTP_BATCH_NB(k_confinement_batch), pred, startEnd, vort, vel, velOut);
 }
 
 void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s)
+                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s, int nb)
 {
-  RTP_BATCH_LAUNCH(k_xsph_batch, pred, startEnd, velIn, velOut, posOut);
+  RTP_BATCH_LAUNCH(RTP_BATCH_NB(k_xsph_batch), pred, startEnd, velIn, velOut, posOut);
 }
 
+#undef RTP_BATCH_NB
 #undef RTP_BATCH_LAUNCH
 #undef RTP_BATCH_STREAM
 

@@ -244,9 +244,10 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
 
 // ----------------------------------------------------------------------------- tuning
 
-// Launch flags a single model takes.
+// Launch flags a single model / a batch takes.
 constexpr int kFluidsLaunchFlags =
     RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
+constexpr int kBatchLaunchFlags = RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
 
 // Validates a launch tuning; allowedFlags masks the RTP_LAUNCH_* bits the caller supports.
 int check_tuning(const rtp_tuning& t, int allowedFlags)
@@ -293,7 +294,7 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
     rtp_tuning t = start;
     t.sortBits = bits;
     float ms = 0.0f;
-    if (check_tuning(t, singleModel ? kFluidsLaunchFlags : RTP_LAUNCH_RESORT))
+    if (check_tuning(t, singleModel ? kFluidsLaunchFlags : kBatchLaunchFlags))
       continue;
     if ((rc = time_one(t, &ms)))
       break;
@@ -302,17 +303,17 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
       top.msPerStep = ms;
     }
   }
-  // single models: every flag combination; batches: with and without re-ordering
-  const int flagChoices = singleModel ? kFluidsLaunchFlags + 1 : 2;
+  // every combination of the flags the caller takes
+  const int flagChoices = singleModel ? kFluidsLaunchFlags + 1 : 4;
   for (int b = 64; b <= 256 && !rc; b += 64)
     for (int fi = 0; fi < flagChoices && !rc; ++fi)
       for (int q : { 8, 12, 16, 24, 32 }) {
-        const int f = singleModel ? fi : (fi ? RTP_LAUNCH_RESORT : 0);
+        const int f = singleModel ? fi : ((fi & 1) ? RTP_LAUNCH_RESORT : 0) | ((fi & 2) ? RTP_LAUNCH_NB_LIST : 0);
         rtp_tuning t = top;
         t.blockSize = b;
         t.flags = f;
         t.queueDepth = q;
-        if (check_tuning(t, singleModel ? kFluidsLaunchFlags : RTP_LAUNCH_RESORT))
+        if (check_tuning(t, singleModel ? kFluidsLaunchFlags : kBatchLaunchFlags))
           continue;
         float ms = 0.0f;
         if ((rc = time_one(t, &ms)))

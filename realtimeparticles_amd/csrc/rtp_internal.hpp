@@ -196,6 +196,7 @@ struct BatchView {
   uint32_t qdepth;                  // survivor-queue depth of the neighbour passes
   const uint32_t* order = nullptr;  // batch-wide thread order (scene slices stay in place)
   uint32_t* keyOut = nullptr;       // correction passes: batch-wide key of each output
+  NbList nb;                        // survivor lists; a thread's slot is its launch-wide id
 };
 void launch_fill_start_end(uint2* startEnd, uint32_t count, hipStream_t s);
 void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const float4* vel,
@@ -203,19 +204,19 @@ void launch_pbf_predict_batch(const BatchView& bv, const float4* pos, const floa
 // ReorderArgs with whole-batch pointers; per-scene counts, bounds and grid come from bv.
 void launch_reorder_batch(const BatchView& bv, const ReorderArgs& a, hipStream_t s);
 void launch_pbf_density_lambda_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                                     float* density, float* lambda, hipStream_t s);
+                                     float* density, float* lambda, hipStream_t s, int nb = 0);
 void launch_pbf_correct_batch(const BatchView& bv, const float4* pred, const float* lambda,
                               const uint2* startEnd, float4* predOut, const float4* pos,
-                              float4* vel, float4* posOut, int last, hipStream_t s);
+                              float4* vel, float4* posOut, int last, hipStream_t s, int nb = 0);
 void launch_pbf_update_vel_batch(const BatchView& bv, const float4* pred, const float4* pos,
                                  float4* vel, float4* posOut, hipStream_t s);
 void launch_pbf_vorticity_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                                const float4* vel, float4* vort, hipStream_t s);
+                                const float4* vel, float4* vort, hipStream_t s, int nb = 0);
 void launch_pbf_confinement_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
                                   const float4* vort, const float4* vel, float4* velOut,
-                                  hipStream_t s);
+                                  hipStream_t s, int nb = 0);
 void launch_pbf_xsph_batch(const BatchView& bv, const float4* pred, const uint2* startEnd,
-                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s);
+                           const float4* velIn, float4* velOut, float4* posOut, hipStream_t s, int nb = 0);
 
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,

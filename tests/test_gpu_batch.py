@@ -105,7 +105,9 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
     b.update(4)
     ref = [b.read_state(k) for k in range(len(scs))]
     for block, bits, queue, flags in ((64, 7, 64, 0), (192, 11, 16, capi.LAUNCH_RESORT),
-                                      (128, 0, 48, capi.LAUNCH_RESORT)):
+                                      (128, 0, 48, capi.LAUNCH_RESORT),
+                                      (64, 0, 16, capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST),
+                                      (256, 8, 12, capi.LAUNCH_NB_LIST)):
         b.restore_checkpoint()
         b.set_tuning(block, bits, queue, flags)
         b.use_graph(False)
@@ -134,6 +136,8 @@ def test_batch_256_scenes_sample_equals_separate_models():
     total, cells = b.totals()
     log(f"256 scenes: {total} particles, {cells} cell keys")
     assert total == sum(sc.n for sc in scs)
+    # the shape the batch's autotuner picks: thread order and survivor lists
+    b.set_tuning(64, 0, 24, capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST)
     b.update(2)
     pick = [0, 77, 128, 255]
     for k in pick:
