@@ -173,8 +173,8 @@ def run_batched(args):
 
     Both time fixed-input substeps (restore every scene's input, one substep).  The batch is
     autotuned first (block size x sort digit width x thread order x queue depth), or takes
-    the shape given by --block/--flags/--sort-bits/--queue (of the flags only the thread
-    order applies to a batch), or the default shape with --no-tune.  The sequential run
+    the shape given by --block/--flags/--sort-bits/--queue (of the flags the thread order
+    and the survivor lists apply to a batch), or the default shape with --no-tune.  The sequential run
     synchronises after every scene, so scenes do not overlap.
     """
     from realtimeparticles_amd import capi, scenes
@@ -187,9 +187,9 @@ def run_batched(args):
     b.set_nb_jacobi_iters(4)
     b.save_checkpoint()
     if args.block or args.flags or args.sort_bits or args.queue:
-        b.set_tuning(args.block or 256, args.sort_bits or 0, args.queue or 0,
-                     args.flags & capi.LAUNCH_RESORT)
-        tuning = {"block": args.block or 256, "flags": args.flags & capi.LAUNCH_RESORT,
+        bflags = args.flags & (capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST)
+        b.set_tuning(args.block or 256, args.sort_bits or 0, args.queue or 0, bflags)
+        tuning = {"block": args.block or 256, "flags": bflags,
                   "sort_bits": args.sort_bits or 0, "queue": args.queue or 0}
     elif args.no_tune:
         tuning = None
