@@ -61,10 +61,13 @@ def test_pbf_parameter_variants(variant):
     gpu.close()
 
 
-def test_graph_replay_equals_direct_launches():
+@pytest.mark.parametrize("flags", [0, 13])
+def test_graph_replay_equals_direct_launches(flags):
+    """flags 13: XCD blocks, thread order and survivor lists inside the captured graphs."""
     sc = small_scene(seed=4)
     a, _ = make_pbf_pair(sc, iters=4)
     b, _ = make_pbf_pair(sc, iters=4)
+    a.set_tuning(64, flags, 0, 16)
     b.use_graph(False)
     for rep in range(3):
         log(f"graph replay {rep}")
@@ -85,11 +88,14 @@ def test_graph_replay_equals_direct_launches():
     b.close()
 
 
-def test_nb_particles_change_keeps_reference_sort():
+@pytest.mark.parametrize("flags", [0, 13])
+def test_nb_particles_change_keeps_reference_sort(flags):
     # Model::setNbParticles (Model.hpp:106) does not reset the grid: the keys of particles
-    # that left the active range stay in the reference's whole-buffer sort.
+    # that left the active range stay in the reference's whole-buffer sort.  flags 13: the
+    # thread order's and survivor lists' sizes follow the count.
     sc = small_scene(seed=5)
     gpu, orc = make_pbf_pair(sc, iters=2)
+    gpu.set_tuning(128, flags, 0, 16)
     gpu.update(1)
     orc.step(1)
     compare_pbf(gpu, orc, sc.n, 1)
