@@ -249,7 +249,7 @@ def test_pbf_65k_long_run(mapping):
     over the whole run, finite state with w = 0, and max |rho/rho0 - 1| recorded."""
     sc = scenes.pbf_scene("pbf-65k")
     gpu, orc = make_pbf_pair(sc, iters=4, **sc.params(mapping))
-    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP, 0, 32)
+    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP | capi.LAUNCH_NB_LIST, 0, 16)
     gpu.save_checkpoint()
     for step in range(1, 31):
         gpu.update(1)
