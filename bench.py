@@ -16,7 +16,8 @@ after another under config.batched_scenes.
 Prints one JSON line.  Besides the throughput it carries
   roofline     : the dominant kernel's algorithmic HBM bytes per launch / its mean launch
                  time (HIP events around each launch on the model's stream, in a second timed
-                 window with per-kernel events), against MI355X's 8 TB/s;
+                 window with per-kernel events), against MI355X's 8 TB/s, beside the
+                 bandwidth a plain device-to-device copy reaches on this GPU (copy_gbs);
   cpu_baseline : the CPU oracle (the reference's kernels restated in C, OpenMP over
                  particles) timed on this host on a bounded sample of the same workload.
 
@@ -64,6 +65,31 @@ KERNEL_OF = {"pbf_predict_hash": "k_predict", "reorder_cells": "k_reorder",
              "pbf_density_lambda": "k_density_lambda", "pbf_correct": "k_correct",
              "pbf_vorticity": "k_vorticity", "pbf_confinement": "k_confinement",
              "pbf_xsph": "k_xsph"}
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10):
+    """Device-to-device copy bandwidth (read + write bytes per second, GB/s) of one
+    nbytes buffer, timed with events on torch's current stream; None without a GPU."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            dst.copy_(src)
+        t1.record()
+        t1.synchronize()
+        gbs = 2.0 * nbytes * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+        del src, dst
+        torch.cuda.empty_cache()
+        return round(gbs, 1)
+    except Exception:
+        return None
 
 
 def measured_traffic(workload: str, name: str):
@@ -365,7 +391,8 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                     "traffic": None if traffic is None else round(traffic),
                     "kernel": dom_name,
-                    "bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 5)}
+                    "bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 5),
+                    "copy_gbs": copy_bandwidth()}
 
     # diagnostic only: the same workload with the "stable" coefficient mapping
     stable = None
