@@ -67,28 +67,37 @@ KERNEL_OF = {"pbf_predict_hash": "k_predict", "reorder_cells": "k_reorder",
              "pbf_xsph": "k_xsph"}
 
 
-def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10):
-    """Device-to-device copy bandwidth (read + write bytes per second, GB/s) of one
-    nbytes buffer, timed with events on torch's current stream; None without a GPU."""
+_COPY_PROBE = """
+import torch
+n, reps = 1 << 30, 10
+src = torch.empty(n, dtype=torch.uint8, device="cuda")
+dst = torch.empty_like(src)
+for _ in range(3):
+    dst.copy_(src)
+t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+t0.record()
+for _ in range(reps):
+    dst.copy_(src)
+t1.record()
+t1.synchronize()
+print(2.0 * n * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9)
+"""
+
+
+def copy_bandwidth():
+    """Device-to-device copy bandwidth (read + write bytes per second, GB/s) of a 1 GiB
+    buffer, timed with events by torch in a child process (torch and this library bring
+    different HIP runtimes, which do not share one process); None if that fails."""
+    import subprocess
     try:
-        import torch
-        if not torch.cuda.is_available():
+        out = subprocess.run([sys.executable, "-c", _COPY_PROBE], capture_output=True, text=True,
+                             timeout=120)
+        if out.returncode != 0:
+            print(f"copy bandwidth not measured: {out.stderr.strip()[-300:]}", file=sys.stderr)
             return None
-        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0.record()
-        for _ in range(reps):
-            dst.copy_(src)
-        t1.record()
-        t1.synchronize()
-        gbs = 2.0 * nbytes * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
-        del src, dst
-        torch.cuda.empty_cache()
-        return round(gbs, 1)
-    except Exception:
+        return round(float(out.stdout.strip().splitlines()[-1]), 1)
+    except Exception as e:
+        print(f"copy bandwidth not measured: {e!r}", file=sys.stderr)
         return None
 
 
@@ -371,6 +380,7 @@ def main():
     from realtimeparticles_amd import scenes
     sc = scenes.pbf_scene(args.workload)
     nc = sc.res[0] * sc.res[1] * sc.res[2]
+    copy_gbs = copy_bandwidth()  # before this process touches the GPU
 
     g = run_gpu(args, sc)
     elapsed, prof, prof_steps = g["elapsed"], g["prof"], g["prof_steps"]
@@ -392,7 +402,7 @@ def main():
                     "traffic": None if traffic is None else round(traffic),
                     "kernel": dom_name,
                     "bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 5),
-                    "copy_gbs": copy_bandwidth()}
+                    "copy_gbs": copy_gbs}
 
     # diagnostic only: the same workload with the "stable" coefficient mapping
     stable = None

@@ -360,11 +360,15 @@ struct rtp_fluids {
       for (uint32_t f = 0; f < fallbackIters; ++f) {
         const int mode = kGated | (f == 0 ? kClampIn : 0);
         const int last = (f + 1 == fallbackIters) ? kLastOnly : kNextIter;
+        // a gated pair reads the same (clamped-on-load) positions: the correction replays
+        // the density pass's lists; with the gate closed neither walks nor reads them
         RTP_PROF(prof, "pbf_fallback", s,
-                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s, mode, la, po));
+                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s,
+                                           lists ? (mode | kNbRecord) : mode, la, po));
         RTP_PROF(prof, "pbf_fallback", s,
                  launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
-                                    nullptr, n, last, c, L, s, mode, la, pc));
+                                    nullptr, n, last, c, L, s, lists ? (mode | kNbReplay) : mode,
+                                    la, pc));
         in = 1 - in;
         if ((rc = order_threads()))
           return rc;

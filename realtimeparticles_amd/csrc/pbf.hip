@@ -1374,6 +1374,8 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
     RTP_DL(kGated | kClampIn)
     RTP_DL(kNbRecord)
     RTP_DL(kFeatures | kNbRecord)
+    RTP_DL(kGated | kNbRecord)
+    RTP_DL(kGated | kClampIn | kNbRecord)
   default: break;
   }
 #undef RTP_DL
@@ -1404,6 +1406,8 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
     RTP_CO(kGated)
     RTP_CO(kGated | kClampIn)
     RTP_CO(kNbReplay)
+    RTP_CO(kGated | kNbReplay)
+    RTP_CO(kGated | kClampIn | kNbReplay)
   default: break;
   }
 #undef RTP_CO
