@@ -78,7 +78,8 @@ def build(verbose: bool = False, jobs: int = 4, extra_flags=()) -> str:
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs,
+          "-lrocprofiler-sdk-roctx"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -262,6 +262,7 @@ struct rtp_batch {
   {
     if (nsteps == 0 || total == 0)
       return RTP_OK;
+    MarkRange mark("rtp_batch_update");
     RTP_HIP_TRY(hipSetDevice(device));
     int rc = upload_consts();
     if (rc)
@@ -272,13 +273,17 @@ struct rtp_batch {
         if ((rc = capture_pair()))
           return rc;
       }
-      for (; done + 2 <= nsteps; done += 2)
+      for (; done + 2 <= nsteps; done += 2) {
+        MarkRange pair("rtp_batch_substep_pair_graph");
         RTP_HIP_TRY(hipGraphLaunch(graphExec, stream));
+      }
       stepped = true;
     }
-    for (; done < nsteps; ++done)
+    for (; done < nsteps; ++done) {
+      MarkRange one("rtp_batch_substep");
       if ((rc = substep()))
         return rc;
+    }
     return RTP_OK;
   }
 

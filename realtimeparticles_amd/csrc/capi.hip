@@ -425,6 +425,7 @@ struct rtp_fluids {
   {
     if (paused || nsteps == 0)
       return RTP_OK;
+    MarkRange mark("rtp_fluids_update");
     RTP_HIP_TRY(hipSetDevice(device));
     uint32_t done = 0;
     if (useGraph && !prof.enabled && nsteps >= 2 && n > 0) {
@@ -433,10 +434,13 @@ struct rtp_fluids {
         if (rc)
           return rc;
       }
-      for (; done + 2 <= nsteps; done += 2)
+      for (; done + 2 <= nsteps; done += 2) {
+        MarkRange pair("rtp_fluids_substep_pair_graph");
         RTP_HIP_TRY(hipGraphLaunch(graphExec, stream));
+      }
     }
     for (; done < nsteps; ++done) {
+      MarkRange one("rtp_fluids_substep");
       const int rc = substep();
       if (rc)
         return rc;

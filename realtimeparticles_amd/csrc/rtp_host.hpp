@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "rtp_internal.hpp"
 
 #define RTP_REQUIRE(cond, msg)                    \
@@ -22,6 +24,15 @@
 
 namespace rtp {
 namespace {
+
+// A roctx range around host work that enqueues device work (visible with
+// rocprofv3 --marker-trace): update calls, graph launches and direct substeps.
+struct MarkRange {
+  explicit MarkRange(const char* name) { roctxRangePush(name); }
+  ~MarkRange() { roctxRangePop(); }
+  MarkRange(const MarkRange&) = delete;
+  MarkRange& operator=(const MarkRange&) = delete;
+};
 
 // ------------------------------------------------------------------------- host constants
 
