@@ -318,3 +318,33 @@ def test_general_forms_outside_short_range(box):
         orc.step(1)
         compare_pbf(gpu, orc, sc.n, step)
     gpu.close()
+
+
+@pytest.mark.parametrize("flags", [0, 13])
+def test_bad_state_matches_oracle(flags):
+    """NaN and +-inf coordinates, a NaN velocity, particles outside the walls and exactly on
+    +-W in a dam break: the hash aliases and clamps them (grid.cl:14-38), the first clamp
+    turns NaN positions into the lower wall, far and NaN neighbours take the exact
+    expressions -- every field against the oracle, bit for bit (NaN compares equal)."""
+    sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=14, extra=0)
+    bad = np.array([
+        [np.nan, 0.2, 0.2, 0], [0.3, np.inf, 0.2, 0], [0.3, 0.2, -np.inf, 0],
+        [2.5, 0.1, 0.1, 0], [0.1, -1.3, 0.1, 0], [0.1, 0.1, 1.7, 0],
+        [2.0, 0.5, 0.5, 0], [-2.0, 0.5, 0.5, 0], [0.5, 1.0, -1.0, 0],
+        [0.31, 0.21, 0.21, 0], [0.32, 0.22, 0.2, 0], [1.95, 0.12, 0.1, 0],
+    ], np.float32)
+    vel = np.zeros_like(bad)
+    vel[9, :3] = np.nan
+    sc.pos = np.concatenate([sc.pos, bad])
+    sc.vel = np.concatenate([sc.vel, vel])
+    sc.n = sc.pos.shape[0]
+    sc.max_n = (sc.n + 511) // 512 * 512 + 512
+    for mapping in ("stable", "literal"):
+        gpu, orc = make_pbf_pair(sc, iters=2, **sc.params(mapping))
+        gpu.set_tuning(64, flags, 0, 16)
+        for step in range(1, 4):
+            log(f"bad state, flags {flags}, {mapping}, step {step}")
+            gpu.update(1)
+            orc.step(1)
+            compare_pbf(gpu, orc, sc.n, step)
+        gpu.close()
