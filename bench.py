@@ -352,7 +352,8 @@ def main():
     ap.add_argument("--workload", default="pbf-1m")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--flags", type=int, default=0,
-                    help="launch flags: 1 XCD-contiguous blocks, 2 LDS wave tiles")
+                    help="launch flags: 1 XCD-contiguous blocks, 2 LDS wave tiles, 4 thread order "
+                         "by current cell, 8 survivor lists")
     ap.add_argument("--sort-bits", type=int, default=0)
     ap.add_argument("--queue", type=int, default=0, help="survivor-queue depth (0 = 16)")
     ap.add_argument("--no-tune", action="store_true",

@@ -500,7 +500,7 @@ class Batch:
         _check(lib().rtp_batch_use_graph(self._h, int(bool(flag))), "use_graph")
 
     def set_tuning(self, block: int = 256, sort_bits: int = 0, queue: int = 0, flags: int = 0) -> None:
-        """flags: LAUNCH_RESORT or 0."""
+        """flags: LAUNCH_RESORT | LAUNCH_NB_LIST, or 0."""
         t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_batch_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
