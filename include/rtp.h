@@ -110,7 +110,11 @@ typedef enum {
   RTP_FIELD_LAMBDA = 1,    /* p_constFactor (float) */
   RTP_FIELD_PRED_POS = 2,  /* p_predPos (float4, final) */
   RTP_FIELD_VORTICITY = 3, /* p_vort (float4; w = |vorticity|) */
-  RTP_FIELD_ACC = 4        /* boids p_acc (float4) */
+  RTP_FIELD_ACC = 4,       /* boids p_acc (float4) */
+  /* with the rendering steps enabled (rtp_*_set_render): */
+  RTP_FIELD_COLOR = 5,       /* p_col (float4) */
+  RTP_FIELD_CAMERA_DIST = 6, /* p_cameraDist (uint32 words, sorted by the last update) */
+  RTP_FIELD_DETECTOR = 7     /* c_partDetector (float8 per cell; count = cells) */
 } rtp_field;
 
 /* ---- library ---- */
@@ -174,6 +178,19 @@ int rtp_fluids_restore_checkpoint(rtp_fluids* h);
 /* Replay pairs of substeps from a captured hipGraph (no reference counterpart; the
  * reference enqueues every kernel each frame).  Default on. */
 int rtp_fluids_use_graph(rtp_fluids* h, int enable);
+/* The rendering steps at the end of every update, Fluids.cpp:458-468: the grid detector
+ * (fillGridDetector, grid.cl:51-60), the colour from p_density (fld_fillFluidColor,
+ * fluids.cl:458-478) and, paused or not, the camera distances (fillCameraDist,
+ * utils.cl:43-52) with the second RadixSort::sort that permutes p_pos, p_col, p_vel and
+ * p_predPos.  The reference always runs them; here they are off by default, because they
+ * only serve drawing and they change the next update's within-cell order.  Enabling resets
+ * p_col, p_cameraDist and the detector as a model reset does (Fluids.cpp:209-215), and so
+ * does every rtp_fluids_load_state while enabled.  Needs 48 bytes per particle of
+ * maxNbParticles and 32 per cell, allocated the first time. */
+int rtp_fluids_set_render(rtp_fluids* h, int enable);
+/* u_cameraPos (Fluids.cpp:132): the point the following updates sort by distance to.
+ * The origin until set. */
+int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3]);
 /* Launch tuning (no reference counterpart), see rtp_tuning (msPerStep is ignored).  Results
  * do not depend on any of it. */
 int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t);
@@ -294,6 +311,13 @@ int rtp_boids_sync(rtp_boids* h);
 int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
 int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count);
 int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4);
+/* Boids.cpp:375-381: the grid detector after every step and, paused or not, the
+ * camera-distance sort that permutes p_pos, p_vel and p_acc (p_col holds bd_fillBoidsColor's
+ * single colour, boids.cl:38-41).  As rtp_fluids_set_render: off by default, enabling resets
+ * p_col, p_cameraDist and the detector (Boids.cpp:265-272). */
+int rtp_boids_set_render(rtp_boids* h, int enable);
+/* u_cameraPos (Boids.cpp:126). */
+int rtp_boids_set_camera(rtp_boids* h, const float pos[3]);
 int rtp_boids_enable_profiling(rtp_boids* h, int enable);
 int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* count);
 

@@ -96,6 +96,32 @@ class _BoidsState(ctypes.Structure):
     ]
 
 
+class _Render(ctypes.Structure):
+    _fields_ = [
+        ("cameraPos", _f32 * 3),
+        ("col", _vp), ("cameraDist", _vp), ("detector", _vp),
+    ]
+
+
+class _RenderBuffers:
+    """p_col, p_cameraDist and c_partDetector of one model, reset as at model reset
+    (Fluids.cpp:209-215, Boids.cpp:265-272)."""
+
+    def __init__(self, max_n: int, c, camera, col_init):
+        self.col = np.empty((max_n, 4), np.float32)
+        self.col[:] = col_init
+        self.dist = np.zeros(max_n, np.uint32)
+        self.det = np.zeros((int(c.numCells), 8), np.float32)
+        self.r = _Render()
+        self.r.col, self.r.cameraDist, self.r.detector = ptr(self.col), ptr(self.dist), ptr(self.det)
+        self.set_camera(camera)
+        lib().orc_reset_camera_dist(ptr(self.dist), max_n)
+
+    def set_camera(self, camera) -> None:
+        for k in range(3):
+            self.r.cameraPos[k] = float(camera[k])
+
+
 _lib = None
 
 
@@ -153,6 +179,16 @@ def lib():
         L.orc_fluids_update.argtypes = [P(_FluidsState)]
         L.orc_boids_update.restype = ctypes.c_int
         L.orc_boids_update.argtypes = [P(_BoidsState)]
+        L.orc_reset_camera_dist.argtypes = [_vp, _u32]
+        L.orc_fill_camera_dist.argtypes = [_vp, P(_f32), _vp, _u32]
+        L.orc_fld_fill_color.argtypes = [_vp, P(FluidParams), _vp, _u32]
+        L.orc_bd_fill_color.argtypes = [_vp, _u32]
+        L.orc_reset_detector.argtypes = [_vp, P(Consts)]
+        L.orc_fill_detector.argtypes = [_vp, _vp, _u32, P(Consts)]
+        L.orc_fluids_frame.restype = ctypes.c_int
+        L.orc_fluids_frame.argtypes = [P(_FluidsState), P(_Render), ctypes.c_int]
+        L.orc_boids_frame.restype = ctypes.c_int
+        L.orc_boids_frame.argtypes = [P(_BoidsState), P(_Render), ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -207,6 +243,8 @@ class OracleFluids:
         self.key = np.zeros(self.max_n, np.uint32)
         self.idx = np.zeros(self.max_n, np.uint32)
         self.se = np.zeros((nc, 2), np.uint32)
+        self.render = None
+        self.paused = False
         self.s = _FluidsState()
         self.s.c = self.c
         self.s.maxN, self.s.n, self.s.iters = self.max_n, self.n, int(iters)
@@ -231,11 +269,23 @@ class OracleFluids:
         self.vel[:] = 0.0
         self.pos[:m] = pos
         self.vel[:m] = vel
+        self.pred[:] = self.pos
         lib().orc_reset_cell_ids(ptr(self.key), self.max_n, ctypes.byref(self.c))
+
+    def enable_render(self, camera) -> None:
+        """The rendering steps of every frame (Fluids.cpp:458-468), with p_col at its
+        initial colour (Fluids.cpp:394) and the camera distances reset (:215)."""
+        self.render = _RenderBuffers(self.max_n, self.c, camera, (0.0, 0.1, 1.0, 0.0))
 
     def step(self, nsteps: int = 1) -> None:
         for _ in range(nsteps):
-            rc = lib().orc_fluids_update(ctypes.byref(self.s))
+            if self.render is not None:
+                rc = lib().orc_fluids_frame(ctypes.byref(self.s), ctypes.byref(self.render.r),
+                                            int(self.paused))
+            elif self.paused:
+                continue
+            else:
+                rc = lib().orc_fluids_update(ctypes.byref(self.s))
             if rc:
                 raise RuntimeError(f"orc_fluids_update failed ({rc})")
 
@@ -253,6 +303,8 @@ class OracleBoids:
         self.key = np.zeros(self.max_n, np.uint32)
         self.idx = np.zeros(self.max_n, np.uint32)
         self.se = np.zeros((nc, 2), np.uint32)
+        self.render = None
+        self.paused = False
         self.s = _BoidsState()
         self.s.c = self.c
         self.s.p = params
@@ -272,8 +324,19 @@ class OracleBoids:
         self.vel[:m] = vel
         lib().orc_reset_cell_ids(ptr(self.key), self.max_n, ctypes.byref(self.c))
 
+    def enable_render(self, camera) -> None:
+        """The rendering steps of every frame (Boids.cpp:375-381), with p_col filled by
+        bd_fillBoidsColor and the camera distances reset (Boids.cpp:267-272)."""
+        self.render = _RenderBuffers(self.max_n, self.c, camera, (1.0, 0.02, 0.02, 0.5))
+
     def step(self, nsteps: int = 1) -> None:
         for _ in range(nsteps):
-            rc = lib().orc_boids_update(ctypes.byref(self.s))
+            if self.render is not None:
+                rc = lib().orc_boids_frame(ctypes.byref(self.s), ctypes.byref(self.render.r),
+                                           int(self.paused))
+            elif self.paused:
+                continue
+            else:
+                rc = lib().orc_boids_update(ctypes.byref(self.s))
             if rc:
                 raise RuntimeError(f"orc_boids_update failed ({rc})")

@@ -687,8 +687,9 @@ void orc_bd_update_pos_wall(orc_f4* vel, float dt, orc_f4* pos, uint32_t n, cons
  * Host step drivers
  * ---------------------------------------------------------------------------------- */
 
-/* physics/ocl/Fluids.cpp:400-471 (pause, colour, grid detector and camera sort omitted). */
-int orc_fluids_update(orc_fluids* s)
+/* physics/ocl/Fluids.cpp:409-457, the physics of one frame.  col (p_col), when given,
+ * rides along in the cell sort's permutation like the other payloads (:417). */
+static int fluids_physics(orc_fluids* s, orc_f4* col)
 {
   const orc_consts* c = &s->c;
   const uint32_t n = s->n;
@@ -698,6 +699,8 @@ int orc_fluids_update(orc_fluids* s)
   if (rc)
     return rc;
   orc_permute_f4(s->idx, s->pos, s->tmp, s->maxN);                     /* p_pos */
+  if (col)
+    orc_permute_f4(s->idx, col, s->tmp, s->maxN);                      /* p_col */
   orc_permute_f4(s->idx, s->vel, s->tmp, s->maxN);                     /* p_vel */
   orc_permute_f4(s->idx, s->pred, s->tmp, s->maxN);                    /* p_predPos */
   orc_reset_start_end(s->se, c);                                       /* :419 */
@@ -722,8 +725,12 @@ int orc_fluids_update(orc_fluids* s)
   return 0;
 }
 
-/* physics/ocl/Boids.cpp:323-384 (3-D rules, bouncing wall, no target, no rendering). */
-int orc_boids_update(orc_boids* s)
+/* physics/ocl/Fluids.cpp:400-471 (pause, colour, grid detector and camera sort omitted). */
+int orc_fluids_update(orc_fluids* s) { return fluids_physics(s, NULL); }
+
+/* physics/ocl/Boids.cpp:332-372, the step of one frame (3-D rules, bouncing wall, no
+ * target).  col (p_col), when given, rides along in the cell sort's permutation (:337). */
+static int boids_physics(orc_boids* s, orc_f4* col)
 {
   const orc_consts* c = &s->c;
   const uint32_t n = s->n;
@@ -732,6 +739,8 @@ int orc_boids_update(orc_boids* s)
   if (rc)
     return rc;
   orc_permute_f4(s->idx, s->pos, s->tmp, s->maxN);
+  if (col)
+    orc_permute_f4(s->idx, col, s->tmp, s->maxN);
   orc_permute_f4(s->idx, s->vel, s->tmp, s->maxN);
   orc_permute_f4(s->idx, s->acc, s->tmp, s->maxN);
   orc_reset_start_end(s->se, c);                                       /* :339-344 */
@@ -742,4 +751,117 @@ int orc_boids_update(orc_boids* s)
   orc_bd_update_vel(s->acc, s->timeStep, s->p.velocityScale, s->vel, n); /* :360-361 */
   orc_bd_update_pos_wall(s->vel, s->timeStep, s->pos, n, c);          /* :369-371 */
   return 0;
+}
+
+/* physics/ocl/Boids.cpp:323-384 (3-D rules, bouncing wall, no target, no rendering). */
+int orc_boids_update(orc_boids* s) { return boids_physics(s, NULL); }
+
+/* ------------------------------------------------------------------------------------
+ * Rendering side of a frame
+ * ---------------------------------------------------------------------------------- */
+
+#define ORC_FAR_DIST 1000000.0f /* define.cl:10 */
+
+/* utils.cl:35-38 */
+void orc_reset_camera_dist(uint32_t* dist, uint32_t maxN)
+{
+  for (uint32_t i = 0; i < maxN; ++i)
+    dist[i] = (uint32_t)ORC_FAR_DIST;
+}
+
+/* utils.cl:43-52 (cameraPos is a float3 buffer: its first element's xyz) */
+void orc_fill_camera_dist(const orc_f4* pos, const float cam[3], uint32_t* dist, uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const float dx = pos[i].x - cam[0], dy = pos[i].y - cam[1], dz = pos[i].z - cam[2];
+    const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+    dist[i] = (uint32_t)fmaxf(ORC_FAR_DIST - len * 100.0f, 0.0f);
+  }
+}
+
+/* fluids.cl:458-478 */
+void orc_fld_fill_color(const float* density, const orc_fluid_params* p, orc_f4* col, uint32_t n)
+{
+  const orc_f4 blue = { 0.0f, 0.1f, 1.0f, 0.5f };
+  const orc_f4 lightBlue = { 0.7f, 0.7f, 1.0f, 0.5f };
+  const orc_f4 darkBlue = { 0.0f, 0.0f, 0.8f, 0.5f };
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const float constraint = 1.0f - density[i] / p->restDensity;
+    orc_f4 color = blue;
+    if (constraint > 0.0f)
+      color = add4(color, div4s(muls4(constraint, sub4(lightBlue, blue)), 0.35f));
+    else if (constraint < 0.0f)
+      color = add4(color, div4s(muls4(constraint, sub4(blue, darkBlue)), 0.35f));
+    col[i] = color;
+  }
+}
+
+/* boids.cl:38-41 */
+void orc_bd_fill_color(orc_f4* col, uint32_t count)
+{
+  const orc_f4 v = { 1.0f, 0.02f, 0.02f, 0.5f };
+  for (uint32_t i = 0; i < count; ++i)
+    col[i] = v;
+}
+
+/* grid.cl:43-46 */
+void orc_reset_detector(float* det, const orc_consts* c)
+{
+  memset(det, 0, sizeof(float) * 8 * (size_t)c->numCells);
+}
+
+/* grid.cl:51-60: the scalar 1.0f widens to all eight components of the float8 */
+void orc_fill_detector(const orc_f4* pos, float* det, uint32_t n, const orc_consts* c)
+{
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t cell = cell1d(pos[i], c);
+    if (cell < c->numCells)
+      for (int k = 0; k < 8; ++k)
+        det[(size_t)cell * 8 + k] = 1.0f;
+  }
+}
+
+/* Fluids.cpp:466-468 / Boids.cpp:379-381: the camera distances of the active particles and
+ * the second RadixSort::sort, over maxN keys, carrying four float4 payloads. */
+static int camera_sort(const orc_render* r, uint32_t* idx, uint32_t n, uint32_t maxN,
+                       orc_f4* tmp, orc_f4* a, orc_f4* b, orc_f4* c, orc_f4* d)
+{
+  orc_fill_camera_dist(a, r->cameraPos, r->cameraDist, n);
+  const int rc = orc_radix_sort(r->cameraDist, idx, maxN);
+  if (rc)
+    return rc;
+  orc_permute_f4(idx, a, tmp, maxN);
+  orc_permute_f4(idx, b, tmp, maxN);
+  orc_permute_f4(idx, c, tmp, maxN);
+  orc_permute_f4(idx, d, tmp, maxN);
+  return 0;
+}
+
+/* physics/ocl/Fluids.cpp:400-471 */
+int orc_fluids_frame(orc_fluids* s, orc_render* r, int paused)
+{
+  if (!paused) {
+    const int rc = fluids_physics(s, r->col);
+    if (rc)
+      return rc;
+    orc_reset_detector(r->detector, &s->c);                          /* :460 */
+    orc_fill_detector(s->pos, r->detector, s->n, &s->c);             /* :461 */
+    orc_fld_fill_color(s->density, &s->p, r->col, s->n);             /* :462 */
+  }
+  return camera_sort(r, s->idx, s->n, s->maxN, s->tmp, s->pos, r->col, s->vel, s->pred);
+}
+
+/* physics/ocl/Boids.cpp:323-384 */
+int orc_boids_frame(orc_boids* s, orc_render* r, int paused)
+{
+  if (!paused) {
+    const int rc = boids_physics(s, r->col);
+    if (rc)
+      return rc;
+    orc_reset_detector(r->detector, &s->c);                          /* :375 */
+    orc_fill_detector(s->pos, r->detector, s->n, &s->c);             /* :376 */
+  }
+  return camera_sort(r, s->idx, s->n, s->maxN, s->tmp, s->pos, r->col, s->vel, s->acc);
 }

@@ -159,6 +159,39 @@ int orc_fluids_update(orc_fluids* s);
 /* Boids::update (Boids.cpp:323-384) with the bouncing wall, without GL or target. */
 int orc_boids_update(orc_boids* s);
 
+/* ---- rendering side of a frame (utils.cl:35-77, grid.cl:43-60, fluids.cl:458-478,
+ *      boids.cl:38-41) ---- */
+/* The buffers a frame's rendering steps touch: u_cameraPos, p_col (maxN), p_cameraDist
+ * (maxN) and c_partDetector (numCells x float8). */
+typedef struct {
+  float cameraPos[3];
+  orc_f4* col;
+  uint32_t* cameraDist;
+  float* detector;
+} orc_render;
+
+/* utils.cl:35-38 over maxN entries. */
+void orc_reset_camera_dist(uint32_t* dist, uint32_t maxN);
+/* utils.cl:43-52: dist = (uint)max(FAR_DIST - length(pos.xyz - cam) * 100, 0) with
+ * length(v) = sqrtf((v.x*v.x + v.y*v.y) + v.z*v.z) and max = fmaxf. */
+void orc_fill_camera_dist(const orc_f4* pos, const float cam[3], uint32_t* dist, uint32_t n);
+/* fluids.cl:458-478 fld_fillFluidColor. */
+void orc_fld_fill_color(const float* density, const orc_fluid_params* p, orc_f4* col, uint32_t n);
+/* boids.cl:38-41 bd_fillBoidsColor. */
+void orc_bd_fill_color(orc_f4* col, uint32_t count);
+/* grid.cl:43-46 and :51-60 (every component of a hit cell's float8 is 1). */
+void orc_reset_detector(float* det, const orc_consts* c);
+void orc_fill_detector(const orc_f4* pos, float* det, uint32_t n, const orc_consts* c);
+
+/* One whole Fluids::update (Fluids.cpp:400-471): unless paused, the physics with p_col
+ * permuted by the cell sort, then the grid detector and the colour; always the camera
+ * distances and their sort over maxN, permuting p_pos, p_col, p_vel and p_predPos. */
+int orc_fluids_frame(orc_fluids* s, orc_render* r, int paused);
+/* One whole Boids::update (Boids.cpp:323-384) with the bouncing wall: unless paused, the
+ * step with p_col permuted by the cell sort and the grid detector; always the camera
+ * sort, permuting p_pos, p_col, p_vel and p_acc. */
+int orc_boids_frame(orc_boids* s, orc_render* r, int paused);
+
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

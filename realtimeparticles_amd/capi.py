@@ -21,11 +21,45 @@ _vp = ctypes.c_void_p
 _int = ctypes.c_int
 
 FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
+FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
 
 
 class RtpError(RuntimeError):
     pass
+
+
+class _RenderFields:
+    """p_col, p_cameraDist and c_partDetector of a model with the rendering steps on
+    (Fluids.cpp:458-468, Boids.cpp:375-381)."""
+
+    _prefix = ""
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), f"rtp_{self._prefix}_{name}")(self._h, *args), name)
+
+    def set_render(self, flag: bool) -> None:
+        self._call("set_render", int(bool(flag)))
+
+    def set_camera(self, xyz) -> None:
+        self._call("set_camera", (_f32 * 3)(*[float(v) for v in xyz]))
+
+    def read_color(self, count: int | None = None) -> np.ndarray:
+        count = self.max_n if count is None else int(count)
+        out = np.zeros((count, 4), np.float32)
+        self._call("read_field", FIELD_COLOR, out.ctypes.data, count)
+        return out
+
+    def read_camera_dist(self, count: int | None = None) -> np.ndarray:
+        count = self.max_n if count is None else int(count)
+        out = np.zeros(count, np.uint32)
+        self._call("read_field", FIELD_CAMERA_DIST, out.ctypes.data, count)
+        return out
+
+    def read_detector(self) -> np.ndarray:
+        out = np.zeros((self.num_cells, 8), np.float32)
+        self._call("read_field", FIELD_DETECTOR, out.ctypes.data, self.num_cells)
+        return out
 
 
 class ModelDesc(ctypes.Structure):
@@ -122,6 +156,8 @@ _SIGS = {
     "rtp_fluids_sync": ([_vp], _int),
     "rtp_fluids_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_fluids_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_fluids_set_render": ([_vp, _int], _int),
+    "rtp_fluids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
     "rtp_fluids_density_error": ([_vp, _vp], _int),
     "rtp_fluids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_fluids_stream": ([_vp, ctypes.POINTER(_vp)], _int),
@@ -157,6 +193,8 @@ _SIGS = {
     "rtp_boids_sync": ([_vp], _int),
     "rtp_boids_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_boids_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_boids_set_render": ([_vp, _int], _int),
+    "rtp_boids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
     "rtp_boids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_boids_enable_profiling": ([_vp, _int], _int),
     "rtp_boids_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
@@ -242,8 +280,10 @@ def _desc(max_n, n, box, res, gravity, max_parts_in_cell, device) -> ModelDesc:
     return d
 
 
-class Fluids:
+class Fluids(_RenderFields):
     """Physics::HIP::Fluids over the C ABI (position based fluids)."""
+
+    _prefix = "fluids"
 
     def __init__(self, max_n: int, n: int, box, res, gravity: float = 9.81,
                  max_parts_in_cell: int = 0, device: int = 0):
@@ -591,8 +631,10 @@ class Mlp:
         return hist
 
 
-class Boids:
+class Boids(_RenderFields):
     """Physics::HIP::Boids over the C ABI (3-D rules, bouncing wall)."""
+
+    _prefix = "boids"
 
     def __init__(self, max_n: int, n: int, box, res, max_parts_in_cell: int = 0,
                  device: int = 0):

@@ -49,6 +49,7 @@ struct GridState {
   int idxCur = 0;   // idx[idxCur] holds the last permutation over [0, sortedN)
   uint32_t sortedN = 0;
   uint32_t dirtyEnd = 0;  // keys in [n, dirtyEnd) are not the reset tail 2*NC+i
+  uint32_t permN = 0;     // range of the permutation in idx[idxCur] (cell or camera sort)
   int sortBits = 0;
 
   int alloc(uint32_t maxN_)
@@ -80,6 +81,7 @@ struct GridState {
     idxCur = 0;
     sortedN = 0;
     dirtyEnd = 0;
+    permN = 0;
   }
   uint32_t sort_range(uint32_t n) const { return std::max(n, dirtyEnd); }
   uint32_t max_key(uint32_t n) const
@@ -107,6 +109,7 @@ struct GridState {
       keyCur = 1 - keyCur;
     idxCur = inB ? 1 : 0;
     sortedN = sn;
+    permN = sn;
     return RTP_OK;
   }
   // Copy p_cellID, the permutation (identity past the sorted range) and the cell table.
@@ -114,14 +117,92 @@ struct GridState {
   {
     if (keys)
       RTP_HIP_TRY(hipMemcpyAsync(keys, key[keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToHost, s));
-    if (perm && sortedN)
-      RTP_HIP_TRY(hipMemcpyAsync(perm, idx[idxCur], sizeof(uint32_t) * sortedN, hipMemcpyDeviceToHost, s));
+    if (perm && permN)
+      RTP_HIP_TRY(hipMemcpyAsync(perm, idx[idxCur], sizeof(uint32_t) * permN, hipMemcpyDeviceToHost, s));
     if (se)
       RTP_HIP_TRY(hipMemcpyAsync(se, startEnd, sizeof(uint2) * g.numCells, hipMemcpyDeviceToHost, s));
     RTP_HIP_TRY(hipStreamSynchronize(s));
     if (perm)
-      for (uint32_t i = sortedN; i < maxN; ++i)
+      for (uint32_t i = permN; i < maxN; ++i)
         perm[i] = i;
+    return RTP_OK;
+  }
+};
+
+// The buffers of a frame's rendering steps (Fluids.cpp:132-164, Boids.cpp:128-153):
+// u_cameraPos, p_col, p_cameraDist and c_partDetector, with gather / sort partners.
+struct RenderState {
+  bool on = false;
+  float4* cam = nullptr;                    // u_cameraPos (xyz)
+  float4 camHost = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4* col[2] = { nullptr, nullptr };    // col[colCur] is p_col
+  uint32_t* key[2] = { nullptr, nullptr };  // key[keyCur] is p_cameraDist
+  float4* det = nullptr;                    // two float4 per cell
+  int colCur = 0, keyCur = 0;
+  // The largest particle count keyed since the reset: keys at or past it still hold
+  // FAR_DIST, the largest key, so the stable sort leaves them where they are.
+  uint32_t high = 0;
+
+  int alloc(uint32_t maxN, uint32_t numCells, bool pairCol)
+  {
+    if (cam)
+      return RTP_OK;
+    int rc = 0;
+    if ((rc = dev_alloc(&cam, 1)) || (rc = dev_alloc(&col[0], maxN)) ||
+        (pairCol && (rc = dev_alloc(&col[1], maxN))) || (rc = dev_alloc(&key[0], maxN)) ||
+        (rc = dev_alloc(&key[1], maxN)) || (rc = dev_alloc(&det, 2u * numCells)))
+      return rc;
+    return RTP_OK;
+  }
+  void release()
+  {
+    dev_free(cam);
+    dev_free(col[0]);
+    dev_free(col[1]);
+    dev_free(key[0]);
+    dev_free(key[1]);
+    dev_free(det);
+  }
+  int set_camera(const float p[3], hipStream_t s)
+  {
+    camHost = make_float4(p[0], p[1], p[2], 0.0f);
+    RTP_HIP_TRY(hipMemcpyAsync(cam, &camHost, sizeof(float4), hipMemcpyHostToDevice, s));
+    return RTP_OK;
+  }
+  // The colour at model reset (Fluids.cpp:394 / bd_fillBoidsColor, Boids.cpp:267) and
+  // resetCameraDist (Fluids.cpp:215, Boids.cpp:272) over maxN, in both partners.
+  void reset(float4 colInit, uint32_t maxN, hipStream_t s)
+  {
+    for (int k = 0; k < 2; ++k) {
+      if (col[k])
+        launch_fill_f4v(col[k], colInit, 0, maxN, s);
+      launch_fill_u32(key[k], maxN, 1000000u, s);
+    }
+    colCur = keyCur = 0;
+    high = 0;
+  }
+  // Fluids.cpp:466-468 / Boids.cpp:379-381: camera distances of the n active particles
+  // (from pos), then the second RadixSort::sort over [0, R) carrying the payloads of g
+  // (sources set, destinations filled in here from dst).  R covers every key that is not
+  // FAR_DIST and the grid's sorted range, past which every ping-pong pair agrees.
+  int camera_sort(GridState& G, uint32_t n, const float4* pos, GatherF4 g, hipStream_t s,
+                  Profiler& prof)
+  {
+    high = std::max(high, n);
+    const uint32_t R = std::max(high, G.sortedN);
+    RTP_PROF(prof, "camera_key", s, launch_camera_key(pos, cam, key[keyCur], n, s));
+    const SortPlan plan = make_sort_plan(R, 1000000u, G.sortBits);
+    bool inB = false;
+    int rc = 0;
+    RTP_PROF(prof, "camera_sort", s,
+             rc = sort_pairs(plan, key[keyCur], key[1 - keyCur], G.idx[0], G.idx[1], G.counts, &inB, s));
+    if (rc)
+      return rc;
+    if (inB)
+      keyCur = 1 - keyCur;
+    G.idxCur = inB ? 1 : 0;
+    G.permN = R;
+    RTP_PROF(prof, "camera_gather", s, launch_gather_f4(G.idx[G.idxCur], 0, R, g, s));
     return RTP_OK;
   }
 };
@@ -207,7 +288,8 @@ struct rtp_fluids {
   Profiler prof;
   bool useGraph = true;
   bool graphValid = false;
-  int graphCur = -1, graphKeyCur = -1, graphPred = -1;
+  int graphCur = -1, graphKeyCur = -1, graphPred = -1, graphCol = -1, graphCam = -1;
+  RenderState render;  // rtp_fluids_set_render
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   // learned warm-start, teacher recording and the post-correction residual pass
@@ -234,8 +316,10 @@ struct rtp_fluids {
   float4* ckVel = nullptr;
   float4* ckPred = nullptr;
   uint32_t* ckKey = nullptr;
-  bool ckValid = false;
-  uint32_t ckN = 0, ckDirty = 0, ckSorted = 0;
+  float4* ckCol = nullptr;
+  uint32_t* ckCam = nullptr;
+  bool ckValid = false, ckRender = false;
+  uint32_t ckN = 0, ckDirty = 0, ckSorted = 0, ckHigh = 0;
 
   void drop_graph()
   {
@@ -395,7 +479,52 @@ struct rtp_fluids {
                launch_pbf_xsph(pred[in], G.startEnd, velTmp, vel[cur], pos[cur], n, c, L, s, po,
                                lists ? kNbReplay : 0));
     }
+    return render.on ? render_frame(true) : RTP_OK;
+  }
+
+  // Fluids.cpp:458-468: after the physics, the grid detector (from p_pos) and the colour
+  // (from p_density); then, paused or not, the camera-distance sort permuting p_pos, p_col,
+  // p_vel and p_predPos.  p_col rides the cell sort like the other payloads (:417), but
+  // the colour fill overwrites its active range, so only its tail [n, sortedN) is gathered.
+  int render_frame(bool physics)
+  {
+    hipStream_t s = stream;
+    GridState& G = grid;
+    RenderState& R = render;
+    if (physics) {
+      GatherF4 t;
+      t.src[0] = R.col[R.colCur];
+      t.dst[0] = R.col[1 - R.colCur];
+      RTP_PROF(prof, "render_col_tail", s, launch_gather_f4(G.idx[G.idxCur], n, G.sortedN, t, s));
+      RTP_PROF(prof, "render_detector", s, launch_detector(pos[cur], R.det, n, c.g, s));
+      RTP_PROF(prof, "render_color", s,
+               launch_fluid_color(density, c.p.restDensity, R.col[1 - R.colCur], n, s));
+      R.colCur = 1 - R.colCur;
+    }
+    GatherF4 g;
+    g.src[0] = pos[cur];
+    g.dst[0] = pos[1 - cur];
+    g.src[1] = vel[cur];
+    g.dst[1] = vel[1 - cur];
+    g.src[2] = pred[predFinal];
+    g.dst[2] = pred[1 - predFinal];
+    g.src[3] = R.col[R.colCur];
+    g.dst[3] = R.col[1 - R.colCur];
+    const int rc = R.camera_sort(G, n, pos[cur], g, s, prof);
+    if (rc)
+      return rc;
+    cur = 1 - cur;
+    predFinal = 1 - predFinal;
+    R.colCur = 1 - R.colCur;
     return RTP_OK;
+  }
+
+  // The colour and camera distances as at model reset, and the detector of the current
+  // positions (Fluids.cpp:209-215).
+  void render_reset()
+  {
+    render.reset(make_float4(0.0f, 0.1f, 1.0f, 0.0f), grid.maxN, stream);
+    launch_detector(pos[cur], render.det, n, c.g, stream);
   }
 
   int capture_pair()
@@ -404,6 +533,8 @@ struct rtp_fluids {
     graphCur = cur;
     graphKeyCur = grid.keyCur;
     graphPred = predFinal;
+    graphCol = render.colCur;
+    graphCam = render.keyCur;
     RTP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = substep();
     if (!rc)
@@ -423,13 +554,29 @@ struct rtp_fluids {
 
   int update(uint32_t nsteps)
   {
-    if (paused || nsteps == 0)
+    if (nsteps == 0)
       return RTP_OK;
+    if (paused) {
+      // A paused update still sorts by camera distance (Fluids.cpp:466-468).  The second
+      // of several paused sorts finds its keys in order and leaves the identity, as every
+      // later one does.
+      if (!render.on)
+        return RTP_OK;
+      RTP_HIP_TRY(hipSetDevice(device));
+      for (uint32_t k = 0; k < std::min(nsteps, 2u); ++k) {
+        const int rc = render_frame(false);
+        if (rc)
+          return rc;
+      }
+      RTP_HIP_TRY(hipGetLastError());
+      return RTP_OK;
+    }
     MarkRange mark("rtp_fluids_update");
     RTP_HIP_TRY(hipSetDevice(device));
     uint32_t done = 0;
     if (useGraph && !prof.enabled && nsteps >= 2 && n > 0) {
-      if (!graphValid || graphCur != cur || graphKeyCur != grid.keyCur || graphPred != predFinal) {
+      if (!graphValid || graphCur != cur || graphKeyCur != grid.keyCur || graphPred != predFinal ||
+          graphCol != render.colCur || graphCam != render.keyCur) {
         const int rc = capture_pair();
         if (rc)
           return rc;
@@ -470,6 +617,9 @@ struct rtp_fluids {
     dev_free(ckVel);
     dev_free(ckPred);
     dev_free(ckKey);
+    dev_free(ckCol);
+    dev_free(ckCam);
+    render.release();
     dev_free(feat);
     dev_free(lamAcc);
     dev_free(resDensity);
@@ -498,10 +648,12 @@ struct rtp_boids {
   bool paused = false;
   float4* pos[2] = { nullptr, nullptr };
   float4* vel[2] = { nullptr, nullptr };
-  float4* acc = nullptr;
+  float4* acc = nullptr;     // p_acc
+  float4* accAlt = nullptr;  // its gather partner, with the rendering steps on
   int cur = 0;
   Launch L;
   Profiler prof;
+  RenderState render;  // rtp_boids_set_render
 
   int step()
   {
@@ -525,10 +677,23 @@ struct rtp_boids {
     a.startEnd = G.startEnd;
     a.g = c.g;
     RTP_PROF(prof, "reorder_cells", s, launch_reorder(a, s));
+    // With the rendering steps on, p_acc rides the cell sort too (Boids.cpp:337): the step
+    // overwrites its active range, so only the tail [n, sortedN) is gathered, and the step
+    // writes the partner.
+    float4* accOut = acc;
+    if (render.on) {
+      GatherF4 t;
+      t.src[0] = acc;
+      t.dst[0] = accAlt;
+      RTP_PROF(prof, "render_acc_tail", s, launch_gather_f4(G.idx[G.idxCur], n, G.sortedN, t, s));
+      accOut = accAlt;
+    }
     // Sorted state now lives in [1 - cur]; the fused step writes the next state to [cur].
     RTP_PROF(prof, "boids_step", s,
-             launch_boids_step(pos[1 - cur], vel[1 - cur], G.startEnd, pos[cur], vel[cur], acc, n,
-                               c, L, s));
+             launch_boids_step(pos[1 - cur], vel[1 - cur], G.startEnd, pos[cur], vel[cur], accOut,
+                               n, c, L, s));
+    if (render.on)
+      std::swap(acc, accAlt);
     // The tail [n, sortedN) was only permuted; carry it over unchanged.
     if (G.sortedN > n) {
       RTP_HIP_TRY(hipMemcpyAsync(pos[cur] + n, pos[1 - cur] + n, sizeof(float4) * (G.sortedN - n),
@@ -536,7 +701,39 @@ struct rtp_boids {
       RTP_HIP_TRY(hipMemcpyAsync(vel[cur] + n, vel[1 - cur] + n, sizeof(float4) * (G.sortedN - n),
                                  hipMemcpyDeviceToDevice, s));
     }
+    return render.on ? render_frame(true) : RTP_OK;
+  }
+
+  // Boids.cpp:375-381: after the step, the grid detector (from p_pos); then, paused or
+  // not, the camera-distance sort permuting p_pos, p_vel and p_acc.  p_col holds one colour
+  // everywhere (bd_fillBoidsColor), which every permutation keeps.
+  int render_frame(bool physics)
+  {
+    hipStream_t s = stream;
+    if (physics)
+      RTP_PROF(prof, "render_detector", s, launch_detector(pos[cur], render.det, n, c.g, s));
+    GatherF4 g;
+    g.src[0] = pos[cur];
+    g.dst[0] = pos[1 - cur];
+    g.src[1] = vel[cur];
+    g.dst[1] = vel[1 - cur];
+    g.src[2] = acc;
+    g.dst[2] = accAlt;
+    const int rc = render.camera_sort(grid, n, pos[cur], g, s, prof);
+    if (rc)
+      return rc;
+    cur = 1 - cur;
+    std::swap(acc, accAlt);
     return RTP_OK;
+  }
+
+  // bd_fillBoidsColor over maxN, the camera distances reset and the detector of the
+  // current positions (Boids.cpp:265-272); p_acc's partner made equal to it.
+  void render_reset()
+  {
+    render.reset(make_float4(1.0f, 0.02f, 0.02f, 0.5f), grid.maxN, stream);
+    launch_detector(pos[cur], render.det, n, c.g, stream);
+    (void)hipMemcpyAsync(accAlt, acc, sizeof(float4) * grid.maxN, hipMemcpyDeviceToDevice, stream);
   }
 
   void release()
@@ -550,11 +747,37 @@ struct rtp_boids {
       dev_free(vel[k]);
     }
     dev_free(acc);
+    dev_free(accAlt);
+    render.release();
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
 };
+
+// p_col (float4), p_cameraDist (uint32) or c_partDetector (float8 per cell, count cells).
+int read_render_field(RenderState& R, int field, uint32_t numCells, float* out, uint32_t count,
+                      hipStream_t s)
+{
+  if (!R.cam)
+    return fail(RTP_ERR_STATE, "the rendering steps were never enabled");
+  const void* src = nullptr;
+  size_t elem = 0;
+  if (field == RTP_FIELD_COLOR) {
+    src = R.col[R.colCur];
+    elem = sizeof(float4);
+  } else if (field == RTP_FIELD_CAMERA_DIST) {
+    src = R.key[R.keyCur];
+    elem = sizeof(uint32_t);
+  } else {
+    RTP_REQUIRE(count <= numCells, "count exceeds the number of cells");
+    src = R.det;
+    elem = 2 * sizeof(float4);
+  }
+  RTP_HIP_TRY(hipMemcpyAsync(out, src, elem * count, hipMemcpyDeviceToHost, s));
+  RTP_HIP_TRY(hipStreamSynchronize(s));
+  return RTP_OK;
+}
 
 // ================================================================================= C ABI
 
@@ -704,6 +927,8 @@ int rtp_fluids_load_state(rtp_fluids* h, const float* pos4, const float* vel4, u
   h->cur = 0;
   h->predFinal = 1;
   h->grid.reset_keys(h->stream);
+  if (h->render.on)
+    h->render_reset();
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   h->drop_graph();
   return RTP_OK;
@@ -752,7 +977,7 @@ int rtp_fluids_read_grid(rtp_fluids* h, uint32_t* cellIDs, uint32_t* sortIdx, ui
 int rtp_fluids_read_field(rtp_fluids* h, int field, float* out, uint32_t count)
 {
   RTP_REQUIRE(h && out, "NULL argument");
-  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_REQUIRE(count <= h->grid.maxN || field == RTP_FIELD_DETECTOR, "count exceeds maxNbParticles");
   RTP_HIP_TRY(hipSetDevice(h->device));
   const void* src = nullptr;
   size_t elem = sizeof(float);
@@ -761,6 +986,10 @@ int rtp_fluids_read_field(rtp_fluids* h, int field, float* out, uint32_t count)
   case RTP_FIELD_LAMBDA: src = h->lambda; break;
   case RTP_FIELD_PRED_POS: src = h->pred[h->predFinal]; elem = sizeof(float4); break;
   case RTP_FIELD_VORTICITY: src = h->vort; elem = sizeof(float4); break;
+  case RTP_FIELD_COLOR:
+  case RTP_FIELD_CAMERA_DIST:
+  case RTP_FIELD_DETECTOR:
+    return read_render_field(h->render, field, h->grid.g.numCells, out, count, h->stream);
   default: return fail(RTP_ERR_INVALID, "unknown field for fluids");
   }
   RTP_HIP_TRY(hipMemcpyAsync(out, src, elem * count, hipMemcpyDeviceToHost, h->stream));
@@ -832,6 +1061,14 @@ int rtp_fluids_save_checkpoint(rtp_fluids* h)
   RTP_HIP_TRY(hipMemcpyAsync(h->ckVel, h->vel[h->cur], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
   RTP_HIP_TRY(hipMemcpyAsync(h->ckPred, h->pred[h->predFinal], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
   RTP_HIP_TRY(hipMemcpyAsync(h->ckKey, h->grid.key[h->grid.keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
+  h->ckRender = h->render.on;
+  if (h->ckRender) {
+    if (!h->ckCol && ((rc = dev_alloc(&h->ckCol, maxN)) || (rc = dev_alloc(&h->ckCam, maxN))))
+      return rc;
+    RTP_HIP_TRY(hipMemcpyAsync(h->ckCol, h->render.col[h->render.colCur], sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    RTP_HIP_TRY(hipMemcpyAsync(h->ckCam, h->render.key[h->render.keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    h->ckHigh = h->render.high;
+  }
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   h->ckN = h->n;
   h->ckDirty = h->grid.dirtyEnd;
@@ -864,6 +1101,15 @@ int rtp_fluids_restore_checkpoint(rtp_fluids* h)
     RTP_HIP_TRY(hipMemcpyAsync(pred, h->ckPred, sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
     RTP_HIP_TRY(hipMemcpyAsync(key, h->ckKey, sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
   }
+  // p_col and p_cameraDist into both partners: their gather and sort ranges need not be
+  // those of the save
+  if (h->ckRender && h->render.on) {
+    for (int k = 0; k < 2; ++k) {
+      RTP_HIP_TRY(hipMemcpyAsync(h->render.col[k], h->ckCol, sizeof(float4) * maxN, hipMemcpyDeviceToDevice, h->stream));
+      RTP_HIP_TRY(hipMemcpyAsync(h->render.key[k], h->ckCam, sizeof(uint32_t) * maxN, hipMemcpyDeviceToDevice, h->stream));
+    }
+    h->render.high = h->ckHigh;
+  }
   h->n = h->ckN;
   h->grid.dirtyEnd = h->ckDirty;
   h->grid.sortedN = h->ckSorted;
@@ -877,6 +1123,47 @@ int rtp_fluids_use_graph(rtp_fluids* h, int enable)
   RTP_REQUIRE(h, "NULL handle");
   h->useGraph = enable != 0;
   h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_set_render(rtp_fluids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  h->drop_graph();
+  if (!enable) {
+    h->render.on = false;
+    return RTP_OK;
+  }
+  if (h->render.on)
+    return RTP_OK;
+  if (!h->render.cam) {
+    const int rc = h->render.alloc(h->grid.maxN, h->grid.g.numCells, true);
+    if (rc) {
+      h->render.release();
+      h->render = RenderState();
+      return rc;
+    }
+    const float origin[3] = { 0.0f, 0.0f, 0.0f };
+    const int rcc = h->render.set_camera(origin, h->stream);
+    if (rcc)
+      return rcc;
+  }
+  h->render.on = true;
+  h->render_reset();
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3])
+{
+  RTP_REQUIRE(h && pos, "NULL argument");
+  RTP_REQUIRE(h->render.cam, "enable the rendering steps before setting the camera");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const int rc = h->render.set_camera(pos, h->stream);
+  if (rc)
+    return rc;
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   return RTP_OK;
 }
 
@@ -1054,6 +1341,8 @@ int rtp_boids_load_state(rtp_boids* h, const float* pos4, const float* vel4, uin
   RTP_HIP_TRY(hipMemsetAsync(h->acc, 0, sizeof(float4) * h->grid.maxN, h->stream));
   h->cur = 0;
   h->grid.reset_keys(h->stream);
+  if (h->render.on)
+    h->render_reset();
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   return RTP_OK;
 }
@@ -1074,8 +1363,20 @@ int rtp_boids_read_state(rtp_boids* h, float* pos4, float* vel4, uint32_t count)
 int rtp_boids_update(rtp_boids* h, uint32_t nsteps)
 {
   RTP_REQUIRE(h, "NULL handle");
-  if (h->paused)
+  if (h->paused) {
+    // a paused update still sorts by camera distance (Boids.cpp:379-381); from the second
+    // paused sort on the keys are in order and the permutation is the identity
+    if (!h->render.on)
+      return RTP_OK;
+    RTP_HIP_TRY(hipSetDevice(h->device));
+    for (uint32_t k = 0; k < std::min(nsteps, 2u); ++k) {
+      const int rc = h->render_frame(false);
+      if (rc)
+        return rc;
+    }
+    RTP_HIP_TRY(hipGetLastError());
     return RTP_OK;
+  }
   RTP_HIP_TRY(hipSetDevice(h->device));
   for (uint32_t k = 0; k < nsteps; ++k) {
     const int rc = h->step();
@@ -1110,10 +1411,53 @@ int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint
 int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
 {
   RTP_REQUIRE(h && out, "NULL argument");
-  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
-  RTP_REQUIRE(field == RTP_FIELD_ACC, "unknown field for boids");
+  RTP_REQUIRE(count <= h->grid.maxN || field == RTP_FIELD_DETECTOR, "count exceeds maxNbParticles");
   RTP_HIP_TRY(hipSetDevice(h->device));
+  if (field == RTP_FIELD_COLOR || field == RTP_FIELD_CAMERA_DIST || field == RTP_FIELD_DETECTOR)
+    return read_render_field(h->render, field, h->grid.g.numCells, out, count, h->stream);
+  RTP_REQUIRE(field == RTP_FIELD_ACC, "unknown field for boids");
   RTP_HIP_TRY(hipMemcpyAsync(out, h->acc, sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_set_render(rtp_boids* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (!enable) {
+    h->render.on = false;
+    return RTP_OK;
+  }
+  if (h->render.on)
+    return RTP_OK;
+  if (!h->render.cam) {
+    int rc = h->render.alloc(h->grid.maxN, h->grid.g.numCells, false);
+    if (!rc && !h->accAlt)
+      rc = dev_alloc(&h->accAlt, h->grid.maxN);
+    if (rc) {
+      h->render.release();
+      h->render = RenderState();
+      return rc;
+    }
+    const float origin[3] = { 0.0f, 0.0f, 0.0f };
+    if ((rc = h->render.set_camera(origin, h->stream)))
+      return rc;
+  }
+  h->render.on = true;
+  h->render_reset();
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_boids_set_camera(rtp_boids* h, const float pos[3])
+{
+  RTP_REQUIRE(h && pos, "NULL argument");
+  RTP_REQUIRE(h->render.cam, "enable the rendering steps before setting the camera");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const int rc = h->render.set_camera(pos, h->stream);
+  if (rc)
+    return rc;
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   return RTP_OK;
 }

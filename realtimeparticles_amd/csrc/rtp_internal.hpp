@@ -125,6 +125,25 @@ struct ReorderArgs {
 void launch_reorder(const ReorderArgs& a, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t begin, uint32_t end, hipStream_t s);
 
+// ------------------------------------------------------------------ rendering side of a frame
+// fillCameraDist (utils.cl:43-52) over [0, n); cam points at u_cameraPos (xyz).
+void launch_camera_key(const float4* pos, const float4* cam, uint32_t* key, uint32_t n,
+                       hipStream_t s);
+// fld_fillFluidColor (fluids.cl:458-478) over [0, n).
+void launch_fluid_color(const float* density, float restDensity, float4* col, uint32_t n,
+                        hipStream_t s);
+void launch_fill_f4v(float4* dst, float4 v, uint32_t begin, uint32_t end, hipStream_t s);
+// resetGridDetector + fillGridDetector (grid.cl:43-60); det holds two float4 per cell.
+void launch_detector(const float4* pos, float4* det, uint32_t n, const GridConsts& g,
+                     hipStream_t s);
+// dst[k][i] = src[k][idx[i]] over [begin, end) for every k whose src is set.
+struct GatherF4 {
+  const float4* src[4] = { nullptr, nullptr, nullptr, nullptr };
+  float4* dst[4] = { nullptr, nullptr, nullptr, nullptr };
+};
+void launch_gather_f4(const uint32_t* idx, uint32_t begin, uint32_t end, const GatherF4& g,
+                      hipStream_t s);
+
 // ---------------------------------------------------------------------------------- PBF
 void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint32_t* key,
                         uint2* startEnd, uint32_t n, const PbfConsts& c, hipStream_t s);

@@ -349,3 +349,105 @@ def test_wall_bounce_halves_and_reverses(oracle):
     b.step(1)
     assert b.pos[0, 0] == 1.0
     assert b.vel[0, 0] == pytest.approx(-0.75, rel=1e-6)
+
+
+# ------------------------------------------------------------------ rendering side of a frame
+
+def test_camera_distance_special_cases(oracle):
+    L = oracle.lib()
+    pos = np.array([f4(3, 4, 0), f4(0, 0, 12), f4(1, 2, 2), f4(0, 0, 0),
+                    f4(2e4, 0, 0), f4(np.inf, np.inf, np.inf), f4(0, 0, 1e-5)], np.float32)
+    cam = (_f32x3 := ctypes.c_float * 3)(0.0, 0.0, 0.0)
+    out = np.zeros(len(pos), np.uint32)
+    L.orc_fill_camera_dist(pos.ctypes.data, cam, out.ctypes.data, len(pos))
+    # 1e6 - 100 * |p - cam|, truncated; clamped at 0 far away and for infinite positions;
+    # distances under half a float ulp of 1e6 / 100 leave the key at the reset value
+    assert list(out) == [999500, 998800, 999700, 1000000, 0, 0, 1000000]
+    cam = _f32x3(1.0, 2.0, 2.0)
+    L.orc_fill_camera_dist(pos.ctypes.data, cam, out.ctypes.data, 4)
+    assert out[2] == 1000000 and out[3] == 999700
+    reset = np.zeros(1024, np.uint32)
+    L.orc_reset_camera_dist(reset.ctypes.data, 1024)
+    assert (reset == 1000000).all()
+
+
+def test_fluid_colour_branches(oracle):
+    L = oracle.lib()
+    p = oracle.fluid_params(restDensity=400.0)
+    dens = np.array([400.0, 0.0, 540.0, 200.0, 600.0], np.float32)
+    col = np.zeros((5, 4), np.float32)
+    L.orc_fld_fill_color(dens.ctypes.data, ctypes.byref(p), col.ctypes.data, 5)
+    blue = np.array([0.0, 0.1, 1.0, 0.5], np.float32)
+    light = np.array([0.7, 0.7, 1.0, 0.5], np.float32)
+    dark = np.array([0.0, 0.0, 0.8, 0.5], np.float32)
+    assert np.array_equal(col[0], blue)                 # at rest density: blue exactly
+    # constraint 1 (empty): blue + (light - blue) / 0.35
+    assert np.allclose(col[1], blue + (light - blue) / 0.35, rtol=1e-6)
+    # constraint -0.35 (compressed by 35 %): dark blue
+    assert np.allclose(col[2], dark, atol=1e-6)
+    # constraint 0.5 towards light blue, -0.5 past dark blue; alpha stays 0.5
+    assert np.allclose(col[3], blue + 0.5 * (light - blue) / 0.35, rtol=1e-6)
+    assert np.allclose(col[4], blue - 0.5 * (blue - dark) / 0.35, rtol=1e-6)
+    assert (col[:, 3] == 0.5).all()
+    bc = np.zeros((3, 4), np.float32)
+    L.orc_bd_fill_color(bc.ctypes.data, 3)
+    assert (bc == np.array([1.0, 0.02, 0.02, 0.5], np.float32)).all()
+
+
+def test_detector_against_brute_force(oracle):
+    c = consts(oracle)
+    rng = np.random.default_rng(5)
+    n = 3000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(-2.2, 2.2, (n, 3)).astype(np.float32) * np.float32([1, 0.5, 0.5])
+    pos[:8, :3] = [[2, 1, 1], [-2, -1, -1], [2, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5],
+                   [-5, 0, 0], [0.05, 0.05, 0.05]]
+    det = np.full((int(c.numCells), 8), 7.0, np.float32)
+    oracle.lib().orc_reset_detector(det.ctypes.data, ctypes.byref(c))
+    assert not det.any()
+    oracle.lib().orc_fill_detector(pos.ctypes.data, det.ctypes.data, n, ctypes.byref(c))
+    w = np.array([2.0, 1.0, 1.0], np.float32)
+    cs = np.float32(c.cellSize)
+    ijk = np.floor((np.clip(pos[:, :3], -w, w) + w) / cs).astype(np.int64)
+    cell = (ijk[:, 0] * 20 + ijk[:, 1]) * 20 + ijk[:, 2]
+    want = np.zeros(int(c.numCells), bool)
+    want[cell[cell < c.numCells]] = True
+    assert np.array_equal(det.any(1), want)
+    assert set(np.unique(det)) <= {0.0, 1.0} and (det.min(1) == det.max(1)).all()
+
+
+def _camera_keys(pos, cam):
+    d = pos[:, :3] - np.asarray(cam, np.float32)
+    ln = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    return np.maximum(np.float32(1e6) - ln * np.float32(100.0), np.float32(0)).astype(np.uint32)
+
+
+def test_render_frame_is_physics_then_stable_camera_sort(oracle):
+    """A frame with rendering = the physics-only frame, then every buffer permuted by the
+    stable argsort of the camera keys (over maxN, with the reset key past n)."""
+    sc, f = lattice_state(oracle, jitter=0.2, iters=2)
+    _, g = lattice_state(oracle, jitter=0.2, iters=2)
+    cam = (7.0, 3.0, -5.0)
+    f.enable_render(cam)
+    f.step(1)
+    g.step(1)
+    keys = np.full(sc.max_n, 1000000, np.uint32)
+    keys[: sc.n] = _camera_keys(g.pos[: sc.n], cam)
+    perm = np.argsort(keys, kind="stable")
+    assert np.array_equal(f.idx, perm)
+    assert np.array_equal(f.render.dist, keys[perm])
+    for a, b in ((f.pos, g.pos), (f.vel, g.vel), (f.pred, g.pred)):
+        assert np.array_equal(a, b[perm])
+    col = np.tile(np.float32([0.0, 0.1, 1.0, 0.0]), (sc.max_n, 1))
+    oracle.lib().orc_fld_fill_color(g.density.ctypes.data, ctypes.byref(g.p), col.ctypes.data, sc.n)
+    assert np.array_equal(f.render.col, col[perm])
+    det = np.zeros_like(f.render.det)
+    oracle.lib().orc_fill_detector(g.pos.ctypes.data, det.ctypes.data, sc.n, ctypes.byref(g.c))
+    assert np.array_equal(f.render.det, det)
+    # the closest particle is drawn last; the inactive tail stays in place
+    assert np.array_equal(perm[sc.n:], np.arange(sc.n, sc.max_n))
+    # a paused frame sorts again: the same keys, already in order, give the identity
+    f.paused = True
+    pos1 = f.pos.copy()
+    f.step(1)
+    assert np.array_equal(f.idx, np.arange(sc.max_n)) and np.array_equal(f.pos, pos1)
