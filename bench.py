@@ -171,6 +171,52 @@ def run_gpu(args, sc, mapping="literal", profile=True):
     return out
 
 
+def run_render(args, sc, tuning, physics_ms):
+    """The same fixed-input substep with the rendering steps of the reference's update on
+    (grid detector, colour, camera-distance keys and the second radix sort, which permutes
+    p_pos, p_col, p_vel and p_predPos), the camera outside the box."""
+    from realtimeparticles_amd import capi
+    m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(**sc.params())
+    m.set_nb_jacobi_iters(sc.iters)
+    if tuning:
+        m.set_tuning(tuning["block"], tuning["flags"], tuning["sort_bits"], tuning["queue"])
+    m.set_render(True)
+    cam = (0.0, 0.5 * sc.box[1], 3.0 * sc.box[2])
+    m.set_camera(cam)
+    m.load_state(sc.pos, sc.vel)
+    m.save_checkpoint()
+
+    def fixed_steps(k):
+        for _ in range(k):
+            m.restore_checkpoint()
+            m.update(1)
+
+    fixed_steps(args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    fixed_steps(args.steps)
+    m.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    m.enable_profiling(True)
+    m.reset_profile()
+    k = max(2, min(args.steps, 40))
+    fixed_steps(k)
+    m.sync()
+    prof = m.profile()
+    m.close()
+    render_ms = {name: round(v[1] / k, 5) for name, v in sorted(prof.items())
+                 if name.startswith(("camera_", "render_"))}
+    out = {"camera": cam, "ms_per_step": round(ms, 5),
+           "ms_per_step_without": round(physics_ms, 5),
+           "kernel_ms_per_step": render_ms}
+    if "camera_gather" in prof:
+        # four float4 payloads read and written plus the permutation, per particle
+        g_ms = prof["camera_gather"][1] / prof["camera_gather"][0]
+        out["camera_gather_gbs"] = round(sc.n * (4 * 32 + 4) / (g_ms * 1e-3) / 1e9, 1)
+    return out
+
+
 def run_cpu_baseline(sc, budget_s: float, max_steps: int):
     from oracle import oracle as O
     O.build()
@@ -365,6 +411,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the substep with the reference's rendering steps on")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-seq-steps", type=int, default=3)
     ap.add_argument("--learn-seeds", default="101,102")
@@ -415,6 +463,14 @@ def main():
             stable["continuing_particle_updates_per_s"] = sc.n * args.steps / gs["continuing_elapsed"]
             stable["continuing_density_error_at_end"] = _err(gs["continuing_density_error"])
 
+    render = None
+    if not args.no_render:
+        tuning = g.get("tuning")
+        if tuning is None and (args.block or args.flags or args.sort_bits or args.queue):
+            tuning = {"block": args.block or 256, "flags": args.flags,
+                      "sort_bits": args.sort_bits or 0, "queue": args.queue or 0}
+        render = run_render(args, sc, tuning, elapsed * 1e3 / args.steps)
+
     learned_cmp = None
     if not args.no_learned:
         learned_cmp = run_learned(args, sc)
@@ -456,6 +512,7 @@ def main():
                 "substeps": args.steps, "after_warmup_substeps": args.warmup,
                 "density_error_at_end": _err(g["continuing_density_error"])},
             "stable_variant": stable,
+            "with_rendering_steps": render,
             "learned_vs_plain": learned_cmp,
             "batched_scenes": batched,
         },

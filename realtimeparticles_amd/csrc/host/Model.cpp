@@ -240,6 +240,17 @@ void* Fluids::devicePositions() const
   return p;
 }
 
+void Fluids::enableRendering(bool enable)
+{
+  check(rtp_fluids_set_render(m_h, enable ? 1 : 0), "rtp_fluids_set_render");
+}
+
+void Fluids::setCameraPos(float x, float y, float z)
+{
+  const float p[3] = { x, y, z };
+  check(rtp_fluids_set_camera(m_h, p), "rtp_fluids_set_camera");
+}
+
 std::vector<rtp_kernel_time> Fluids::kernelTimes() const
 {
   int count = 0;
@@ -335,6 +346,17 @@ void* Boids::devicePositions() const
   void* p = nullptr;
   check(rtp_boids_device_state(m_h, &p, nullptr), "rtp_boids_device_state");
   return p;
+}
+
+void Boids::enableRendering(bool enable)
+{
+  check(rtp_boids_set_render(m_h, enable ? 1 : 0), "rtp_boids_set_render");
+}
+
+void Boids::setCameraPos(float x, float y, float z)
+{
+  const float p[3] = { x, y, z };
+  check(rtp_boids_set_camera(m_h, p), "rtp_boids_set_camera");
 }
 
 std::vector<rtp_kernel_time> Boids::kernelTimes() const

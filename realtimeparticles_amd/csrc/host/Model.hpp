@@ -122,6 +122,11 @@ class Model {
   virtual void readState(float* pos4, float* vel4, size_t count) const = 0;
   // Device pointer of the float4 position buffer (the GL VBO's role).
   virtual void* devicePositions() const = 0;
+  // The rendering steps at the end of every update (grid detector, colour, camera-distance
+  // sort; Fluids.cpp:458-468, Boids.cpp:375-381), off until enabled, and the camera
+  // position they sort by (the reference's u_cameraPos buffer, Model.hpp:69).
+  virtual void enableRendering(bool enable) = 0;
+  virtual void setCameraPos(float x, float y, float z) = 0;
   // Per-kernel timings collected while profiling was enabled.
   virtual std::vector<rtp_kernel_time> kernelTimes() const = 0;
 
@@ -166,6 +171,8 @@ class Fluids : public Model {
   std::vector<rtp_kernel_time> kernelTimes() const override;
   // max |rho/rho0 - 1|, mean |rho/rho0 - 1|, max compression of the last update.
   std::array<float, 3> densityError() const;
+  void enableRendering(bool enable) override;
+  void setCameraPos(float x, float y, float z) override;
   rtp_fluids* handle() const { return m_h; }
 
  protected:
@@ -194,6 +201,8 @@ class Boids : public Model {
   void readState(float* pos4, float* vel4, size_t count) const override;
   void* devicePositions() const override;
   std::vector<rtp_kernel_time> kernelTimes() const override;
+  void enableRendering(bool enable) override;
+  void setCameraPos(float x, float y, float z) override;
   rtp_boids* handle() const { return m_h; }
 
  protected:
