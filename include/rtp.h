@@ -311,6 +311,21 @@ int rtp_boids_sync(rtp_boids* h);
 int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
 int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count);
 int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4);
+/* Model::setDimension (physics/Model.hpp:109-113) for boids: 2 selects the rules of
+ * bd_applyBoidsRulesWithGrid2D (boids.cl:137-221: the nine YZ cells of the x layer
+ * GRID_RES_X / 2, Boids.cpp:346-349), 3 (the default) the 3-D rules. */
+int rtp_boids_set_dimension(rtp_boids* h, int dim);
+/* Model::setBoundary (physics/Model.hpp:116-119, Boundary :38-44): the wall of bd_updatePosAndApplyWallBC
+ * (bouncing, the default) or of bd_updatePosAndApplyPeriodicBC (cyclic, boids.cl:289-315),
+ * Boids.cpp:363-372. */
+typedef enum { RTP_BOUNDARY_BOUNCING = 0, RTP_BOUNDARY_CYCLIC = 1 } rtp_boundary;
+int rtp_boids_set_boundary(rtp_boids* h, int boundary);
+/* The target rule, bd_addTargetRule (boids.cl:226-241, Boids.cpp:351-358): while active,
+ * every step adds sign * (pos - p) * clamp(1.3 / |pos - p|, 0, 0.7) to the acceleration of
+ * each boid p within radius of pos (TargetParams, Boids.hpp:22-26).  The reference moves
+ * the target on the host every frame (Target::updatePos, physics/utils/Target.cpp:27-50);
+ * here the caller passes the position of the frame. */
+int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float radius, int sign);
 /* Boids.cpp:375-381: the grid detector after every step and, paused or not, the
  * camera-distance sort that permutes p_pos, p_vel and p_acc (p_col holds bd_fillBoidsColor's
  * single colour, boids.cl:38-41).  As rtp_fluids_set_render: off by default, enabling resets

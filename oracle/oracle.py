@@ -93,6 +93,8 @@ class _BoidsState(ctypes.Structure):
         ("pos", _vp), ("vel", _vp), ("acc", _vp), ("tmp", _vp),
         ("key", _vp), ("idx", _vp),
         ("se", _vp),
+        ("dim", ctypes.c_int32), ("cyclic", ctypes.c_int32), ("targetOn", ctypes.c_int32),
+        ("target", _f32 * 3), ("targetRadius", _f32), ("targetSign", ctypes.c_int32),
     ]
 
 
@@ -175,6 +177,9 @@ def lib():
         L.orc_bd_rules3d.argtypes = [_vp, _vp, _vp, P(BoidsParams), _vp, _u32, P(Consts)]
         L.orc_bd_update_vel.argtypes = [_vp, _f32, _f32, _vp, _u32]
         L.orc_bd_update_pos_wall.argtypes = [_vp, _f32, _vp, _u32, P(Consts)]
+        L.orc_bd_rules2d.argtypes = [_vp, _vp, _vp, P(BoidsParams), _vp, _u32, P(Consts)]
+        L.orc_bd_add_target.argtypes = [_vp, P(_f32), _f32, ctypes.c_int32, _vp, _u32]
+        L.orc_bd_update_pos_periodic.argtypes = [_vp, _f32, _vp, _u32, P(Consts)]
         L.orc_fluids_update.restype = ctypes.c_int
         L.orc_fluids_update.argtypes = [P(_FluidsState)]
         L.orc_boids_update.restype = ctypes.c_int
@@ -310,6 +315,7 @@ class OracleBoids:
         self.s.p = params
         self.s.timeStep = float(time_step)
         self.s.maxN, self.s.n = self.max_n, self.n
+        self.s.dim = 3
         for name, arr in (("pos", self.pos), ("vel", self.vel), ("acc", self.acc),
                           ("tmp", self.tmp), ("key", self.key), ("idx", self.idx), ("se", self.se)):
             setattr(self.s, name, ptr(arr))
@@ -323,6 +329,21 @@ class OracleBoids:
         self.pos[:m] = pos
         self.vel[:m] = vel
         lib().orc_reset_cell_ids(ptr(self.key), self.max_n, ctypes.byref(self.c))
+
+    def set_dimension(self, dim: int) -> None:
+        self.s.dim = int(dim)
+
+    def set_cyclic(self, flag: bool) -> None:
+        self.s.cyclic = int(bool(flag))
+
+    def set_target(self, active: bool, pos=(0.0, 0.0, 0.0), radius: float = 2.0,
+                   sign: int = 1) -> None:
+        """Target::pos / radiusEffect / signEffect of the next frames (Target.hpp)."""
+        self.s.targetOn = int(bool(active))
+        for k in range(3):
+            self.s.target[k] = float(pos[k])
+        self.s.targetRadius = float(radius)
+        self.s.targetSign = int(sign)
 
     def enable_render(self, camera) -> None:
         """The rendering steps of every frame (Boids.cpp:375-381), with p_col filled by

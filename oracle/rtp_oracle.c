@@ -657,6 +657,88 @@ void orc_bd_rules3d(const orc_f4* position, const orc_f4* velocity, const orc_u2
   }
 }
 
+/* boids.cl:137-221 */
+void orc_bd_rules2d(const orc_f4* position, const orc_f4* velocity, const orc_u2* se,
+                    const orc_boids_params* prm, orc_f4* acc, uint32_t n, const orc_consts* c)
+{
+  const int32_t rx = c->res[0], ry = c->res[1];
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = position[i];
+    uint32_t c3[3];
+    cell3d(pos, c, c3);
+    int count = 0;
+    orc_f4 newAcc = { 0.0f, 0.0f, 0.0f, 0.0f };
+    orc_f4 avgPos = { 0.0f, 0.0f, 0.0f, 0.0f };
+    orc_f4 avgVel = { 0.0f, 0.0f, 0.0f, 0.0f };
+    orc_f4 repulse = { 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int iY = -1; iY <= 1; ++iY)
+      for (int iZ = -1; iZ <= 1; ++iZ) {
+        const int32_t nx = (int32_t)c3[0], ny = (int32_t)c3[1] + iY, nz = (int32_t)c3[2] + iZ;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= rx || ny >= rx || nz >= rx)
+          continue;
+        const uint32_t cn = (uint32_t)((rx / 2 * rx + ny) * ry + nz);
+        if (cn >= c->numCells)
+          continue;
+        for (uint32_t e = se[cn].x; e <= se[cn].y; ++e) {
+          const orc_f4 posN = position[e];
+          const orc_f4 vec = sub4(pos, posN);
+          const float sq = dot4(vec, vec);
+          if (sq < c->effectRadiusSq && sq > ORC_FLOAT_EPS) {
+            avgPos = add4(avgPos, posN);
+            avgVel = add4(avgVel, normalize4(velocity[e]));
+            repulse = add4(repulse, div4s(vec, sq));
+            ++count;
+          }
+        }
+      }
+    if (count != 0) {
+      avgPos = div4s(avgPos, (float)count);
+      avgPos = sub4(avgPos, pos);
+      avgPos = mul4s(normalize4(avgPos), prm->velocityScale);
+      avgVel = mul4s(normalize4(avgVel), prm->velocityScale);
+      repulse = mul4s(normalize4(repulse), prm->velocityScale);
+      newAcc = add4(add4(mul4s(avgVel, prm->alignmentScale), mul4s(repulse, prm->separationScale)),
+                    mul4s(avgPos, prm->cohesionScale));
+    }
+    acc[i] = newAcc;
+  }
+}
+
+/* boids.cl:226-241 (MAX_STEERING 0.5f, boids.cl:10) */
+void orc_bd_add_target(const orc_f4* pos, const float target[3], float radius, int32_t sign,
+                       orc_f4* acc, uint32_t n)
+{
+  const orc_f4 t = { target[0], target[1], target[2], 0.0f };
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 vec = sub4(t, pos[i]);
+    const float dist = len4(vec);
+    if (dist < radius)
+      acc[i] = add4(acc[i], mul4s(muls4((float)sign, vec), clampf(1.3f / dist, 0.0f, 1.4f * 0.5f)));
+  }
+}
+
+/* boids.cl:289-315 */
+void orc_bd_update_pos_periodic(const orc_f4* vel, float dt, orc_f4* pos, uint32_t n,
+                                const orc_consts* c)
+{
+  const float wx = c->absWall[0], wy = c->absWall[1], wz = c->absWall[2];
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 np = add4(pos[i], mul4s(vel[i], dt));
+    orc_f4 cp = { clampf(np.x, -wx, wx), clampf(np.y, -wy, wy), clampf(np.z, -wz, wz),
+                  clampf(np.w, 0.0f, 0.0f) };
+    if (!(cp.x == np.x))
+      cp.x *= -1.0f;
+    if (!(cp.y == np.y))
+      cp.y *= -1.0f;
+    if (!(cp.z == np.z))
+      cp.z *= -1.0f;
+    pos[i] = cp;
+  }
+}
+
 /* boids.cl:246-259 */
 void orc_bd_update_vel(const orc_f4* acc, float dt, float maxVel, orc_f4* vel, uint32_t n)
 {
@@ -728,8 +810,8 @@ static int fluids_physics(orc_fluids* s, orc_f4* col)
 /* physics/ocl/Fluids.cpp:400-471 (pause, colour, grid detector and camera sort omitted). */
 int orc_fluids_update(orc_fluids* s) { return fluids_physics(s, NULL); }
 
-/* physics/ocl/Boids.cpp:332-372, the step of one frame (3-D rules, bouncing wall, no
- * target).  col (p_col), when given, rides along in the cell sort's permutation (:337). */
+/* physics/ocl/Boids.cpp:332-372, the step of one frame.  col (p_col), when given, rides
+ * along in the cell sort's permutation (:337). */
 static int boids_physics(orc_boids* s, orc_f4* col)
 {
   const orc_consts* c = &s->c;
@@ -747,13 +829,21 @@ static int boids_physics(orc_boids* s, orc_f4* col)
   orc_fill_start(s->key, s->se, n, c);
   orc_fill_end(s->key, s->se, n, s->maxN, c);
   orc_adjust_end(s->se, c);
-  orc_bd_rules3d(s->pos, s->vel, s->se, &s->p, s->acc, n, c);          /* :349 */
+  if (s->dim == 2)                                                     /* :346-349 */
+    orc_bd_rules2d(s->pos, s->vel, s->se, &s->p, s->acc, n, c);
+  else
+    orc_bd_rules3d(s->pos, s->vel, s->se, &s->p, s->acc, n, c);
+  if (s->targetOn)                                                     /* :351-358 */
+    orc_bd_add_target(s->pos, s->target, s->targetRadius, s->targetSign, s->acc, n);
   orc_bd_update_vel(s->acc, s->timeStep, s->p.velocityScale, s->vel, n); /* :360-361 */
-  orc_bd_update_pos_wall(s->vel, s->timeStep, s->pos, n, c);          /* :369-371 */
+  if (s->cyclic)                                                       /* :363-372 */
+    orc_bd_update_pos_periodic(s->vel, s->timeStep, s->pos, n, c);
+  else
+    orc_bd_update_pos_wall(s->vel, s->timeStep, s->pos, n, c);
   return 0;
 }
 
-/* physics/ocl/Boids.cpp:323-384 (3-D rules, bouncing wall, no target, no rendering). */
+/* physics/ocl/Boids.cpp:323-384 without the rendering steps. */
 int orc_boids_update(orc_boids* s) { return boids_physics(s, NULL); }
 
 /* ------------------------------------------------------------------------------------

@@ -132,6 +132,20 @@ void orc_bd_rules3d(const orc_f4* pos, const orc_f4* vel, const orc_u2* se,
                     const orc_boids_params* p, orc_f4* acc, uint32_t n, const orc_consts* c);
 void orc_bd_update_vel(const orc_f4* acc, float dt, float maxVel, orc_f4* vel, uint32_t n);
 void orc_bd_update_pos_wall(orc_f4* vel, float dt, orc_f4* pos, uint32_t n, const orc_consts* c);
+/* boids.cl:137-221: the 9 cells of the YZ plane, at neighbour index
+ * (RX/2 * RX + y) * RY + z with every coordinate range-checked against RX, as written (the
+ * two agree with the grid's own indexing on cubic grids).  An index past the cell table,
+ * which only non-cubic grids produce and where the reference reads out of bounds, is
+ * skipped. */
+void orc_bd_rules2d(const orc_f4* pos, const orc_f4* vel, const orc_u2* se,
+                    const orc_boids_params* p, orc_f4* acc, uint32_t n, const orc_consts* c);
+/* boids.cl:226-241: acc += sign * (target - pos) * clamp(1.3 / dist, 0, 1.4 * 0.5) within
+ * the radius. */
+void orc_bd_add_target(const orc_f4* pos, const float target[3], float radius, int32_t sign,
+                       orc_f4* acc, uint32_t n);
+/* boids.cl:289-315: a coordinate clamped at a wall re-enters at the opposite wall. */
+void orc_bd_update_pos_periodic(const orc_f4* vel, float dt, orc_f4* pos, uint32_t n,
+                                const orc_consts* c);
 
 /* ---- host step drivers (caller-owned arrays, all maxN long except se = numCells) ---- */
 typedef struct {
@@ -152,11 +166,19 @@ typedef struct {
   orc_f4 *pos, *vel, *acc, *tmp;
   uint32_t *key, *idx;
   orc_u2* se;
+  /* variants of Boids::update (Boids.cpp:346-372) */
+  int32_t dim;          /* 2: bd_applyBoidsRulesWithGrid2D, else the 3-D rules */
+  int32_t cyclic;       /* Boundary::CyclicWall: bd_updatePosAndApplyPeriodicBC */
+  int32_t targetOn;     /* isTargetActivated(): bd_addTargetRule */
+  float target[3];      /* the target position of this frame (Target::pos) */
+  float targetRadius;   /* TargetParams.targetRadiusEffect */
+  int32_t targetSign;   /* TargetParams.targetSignEffect */
 } orc_boids;
 
 /* Fluids::update (Fluids.cpp:400-471) without GL, colour or the camera-distance sort. */
 int orc_fluids_update(orc_fluids* s);
-/* Boids::update (Boids.cpp:323-384) with the bouncing wall, without GL or target. */
+/* Boids::update (Boids.cpp:323-384) without GL: the rules of s->dim, the target rule when
+ * s->targetOn, the cyclic or bouncing wall. */
 int orc_boids_update(orc_boids* s);
 
 /* ---- rendering side of a frame (utils.cl:35-77, grid.cl:43-60, fluids.cl:458-478,

@@ -22,6 +22,7 @@ _int = ctypes.c_int
 
 FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
 FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
+BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
 
 
@@ -194,6 +195,9 @@ _SIGS = {
     "rtp_boids_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_boids_read_field": ([_vp, _int, _vp, _u32], _int),
     "rtp_boids_set_render": ([_vp, _int], _int),
+    "rtp_boids_set_dimension": ([_vp, _int], _int),
+    "rtp_boids_set_boundary": ([_vp, _int], _int),
+    "rtp_boids_set_target": ([_vp, _int, ctypes.POINTER(_f32), _f32, _int], _int),
     "rtp_boids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
     "rtp_boids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_boids_enable_profiling": ([_vp, _int], _int),
@@ -667,6 +671,22 @@ class Boids(_RenderFields):
     def set_nb_particles(self, n: int) -> None:
         _check(lib().rtp_boids_set_nb_particles(self._h, int(n)), "set_nb_particles")
         self.n = int(n)
+
+    def set_dimension(self, dim: int) -> None:
+        """2: the 2-D rules (boids.cl:137-221); 3: the 3-D rules."""
+        _check(lib().rtp_boids_set_dimension(self._h, int(dim)), "set_dimension")
+
+    def set_cyclic(self, flag: bool) -> None:
+        """The cyclic wall (boids.cl:289-315) instead of the bouncing one."""
+        _check(lib().rtp_boids_set_boundary(self._h, BOUNDARY_CYCLIC if flag else BOUNDARY_BOUNCING),
+               "set_boundary")
+
+    def set_target(self, active: bool, pos=(0.0, 0.0, 0.0), radius: float = 2.0,
+                   sign: int = 1) -> None:
+        """The target rule (boids.cl:226-241) at pos for the following steps."""
+        p = (_f32 * 3)(*[float(v) for v in pos])
+        _check(lib().rtp_boids_set_target(self._h, int(bool(active)), p, float(radius), int(sign)),
+               "set_target")
 
     def load_state(self, pos, vel) -> None:
         pos, vel = _f4(pos), _f4(vel)

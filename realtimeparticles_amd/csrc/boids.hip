@@ -1,10 +1,12 @@
-// 3-D boids steering pass for gfx950.
+// Boids steering pass for gfx950.
 //
 // Reference: physics/ocl/kernels/boids.cl:46-132 (bd_applyBoidsRulesWithGrid3D),
-// :246-259 (bd_updateVel), :264-284 (bd_updatePosAndApplyWallBC), step order
-// physics/ocl/Boids.cpp:323-384.  The three kernels are fused into one pass that reads the
-// sorted (position, velocity) buffers and writes the next state into the other pair of
-// buffers, so no boid reads a neighbour's already-updated state.
+// :137-221 (bd_applyBoidsRulesWithGrid2D), :226-241 (bd_addTargetRule), :246-259
+// (bd_updateVel), :264-284 (bd_updatePosAndApplyWallBC), :289-315
+// (bd_updatePosAndApplyPeriodicBC), step order physics/ocl/Boids.cpp:323-384.  The kernels
+// of one step are fused into one pass that reads the sorted (position, velocity) buffers and
+// writes the next state into the other pair of buffers, so no boid reads a neighbour's
+// already-updated state.
 #include "rtp_device.hpp"
 
 namespace rtp {
@@ -37,6 +39,7 @@ __device__ __forceinline__ V3 normalize3(V3 v)
   return V3{ v.x / l, v.y / l, v.z / l };
 }
 
+template <int DIM2>
 __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
                                                     const float4* __restrict__ velocity,
                                                     const uint2* __restrict__ se,
@@ -54,37 +57,57 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
   int count = 0;
   V3 avgPos{ 0.0f, 0.0f, 0.0f }, avgVel{ 0.0f, 0.0f, 0.0f }, rep{ 0.0f, 0.0f, 0.0f };
-  for (int ix = -1; ix <= 1; ++ix) {
-    const int32_t nx = (int32_t)cx + ix;
-    if (nx < 0 || nx >= rx)
-      continue;
-    for (int iy = -1; iy <= 1; ++iy) {
-      const int32_t ny = (int32_t)cy + iy;
-      if (ny < 0 || ny >= ry)
-        continue;
+  auto visit = [&](uint32_t cell) {
+    const uint2 r = se[cell];
+    for (uint32_t e = r.x; e <= r.y; ++e) {
+      const float4 q = position[e];
+      const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
+      const float sq = vx * vx + vy * vy + vz * vz;
+      if (sq < c.radiusSq && sq > kFloatEps) {  // boids.cl:102-103
+        avgPos.x = avgPos.x + q.x;
+        avgPos.y = avgPos.y + q.y;
+        avgPos.z = avgPos.z + q.z;
+        const float4 ve = velocity[e];
+        const V3 nv = normalize3(V3{ ve.x, ve.y, ve.z });
+        avgVel.x = avgVel.x + nv.x;
+        avgVel.y = avgVel.y + nv.y;
+        avgVel.z = avgVel.z + nv.z;
+        rep.x = rep.x + vx / sq;
+        rep.y = rep.y + vy / sq;
+        rep.z = rep.z + vz / sq;
+        ++count;
+      }
+    }
+  };
+  if (DIM2) {
+    // boids.cl:166-179: the YZ neighbours in the x layer RX/2, every coordinate checked
+    // against RX and the index formed as (RX/2 * RX + y) * RY + z.  On the cubic grids the
+    // reference runs in 2-D these are the grid's own bounds and index; an index past the
+    // table (a non-cubic grid, where the reference reads out of bounds) is skipped.
+    const int32_t nx = (int32_t)cx;
+    for (int iy = -1; iy <= 1; ++iy)
       for (int iz = -1; iz <= 1; ++iz) {
-        const int32_t nz = (int32_t)cz + iz;
-        if (nz < 0 || nz >= rz)
+        const int32_t ny = (int32_t)cy + iy, nz = (int32_t)cz + iz;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= rx || ny >= rx || nz >= rx)
           continue;
-        const uint2 r = se[(nx * ry + ny) * rz + nz];
-        for (uint32_t e = r.x; e <= r.y; ++e) {
-          const float4 q = position[e];
-          const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
-          const float sq = vx * vx + vy * vy + vz * vz;
-          if (sq < c.radiusSq && sq > kFloatEps) {  // boids.cl:102-103
-            avgPos.x = avgPos.x + q.x;
-            avgPos.y = avgPos.y + q.y;
-            avgPos.z = avgPos.z + q.z;
-            const float4 ve = velocity[e];
-            const V3 nv = normalize3(V3{ ve.x, ve.y, ve.z });
-            avgVel.x = avgVel.x + nv.x;
-            avgVel.y = avgVel.y + nv.y;
-            avgVel.z = avgVel.z + nv.z;
-            rep.x = rep.x + vx / sq;
-            rep.y = rep.y + vy / sq;
-            rep.z = rep.z + vz / sq;
-            ++count;
-          }
+        const uint32_t cell = (uint32_t)((rx / 2 * rx + ny) * ry + nz);
+        if (cell < c.g.numCells)
+          visit(cell);
+      }
+  } else {
+    for (int ix = -1; ix <= 1; ++ix) {
+      const int32_t nx = (int32_t)cx + ix;
+      if (nx < 0 || nx >= rx)
+        continue;
+      for (int iy = -1; iy <= 1; ++iy) {
+        const int32_t ny = (int32_t)cy + iy;
+        if (ny < 0 || ny >= ry)
+          continue;
+        for (int iz = -1; iz <= 1; ++iz) {
+          const int32_t nz = (int32_t)cz + iz;
+          if (nz < 0 || nz >= rz)
+            continue;
+          visit((uint32_t)((nx * ry + ny) * rz + nz));
         }
       }
     }
@@ -102,6 +125,16 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
     acc.y = (b.y * c.p.alignmentScale + d.y * c.p.separationScale) + a.y * c.p.cohesionScale;
     acc.z = (b.z * c.p.alignmentScale + d.z * c.p.separationScale) + a.z * c.p.cohesionScale;
   }
+  if (c.targetOn) {  // bd_addTargetRule, boids.cl:226-241 (MAX_STEERING 0.5f, :10)
+    const V3 tv{ c.target[0] - p.x, c.target[1] - p.y, c.target[2] - p.z };
+    const float dist = __builtin_sqrtf(tv.x * tv.x + tv.y * tv.y + tv.z * tv.z);
+    if (dist < c.targetRadius) {
+      const float k = clampf(1.3f / dist, 0.0f, 1.4f * 0.5f);
+      acc.x = acc.x + (c.targetSign * tv.x) * k;
+      acc.y = acc.y + (c.targetSign * tv.y) * k;
+      acc.z = acc.z + (c.targetSign * tv.z) * k;
+    }
+  }
   accOut[i] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 
   // bd_updateVel, boids.cl:246-259 (maxVelocity = velocityScale, Boids.cpp:217)
@@ -112,13 +145,22 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
   const V3 dir = normalize3(nv);
   V3 vel{ dir.x * norm, dir.y * norm, dir.z * norm };
 
-  // bd_updatePosAndApplyWallBC, boids.cl:264-284
+  // bd_updatePosAndApplyWallBC, boids.cl:264-284, or bd_updatePosAndApplyPeriodicBC,
+  // :289-315 (a clamped coordinate re-enters at the opposite wall, the velocity is kept)
   const V3 np{ p.x + vel.x * c.dt, p.y + vel.y * c.dt, p.z + vel.z * c.dt };
-  const V3 cp{ clampf(np.x, -c.g.absWall[0], c.g.absWall[0]),
-               clampf(np.y, -c.g.absWall[1], c.g.absWall[1]),
-               clampf(np.z, -c.g.absWall[2], c.g.absWall[2]) };
-  if (!(cp.x == np.x && cp.y == np.y && cp.z == np.z))
+  V3 cp{ clampf(np.x, -c.g.absWall[0], c.g.absWall[0]),
+         clampf(np.y, -c.g.absWall[1], c.g.absWall[1]),
+         clampf(np.z, -c.g.absWall[2], c.g.absWall[2]) };
+  if (c.cyclic) {
+    if (!(cp.x == np.x))
+      cp.x = cp.x * -1.0f;
+    if (!(cp.y == np.y))
+      cp.y = cp.y * -1.0f;
+    if (!(cp.z == np.z))
+      cp.z = cp.z * -1.0f;
+  } else if (!(cp.x == np.x && cp.y == np.y && cp.z == np.z)) {
     vel = V3{ vel.x * -0.5f, vel.y * -0.5f, vel.z * -0.5f };
+  }
   posOut[i] = make_float4(cp.x, cp.y, cp.z, 0.0f);
   velOut[i] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
@@ -143,8 +185,12 @@ void launch_boids_step(const float4* pos, const float4* vel, const uint2* startE
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_boids_step, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos, vel,
-                     startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+  if (c.dim2)
+    hipLaunchKernelGGL(k_boids_step<1>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
+                       vel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+  else
+    hipLaunchKernelGGL(k_boids_step<0>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
+                       vel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
 }
 
 }  // namespace rtp

@@ -1421,6 +1421,37 @@ int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
   return RTP_OK;
 }
 
+int rtp_boids_set_dimension(rtp_boids* h, int dim)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(dim == 2 || dim == 3, "dimension must be 2 or 3");
+  h->c.dim2 = dim == 2 ? 1 : 0;
+  return RTP_OK;
+}
+
+int rtp_boids_set_boundary(rtp_boids* h, int boundary)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(boundary == RTP_BOUNDARY_BOUNCING || boundary == RTP_BOUNDARY_CYCLIC,
+              "unknown boundary");
+  h->c.cyclic = boundary == RTP_BOUNDARY_CYCLIC ? 1 : 0;
+  return RTP_OK;
+}
+
+int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float radius, int sign)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(!active || pos, "NULL target position");
+  h->c.targetOn = active ? 1 : 0;
+  if (active) {
+    for (int k = 0; k < 3; ++k)
+      h->c.target[k] = pos[k];
+    h->c.targetRadius = radius;
+    h->c.targetSign = (float)sign;
+  }
+  return RTP_OK;
+}
+
 int rtp_boids_set_render(rtp_boids* h, int enable)
 {
   RTP_REQUIRE(h, "NULL handle");

@@ -50,6 +50,38 @@ std::vector<F3> sphere_grid(std::array<int, 3> res, F3 s, F3 e)
   return v;
 }
 
+// utils/Geometry.cpp:71-133 GenerateRectangularGrid (uniform) in the YZ plane: res[0] rows
+// along y, res[1] along z, start + i * (end - start) / res.
+std::vector<F3> rect_grid_yz(std::array<int, 2> res, F3 s, F3 e)
+{
+  std::vector<F3> v;
+  v.reserve((size_t)res[0] * res[1]);
+  const F3 sp{ 0.0f, (e.y - s.y) / res[0], (e.z - s.z) / res[1] };
+  for (int iy = 0; iy < res[0]; ++iy)
+    for (int iz = 0; iz < res[1]; ++iz)
+      v.push_back(F3{ s.x + 0 * sp.x, s.y + iy * sp.y, s.z + iz * sp.z });
+  return v;
+}
+
+// utils/Geometry.cpp:136-196 GenerateCircularGrid in the YZ plane: res[0] angles times
+// res[1] rings of the circle inscribed in [s, e], the innermost ring one spacing out.
+std::vector<F3> circle_grid_yz(std::array<int, 2> res, F3 s, F3 e)
+{
+  const float PI_F = 3.1415927f;
+  const F3 vec{ e.x - s.x, e.y - s.y, e.z - s.z };
+  const F3 c{ s.x + vec.x / 2.0f, s.y + vec.y / 2.0f, s.z + vec.z / 2.0f };
+  const float radius = std::sqrt(vec.x * vec.x + vec.y * vec.y + vec.z * vec.z) / 2.0f;
+  const float da = 2.0f * PI_F / res[0], dr = radius / res[1];
+  std::vector<F3> v;
+  v.reserve((size_t)res[0] * res[1]);
+  for (int io = 0; io < res[0]; ++io)
+    for (int ir = 0; ir < res[1]; ++ir) {
+      const float r = (ir + 1) * dr;
+      v.push_back(F3{ c.x, c.y + r * std::cos(io * da), c.z + r * std::sin(io * da) });
+    }
+  return v;
+}
+
 std::vector<float> to_float4(const std::vector<F3>& v)
 {
   std::vector<float> out(v.size() * 4, 0.0f);
@@ -81,6 +113,22 @@ rtp_model_desc make_desc(const Physics::ModelParams& p)
 }  // namespace
 
 namespace Utils {
+// utils/Parameters.hpp:89-98 (ALL_NB_PARTICLES, subdiv2D column)
+std::array<int, 2> GetNbParticlesSubdiv2D(NbParticles nb)
+{
+  switch (nb) {
+  case P512: return { 32, 16 };
+  case P1K: return { 32, 32 };
+  case P4K: return { 64, 64 };
+  case P8K: return { 128, 64 };
+  case P16K: return { 128, 128 };
+  case P32K: return { 128, 256 };
+  case P65K: return { 256, 256 };
+  case P130K: return { 256, 512 };
+  }
+  return { 0, 0 };
+}
+
 std::array<int, 3> GetNbParticlesSubdiv3D(NbParticles nb)
 {
   switch (nb) {
@@ -165,18 +213,47 @@ void Fluids::update()
   check(rtp_fluids_sync(m_h), "rtp_fluids_sync");  // a frame ends synchronised (Context.cpp:746)
 }
 
-// Fluids::reset / initFluidsParticles (Fluids.cpp:196-216, 335-398), 3-D cases.
+// Fluids::reset / initFluidsParticles (Fluids.cpp:196-216, 273-398): the 2-D cases lie in
+// the YZ plane (:287-335), the 3-D ones fill boxes and a sphere.
 void Fluids::reset()
 {
   if (!m_init)
     return;
-  const rtp_fluid_params defaults{ 450.0f, 600.0f, 0.010f, 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
+  const bool d2 = m_dimension == Geometry::Dimension::dim2D;
+  const rtp_fluid_params defaults{ 450.0f, 600.0f, 0.010f, d2 ? 2u : 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
   check(rtp_fluids_set_params(m_h, &defaults), "rtp_fluids_set_params");
   m_nbJacobiIters = 2;
   check(rtp_fluids_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_fluids_set_nb_jacobi_iters");
   const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
   std::vector<F3> verts;
-  switch (m_case) {
+  if (d2) {
+    switch (m_case) {
+    case Utils::FLUIDS_DAM:
+      m_currNbParticles = Utils::P4K;
+      verts = rect_grid_yz(Utils::GetNbParticlesSubdiv2D(Utils::P4K), F3{ 0.0f, b.y / -2.0f, b.z / -2.0f },
+                           F3{ 0.0f, 0.0f, 0.0f });
+      break;
+    case Utils::FLUIDS_BOMB:
+      m_currNbParticles = Utils::P4K;
+      verts = rect_grid_yz(Utils::GetNbParticlesSubdiv2D(Utils::P4K), F3{ 0.0f, b.y / -6.0f, b.z / -6.0f },
+                           F3{ 0.0f, b.y / 6.0f, b.z / 6.0f });
+      break;
+    case Utils::FLUIDS_DROP: {
+      m_currNbParticles = Utils::P512 + Utils::P4K;
+      verts = rect_grid_yz(Utils::GetNbParticlesSubdiv2D(Utils::P512),
+                           F3{ 0.0f, 2.0f * b.y / 10.0f, b.z / -10.0f }, F3{ 0.0f, 4.0f * b.y / 10.0f, b.z / 10.0f });
+      // a 64 x 128 floor of which only the first 4,096 points are active, as in the reference
+      const auto bottom = rect_grid_yz({ 64, 128 }, F3{ 0.0f, b.y / -2.0f, b.z / -2.0f },
+                                       F3{ 0.0f, 0.0f, b.z / 2.0f });
+      verts.insert(verts.end(), bottom.begin(), bottom.end());
+      break;
+    }
+    default:
+      throw std::runtime_error("Fluids::reset: unknown case");
+    }
+    if (verts.size() > m_maxNbParticles)
+      verts.resize(m_maxNbParticles);
+  } else switch (m_case) {
   case Utils::FLUIDS_DAM:
     m_currNbParticles = Utils::P130K;
     verts = box_grid(Utils::GetNbParticlesSubdiv3D(Utils::P130K), F3{ b.x / -2.0f, b.y / -2.0f, b.z / -2.0f },
@@ -296,14 +373,17 @@ void Boids::update()
 {
   if (!m_init)
     return;
-  if (m_boundary != Boundary::BouncingWall)
-    throw std::runtime_error("Boids::update: only the bouncing wall is provided");
+  check(rtp_boids_set_dimension(m_h, m_dimension == Geometry::Dimension::dim2D ? 2 : 3),
+        "rtp_boids_set_dimension");
+  check(rtp_boids_set_boundary(m_h, m_boundary == Boundary::CyclicWall ? RTP_BOUNDARY_CYCLIC
+                                                                       : RTP_BOUNDARY_BOUNCING),
+        "rtp_boids_set_boundary");
   check(rtp_boids_update(m_h, 1), "rtp_boids_update");
   check(rtp_boids_sync(m_h), "rtp_boids_sync");
 }
 
-// Boids::reset / initBoidsParticles (Boids.cpp:228-321), 3-D: a sphere lattice, with the
-// velocity buffer initialised from the positions.
+// Boids::reset / initBoidsParticles (Boids.cpp:228-321): a circle lattice in the YZ plane
+// (2-D) or a sphere lattice (3-D), with the velocity buffer initialised from the positions.
 void Boids::reset()
 {
   if (!m_init)
@@ -320,8 +400,12 @@ void Boids::reset()
   if (m_currNbParticles > m_maxNbParticles)
     throw std::runtime_error("Boids::reset: case needs more particles than maxNbParticles");
   const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
-  const auto verts = sphere_grid(Utils::GetNbParticlesSubdiv3D((Utils::NbParticles)m_currNbParticles),
-                                 F3{ b.x / -6.0f, b.y / -6.0f, b.z / -6.0f }, F3{ b.x / 6.0f, b.y / 6.0f, b.z / 6.0f });
+  const auto nb = (Utils::NbParticles)m_currNbParticles;
+  const auto verts = m_dimension == Geometry::Dimension::dim2D
+                         ? circle_grid_yz(Utils::GetNbParticlesSubdiv2D(nb), F3{ 0.0f, b.y / -6.0f, b.z / -6.0f },
+                                          F3{ 0.0f, b.y / 6.0f, b.z / 6.0f })
+                         : sphere_grid(Utils::GetNbParticlesSubdiv3D(nb), F3{ b.x / -6.0f, b.y / -6.0f, b.z / -6.0f },
+                                       F3{ b.x / 6.0f, b.y / 6.0f, b.z / 6.0f });
   const std::vector<float> pos = to_float4(verts);
   check(rtp_boids_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_boids_set_nb_particles");
   check(rtp_boids_load_state(m_h, pos.data(), pos.data(), (uint32_t)verts.size()), "rtp_boids_load_state");

@@ -47,7 +47,8 @@ enum NbParticles {
   P512 = 1 << 9, P1K = 1 << 10, P4K = 1 << 12, P8K = 1 << 13,
   P16K = 1 << 14, P32K = 1 << 15, P65K = 1 << 16, P130K = 1 << 17
 };
-// 3-D lattice subdivisions of utils/Parameters.hpp:89-98.
+// 2-D and 3-D lattice subdivisions of utils/Parameters.hpp:89-98.
+std::array<int, 2> GetNbParticlesSubdiv2D(NbParticles nb);
 std::array<int, 3> GetNbParticlesSubdiv3D(NbParticles nb);
 }  // namespace Utils
 

@@ -68,6 +68,13 @@ struct BoidsConsts {
   float radiusSq;  // EFFECT_RADIUS_SQUARED
   float dt;        // update time step
   float vmin;      // 0.2f * maxVelocity (boids.cl:256)
+  // variants of Boids::update (Boids.cpp:346-372)
+  int32_t dim2 = 0;      // bd_applyBoidsRulesWithGrid2D instead of the 3-D rules
+  int32_t cyclic = 0;    // Boundary::CyclicWall: bd_updatePosAndApplyPeriodicBC
+  int32_t targetOn = 0;  // bd_addTargetRule with the fields below
+  float target[3] = { 0.0f, 0.0f, 0.0f };
+  float targetRadius = 0.0f;
+  float targetSign = 0.0f;  // TargetParams.targetSignEffect as the float the rule multiplies by
 };
 
 // Survivor-queue depth of the neighbour passes when a tuning gives 0.

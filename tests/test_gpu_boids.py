@@ -79,3 +79,60 @@ def test_boids_shrinking_count():
         pos, vel = gpu.read_state(3000)
         _eq(pos[:, :3], orc.pos[:3000, :3], f"pos step {step}")
     gpu.close()
+
+
+def _target_path(step, planar):
+    """A target moving on a closed curve through the flock; in 2-D it stays in the YZ plane
+    as the reference's does (Target::updatePos, physics/utils/Target.cpp:27-50)."""
+    a = 0.05 * step
+    return (0.0 if planar else 0.6 * np.cos(a), 0.4 * np.sin(a), 0.3 * np.sin(2 * a))
+
+
+@pytest.mark.parametrize("variant", ["2d", "cyclic", "target", "2d-cyclic-target"])
+def test_boids_variants_bit_exact(variant):
+    """The 2-D rules (boids.cl:137-221), the cyclic wall (:289-315) and the target rule
+    (:226-241), alone and together, over 120 steps against the oracle."""
+    sc = scenes.boids_flock(8192, seed=21)
+    pos, vel = sc.pos.copy(), sc.vel.copy()
+    if "2d" in variant:  # a flock in the YZ plane, as the reference's 2-D cases lay it out
+        pos[:, 0] = 0.0
+        vel[:, 0] = 0.0
+    gpu = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+    gpu.set_params(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
+    gpu.set_time_step(sc.time_step)
+    gpu.load_state(pos, vel)
+    p = O.BoidsParams(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
+    orc = O.OracleBoids(sc.box, sc.res, sc.max_n, sc.n, p, time_step=sc.time_step)
+    orc.upload(pos, vel)
+    if "2d" in variant:
+        gpu.set_dimension(2)
+        orc.set_dimension(2)
+    if "cyclic" in variant:
+        gpu.set_cyclic(True)
+        orc.set_cyclic(True)
+    n = sc.n
+    wrapped = 0
+    log(f"boids {variant}: {n} boids, 120 steps")
+    for step in range(1, 121):
+        if "target" in variant:
+            t = _target_path(step, "2d" in variant)
+            sign = 1 if (step // 40) % 2 == 0 else -1
+            gpu.set_target(True, t, 0.8, sign)
+            orc.set_target(True, t, 0.8, sign)
+        gpu.update(1)
+        orc.step(1)
+        pos_g, vel_g = gpu.read_state(n)
+        _eq(pos_g[:, :3], orc.pos[:n, :3], f"{variant} pos step {step}")
+        _eq(vel_g[:, :3], orc.vel[:n, :3], f"{variant} vel step {step}")
+        _eq(gpu.read_acc(n)[:, :3], orc.acc[:n, :3], f"{variant} acc step {step}")
+        if step % 40 == 0:
+            keys, idx, se = gpu.read_grid()
+            _eq(keys, orc.key, f"{variant} cellIDs step {step}")
+            _eq(idx, orc.idx, f"{variant} sort index step {step}")
+        if "cyclic" in variant:
+            wrapped += int((np.abs(orc.pos[:n, :3]) == 1.0).any(1).sum())
+    if "2d" in variant:
+        assert not gpu.read_state(n)[0][:, 0].any()  # the flock stays in its plane
+    if "cyclic" in variant:
+        assert wrapped > 0
+    gpu.close()

@@ -48,3 +48,40 @@ def test_cpp_boids_matches_oracle(tmp_path):
     orc.step(3)
     _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
     _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
+
+
+def test_cpp_2d_fluids_dam_matches_oracle(tmp_path):
+    """Fluids.cpp:287-298, 317-320: the 2-D dam break is a 64 x 64 rectangle in the YZ
+    plane over the lower-left quarter of the box."""
+    n, max_n, (pos0, vel0, pos1, vel1) = run_demo("fluids2d", tmp_path, 3)
+    assert n == 4096
+    assert not pos0[:, 0].any()
+    ys, zs = np.unique(pos0[:, 1]), np.unique(pos0[:, 2])
+    assert len(ys) == 64 and len(zs) == 64
+    assert ys[0] == -5.0 and zs[0] == -5.0 and ys[-1] < 0.0 and zs[-1] < 0.0
+    orc = O.OracleFluids((10, 10, 10), (30, 30, 30), max_n, n, O.fluid_params(dim=2), 2,
+                         gravity=9.81)
+    orc.upload(pos0, vel0)
+    orc.step(3)
+    _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
+    _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
+    assert not pos1[:, 0].any()
+
+
+@pytest.mark.parametrize("kind", ["boids2d", "boidscyclic"])
+def test_cpp_boids_variants_match_oracle(tmp_path, kind):
+    n, max_n, (pos0, vel0, pos1, vel1) = run_demo(kind, tmp_path, 3)
+    assert n == 65536
+    if kind == "boids2d":
+        # Boids.cpp:291-298: 256 angles x 256 rings of a circle in the YZ plane
+        assert not pos0[:, 0].any()
+        r = np.hypot(pos0[:, 1].astype(np.float64), pos0[:, 2].astype(np.float64))
+        assert len(np.unique(np.round(r, 4))) == 256
+    orc = O.OracleBoids((10, 10, 10), (30, 30, 30), max_n, n,
+                        O.BoidsParams(0.5, 1.6, 1.6, 1.45), time_step=0.1)
+    orc.set_dimension(2 if kind == "boids2d" else 3)
+    orc.set_cyclic(kind == "boidscyclic")
+    orc.upload(pos0, vel0)
+    orc.step(3)
+    _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
+    _eq(vel1[:, :3], orc.vel[:n, :3], "vel")

@@ -351,19 +351,86 @@ def test_wall_bounce_halves_and_reverses(oracle):
     assert b.vel[0, 0] == pytest.approx(-0.75, rel=1e-6)
 
 
+def test_cyclic_wall_wraps_without_touching_velocity(oracle):
+    pos = np.array([[0.995, 0.0, 0.0, 0], [0.0, -0.998, 0.0, 0], [0.3, 0.2, 0.1, 0]], np.float32)
+    vel = np.array([[1.5, 0.0, 0.0, 0], [0.0, -0.5, 0.0, 0], [0.5, 0.0, 0.0, 0]], np.float32)
+    b = boids(oracle, pos, vel, dt=0.01, vs=2.0)
+    b.set_cyclic(True)
+    b.step(1)
+    # past +W in x: re-enters at -W; past -W in y: at +W; the velocity is kept
+    got = {tuple(p[:3]) for p in b.pos[:3]}
+    assert (-1.0, 0.0, 0.0) in got and (0.0, 1.0, 0.0) in got
+    assert np.float32(0.305) in {np.float32(p[0]) for p in b.pos[:3]}
+    speeds = sorted(np.round(np.linalg.norm(b.vel[:3, :3], axis=1), 6))
+    assert speeds == [0.5, 0.5, 1.5]
+
+
+def test_target_rule_closed_form(oracle):
+    L = oracle.lib()
+    pos = np.array([f4(0, 0, 0), f4(0, 0, 0), f4(0, 0, 0), f4(3, 0, 0)], np.float32)
+    acc = np.zeros((4, 4), np.float32)
+    acc[1] = (0.25, 0.5, 0, 0)
+    tgt = (ctypes.c_float * 3)(1.0, 0.0, 0.0)
+    L.orc_bd_add_target(pos.ctypes.data, tgt, 2.5, 1, acc.ctypes.data, 3)
+    # distance 1: min(1.3 / 1, 1.4 * MAX_STEERING = 0.7) = 0.7 towards the target
+    assert acc[0].tolist() == pytest.approx([0.7, 0, 0, 0])
+    assert acc[1].tolist() == pytest.approx([0.95, 0.5, 0, 0])
+    tgt2 = (ctypes.c_float * 3)(0.0, 2.0, 0.0)
+    a2 = np.zeros((4, 4), np.float32)
+    L.orc_bd_add_target(pos.ctypes.data, tgt2, 2.5, -1, a2.ctypes.data, 4)
+    # distance 2: 1.3 / 2 = 0.65 per unit, repelled (sign -1); the boid at (3, 0, 0) is
+    # sqrt(13) away, outside the radius
+    assert a2[0].tolist() == pytest.approx([0, -1.3, 0, 0])
+    assert not a2[3].any()
+
+
+def test_2d_rules_see_only_the_yz_plane(oracle):
+    # a planar flock at x = 0 in a cubic grid: the 2-D rules equal the 3-D ones
+    rng = np.random.default_rng(4)
+    n = 300
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 1:3] = rng.uniform(-0.9, 0.9, (n, 2))
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, 1:3] = rng.normal(size=(n, 2))
+    a, b = boids(oracle, pos, vel), boids(oracle, pos, vel)
+    b.set_dimension(2)
+    for _ in range(5):
+        a.step(1)
+        b.step(1)
+    assert np.array_equal(a.pos, b.pos) and np.array_equal(a.acc, b.acc)
+    assert a.acc[:n, 1:].any()
+    # The 2-D rules visit the x layer RX/2 whatever the boid's own x.  A (layer 10) and B
+    # (layer 9) are 0.004 apart: the 3-D rules let each see the other; the 2-D rules let B
+    # see A but not A see B.  The third boid takes sorted slot 0 (Q1).
+    pos2 = np.array([[0.002, 0.01, 0.01, 0], [-0.002, 0.01, 0.01, 0], [-0.5, 0.5, 0.5, 0]],
+                    np.float32)
+    vel2 = np.array([[0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 1, 0]], np.float32)
+    c3, c2 = boids(oracle, pos2, vel2, dt=1e-6), boids(oracle, pos2, vel2, dt=1e-6)
+    c2.set_dimension(2)
+    c3.step(1)
+    c2.step(1)
+
+    def acc_of(b, x_sign):
+        k = [i for i in range(3) if np.sign(b.pos[i, 0]) == x_sign and abs(b.pos[i, 0]) < 0.01]
+        return b.acc[k[0]]
+    assert acc_of(c3, 1).any() and acc_of(c3, -1).any()
+    assert not acc_of(c2, 1).any() and acc_of(c2, -1).any()
+
+
 # ------------------------------------------------------------------ rendering side of a frame
 
 def test_camera_distance_special_cases(oracle):
     L = oracle.lib()
     pos = np.array([f4(3, 4, 0), f4(0, 0, 12), f4(1, 2, 2), f4(0, 0, 0),
                     f4(2e4, 0, 0), f4(np.inf, np.inf, np.inf), f4(0, 0, 1e-5)], np.float32)
-    cam = (_f32x3 := ctypes.c_float * 3)(0.0, 0.0, 0.0)
+    f32x3 = ctypes.c_float * 3
+    cam = f32x3(0.0, 0.0, 0.0)
     out = np.zeros(len(pos), np.uint32)
     L.orc_fill_camera_dist(pos.ctypes.data, cam, out.ctypes.data, len(pos))
     # 1e6 - 100 * |p - cam|, truncated; clamped at 0 far away and for infinite positions;
     # distances under half a float ulp of 1e6 / 100 leave the key at the reset value
     assert list(out) == [999500, 998800, 999700, 1000000, 0, 0, 1000000]
-    cam = _f32x3(1.0, 2.0, 2.0)
+    cam = f32x3(1.0, 2.0, 2.0)
     L.orc_fill_camera_dist(pos.ctypes.data, cam, out.ctypes.data, 4)
     assert out[2] == 1000000 and out[3] == 999700
     reset = np.zeros(1024, np.uint32)
