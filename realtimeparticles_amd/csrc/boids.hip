@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
     }
   };
   if (DIM2) {
-    // boids.cl:166-179: the YZ neighbours in the x layer RX/2, every coordinate checked
+    // boids.cl:170-182: the YZ neighbours in the x layer RX/2, every coordinate checked
     // against RX and the index formed as (RX/2 * RX + y) * RY + z.  On the cubic grids the
     // reference runs in 2-D these are the grid's own bounds and index; an index past the
     // table (a non-cubic grid, where the reference reads out of bounds) is skipped.
