@@ -217,6 +217,59 @@ def run_render(args, sc, tuning, physics_ms):
     return out
 
 
+def run_boids(args):
+    """Continuing boids steps from a fixed input: the 8,192-boid flock (box 2, grid 20^3,
+    dt 0.01) and the reference app's largest case, 131,072 boids in a 10^3 box on a 30^3
+    grid with the reference's default rules (Boids.cpp:44-69) and dt 0.1 (Boids.cpp:334).
+    Every launch shape (block size x XCD map) runs the same steps from the same input; the
+    fastest is reported beside the default one."""
+    from realtimeparticles_amd import capi, scenes
+    cases = (("flock-8k", scenes.boids_flock(8192), None),
+             ("app-131k", scenes.boids_flock(131072, seed=3, box=10.0, res=30),
+              (0.5, 1.6, 1.6, 1.45, 0.1)))
+    out = {}
+    for name, sc, prm in cases:
+        m = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+        if prm is None:
+            m.set_params(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
+            m.set_time_step(sc.time_step)
+        else:
+            m.set_params(*prm[:4])
+            m.set_time_step(prm[4])
+
+        def timed(block, flags):
+            m.set_tuning(block, flags)
+            m.load_state(sc.pos, sc.vel)
+            m.update(args.warmup)
+            m.sync()
+            t0 = time.perf_counter()
+            m.update(args.steps)
+            m.sync()
+            return (time.perf_counter() - t0) * 1e3 / args.steps
+
+        shapes = {(b, f): timed(b, f) for b in (64, 128, 192, 256)
+                  for f in (0, capi.LAUNCH_XCD_MAP)}
+        (bb, bf), best_ms = min(shapes.items(), key=lambda kv: kv[1])
+        # per-kernel times of the same steps at the fastest shape
+        m.set_tuning(bb, bf)
+        m.load_state(sc.pos, sc.vel)
+        m.update(args.warmup)
+        m.enable_profiling(True)
+        m.update(args.steps)
+        m.sync()
+        prof = m.profile()
+        m.close()
+        out[name] = {"boids": sc.n, "box": sc.box, "grid": sc.res,
+                     "steps": args.steps, "after_warmup_steps": args.warmup,
+                     "boid_updates_per_s": sc.n / (best_ms * 1e-3),
+                     "ms_per_step": round(best_ms, 5),
+                     "launch": {"block": bb, "flags": bf},
+                     "ms_per_step_default_launch": round(shapes[(256, 0)], 5),
+                     "kernel_ms_per_step": {kk: round(v[1] / args.steps, 5)
+                                            for kk, v in sorted(prof.items())}}
+    return out
+
+
 def run_cpu_baseline(sc, budget_s: float, max_steps: int):
     from oracle import oracle as O
     O.build()
@@ -411,6 +464,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-boids", action="store_true", help="skip the boids steps")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the substep with the reference's rendering steps on")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -471,6 +525,10 @@ def main():
                       "sort_bits": args.sort_bits or 0, "queue": args.queue or 0}
         render = run_render(args, sc, tuning, elapsed * 1e3 / args.steps)
 
+    boids = None
+    if not args.no_boids:
+        boids = run_boids(args)
+
     learned_cmp = None
     if not args.no_learned:
         learned_cmp = run_learned(args, sc)
@@ -513,6 +571,7 @@ def main():
                 "density_error_at_end": _err(g["continuing_density_error"])},
             "stable_variant": stable,
             "with_rendering_steps": render,
+            "boids": boids,
             "learned_vs_plain": learned_cmp,
             "batched_scenes": batched,
         },

@@ -315,6 +315,9 @@ int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4);
  * bd_applyBoidsRulesWithGrid2D (boids.cl:137-221: the nine YZ cells of the x layer
  * GRID_RES_X / 2, Boids.cpp:346-349), 3 (the default) the 3-D rules. */
 int rtp_boids_set_dimension(rtp_boids* h, int dim);
+/* Launch shape of the boids step (no reference counterpart): the block size and, of the
+ * flags, RTP_LAUNCH_XCD_MAP.  Results do not depend on it.  Default 256 threads. */
+int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t);
 /* Model::setBoundary (physics/Model.hpp:116-119, Boundary :38-44): the wall of bd_updatePosAndApplyWallBC
  * (bouncing, the default) or of bd_updatePosAndApplyPeriodicBC (cyclic, boids.cl:289-315),
  * Boids.cpp:363-372. */

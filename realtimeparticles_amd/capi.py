@@ -196,6 +196,7 @@ _SIGS = {
     "rtp_boids_read_field": ([_vp, _int, _vp, _u32], _int),
     "rtp_boids_set_render": ([_vp, _int], _int),
     "rtp_boids_set_dimension": ([_vp, _int], _int),
+    "rtp_boids_set_tuning": ([_vp, ctypes.POINTER(Tuning)], _int),
     "rtp_boids_set_boundary": ([_vp, _int], _int),
     "rtp_boids_set_target": ([_vp, _int, ctypes.POINTER(_f32), _f32, _int], _int),
     "rtp_boids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
@@ -671,6 +672,11 @@ class Boids(_RenderFields):
     def set_nb_particles(self, n: int) -> None:
         _check(lib().rtp_boids_set_nb_particles(self._h, int(n)), "set_nb_particles")
         self.n = int(n)
+
+    def set_tuning(self, block: int = 256, flags: int = 0) -> None:
+        """Block size of the boids step; flags: LAUNCH_XCD_MAP or 0."""
+        t = _tuning(block, flags, 0, 0)
+        _check(lib().rtp_boids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def set_dimension(self, dim: int) -> None:
         """2: the 2-D rules (boids.cl:137-221); 3: the 3-D rules."""

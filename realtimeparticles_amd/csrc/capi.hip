@@ -1421,6 +1421,17 @@ int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
   return RTP_OK;
 }
 
+int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t)
+{
+  RTP_REQUIRE(h && t, "NULL argument");
+  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP);
+  if (rc)
+    return rc;
+  h->L.block = t->blockSize;
+  h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
+  return RTP_OK;
+}
+
 int rtp_boids_set_dimension(rtp_boids* h, int dim)
 {
   RTP_REQUIRE(h, "NULL handle");

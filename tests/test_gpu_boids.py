@@ -136,3 +136,17 @@ def test_boids_variants_bit_exact(variant):
     if "cyclic" in variant:
         assert wrapped > 0
     gpu.close()
+
+
+@pytest.mark.parametrize("block,flags", [(64, capi.LAUNCH_XCD_MAP), (128, 0), (192, capi.LAUNCH_XCD_MAP)])
+def test_boids_launch_shapes_keep_bits(block, flags):
+    sc = scenes.boids_flock(8192, seed=31)
+    gpu, orc = make_pair(sc)
+    gpu.set_tuning(block, flags)
+    for step in range(1, 41):
+        gpu.update(1)
+        orc.step(1)
+    pos, vel = gpu.read_state(sc.n)
+    _eq(pos[:, :3], orc.pos[:sc.n, :3], f"pos, block {block} flags {flags}")
+    _eq(vel[:, :3], orc.vel[:sc.n, :3], f"vel, block {block} flags {flags}")
+    gpu.close()
