@@ -217,6 +217,35 @@ def run_render(args, sc, tuning, physics_ms):
     return out
 
 
+def run_dam_65k(args):
+    """The 65,536-particle dam break (64x32x32 lattice, box 4x2x2, grid 40^3/2) run as
+    specified: 300 frames of two substeps from the input, graph replay, autotuned launch
+    shape, the dam-break coefficients."""
+    from realtimeparticles_amd import capi, scenes
+    sc = scenes.pbf_scene("pbf-65k")
+    m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(**sc.params())
+    m.set_nb_jacobi_iters(sc.iters)
+    m.load_state(sc.pos, sc.vel)
+    m.save_checkpoint()
+    tuning = m.autotune(trial_steps=3)
+    runs = []
+    for _ in range(3):
+        m.load_state(sc.pos, sc.vel)
+        m.sync()
+        t0 = time.perf_counter()
+        m.update(600)
+        m.sync()
+        runs.append(time.perf_counter() - t0)
+    err = m.density_error()
+    m.close()
+    el = min(runs)
+    return {"particles": sc.n, "frames": 300, "substeps": 600, "launch_tuning": tuning,
+            "seconds": round(el, 5), "runs_seconds": [round(r, 5) for r in runs],
+            "particle_updates_per_s": sc.n * 600 / el, "frames_per_s": 300 / el,
+            "ms_per_substep": round(el * 1e3 / 600, 5), "density_error_at_end": _err(err)}
+
+
 def run_boids(args):
     """Continuing boids steps from a fixed input: the 8,192-boid flock (box 2, grid 20^3,
     dt 0.01) and the reference app's largest case, 131,072 boids in a 10^3 box on a 30^3
@@ -465,6 +494,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-boids", action="store_true", help="skip the boids steps")
+    ap.add_argument("--no-65k", action="store_true", help="skip the 65,536-particle 300-frame run")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the substep with the reference's rendering steps on")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -529,6 +559,10 @@ def main():
     if not args.no_boids:
         boids = run_boids(args)
 
+    dam_65k = None
+    if not args.no_65k:
+        dam_65k = run_dam_65k(args)
+
     learned_cmp = None
     if not args.no_learned:
         learned_cmp = run_learned(args, sc)
@@ -572,6 +606,7 @@ def main():
             "stable_variant": stable,
             "with_rendering_steps": render,
             "boids": boids,
+            "pbf_65k_300_frames": dam_65k,
             "learned_vs_plain": learned_cmp,
             "batched_scenes": batched,
         },

@@ -7,6 +7,7 @@
 // of one step are fused into one pass that reads the sorted (position, velocity) buffers and
 // writes the next state into the other pair of buffers, so no boid reads a neighbour's
 // already-updated state.
+#include "rtp_crmath.hpp"
 #include "rtp_device.hpp"
 
 namespace rtp {
@@ -39,9 +40,23 @@ __device__ __forceinline__ V3 normalize3(V3 v)
   return V3{ v.x / l, v.y / l, v.z / l };
 }
 
+// fast_normalize(velocity[e]) of boids.cl:105, once per boid: the rules read a neighbour's
+// unit velocity from here instead of normalising it again for every pair.
+__global__ __launch_bounds__(256) void k_boids_unit_vel(const float4* __restrict__ velocity,
+                                                        float4* __restrict__ unitVel, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 v = velocity[i];
+  const V3 u = normalize3(V3{ v.x, v.y, v.z });
+  unitVel[i] = make_float4(u.x, u.y, u.z, 0.0f);
+}
+
 template <int DIM2>
 __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
                                                     const float4* __restrict__ velocity,
+                                                    const float4* __restrict__ unitVel,
                                                     const uint2* __restrict__ se,
                                                     float4* __restrict__ posOut,
                                                     float4* __restrict__ velOut,
@@ -67,14 +82,27 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
         avgPos.x = avgPos.x + q.x;
         avgPos.y = avgPos.y + q.y;
         avgPos.z = avgPos.z + q.z;
-        const float4 ve = velocity[e];
-        const V3 nv = normalize3(V3{ ve.x, ve.y, ve.z });
+        const float4 nv = unitVel[e];
         avgVel.x = avgVel.x + nv.x;
         avgVel.y = avgVel.y + nv.y;
         avgVel.z = avgVel.z + nv.z;
-        rep.x = rep.x + vx / sq;
-        rep.y = rep.y + vy / sq;
-        rep.z = rep.z + vz / sq;
+        // vec / squaredDist: sq lies in (1e-8, radiusSq] with radiusSq <= 2^30 (checked on
+        // the host), so the short division applies unless a numerator is a nonzero value
+        // under 2^-90
+        const float y = rcp_refined(sq);
+        float qx = div_short(vx, sq, y), qy = div_short(vy, sq, y), qz = div_short(vz, sq, y);
+        const bool general = !c.shortDiv || tiny_numerator3(vx, vy, vz);
+        if (__builtin_amdgcn_ballot_w64(general) != 0) {
+          rare_path();
+          if (general) {
+            qx = vx / sq;
+            qy = vy / sq;
+            qz = vz / sq;
+          }
+        }
+        rep.x = rep.x + qx;
+        rep.y = rep.y + qy;
+        rep.z = rep.z + qz;
         ++count;
       }
     }
@@ -179,18 +207,23 @@ void launch_boids_hash(const float4* pos, uint32_t* key, uint2* startEnd, uint32
                      n, g);
 }
 
-void launch_boids_step(const float4* pos, const float4* vel, const uint2* startEnd,
-                       float4* posOut, float4* velOut, float4* acc, uint32_t n,
-                       const BoidsConsts& c, const Launch& L, hipStream_t s)
+void launch_boids_step(const float4* pos, const float4* vel, float4* unitVel,
+                       const uint2* startEnd, float4* posOut, float4* velOut, float4* acc,
+                       uint32_t n, uint32_t sortN, const BoidsConsts& c, const Launch& L,
+                       hipStream_t s)
 {
   if (n == 0)
     return;
+  // every sorted slot a cell range can reach: stale keys of a shrunk count sort among the
+  // active ones
+  hipLaunchKernelGGL(k_boids_unit_vel, dim3(blocks_for(sortN, 256)), dim3(256), 0, s, vel,
+                     unitVel, sortN);
   if (c.dim2)
     hipLaunchKernelGGL(k_boids_step<1>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
-                       vel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+                       vel, unitVel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
   else
     hipLaunchKernelGGL(k_boids_step<0>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
-                       vel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+                       vel, unitVel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
 }
 
 }  // namespace rtp

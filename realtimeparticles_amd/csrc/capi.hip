@@ -650,6 +650,7 @@ struct rtp_boids {
   float4* vel[2] = { nullptr, nullptr };
   float4* acc = nullptr;     // p_acc
   float4* accAlt = nullptr;  // its gather partner, with the rendering steps on
+  float4* unitVel = nullptr; // the sorted velocities normalised, read by the rules
   int cur = 0;
   Launch L;
   Profiler prof;
@@ -690,8 +691,8 @@ struct rtp_boids {
     }
     // Sorted state now lives in [1 - cur]; the fused step writes the next state to [cur].
     RTP_PROF(prof, "boids_step", s,
-             launch_boids_step(pos[1 - cur], vel[1 - cur], G.startEnd, pos[cur], vel[cur], accOut,
-                               n, c, L, s));
+             launch_boids_step(pos[1 - cur], vel[1 - cur], unitVel, G.startEnd, pos[cur], vel[cur],
+                               accOut, n, G.sortedN, c, L, s));
     if (render.on)
       std::swap(acc, accAlt);
     // The tail [n, sortedN) was only permuted; carry it over unchanged.
@@ -748,6 +749,7 @@ struct rtp_boids {
     }
     dev_free(acc);
     dev_free(accAlt);
+    dev_free(unitVel);
     render.release();
     if (stream)
       (void)hipStreamDestroy(stream);
@@ -1262,6 +1264,7 @@ int rtp_boids_create(const rtp_model_desc* d, rtp_boids** out)
                                   (float)(d->gridRes[0] * d->gridRes[0]));
   h->c.p = rtp_boids_params{ 0.5f, 1.6f, 1.6f, 1.45f };  // Boids.cpp:44-69 defaults
   h->c.dt = 0.1f;                                        // Boids.cpp:334
+  h->c.shortDiv = h->c.radiusSq <= 0x1p30f ? 1 : 0;
   h->c.vmin = 0.2f * h->c.p.velocityScale;
   h->n = d->currNbParticles;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1271,7 +1274,8 @@ int rtp_boids_create(const rtp_model_desc* d, rtp_boids** out)
   const uint32_t maxN = d->maxNbParticles;
   if ((rc = h->grid.alloc(maxN)) || (rc = dev_alloc(&h->pos[0], maxN)) ||
       (rc = dev_alloc(&h->pos[1], maxN)) || (rc = dev_alloc(&h->vel[0], maxN)) ||
-      (rc = dev_alloc(&h->vel[1], maxN)) || (rc = dev_alloc(&h->acc, maxN))) {
+      (rc = dev_alloc(&h->vel[1], maxN)) || (rc = dev_alloc(&h->acc, maxN)) ||
+      (rc = dev_alloc(&h->unitVel, maxN))) {
     h->release();
     delete h;
     return rc;

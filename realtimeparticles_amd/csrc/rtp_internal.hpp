@@ -75,6 +75,7 @@ struct BoidsConsts {
   float target[3] = { 0.0f, 0.0f, 0.0f };
   float targetRadius = 0.0f;
   float targetSign = 0.0f;  // TargetParams.targetSignEffect as the float the rule multiplies by
+  int32_t shortDiv = 0;     // radiusSq <= 2^30: the pair divisions take the short form
 };
 
 // Survivor-queue depth of the neighbour passes when a tuning gives 0.
@@ -279,8 +280,9 @@ void launch_density_error(const float* density, uint32_t n, float rho0, float* m
 // -------------------------------------------------------------------------------- boids
 void launch_boids_hash(const float4* pos, uint32_t* key, uint2* startEnd, uint32_t n,
                        const GridConsts& g, hipStream_t s);
-void launch_boids_step(const float4* pos, const float4* vel, const uint2* startEnd,
-                       float4* posOut, float4* velOut, float4* acc, uint32_t n,
+// The rules read unit velocities from unitVel, written first over the sorted range sortN.
+void launch_boids_step(const float4* pos, const float4* vel, float4* unitVel, const uint2* startEnd,
+                       float4* posOut, float4* velOut, float4* acc, uint32_t n, uint32_t sortN,
                        const BoidsConsts& c, const Launch& L, hipStream_t s);
 
 }  // namespace rtp
