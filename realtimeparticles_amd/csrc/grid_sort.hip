@@ -343,6 +343,99 @@ __global__ __launch_bounds__(256) void k_reorder_batch(ReorderArgs a, BatchView 
   reorder_body(i, l, sc.kbase, o);
 }
 
+// Stable LSD sort of up to kSmallSortMax (key, index) pairs by one workgroup of
+// kSmallThreads, 4-bit digits, all passes in one launch and in LDS: the pairs are read
+// once (coalesced), ping-pong between two LDS buffers, and the result is written once to
+// the buffer the multi-launch sort would leave it in (B after an odd number of passes).
+// Thread t owns the contiguous chunk [t*CH, (t+1)*CH) of a pass's input.  Its digit counts
+// sit in LDS column t of a digit-major table whose exclusive scan gives, for each
+// (digit, thread), the number of keys of smaller digits plus those of the same digit in
+// earlier chunks; the thread then scatters its chunk in order, so equal keys keep their
+// input order.
+constexpr uint32_t kSmallThreads = 512;
+
+template <int CH>
+__global__ __launch_bounds__(512) void k_sort_small(uint32_t* __restrict__ keyA,
+                                                    uint32_t* __restrict__ keyB,
+                                                    uint32_t* __restrict__ idxA,
+                                                    uint32_t* __restrict__ idxB, uint32_t n,
+                                                    int passes)
+{
+  __shared__ uint32_t sk[2][kSmallSortMax];
+  __shared__ uint32_t si[2][kSmallSortMax];
+  __shared__ uint16_t cnt[16 * kSmallThreads];
+  __shared__ uint32_t wsum[kSmallThreads / 64];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < n; i += kSmallThreads) {
+    sk[0][i] = keyA[i];
+    si[0][i] = i;
+  }
+  __syncthreads();
+  const uint32_t lo = t * CH;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int in = pass & 1;
+    const int shift = 4 * pass;
+    uint32_t k[CH], ix[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const uint32_t i = lo + (uint32_t)j;
+      k[j] = i < n ? sk[in][i] : 0u;
+      ix[j] = i < n ? si[in][i] : 0u;
+    }
+#pragma unroll
+    for (int d = 0; d < 16; ++d)
+      cnt[d * kSmallThreads + t] = 0;
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (lo + (uint32_t)j < n)
+        cnt[((k[j] >> shift) & 15u) * kSmallThreads + t] += 1;
+    __syncthreads();
+    // exclusive scan of the table in linear order: thread t takes entries [16t, 16t + 16)
+    uint32_t loc[16];
+    uint32_t run = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      loc[e] = run;
+      run += cnt[16 * t + e];
+    }
+    uint32_t incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if ((t & 63u) >= (uint32_t)o)
+        incl += v;
+    }
+    if ((t & 63u) == 63u)
+      wsum[t >> 6] = incl;
+    __syncthreads();
+    uint32_t base = incl - run;
+    for (uint32_t w = 0; w < (t >> 6); ++w)
+      base += wsum[w];
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      cnt[16 * t + e] = (uint16_t)(base + loc[e]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      if (lo + (uint32_t)j < n) {
+        const uint32_t slot = ((k[j] >> shift) & 15u) * kSmallThreads + t;
+        const uint32_t pos = cnt[slot];
+        cnt[slot] = (uint16_t)(pos + 1u);
+        sk[1 - in][pos] = k[j];
+        si[1 - in][pos] = ix[j];
+      }
+    }
+    __syncthreads();
+  }
+  const int res = passes & 1;
+  uint32_t* kout = res ? keyB : keyA;
+  uint32_t* iout = res ? idxB : idxA;
+  for (uint32_t i = t; i < n; i += kSmallThreads) {
+    kout[i] = sk[res][i];
+    iout[i] = si[res][i];
+  }
+}
+
 template <int BITS>
 int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
              uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
@@ -455,7 +548,26 @@ SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
     p.bits = 4;
   p.subtiles = (n + (uint32_t)(kSortItems * 64) - 1u) / (uint32_t)(kSortItems * 64);
   p.countWords = ((size_t)1u << p.bits) * p.subtiles;
+  p.keyBits = keyBits;
   return p;
+}
+
+// Whether the current device gives one workgroup the LDS k_sort_small declares (checked
+// once per device; otherwise small sorts take the multi-launch path).
+bool small_sort_fits()
+{
+  static int ok[64] = {};  // 0 unknown, 1 yes, 2 no
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return false;
+  if (ok[dev] == 0) {
+    hipFuncAttributes a{};
+    int limit = 0;
+    const bool q = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_sort_small<16>)) == hipSuccess &&
+                   hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess;
+    ok[dev] = (q && a.sharedSizeBytes <= (size_t)limit) ? 1 : 2;
+  }
+  return ok[dev] == 1;
 }
 
 int sort_pairs(const SortPlan& p, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA, uint32_t* idxB,
@@ -464,6 +576,17 @@ int sort_pairs(const SortPlan& p, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA
   *resultInB = false;
   if (p.n == 0)
     return RTP_OK;
+  if (p.n <= kSmallSortMax && small_sort_fits()) {  // one workgroup, all passes in LDS
+    const int passes = (p.keyBits + 3) / 4;
+    if (p.n <= 4096u)
+      hipLaunchKernelGGL(k_sort_small<8>, dim3(1), dim3(kSmallThreads), 0, s, keyA, keyB, idxA, idxB,
+                         p.n, passes);
+    else
+      hipLaunchKernelGGL(k_sort_small<16>, dim3(1), dim3(kSmallThreads), 0, s, keyA, keyB, idxA,
+                         idxB, p.n, passes);
+    *resultInB = (passes & 1) != 0;
+    return RTP_OK;
+  }
   uint32_t *kin = keyA, *kout = keyB, *iin = idxA, *iout = idxB;
   bool inB = false;
   for (int pass = 0; pass < p.passes; ++pass) {

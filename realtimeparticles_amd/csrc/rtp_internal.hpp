@@ -101,7 +101,10 @@ struct SortPlan {
   int bits = 0;            // digit width
   uint32_t subtiles = 0;   // wave-owned subtiles of kSortItems*64 keys
   size_t countWords = 0;   // radix * subtiles
+  int keyBits = 0;         // significant bits of the largest key
 };
+// Sorts of at most this many keys run in one workgroup, in LDS (all passes in one launch).
+constexpr uint32_t kSmallSortMax = 8192;
 constexpr int kSortItems = 32;  // keys per lane in one subtile
 // Words of the counts buffer a sort of up to maxN keys needs (any digit width): the
 // digit-major count table (at most 2048 digits x subtiles) followed by its scan's segment sums.
