@@ -159,3 +159,23 @@ def test_boids_render_frames_bit_exact():
         orc.step(1)
     compare_boids(gpu, orc, "frame 37 (n = 6000)")
     gpu.close()
+
+
+def test_fluids_render_full_and_empty():
+    """Every slot active (n = maxN: no FAR_DIST tail) and no particle at all."""
+    sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=14, extra=0)
+    gpu, orc = make_pbf_pair(sc, iters=2, **sc.params("stable"))
+    cam = (3.0, 2.0, 1.0)
+    gpu.set_render(True)
+    gpu.set_camera(cam)
+    orc.enable_render(cam)
+    for f in range(1, 4):
+        gpu.update(1)
+        orc.step(1)
+        compare_fluids(gpu, orc, f"full frame {f}")
+    gpu.set_nb_particles(0)
+    orc.s.n = 0
+    gpu.update(2)
+    orc.step(2)
+    compare_fluids(gpu, orc, "empty frames")
+    gpu.close()
