@@ -127,3 +127,21 @@ def oracle_teacher(sc, prm, iters, return_grid=False):
     if return_grid:
         return feat, acc, (f.key.copy(), f.idx.copy(), f.se.copy())
     return feat, acc
+
+
+def compare_fluids_render(gpu, orc, what):
+    """Every buffer of a fluids model with the rendering steps on, over maxN."""
+    m = orc.max_n
+    pos, vel = gpu.read_state(m)
+    _eq(pos, orc.pos, f"{what}: p_pos")
+    _eq(vel, orc.vel, f"{what}: p_vel")
+    _eq(gpu.read_field(capi.FIELD_PRED_POS, m), orc.pred, f"{what}: p_predPos")
+    _eq(gpu.read_color(), orc.render.col, f"{what}: p_col")
+    _eq(gpu.read_camera_dist(), orc.render.dist, f"{what}: p_cameraDist")
+    keys, idx, se = gpu.read_grid()
+    _eq(keys, orc.key, f"{what}: p_cellID")
+    _eq(idx, orc.idx, f"{what}: RadixSortIndices")
+    _eq(se, orc.se, f"{what}: c_startEndPartID")
+    _eq(gpu.read_detector(), orc.render.det, f"{what}: c_partDetector")
+    n = gpu.n
+    _eq(gpu.read_field(capi.FIELD_DENSITY, n), orc.density[:n], f"{what}: p_density")

@@ -11,7 +11,7 @@
 import numpy as np
 import pytest
 
-from helpers import _eq, compare_pbf, log, make_pbf_pair, oracle_teacher
+from helpers import _eq, compare_fluids_render, compare_pbf, log, make_pbf_pair, oracle_teacher
 from realtimeparticles_amd import capi, scenes
 
 pytestmark = pytest.mark.gpu
@@ -99,4 +99,44 @@ def test_pbf_4m_replay_and_graph_determinism(scene_4m, mapping, substeps):
         # ones move particles further, as the reference's unclamped correction does
         assert over <= 0.1
     log(f"pbf-4m ({mapping}) density error after {substeps} substeps: {a_err}")
+    gpu.close()
+
+
+def test_pbf_1m_rendering_steps_bit_exact():
+    """The bench workload with the rendering steps on: two frames against the oracle's whole
+    frame, every buffer over maxN (the camera sort then permutes 1,049,088 entries)."""
+    sc = scenes.pbf_scene("pbf-1m")
+    gpu, orc = make_pbf_pair(sc, iters=sc.iters)
+    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP | capi.LAUNCH_NB_LIST, 0, 16)
+    cam = (0.0, 4.0, 12.0)
+    gpu.set_render(True)
+    gpu.set_camera(cam)
+    orc.enable_render(cam)
+    for f in (1, 2):
+        log(f"pbf-1m with rendering steps, frame {f}")
+        gpu.update(1)
+        orc.step(1)
+        compare_fluids_render(gpu, orc, f"1M frame {f}")
+    gpu.close()
+
+
+def test_boids_131k_app_case_bit_exact():
+    """The reference app's largest boids case (131,072 boids, 10^3 box, 30^3 grid, default
+    rules, dt 0.1) over 10 steps at the bench's tuned launch shape."""
+    from oracle import oracle as O
+    sc = scenes.boids_flock(131072, seed=3, box=10.0, res=30)
+    gpu = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+    gpu.set_tuning(64, capi.LAUNCH_XCD_MAP)
+    gpu.load_state(sc.pos, sc.vel)
+    orc = O.OracleBoids(sc.box, sc.res, sc.max_n, sc.n, O.BoidsParams(0.5, 1.6, 1.6, 1.45),
+                        time_step=0.1)
+    orc.upload(sc.pos, sc.vel)
+    for step in range(1, 11):
+        gpu.update(1)
+        orc.step(1)
+    log("boids 131K: 10 steps")
+    pos, vel = gpu.read_state(sc.n)
+    _eq(pos[:, :3], orc.pos[:sc.n, :3], "131K boids pos")
+    _eq(vel[:, :3], orc.vel[:sc.n, :3], "131K boids vel")
+    _eq(gpu.read_acc(sc.n)[:, :3], orc.acc[:sc.n, :3], "131K boids acc")
     gpu.close()

@@ -13,28 +13,11 @@ hipGraph.
 import numpy as np
 import pytest
 
-from helpers import _eq, log, make_pbf_pair
+from helpers import _eq, compare_fluids_render as compare_fluids, log, make_pbf_pair
 from oracle import oracle as O
 from realtimeparticles_amd import capi, scenes
 
 pytestmark = pytest.mark.gpu
-
-
-def compare_fluids(gpu, orc, what):
-    m = orc.max_n
-    pos, vel = gpu.read_state(m)
-    _eq(pos, orc.pos, f"{what}: p_pos")
-    _eq(vel, orc.vel, f"{what}: p_vel")
-    _eq(gpu.read_field(capi.FIELD_PRED_POS, m), orc.pred, f"{what}: p_predPos")
-    _eq(gpu.read_color(), orc.render.col, f"{what}: p_col")
-    _eq(gpu.read_camera_dist(), orc.render.dist, f"{what}: p_cameraDist")
-    keys, idx, se = gpu.read_grid()
-    _eq(keys, orc.key, f"{what}: p_cellID")
-    _eq(idx, orc.idx, f"{what}: RadixSortIndices")
-    _eq(se, orc.se, f"{what}: c_startEndPartID")
-    _eq(gpu.read_detector(), orc.render.det, f"{what}: c_partDetector")
-    n = gpu.n
-    _eq(gpu.read_field(capi.FIELD_DENSITY, n), orc.density[:n], f"{what}: p_density")
 
 
 @pytest.mark.parametrize("flags", [0, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP | capi.LAUNCH_NB_LIST])
