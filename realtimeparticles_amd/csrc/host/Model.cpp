@@ -415,6 +415,13 @@ void Boids::setBoidsParams(const rtp_boids_params& p) { check(rtp_boids_set_para
 
 void Boids::setTimeStep(float dt) { check(rtp_boids_set_time_step(m_h, dt), "rtp_boids_set_time_step"); }
 
+void Boids::setTarget(bool active, std::array<float, 3> pos, float radius, int sign)
+{
+  check(rtp_boids_set_target(m_h, active ? 1 : 0, pos.data(), radius, sign), "rtp_boids_set_target");
+  m_targetActive = active;
+  m_targetPos = pos;
+}
+
 void Boids::loadState(const float* pos4, const float* vel4, size_t count)
 {
   check(rtp_boids_load_state(m_h, pos4, vel4, (uint32_t)count), "rtp_boids_load_state");

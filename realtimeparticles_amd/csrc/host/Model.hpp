@@ -197,6 +197,13 @@ class Boids : public Model {
 
   void setBoidsParams(const rtp_boids_params& p);
   void setTimeStep(float dt);
+  // The target rule (Boids.cpp:351-358, TargetParams Boids.hpp:22-26): active or not, where
+  // the target is for the next updates, its radius of effect and sign (1 attracts, -1
+  // repels).  The reference moves the target itself each frame (Target::updatePos); here
+  // the caller does, by calling this again.
+  void setTarget(bool active, std::array<float, 3> pos, float radius = 2.0f, int sign = 1);
+  std::array<float, 3> targetPos() const override { return m_targetPos; }
+  bool isTargetActivated() const override { return m_targetActive; }
 
   void loadState(const float* pos4, const float* vel4, size_t count) override;
   void readState(float* pos4, float* vel4, size_t count) const override;
@@ -213,6 +220,8 @@ class Boids : public Model {
 
  private:
   rtp_boids* m_h = nullptr;
+  bool m_targetActive = false;
+  std::array<float, 3> m_targetPos{ 0.0f, 0.0f, 0.0f };
 };
 
 }  // namespace HIP

@@ -2,10 +2,11 @@
 // (app/ParticleSystemApp.cpp:275-315 CreateModel, :377 update) and dumps the state before and
 // after `steps` updates so a test can replay the same steps on the CPU oracle.
 //
-//   model_demo fluids|boids|fluids2d|boids2d|boidscyclic <out.bin> <steps>
+//   model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget <out.bin> <steps>
 //
 // The 2d kinds switch the model to 2-D (Model::setDimension, which resets it); boidscyclic
-// selects the cyclic wall (Model::setBoundary).
+// selects the cyclic wall (Model::setBoundary); boidstarget turns the target rule on with a
+// target at (1, 0.5, 0), radius 3, attracting.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,12 +19,13 @@ static void put(FILE* f, const void* p, size_t n) { std::fwrite(p, 1, n, f); }
 int main(int argc, char** argv)
 {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic out.bin steps\n");
+    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget out.bin steps\n");
     return 2;
   }
   const bool fluids = std::strncmp(argv[1], "fluids", 6) == 0;
   const bool dim2 = std::strstr(argv[1], "2d") != nullptr;
   const bool cyclic = std::strcmp(argv[1], "boidscyclic") == 0;
+  const bool target = std::strcmp(argv[1], "boidstarget") == 0;
   const int steps = std::atoi(argv[3]);
   Physics::ModelParams p;
   p.maxNbParticles = Utils::P130K;            // app: largest entry of ALL_NB_PARTICLES
@@ -37,6 +39,14 @@ int main(int argc, char** argv)
       model->setDimension(Geometry::Dimension::dim2D);  // app/ParticleSystemApp.cpp:428-431
     if (cyclic)
       model->setBoundary(Physics::Boundary::CyclicWall);
+    if (target) {
+      auto* boids = dynamic_cast<Physics::HIP::Boids*>(model.get());
+      if (!boids)
+        return 4;
+      boids->setTarget(true, { 1.0f, 0.5f, 0.0f }, 3.0f, 1);
+      if (!model->isTargetActivated() || model->targetPos()[1] != 0.5f)
+        return 5;
+    }
     const size_t n = model->nbParticles();
     std::vector<float> pos0(4 * n), vel0(4 * n), pos1(4 * n), vel1(4 * n);
     model->readState(pos0.data(), vel0.data(), n);

@@ -68,7 +68,7 @@ def test_cpp_2d_fluids_dam_matches_oracle(tmp_path):
     assert not pos1[:, 0].any()
 
 
-@pytest.mark.parametrize("kind", ["boids2d", "boidscyclic"])
+@pytest.mark.parametrize("kind", ["boids2d", "boidscyclic", "boidstarget"])
 def test_cpp_boids_variants_match_oracle(tmp_path, kind):
     n, max_n, (pos0, vel0, pos1, vel1) = run_demo(kind, tmp_path, 3)
     assert n == 65536
@@ -81,6 +81,8 @@ def test_cpp_boids_variants_match_oracle(tmp_path, kind):
                         O.BoidsParams(0.5, 1.6, 1.6, 1.45), time_step=0.1)
     orc.set_dimension(2 if kind == "boids2d" else 3)
     orc.set_cyclic(kind == "boidscyclic")
+    if kind == "boidstarget":
+        orc.set_target(True, (1.0, 0.5, 0.0), 3.0, 1)
     orc.upload(pos0, vel0)
     orc.step(3)
     _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
