@@ -162,3 +162,34 @@ def test_fluids_render_full_and_empty():
     orc.step(2)
     compare_fluids(gpu, orc, "empty frames")
     gpu.close()
+
+
+def test_render_and_variant_argument_errors():
+    """Bad arguments of the rendering and boids-variant entry points fail with an error and
+    leave the handle usable."""
+    sc = scenes.boids_flock(1024, seed=2)
+    b = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+    b.load_state(sc.pos, sc.vel)
+    with pytest.raises(capi.RtpError, match="enable the rendering steps"):
+        b.set_camera((0.0, 0.0, 0.0))
+    with pytest.raises(capi.RtpError, match="never enabled"):
+        b.read_color()
+    with pytest.raises(capi.RtpError, match="dimension"):
+        b.set_dimension(4)
+    with pytest.raises(capi.RtpError, match="boundary"):
+        capi._check(capi.lib().rtp_boids_set_boundary(b._h, 7), "rtp_boids_set_boundary")
+    with pytest.raises(capi.RtpError, match="target"):
+        capi._check(capi.lib().rtp_boids_set_target(b._h, 1, None, 1.0, 1), "rtp_boids_set_target")
+    with pytest.raises(capi.RtpError, match="flags"):
+        b.set_tuning(64, capi.LAUNCH_RESORT)
+    with pytest.raises(capi.RtpError, match="blockSize"):
+        b.set_tuning(96, 0)
+    b.update(2)  # still steps
+    f = capi.Fluids(4096, 4096, (4, 2, 2), (40, 20, 20))
+    with pytest.raises(capi.RtpError, match="cells"):
+        f.set_render(True)
+        capi._check(capi.lib().rtp_fluids_read_field(f._h, capi.FIELD_DETECTOR,
+                                                     np.zeros(8 * 20000, np.float32).ctypes.data,
+                                                     20000), "read_field")
+    f.close()
+    b.close()
