@@ -117,6 +117,28 @@ typedef enum {
   RTP_FIELD_DETECTOR = 7     /* c_partDetector (float8 per cell; count = cells) */
 } rtp_field;
 
+/* ---- the JSON parameter channel ----
+ * The reference's models keep their parameters in a JSON tree (initFluidsJson,
+ * physics/ocl/Fluids.cpp:52-74; initBoidsJson, physics/ocl/Boids.cpp:41-73) that its
+ * parameter widget edits and updateModelWithInputJson (physics/ocl/OclModel.hpp:48-54)
+ * applies through transferJsonInputsToModel (Fluids.cpp:218-251, Boids.cpp:177-210).  These
+ * read and write that tree; every numeric entry is [value, min, max].  Text out is
+ * NUL-terminated; *len gets the length without the NUL, and a buffer too small fails with
+ * RTP_ERR_INVALID after writing what fits. */
+/* TargetParams (Boids.hpp:22-26) plus the target's on/visible switches (Target.hpp). */
+typedef struct {
+  int32_t active;   /* "Enable##Target" */
+  int32_t visible;  /* "Show" while active */
+  float radius;     /* "Radius" */
+  int32_t sign;     /* "Attract" as an integer: true 1, false 0 */
+} rtp_target_params;
+int rtp_fluid_params_from_json(const char* json, rtp_fluid_params* p, uint32_t* nbJacobiIters);
+int rtp_fluid_params_to_json(const rtp_fluid_params* p, uint32_t nbJacobiIters, char* buf,
+                             size_t cap, size_t* len);
+int rtp_boids_params_from_json(const char* json, rtp_boids_params* p, rtp_target_params* t);
+int rtp_boids_params_to_json(const rtp_boids_params* p, const rtp_target_params* t, char* buf,
+                             size_t cap, size_t* len);
+
 /* ---- library ---- */
 int rtp_abi_version(void);
 /* Text of the last error on this thread ("" if none). */
@@ -191,6 +213,12 @@ int rtp_fluids_set_render(rtp_fluids* h, int enable);
 /* u_cameraPos (Fluids.cpp:132): the point the following updates sort by distance to.
  * The origin until set. */
 int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3]);
+/* Fluids::updateModelWithInputJson (OclModel.hpp:48-54, Fluids.cpp:218-251): the fluid
+ * parameters and the Jacobi iteration count from the "Fluids" tree; FluidParams.dim keeps
+ * the model's dimension. */
+int rtp_fluids_update_with_json(rtp_fluids* h, const char* json);
+/* The model's parameters as that tree (getInputJson's role). */
+int rtp_fluids_params_json(rtp_fluids* h, char* buf, size_t cap, size_t* len);
 /* Launch tuning (no reference counterpart), see rtp_tuning (msPerStep is ignored).  Results
  * do not depend on any of it. */
 int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t);
@@ -336,6 +364,10 @@ int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float rad
 int rtp_boids_set_render(rtp_boids* h, int enable);
 /* u_cameraPos (Boids.cpp:126). */
 int rtp_boids_set_camera(rtp_boids* h, const float pos[3]);
+/* Boids::updateModelWithInputJson (Boids.cpp:177-210): the rules and the target's switch,
+ * radius and sign from the "Boids" tree; the target keeps its position. */
+int rtp_boids_update_with_json(rtp_boids* h, const char* json);
+int rtp_boids_params_json(rtp_boids* h, char* buf, size_t cap, size_t* len);
 int rtp_boids_enable_profiling(rtp_boids* h, int enable);
 int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* count);
 

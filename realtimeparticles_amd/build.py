@@ -24,7 +24,7 @@ OBJ_DIR = os.path.join(OUT_DIR, "obj")
 
 HIP_SOURCES = ["grid_sort.hip", "pbf.hip", "boids.hip", "mlp.hip", "capi.hip", "batch.hip",
                "selfcheck.hip", "render.hip"]
-HOST_SOURCES = ["host/Model.cpp"]
+HOST_SOURCES = ["host/Model.cpp", "host/json_params.cpp"]
 
 ARCH = os.environ.get("RTP_OFFLOAD_ARCH", "gfx950")
 

@@ -108,6 +108,13 @@ class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", _u32), ("totalMs", ctypes.c_double)]
 
 
+class TargetParams(ctypes.Structure):
+    """rtp_target_params: TargetParams (Boids.hpp:22-26) plus the target's switches."""
+
+    _fields_ = [("active", ctypes.c_int32), ("visible", ctypes.c_int32), ("radius", _f32),
+                ("sign", ctypes.c_int32)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("blockSize", _i32), ("flags", _i32), ("sortBits", _i32), ("queueDepth", _i32),
                 ("msPerStep", _f32)]
@@ -158,6 +165,16 @@ _SIGS = {
     "rtp_fluids_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_fluids_read_field": ([_vp, _int, _vp, _u32], _int),
     "rtp_fluids_set_render": ([_vp, _int], _int),
+    "rtp_fluids_update_with_json": ([_vp, ctypes.c_char_p], _int),
+    "rtp_fluids_params_json": ([_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
+    "rtp_boids_update_with_json": ([_vp, ctypes.c_char_p], _int),
+    "rtp_boids_params_json": ([_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
+    "rtp_fluid_params_from_json": ([ctypes.c_char_p, ctypes.POINTER(FluidParams), ctypes.POINTER(_u32)], _int),
+    "rtp_fluid_params_to_json": ([ctypes.POINTER(FluidParams), _u32, ctypes.c_char_p, ctypes.c_size_t,
+                                  ctypes.POINTER(ctypes.c_size_t)], _int),
+    "rtp_boids_params_from_json": ([ctypes.c_char_p, ctypes.POINTER(BoidsParams), ctypes.POINTER(TargetParams)], _int),
+    "rtp_boids_params_to_json": ([ctypes.POINTER(BoidsParams), ctypes.POINTER(TargetParams), ctypes.c_char_p,
+                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
     "rtp_fluids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
     "rtp_fluids_density_error": ([_vp, _vp], _int),
     "rtp_fluids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
@@ -259,6 +276,39 @@ def _check(rc: int, what: str) -> None:
         raise RtpError(f"{what} failed ({rc}): {msg}")
 
 
+def _json_text(fn, *args) -> str:
+    """Call a *_json writer twice: once for the length, once into a buffer of that size."""
+    n = ctypes.c_size_t(0)
+    _check(fn(*args, None, 0, ctypes.byref(n)), fn.__name__)
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(fn(*args, buf, n.value + 1, ctypes.byref(n)), fn.__name__)
+    return buf.value.decode()
+
+
+def fluid_params_from_json(text: str):
+    """The "Fluids" tree (Fluids.cpp:52-74) -> (FluidParams, Jacobi iterations)."""
+    p, it = FluidParams(), _u32(0)
+    _check(lib().rtp_fluid_params_from_json(text.encode(), ctypes.byref(p), ctypes.byref(it)),
+           "rtp_fluid_params_from_json")
+    return p, int(it.value)
+
+
+def fluid_params_to_json(p, iters: int) -> str:
+    return _json_text(lib().rtp_fluid_params_to_json, ctypes.byref(p), int(iters))
+
+
+def boids_params_from_json(text: str):
+    """The "Boids" tree (Boids.cpp:41-73) -> (BoidsParams, TargetParams)."""
+    p, t = BoidsParams(), TargetParams()
+    _check(lib().rtp_boids_params_from_json(text.encode(), ctypes.byref(p), ctypes.byref(t)),
+           "rtp_boids_params_from_json")
+    return p, t
+
+
+def boids_params_to_json(p, t) -> str:
+    return _json_text(lib().rtp_boids_params_to_json, ctypes.byref(p), ctypes.byref(t))
+
+
 def device_count() -> int:
     c = _int(0)
     _check(lib().rtp_device_count(ctypes.byref(c)), "rtp_device_count")
@@ -352,6 +402,13 @@ class Fluids(_RenderFields):
 
     def pause(self, flag: bool) -> None:
         _check(lib().rtp_fluids_pause(self._h, int(bool(flag))), "rtp_fluids_pause")
+
+    def update_with_json(self, text: str) -> None:
+        """updateModelWithInputJson: parameters and iteration count from the "Fluids" tree."""
+        _check(lib().rtp_fluids_update_with_json(self._h, text.encode()), "update_with_json")
+
+    def params_json(self) -> str:
+        return _json_text(lib().rtp_fluids_params_json, self._h)
 
     def sync(self) -> None:
         _check(lib().rtp_fluids_sync(self._h), "rtp_fluids_sync")
@@ -712,6 +769,13 @@ class Boids(_RenderFields):
 
     def pause(self, flag: bool) -> None:
         _check(lib().rtp_boids_pause(self._h, int(bool(flag))), "rtp_boids_pause")
+
+    def update_with_json(self, text: str) -> None:
+        """updateModelWithInputJson: rules and target switches from the "Boids" tree."""
+        _check(lib().rtp_boids_update_with_json(self._h, text.encode()), "update_with_json")
+
+    def params_json(self) -> str:
+        return _json_text(lib().rtp_boids_params_json, self._h)
 
     def sync(self) -> None:
         _check(lib().rtp_boids_sync(self._h), "rtp_boids_sync")

@@ -655,6 +655,7 @@ struct rtp_boids {
   Launch L;
   Profiler prof;
   RenderState render;  // rtp_boids_set_render
+  bool targetVisible = false;  // the "Show" switch of the parameter tree (drawing only)
 
   int step()
   {
@@ -1157,6 +1158,26 @@ int rtp_fluids_set_render(rtp_fluids* h, int enable)
   return RTP_OK;
 }
 
+int rtp_fluids_update_with_json(rtp_fluids* h, const char* json)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  rtp_fluid_params p = h->c.p;
+  uint32_t iters = 0;
+  int rc = rtp_fluid_params_from_json(json, &p, &iters);
+  if (rc)
+    return rc;
+  p.dim = h->c.p.dim;  // Fluids.cpp:234: from the model's dimension, not from the tree
+  if ((rc = rtp_fluids_set_nb_jacobi_iters(h, iters > 1000u ? 1000u : iters)))
+    return rc;
+  return rtp_fluids_set_params(h, &p);
+}
+
+int rtp_fluids_params_json(rtp_fluids* h, char* buf, size_t cap, size_t* len)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return rtp_fluid_params_to_json(&h->c.p, h->iters, buf, cap, len);
+}
+
 int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3])
 {
   RTP_REQUIRE(h && pos, "NULL argument");
@@ -1264,6 +1285,8 @@ int rtp_boids_create(const rtp_model_desc* d, rtp_boids** out)
                                   (float)(d->gridRes[0] * d->gridRes[0]));
   h->c.p = rtp_boids_params{ 0.5f, 1.6f, 1.6f, 1.45f };  // Boids.cpp:44-69 defaults
   h->c.dt = 0.1f;                                        // Boids.cpp:334
+  h->c.targetRadius = 2.0f;  // Target::Target, physics/utils/Target.cpp:11-21
+  h->c.targetSign = 1.0f;
   h->c.shortDiv = h->c.radiusSq <= 0x1p30f ? 1 : 0;
   h->c.vmin = 0.2f * h->c.p.velocityScale;
   h->n = d->currNbParticles;
@@ -1465,6 +1488,34 @@ int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float rad
     h->c.targetSign = (float)sign;
   }
   return RTP_OK;
+}
+
+int rtp_boids_update_with_json(rtp_boids* h, const char* json)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  rtp_boids_params p{};
+  rtp_target_params t{};
+  int rc = rtp_boids_params_from_json(json, &p, &t);
+  if (rc)
+    return rc;
+  if ((rc = rtp_boids_set_params(h, &p)))
+    return rc;
+  h->c.targetOn = t.active ? 1 : 0;
+  h->c.targetRadius = t.radius;
+  h->c.targetSign = (float)t.sign;
+  h->targetVisible = t.visible != 0;
+  return RTP_OK;
+}
+
+int rtp_boids_params_json(rtp_boids* h, char* buf, size_t cap, size_t* len)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  rtp_target_params t{};
+  t.active = h->c.targetOn;
+  t.visible = h->targetVisible ? 1 : 0;
+  t.radius = h->c.targetRadius;
+  t.sign = (int32_t)h->c.targetSign;
+  return rtp_boids_params_to_json(&h->c.p, &t, buf, cap, len);
 }
 
 int rtp_boids_set_render(rtp_boids* h, int enable)

@@ -1,0 +1,63 @@
+"""updateModelWithInputJson on the GPU models (OclModel.hpp:48-54): the parameters a JSON
+tree sets are the ones the next updates run with, and the model reads them back as the same
+tree."""
+import copy
+import json
+
+import numpy as np
+import pytest
+
+from helpers import _eq
+from oracle import oracle as O
+from realtimeparticles_amd import capi, scenes
+from test_json_params import BOIDS, FLUIDS
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fluids_update_with_json():
+    f = copy.deepcopy(FLUIDS)
+    f["Fluids"]["Nb Jacobi Iterations"] = [3, 1, 6]
+    f["Fluids"]["Vorticity Confinement"]["Enable##Vorticity"] = False
+    m = capi.Fluids(4096, 4096, (4, 2, 2), (40, 20, 20))
+    dim = m.params().dim
+    m.update_with_json(json.dumps(f))
+    p = m.params()
+    want, iters = capi.fluid_params_from_json(json.dumps(f))
+    assert iters == 3 and p.dim == dim and p.isVorticityConfEnabled == 0
+    assert all(getattr(p, n) == getattr(want, n) for n, _ in p._fields_ if n != "dim")
+    back = json.loads(m.params_json())
+    assert back["Fluids"]["Nb Jacobi Iterations"][0] == 3
+    assert back["Fluids"]["Vorticity Confinement"]["Enable##Vorticity"] is False
+    with pytest.raises(capi.RtpError, match="missing key"):
+        m.update_with_json('{"Fluids": {}}')
+    m.close()
+
+
+def test_boids_update_with_json_runs_those_rules():
+    sc = scenes.boids_flock(4096, seed=17)
+    b = copy.deepcopy(BOIDS)
+    b["Boids"]["Velocity"] = [1.5, 0.01, 5.0]
+    b["Boids"]["Cohesion"]["Enable##Cohesion"] = False
+    b["Boids"]["Target"].update({"Enable##Target": True, "Radius": [0.9, 1, 20], "Attract": True})
+    gpu = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+    gpu.set_time_step(0.01)
+    gpu.load_state(sc.pos, sc.vel)
+    gpu.set_target(True, (0.2, 0.1, -0.3), 2.0, 1)  # the position the tree keeps
+    gpu.update_with_json(json.dumps(b))
+    rules, tgt = capi.boids_params_from_json(json.dumps(b))
+    orc = O.OracleBoids(sc.box, sc.res, sc.max_n, sc.n,
+                        O.BoidsParams(rules.velocityScale, rules.alignmentScale,
+                                      rules.separationScale, rules.cohesionScale), time_step=0.01)
+    orc.set_target(True, (0.2, 0.1, -0.3), tgt.radius, tgt.sign)
+    orc.upload(sc.pos, sc.vel)
+    gpu.update(20)
+    orc.step(20)
+    pos, vel = gpu.read_state(sc.n)
+    _eq(pos[:, :3], orc.pos[:sc.n, :3], "pos")
+    _eq(vel[:, :3], orc.vel[:sc.n, :3], "vel")
+    back = json.loads(gpu.params_json())
+    assert back["Boids"]["Target"]["Enable##Target"] is True
+    assert back["Boids"]["Target"]["Radius"][0] == np.float32(0.9)
+    assert back["Boids"]["Cohesion"]["Scale##Cohesion"][0] == 0.0
+    gpu.close()
