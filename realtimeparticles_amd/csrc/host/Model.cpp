@@ -317,6 +317,25 @@ void* Fluids::devicePositions() const
   return p;
 }
 
+void Fluids::updateModelWithInputJson(const std::string& json)
+{
+  check(rtp_fluids_update_with_json(m_h, json.c_str()), "rtp_fluids_update_with_json");
+  rtp_fluid_params p{};
+  uint32_t iters = 0;
+  check(rtp_fluid_params_from_json(json.c_str(), &p, &iters), "rtp_fluid_params_from_json");
+  m_nbJacobiIters = iters;
+}
+
+std::string Fluids::inputJson() const
+{
+  size_t n = 0;
+  check(rtp_fluids_params_json(m_h, nullptr, 0, &n), "rtp_fluids_params_json");
+  std::string s(n + 1, '\0');
+  check(rtp_fluids_params_json(m_h, &s[0], s.size(), &n), "rtp_fluids_params_json");
+  s.resize(n);
+  return s;
+}
+
 void Fluids::enableRendering(bool enable)
 {
   check(rtp_fluids_set_render(m_h, enable ? 1 : 0), "rtp_fluids_set_render");
@@ -437,6 +456,25 @@ void* Boids::devicePositions() const
   void* p = nullptr;
   check(rtp_boids_device_state(m_h, &p, nullptr), "rtp_boids_device_state");
   return p;
+}
+
+void Boids::updateModelWithInputJson(const std::string& json)
+{
+  check(rtp_boids_update_with_json(m_h, json.c_str()), "rtp_boids_update_with_json");
+  rtp_boids_params p{};
+  rtp_target_params t{};
+  check(rtp_boids_params_from_json(json.c_str(), &p, &t), "rtp_boids_params_from_json");
+  m_targetActive = t.active != 0;
+}
+
+std::string Boids::inputJson() const
+{
+  size_t n = 0;
+  check(rtp_boids_params_json(m_h, nullptr, 0, &n), "rtp_boids_params_json");
+  std::string s(n + 1, '\0');
+  check(rtp_boids_params_json(m_h, &s[0], s.size(), &n), "rtp_boids_params_json");
+  s.resize(n);
+  return s;
 }
 
 void Boids::enableRendering(bool enable)

@@ -128,6 +128,10 @@ class Model {
   // position they sort by (the reference's u_cameraPos buffer, Model.hpp:69).
   virtual void enableRendering(bool enable) = 0;
   virtual void setCameraPos(float x, float y, float z) = 0;
+  // updateModelWithInputJson / getInputJson (physics/Model.hpp, OclModel.hpp:48-54) with
+  // the JSON tree as text: the "Fluids" or "Boids" tree of the reference's widget.
+  virtual void updateModelWithInputJson(const std::string& json) = 0;
+  virtual std::string inputJson() const = 0;
   // Per-kernel timings collected while profiling was enabled.
   virtual std::vector<rtp_kernel_time> kernelTimes() const = 0;
 
@@ -174,6 +178,8 @@ class Fluids : public Model {
   std::array<float, 3> densityError() const;
   void enableRendering(bool enable) override;
   void setCameraPos(float x, float y, float z) override;
+  void updateModelWithInputJson(const std::string& json) override;
+  std::string inputJson() const override;
   rtp_fluids* handle() const { return m_h; }
 
  protected:
@@ -211,6 +217,8 @@ class Boids : public Model {
   std::vector<rtp_kernel_time> kernelTimes() const override;
   void enableRendering(bool enable) override;
   void setCameraPos(float x, float y, float z) override;
+  void updateModelWithInputJson(const std::string& json) override;
+  std::string inputJson() const override;
   rtp_boids* handle() const { return m_h; }
 
  protected:

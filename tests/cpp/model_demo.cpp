@@ -2,11 +2,13 @@
 // (app/ParticleSystemApp.cpp:275-315 CreateModel, :377 update) and dumps the state before and
 // after `steps` updates so a test can replay the same steps on the CPU oracle.
 //
-//   model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget <out.bin> <steps>
+//   model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget|fluidsjson <out.bin> <steps>
 //
 // The 2d kinds switch the model to 2-D (Model::setDimension, which resets it); boidscyclic
 // selects the cyclic wall (Model::setBoundary); boidstarget turns the target rule on with a
-// target at (1, 0.5, 0), radius 3, attracting.
+// target at (1, 0.5, 0), radius 3, attracting; fluidsjson applies a parameter tree with three
+// Jacobi iterations and vorticity confinement off (Model::updateModelWithInputJson) and
+// checks that inputJson() gives it back.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,13 +21,14 @@ static void put(FILE* f, const void* p, size_t n) { std::fwrite(p, 1, n, f); }
 int main(int argc, char** argv)
 {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget out.bin steps\n");
+    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget|fluidsjson out.bin steps\n");
     return 2;
   }
   const bool fluids = std::strncmp(argv[1], "fluids", 6) == 0;
   const bool dim2 = std::strstr(argv[1], "2d") != nullptr;
   const bool cyclic = std::strcmp(argv[1], "boidscyclic") == 0;
   const bool target = std::strcmp(argv[1], "boidstarget") == 0;
+  const bool withJson = std::strcmp(argv[1], "fluidsjson") == 0;
   const int steps = std::atoi(argv[3]);
   Physics::ModelParams p;
   p.maxNbParticles = Utils::P130K;            // app: largest entry of ALL_NB_PARTICLES
@@ -39,6 +42,20 @@ int main(int argc, char** argv)
       model->setDimension(Geometry::Dimension::dim2D);  // app/ParticleSystemApp.cpp:428-431
     if (cyclic)
       model->setBoundary(Physics::Boundary::CyclicWall);
+    if (withJson) {
+      const std::string tree =
+          "{\"Fluids\": {\"Rest Density\": [450.0, 10.0, 1000.0], \"Relax CFM\": [600.0, 100.0, 1000.0],"
+          " \"Time Step\": [0.01, 0.0001, 0.02], \"Nb Jacobi Iterations\": [3, 1, 6],"
+          " \"Artificial Pressure\": {\"Enable##Pressure\": true, \"Coefficient##Pressure\": [0.001, 0.0, 0.001],"
+          " \"Radius\": [0.006, 0.001, 0.015], \"Exp\": [4, 1, 6]},"
+          " \"Vorticity Confinement\": {\"Enable##Vorticity\": false, \"Coefficient##Vorticity\": [0.0004, 0.0, 0.001],"
+          " \"xSPH Viscosity Coefficient\": [0.0001, 0.0, 0.001]}}}";
+      model->updateModelWithInputJson(tree);
+      const std::string back = model->inputJson();
+      if (back.find("\"Nb Jacobi Iterations\": [3,") == std::string::npos ||
+          back.find("\"Enable##Vorticity\": false") == std::string::npos)
+        return 6;
+    }
     if (target) {
       auto* boids = dynamic_cast<Physics::HIP::Boids*>(model.get());
       if (!boids)
