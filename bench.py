@@ -129,7 +129,7 @@ def run_gpu(args, sc, mapping="literal", profile=True):
     if args.block or args.flags or args.sort_bits or args.queue:
         m.set_tuning(args.block or 256, args.flags, args.sort_bits or 0, args.queue or 0)
     elif not args.no_tune:
-        out["tuning"] = m.autotune(trial_steps=3)  # restores the input afterwards
+        out["tuning"] = m.autotune(trial_steps=5)  # restores the input afterwards
     m.sync()
 
     def fixed_steps(k):
