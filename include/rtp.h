@@ -213,6 +213,12 @@ int rtp_fluids_set_render(rtp_fluids* h, int enable);
 /* u_cameraPos (Fluids.cpp:132): the point the following updates sort by distance to.
  * The origin until set. */
 int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3]);
+/* Device addresses of the current p_col (float4 x maxN), p_cameraDist (uint32 x maxN) and
+ * c_partDetector (float8 per cell), for an app that copies them into its GL buffers (the
+ * role of cl::BufferGL, Context.cpp:517-541).  p_col and p_cameraDist each live in one of
+ * two buffers, which an update may switch: ask again after every update.  Any pointer
+ * argument may be NULL. */
+int rtp_fluids_device_render(rtp_fluids* h, void** col4, void** cameraDist, void** detector);
 /* Fluids::updateModelWithInputJson (OclModel.hpp:48-54, Fluids.cpp:218-251): the fluid
  * parameters and the Jacobi iteration count from the "Fluids" tree; FluidParams.dim keeps
  * the model's dimension. */
@@ -364,6 +370,7 @@ int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float rad
 int rtp_boids_set_render(rtp_boids* h, int enable);
 /* u_cameraPos (Boids.cpp:126). */
 int rtp_boids_set_camera(rtp_boids* h, const float pos[3]);
+int rtp_boids_device_render(rtp_boids* h, void** col4, void** cameraDist, void** detector);
 /* Boids::updateModelWithInputJson (Boids.cpp:177-210): the rules and the target's switch,
  * radius and sign from the "Boids" tree; the target keeps its position. */
 int rtp_boids_update_with_json(rtp_boids* h, const char* json);

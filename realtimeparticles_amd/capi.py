@@ -57,6 +57,12 @@ class _RenderFields:
         self._call("read_field", FIELD_CAMERA_DIST, out.ctypes.data, count)
         return out
 
+    def device_render(self):
+        """Device addresses (ints) of p_col, p_cameraDist and c_partDetector."""
+        a, b, c = _vp(), _vp(), _vp()
+        self._call("device_render", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
     def read_detector(self) -> np.ndarray:
         out = np.zeros((self.num_cells, 8), np.float32)
         self._call("read_field", FIELD_DETECTOR, out.ctypes.data, self.num_cells)
@@ -165,6 +171,8 @@ _SIGS = {
     "rtp_fluids_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_fluids_read_field": ([_vp, _int, _vp, _u32], _int),
     "rtp_fluids_set_render": ([_vp, _int], _int),
+    "rtp_fluids_device_render": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
+    "rtp_boids_device_render": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_fluids_update_with_json": ([_vp, ctypes.c_char_p], _int),
     "rtp_fluids_params_json": ([_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
     "rtp_boids_update_with_json": ([_vp, ctypes.c_char_p], _int),

@@ -758,6 +758,20 @@ struct rtp_boids {
   }
 };
 
+// Device addresses of the current rendering buffers.
+int render_pointers(const RenderState& R, void** col4, void** cameraDist, void** detector)
+{
+  if (!R.cam)
+    return fail(RTP_ERR_STATE, "the rendering steps were never enabled");
+  if (col4)
+    *col4 = R.col[R.colCur];
+  if (cameraDist)
+    *cameraDist = R.key[R.keyCur];
+  if (detector)
+    *detector = R.det;
+  return RTP_OK;
+}
+
 // p_col (float4), p_cameraDist (uint32) or c_partDetector (float8 per cell, count cells).
 int read_render_field(RenderState& R, int field, uint32_t numCells, float* out, uint32_t count,
                       hipStream_t s)
@@ -1178,6 +1192,12 @@ int rtp_fluids_params_json(rtp_fluids* h, char* buf, size_t cap, size_t* len)
   return rtp_fluid_params_to_json(&h->c.p, h->iters, buf, cap, len);
 }
 
+int rtp_fluids_device_render(rtp_fluids* h, void** col4, void** cameraDist, void** detector)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return render_pointers(h->render, col4, cameraDist, detector);
+}
+
 int rtp_fluids_set_camera(rtp_fluids* h, const float pos[3])
 {
   RTP_REQUIRE(h && pos, "NULL argument");
@@ -1545,6 +1565,12 @@ int rtp_boids_set_render(rtp_boids* h, int enable)
   h->render_reset();
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
   return RTP_OK;
+}
+
+int rtp_boids_device_render(rtp_boids* h, void** col4, void** cameraDist, void** detector)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return render_pointers(h->render, col4, cameraDist, detector);
 }
 
 int rtp_boids_set_camera(rtp_boids* h, const float pos[3])

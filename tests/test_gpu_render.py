@@ -193,3 +193,22 @@ def test_render_and_variant_argument_errors():
                                                      20000), "read_field")
     f.close()
     b.close()
+
+
+def test_device_render_pointers():
+    sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=15, extra=512)
+    gpu, _ = make_pbf_pair(sc, iters=1)
+    with pytest.raises(capi.RtpError, match="never enabled"):
+        gpu.device_render()
+    gpu.set_render(True)
+    col0, dist0, det0 = gpu.device_render()
+    assert col0 and dist0 and det0
+    gpu.update(1)
+    col1, dist1, det1 = gpu.device_render()
+    assert det1 == det0
+    # a paused update only sorts, so p_col moves to the other buffer of its pair
+    gpu.pause(True)
+    gpu.update(1)
+    col2, _, _ = gpu.device_render()
+    assert col2 != col1
+    gpu.close()
