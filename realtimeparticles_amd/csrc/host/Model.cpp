@@ -336,6 +336,13 @@ std::string Fluids::inputJson() const
   return s;
 }
 
+void* Fluids::deviceColors() const
+{
+  void* p = nullptr;
+  check(rtp_fluids_device_render(m_h, &p, nullptr, nullptr), "rtp_fluids_device_render");
+  return p;
+}
+
 void Fluids::enableRendering(bool enable)
 {
   check(rtp_fluids_set_render(m_h, enable ? 1 : 0), "rtp_fluids_set_render");
@@ -475,6 +482,13 @@ std::string Boids::inputJson() const
   check(rtp_boids_params_json(m_h, &s[0], s.size(), &n), "rtp_boids_params_json");
   s.resize(n);
   return s;
+}
+
+void* Boids::deviceColors() const
+{
+  void* p = nullptr;
+  check(rtp_boids_device_render(m_h, &p, nullptr, nullptr), "rtp_boids_device_render");
+  return p;
 }
 
 void Boids::enableRendering(bool enable)

@@ -128,6 +128,9 @@ class Model {
   // position they sort by (the reference's u_cameraPos buffer, Model.hpp:69).
   virtual void enableRendering(bool enable) = 0;
   virtual void setCameraPos(float x, float y, float z) = 0;
+  // Device pointer of the float4 colour buffer p_col (the colour VBO's role), valid until
+  // the next update; the rendering steps must be on.
+  virtual void* deviceColors() const = 0;
   // updateModelWithInputJson / getInputJson (physics/Model.hpp, OclModel.hpp:48-54) with
   // the JSON tree as text: the "Fluids" or "Boids" tree of the reference's widget.
   virtual void updateModelWithInputJson(const std::string& json) = 0;
@@ -180,6 +183,7 @@ class Fluids : public Model {
   void setCameraPos(float x, float y, float z) override;
   void updateModelWithInputJson(const std::string& json) override;
   std::string inputJson() const override;
+  void* deviceColors() const override;
   rtp_fluids* handle() const { return m_h; }
 
  protected:
@@ -219,6 +223,7 @@ class Boids : public Model {
   void setCameraPos(float x, float y, float z) override;
   void updateModelWithInputJson(const std::string& json) override;
   std::string inputJson() const override;
+  void* deviceColors() const override;
   rtp_boids* handle() const { return m_h; }
 
  protected:

@@ -56,6 +56,12 @@ int main(int argc, char** argv)
           back.find("\"Enable##Vorticity\": false") == std::string::npos)
         return 6;
     }
+    if (withJson) {  // the rendering steps through the Model surface as well
+      model->enableRendering(true);
+      model->setCameraPos(0.0f, 2.0f, 30.0f);
+      if (!model->deviceColors())
+        return 7;
+    }
     if (target) {
       auto* boids = dynamic_cast<Physics::HIP::Boids*>(model.get());
       if (!boids)

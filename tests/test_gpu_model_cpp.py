@@ -69,11 +69,13 @@ def test_cpp_2d_fluids_dam_matches_oracle(tmp_path):
 
 
 def test_cpp_fluids_json_tree_matches_oracle(tmp_path):
-    """Model::updateModelWithInputJson with three Jacobi iterations and vorticity off."""
+    """Model::updateModelWithInputJson with three Jacobi iterations and vorticity off, and
+    the rendering steps on through Model::enableRendering / setCameraPos."""
     n, max_n, (pos0, vel0, pos1, vel1) = run_demo("fluidsjson", tmp_path, 3)
     orc = O.OracleFluids((10, 10, 10), (30, 30, 30), max_n, n,
                          O.fluid_params(isVorticityConfEnabled=0), 3, gravity=9.81)
     orc.upload(pos0, vel0)
+    orc.enable_render((0.0, 2.0, 30.0))
     orc.step(3)
     _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
     _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
