@@ -1181,9 +1181,10 @@ int rtp_fluids_update_with_json(rtp_fluids* h, const char* json)
   if (rc)
     return rc;
   p.dim = h->c.p.dim;  // Fluids.cpp:234: from the model's dimension, not from the tree
-  if ((rc = rtp_fluids_set_nb_jacobi_iters(h, iters > 1000u ? 1000u : iters)))
+  // the parameters first: a rejected set (rest density 0) leaves the model as it was
+  if ((rc = rtp_fluids_set_params(h, &p)))
     return rc;
-  return rtp_fluids_set_params(h, &p);
+  return rtp_fluids_set_nb_jacobi_iters(h, iters > 1000u ? 1000u : iters);
 }
 
 int rtp_fluids_params_json(rtp_fluids* h, char* buf, size_t cap, size_t* len)

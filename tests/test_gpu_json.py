@@ -31,6 +31,12 @@ def test_fluids_update_with_json():
     assert back["Fluids"]["Vorticity Confinement"]["Enable##Vorticity"] is False
     with pytest.raises(capi.RtpError, match="missing key"):
         m.update_with_json('{"Fluids": {}}')
+    bad = copy.deepcopy(f)
+    bad["Fluids"]["Rest Density"] = [0.0, 10.0, 1000.0]
+    bad["Fluids"]["Nb Jacobi Iterations"] = [5, 1, 6]
+    with pytest.raises(capi.RtpError, match="restDensity"):
+        m.update_with_json(json.dumps(bad))
+    assert json.loads(m.params_json())["Fluids"]["Nb Jacobi Iterations"][0] == 3  # unchanged
     m.close()
 
 
