@@ -45,7 +45,7 @@ typedef struct {
   uint32_t isArtPressureEnabled;
   float artPressureRadius;
   float artPressureCoeff;
-  uint32_t artPressureExp;
+  uint32_t artPressureExp;  /* at most 64 (a longer power is rejected) */
   uint32_t isVorticityConfEnabled;
   float vorticityConfCoeff;
   float xsphViscosityCoeff;

@@ -426,6 +426,8 @@ int rtp_batch_destroy(rtp_batch* h)
 int rtp_batch_set_params(rtp_batch* h, uint32_t scene, const rtp_fluid_params* p)
 {
   RTP_REQUIRE(h && p, "NULL argument");
+  RTP_REQUIRE(p->restDensity != 0.0f, "restDensity must be non-zero");
+  RTP_REQUIRE(p->artPressureExp <= kMaxArtPressureExp, "artPressureExp must be at most 64");
   const uint32_t count = (uint32_t)h->scenes.size();
   RTP_REQUIRE(scene == 0xffffffffu || scene < count, "scene index out of range");
   if (scene == 0xffffffffu) {

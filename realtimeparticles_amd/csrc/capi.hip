@@ -880,6 +880,7 @@ int rtp_fluids_set_params(rtp_fluids* h, const rtp_fluid_params* p)
 {
   RTP_REQUIRE(h && p, "NULL argument");
   RTP_REQUIRE(p->restDensity != 0.0f, "restDensity must be non-zero");
+  RTP_REQUIRE(p->artPressureExp <= kMaxArtPressureExp, "artPressureExp must be at most 64");
   h->c.p = *p;
   h->refresh_consts();
   return RTP_OK;

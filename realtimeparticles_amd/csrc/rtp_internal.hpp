@@ -78,6 +78,10 @@ struct BoidsConsts {
   int32_t shortDiv = 0;     // radiusSq <= 2^30: the pair divisions take the short form
 };
 
+// Largest artificial-pressure exponent a parameter set may carry: the kernels raise to it
+// with a left-to-right product, one multiplication per unit (the reference's range is 1-6).
+constexpr uint32_t kMaxArtPressureExp = 64;
+
 // Survivor-queue depth of the neighbour passes when a tuning gives 0.
 constexpr int kDefaultQueue = 16;
 

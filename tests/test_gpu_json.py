@@ -67,3 +67,23 @@ def test_boids_update_with_json_runs_those_rules():
     assert back["Boids"]["Target"]["Radius"][0] == np.float32(0.9)
     assert back["Boids"]["Cohesion"]["Scale##Cohesion"][0] == 0.0
     gpu.close()
+
+
+def test_parameter_limits():
+    """An exponent the kernels would loop over for too long, or a zero rest density, is
+    rejected on the host by a model, by a batch and through a JSON tree."""
+    m = capi.Fluids(4096, 4096, (4, 2, 2), (40, 20, 20))
+    m.set_params(artPressureExp=64)
+    with pytest.raises(capi.RtpError, match="artPressureExp"):
+        m.set_params(artPressureExp=65)
+    f = copy.deepcopy(FLUIDS)
+    f["Fluids"]["Artificial Pressure"]["Exp"] = [1e10, 1, 6]
+    with pytest.raises(capi.RtpError, match="artPressureExp"):
+        m.update_with_json(json.dumps(f))
+    m.close()
+    b = capi.Batch([(2048, (1.0, 1.0, 1.0), (10, 10, 10), 9.8)])
+    with pytest.raises(capi.RtpError, match="artPressureExp"):
+        b.set_params(artPressureExp=1000)
+    with pytest.raises(capi.RtpError, match="restDensity"):
+        b.set_params(restDensity=0.0)
+    b.close()
