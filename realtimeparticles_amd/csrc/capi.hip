@@ -191,7 +191,8 @@ struct RenderState {
     high = std::max(high, n);
     const uint32_t R = std::max(high, G.sortedN);
     RTP_PROF(prof, "camera_key", s, launch_camera_key(pos, cam, key[keyCur], n, s));
-    const SortPlan plan = make_sort_plan(R, 1000000u, G.sortBits);
+    // keys below 2^20: two passes of 10-bit digits whatever width the cell sort is tuned to
+    const SortPlan plan = make_sort_plan(R, 1000000u, 10);
     bool inB = false;
     int rc = 0;
     RTP_PROF(prof, "camera_sort", s,
