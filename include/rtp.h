@@ -138,6 +138,13 @@ int rtp_fluid_params_to_json(const rtp_fluid_params* p, uint32_t nbJacobiIters, 
 int rtp_boids_params_from_json(const char* json, rtp_boids_params* p, rtp_target_params* t);
 int rtp_boids_params_to_json(const rtp_boids_params* p, const rtp_target_params* t, char* buf,
                              size_t cap, size_t* len);
+/* The two tree operations of Model::updateInputJson (physics/Model.hpp:164-172), for a C++
+ * Model that keeps its tree as text: rtp_json_equal sets *equal to 1 when json::diff(a, b)
+ * would be empty (equal values: objects as maps, numbers by value), else 0;
+ * rtp_json_merge_patch writes target.merge_patch(patch) (RFC 7386: an object patch merges
+ * key by key and a null removes the key; any other patch replaces the target). */
+int rtp_json_equal(const char* a, const char* b, int* equal);
+int rtp_json_merge_patch(const char* target, const char* patch, char* buf, size_t cap, size_t* len);
 
 /* ---- library ---- */
 int rtp_abi_version(void);
@@ -157,8 +164,9 @@ int rtp_fluids_destroy(rtp_fluids* h);
 /* Fluids::transferKernelInputsToGPU (Fluids.cpp:253-271). */
 int rtp_fluids_set_params(rtp_fluids* h, const rtp_fluid_params* p);
 int rtp_fluids_get_params(rtp_fluids* h, rtp_fluid_params* p);
-/* m_nbJacobiIters, set from "Nb Jacobi Iterations" (Fluids.cpp:227). */
+/* m_nbJacobiIters, set from "Nb Jacobi Iterations" (Fluids.cpp:227); at most 1000. */
 int rtp_fluids_set_nb_jacobi_iters(rtp_fluids* h, uint32_t iters);
+int rtp_fluids_get_nb_jacobi_iters(rtp_fluids* h, uint32_t* iters);
 /* Model::setNbParticles (physics/Model.hpp:106). */
 int rtp_fluids_set_nb_particles(rtp_fluids* h, uint32_t n);
 int rtp_fluids_get_nb_particles(rtp_fluids* h, uint32_t* n);
@@ -361,8 +369,13 @@ int rtp_boids_set_boundary(rtp_boids* h, int boundary);
  * every step adds sign * (pos - p) * clamp(1.3 / |pos - p|, 0, 0.7) to the acceleration of
  * each boid p within radius of pos (TargetParams, Boids.hpp:22-26).  The reference moves
  * the target on the host every frame (Target::updatePos, physics/utils/Target.cpp:27-50);
- * here the caller passes the position of the frame. */
+ * here the caller passes the position of the frame.  A non-NULL pos is stored even while
+ * inactive, so a target switched on later (here or by a JSON tree) starts there; radius
+ * and sign are taken only with active set. */
 int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float radius, int sign);
+/* Boids::targetPos / isTargetActivated / isTargetVisible (physics/ocl/Boids.hpp:38-42): the
+ * target's switch, "Show" state, radius and sign, and its last position (pos may be NULL). */
+int rtp_boids_get_target(rtp_boids* h, rtp_target_params* t, float pos[3]);
 /* Boids.cpp:375-381: the grid detector after every step and, paused or not, the
  * camera-distance sort that permutes p_pos, p_vel and p_acc (p_col holds bd_fillBoidsColor's
  * single colour, boids.cl:38-41).  As rtp_fluids_set_render: off by default, enabling resets

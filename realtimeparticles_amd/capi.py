@@ -161,6 +161,7 @@ _SIGS = {
     "rtp_fluids_set_params": ([_vp, ctypes.POINTER(FluidParams)], _int),
     "rtp_fluids_get_params": ([_vp, ctypes.POINTER(FluidParams)], _int),
     "rtp_fluids_set_nb_jacobi_iters": ([_vp, _u32], _int),
+    "rtp_fluids_get_nb_jacobi_iters": ([_vp, ctypes.POINTER(_u32)], _int),
     "rtp_fluids_set_nb_particles": ([_vp, _u32], _int),
     "rtp_fluids_get_nb_particles": ([_vp, ctypes.POINTER(_u32)], _int),
     "rtp_fluids_load_state": ([_vp, _vp, _vp, _u32], _int),
@@ -224,6 +225,10 @@ _SIGS = {
     "rtp_boids_set_tuning": ([_vp, ctypes.POINTER(Tuning)], _int),
     "rtp_boids_set_boundary": ([_vp, _int], _int),
     "rtp_boids_set_target": ([_vp, _int, ctypes.POINTER(_f32), _f32, _int], _int),
+    "rtp_boids_get_target": ([_vp, ctypes.POINTER(TargetParams), ctypes.POINTER(_f32)], _int),
+    "rtp_json_equal": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_int)], _int),
+    "rtp_json_merge_patch": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                              ctypes.POINTER(ctypes.c_size_t)], _int),
     "rtp_boids_set_camera": ([_vp, ctypes.POINTER(_f32)], _int),
     "rtp_boids_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
     "rtp_boids_enable_profiling": ([_vp, _int], _int),
@@ -317,6 +322,18 @@ def boids_params_to_json(p, t) -> str:
     return _json_text(lib().rtp_boids_params_to_json, ctypes.byref(p), ctypes.byref(t))
 
 
+def json_equal(a: str, b: str) -> bool:
+    """Whether two JSON texts hold equal values (json::diff(a, b) empty)."""
+    eq = _int(0)
+    _check(lib().rtp_json_equal(a.encode(), b.encode(), ctypes.byref(eq)), "rtp_json_equal")
+    return bool(eq.value)
+
+
+def json_merge_patch(target: str, patch: str) -> str:
+    """target.merge_patch(patch) (RFC 7386) as text."""
+    return _json_text(lib().rtp_json_merge_patch, target.encode(), patch.encode())
+
+
 def device_count() -> int:
     c = _int(0)
     _check(lib().rtp_device_count(ctypes.byref(c)), "rtp_device_count")
@@ -386,6 +403,11 @@ class Fluids(_RenderFields):
 
     def set_nb_jacobi_iters(self, iters: int) -> None:
         _check(lib().rtp_fluids_set_nb_jacobi_iters(self._h, int(iters)), "set_nb_jacobi_iters")
+
+    def nb_jacobi_iters(self) -> int:
+        it = _u32(0)
+        _check(lib().rtp_fluids_get_nb_jacobi_iters(self._h, ctypes.byref(it)), "get_nb_jacobi_iters")
+        return int(it.value)
 
     def set_nb_particles(self, n: int) -> None:
         _check(lib().rtp_fluids_set_nb_particles(self._h, int(n)), "set_nb_particles")
@@ -758,6 +780,12 @@ class Boids(_RenderFields):
         p = (_f32 * 3)(*[float(v) for v in pos])
         _check(lib().rtp_boids_set_target(self._h, int(bool(active)), p, float(radius), int(sign)),
                "set_target")
+
+    def target(self):
+        """(TargetParams, position) of the target rule."""
+        t, p = TargetParams(), (_f32 * 3)()
+        _check(lib().rtp_boids_get_target(self._h, ctypes.byref(t), p), "get_target")
+        return t, tuple(p)
 
     def load_state(self, pos, vel) -> None:
         pos, vel = _f4(pos), _f4(vel)

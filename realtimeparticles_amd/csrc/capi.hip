@@ -903,6 +903,13 @@ int rtp_fluids_set_nb_jacobi_iters(rtp_fluids* h, uint32_t iters)
   return RTP_OK;
 }
 
+int rtp_fluids_get_nb_jacobi_iters(rtp_fluids* h, uint32_t* iters)
+{
+  RTP_REQUIRE(h && iters, "NULL argument");
+  *iters = h->iters;
+  return RTP_OK;
+}
+
 int rtp_fluids_set_nb_particles(rtp_fluids* h, uint32_t n)
 {
   RTP_REQUIRE(h, "NULL handle");
@@ -1504,12 +1511,26 @@ int rtp_boids_set_target(rtp_boids* h, int active, const float pos[3], float rad
   RTP_REQUIRE(h, "NULL handle");
   RTP_REQUIRE(!active || pos, "NULL target position");
   h->c.targetOn = active ? 1 : 0;
-  if (active) {
+  if (pos)
     for (int k = 0; k < 3; ++k)
       h->c.target[k] = pos[k];
+  if (active) {
     h->c.targetRadius = radius;
     h->c.targetSign = (float)sign;
   }
+  return RTP_OK;
+}
+
+int rtp_boids_get_target(rtp_boids* h, rtp_target_params* t, float pos[3])
+{
+  RTP_REQUIRE(h && t, "NULL argument");
+  t->active = h->c.targetOn;
+  t->visible = h->targetVisible ? 1 : 0;
+  t->radius = h->c.targetRadius;
+  t->sign = (int32_t)h->c.targetSign;
+  if (pos)
+    for (int k = 0; k < 3; ++k)
+      pos[k] = h->c.target[k];
   return RTP_OK;
 }
 

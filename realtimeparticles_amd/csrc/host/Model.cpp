@@ -12,6 +12,23 @@ void check(int rc, const char* what)
     throw std::runtime_error(std::string(what) + ": " + rtp_last_error());
 }
 
+// The initial parameter trees, initFluidsJson (physics/ocl/Fluids.cpp:52-74) and
+// initBoidsJson (physics/ocl/Boids.cpp:41-73), with their float literals written so that
+// they read back as the same floats.
+const char* const kInitFluidsJson =
+    "{\"Fluids\": {\"Rest Density\": [450, 10, 1000], \"Relax CFM\": [600, 100, 1000], "
+    "\"Time Step\": [0.01, 0.0001, 0.02], \"Nb Jacobi Iterations\": [2, 1, 6], "
+    "\"Artificial Pressure\": {\"Enable##Pressure\": true, \"Coefficient##Pressure\": [0.001, 0, 0.001], "
+    "\"Radius\": [0.006, 0.001, 0.015], \"Exp\": [4, 1, 6]}, "
+    "\"Vorticity Confinement\": {\"Enable##Vorticity\": true, \"Coefficient##Vorticity\": [0.0004, 0, 0.001], "
+    "\"xSPH Viscosity Coefficient\": [0.0001, 0, 0.001]}}}";
+const char* const kInitBoidsJson =
+    "{\"Boids\": {\"Velocity\": [0.5, 0.01, 5], "
+    "\"Target\": {\"Enable##Target\": false, \"Show\": true, \"Radius\": [2, 1, 20], \"Attract\": true}, "
+    "\"Alignment\": {\"Enable##Alignment\": true, \"Scale##Alignment\": [1.6, 0, 3]}, "
+    "\"Cohesion\": {\"Enable##Cohesion\": true, \"Scale##Cohesion\": [1.45, 0, 3]}, "
+    "\"Separation\": {\"Enable##Separation\": true, \"Scale##Separation\": [1.6, 0, 3]}}}";
+
 struct F3 {
   float x, y, z;
 };
@@ -189,6 +206,23 @@ void Model::enableProfiling(bool enable)
     applyProfiling();
 }
 
+void Model::updateInputJson(const std::string& newJson)
+{
+  int equal = 0;
+  check(rtp_json_equal(m_inputJson.c_str(), newJson.c_str(), &equal), "rtp_json_equal");
+  if (equal)
+    return;
+  size_t n = 0;
+  check(rtp_json_merge_patch(m_inputJson.c_str(), newJson.c_str(), nullptr, 0, &n),
+        "rtp_json_merge_patch");
+  std::string merged(n + 1, '\0');
+  check(rtp_json_merge_patch(m_inputJson.c_str(), newJson.c_str(), &merged[0], merged.size(), &n),
+        "rtp_json_merge_patch");
+  merged.resize(n);
+  m_inputJson = merged;
+  updateModelWithInputJson(m_inputJson);
+}
+
 namespace HIP {
 
 // ------------------------------------------------------------------------------ Fluids
@@ -198,6 +232,7 @@ Fluids::Fluids(const ModelParams& params) : Model(params)
   const rtp_model_desc d = make_desc(params);
   check(rtp_fluids_create(&d, &m_h), "rtp_fluids_create");
   check(rtp_fluids_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_fluids_set_nb_jacobi_iters");
+  resetInputJson(kInitFluidsJson);  // Fluids.cpp:76: the model starts with the initial tree
   m_init = true;
   if (m_case != Utils::CASE_INVALID)
     reset();
@@ -220,10 +255,13 @@ void Fluids::reset()
   if (!m_init)
     return;
   const bool d2 = m_dimension == Geometry::Dimension::dim2D;
-  const rtp_fluid_params defaults{ 450.0f, 600.0f, 0.010f, d2 ? 2u : 3u, 1u, 0.006f, 0.001f, 4u, 1u, 0.0004f, 0.0001f };
-  check(rtp_fluids_set_params(m_h, &defaults), "rtp_fluids_set_params");
-  m_nbJacobiIters = 2;
-  check(rtp_fluids_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_fluids_set_nb_jacobi_iters");
+  // FluidParams.dim follows the model's dimension (Fluids.cpp:234), then the initial tree
+  rtp_fluid_params p{};
+  check(rtp_fluids_get_params(m_h, &p), "rtp_fluids_get_params");
+  p.dim = d2 ? 2u : 3u;
+  check(rtp_fluids_set_params(m_h, &p), "rtp_fluids_set_params");
+  resetInputJson(kInitFluidsJson);
+  updateModelWithInputJson(getInputJson());
   const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
   std::vector<F3> verts;
   if (d2) {
@@ -320,9 +358,9 @@ void* Fluids::devicePositions() const
 void Fluids::updateModelWithInputJson(const std::string& json)
 {
   check(rtp_fluids_update_with_json(m_h, json.c_str()), "rtp_fluids_update_with_json");
-  rtp_fluid_params p{};
+  // the count the model runs (the library caps it at 1000)
   uint32_t iters = 0;
-  check(rtp_fluid_params_from_json(json.c_str(), &p, &iters), "rtp_fluid_params_from_json");
+  check(rtp_fluids_get_nb_jacobi_iters(m_h, &iters), "rtp_fluids_get_nb_jacobi_iters");
   m_nbJacobiIters = iters;
 }
 
@@ -388,6 +426,7 @@ Boids::Boids(const ModelParams& params) : Model(params)
 {
   const rtp_model_desc d = make_desc(params);
   check(rtp_boids_create(&d, &m_h), "rtp_boids_create");
+  resetInputJson(kInitBoidsJson);  // Boids.cpp:75
   m_init = true;
   if (m_case != Utils::CASE_INVALID)
     reset();
@@ -414,8 +453,7 @@ void Boids::reset()
 {
   if (!m_init)
     return;
-  const rtp_boids_params defaults{ 0.5f, 1.6f, 1.6f, 1.45f };
-  check(rtp_boids_set_params(m_h, &defaults), "rtp_boids_set_params");
+  resetInputJson(kInitBoidsJson);
   switch (m_case) {
   case Utils::BOIDS_SMALL: m_currNbParticles = Utils::P512; break;
   case Utils::BOIDS_MEDIUM: m_currNbParticles = Utils::P16K; break;
@@ -423,6 +461,7 @@ void Boids::reset()
   case Utils::BOIDS_XLARGE: m_currNbParticles = Utils::P130K; break;
   default: throw std::runtime_error("Boids::reset: unknown case");
   }
+  updateModelWithInputJson(getInputJson());  // Boids.cpp:259
   if (m_currNbParticles > m_maxNbParticles)
     throw std::runtime_error("Boids::reset: case needs more particles than maxNbParticles");
   const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
@@ -444,8 +483,17 @@ void Boids::setTimeStep(float dt) { check(rtp_boids_set_time_step(m_h, dt), "rtp
 void Boids::setTarget(bool active, std::array<float, 3> pos, float radius, int sign)
 {
   check(rtp_boids_set_target(m_h, active ? 1 : 0, pos.data(), radius, sign), "rtp_boids_set_target");
-  m_targetActive = active;
-  m_targetPos = pos;
+  readTarget();
+}
+
+void Boids::readTarget()
+{
+  rtp_target_params t{};
+  float pos[3] = { 0.0f, 0.0f, 0.0f };
+  check(rtp_boids_get_target(m_h, &t, pos), "rtp_boids_get_target");
+  m_targetActive = t.active != 0;
+  m_targetVisible = t.visible != 0;
+  m_targetPos = { pos[0], pos[1], pos[2] };
 }
 
 void Boids::loadState(const float* pos4, const float* vel4, size_t count)
@@ -467,11 +515,9 @@ void* Boids::devicePositions() const
 
 void Boids::updateModelWithInputJson(const std::string& json)
 {
+  // the target keeps its last position (rtp_boids_set_target stores it even while off)
   check(rtp_boids_update_with_json(m_h, json.c_str()), "rtp_boids_update_with_json");
-  rtp_boids_params p{};
-  rtp_target_params t{};
-  check(rtp_boids_params_from_json(json.c_str(), &p, &t), "rtp_boids_params_from_json");
-  m_targetActive = t.active != 0;
+  readTarget();
 }
 
 std::string Boids::inputJson() const

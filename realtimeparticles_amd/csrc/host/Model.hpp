@@ -4,9 +4,12 @@
 //
 // Differences at this boundary:
 //   * no GL VBOs: the models own their float4 buffers on the GPU; devicePositions() and
-//     readPositions() stand in for the shared VBO;
-//   * no JSON parameter tree: typed setters (setFluidParams, setBoidsParams, ...) replace
-//     updateInputJson/transferJsonInputsToModel (Fluids.cpp:218-251, Boids.cpp:177-210);
+//     readState() stand in for the shared VBO;
+//   * the JSON parameter tree is kept as text (getInputJson / resetInputJson /
+//     updateInputJson, physics/Model.hpp:154-174, with the library's reader for the
+//     reference's JSON library); typed setters (setFluidParams, setBoidsParams, ...) sit
+//     beside it and change the model without touching the tree, and inputJson() writes the
+//     model's current parameters as a tree;
 //   * errors from the C ABI are thrown as std::runtime_error, like the reference's JSON and
 //     program-build failures (Fluids.cpp:249, Context.cpp:279).
 #pragma once
@@ -79,7 +82,7 @@ class Model;
 // physics/Model.cpp:10-24.  CLOUDS is not provided by this backend (returns nullptr).
 std::unique_ptr<Model> CreateModel(ModelType type, ModelParams params);
 
-// physics/Model.hpp:81-216 minus the JSON channel and the displayable-quantity registry.
+// physics/Model.hpp:81-216 minus the displayable-quantity registry.
 class Model {
  public:
   explicit Model(const ModelParams& params);
@@ -131,9 +134,19 @@ class Model {
   // Device pointer of the float4 colour buffer p_col (the colour VBO's role), valid until
   // the next update; the rendering steps must be on.
   virtual void* deviceColors() const = 0;
-  // updateModelWithInputJson / getInputJson (physics/Model.hpp, OclModel.hpp:48-54) with
-  // the JSON tree as text: the "Fluids" or "Boids" tree of the reference's widget.
+  // The parameter tree of the JSON channel (physics/Model.hpp:154-172) as text: the
+  // "Fluids" or "Boids" tree of the reference's widget (ui/PhysicsWidget.cpp:155-159 reads
+  // it with getInputJson and sends edits with updateInputJson).  reset() puts the
+  // reference's initial tree back and applies it, as Fluids::reset / Boids::reset do.
+  std::string getInputJson() const { return m_inputJson; }
+  void resetInputJson(const std::string& newJson) { m_inputJson = newJson; }
+  // No-op when newJson equals the tree as a JSON value (json::diff empty); otherwise the
+  // tree becomes tree.merge_patch(newJson) and is applied with updateModelWithInputJson.
+  void updateInputJson(const std::string& newJson);
+  // updateModelWithInputJson (physics/Model.hpp:174, OclModel.hpp:48-54): applies a whole
+  // tree to the model (transferJsonInputsToModel) without storing it.
   virtual void updateModelWithInputJson(const std::string& json) = 0;
+  // The model's current parameters written as a tree (typed setters included).
   virtual std::string inputJson() const = 0;
   // Per-kernel timings collected while profiling was enabled.
   virtual std::vector<rtp_kernel_time> kernelTimes() const = 0;
@@ -155,6 +168,9 @@ class Model {
   Utils::PhysicsCase m_case;
   Boundary m_boundary = Boundary::BouncingWall;
   int m_device = 0;
+
+ private:
+  std::string m_inputJson;  // the JSON channel's tree (physics/Model.hpp:214)
 };
 
 namespace HIP {
@@ -214,6 +230,9 @@ class Boids : public Model {
   void setTarget(bool active, std::array<float, 3> pos, float radius = 2.0f, int sign = 1);
   std::array<float, 3> targetPos() const override { return m_targetPos; }
   bool isTargetActivated() const override { return m_targetActive; }
+  // Boids.hpp:42: the "Show" switch of the tree while the target is active (the app draws
+  // the target when set, app/ParticleSystemApp.cpp:380).
+  bool isTargetVisible() const override { return m_targetVisible; }
 
   void loadState(const float* pos4, const float* vel4, size_t count) override;
   void readState(float* pos4, float* vel4, size_t count) const override;
@@ -232,8 +251,11 @@ class Boids : public Model {
   void applyProfiling() override;
 
  private:
+  void readTarget();
+
   rtp_boids* m_h = nullptr;
   bool m_targetActive = false;
+  bool m_targetVisible = false;
   std::array<float, 3> m_targetPos{ 0.0f, 0.0f, 0.0f };
 };
 

@@ -356,6 +356,147 @@ std::string num(double v)
 
 std::string range3(double v, double lo, double hi) { return "[" + num(v) + ", " + num(lo) + ", " + num(hi) + "]"; }
 
+// ---- the tree operations of the reference's Model (physics/Model.hpp:154-172), on JVal
+
+// Value equality as the reference's JSON library defines it: objects compare as maps (key
+// order does not matter; a repeated key counts with its last value, as the library keeps
+// it), arrays element-wise, numbers by value.
+const JVal* last_of(const JVal& obj, const std::string& key)
+{
+  const JVal* r = nullptr;
+  for (const auto& kv : obj.o)
+    if (kv.first == key)
+      r = &kv.second;
+  return r;
+}
+
+bool same(const JVal& a, const JVal& b)
+{
+  if (a.type != b.type)
+    return false;
+  switch (a.type) {
+  case JVal::Null: return true;
+  case JVal::Bool: return a.b == b.b;
+  case JVal::Num: return a.n == b.n;
+  case JVal::Str: return a.s == b.s;
+  case JVal::Arr:
+    if (a.a.size() != b.a.size())
+      return false;
+    for (size_t k = 0; k < a.a.size(); ++k)
+      if (!same(a.a[k], b.a[k]))
+        return false;
+    return true;
+  case JVal::Obj: {
+    size_t na = 0, nb = 0;
+    for (size_t k = 0; k < a.o.size(); ++k) {
+      if (last_of(a, a.o[k].first) != &a.o[k].second)
+        continue;  // an earlier duplicate
+      ++na;
+      const JVal* x = last_of(b, a.o[k].first);
+      if (!x || !same(a.o[k].second, *x))
+        return false;
+    }
+    for (size_t k = 0; k < b.o.size(); ++k)
+      if (last_of(b, b.o[k].first) == &b.o[k].second)
+        ++nb;
+    return na == nb;
+  }
+  }
+  return false;
+}
+
+// RFC 7386 JSON merge patch, the library's merge_patch: an object patch merges key by key
+// (null removes the key), anything else replaces the target.
+void merge_patch(JVal& target, const JVal& patch)
+{
+  if (patch.type != JVal::Obj) {
+    target = patch;
+    return;
+  }
+  if (target.type != JVal::Obj) {
+    target = JVal{};
+    target.type = JVal::Obj;
+  }
+  for (const auto& kv : patch.o) {
+    auto it = target.o.begin();
+    while (it != target.o.end() && it->first != kv.first)
+      ++it;
+    if (kv.second.type == JVal::Null) {
+      // remove every entry of the key
+      for (size_t k = 0; k < target.o.size();)
+        if (target.o[k].first == kv.first)
+          target.o.erase(target.o.begin() + (long)k);
+        else
+          ++k;
+      continue;
+    }
+    if (it == target.o.end()) {
+      target.o.emplace_back(kv.first, JVal{});
+      merge_patch(target.o.back().second, kv.second);
+    } else {
+      merge_patch(it->second, kv.second);
+    }
+  }
+}
+
+void write_string(std::string& out, const std::string& s)
+{
+  out.push_back('"');
+  for (const char ch : s) {
+    const unsigned char c = (unsigned char)ch;
+    if (c == '"' || c == '\\') {
+      out.push_back('\\');
+      out.push_back(ch);
+    } else if (c < 0x20) {
+      char t[8];
+      std::snprintf(t, sizeof(t), "\\u%04x", c);
+      out += t;
+    } else {
+      out.push_back(ch);
+    }
+  }
+  out.push_back('"');
+}
+
+// Numbers are written with 17 significant digits, so every double reads back exactly.
+void write(std::string& out, const JVal& v)
+{
+  switch (v.type) {
+  case JVal::Null: out += "null"; break;
+  case JVal::Bool: out += v.b ? "true" : "false"; break;
+  case JVal::Num: {
+    char t[40];
+    if (std::isfinite(v.n) && v.n == std::floor(v.n) && std::fabs(v.n) < 1e15)
+      std::snprintf(t, sizeof(t), "%.0f", v.n);
+    else
+      std::snprintf(t, sizeof(t), "%.17g", v.n);
+    out += t;
+    break;
+  }
+  case JVal::Str: write_string(out, v.s); break;
+  case JVal::Arr:
+    out.push_back('[');
+    for (size_t k = 0; k < v.a.size(); ++k) {
+      if (k)
+        out += ", ";
+      write(out, v.a[k]);
+    }
+    out.push_back(']');
+    break;
+  case JVal::Obj:
+    out.push_back('{');
+    for (size_t k = 0; k < v.o.size(); ++k) {
+      if (k)
+        out += ", ";
+      write_string(out, v.o[k].first);
+      out += ": ";
+      write(out, v.o[k].second);
+    }
+    out.push_back('}');
+    break;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -381,7 +522,9 @@ int rtp_fluid_params_from_json(const char* json, rtp_fluid_params* p, uint32_t* 
       !first_number(vc.at("Coefficient##Vorticity"), &vcc) ||
       !first_number(vc.at("xSPH Viscosity Coefficient"), &xs))
     return bad();
-  // Fluids.cpp:229-246; a negative iteration count runs no iteration, as the loop there
+  // Fluids.cpp:229-246.  A negative count runs no iteration: a deliberate deviation, since
+  // the reference stores it in a size_t (m_nbJacobiIters), where -1 wraps to SIZE_MAX and
+  // its iteration loop would not end.
   const int32_t n = to_int(it);
   *iters = n > 0 ? (uint32_t)n : 0u;
   p->restDensity = (float)rho;
@@ -479,6 +622,30 @@ int rtp_boids_params_to_json(const rtp_boids_params* p, const rtp_target_params*
       "\"Cohesion\": {\"Enable##Cohesion\": true, \"Scale##Cohesion\": " + range3(p->cohesionScale, 0.0f, 3.0f) + "}, " +
       "\"Separation\": {\"Enable##Separation\": true, \"Scale##Separation\": " + range3(p->separationScale, 0.0f, 3.0f) + "}}}";
   return put(buf, cap, len, s);
+}
+
+int rtp_json_equal(const char* a, const char* b, int* equal)
+{
+  if (!equal) {
+    g_jsonError = "NULL argument";
+    return bad();
+  }
+  JVal x, y;
+  if (!parse(a, x) || !parse(b, y))
+    return bad();
+  *equal = same(x, y) ? 1 : 0;
+  return RTP_OK;
+}
+
+int rtp_json_merge_patch(const char* target, const char* patch, char* buf, size_t cap, size_t* len)
+{
+  JVal t, q;
+  if (!parse(target, t) || !parse(patch, q))
+    return bad();
+  merge_patch(t, q);
+  std::string out;
+  write(out, t);
+  return put(buf, cap, len, out);
 }
 
 }  // extern "C"

@@ -100,3 +100,32 @@ def test_cpp_boids_variants_match_oracle(tmp_path, kind):
     orc.step(3)
     _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
     _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
+
+
+def test_cpp_fluids_input_json_channel(tmp_path):
+    """Model::getInputJson / updateInputJson / resetInputJson (physics/Model.hpp:154-172) as
+    the reference's widget drives them: an equal tree is a no-op, a partial tree is merged
+    (three iterations, vorticity off) and applied, reset() restores the initial tree, a
+    count past the cap reads back as the count that runs.  The steps after the edit match
+    the oracle at the merged parameters."""
+    n, max_n, (pos0, vel0, pos1, vel1) = run_demo("fluidsinput", tmp_path, 3)
+    orc = O.OracleFluids((10, 10, 10), (30, 30, 30), max_n, n,
+                         O.fluid_params(isVorticityConfEnabled=0), 3, gravity=9.81)
+    orc.upload(pos0, vel0)
+    orc.step(3)
+    _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
+    _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
+
+
+def test_cpp_boids_target_switched_on_by_the_tree(tmp_path):
+    """A target parked at (1, 0.5, 0) while off, then switched on by a partial tree: the
+    kernels steer to where targetPos() says (radius 2 and attraction from the tree), and
+    isTargetVisible() follows "Show"."""
+    n, max_n, (pos0, vel0, pos1, vel1) = run_demo("boidsinput", tmp_path, 3)
+    orc = O.OracleBoids((10, 10, 10), (30, 30, 30), max_n, n,
+                        O.BoidsParams(0.5, 1.6, 1.6, 1.45), time_step=0.1)
+    orc.set_target(True, (1.0, 0.5, 0.0), 2.0, 1)
+    orc.upload(pos0, vel0)
+    orc.step(3)
+    _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
+    _eq(vel1[:, :3], orc.vel[:n, :3], "vel")

@@ -124,3 +124,68 @@ def test_short_buffer_reports_length():
     buf = ctypes.create_string_buffer(16)
     rc = capi.lib().rtp_fluid_params_to_json(ctypes.byref(p), 2, buf, 16, ctypes.byref(n))
     assert rc == -1 and n.value > 16 and len(buf.value) == 15
+
+
+# ---------------------------------------------------------------- tree operations of
+# Model::updateInputJson (physics/Model.hpp:164-172): json::diff(...).empty() and merge_patch
+
+# RFC 7386 Appendix A: original, patch, result
+RFC7386_CASES = [
+    ({"a": "b"}, {"a": "c"}, {"a": "c"}),
+    ({"a": "b"}, {"b": "c"}, {"a": "b", "b": "c"}),
+    ({"a": "b"}, {"a": None}, {}),
+    ({"a": "b", "b": "c"}, {"a": None}, {"b": "c"}),
+    ({"a": ["b"]}, {"a": "c"}, {"a": "c"}),
+    ({"a": "c"}, {"a": ["b"]}, {"a": ["b"]}),
+    ({"a": {"b": "c"}}, {"a": {"b": "d", "c": None}}, {"a": {"b": "d"}}),
+    ({"a": [{"b": "c"}]}, {"a": [1]}, {"a": [1]}),
+    (["a", "b"], ["c", "d"], ["c", "d"]),
+    ({"a": "b"}, ["c"], ["c"]),
+    ({"a": "foo"}, None, None),
+    ({"a": "foo"}, "bar", "bar"),
+    ({"e": None}, {"a": 1}, {"e": None, "a": 1}),
+    ([1, 2], {"a": "b", "c": None}, {"a": "b"}),
+    ({}, {"a": {"bb": {"ccc": None}}}, {"a": {"bb": {}}}),
+]
+
+
+@pytest.mark.parametrize("orig,patch,want", RFC7386_CASES)
+def test_merge_patch_rfc7386_vectors(orig, patch, want):
+    got = capi.json_merge_patch(json.dumps(orig), json.dumps(patch))
+    assert json.loads(got) == want
+
+
+def test_merge_patch_of_a_widget_edit():
+    # the widget sends the whole edited tree (ui/PhysicsWidget.cpp:155-159); a partial
+    # tree changes only its own keys
+    edit = {"Fluids": {"Nb Jacobi Iterations": [3, 1, 6],
+                       "Vorticity Confinement": {"Enable##Vorticity": False}}}
+    merged = json.loads(capi.json_merge_patch(json.dumps(FLUIDS), json.dumps(edit)))
+    want = copy.deepcopy(FLUIDS)
+    want["Fluids"]["Nb Jacobi Iterations"] = [3, 1, 6]
+    want["Fluids"]["Vorticity Confinement"]["Enable##Vorticity"] = False
+    assert merged == want
+    p, iters = capi.fluid_params_from_json(json.dumps(merged))
+    assert iters == 3 and p.isVorticityConfEnabled == 0 and p.restDensity == 450.0
+    # numbers survive the text round trip exactly
+    odd = {"x": [0.1, 1e-310, 123456789.123456789, -0.0, 2 ** 53]}
+    assert json.loads(capi.json_merge_patch("{}", json.dumps(odd))) == odd
+
+
+def test_json_equal_is_value_equality():
+    eq = capi.json_equal
+    assert eq(json.dumps(FLUIDS), json.dumps(FLUIDS))
+    # key order does not matter, numbers compare by value
+    assert eq('{"a": 1, "b": [1, 2.0]}', '{"b": [1.0, 2], "a": 1.0}')
+    assert eq('{"a": {"x": null}}', '{"a": {"x": null}}')
+    assert not eq('{"a": [1, 2]}', '{"a": [2, 1]}')
+    assert not eq('{"a": 1}', '{"a": 1, "b": 2}')
+    assert not eq('{"a": true}', '{"a": 1}')
+    assert not eq('{"a": "1"}', '{"a": 1}')
+    changed = copy.deepcopy(FLUIDS)
+    changed["Fluids"]["Artificial Pressure"]["Exp"][0] = 5
+    assert not eq(json.dumps(FLUIDS), json.dumps(changed))
+    with pytest.raises(capi.RtpError, match="malformed"):
+        eq("{", "{}")
+    with pytest.raises(capi.RtpError, match="malformed"):
+        capi.json_merge_patch("{}", "[1,")
