@@ -402,6 +402,19 @@ int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxK
  * start/end with the cap.  keys/idx are maxN long, startEnd 2*NC. */
 int rtp_grid_build(const rtp_model_desc* desc, const float* pos4, uint32_t* keys, uint32_t* idx,
                    uint32_t* startEnd);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the grid
+ * build of rtp_grid_build, with the same results, on DEVICE buffers the caller owns -- pos4
+ * (currNbParticles float4), keys and idx (maxNbParticles u32 each) and startEnd (2*NC u32) --
+ * queued on `stream` (NULL: the null stream) without waiting for it, and with no allocation.
+ * The sort's partner buffers and digit counts live in a device array the library holds on
+ * each GPU it runs on until the process ends (about 3 x 4 B x RTP_NOALLOC_MAX_PARTICLES);
+ * maxNbParticles above RTP_NOALLOC_MAX_PARTICLES is RTP_ERR_INVALID.  Every call on one GPU
+ * uses that GPU's one array, so calls that overlap on one GPU (two streams or two threads)
+ * race; calls in one stream are ordered and safe. */
+#define RTP_NOALLOC_MAX_PARTICLES 2097152u
+int rtp_grid_build_noalloc(const rtp_model_desc* desc, const float* pos4, uint32_t* keys,
+                           uint32_t* idx, uint32_t* startEnd, void* stream);
 
 /* ============================ Self-checks ============================== */
 /* No counterpart in the reference.  The neighbour passes evaluate correctly rounded square

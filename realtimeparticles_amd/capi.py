@@ -254,6 +254,7 @@ _SIGS = {
     "rtp_batch_sizes": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _vp, _vp], _int),
     "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
+    "rtp_grid_build_noalloc": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp, _vp], _int),
     "rtp_selfcheck_sqrt": ([_int, _u32, _u32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_u32)],
                            _int),
     "rtp_selfcheck_division": ([_int, ctypes.c_int32, ctypes.c_int32, _u32, _u32, _u32,
@@ -868,6 +869,19 @@ def grid_build(pos, max_n: int, box, res, max_parts_in_cell: int = 100, device: 
     _check(lib().rtp_grid_build(ctypes.byref(d), pos.ctypes.data, keys.ctypes.data,
                                 idx.ctypes.data, se.ctypes.data), "rtp_grid_build")
     return keys, idx, se
+
+
+NOALLOC_MAX_PARTICLES = 2097152  # RTP_NOALLOC_MAX_PARTICLES
+
+
+def grid_build_noalloc(pos_dev: int, n: int, max_n: int, box, res, keys_dev: int,
+                       idx_dev: int, start_end_dev: int, stream: int = 0,
+                       max_parts_in_cell: int = 100, device: int = 0) -> None:
+    """rtp_grid_build on device buffers (addresses as ints), queued on `stream`; allocates
+    nothing.  The caller synchronizes the stream before reading the outputs."""
+    d = _desc(max_n, n, box, res, 0.0, max_parts_in_cell, device)
+    _check(lib().rtp_grid_build_noalloc(ctypes.byref(d), pos_dev or None, keys_dev, idx_dev,
+                                        start_end_dev, stream or None), "rtp_grid_build_noalloc")
 
 
 def selfcheck_sqrt(lo: int, hi: int, device: int = 0):

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -207,6 +208,42 @@ struct RenderState {
     return RTP_OK;
   }
 };
+
+// -------------------------------------------------------------- no-allocation grid build
+
+// The sort's partner key and index buffers and its digit counts for rtp_grid_build_noalloc:
+// one array per GPU, held until the process ends.  Host code never passes `scratch` itself
+// (its name on the host is a stand-in, not a device address); it passes the address
+// hipGetSymbolAddress returns on each device.
+constexpr uint32_t kNoAllocMax = RTP_NOALLOC_MAX_PARTICLES;
+constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocMax + count_capacity(kNoAllocMax);
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+
+// The current device's copy of `scratch`, looked up once per device (the device must be
+// current).  Returns nullptr after recording the error.
+uint32_t* scratch_on_current_device()
+{
+  static std::mutex mu;
+  static uint32_t* ptr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fail(RTP_ERR_HIP, "hipGetDevice failed or device ordinal too large");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail(RTP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  return ptr[dev];
+}
 
 }  // namespace
 }  // namespace rtp
@@ -1984,6 +2021,67 @@ int rtp_grid_build(const rtp_model_desc* d, const float* pos4, uint32_t* keys, u
   prof.destroy();
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+int rtp_grid_build_noalloc(const rtp_model_desc* d, const float* pos4, uint32_t* keys,
+                           uint32_t* idx, uint32_t* startEnd, void* stream)
+{
+  RTP_REQUIRE(d && keys && idx && startEnd && (d->currNbParticles == 0 || pos4), "NULL argument");
+  RTP_REQUIRE(d->maxNbParticles > 0 && d->currNbParticles <= d->maxNbParticles,
+              "particle counts out of range");
+  if (d->maxNbParticles > kNoAllocMax)
+    return fail(RTP_ERR_INVALID, "rtp_grid_build_noalloc: maxNbParticles " +
+                                     std::to_string(d->maxNbParticles) + " exceeds the library's " +
+                                     std::to_string(kNoAllocMax) + "-slot scratch array");
+  int rc = check_device(d->device);
+  if (rc)
+    return rc;
+  GridConsts g{};
+  if ((rc = make_grid(d, 100u, &g)))
+    return rc;
+  uint32_t* const base = scratch_on_current_device();
+  if (!base)
+    return RTP_ERR_HIP;
+  const uint32_t n = d->currNbParticles, maxN = d->maxNbParticles;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* const keyScratch = base;
+  uint32_t* const idxScratch = base + kNoAllocMax;
+  uint32_t* const counts = base + 2 * (size_t)kNoAllocMax;
+
+  // resetCellIDs over maxN, the identity permutation past the sorted range, then the keys of
+  // the n positions into whichever buffer the sort starts from, so that its result lands in
+  // the caller's keys and idx.  No scratch word needs to start at zero: every count a sort
+  // pass reads is written earlier in that pass, and the partners are read only where a pass
+  // has written them.
+  const SortPlan plan = make_sort_plan(n, max_active_key(g), 0);
+  const bool inB = sort_result_in_b(plan);
+  uint32_t* const keyA = inB ? keyScratch : keys;
+  uint32_t* const keyB = inB ? keys : keyScratch;
+  uint32_t* const idxA = inB ? idxScratch : idx;
+  uint32_t* const idxB = inB ? idx : idxScratch;
+  launch_reset_cell_ids(keys, maxN, g.numCells, s);
+  launch_fill_identity(idx, n, maxN, s);
+  launch_fill_keys_and_reset_cells(reinterpret_cast<const float4*>(pos4), n, keyA,
+                                   reinterpret_cast<uint2*>(startEnd), g, s);
+  bool landedInB = false;
+  if ((rc = sort_pairs(plan, keyA, keyB, idxA, idxB, counts, &landedInB, s)))
+    return rc;
+  if (landedInB != inB)
+    return fail(RTP_ERR_STATE, "rtp_grid_build_noalloc: sort result in an unexpected buffer");
+  if (n) {
+    ReorderArgs a{};
+    a.key = keys;
+    a.idx = idx;
+    a.sortN = n;
+    a.n = n;
+    a.maxN = maxN;
+    a.startEnd = reinterpret_cast<uint2*>(startEnd);
+    a.g = g;
+    launch_reorder(a, s);
+  }
+  if (hipGetLastError() != hipSuccess)
+    return fail(RTP_ERR_HIP, "grid launch failed");
+  return RTP_OK;
 }
 
 }  // extern "C"

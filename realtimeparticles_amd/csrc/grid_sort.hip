@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(const uint32_t* __restrict
 // per-segment sums, a scan of the segment sums, and a rescan of every segment from its base.
 // A thread owns 16 consecutive words (four 16-byte loads), a block owns kScanSeg words.
 constexpr uint32_t kScanPer = 16;
-constexpr uint32_t kScanSeg = 256 * kScanPer;
+static_assert(kScanSeg == 256 * kScanPer, "a scan block owns 256 threads x kScanPer words");
 
 __device__ __forceinline__ void load16(const uint32_t* __restrict__ data, uint32_t b, uint32_t m,
                                        uint32_t v[kScanPer])
@@ -525,13 +525,6 @@ void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, u
 
 uint32_t scan_partials_needed(uint32_t m16) { return (m16 + kScanSeg - 1u) / kScanSeg; }
 
-size_t count_capacity(uint32_t maxN)
-{
-  const size_t subtiles = std::max<size_t>((maxN + (size_t)kSortItems * 64 - 1) / ((size_t)kSortItems * 64), 1);
-  const size_t table = (size_t)2048 * subtiles;
-  return table + table / kScanSeg + 64;
-}
-
 SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
 {
   SortPlan p;
@@ -570,14 +563,26 @@ bool small_sort_fits()
   return ok[dev] == 1;
 }
 
+// Passes of the one-workgroup sort: 4-bit digits over the significant bits.
+static int small_sort_passes(const SortPlan& p) { return (p.keyBits + 3) / 4; }
+
+static bool use_small_sort(const SortPlan& p) { return p.n <= kSmallSortMax && small_sort_fits(); }
+
+bool sort_result_in_b(const SortPlan& p)
+{
+  if (p.n == 0)
+    return false;
+  return ((use_small_sort(p) ? small_sort_passes(p) : p.passes) & 1) != 0;
+}
+
 int sort_pairs(const SortPlan& p, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA, uint32_t* idxB,
                uint32_t* counts, bool* resultInB, hipStream_t s)
 {
   *resultInB = false;
   if (p.n == 0)
     return RTP_OK;
-  if (p.n <= kSmallSortMax && small_sort_fits()) {  // one workgroup, all passes in LDS
-    const int passes = (p.keyBits + 3) / 4;
+  if (use_small_sort(p)) {  // one workgroup, all passes in LDS
+    const int passes = small_sort_passes(p);
     if (p.n <= 4096u)
       hipLaunchKernelGGL(k_sort_small<8>, dim3(1), dim3(kSmallThreads), 0, s, keyA, keyB, idxA, idxB,
                          p.n, passes);

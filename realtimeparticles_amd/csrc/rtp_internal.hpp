@@ -110,14 +110,25 @@ struct SortPlan {
 // Sorts of at most this many keys run in one workgroup, in LDS (all passes in one launch).
 constexpr uint32_t kSmallSortMax = 8192;
 constexpr int kSortItems = 32;  // keys per lane in one subtile
+// Words of the count table one block of the multi-block scan owns (256 threads x 16 words).
+constexpr uint32_t kScanSeg = 4096;
 // Words of the counts buffer a sort of up to maxN keys needs (any digit width): the
 // digit-major count table (at most 2048 digits x subtiles) followed by its scan's segment sums.
-size_t count_capacity(uint32_t maxN);
+constexpr size_t count_capacity(uint32_t maxN)
+{
+  const size_t per = (size_t)kSortItems * 64;
+  const size_t subtiles = maxN ? (maxN + per - 1) / per : 1;
+  const size_t table = (size_t)2048 * subtiles;
+  return table + table / kScanSeg + 64;
+}
 SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride);
 // keys/idx ping-pong through (keyA, idxA) <-> (keyB, idxB); returns whether the sorted
 // result ended in the B buffers.  idxIn == nullptr means the identity permutation.
 int sort_pairs(const SortPlan& plan, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA,
                uint32_t* idxB, uint32_t* counts, bool* resultInB, hipStream_t s);
+// Whether sort_pairs(plan, ...) on the current device leaves its result in the B buffers
+// (known before the launch, so a caller can put its own output buffers on that side).
+bool sort_result_in_b(const SortPlan& plan);
 
 void launch_fill_keys_and_reset_cells(const float4* pos, uint32_t n, uint32_t* key,
                                       uint2* startEnd, const GridConsts& g, hipStream_t s);
