@@ -267,10 +267,15 @@ int rtp_mlp_train(rtp_mlp* h, const float* feat, const float* target, uint32_t n
                   uint32_t batch, float lr, uint32_t seed, float* epochLoss);
 /* Use the network for the first of learnedIters Jacobi iterations of every substep (NULL
  * switches back to the plain solver).  With fallbackIters > 0, a density pass after those
- * iterations measures the mean |rho/rho0 - 1|; when it exceeds tau, fallbackIters plain
- * iterations follow. */
+ * iterations measures every particle's |rho/rho0 - 1|; the particles above tau, and every
+ * particle within h of one of them, run fallbackIters plain iterations (the others keep
+ * their positions and densities, and their lambda reads 0).  No particle above tau: the
+ * substep equals one without fallback, bit for bit. */
 int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters,
                               uint32_t fallbackIters, float tau);
+/* Particles above tau (seeds) and in the fallback set (seeds and their neighbours) of the
+ * last learned substep with fallback; waits for the handle's stream. */
+int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles);
 /* Run a post-correction density pass in every substep, plain or learned, after any
  * fallback iterations (diagnostics: the error the substep leaves). */
 int rtp_fluids_set_residual_pass(rtp_fluids* h, int enable);

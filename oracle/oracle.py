@@ -174,6 +174,7 @@ def lib():
         L.orc_fld_xsph.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
         L.orc_fld_update_pos.argtypes = [_vp, _vp, _u32]
         L.orc_fld_features.argtypes = [_vp, _vp, _vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_fallback_set.argtypes = [_vp, _vp, _vp, _f32, _f32, _vp, _u32, P(Consts)]
         L.orc_bd_rules3d.argtypes = [_vp, _vp, _vp, P(BoidsParams), _vp, _u32, P(Consts)]
         L.orc_bd_update_vel.argtypes = [_vp, _f32, _f32, _vp, _u32]
         L.orc_bd_update_pos_wall.argtypes = [_vp, _f32, _vp, _u32, P(Consts)]
@@ -293,6 +294,79 @@ class OracleFluids:
                 rc = lib().orc_fluids_update(ctypes.byref(self.s))
             if rc:
                 raise RuntimeError(f"orc_fluids_update failed ({rc})")
+
+
+    def learned_step(self, network, learned_iters: int, fallback_iters: int = 0,
+                     tau: float = 0.0) -> np.ndarray:
+        """One learned substep (no reference counterpart), composed of the reference's
+        kernels in the order of Fluids::update (physics/ocl/Fluids.cpp:400-471):
+
+        - predict, cell keys, sort, permutation, cell ranges (:412-424);
+        - learned_iters Jacobi iterations (:427-439) whose first takes its lambda from
+          network(features) -- the 8 inputs of the first density pass (orc_fld_features);
+        - with fallback_iters > 0: the post-correction density, the fallback set
+          (orc_fallback_set at tau) and fallback_iters Jacobi iterations that correct only the
+          set's particles, with every other particle's lambda 0 (its density is kept; an
+          empty set changes nothing);
+        - the velocity update, vorticity confinement, XSPH and commit (:442-457).
+
+        Returns the fallback set's flags: 2 seed, 1 other member, 0 outside (all 0 without
+        fallback)."""
+        L, c, p = lib(), ctypes.byref(self.c), ctypes.byref(self.p)
+        n, max_n = self.n, self.max_n
+        L.orc_fld_predict(ptr(self.pos), ptr(self.vel), p, ptr(self.pred), n, c)
+        L.orc_fill_cell_ids(ptr(self.pred), ptr(self.key), n, c)
+        if L.orc_radix_sort(ptr(self.key), ptr(self.idx), max_n):
+            raise RuntimeError("orc_radix_sort failed")
+        for buf in (self.pos, self.vel, self.pred):
+            L.orc_permute_f4(ptr(self.idx), ptr(buf), ptr(self.tmp), max_n)
+        L.orc_reset_start_end(ptr(self.se), c)
+        L.orc_fill_start(ptr(self.key), ptr(self.se), n, c)
+        L.orc_fill_end(ptr(self.key), ptr(self.se), n, max_n, c)
+        L.orc_adjust_end(ptr(self.se), c)
+        for it in range(learned_iters):
+            L.orc_fld_boundary(ptr(self.pred), n, c)
+            L.orc_fld_density(ptr(self.pred), ptr(self.se), p, ptr(self.density), n, c)
+            L.orc_fld_lambda(ptr(self.pred), ptr(self.density), ptr(self.se), p, ptr(self.lam),
+                             n, c)
+            if it == 0:
+                feat = np.zeros((n, 8), np.float32)
+                L.orc_fld_features(ptr(self.pred), ptr(self.density), ptr(self.se),
+                                   ptr(self.lam), ptr(self.vel), p, ptr(feat), n, c)
+                self.lam[:n] = np.asarray(network(feat), np.float32)
+            L.orc_fld_correction(ptr(self.lam), ptr(self.se), ptr(self.pred), p, ptr(self.corr),
+                                 n, c)
+            L.orc_fld_correct_pos(ptr(self.corr), ptr(self.pred), n)
+        act = np.zeros(n, np.uint8)
+        if fallback_iters > 0:
+            res = np.zeros(max_n, np.float32)
+            L.orc_fld_density(ptr(self.pred), ptr(self.se), p, ptr(res), n, c)
+            L.orc_fallback_set(ptr(self.pred), ptr(self.se), ptr(res), float(self.p.restDensity),
+                               float(tau), ptr(act), n, c)
+            on = act > 0
+            dens = np.zeros(max_n, np.float32)
+            lam = np.zeros(max_n, np.float32)
+            for _ in range(fallback_iters if on.any() else 0):
+                L.orc_fld_boundary(ptr(self.pred), n, c)
+                L.orc_fld_density(ptr(self.pred), ptr(self.se), p, ptr(dens), n, c)
+                L.orc_fld_lambda(ptr(self.pred), ptr(dens), ptr(self.se), p, ptr(lam), n, c)
+                self.density[:n][on] = dens[:n][on]
+                self.lam[:n] = np.where(on, lam[:n], np.float32(0.0))
+                L.orc_fld_correction(ptr(self.lam), ptr(self.se), ptr(self.pred), p,
+                                     ptr(self.corr), n, c)
+                moved = self.pred.copy()
+                L.orc_fld_correct_pos(ptr(self.corr), ptr(moved), n)
+                self.pred[:n][on] = moved[:n][on]
+        L.orc_fld_update_vel(ptr(self.pred), ptr(self.pos), p, ptr(self.vel), n, c)
+        if self.p.isVorticityConfEnabled:
+            L.orc_fld_vorticity(ptr(self.pred), ptr(self.se), ptr(self.vel), p, ptr(self.vort),
+                                n, c)
+            L.orc_fld_confinement(ptr(self.pred), ptr(self.se), ptr(self.vort), p, ptr(self.vel),
+                                  n, c)
+            self.vel_in[:] = self.vel
+            L.orc_fld_xsph(ptr(self.pred), ptr(self.se), ptr(self.vel_in), p, ptr(self.vel), n, c)
+        L.orc_fld_update_pos(ptr(self.pred), ptr(self.pos), n)
+        return act
 
 
 class OracleBoids:

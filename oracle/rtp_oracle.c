@@ -846,6 +846,28 @@ static int boids_physics(orc_boids* s, orc_f4* col)
 /* physics/ocl/Boids.cpp:323-384 without the rendering steps. */
 int orc_boids_update(orc_boids* s) { return boids_physics(s, NULL); }
 
+/* The learned substep's fallback set (no reference counterpart; the rule the library applies
+ * after its learned iterations).  A particle whose |density/rho0 - 1| exceeds tau is a seed;
+ * every particle a seed's 27-cell walk (fluids.cl:101-123) meets at length(p_seed - p_j) < h
+ * joins the set.  act: n flags (2 = seed, 1 = other member, 0 = outside). */
+void orc_fallback_set(const orc_f4* pred, const orc_u2* se, const float* density, float rho0,
+                      float tau, uint8_t* act, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+  memset(act, 0, n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!(fabsf(density[i] / rho0 - 1.0f) > tau))
+      continue;
+    act[i] = 2;
+    const orc_f4 pos = pred[i];
+    ORC_FLD_FOR_NEIGHBOURS(c, pos, e)
+    {
+      if (len4(sub4(pos, pred[e])) < h && act[e] == 0)
+        act[e] = 1;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------------------
  * Rendering side of a frame
  * ---------------------------------------------------------------------------------- */

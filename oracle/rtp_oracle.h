@@ -126,6 +126,11 @@ void orc_fld_update_pos(const orc_f4* pred, orc_f4* pos, uint32_t n);
 void orc_fld_features(const orc_f4* pred, const float* density, const orc_u2* se,
                       const float* lambda, const orc_f4* vel, const orc_fluid_params* p,
                       float* feat, uint32_t n, const orc_consts* c);
+/* The learned substep's fallback set: seeds are the particles with |density/rho0 - 1| > tau;
+ * every particle a seed's 27-cell walk meets at length(p_seed - p_j) < h joins them.
+ * act: n flags, 2 = seed, 1 = other member, 0 = outside. */
+void orc_fallback_set(const orc_f4* pred, const orc_u2* se, const float* density, float rho0,
+                      float tau, uint8_t* act, uint32_t n, const orc_consts* c);
 
 /* ---- boids.cl ---- */
 void orc_bd_rules3d(const orc_f4* pos, const orc_f4* vel, const orc_u2* se,

@@ -205,6 +205,7 @@ _SIGS = {
     "rtp_mlp_gradients": ([_vp, _vp, _vp, _u32, _vp, _vp], _int),
     "rtp_mlp_train": ([_vp, _vp, _vp, _u32, _u32, _u32, _f32, _u32, _vp], _int),
     "rtp_fluids_set_warm_start": ([_vp, _vp, _u32, _u32, _f32], _int),
+    "rtp_fluids_fallback_count": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "rtp_fluids_set_residual_pass": ([_vp, _int], _int),
     "rtp_fluids_residual_error": ([_vp, _vp], _int),
     "rtp_fluids_record_teacher": ([_vp, _u32, _vp, _vp], _int),
@@ -513,6 +514,13 @@ class Fluids(_RenderFields):
         out = np.zeros(3, np.float32)
         _check(lib().rtp_fluids_residual_error(self._h, out.ctypes.data), "residual_error")
         return out
+
+    def fallback_count(self):
+        """(seeds, particles) of the last learned substep's fallback set."""
+        a, b = _u32(0), _u32(0)
+        _check(lib().rtp_fluids_fallback_count(self._h, ctypes.byref(a), ctypes.byref(b)),
+               "fallback_count")
+        return int(a.value), int(b.value)
 
     def record_teacher(self, teacher_iters: int = 20):
         feat = np.zeros((self.n, 8), np.float32)

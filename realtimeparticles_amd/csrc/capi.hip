@@ -346,7 +346,8 @@ struct rtp_fluids {
   float* resMetric = nullptr; // max, sum, max compression of the residual pass + partials
   rtp_mlp* mlp = nullptr;
   uint32_t learnedIters = 0, fallbackIters = 0;
-  float tau = 0.0f;
+  float tau = 0.0f;  // per-particle fallback threshold on |rho/rho0 - 1|
+  FallbackSet fb;    // allocated with the first fallback setting
   bool residualPass = false;
   uint32_t teacherIters = 0;  // > 0 while rtp_fluids_record_teacher runs its substep
   // checkpoint (rtp_fluids_save_checkpoint)
@@ -418,8 +419,6 @@ struct rtp_fluids {
     la.feat = feat;
     la.vel = vel[cur];
     la.lamAcc = lamAcc;
-    la.gate = resMetric + 1;
-    la.tau = tau * (float)n;
     int in = 1 - P;
     // Thread order of the neighbour passes: the grid's order for the first pass; with
     // L.resort, after every correction the particles sorted by the cells they now sit in.
@@ -476,24 +475,19 @@ struct rtp_fluids {
                launch_pbf_density_lambda(pred[in], G.startEnd, resDensity, nullptr, n, c, L, s, kDensOnly, la, po));
       launch_density_error(resDensity, n, c.p.restDensity, resMetric, s);
     };
-    const bool gated = learned && fallbackIters > 0;
-    if (gated) {
+    // per-particle fallback: the particles the learned iterations left above tau, and their
+    // neighbours within h, run plain iterations over a compacted list; the rest keep their
+    // positions (the thread order of the later passes stays the last learned one's)
+    if (learned && fallbackIters > 0) {
       residual();
+      RTP_PROF(prof, "pbf_fallback_set", s,
+               launch_fallback_set(pred[in], G.startEnd, resDensity, tau, n, c, L, fb, s));
       for (uint32_t f = 0; f < fallbackIters; ++f) {
-        const int mode = kGated | (f == 0 ? kClampIn : 0);
-        const int last = (f + 1 == fallbackIters) ? kLastOnly : kNextIter;
-        // a gated pair reads the same (clamped-on-load) positions: the correction replays
-        // the density pass's lists; with the gate closed neither walks nor reads them
         RTP_PROF(prof, "pbf_fallback", s,
-                 launch_pbf_density_lambda(pred[in], G.startEnd, density, lambda, n, c, L, s,
-                                           lists ? (mode | kNbRecord) : mode, la, po));
-        RTP_PROF(prof, "pbf_fallback", s,
-                 launch_pbf_correct(pred[in], lambda, G.startEnd, pred[1 - in], pos[cur], vel[cur],
-                                    nullptr, n, last, c, L, s, lists ? (mode | kNbReplay) : mode,
-                                    la, pc));
+                 launch_fallback_iteration(pred[in], pred[1 - in], density, lambda, G.startEnd, n,
+                                           f == 0, f + 1 == fallbackIters, c, L, fb,
+                                           lists ? &po.nb : nullptr, s));
         in = 1 - in;
-        if ((rc = order_threads()))
-          return rc;
       }
     }
     if (residualPass)  // after any fallback iterations: the error the substep leaves
@@ -659,6 +653,10 @@ struct rtp_fluids {
     dev_free(ckCam);
     render.release();
     dev_free(feat);
+    dev_free(fb.act);
+    dev_free(fb.seeds);
+    dev_free(fb.list);
+    dev_free(fb.count);
     dev_free(lamAcc);
     dev_free(resDensity);
     dev_free(resMetric);
@@ -1883,11 +1881,41 @@ int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters
   RTP_REQUIRE(learnedIters <= 1000 && fallbackIters <= 1000, "iteration count out of range");
   if (mlp && mlp->device != h->device)
     return fail(RTP_ERR_INVALID, "network and model live on different devices");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (mlp && fallbackIters > 0 && !h->fb.act) {
+    const uint32_t maxN = h->grid.maxN;
+    FallbackSet& fb = h->fb;
+    int rc = 0;
+    if ((rc = dev_alloc(&fb.act, maxN)) || (rc = dev_alloc(&fb.seeds, maxN)) ||
+        (rc = dev_alloc(&fb.list, maxN)) || (rc = dev_alloc(&fb.count, 4))) {
+      dev_free(fb.act);
+      dev_free(fb.seeds);
+      dev_free(fb.list);
+      dev_free(fb.count);
+      fb = FallbackSet();
+      return rc;
+    }
+    launch_fill_u32(fb.count, 4, 0u, h->stream);
+  }
   h->mlp = mlp;
   h->learnedIters = mlp ? learnedIters : 0;
   h->fallbackIters = mlp ? fallbackIters : 0;
   h->tau = tau;
   h->drop_graph();
+  return RTP_OK;
+}
+
+int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles)
+{
+  RTP_REQUIRE(h && seeds && particles, "NULL argument");
+  uint32_t v[2] = { 0u, 0u };
+  if (h->fb.count) {
+    RTP_HIP_TRY(hipSetDevice(h->device));
+    RTP_HIP_TRY(hipMemcpyAsync(v, h->fb.count, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  *seeds = v[0];
+  *particles = v[1];
   return RTP_OK;
 }
 

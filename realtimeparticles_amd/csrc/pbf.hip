@@ -613,7 +613,6 @@ __global__ __launch_bounds__(256) void k_predict(const float4* __restrict__ pos,
 //   kAccum    : add lambda into la.lamAcc (teacher runs), or start it when la.accumInit
 //   kDensOnly : density only, into `density` (the post-correction residual pass)
 //   kClampIn  : clamp positions on load (first fallback iteration)
-//   kGated    : do nothing unless *la.gate > la.tau (fallback iterations)
 template <int MODE>
 __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __restrict__ pred,
                                                     const uint2* __restrict__ se,
@@ -622,8 +621,6 @@ __device__ __forceinline__ void density_lambda_body(uint32_t i, const float4* __
                                                     const PbfConsts& c, const LearnArgs& la,
                                                     const WalkCtx& T)
 {
-  if ((MODE & kGated) && !(la.gate[0] > la.tau))
-    return;
   constexpr bool CL = (MODE & kClampIn) != 0;
   const float4 p = ldp<CL>(pred, i, c);
   float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f, count = 0.0f;
@@ -720,8 +717,7 @@ __global__ __launch_bounds__(256) void k_density_lambda(const float4* __restrict
 //   kNextIter : fld_applyBoundaryCondition of the next iteration
 //   kLastVel  : fld_updateVel (:263-273) and, without vorticity, fld_updatePosition (:444-450)
 //   kLastOnly : nothing more (the learned step updates the velocity after its fallback)
-// MODE bits kClampIn / kGated as for k_density_lambda; a gated pass that does not run
-// copies its input positions through, so the buffer sequence does not depend on the gate.
+// MODE bit kClampIn as for k_density_lambda.
 // The cell the next pass's walk starts from, the key of the next thread order.
 __device__ __forceinline__ void record_cell(uint32_t i, const float4 q, const PbfConsts& c,
                                             const PassOrder& po)
@@ -741,12 +737,7 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
                                              int last, const PbfConsts& c, const LearnArgs& la,
                                              const WalkCtx& T, const PassOrder& po)
 {
-  if ((MODE & kGated) && !(la.gate[0] > la.tau)) {
-    const float4 q = pred[i];
-    predOut[i] = q;
-    record_cell(i, q, c, po);
-    return;
-  }
+  (void)la;
   constexpr bool CL = (MODE & kClampIn) != 0;
   const float4 p = ldp<CL>(pred, i, c);
   const float li = lambda[i];
@@ -822,6 +813,130 @@ __global__ __launch_bounds__(256) void k_correct(const float4* __restrict__ pred
   RTP_NB_CTX
   if (t < n)
     correct_body<MODE>(po.particle(t), pred, lambda, se, predOut, pos, vel, posOut, last, c, la, W, po);
+}
+
+// ------------------------------------------------- learned substep: per-particle fallback
+// No reference counterpart.  After the learned iterations, particles whose post-correction
+// |rho/rho0 - 1| exceeds tau seed the fallback set; a seed and every particle within h of it
+// in the seed's walk run the plain fallback iterations, over a compacted list.
+
+// Append the lanes with `on` to list at count[0] (one atomic per wave).
+__device__ __forceinline__ void wave_append(bool on, uint32_t value, uint32_t* __restrict__ list,
+                                            uint32_t* __restrict__ count)
+{
+  const uint64_t m = __builtin_amdgcn_ballot_w64(on);
+  if (m == 0)
+    return;
+  const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader)
+    base = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
+  base = __shfl(base, (int)leader, 64);
+  if (on)
+    list[base + below] = value;
+}
+
+// Seeds: |density / rho0 - 1| > tau (the expression of the density-error metric); their
+// flags in act, their indices in fs.seeds.
+__global__ __launch_bounds__(256) void k_fb_mark(const float* __restrict__ resDensity, uint32_t n,
+                                                 float rho0, float tau, FallbackSet fs)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool seed = false;
+  if (t < n) {
+    seed = fabsf(resDensity[t] / rho0 - 1.0f) > tau;
+    fs.act[t] = seed ? 1u : 0u;
+  }
+  wave_append(seed, t, fs.seeds, fs.count);
+}
+
+// Every particle the walk of a seed meets within h (r = sqrt(d2) < h, d2 as the density
+// pass computes it) joins the set; the seed itself (r = 0) is already in.
+__global__ __launch_bounds__(256) void k_fb_dilate(const float4* __restrict__ pred,
+                                                   const uint2* __restrict__ se, PbfConsts c,
+                                                   uint32_t qdepth, FallbackSet fs)
+{
+  RTP_PLAIN_CTX(qdepth)
+  const uint32_t cnt = fs.count[0];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t e = fs.seeds[k];
+    walk27(pred[e], pred, se, c, [](uint32_t) { return 0; },
+           [&](uint32_t j, int, float, float, float, float d2) {
+             if (__builtin_sqrtf(d2) < c.h)
+               fs.act[j] = 1u;
+           }, W);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fb_compact(uint32_t n, FallbackSet fs)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = t < n && fs.act[t] != 0u;
+  wave_append(on, t, fs.list, fs.count + 1);
+}
+
+// Positions of the next fallback iteration for the particles outside the set (the set's
+// own are overwritten by the correction): copied, clamped to the walls on the first
+// iteration as fld_applyBoundaryCondition clamps every particle, and lambda outside the set
+// is 0.  An empty set leaves the substep as it was: positions copied as they are.
+__global__ __launch_bounds__(256) void k_fb_carry(const float4* __restrict__ predIn,
+                                                  float4* __restrict__ predOut,
+                                                  float* __restrict__ lambda, uint32_t n,
+                                                  int clamp, PbfConsts c, FallbackSet fs)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n)
+    return;
+  const bool any = fs.count[1] != 0u;
+  const float4 q = predIn[t];
+  predOut[t] = (clamp && any) ? clamp_pos(q, c) : q;
+  if (any)
+    lambda[t] = 0.0f;
+}
+
+// fld_computeDensity + fld_computeConstraintFactor over the set (list position k is the
+// thread's survivor-list slot).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_density_lambda_local(const float4* __restrict__ pred,
+                                                              const uint2* __restrict__ se,
+                                                              float* __restrict__ density,
+                                                              float* __restrict__ lambda,
+                                                              PbfConsts c, uint32_t qdepth,
+                                                              FallbackSet fs, NbList nb)
+{
+  RTP_PLAIN_CTX(qdepth)
+  W.nb = nb;
+  const LearnArgs la;
+  const uint32_t cnt = fs.count[1];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    W.slot = k;
+    density_lambda_body<MODE>(fs.list[k], pred, se, density, lambda, c, la, W);
+  }
+}
+
+// fld_computeConstraintCorrection + fld_correctPosition over the set (and, unless `last`,
+// the next iteration's boundary condition).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_correct_local(const float4* __restrict__ pred,
+                                                       const float* __restrict__ lambda,
+                                                       const uint2* __restrict__ se,
+                                                       float4* __restrict__ predOut, int last,
+                                                       PbfConsts c, uint32_t qdepth,
+                                                       FallbackSet fs, NbList nb)
+{
+  RTP_PLAIN_CTX(qdepth)
+  W.nb = nb;
+  const LearnArgs la;
+  const PassOrder po;
+  const uint32_t cnt = fs.count[1];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    W.slot = k;
+    correct_body<MODE>(fs.list[k], pred, lambda, se, predOut, nullptr, nullptr, nullptr, last, c,
+                       la, W, po);
+  }
 }
 
 // fld_updateVel (+ fld_updatePosition) for a step with zero Jacobi iterations.
@@ -1370,12 +1485,8 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
     RTP_DL(kAccum)
     RTP_DL(kFeatures | kAccum)
     RTP_DL(kDensOnly)
-    RTP_DL(kGated)
-    RTP_DL(kGated | kClampIn)
     RTP_DL(kNbRecord)
     RTP_DL(kFeatures | kNbRecord)
-    RTP_DL(kGated | kNbRecord)
-    RTP_DL(kGated | kClampIn | kNbRecord)
   default: break;
   }
 #undef RTP_DL
@@ -1403,14 +1514,70 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
     break;
   switch (mode) {
     RTP_CO(0)
-    RTP_CO(kGated)
-    RTP_CO(kGated | kClampIn)
     RTP_CO(kNbReplay)
-    RTP_CO(kGated | kNbReplay)
-    RTP_CO(kGated | kClampIn | kNbReplay)
   default: break;
   }
 #undef RTP_CO
+}
+
+// Threads of a pass over a compacted list: the list holds at most n entries, and a
+// grid-stride loop covers any count with at most this many threads.
+static uint32_t list_blocks(uint32_t n, uint32_t block)
+{
+  const uint32_t cap = (1u << 18) / block;
+  const uint32_t b = blocks_for(n, block);
+  return b < cap ? b : cap;
+}
+
+void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
+                         float tau, uint32_t n, const PbfConsts& c, const Launch& L,
+                         const FallbackSet& fs, hipStream_t s)
+{
+  launch_fill_u32(fs.count, 2, 0u, s);
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_fb_mark, dim3(blocks_for(n, 256)), dim3(256), 0, s, resDensity, n,
+                     c.p.restDensity, tau, fs);
+  hipLaunchKernelGGL(k_fb_dilate, dim3(list_blocks(n, L.block)), dim3(L.block), queue_bytes(L), s,
+                     pred, startEnd, c, (uint32_t)L.queue, fs);
+  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, fs);
+}
+
+void launch_fallback_iteration(const float4* predIn, float4* predOut, float* density,
+                               float* lambda, const uint2* startEnd, uint32_t n, bool first,
+                               bool last, const PbfConsts& c, const Launch& L,
+                               const FallbackSet& fs, const NbList* nb, hipStream_t s)
+{
+  if (n == 0)
+    return;
+  hipLaunchKernelGGL(k_fb_carry, dim3(blocks_for(n, 256)), dim3(256), 0, s, predIn, predOut,
+                     lambda, n, first ? 1 : 0, c, fs);
+  const dim3 g(list_blocks(n, L.block)), b(L.block);
+  const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
+  const NbList lists = nb ? *nb : NbList();
+  const int lastMode = last ? kLastOnly : kNextIter;
+  // the correction replays the density pass's lists: the same positions, clamped the same way
+  if (nb && first) {
+    hipLaunchKernelGGL(k_density_lambda_local<kClampIn | kNbRecord>, g, b, lds, s, predIn,
+                       startEnd, density, lambda, c, q, fs, lists);
+    hipLaunchKernelGGL(k_correct_local<kClampIn | kNbReplay>, g, b, lds, s, predIn, lambda,
+                       startEnd, predOut, lastMode, c, q, fs, lists);
+  } else if (nb) {
+    hipLaunchKernelGGL(k_density_lambda_local<kNbRecord>, g, b, lds, s, predIn, startEnd,
+                       density, lambda, c, q, fs, lists);
+    hipLaunchKernelGGL(k_correct_local<kNbReplay>, g, b, lds, s, predIn, lambda, startEnd,
+                       predOut, lastMode, c, q, fs, lists);
+  } else if (first) {
+    hipLaunchKernelGGL(k_density_lambda_local<kClampIn>, g, b, lds, s, predIn, startEnd, density,
+                       lambda, c, q, fs, lists);
+    hipLaunchKernelGGL(k_correct_local<kClampIn>, g, b, lds, s, predIn, lambda, startEnd, predOut,
+                       lastMode, c, q, fs, lists);
+  } else {
+    hipLaunchKernelGGL(k_density_lambda_local<0>, g, b, lds, s, predIn, startEnd, density, lambda,
+                       c, q, fs, lists);
+    hipLaunchKernelGGL(k_correct_local<0>, g, b, lds, s, predIn, lambda, startEnd, predOut,
+                       lastMode, c, q, fs, lists);
+  }
 }
 
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,

@@ -179,13 +179,11 @@ struct LearnArgs {
   const float4* vel = nullptr;       // sorted velocities (speed feature)
   float* lamAcc = nullptr;           // accumulated lambda (kAccum)
   int accumInit = 0;
-  const float* gate = nullptr;       // summed residual density error (kGated)
-  float tau = 0.0f;                  // threshold on *gate (the mean threshold times n)
 };
 // kNbRecord / kNbReplay: the pass records its threads' survivor lists / replays them
 // (NbList below).
 enum PassMode {
-  kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kGated = 16, kNbRecord = 32, kNbReplay = 64
+  kFeatures = 1, kAccum = 2, kDensOnly = 4, kClampIn = 8, kNbRecord = 32, kNbReplay = 64
 };
 // Survivor lists.  Two neighbour passes over the same positions, the same cell table and
 // the same thread order visit the same candidates and run their bodies on the same
@@ -274,6 +272,28 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
                         uint32_t n, int last, const PbfConsts& c, const Launch& L, hipStream_t s,
                         int mode = 0, const LearnArgs& la = LearnArgs(),
                         const PassOrder& po = PassOrder());
+// The learned substep's per-particle fallback (no reference counterpart).  The set: the
+// particles whose post-correction |rho/rho0 - 1| (resDensity) exceeds tau, and every
+// particle within h of one of them in its walk.  act, seeds and list hold maxN entries;
+// count[0] = seeds, count[1] = particles in the set.
+struct FallbackSet {
+  uint32_t* act = nullptr;
+  uint32_t* seeds = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t* count = nullptr;
+};
+void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
+                         float tau, uint32_t n, const PbfConsts& c, const Launch& L,
+                         const FallbackSet& fs, hipStream_t s);
+// One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, with the
+// lambda of every other particle 0 (unless the set is empty) and its position carried from
+// predIn to predOut (clamped to the walls on the first iteration, as every particle's is at
+// an iteration's start).
+// nb: survivor lists recorded by the density pass and replayed by the correction.
+void launch_fallback_iteration(const float4* predIn, float4* predOut, float* density,
+                               float* lambda, const uint2* startEnd, uint32_t n, bool first,
+                               bool last, const PbfConsts& c, const Launch& L,
+                               const FallbackSet& fs, const NbList* nb, hipStream_t s);
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
                            uint32_t n, const PbfConsts& c, hipStream_t s);
 // nb: 0, kNbRecord or kNbReplay (the post passes share their positions and thread order).

@@ -7,8 +7,9 @@ one correction step, the sum over the solver's default four iterations leaves a 
 density error than the four plain iterations themselves; sums over many more iterations
 overshoot (every particle at the free surface keeps a lambda of the same sign while its
 displacement saturates), so the target stays at that count.  A learned substep runs
-`learned_iters` iterations whose first uses the network's lambda, measures the mean
-post-correction density error, and falls back to plain iterations when it exceeds tau.
+`learned_iters` iterations whose first uses the network's lambda; with fallback, every
+particle whose post-correction |rho/rho0 - 1| exceeds tau, and every particle within h of
+one, then runs plain iterations (the others keep their positions).
 
 Everything here runs on the GPU through the C ABI: teacher substeps, the training loop
 (forward, backward and Adam kernels) and the evaluation.
@@ -86,6 +87,7 @@ class Variant:
     residual_max: float
     residual_mean: float
     residual_compression: float
+    fallback_set: tuple = (0, 0)  # (seeds, particles) of the fallback set, learned runs
 
 
 def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup: int = 3,
@@ -127,8 +129,9 @@ def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup:
         m.update(1)
         r = m.residual_error()
         m.set_residual_pass(False)
+        fset = m.fallback_count() if (mlp is not None and fb > 0) else (0, 0)
         out.append(Variant(name, sc.n * steps / el, el * 1e3 / steps, float(r[0]), float(r[1]),
-                           float(r[2])))
+                           float(r[2]), fset))
     m.set_warm_start(None)
     m.close()
     return out

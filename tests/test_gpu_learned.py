@@ -3,14 +3,17 @@
 * the BF16 matrix-core network against the FP64 oracle (oracle/mlp_oracle.py): forward
   values and the gradients of the training loss;
 * the teacher features and accumulated-lambda targets against the C oracle (exact);
-* the fallback gate, and what the trained network gains over the same iteration budget
-  without it.
+* the learned substep with its per-particle fallback against the oracle's composition of
+  the reference kernels (given the network's lambda), bit for bit, at a small size and at
+  the 1,048,576-particle dam break;
+* what the trained network gains over the same iteration budget without it.
 """
 import numpy as np
 import pytest
 
-from helpers import _eq, log, oracle_teacher
+from helpers import _eq, compare_pbf, log, oracle_teacher
 from oracle import mlp_oracle as MO
+from oracle import oracle as O
 from realtimeparticles_amd import capi, learned, scenes
 
 pytestmark = pytest.mark.gpu
@@ -94,43 +97,142 @@ def test_teacher_features_and_targets_match_oracle(mapping):
     gpu.close()
 
 
-def test_fallback_gate():
-    sc = small_scene(22)
-    prm = sc.params("stable")
-    p, _, mu, sigma = rand_net(5, 1)
+def _small_net(seed):
+    """A random network with a small output, so the learned lambda stays near the scale of
+    the plain one."""
+    p, _, _, _ = rand_net(seed, 1)
     mlp = capi.Mlp()
     mlp.set_params(p * 0.1)
     mlp.set_normalization(np.zeros(8, np.float32), np.ones(8, np.float32), 1e-4)
+    return mlp
 
-    def run(fb, tau, diag=False):
-        m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
-        m.set_params(**prm)
-        m.load_state(sc.pos, sc.vel)
-        m.set_warm_start(mlp, 1, fb, tau)
-        m.set_residual_pass(diag)
-        m.update(1)
-        out = m.read_state()[0], m.residual_error()
+
+def _learned_run(sc, prm, mlp, learned_iters, fb, tau, tuning=None, residual=False):
+    """The GPU model after one learned substep, and the oracle's substep from the same
+    input with the network's lambda on the oracle's own features."""
+    gpu = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    gpu.set_params(**prm)
+    gpu.set_nb_jacobi_iters(4)
+    if tuning:
+        gpu.set_tuning(*tuning)
+    gpu.load_state(sc.pos, sc.vel)
+    gpu.set_warm_start(mlp, learned_iters, fb, tau)
+    gpu.set_residual_pass(residual)
+    gpu.update(1)
+    orc = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), 4,
+                         gravity=sc.gravity)
+    orc.upload(sc.pos, sc.vel)
+    act = orc.learned_step(mlp.forward, learned_iters, fb, tau)
+    return gpu, orc, act
+
+
+def _threshold_between(sc, prm, mlp):
+    """A threshold between the mean and the max residual error of learned-1, so that some
+    particles but not all seed the fallback set."""
+    gpu = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    gpu.set_params(**prm)
+    gpu.load_state(sc.pos, sc.vel)
+    gpu.set_warm_start(mlp, 1, 0, 0.0)
+    gpu.set_residual_pass(True)
+    gpu.update(1)
+    r = gpu.residual_error()
+    gpu.close()
+    return float(r[1] + 0.5 * (r[0] - r[1]))
+
+
+@pytest.mark.parametrize("shape", ["default", "tuned"])
+@pytest.mark.parametrize("gate", ["none", "some", "all"])
+def test_learned_substep_matches_oracle(gate, shape):
+    """Learned-1 with 3 fallback iterations against the oracle's learned substep, bit for
+    bit on every field: no particle above tau (the substep equals one without fallback),
+    some, and every particle (the fallback is plain iterations)."""
+    sc = small_scene(22)
+    prm = sc.params("stable")
+    mlp = _small_net(5)
+    tau = {"none": 1e30, "all": -1.0}.get(gate) or _threshold_between(sc, prm, mlp)
+    tuning = (64, capi.LAUNCH_XCD_MAP | capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST, 0, 16) \
+        if shape == "tuned" else None
+    gpu, orc, act = _learned_run(sc, prm, mlp, 1, 3, tau, tuning)
+    log(f"gate {gate} (tau {tau:.4g}), {shape}: oracle set {int((act > 0).sum())} of {sc.n}, "
+        f"{int((act == 2).sum())} seeds; GPU {gpu.fallback_count()}")
+    compare_pbf(gpu, orc, sc.n, 1)
+    assert gpu.fallback_count() == (int((act == 2).sum()), int((act > 0).sum()))
+    seeds, members = gpu.fallback_count()
+    if gate == "none":
+        assert members == 0
+        ref, _, _ = _learned_run(sc, prm, mlp, 1, 0, 0.0, tuning)
+        _eq(gpu.read_state()[0], ref.read_state()[0], "no particle above tau: no fallback")
+        ref.close()
+    elif gate == "all":
+        assert seeds == members == sc.n
+    else:
+        assert 0 < seeds < members < sc.n
+    gpu.close()
+    mlp.close()
+
+
+def test_fallback_residual_pass_changes_nothing():
+    """The diagnostic post-correction pass after the fallback reads the state and leaves it
+    alone, and the fallback lowers the error it measures."""
+    sc = small_scene(22)
+    prm = sc.params("stable")
+    mlp = _small_net(5)
+    tau = _threshold_between(sc, prm, mlp)
+    a, _, _ = _learned_run(sc, prm, mlp, 1, 3, tau, residual=False)
+    b, _, _ = _learned_run(sc, prm, mlp, 1, 3, tau, residual=True)
+    c, _, _ = _learned_run(sc, prm, mlp, 1, 0, tau, residual=True)
+    _eq(b.read_state()[0], a.read_state()[0], "diagnostic pass changes nothing")
+    rb, rc = b.residual_error(), c.residual_error()
+    log(f"residual without fallback {rc}, with {rb}")
+    assert rb[1] < rc[1]
+    for m in (a, b, c):
         m.close()
+    mlp.close()
+
+
+def test_learned_substep_1m_matches_oracle():
+    """Config E at its size: the 1,048,576-particle dam break (128x128x64 lattice), learned-1
+    with 3 fallback iterations at the benchmark's launch shape.  The network's lambda on the
+    oracle's features against the FP64 network oracle (within the bf16 network's tolerance),
+    and everything downstream of it -- positions, velocities, densities, lambdas, predicted
+    positions, the grid and the fallback set -- bit for bit against the oracle."""
+    sc = scenes.pbf_scene("pbf-1m")
+    prm = sc.params("stable")
+    mlp = _small_net(9)
+    tau = _threshold_between(sc, prm, mlp)
+    captured = {}
+
+    def network(feat):
+        captured["feat"] = feat
+        out = mlp.forward(feat)
+        captured["out"] = out
         return out
 
-    p0, r0 = run(0, 0.0, diag=True)
-    p1, r1 = run(3, 1e30)   # gate closed: identical to no fallback
-    _eq(p1, p0, "gate closed")
-    p2, r2 = run(3, -1.0)   # gate open: three more plain iterations
-    assert not np.array_equal(p2, p0)
-    # the gate's pass precedes the fallback and matches the diagnostic pass bit for bit
-    # (a deterministic reduction)
-    assert np.array_equal(r1, r0) and np.array_equal(r2, r0)
-    # with the diagnostic pass on, the error left after the fallback iterations is read
-    p5, r5 = run(3, -1.0, diag=True)
-    _eq(p5, p2, "diagnostic pass changes nothing")
-    log(f"residual without fallback {r0}, after three fallback iterations {r5}")
-    assert r5[1] < r0[1]
-    # a threshold just below / above the measured mean decides the gate
-    p3, _ = run(3, float(r0[1]) * 0.999)
-    _eq(p3, p2, "gate open at tau < mean")
-    p4, _ = run(3, float(r0[1]) * 1.001)
-    _eq(p4, p0, "gate closed at tau > mean")
+    tuning = (64, capi.LAUNCH_XCD_MAP | capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST, 0, 16)
+    gpu = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    gpu.set_params(**prm)
+    gpu.set_nb_jacobi_iters(4)
+    gpu.set_tuning(*tuning)
+    gpu.load_state(sc.pos, sc.vel)
+    gpu.set_warm_start(mlp, 1, 3, tau)
+    gpu.update(1)
+    orc = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), 4,
+                         gravity=sc.gravity)
+    orc.upload(sc.pos, sc.vel)
+    log("pbf-1m learned substep: oracle")
+    act = orc.learned_step(network, 1, 3, tau)
+    log(f"pbf-1m learned substep: tau {tau:.4g}, fallback set {int((act > 0).sum())} particles, "
+        f"{int((act == 2).sum())} seeds; GPU {gpu.fallback_count()}")
+    compare_pbf(gpu, orc, sc.n, 1)
+    assert gpu.fallback_count() == (int((act == 2).sum()), int((act > 0).sum()))
+    assert 0 < int((act == 2).sum()) < sc.n
+    p = mlp.params()
+    ref, _ = MO.forward(p.astype(np.float64), captured["feat"].astype(np.float64),
+                        np.zeros(8), np.ones(8), 1e-4, bf16_ops=True)
+    err = np.abs(captured["out"].astype(np.float64) - ref)
+    log(f"pbf-1m network lambda: max abs err {err.max():.3e} over {err.shape[0]} rows")
+    assert np.all(err <= 1e-2 * (1e-4 + np.abs(ref)))
+    gpu.close()
     mlp.close()
 
 
@@ -175,15 +277,19 @@ def test_thread_order_keeps_learned_bits():
         m.set_params(**sc.params("stable"))
         m.load_state(sc.pos, sc.vel)
         m.set_tuning(128, flags, 0, 32)
-        m.set_warm_start(mlp, 2, 2, -1.0)
+        m.set_warm_start(mlp, 2, 2, 0.05)
         m.set_residual_pass(True)
         m.update(3)
-        out.append((m.read_state(), m.read_field(capi.FIELD_LAMBDA), m.residual_error()))
+        out.append((m.read_state(), m.read_field(capi.FIELD_LAMBDA), m.residual_error(),
+                    m.fallback_count()))
         m.close()
+    log(f"fallback set of the last substep: {out[0][3]}")
+    assert 0 < out[0][3][0]
     for o in out[1:]:
         _eq(o[0][0], out[0][0][0], "positions")
         _eq(o[0][1], out[0][0][1], "velocities")
         _eq(o[1], out[0][1], "lambda")
         _eq(o[2], out[0][2], "residual")
+        assert o[3] == out[0][3]
     mlp.close()
 

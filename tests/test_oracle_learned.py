@@ -1,0 +1,122 @@
+"""Pins of the oracle's learned substep (oracle.OracleFluids.learned_step) and of its
+fallback-set rule (orc_fallback_set), on the CPU.
+
+The learned substep has no reference counterpart; what pins its composition is the plain
+solver it must reduce to:
+- with the plain lambda as its "network" it is the plain substep (Fluids.cpp:400-471) with
+  the same iteration count, bit for bit;
+- with a fallback set of every particle, learned-1 plus F fallback iterations is the plain
+  substep with 1 + F iterations; with an empty set it is learned-1 alone.
+The fallback set is pinned against a brute-force evaluation of its definition on positions
+whose cell lists were built from those same positions (then the 27-cell walk sees every
+particle within h).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from realtimeparticles_amd import scenes
+
+
+def _scene(seed=3):
+    return scenes.dam_break((24, 16, 16), (4, 2, 2), (40, 20, 20), seed=seed, name="pin")
+
+
+def _pair(sc, prm, iters):
+    a = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), iters,
+                       gravity=sc.gravity)
+    b = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), iters,
+                       gravity=sc.gravity)
+    a.upload(sc.pos, sc.vel)
+    b.upload(sc.pos, sc.vel)
+    return a, b
+
+
+def _same(a, b):
+    for name in ("pos", "vel", "pred", "density", "lam", "key", "idx", "se"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+
+
+def _plain_lambda(feat):
+    return feat[:, 1]  # the first density pass's own lambda
+
+
+@pytest.mark.parametrize("mapping", ["stable", "literal"])
+@pytest.mark.parametrize("iters", [1, 3])
+def test_learned_step_with_plain_lambda_is_the_plain_substep(oracle, mapping, iters):
+    sc = _scene()
+    a, b = _pair(sc, sc.params(mapping), iters)
+    for _ in range(2):
+        a.step(1)
+        act = b.learned_step(_plain_lambda, iters)
+        assert not act.any()
+        _same(a, b)
+
+
+@pytest.mark.parametrize("fallback", [1, 3])
+def test_fallback_over_every_particle_is_plain_iterations(oracle, fallback):
+    sc = _scene(4)
+    prm = sc.params("stable")
+    a, b = _pair(sc, prm, 1 + fallback)
+    a.step(1)
+    act = b.learned_step(_plain_lambda, 1, fallback, -1.0)
+    assert (act == 2).all()
+    _same(a, b)
+
+
+def test_empty_fallback_set_is_learned_alone(oracle):
+    """Also with particles pushed past the clamp walls by the learned correction: an empty
+    set applies no boundary condition."""
+    sc = _scene(5)
+    prm = sc.params("stable")
+    a, b = _pair(sc, prm, 1)
+    net = lambda f: (f[:, 1] * np.float32(40.0)).astype(np.float32)  # noqa: E731
+    a.learned_step(net, 1)
+    act = b.learned_step(net, 1, 3, 1e30)
+    assert not act.any()
+    _same(a, b)
+    lo = -np.float32(sc.box[0]) / 2 + np.float32(0.01)
+    assert (b.pred[: sc.n, 0] < lo).any()  # a clamp would have moved these
+
+
+def test_fallback_set_matches_brute_force(oracle):
+    """Seeds |rho/rho0 - 1| > tau and every particle within h of a seed, on positions gridded
+    as they are (so the walk covers every particle within h)."""
+    sc = _scene(6)
+    prm = sc.params("stable")
+    f = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), 1,
+                       gravity=sc.gravity)
+    rng = np.random.default_rng(0)
+    pos = sc.pos.copy()
+    pos[:, :3] += rng.uniform(-0.03, 0.03, (sc.n, 3)).astype(np.float32)
+    f.upload(pos, sc.vel)
+    L, c, p, pt, n = O.lib(), ctypes.byref(f.c), ctypes.byref(f.p), O.ptr, sc.n
+    L.orc_fill_cell_ids(pt(f.pred), pt(f.key), n, c)
+    assert L.orc_radix_sort(pt(f.key), pt(f.idx), f.max_n) == 0
+    L.orc_permute_f4(pt(f.idx), pt(f.pred), pt(f.tmp), f.max_n)
+    L.orc_reset_start_end(pt(f.se), c)
+    L.orc_fill_start(pt(f.key), pt(f.se), n, c)
+    L.orc_fill_end(pt(f.key), pt(f.se), n, f.max_n, c)
+    L.orc_adjust_end(pt(f.se), c)
+    dens = np.zeros(f.max_n, np.float32)
+    L.orc_fld_density(pt(f.pred), pt(f.se), p, pt(dens), n, c)
+    rho0 = np.float32(prm["restDensity"])
+    err = np.abs(dens[:n] / rho0 - np.float32(1.0))
+    tau = float(np.quantile(err, 0.9))
+    act = np.zeros(n, np.uint8)
+    L.orc_fallback_set(pt(f.pred), pt(f.se), pt(dens), float(rho0), tau, pt(act), n, c)
+    seeds = err > np.float32(tau)
+    assert np.array_equal(act == 2, seeds)
+    q = f.pred[:n, :3].astype(np.float32)
+    want = seeds.copy()
+    h = np.float32(f.c.effectRadius)
+    for i in np.flatnonzero(seeds):
+        d = q[i] - q
+        r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+        want |= r < h
+    first = 0  # sorted element 0 is in no cell list (Q1): no walk meets it; left out
+    got = act > 0
+    assert np.array_equal(np.delete(got, first), np.delete(want, first))
+    assert 0 < seeds.sum() < got.sum() < n
