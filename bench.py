@@ -438,10 +438,11 @@ def run_learned(args, sc):
     solver's 4 plain iterations.  The network is trained on the GPU.
 
     The fallback (plain iterations for the particles a learned iteration leaves above tau,
-    and their neighbours) is calibrated on the first training seed: tau is a fraction of
-    plain-4's max |rho/rho0 - 1| there, and the fallback shape kept is the fastest whose max
-    error there is at most plain-4's.  Then every variant is timed on fixed-input substeps of
-    the evaluated scene and its post-correction density error is read.
+    and their neighbours) is calibrated on the training seeds: tau is a fraction of plain-4's
+    max |rho/rho0 - 1| on each, and the fallback shape kept (iterations, fraction) is the
+    fastest whose max error is at most 0.9 x plain-4's on every training seed.  Then every variant
+    is timed on fixed-input substeps of the evaluated scene (tau: the fraction of the training
+    seeds' mean plain-4 max) and its post-correction density error is read.
     Uses the "stable" coefficient mapping: with the literal dam-break coefficients the
     plain solver itself scatters the lattice within one substep (mean |rho/rho0 - 1|
     near 0.9 after 4 iterations), which leaves no error level to compare at.
@@ -457,24 +458,40 @@ def run_learned(args, sc):
                                  seed=1)
     t_train = time.perf_counter() - t0
 
-    # calibration of the fallback on a training seed
-    cal = scenes.pbf_scene(args.workload, seed=seeds[0])
-    cal_plain = learned.evaluate(cal, "stable", [("plain-4", None, 4, 0, 0.0)],
-                                 steps=args.learn_steps, warmup=5, tune=not args.no_tune)[0]
-    shapes = [(f, frac) for f in (1, 2, 3) for frac in (1.0, 0.8, 0.6, 0.4)]
-    cal_runs = learned.evaluate(cal, "stable", [
-        (f"fb{f}@{frac}", mlp, 1, f, cal_plain.residual_max * frac) for f, frac in shapes],
-        steps=args.learn_steps, warmup=5, tune=not args.no_tune)
-    ok = [(v, shape) for v, shape in zip(cal_runs, shapes) if v.residual_max <= cal_plain.residual_max]
-    best = min(ok, key=lambda e: e[0].ms_per_step) if ok else None
+    # calibration of the fallback on the training seeds: tau is a fraction of each seed's
+    # plain-4 max error; the shape kept is the fastest whose max error stays below plain-4's by
+    # a margin on every training seed (the max is set by single surface particles and varies
+    # from scene to scene, and from one training run to the next)
+    margin = 0.9
+    shapes = [(f, frac) for f in (2, 3, 4, 5, 6) for frac in (1.0, 0.8, 0.6)]
+    per_seed = []
+    for seed in seeds:
+        cal = scenes.pbf_scene(args.workload, seed=seed)
+        p4 = learned.evaluate(cal, "stable", [("plain-4", None, 4, 0, 0.0)],
+                              steps=args.learn_steps, warmup=5, tune=not args.no_tune)[0]
+        runs = learned.evaluate(cal, "stable", [
+            (f"fb{f}@{frac}", mlp, 1, f, p4.residual_max * frac) for f, frac in shapes],
+            steps=args.learn_steps, warmup=5, tune=not args.no_tune)
+        per_seed.append((seed, p4, runs))
+    ok = [k for k in range(len(shapes))
+          if all(runs[k].residual_max <= margin * p4.residual_max for _, p4, runs in per_seed)]
+    best = min(ok, key=lambda k: sum(runs[k].ms_per_step for _, _, runs in per_seed)) if ok else None
     calibration = {
-        "seed": seeds[0], "plain4_max_error": cal_plain.residual_max,
-        "plain4_ms_per_step": round(cal_plain.ms_per_step, 4),
-        "candidates": {v.name: {"ms_per_step": round(v.ms_per_step, 4), "max_error": v.residual_max,
-                                "fallback_set": list(v.fallback_set)} for v in cal_runs},
-        "chosen": None if best is None else {"fallback_iters": best[1][0],
-                                             "tau_fraction_of_plain4_max": best[1][1]},
+        "seeds": seeds,
+        "rule": f"tau = fraction x the seed's plain-4 max |rho/rho0 - 1|; keep the fastest shape "
+                f"whose max error is at most {margin} x plain-4's on every training seed",
+        "per_seed": {str(seed): {
+            "plain4": {"ms_per_step": round(p4.ms_per_step, 4), "max_error": p4.residual_max},
+            "candidates": {v.name: {"ms_per_step": round(v.ms_per_step, 4),
+                                    "max_error": v.residual_max,
+                                    "fallback_set": list(v.fallback_set)} for v in runs}}
+            for seed, p4, runs in per_seed},
+        "chosen": None if best is None else {"fallback_iters": shapes[best][0],
+                                             "tau_fraction_of_plain4_max": shapes[best][1]},
     }
+    # tau on the evaluated scene: the chosen fraction of the training seeds' mean plain-4 max
+    best = None if best is None else (None, shapes[best])
+    tau_base = sum(p4.residual_max for _, p4, _ in per_seed) / len(per_seed)
 
     plain = [1, 2, 3, 4, 6, 8]
     runs = [(f"plain-{k}", None, k, 0, 0.0) for k in plain]
@@ -482,7 +499,7 @@ def run_learned(args, sc):
     tau = None
     if best is not None:
         fb, frac = best[1]
-        tau = cal_plain.residual_max * frac
+        tau = tau_base * frac
         runs.append(("learned-1+fallback", mlp, 1, fb, tau))
     res = {v.name: v for v in learned.evaluate(sc, "stable", runs, steps=args.learn_steps,
                                                 warmup=5, tune=not args.no_tune)}

@@ -878,23 +878,31 @@ __global__ __launch_bounds__(256) void k_fb_compact(uint32_t n, FallbackSet fs)
   wave_append(on, t, fs.list, fs.count + 1);
 }
 
-// Positions of the next fallback iteration for the particles outside the set (the set's
-// own are overwritten by the correction): copied, clamped to the walls on the first
-// iteration as fld_applyBoundaryCondition clamps every particle, and lambda outside the set
-// is 0.  An empty set leaves the substep as it was: positions copied as they are.
-__global__ __launch_bounds__(256) void k_fb_carry(const float4* __restrict__ predIn,
-                                                  float4* __restrict__ predOut,
+// The start of the fallback iterations when the set is not empty: fld_applyBoundaryCondition
+// on every particle, in place (later iterations keep the outsiders' positions, and clamp the
+// set's own corrections), and lambda 0 outside the set (the set's own is written by the
+// density pass).  An empty set leaves the substep as it was.
+__global__ __launch_bounds__(256) void k_fb_begin(float4* __restrict__ pred,
                                                   float* __restrict__ lambda, uint32_t n,
-                                                  int clamp, PbfConsts c, FallbackSet fs)
+                                                  PbfConsts c, FallbackSet fs)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n)
+  if (t >= n || fs.count[1] == 0u)
     return;
-  const bool any = fs.count[1] != 0u;
-  const float4 q = predIn[t];
-  predOut[t] = (clamp && any) ? clamp_pos(q, c) : q;
-  if (any)
-    lambda[t] = 0.0f;
+  pred[t] = clamp_pos(pred[t], c);
+  lambda[t] = 0.0f;
+}
+
+// The set's corrected positions, computed into the partner buffer, back into place.
+__global__ __launch_bounds__(256) void k_fb_scatter(float4* __restrict__ pred,
+                                                    const float4* __restrict__ corrected,
+                                                    FallbackSet fs)
+{
+  const uint32_t cnt = fs.count[1];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t i = fs.list[k];
+    pred[i] = corrected[i];
+  }
 }
 
 // fld_computeDensity + fld_computeConstraintFactor over the set (list position k is the
@@ -1521,10 +1529,11 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
 }
 
 // Threads of a pass over a compacted list: the list holds at most n entries, and a
-// grid-stride loop covers any count with at most this many threads.
+// grid-stride loop covers any count with at most 2^16 threads (sets are a few percent of
+// the particles; a larger grid only adds workgroups that find no entry).
 static uint32_t list_blocks(uint32_t n, uint32_t block)
 {
-  const uint32_t cap = (1u << 18) / block;
+  const uint32_t cap = (1u << 16) / block;
   const uint32_t b = blocks_for(n, block);
   return b < cap ? b : cap;
 }
@@ -1543,41 +1552,33 @@ void launch_fallback_set(const float4* pred, const uint2* startEnd, const float*
   hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, fs);
 }
 
-void launch_fallback_iteration(const float4* predIn, float4* predOut, float* density,
-                               float* lambda, const uint2* startEnd, uint32_t n, bool first,
-                               bool last, const PbfConsts& c, const Launch& L,
-                               const FallbackSet& fs, const NbList* nb, hipStream_t s)
+void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,
+                               const uint2* startEnd, uint32_t n, bool first, bool last,
+                               const PbfConsts& c, const Launch& L, const FallbackSet& fs,
+                               const NbList* nb, hipStream_t s)
 {
   if (n == 0)
     return;
-  hipLaunchKernelGGL(k_fb_carry, dim3(blocks_for(n, 256)), dim3(256), 0, s, predIn, predOut,
-                     lambda, n, first ? 1 : 0, c, fs);
+  if (first)
+    hipLaunchKernelGGL(k_fb_begin, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, lambda, n, c,
+                       fs);
   const dim3 g(list_blocks(n, L.block)), b(L.block);
   const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
   const NbList lists = nb ? *nb : NbList();
   const int lastMode = last ? kLastOnly : kNextIter;
-  // the correction replays the density pass's lists: the same positions, clamped the same way
-  if (nb && first) {
-    hipLaunchKernelGGL(k_density_lambda_local<kClampIn | kNbRecord>, g, b, lds, s, predIn,
-                       startEnd, density, lambda, c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<kClampIn | kNbReplay>, g, b, lds, s, predIn, lambda,
-                       startEnd, predOut, lastMode, c, q, fs, lists);
-  } else if (nb) {
-    hipLaunchKernelGGL(k_density_lambda_local<kNbRecord>, g, b, lds, s, predIn, startEnd,
-                       density, lambda, c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<kNbReplay>, g, b, lds, s, predIn, lambda, startEnd,
-                       predOut, lastMode, c, q, fs, lists);
-  } else if (first) {
-    hipLaunchKernelGGL(k_density_lambda_local<kClampIn>, g, b, lds, s, predIn, startEnd, density,
+  // the correction replays the density pass's lists: the same positions
+  if (nb) {
+    hipLaunchKernelGGL(k_density_lambda_local<kNbRecord>, g, b, lds, s, pred, startEnd, density,
                        lambda, c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<kClampIn>, g, b, lds, s, predIn, lambda, startEnd, predOut,
+    hipLaunchKernelGGL(k_correct_local<kNbReplay>, g, b, lds, s, pred, lambda, startEnd, scratch,
                        lastMode, c, q, fs, lists);
   } else {
-    hipLaunchKernelGGL(k_density_lambda_local<0>, g, b, lds, s, predIn, startEnd, density, lambda,
+    hipLaunchKernelGGL(k_density_lambda_local<0>, g, b, lds, s, pred, startEnd, density, lambda,
                        c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<0>, g, b, lds, s, predIn, lambda, startEnd, predOut,
-                       lastMode, c, q, fs, lists);
+    hipLaunchKernelGGL(k_correct_local<0>, g, b, lds, s, pred, lambda, startEnd, scratch, lastMode,
+                       c, q, fs, lists);
   }
+  hipLaunchKernelGGL(k_fb_scatter, g, dim3(L.block), 0, s, pred, scratch, fs);
 }
 
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,

@@ -285,15 +285,15 @@ struct FallbackSet {
 void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
                          float tau, uint32_t n, const PbfConsts& c, const Launch& L,
                          const FallbackSet& fs, hipStream_t s);
-// One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, with the
-// lambda of every other particle 0 (unless the set is empty) and its position carried from
-// predIn to predOut (clamped to the walls on the first iteration, as every particle's is at
-// an iteration's start).
-// nb: survivor lists recorded by the density pass and replayed by the correction.
-void launch_fallback_iteration(const float4* predIn, float4* predOut, float* density,
-                               float* lambda, const uint2* startEnd, uint32_t n, bool first,
-                               bool last, const PbfConsts& c, const Launch& L,
-                               const FallbackSet& fs, const NbList* nb, hipStream_t s);
+// One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, in place in
+// pred: the first (when the set is not empty) clamps every particle to the walls and sets
+// the lambda of every particle outside the set to 0; the set's corrections are computed into
+// scratch (indexed by particle) and then written back.  nb: survivor lists recorded by the
+// density pass and replayed by the correction.
+void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,
+                               const uint2* startEnd, uint32_t n, bool first, bool last,
+                               const PbfConsts& c, const Launch& L, const FallbackSet& fs,
+                               const NbList* nb, hipStream_t s);
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
                            uint32_t n, const PbfConsts& c, hipStream_t s);
 // nb: 0, kNbRecord or kNbReplay (the post passes share their positions and thread order).
