@@ -399,9 +399,11 @@ int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* cou
 /* ========================= Grid building blocks ======================== */
 /* RadixSort::sort on one key buffer (physics/utils/RadixSort.cpp:122-161): stable
  * ascending sort of n 32-bit keys with maxKey >= every key; permOut[i] is the original
- * position of the key now at i.  Host arrays in and out (for tests and tools). */
+ * position of the key now at i.  digitBits: largest digit width of the multi-launch sort,
+ * 4..11 (0 = 9; sorts of at most 8,192 keys run in one workgroup with 4-bit digits).  Host
+ * arrays in and out (for tests and tools). */
 int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxKey,
-                   uint32_t* keysOut, uint32_t* permOut);
+                   int32_t digitBits, uint32_t* keysOut, uint32_t* permOut);
 /* The grid part of Fluids::update for one batch of positions (Fluids.cpp:415-424):
  * fillCellIDs over n, sort of all maxN keys (tail keys 2*NC+i as after resetCellIDs), cell
  * start/end with the cap.  keys/idx are maxN long, startEnd 2*NC. */

@@ -253,7 +253,7 @@ _SIGS = {
     "rtp_batch_enable_profiling": ([_vp, _int], _int),
     "rtp_batch_profile": ([_vp, _vp, _int, ctypes.POINTER(_int)], _int),
     "rtp_batch_sizes": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
-    "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _vp, _vp], _int),
+    "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _int, _vp, _vp], _int),
     "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
     "rtp_grid_build_noalloc": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp, _vp], _int),
     "rtp_selfcheck_sqrt": ([_int, _u32, _u32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_u32)],
@@ -852,16 +852,17 @@ class Boids(_RenderFields):
                 for i in range(cnt.value)}
 
 
-def sort_pairs(keys, max_key: int | None = None, device: int = 0):
-    """Stable sort of uint32 keys on the GPU; returns (sorted_keys, permutation)."""
+def sort_pairs(keys, max_key: int | None = None, device: int = 0, digit_bits: int = 0):
+    """Stable sort of uint32 keys on the GPU; returns (sorted_keys, permutation).  digit_bits:
+    the largest digit width of the multi-launch sort (4..11, 0 = default)."""
     keys = np.ascontiguousarray(keys, dtype=np.uint32)
     n = keys.shape[0]
     if max_key is None:
         max_key = int(keys.max()) if n else 0
     out = np.zeros(n, np.uint32)
     perm = np.zeros(n, np.uint32)
-    _check(lib().rtp_sort_pairs(int(device), keys.ctypes.data, n, int(max_key), out.ctypes.data,
-                                perm.ctypes.data), "rtp_sort_pairs")
+    _check(lib().rtp_sort_pairs(int(device), keys.ctypes.data, n, int(max_key), int(digit_bits),
+                                out.ctypes.data, perm.ctypes.data), "rtp_sort_pairs")
     return out, perm
 
 

@@ -1961,9 +1961,10 @@ int rtp_fluids_record_teacher(rtp_fluids* h, uint32_t teacherIters, float* feat,
 // ------------------------------------------------------------------------ building blocks
 
 int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxKey,
-                   uint32_t* keysOut, uint32_t* permOut)
+                   int32_t digitBits, uint32_t* keysOut, uint32_t* permOut)
 {
   RTP_REQUIRE(n == 0 || (keysIn && keysOut && permOut), "NULL argument");
+  RTP_REQUIRE(digitBits == 0 || (digitBits >= 4 && digitBits <= 11), "digitBits must be 0 or 4..11");
   if (n == 0)
     return RTP_OK;
   int rc = check_device(device);
@@ -1972,7 +1973,7 @@ int rtp_sort_pairs(int device, const uint32_t* keysIn, uint32_t n, uint32_t maxK
   hipStream_t s = nullptr;
   RTP_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   uint32_t *ka = nullptr, *kb = nullptr, *ia = nullptr, *ib = nullptr, *cnt = nullptr;
-  const SortPlan plan = make_sort_plan(n, maxKey, 0);
+  const SortPlan plan = make_sort_plan(n, maxKey, digitBits);
   bool inB = false;
   rc = dev_alloc(&ka, n);
   if (!rc) rc = dev_alloc(&kb, n);

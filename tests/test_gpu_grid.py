@@ -22,6 +22,21 @@ def test_sort_pairs_is_stable_sort(n, hi):
     _eq(out, keys[ref], "keys")
 
 
+@pytest.mark.parametrize("bits", [4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("n", [8193, 300001])
+def test_sort_pairs_every_digit_width(n, bits):
+    """Each digit width of the multi-launch sort (the widths the launch tuning picks from),
+    over keys of 2 NC + maxN size, as the stale tail of a shrunk model produces."""
+    log(f"sort_pairs n={n} bits={bits}")
+    rng = np.random.default_rng(bits * 7 + n)
+    hi = 2 * 256000 + n
+    keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    out, perm = capi.sort_pairs(keys, max_key=hi - 1, digit_bits=bits)
+    ref = np.argsort(keys, kind="stable")
+    _eq(perm, ref.astype(np.uint32), "perm")
+    _eq(out, keys[ref], "keys")
+
+
 def test_sort_pairs_nearly_sorted_input():
     # the next step's keys are the previous order with a few cells changed
     rng = np.random.default_rng(4)

@@ -92,14 +92,15 @@ def test_graph_replay_equals_direct_launches(flags):
     b.close()
 
 
-@pytest.mark.parametrize("flags", [0, 13])
-def test_nb_particles_change_keeps_reference_sort(flags):
+@pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11)])
+def test_nb_particles_change_keeps_reference_sort(flags, sort_bits):
     # Model::setNbParticles (Model.hpp:106) does not reset the grid: the keys of particles
     # that left the active range stay in the reference's whole-buffer sort.  flags 13: the
-    # thread order's and survivor lists' sizes follow the count.
+    # thread order's and survivor lists' sizes follow the count.  sort_bits: the digit width
+    # of the sort over the stale-tail keys (up to 2 NC + maxN).
     sc = small_scene(seed=5)
     gpu, orc = make_pbf_pair(sc, iters=2)
-    gpu.set_tuning(128, flags, 0, 16)
+    gpu.set_tuning(128, flags, sort_bits, 16)
     gpu.update(1)
     orc.step(1)
     compare_pbf(gpu, orc, sc.n, 1)
@@ -187,23 +188,23 @@ def test_checkpoint_restore_replays_identically():
     gpu.close()
 
 
-@pytest.mark.parametrize("block,flags,queue", [(256, 2, 16), (128, 2, 24), (64, 3, 48),
-                                               (192, 1, 64), (256, 3, 16), (256, 0, 64),
-                                               (128, 0, 8), (64, 4, 64), (128, 6, 32),
-                                               (256, 7, 16), (192, 5, 40), (64, 13, 16),
-                                               (256, 8, 8), (128, 12, 24), (64, 15, 12)])
+@pytest.mark.parametrize("block,flags,queue,sort_bits", [
+    (256, 2, 16, 0), (128, 2, 24, 7), (64, 3, 48, 0), (192, 1, 64, 10), (256, 3, 16, 11),
+    (256, 0, 64, 0), (128, 0, 8, 8), (64, 4, 64, 0), (128, 6, 32, 9), (256, 7, 16, 0),
+    (192, 5, 40, 4), (64, 13, 16, 10), (256, 8, 8, 0), (128, 12, 24, 11), (64, 15, 12, 7),
+    (64, 13, 16, 5), (128, 13, 16, 6)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
-def test_launch_shapes_do_not_change_bits(block, flags, queue, mapping):
+def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mapping):
     """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell, the
-    survivor-queue depth and replayed survivor lists change where candidates are read from,
-    which thread handles a particle and when the neighbour bodies run -- never which
-    candidates or in which order."""
+    survivor-queue depth, replayed survivor lists and the sort's digit width change where
+    candidates are read from, which thread handles a particle and when the neighbour bodies
+    run -- never which candidates or in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
     prm = sc.params(mapping)
     gpu, orc = make_pbf_pair(sc, iters=3, **{k: v for k, v in prm.items()})
-    gpu.set_tuning(block, flags, 0, queue)
+    gpu.set_tuning(block, flags, sort_bits, queue)
     for step in range(1, 5):
-        log(f"block {block} flags {flags} queue {queue} {mapping} step {step}")
+        log(f"block {block} flags {flags} queue {queue} bits {sort_bits} {mapping} step {step}")
         gpu.update(1)
         orc.step(1)
         compare_pbf(gpu, orc, sc.n, step)
