@@ -58,7 +58,9 @@ def test_mlp_gradients_match_fp64_oracle():
                          "w3": (4736, 4800), "b3": (4800, 4801)}.items():
         rel = np.linalg.norm(g[a:b] - gr[a:b]) / max(np.linalg.norm(gr[a:b]), 1e-12)
         log(f"grad {name}: rel err {rel:.2e}")
-        assert rel < 5e-2, name
+        # measured on the MI355X: at most 7e-4 (W1); bf16 operands are rounded the same way
+        # on both sides, so what remains is FP32 against FP64 accumulation
+        assert rel < 1e-3, name
     # the parameters are unchanged by a gradient evaluation
     assert np.array_equal(m.params(), p)
     m.close()
