@@ -114,8 +114,34 @@ typedef enum {
   /* with the rendering steps enabled (rtp_*_set_render): */
   RTP_FIELD_COLOR = 5,       /* p_col (float4) */
   RTP_FIELD_CAMERA_DIST = 6, /* p_cameraDist (uint32 words, sorted by the last update) */
-  RTP_FIELD_DETECTOR = 7     /* c_partDetector (float8 per cell; count = cells) */
+  RTP_FIELD_DETECTOR = 7,    /* c_partDetector (float8 per cell; count = cells) */
+  /* the Clouds model (Clouds.cpp:161-188): */
+  RTP_FIELD_TEMPERATURE = 8,     /* p_temp */
+  RTP_FIELD_VAPOR_DENSITY = 9,   /* p_vaporDens */
+  RTP_FIELD_CLOUD_DENSITY = 10,  /* p_cloudDens */
+  RTP_FIELD_BUOYANCY = 11,       /* p_buoyancy */
+  RTP_FIELD_PARTICLE_ID = 12,    /* p_partID (float) */
+  RTP_FIELD_TOT_CORR_POS = 13,   /* p_totCorrPos (float4) */
+  RTP_FIELD_LAPLACIAN_TEMP = 14, /* p_laplacianTemp */
+  RTP_FIELD_LAMBDA_TEMP = 15     /* p_constFactorTemp */
 } rtp_field;
+
+/* physics/ocl/Clouds.hpp:14-48 CloudKernelInputs == clouds.cl:18-32 CloudParams (52 bytes). */
+typedef struct {
+  uint32_t dim;
+  float timeStep;
+  float restDensity;
+  float groundHeatCoeff;
+  float buoyancyCoeff;
+  float gravCoeff;
+  float adiabaticLapseRate;
+  float phaseTransitionRate;
+  float latentHeatCoeff;
+  uint32_t isTempSmoothingEnabled;
+  float relaxCFM;
+  float initVaporDensityCoeff;
+  float windCoeff;
+} rtp_cloud_params;
 
 /* ---- the JSON parameter channel ----
  * The reference's models keep their parameters in a JSON tree (initFluidsJson,
@@ -396,6 +422,39 @@ int rtp_boids_params_json(rtp_boids* h, char* buf, size_t cap, size_t* len);
 int rtp_boids_enable_profiling(rtp_boids* h, int enable);
 int rtp_boids_profile(rtp_boids* h, rtp_kernel_time* out, int capacity, int* count);
 
+/* ================================ Clouds ================================ */
+/* physics/ocl/Clouds.{hpp,cpp}: position based fluids (FluidParams) coupled to a
+ * temperature, vapour and cloud density per particle (CloudParams), with periodic x/z walls,
+ * a temperature-smoothing constraint and a wind; on the fluids model's grid and sort. */
+typedef struct rtp_clouds rtp_clouds;
+/* Clouds::Clouds (Clouds.cpp:100-119) + reset; the parameters start at FluidKernelInputs'
+ * and CloudKernelInputs' defaults (Fluids.hpp:17-32, Clouds.hpp:14-48), 2 Jacobi iterations. */
+int rtp_clouds_create(const rtp_model_desc* desc, rtp_clouds** out);
+int rtp_clouds_destroy(rtp_clouds* h);
+/* transferJsonInputsToModel's typed result (Clouds.cpp:253-303); either pointer may be NULL. */
+int rtp_clouds_set_params(rtp_clouds* h, const rtp_fluid_params* p, const rtp_cloud_params* q);
+int rtp_clouds_get_params(rtp_clouds* h, rtp_fluid_params* p, rtp_cloud_params* q);
+/* m_nbJacobiIters (Clouds.hpp:71) */
+int rtp_clouds_set_nb_jacobi_iters(rtp_clouds* h, uint32_t iters);
+int rtp_clouds_get_nb_jacobi_iters(rtp_clouds* h, uint32_t* iters);
+/* Model::setNbParticles (Model.hpp:106) */
+int rtp_clouds_set_nb_particles(rtp_clouds* h, uint32_t n);
+/* Clouds::initCloudsParticles (Clouds.cpp:401-500) with the caller's positions and
+ * velocities: the tail at +inf, cloud density 0, particle ids, the temperature and vapour
+ * density of every slot from its altitude (cld_initTemperature, cld_initVaporDensity), and
+ * resetCellIDs.  count becomes the particle count. */
+int rtp_clouds_load_state(rtp_clouds* h, const float* pos4, const float* vel4, uint32_t count);
+int rtp_clouds_read_state(rtp_clouds* h, float* pos4, float* vel4, uint32_t count);
+/* Clouds::update (Clouds.cpp:503-610) without the rendering steps, nsteps times. */
+int rtp_clouds_update(rtp_clouds* h, uint32_t nsteps);
+int rtp_clouds_pause(rtp_clouds* h, int pause);
+int rtp_clouds_sync(rtp_clouds* h);
+int rtp_clouds_read_grid(rtp_clouds* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
+/* RTP_FIELD_DENSITY, _LAMBDA, _PRED_POS, _VORTICITY and the Clouds fields 8..15. */
+int rtp_clouds_read_field(rtp_clouds* h, int field, float* out, uint32_t count);
+int rtp_clouds_enable_profiling(rtp_clouds* h, int enable);
+int rtp_clouds_profile(rtp_clouds* h, rtp_kernel_time* out, int capacity, int* count);
+
 /* ========================= Grid building blocks ======================== */
 /* RadixSort::sort on one key buffer (physics/utils/RadixSort.cpp:122-161): stable
  * ascending sort of n 32-bit keys with maxKey >= every key; permOut[i] is the original
@@ -435,6 +494,11 @@ int rtp_grid_build_noalloc(const rtp_model_desc* desc, const float* pos4, uint32
 int rtp_selfcheck_sqrt(int device, uint32_t lo, uint32_t hi, uint64_t* mismatches, uint32_t* firstBad);
 int rtp_selfcheck_division(int device, int32_t nExp, int32_t dExp, uint32_t dSigLo, uint32_t dSigHi,
                            uint32_t nSigStep, uint64_t* mismatches, uint32_t* firstBad);
+/* The Clouds model's pinned exponential on the device: y64 = exp(x64) (double), y32 =
+ * exp(x32) (float) and svd32 = saturationVaporDensity(x32) (clouds.cl:91-96), n values each,
+ * for comparison with the CPU's evaluation of the same algorithm. */
+int rtp_selfcheck_exp(int device, const double* x64, const float* x32, uint32_t n, double* y64,
+                      float* y32, float* svd32);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,26 @@ class FluidParams(ctypes.Structure):
     ]
 
 
+class CloudParams(ctypes.Structure):
+    """clouds.cl:18-32 CloudParams == Clouds.hpp:14-48 CloudKernelInputs (52 bytes)."""
+
+    _fields_ = [
+        ("dim", _u32),
+        ("timeStep", _f32),
+        ("restDensity", _f32),
+        ("groundHeatCoeff", _f32),
+        ("buoyancyCoeff", _f32),
+        ("gravCoeff", _f32),
+        ("adiabaticLapseRate", _f32),
+        ("phaseTransitionRate", _f32),
+        ("latentHeatCoeff", _f32),
+        ("isTempSmoothingEnabled", _u32),
+        ("relaxCFM", _f32),
+        ("initVaporDensityCoeff", _f32),
+        ("windCoeff", _f32),
+    ]
+
+
 class BoidsParams(ctypes.Structure):
     """physics/ocl/kernels/boids.cl:12-17 (BoidsRuleParams)."""
 
@@ -81,6 +101,14 @@ class _FluidsState(ctypes.Structure):
         ("key", _vp), ("idx", _vp),
         ("se", _vp),
     ]
+
+
+class _CloudsState(ctypes.Structure):
+    _fields_ = [("c", Consts), ("p", FluidParams), ("q", CloudParams), ("maxN", _u32), ("n", _u32),
+                ("iters", _u32)] + [(k, _vp) for k in (
+                    "pos", "vel", "pred", "totCorr", "corr", "vort", "velIn", "tmp", "temp",
+                    "tempIn", "buoyancy", "vapor", "vaporIn", "cloud", "cloudIn", "gen", "partID",
+                    "density", "lambda_", "lap", "lambdaT", "corrT", "ftmp", "key", "idx", "se")]
 
 
 class _BoidsState(ctypes.Structure):
@@ -195,6 +223,42 @@ def lib():
         L.orc_fluids_frame.argtypes = [P(_FluidsState), P(_Render), ctypes.c_int]
         L.orc_boids_frame.restype = ctypes.c_int
         L.orc_boids_frame.argtypes = [P(_BoidsState), P(_Render), ctypes.c_int]
+        L.orc_exp64.restype = ctypes.c_double
+        L.orc_exp64.argtypes = [ctypes.c_double]
+        L.orc_exp32.restype = _f32
+        L.orc_exp32.argtypes = [_f32]
+        L.orc_cld_ext_heat.restype = _f32
+        L.orc_cld_ext_heat.argtypes = [_f32, P(Consts)]
+        L.orc_cld_env_temp.restype = _f32
+        L.orc_cld_env_temp.argtypes = [_f32, P(Consts)]
+        L.orc_cld_svd.restype = _f32
+        L.orc_cld_svd.argtypes = [_f32]
+        L.orc_cld_init_temp.argtypes = [_vp, _vp, _u32, P(Consts)]
+        L.orc_cld_init_vapor.argtypes = [P(CloudParams), _vp, _vp, _u32]
+        L.orc_cld_heat_ground.argtypes = [_vp, _vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_buoyancy.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_adiabatic.argtypes = [_vp, _vp, P(CloudParams), _vp, _u32]
+        L.orc_cld_generate.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _u32]
+        L.orc_cld_phase.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _vp, _u32]
+        L.orc_cld_latent.argtypes = [_vp, _vp, P(CloudParams), _vp, _u32]
+        L.orc_cld_predict.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _vp, _u32]
+        L.orc_cld_correct_temp.argtypes = [_vp, _vp, _u32]
+        L.orc_permute_f1.argtypes = [_vp, _vp, _vp, _u32]
+        L.orc_cld_boundary.argtypes = [_vp, _u32, P(Consts)]
+        for name in ("orc_cld_density",):
+            getattr(L, name).argtypes = [_vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_lambda.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_correction.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_laplacian_temp.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_lambda_temp.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_correction_temp.argtypes = [_vp, _vp, _vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_vorticity.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_confinement.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_xsph.argtypes = [_vp, _vp, _vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_cld_update_pos.argtypes = [_vp, P(CloudParams), _vp, _u32, P(Consts)]
+        L.orc_cld_update_vel.argtypes = [_vp, P(FluidParams), _vp, _u32, P(Consts)]
+        L.orc_clouds_update.restype = ctypes.c_int
+        L.orc_clouds_update.argtypes = [P(_CloudsState)]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -367,6 +431,82 @@ class OracleFluids:
             L.orc_fld_xsph(ptr(self.pred), ptr(self.se), ptr(self.vel_in), p, ptr(self.vel), n, c)
         L.orc_fld_update_pos(ptr(self.pred), ptr(self.pos), n)
         return act
+
+
+def cloud_params(**kw) -> CloudParams:
+    """CloudKernelInputs defaults (physics/ocl/Clouds.hpp:14-48), overridable by keyword."""
+    q = CloudParams(dim=3, timeStep=0.01, restDensity=400.0, groundHeatCoeff=10.0,
+                    buoyancyCoeff=0.10, gravCoeff=0.0005, adiabaticLapseRate=5.0,
+                    phaseTransitionRate=0.3485, latentHeatCoeff=0.07, isTempSmoothingEnabled=1,
+                    relaxCFM=600.0, initVaporDensityCoeff=0.75, windCoeff=1.0)
+    for k, v in kw.items():
+        setattr(q, k, v)
+    return q
+
+
+class OracleClouds:
+    """Clouds::update (physics/ocl/Clouds.cpp:503-610, physics part) on numpy-owned buffers."""
+
+    def __init__(self, box, res, max_n: int, n: int, params: FluidParams, cloud: CloudParams,
+                 iters: int, gravity: float = 9.81, max_parts_in_cell: int = 100):
+        self.c = make_consts(box, res, max_parts_in_cell, gravity)
+        nc = int(self.c.numCells)
+        self.max_n, self.n = int(max_n), int(n)
+        f4 = lambda: np.zeros((self.max_n, 4), np.float32)  # noqa: E731
+        f1 = lambda: np.zeros(self.max_n, np.float32)  # noqa: E731
+        self.bufs = {}
+        for k in ("pos", "vel", "pred", "totCorr", "corr", "vort", "velIn", "tmp"):
+            self.bufs[k] = f4()
+        for k in ("temp", "tempIn", "buoyancy", "vapor", "vaporIn", "cloud", "cloudIn", "gen",
+                  "partID", "density", "lambda_", "lap", "lambdaT", "corrT", "ftmp"):
+            self.bufs[k] = f1()
+        self.bufs["key"] = np.zeros(self.max_n, np.uint32)
+        self.bufs["idx"] = np.zeros(self.max_n, np.uint32)
+        self.bufs["se"] = np.zeros((nc, 2), np.uint32)
+        self.s = _CloudsState()
+        self.s.c = self.c
+        self.s.maxN, self.s.n, self.s.iters = self.max_n, self.n, int(iters)
+        self.set_params(params, cloud)
+        for k, arr in self.bufs.items():
+            setattr(self.s, k, ptr(arr))
+
+    def __getattr__(self, name):
+        bufs = self.__dict__.get("bufs")
+        if bufs is not None and name in bufs:
+            return bufs[name]
+        if bufs is not None and name == "lam":
+            return bufs["lambda_"]
+        raise AttributeError(name)
+
+    def set_params(self, params: FluidParams, cloud: CloudParams) -> None:
+        self.p, self.q = params, cloud
+        self.s.p, self.s.q = params, cloud
+
+    def set_iters(self, iters: int) -> None:
+        self.s.iters = int(iters)
+
+    def upload(self, pos: np.ndarray, vel: np.ndarray) -> None:
+        """Clouds::initCloudsParticles (Clouds.cpp:401-500) from given positions and velocities:
+        the tail at +inf, cloud density 0, particle ids, then cld_initTemperature and
+        cld_initVaporDensity over maxN, and resetCellIDs (:396)."""
+        m = pos.shape[0]
+        b = self.bufs
+        b["pos"][:] = (np.inf, np.inf, np.inf, 0.0)
+        b["vel"][:] = 0.0
+        b["pos"][:m] = pos
+        b["vel"][:m] = vel
+        b["cloud"][:] = 0.0
+        b["partID"][:] = np.arange(self.max_n, dtype=np.float32)
+        L = lib()
+        L.orc_cld_init_temp(ptr(b["pos"]), ptr(b["temp"]), self.max_n, ctypes.byref(self.c))
+        L.orc_cld_init_vapor(ctypes.byref(self.q), ptr(b["temp"]), ptr(b["vapor"]), self.max_n)
+        L.orc_reset_cell_ids(ptr(b["key"]), self.max_n, ctypes.byref(self.c))
+
+    def step(self, nsteps: int = 1) -> None:
+        for _ in range(nsteps):
+            rc = lib().orc_clouds_update(ctypes.byref(self.s))
+            if rc:
+                raise RuntimeError(f"orc_clouds_update failed ({rc})")
 
 
 class OracleBoids:

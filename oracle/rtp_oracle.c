@@ -977,3 +977,506 @@ int orc_boids_frame(orc_boids* s, orc_render* r, int paused)
   }
   return camera_sort(r, s->idx, s->n, s->maxN, s->tmp, s->pos, r->col, s->vel, s->acc);
 }
+
+/* ------------------------------------------------------------------------------------
+ * clouds.cl + Clouds.cpp (the Clouds PBF variant)
+ * ---------------------------------------------------------------------------------- */
+
+/* The exponential of clouds.cl (exp of a float in externalHeatSource and the wind of
+ * cld_updatePosition; exp of a double in saturationVaporDensity, whose literals are double)
+ * is vendor-defined in OpenCL.  Pinned here by a published method, Cody-Waite argument
+ * reduction (x = k ln2 + r, ln2 split in a 31-bit head and a tail, |r| <= ln2/2) followed by
+ * the degree-13 Taylor polynomial of e^r in Horner form with fused multiply-adds, and 2^k
+ * applied as two exact power-of-two factors; the float version rounds the double result.
+ * Both are within 1 ulp of the correctly rounded result (checked against libm). */
+static const double ORC_EXP_C[14] = {
+  0x1.0000000000000p+0,  0x1.0000000000000p+0,  0x1.0000000000000p-1,  0x1.5555555555555p-3,
+  0x1.5555555555555p-5,  0x1.1111111111111p-7,  0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+  0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+  0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33
+};
+
+static inline double pow2i(int k) /* 2^k for -1022 <= k <= 1023 */
+{
+  const uint64_t b = (uint64_t)(k + 1023) << 52;
+  double d;
+  memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+double orc_exp64(double x)
+{
+  if (x != x)
+    return x + x;
+  if (x > 709.782712893384)
+    return INFINITY;
+  if (x < -745.1332191019412)
+    return 0.0;
+  const double k = rint(x * 0x1.71547652b82fep+0);
+  double r = fma(-k, 0x1.62e42fec00000p-1, x);
+  r = fma(-k, 0x1.d1cf79abc9e3bp-32, r);
+  double p = ORC_EXP_C[13];
+  for (int i = 12; i >= 0; --i)
+    p = fma(p, r, ORC_EXP_C[i]);
+  const int ki = (int)k;
+  const int k1 = ki / 2;
+  const int k2 = ki - k1;
+  return (p * pow2i(k1)) * pow2i(k2);
+}
+
+float orc_exp32(float x) { return (float)orc_exp64((double)x); }
+
+/* clouds.cl:67-70 */
+float orc_cld_ext_heat(float altitude, const orc_consts* c)
+{
+  return clampf(orc_exp32(-(altitude + c->absWall[1]) / 3.0f), 0.0f, 1.0f);
+}
+
+/* clouds.cl:75-86 */
+float orc_cld_env_temp(float altitude, const orc_consts* c)
+{
+  const float coeff = 3.5f;
+  const float tempGround = 293.0f;
+  return -coeff * (altitude + c->absWall[1]) + tempGround;
+}
+
+/* clouds.cl:91-96: the literals are double, so the expression is evaluated in double and
+ * the result rounded to the float return type. */
+float orc_cld_svd(float temperature)
+{
+  const double t = (double)temperature;
+  return (float)(217 * orc_exp64(19.5 - 4303.4 / (t - 29.5)) / t);
+}
+
+/* clouds.cl:116-121 */
+void orc_cld_init_temp(const orc_f4* pos, float* temp, uint32_t count, const orc_consts* c)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)count; ++i)
+    temp[i] = orc_cld_env_temp(pos[i].y, c);
+}
+
+/* clouds.cl:127-133 */
+void orc_cld_init_vapor(const orc_cloud_params* q, const float* temp, float* vapor, uint32_t count)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)count; ++i)
+    vapor[i] = q->initVaporDensityCoeff * orc_cld_svd(temp[i]);
+}
+
+/* clouds.cl:159-167 */
+void orc_cld_heat_ground(const float* tempIn, const orc_f4* pos, const orc_cloud_params* q,
+                         float* temp, uint32_t n, const orc_consts* c)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    temp[i] = fminf(tempIn[i] + orc_cld_ext_heat(pos[i].y, c) * q->groundHeatCoeff * q->timeStep,
+                    313.0f);
+}
+
+/* clouds.cl:173-183 */
+void orc_cld_buoyancy(const float* temp, const orc_f4* pos, const float* cloudDens,
+                      const orc_cloud_params* q, float* buoyancy, uint32_t n, const orc_consts* c)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const float envTemp = orc_cld_env_temp(pos[i].y, c);
+    buoyancy[i] = q->buoyancyCoeff * (temp[i] - envTemp) / envTemp -
+                  q->gravCoeff * c->gravity * cloudDens[i];
+  }
+}
+
+/* clouds.cl:189-198 */
+void orc_cld_adiabatic(const float* temp, const orc_f4* vel, const orc_cloud_params* q,
+                       float* tempOut, uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    tempOut[i] = fmaxf(temp[i] - q->adiabaticLapseRate * vel[i].y * q->timeStep, 223.0f);
+}
+
+/* clouds.cl:207-220 */
+void orc_cld_generate(const float* temp, const float* vapor, const float* cloudDens,
+                      const orc_cloud_params* q, float* gen, uint32_t n)
+{
+  (void)cloudDens;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    gen[i] = q->phaseTransitionRate * (vapor[i] - orc_cld_svd(temp[i]));
+}
+
+/* clouds.cl:226-237 */
+void orc_cld_phase(const float* vaporIn, const float* cloudIn, const float* gen,
+                   const orc_cloud_params* q, float* vapor, float* cloudDens, uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    cloudDens[i] = fmaxf(cloudIn[i] + gen[i] * q->timeStep, 0.0f);
+    vapor[i] = fmaxf(vaporIn[i] - gen[i] * q->timeStep, 0.0f);
+  }
+}
+
+/* clouds.cl:243-251 */
+void orc_cld_latent(const float* tempIn, const float* gen, const orc_cloud_params* q, float* temp,
+                    uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    temp[i] = tempIn[i] + fmaxf(q->latentHeatCoeff * gen[i] * q->timeStep, 0.0f);
+}
+
+/* clouds.cl:257-273 */
+void orc_cld_predict(const orc_f4* pos, const orc_f4* vel, const float* buoyancy,
+                     const orc_cloud_params* q, orc_f4* pred, orc_f4* totCorr, uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 f = { 0.0f, buoyancy[i], 0.0f, 0.0f };
+    const orc_f4 predVel = add4(vel[i], mul4s(f, q->timeStep));
+    totCorr[i] = mul4s(predVel, q->timeStep);
+    pred[i] = add4(pos[i], mul4s(predVel, q->timeStep));
+  }
+}
+
+/* clouds.cl:279-300: periodic in x and z (x - 2 clamp(x)), wall in y */
+void orc_cld_boundary(orc_f4* pred, uint32_t n, const orc_consts* c)
+{
+  const float wx = c->absWall[0], wy = c->absWall[1], wz = c->absWall[2];
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 p = pred[i];
+    const orc_f4 cl = { clampf(p.x, -wx, wx), clampf(p.y, -wy, wy), clampf(p.z, -wz, wz),
+                        clampf(p.w, 0.0f, 0.0f) };
+    if (fabsf(p.x) > wx)
+      pred[i].x = p.x - 2 * cl.x;
+    if (fabsf(p.y) > wy)
+      pred[i].y = cl.y;
+    if (fabsf(p.z) > wz)
+      pred[i].z = p.z - 2 * cl.z;
+  }
+}
+
+/* The cells of clouds.cl:322-353 in the loop's order (x outer, z inner): x and z wrap, and
+ * a wrapped neighbour's position is shifted by absWall * (+-2); cells past the y walls are
+ * skipped.  Returns the number of cells. */
+static inline int cld_neighbour_cells(orc_f4 pos, const orc_consts* c, int32_t cells[27],
+                                      orc_f4 shift[27])
+{
+  uint32_t u3[3];
+  cell3d(pos, c, u3);
+  const int32_t cx = (int32_t)u3[0], cy = (int32_t)u3[1], cz = (int32_t)u3[2];
+  const int32_t rx = c->res[0], ry = c->res[1], rz = c->res[2];
+  const orc_f4 absWall = { c->absWall[0], c->absWall[1], c->absWall[2], 0.0f };
+  int k = 0;
+  for (int iX = -1; iX <= 1; ++iX)
+    for (int iY = -1; iY <= 1; ++iY)
+      for (int iZ = -1; iZ <= 1; ++iZ) {
+        orc_f4 sign = { 0.0f, 0.0f, 0.0f, 0.0f };
+        const int32_t nx = (cx + iX + rx) % rx, ny = (cy + iY + ry) % ry, nz = (cz + iZ + rz) % rz;
+        if (cx + iX >= rx)
+          sign.x = 2.0f;
+        else if (cx + iX < 0)
+          sign.x = -2.0f;
+        if (cy + iY >= ry || cy + iY < 0)
+          continue;
+        if (cz + iZ >= rz)
+          sign.z = 2.0f;
+        else if (cz + iZ < 0)
+          sign.z = -2.0f;
+        cells[k] = (nx * ry + ny) * rz + nz;
+        const orc_f4 s = { absWall.x * sign.x, absWall.y * sign.y, absWall.z * sign.z,
+                           absWall.w * sign.w };
+        shift[k] = s;
+        ++k;
+      }
+  return k;
+}
+
+/* vec = pos - p_e - absWall * sign, over every candidate e of the cells above */
+#define ORC_CLD_FOR_NEIGHBOURS(c, pos, src, e, vec)                                        \
+  int32_t cells_[27];                                                                    \
+  orc_f4 shift_[27];                                                                     \
+  const int ncells_ = cld_neighbour_cells(pos, c, cells_, shift_);                       \
+  for (int k_ = 0; k_ < ncells_; ++k_)                                                   \
+    for (uint32_t e = se[cells_[k_]].x; e <= se[cells_[k_]].y; ++e)                      \
+      for (orc_f4 vec = sub4(sub4(pos, src[e]), shift_[k_]), *once_ = &vec; once_;       \
+           once_ = NULL)
+
+/* clouds.cl:305-359 */
+void orc_cld_density(const orc_f4* pred, const orc_u2* se, const orc_fluid_params* p,
+                     float* density, uint32_t n, const orc_consts* c)
+{
+  (void)p;
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    float fluidDensity = 0.0f;
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      fluidDensity += poly6(vec, h, c);
+    }
+    density[i] = fluidDensity;
+  }
+}
+
+/* clouds.cl:365-435 */
+void orc_cld_lambda(const orc_f4* pred, const float* density, const orc_u2* se,
+                    const orc_fluid_params* p, float* lambda, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    const float densityC = density[i] / p->restDensity - 1.0f;
+    orc_f4 sumGradCi = { 0.0f, 0.0f, 0.0f, 0.0f };
+    float sumSqGradC = 0.0f;
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      const orc_f4 grad = grad_spiky(vec, h, c);
+      sumGradCi = add4(sumGradCi, grad);
+      sumSqGradC += dot4(grad, grad);
+    }
+    sumSqGradC += dot4(sumGradCi, sumGradCi);
+    sumSqGradC /= p->restDensity * p->restDensity;
+    lambda[i] = -densityC / (sumSqGradC + p->relaxCFM);
+  }
+}
+
+/* clouds.cl:441-502 */
+void orc_cld_correction(const float* lambda, const orc_u2* se, const orc_f4* pred,
+                        const orc_fluid_params* p, orc_f4* corrOut, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    const float lambdaI = lambda[i];
+    orc_f4 corr = { 0.0f, 0.0f, 0.0f, 0.0f };
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      const float s = lambdaI + lambda[e] + art_pressure(vec, p, c);
+      corr = add4(corr, muls4(s, grad_spiky(vec, h, c)));
+    }
+    corrOut[i] = div4s(corr, p->restDensity);
+  }
+}
+
+/* clouds.cl:508-569 */
+void orc_cld_laplacian_temp(const orc_f4* posP, const float* tempP, const orc_u2* se,
+                            const orc_cloud_params* q, float* lap, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = posP[i];
+    const float temp = tempP[i];
+    float laplacian = 0.0f;
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, posP, e, vec)
+    {
+      laplacian += (temp - tempP[e]) * dot4(vec, grad_spiky(vec, h, c)) / (dot4(vec, vec) + ORC_FLOAT_EPS);
+    }
+    lap[i] = laplacian / q->restDensity;
+  }
+}
+
+/* clouds.cl:575-648 */
+void orc_cld_lambda_temp(const orc_f4* posP, const float* lap, const orc_u2* se,
+                         const orc_cloud_params* q, float* lambdaT, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = posP[i];
+    const float laplacian = lap[i];
+    float sumGradCi = 0.0f, sumSqGradC = 0.0f;
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, posP, e, vec)
+    {
+      const orc_f4 grad = grad_spiky(vec, h, c);
+      const float derivativeTemp = dot4(vec, grad) / (dot4(vec, vec) * q->restDensity + ORC_FLOAT_EPS);
+      sumGradCi += derivativeTemp;
+      sumSqGradC += derivativeTemp * derivativeTemp;
+    }
+    sumSqGradC += sumGradCi * sumGradCi;
+    lambdaT[i] = -laplacian / (sumSqGradC + q->relaxCFM);
+  }
+}
+
+/* clouds.cl:654-721 */
+void orc_cld_correction_temp(const float* lambdaT, const orc_u2* se, const orc_f4* posP,
+                             const orc_cloud_params* q, float* corrT, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = posP[i];
+    const float lambdaI = lambdaT[i];
+    float corr = 0.0f;
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, posP, e, vec)
+    {
+      const orc_f4 grad = grad_spiky(vec, h, c);
+      const float derivativeTemp = dot4(vec, grad) / (dot4(vec, vec) * q->restDensity + ORC_FLOAT_EPS);
+      corr += (lambdaI + lambdaT[e]) * derivativeTemp;
+    }
+    corrT[i] = corr;
+  }
+}
+
+/* clouds.cl:727-784 */
+void orc_cld_vorticity(const orc_f4* pred, const orc_u2* se, const orc_f4* vel,
+                       const orc_fluid_params* p, orc_f4* vort, uint32_t n, const orc_consts* c)
+{
+  (void)p;
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    const orc_f4 velocity = vel[i];
+    orc_f4 w = { 0.0f, 0.0f, 0.0f, 0.0f };
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      w = add4(w, cross4(sub4(vel[e], velocity), grad_spiky(vec, h, c)));
+    }
+    vort[i] = w;
+  }
+}
+
+/* clouds.cl:790-850 */
+void orc_cld_confinement(const orc_f4* pred, const orc_u2* se, const orc_f4* vort,
+                         const orc_fluid_params* p, orc_f4* vel, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    const orc_f4 vorticity = vort[i];
+    orc_f4 nn = { 0.0f, 0.0f, 0.0f, 0.0f };
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      nn = add4(nn, muls4(len4(vort[e]), grad_spiky(vec, h, c)));
+    }
+    const orc_f4 f = mul4s(muls4(p->vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p->timeStep);
+    vel[i] = add4(vel[i], f);
+  }
+}
+
+/* clouds.cl:856-914 */
+void orc_cld_xsph(const orc_f4* pred, const orc_u2* se, const orc_f4* velIn,
+                  const orc_fluid_params* p, orc_f4* velOut, uint32_t n, const orc_consts* c)
+{
+  const float h = c->effectRadius;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 pos = pred[i];
+    const orc_f4 velocity = velIn[i];
+    orc_f4 viscosity = { 0.0f, 0.0f, 0.0f, 0.0f };
+    ORC_CLD_FOR_NEIGHBOURS(c, pos, pred, e, vec)
+    {
+      viscosity = add4(viscosity, mul4s(sub4(velIn[e], velocity), poly6(vec, h, c)));
+    }
+    velOut[i] = add4(velocity, muls4(p->xsphViscosityCoeff, viscosity));
+  }
+}
+
+/* clouds.cl:931-935 */
+void orc_cld_correct_temp(const float* corrT, float* temp, uint32_t n)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
+    temp[i] += 0.3f * corrT[i];
+}
+
+/* clouds.cl:942-952 */
+void orc_cld_update_pos(const orc_f4* pred, const orc_cloud_params* q, orc_f4* pos, uint32_t n,
+                        const orc_consts* c)
+{
+  const float wy = c->absWall[1];
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    pos[i] = pred[i];
+    pos[i].x += (1 - orc_exp32(-(pred[i].y + wy) * 0.2f)) * q->windCoeff * q->timeStep *
+                (float)(q->dim - 2);
+    pos[i].z += (1 - orc_exp32(-(pred[i].y + wy) * 0.3f)) * 0.7f * q->windCoeff * q->timeStep;
+  }
+}
+
+/* clouds.cl:958-966 */
+void orc_cld_update_vel(const orc_f4* totCorr, const orc_fluid_params* p, orc_f4* vel, uint32_t n,
+                        const orc_consts* c)
+{
+  const float m = c->maxVel;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const orc_f4 v = div4s(totCorr[i], p->timeStep + ORC_FLOAT_EPS);
+    const orc_f4 r = { clampf(v.x, -m, m), clampf(v.y, -m, m), clampf(v.z, -m, m), clampf(v.w, -m, m) };
+    vel[i] = r;
+  }
+}
+
+/* RadixSort.cpp:163-189 permutateFloat for a float payload */
+void orc_permute_f1(const uint32_t* idx, float* buf, float* tmp, uint32_t maxN)
+{
+  memcpy(tmp, buf, sizeof(float) * maxN);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)maxN; ++i)
+    buf[i] = tmp[idx[i]];
+}
+
+/* physics/ocl/Clouds.cpp:503-610 without the rendering steps (grid detector, colour, camera
+ * sort). */
+int orc_clouds_update(orc_clouds* s)
+{
+  const orc_consts* c = &s->c;
+  const uint32_t n = s->n, maxN = s->maxN;
+  memcpy(s->tempIn, s->temp, sizeof(float) * maxN);                      /* :516 */
+  orc_cld_heat_ground(s->tempIn, s->pos, &s->q, s->temp, n, c);          /* :517 */
+  orc_cld_buoyancy(s->temp, s->pos, s->cloud, &s->q, s->buoyancy, n, c); /* :519 */
+  orc_cld_adiabatic(s->temp, s->vel, &s->q, s->tempIn, n);               /* :521 */
+  orc_cld_generate(s->tempIn, s->vapor, s->cloud, &s->q, s->gen, n);     /* :523 */
+  memcpy(s->vaporIn, s->vapor, sizeof(float) * maxN);                    /* :525-526 */
+  memcpy(s->cloudIn, s->cloud, sizeof(float) * maxN);
+  orc_cld_phase(s->vaporIn, s->cloudIn, s->gen, &s->q, s->vapor, s->cloud, n); /* :527 */
+  orc_cld_latent(s->tempIn, s->gen, &s->q, s->temp, n);                  /* :529 */
+  orc_cld_predict(s->pos, s->vel, s->buoyancy, &s->q, s->pred, s->totCorr, n); /* :534 */
+  orc_cld_boundary(s->pred, n, c);                                       /* :537-538 */
+  orc_fill_cell_ids(s->pred, s->key, n, c);                              /* :541 */
+  const int rc = orc_radix_sort(s->key, s->idx, maxN);                   /* :543 */
+  if (rc)
+    return rc;
+  orc_permute_f4(s->idx, s->pos, s->tmp, maxN);
+  orc_permute_f4(s->idx, s->vel, s->tmp, maxN);
+  orc_permute_f4(s->idx, s->pred, s->tmp, maxN);
+  orc_permute_f4(s->idx, s->totCorr, s->tmp, maxN);
+  orc_permute_f1(s->idx, s->temp, s->ftmp, maxN);
+  orc_permute_f1(s->idx, s->buoyancy, s->ftmp, maxN);
+  orc_permute_f1(s->idx, s->vapor, s->ftmp, maxN);
+  orc_permute_f1(s->idx, s->cloud, s->ftmp, maxN);
+  orc_permute_f1(s->idx, s->partID, s->ftmp, maxN);
+  orc_reset_start_end(s->se, c);                                         /* :545-547 */
+  orc_fill_start(s->key, s->se, n, c);
+  orc_fill_end(s->key, s->se, n, maxN, c);
+  orc_adjust_end(s->se, c);                                              /* :549-550 */
+  if (s->q.isTempSmoothingEnabled) {                                     /* :554-567 */
+    orc_cld_laplacian_temp(s->pos, s->temp, s->se, &s->q, s->lap, n, c);
+    orc_cld_lambda_temp(s->pos, s->lap, s->se, &s->q, s->lambdaT, n, c);
+    orc_cld_correction_temp(s->lambdaT, s->se, s->pos, &s->q, s->corrT, n, c);
+    orc_cld_correct_temp(s->corrT, s->temp, n);
+  }
+  for (uint32_t it = 0; it < s->iters; ++it) {                           /* :570-587 */
+    orc_cld_density(s->pred, s->se, &s->p, s->density, n, c);
+    orc_cld_lambda(s->pred, s->density, s->se, &s->p, s->lambda, n, c);
+    orc_cld_correction(s->lambda, s->se, s->pred, &s->p, s->corr, n, c);
+    orc_fld_correct_pos(s->corr, s->pred, n);
+    orc_fld_correct_pos(s->corr, s->totCorr, n);
+    orc_cld_boundary(s->pred, n, c);
+  }
+  orc_cld_update_vel(s->totCorr, &s->p, s->vel, n, c);                   /* :590 */
+  if (s->p.isVorticityConfEnabled) {                                     /* :592-602 */
+    orc_cld_vorticity(s->pred, s->se, s->vel, &s->p, s->vort, n, c);
+    orc_cld_confinement(s->pred, s->se, s->vort, &s->p, s->vel, n, c);
+    memcpy(s->velIn, s->vel, sizeof(orc_f4) * maxN);
+    orc_cld_xsph(s->pred, s->se, s->velIn, &s->p, s->vel, n, c);
+  }
+  orc_cld_update_pos(s->pred, &s->q, s->pos, n, c);                      /* :605 */
+  return 0;
+}

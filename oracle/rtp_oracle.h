@@ -2,9 +2,9 @@
  * CPU oracle for the RealTimeParticles physics hot path — TEST INFRASTRUCTURE ONLY.
  *
  * This is a plain-C restatement of the reference's OpenCL kernels and host step orders
- * (physics/ocl/kernels/{define,sph,grid,radixSort,fluids,boids}.cl,
+ * (physics/ocl/kernels/{define,sph,grid,radixSort,fluids,boids,clouds}.cl,
  *  physics/ocl/Fluids.cpp:97-126,400-471, physics/ocl/Boids.cpp:96-120,323-384,
- *  physics/utils/RadixSort.cpp:122-189).  Only the test suite, the smoke check and the
+ *  physics/ocl/Clouds.cpp:126-145,503-610, physics/utils/RadixSort.cpp:122-189).  Only the test suite, the smoke check and the
  * benchmark's CPU-baseline leg may load it; the HIP library never links or calls it.
  *
  * Each kernel is one C function that loops over the reference's work-items.  The
@@ -18,6 +18,7 @@
  *   clamp(x,lo,hi)  = fminf(fmaxf(x,lo),hi);  step(e,x) = x < e ? 0 : 1
  *   division        = IEEE division (no -cl-fast-relaxed-math reciprocal)
  *   denormals       = IEEE (not flushed)
+ *   exp             = orc_exp64 / orc_exp32 below (Cody-Waite + Taylor; the Clouds model)
  */
 #ifndef RTP_ORACLE_H
 #define RTP_ORACLE_H
@@ -218,6 +219,88 @@ int orc_fluids_frame(orc_fluids* s, orc_render* r, int paused);
  * step with p_col permuted by the cell sort and the grid detector; always the camera
  * sort, permuting p_pos, p_col, p_vel and p_acc. */
 int orc_boids_frame(orc_boids* s, orc_render* r, int paused);
+
+/* ---- clouds.cl + Clouds.cpp (the Clouds PBF variant) ---- */
+/* clouds.cl:18-32 CloudParams (52 bytes). */
+typedef struct {
+  uint32_t dim;
+  float timeStep;
+  float restDensity;
+  float groundHeatCoeff;
+  float buoyancyCoeff;
+  float gravCoeff;
+  float adiabaticLapseRate;
+  float phaseTransitionRate;
+  float latentHeatCoeff;
+  uint32_t isTempSmoothingEnabled;
+  float relaxCFM;
+  float initVaporDensityCoeff;
+  float windCoeff;
+} orc_cloud_params;
+
+/* exp pinned by Cody-Waite reduction + degree-13 Taylor polynomial (double; the float form
+ * rounds the double result). */
+double orc_exp64(double x);
+float orc_exp32(float x);
+float orc_cld_ext_heat(float altitude, const orc_consts* c);  /* clouds.cl:67-70 */
+float orc_cld_env_temp(float altitude, const orc_consts* c);  /* clouds.cl:75-86 */
+float orc_cld_svd(float temperature);                          /* clouds.cl:91-96 */
+void orc_cld_init_temp(const orc_f4* pos, float* temp, uint32_t count, const orc_consts* c);
+void orc_cld_init_vapor(const orc_cloud_params* q, const float* temp, float* vapor, uint32_t count);
+void orc_cld_heat_ground(const float* tempIn, const orc_f4* pos, const orc_cloud_params* q,
+                         float* temp, uint32_t n, const orc_consts* c);
+void orc_cld_buoyancy(const float* temp, const orc_f4* pos, const float* cloudDens,
+                      const orc_cloud_params* q, float* buoyancy, uint32_t n, const orc_consts* c);
+void orc_cld_adiabatic(const float* temp, const orc_f4* vel, const orc_cloud_params* q,
+                       float* tempOut, uint32_t n);
+void orc_cld_generate(const float* temp, const float* vapor, const float* cloudDens,
+                      const orc_cloud_params* q, float* gen, uint32_t n);
+void orc_cld_phase(const float* vaporIn, const float* cloudIn, const float* gen,
+                   const orc_cloud_params* q, float* vapor, float* cloudDens, uint32_t n);
+void orc_cld_latent(const float* tempIn, const float* gen, const orc_cloud_params* q, float* temp,
+                    uint32_t n);
+void orc_cld_predict(const orc_f4* pos, const orc_f4* vel, const float* buoyancy,
+                     const orc_cloud_params* q, orc_f4* pred, orc_f4* totCorr, uint32_t n);
+void orc_cld_boundary(orc_f4* pred, uint32_t n, const orc_consts* c);
+void orc_cld_density(const orc_f4* pred, const orc_u2* se, const orc_fluid_params* p,
+                     float* density, uint32_t n, const orc_consts* c);
+void orc_cld_lambda(const orc_f4* pred, const float* density, const orc_u2* se,
+                    const orc_fluid_params* p, float* lambda, uint32_t n, const orc_consts* c);
+void orc_cld_correction(const float* lambda, const orc_u2* se, const orc_f4* pred,
+                        const orc_fluid_params* p, orc_f4* corr, uint32_t n, const orc_consts* c);
+void orc_cld_laplacian_temp(const orc_f4* posP, const float* tempP, const orc_u2* se,
+                            const orc_cloud_params* q, float* lap, uint32_t n, const orc_consts* c);
+void orc_cld_lambda_temp(const orc_f4* posP, const float* lap, const orc_u2* se,
+                         const orc_cloud_params* q, float* lambdaT, uint32_t n, const orc_consts* c);
+void orc_cld_correction_temp(const float* lambdaT, const orc_u2* se, const orc_f4* posP,
+                             const orc_cloud_params* q, float* corrT, uint32_t n, const orc_consts* c);
+void orc_cld_vorticity(const orc_f4* pred, const orc_u2* se, const orc_f4* vel,
+                       const orc_fluid_params* p, orc_f4* vort, uint32_t n, const orc_consts* c);
+void orc_cld_confinement(const orc_f4* pred, const orc_u2* se, const orc_f4* vort,
+                         const orc_fluid_params* p, orc_f4* vel, uint32_t n, const orc_consts* c);
+void orc_cld_xsph(const orc_f4* pred, const orc_u2* se, const orc_f4* velIn,
+                  const orc_fluid_params* p, orc_f4* velOut, uint32_t n, const orc_consts* c);
+void orc_cld_correct_temp(const float* corrT, float* temp, uint32_t n);
+void orc_cld_update_pos(const orc_f4* pred, const orc_cloud_params* q, orc_f4* pos, uint32_t n,
+                        const orc_consts* c);
+void orc_cld_update_vel(const orc_f4* totCorr, const orc_fluid_params* p, orc_f4* vel, uint32_t n,
+                        const orc_consts* c);
+void orc_permute_f1(const uint32_t* idx, float* buf, float* tmp, uint32_t maxN);
+
+typedef struct {
+  orc_consts c;
+  orc_fluid_params p;
+  orc_cloud_params q;
+  uint32_t maxN, n, iters;
+  orc_f4 *pos, *vel, *pred, *totCorr, *corr, *vort, *velIn, *tmp;
+  float *temp, *tempIn, *buoyancy, *vapor, *vaporIn, *cloud, *cloudIn, *gen, *partID;
+  float *density, *lambda, *lap, *lambdaT, *corrT, *ftmp;
+  uint32_t *key, *idx;
+  orc_u2* se;
+} orc_clouds;
+
+/* Clouds::update (Clouds.cpp:503-610) without GL, colour, grid detector or the camera sort. */
+int orc_clouds_update(orc_clouds* s);
 
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_num_threads(void);

@@ -23,7 +23,7 @@ LIB = os.path.join(OUT_DIR, "librtp.so")
 OBJ_DIR = os.path.join(OUT_DIR, "obj")
 
 HIP_SOURCES = ["grid_sort.hip", "pbf.hip", "boids.hip", "mlp.hip", "capi.hip", "batch.hip",
-               "selfcheck.hip", "render.hip"]
+               "selfcheck.hip", "render.hip", "clouds.hip"]
 HOST_SOURCES = ["host/Model.cpp", "host/json_params.cpp"]
 
 ARCH = os.environ.get("RTP_OFFLOAD_ARCH", "gfx950")

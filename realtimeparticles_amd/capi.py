@@ -22,6 +22,8 @@ _int = ctypes.c_int
 
 FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(5)
 FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
+(FIELD_TEMPERATURE, FIELD_VAPOR_DENSITY, FIELD_CLOUD_DENSITY, FIELD_BUOYANCY, FIELD_PARTICLE_ID,
+ FIELD_TOT_CORR_POS, FIELD_LAPLACIAN_TEMP, FIELD_LAMBDA_TEMP) = range(8, 16)
 BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
 
@@ -96,6 +98,26 @@ class FluidParams(ctypes.Structure):
         ("isVorticityConfEnabled", _u32),
         ("vorticityConfCoeff", _f32),
         ("xsphViscosityCoeff", _f32),
+    ]
+
+
+class CloudParams(ctypes.Structure):
+    """rtp_cloud_params == CloudKernelInputs (physics/ocl/Clouds.hpp:14-48)."""
+
+    _fields_ = [
+        ("dim", _u32),
+        ("timeStep", _f32),
+        ("restDensity", _f32),
+        ("groundHeatCoeff", _f32),
+        ("buoyancyCoeff", _f32),
+        ("gravCoeff", _f32),
+        ("adiabaticLapseRate", _f32),
+        ("phaseTransitionRate", _f32),
+        ("latentHeatCoeff", _f32),
+        ("isTempSmoothingEnabled", _u32),
+        ("relaxCFM", _f32),
+        ("initVaporDensityCoeff", _f32),
+        ("windCoeff", _f32),
     ]
 
 
@@ -256,6 +278,23 @@ _SIGS = {
     "rtp_sort_pairs": ([_int, _vp, _u32, _u32, _int, _vp, _vp], _int),
     "rtp_grid_build": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp], _int),
     "rtp_grid_build_noalloc": ([ctypes.POINTER(ModelDesc), _vp, _vp, _vp, _vp, _vp], _int),
+    "rtp_clouds_create": ([ctypes.POINTER(ModelDesc), ctypes.POINTER(_vp)], _int),
+    "rtp_clouds_destroy": ([_vp], _int),
+    "rtp_clouds_set_params": ([_vp, _vp, _vp], _int),
+    "rtp_clouds_get_params": ([_vp, _vp, _vp], _int),
+    "rtp_clouds_set_nb_jacobi_iters": ([_vp, _u32], _int),
+    "rtp_clouds_get_nb_jacobi_iters": ([_vp, ctypes.POINTER(_u32)], _int),
+    "rtp_clouds_set_nb_particles": ([_vp, _u32], _int),
+    "rtp_clouds_load_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_clouds_read_state": ([_vp, _vp, _vp, _u32], _int),
+    "rtp_clouds_update": ([_vp, _u32], _int),
+    "rtp_clouds_pause": ([_vp, _int], _int),
+    "rtp_clouds_sync": ([_vp], _int),
+    "rtp_clouds_read_grid": ([_vp, _vp, _vp, _vp], _int),
+    "rtp_clouds_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_clouds_enable_profiling": ([_vp, _int], _int),
+    "rtp_clouds_profile": ([_vp, ctypes.POINTER(KernelTime), _int, ctypes.POINTER(_int)], _int),
+    "rtp_selfcheck_exp": ([_int, _vp, _vp, _u32, _vp, _vp, _vp], _int),
     "rtp_selfcheck_sqrt": ([_int, _u32, _u32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_u32)],
                            _int),
     "rtp_selfcheck_division": ([_int, ctypes.c_int32, ctypes.c_int32, _u32, _u32, _u32,
@@ -891,6 +930,119 @@ def grid_build_noalloc(pos_dev: int, n: int, max_n: int, box, res, keys_dev: int
     d = _desc(max_n, n, box, res, 0.0, max_parts_in_cell, device)
     _check(lib().rtp_grid_build_noalloc(ctypes.byref(d), pos_dev or None, keys_dev, idx_dev,
                                         start_end_dev, stream or None), "rtp_grid_build_noalloc")
+
+
+def selfcheck_exp(x64, x32, device: int = 0):
+    """The Clouds model's pinned exp on the device: (exp(x64) in double, exp(x32) in float,
+    saturationVaporDensity(x32))."""
+    x64 = np.ascontiguousarray(x64, np.float64)
+    x32 = np.ascontiguousarray(x32, np.float32)
+    assert x64.shape == x32.shape
+    n = x64.shape[0]
+    y64 = np.zeros(n, np.float64)
+    y32 = np.zeros(n, np.float32)
+    s32 = np.zeros(n, np.float32)
+    _check(lib().rtp_selfcheck_exp(int(device), x64.ctypes.data, x32.ctypes.data, n,
+                                   y64.ctypes.data, y32.ctypes.data, s32.ctypes.data),
+           "rtp_selfcheck_exp")
+    return y64, y32, s32
+
+
+CLOUD_DEFAULTS = dict(dim=3, timeStep=0.01, restDensity=400.0, groundHeatCoeff=10.0,
+                      buoyancyCoeff=0.10, gravCoeff=0.0005, adiabaticLapseRate=5.0,
+                      phaseTransitionRate=0.3485, latentHeatCoeff=0.07, isTempSmoothingEnabled=1,
+                      relaxCFM=600.0, initVaporDensityCoeff=0.75, windCoeff=1.0)
+
+
+class Clouds:
+    """Physics::HIP::Clouds over the C ABI: the Clouds model (physics/ocl/Clouds.hpp)."""
+
+    def __init__(self, max_n: int, n: int, box, res, gravity: float = 9.81,
+                 max_parts_in_cell: int = 0, device: int = 0):
+        self._h = _vp()
+        self.max_n, self.n = int(max_n), int(n)
+        self.desc = _desc(max_n, n, box, res, gravity, max_parts_in_cell, device)
+        _check(lib().rtp_clouds_create(ctypes.byref(self.desc), ctypes.byref(self._h)),
+               "rtp_clouds_create")
+        self.num_cells = int(res[0]) * int(res[1]) * int(res[2])
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtp_clouds_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), f"rtp_clouds_{name}")(self._h, *args), f"rtp_clouds_{name}")
+
+    def set_params(self, fluid: dict | None = None, cloud: dict | None = None):
+        p = FluidParams(**{**FLUID_DEFAULTS, **(fluid or {})}) if fluid is not None else None
+        q = CloudParams(**{**CLOUD_DEFAULTS, **(cloud or {})}) if cloud is not None else None
+        self._call("set_params", ctypes.byref(p) if p else None, ctypes.byref(q) if q else None)
+        return p, q
+
+    def params(self):
+        p, q = FluidParams(), CloudParams()
+        self._call("get_params", ctypes.byref(p), ctypes.byref(q))
+        return p, q
+
+    def set_nb_jacobi_iters(self, iters: int) -> None:
+        self._call("set_nb_jacobi_iters", int(iters))
+
+    def set_nb_particles(self, n: int) -> None:
+        self._call("set_nb_particles", int(n))
+        self.n = int(n)
+
+    def load_state(self, pos, vel) -> None:
+        pos, vel = _f4(pos), _f4(vel)
+        assert pos.shape == vel.shape
+        self._call("load_state", pos.ctypes.data, vel.ctypes.data, pos.shape[0])
+        self.n = pos.shape[0]
+
+    def read_state(self, count: int | None = None):
+        count = self.n if count is None else int(count)
+        pos = np.zeros((count, 4), np.float32)
+        vel = np.zeros((count, 4), np.float32)
+        self._call("read_state", pos.ctypes.data, vel.ctypes.data, count)
+        return pos, vel
+
+    def update(self, nsteps: int = 1) -> None:
+        self._call("update", int(nsteps))
+
+    def pause(self, flag: bool) -> None:
+        self._call("pause", int(bool(flag)))
+
+    def sync(self) -> None:
+        self._call("sync")
+
+    def read_grid(self):
+        keys = np.zeros(self.max_n, np.uint32)
+        idx = np.zeros(self.max_n, np.uint32)
+        se = np.zeros((self.num_cells, 2), np.uint32)
+        self._call("read_grid", keys.ctypes.data, idx.ctypes.data, se.ctypes.data)
+        return keys, idx, se
+
+    def read_field(self, field: int, count: int | None = None) -> np.ndarray:
+        count = self.n if count is None else int(count)
+        vec = field in (FIELD_PRED_POS, FIELD_VORTICITY, FIELD_TOT_CORR_POS)
+        out = np.zeros((count, 4) if vec else count, np.float32)
+        self._call("read_field", int(field), out.ctypes.data, count)
+        return out
+
+    def enable_profiling(self, flag: bool) -> None:
+        self._call("enable_profiling", int(bool(flag)))
+
+    def profile(self) -> dict:
+        buf = (KernelTime * 64)()
+        cnt = _int(0)
+        self._call("profile", buf, 64, ctypes.byref(cnt))
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].totalMs)
+                for i in range(min(cnt.value, 64))}
 
 
 def selfcheck_sqrt(lo: int, hi: int, device: int = 0):

@@ -1,0 +1,966 @@
+// The Clouds model (physics/ocl/Clouds.{hpp,cpp}, physics/ocl/kernels/clouds.cl): position
+// based fluids coupled to a per-particle temperature, vapour and cloud density, with a
+// temperature-smoothing constraint, on the uniform grid and radix sort of the fluids model.
+//
+// One Clouds::update (Clouds.cpp:503-610, without the rendering steps) is
+//   k_cld_thermo        heat from the ground, buoyancy, adiabatic cooling, cloud generation,
+//                       phase transition, latent heat, predicted position, the mixed
+//                       boundary (periodic x/z, walls in y) and the cell key, fused
+//   radix sort          of the keys, and k_reorder + k_cld_gather for the ten payloads
+//   k_cld_temp_lambda   Laplacian of the temperature and its constraint factor (one walk)
+//   k_cld_temp_correct  the temperature correction, applied at 0.3
+//   iters x {k_cld_density_lambda, k_cld_correct}   (the last also updates the velocity)
+//   k_cld_vorticity, k_cld_confinement, k_cld_xsph (+ cld_updatePosition), or
+//   k_cld_update_pos without vorticity.
+// The neighbour walks visit the reference's 27 cells in its order (x outer, z inner), with x
+// and z periodic -- a wrapped neighbour's position is shifted by +-2 ABS_WALL -- and y cells
+// beyond the walls skipped (clouds.cl:322-353).  Every expression follows clouds.cl's order
+// with the float4 lanes the reference evaluates (w included), so the results are the CPU
+// oracle's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rtp_device.hpp"
+#include "rtp_exp.hpp"
+#include "rtp_host.hpp"
+#include "rtp_internal.hpp"
+
+namespace rtp {
+namespace {
+
+struct CloudConsts {
+  PbfConsts f;         // grid, FluidParams and the SPH constants of the program
+  rtp_cloud_params q;  // CloudParams (clouds.cl:18-32)
+};
+
+// ---------------------------------------------------------------- float4 as OpenCL sees it
+
+__device__ __forceinline__ float4 sub4(float4 a, float4 b)
+{
+  return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+__device__ __forceinline__ float4 add4(float4 a, float4 b)
+{
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 mul4s(float4 a, float s)
+{
+  return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
+}
+__device__ __forceinline__ float4 muls4(float s, float4 a)
+{
+  return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
+}
+__device__ __forceinline__ float4 div4s(float4 a, float s)
+{
+  return make_float4(a.x / s, a.y / s, a.z / s, a.w / s);
+}
+__device__ __forceinline__ float dot4(float4 a, float4 b)
+{
+  return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
+}
+__device__ __forceinline__ float len4(float4 v) { return __builtin_sqrtf(dot4(v, v)); }
+__device__ __forceinline__ float4 normalize4(float4 v)
+{
+  if (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f && v.w == 0.0f)
+    return v;
+  return div4s(v, __builtin_sqrtf(dot4(v, v)));
+}
+__device__ __forceinline__ float4 cross4(float4 a, float4 b)
+{
+  return make_float4(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0.0f);
+}
+__device__ __forceinline__ float step1(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
+
+// sph.cl:10-14
+__device__ __forceinline__ float poly6(float4 vec, const PbfConsts& c)
+{
+  const float r = len4(vec);
+  return ((1.0f - step1(c.h, r)) * c.poly6) * pown(c.h * c.h - r * r, 3);
+}
+
+// sph.cl:26-34
+__device__ __forceinline__ float4 grad_spiky(float4 vec, const PbfConsts& c)
+{
+  const float r = len4(vec);
+  if (r <= kFloatEps)
+    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 g = mul4s(vec, 1.0f - step1(c.h, r));
+  g = mul4s(g, c.spiky);
+  g = mul4s(g, -3.0f);
+  g = mul4s(g, pown(c.h - r, 2));
+  return div4s(g, r);
+}
+
+// clouds.cl:106-114 (poly6L(artPressureRadius * h) is folded into c.wdq)
+__device__ __forceinline__ float art_pressure(float4 vec, const PbfConsts& c)
+{
+  if (c.p.isArtPressureEnabled == 0)
+    return 0.0f;
+  return -c.p.artPressureCoeff * pown(poly6(vec, c) / c.wdq, c.p.artPressureExp);
+}
+
+// clouds.cl:67-70
+__device__ __forceinline__ float ext_heat(float altitude, const GridConsts& g)
+{
+  return clampf(exp32(-(altitude + g.absWall[1]) / 3.0f), 0.0f, 1.0f);
+}
+
+// clouds.cl:75-86
+__device__ __forceinline__ float env_temp(float altitude, const GridConsts& g)
+{
+  const float coeff = 3.5f;
+  const float tempGround = 293.0f;
+  return -coeff * (altitude + g.absWall[1]) + tempGround;
+}
+
+// clouds.cl:91-96 (double literals: evaluated in double, rounded to the float result)
+__device__ __forceinline__ float svd(float temperature)
+{
+  const double t = (double)temperature;
+  return (float)(217 * exp64(19.5 - 4303.4 / (t - 29.5)) / t);
+}
+
+// clouds.cl:279-300 on a predicted position
+__device__ __forceinline__ float4 mixed_boundary(float4 p, const GridConsts& g)
+{
+  const float wx = g.absWall[0], wy = g.absWall[1], wz = g.absWall[2];
+  const float cx = clampf(p.x, -wx, wx), cy = clampf(p.y, -wy, wy), cz = clampf(p.z, -wz, wz);
+  float4 q = p;
+  if (fabsf(p.x) > wx)
+    q.x = p.x - 2 * cx;
+  if (fabsf(p.y) > wy)
+    q.y = cy;
+  if (fabsf(p.z) > wz)
+    q.z = p.z - 2 * cz;
+  return q;
+}
+
+// The 27-cell walk of clouds.cl:322-353: f(e, vec) for every candidate e of every visited
+// cell, vec = pos - p_e - absWall * sign (sign +-2 on a wrapped x or z).
+template <class F>
+__device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restrict__ src,
+                                         const uint2* __restrict__ se, const GridConsts& g, F&& f)
+{
+  uint32_t ux, uy, uz;
+  cell3d(pos, g, ux, uy, uz);
+  const int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
+  const int32_t rx = g.res[0], ry = g.res[1], rz = g.res[2];
+  for (int ix = -1; ix <= 1; ++ix) {
+    const int32_t ax = cx + ix;
+    const int32_t nx = (ax + rx) % rx;
+    const float sx = ax >= rx ? 2.0f : (ax < 0 ? -2.0f : 0.0f);
+    for (int iy = -1; iy <= 1; ++iy) {
+      const int32_t ay = cy + iy;
+      if (ay >= ry || ay < 0)
+        continue;
+      const int32_t ny = (ay + ry) % ry;
+      for (int iz = -1; iz <= 1; ++iz) {
+        const int32_t az = cz + iz;
+        const int32_t nz = (az + rz) % rz;
+        const float sz = az >= rz ? 2.0f : (az < 0 ? -2.0f : 0.0f);
+        const float4 shift =
+          make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * sz, 0.0f * 0.0f);
+        const uint2 r = se[(nx * ry + ny) * rz + nz];
+        for (uint32_t e = r.x; e <= r.y; ++e)
+          f(e, sub4(sub4(pos, src[e]), shift));
+      }
+    }
+  }
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+// --------------------------------------------------------------------------------- kernels
+
+// cld_initTemperature (clouds.cl:116-121), cld_initVaporDensity (:127-133), and the host's
+// zero cloud density and particle ids (Clouds.cpp:485-494), over all maxN slots.
+__global__ __launch_bounds__(256) void k_cld_init(const float4* __restrict__ pos,
+                                                  float* __restrict__ temp,
+                                                  float* __restrict__ vapor,
+                                                  float* __restrict__ cloud,
+                                                  float* __restrict__ pid, uint32_t count,
+                                                  CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  const float t = env_temp(pos[i].y, c.f.g);
+  temp[i] = t;
+  vapor[i] = c.q.initVaporDensityCoeff * svd(t);
+  cloud[i] = 0.0f;
+  pid[i] = (float)i;
+}
+
+// Clouds.cpp:516-541: the thermodynamics (clouds.cl:159-251), cld_predictPosition (:257-273),
+// cld_applyMixedBoundaryConditions (:279-300) and fillCellIDs (grid.cl:76-86) of particle t;
+// resetStartEndCell (grid.cl:91-96) over the cells.  Intermediate buffers of the reference
+// (p_tempIn, p_cloudGen, p_vaporDensIn, p_cloudDensIn) stay in registers.
+__global__ __launch_bounds__(256) void k_cld_thermo(const float4* __restrict__ pos,
+                                                    const float4* __restrict__ vel,
+                                                    float* __restrict__ temp,
+                                                    float* __restrict__ vapor,
+                                                    float* __restrict__ cloud,
+                                                    float* __restrict__ buoyancy,
+                                                    float4* __restrict__ pred,
+                                                    float4* __restrict__ totCorr,
+                                                    uint32_t* __restrict__ key,
+                                                    uint2* __restrict__ startEnd, uint32_t n,
+                                                    CloudConsts c)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < c.f.g.numCells)
+    startEnd[t] = make_uint2(1u, 0u);
+  if (t >= n)
+    return;
+  const rtp_cloud_params& q = c.q;
+  const GridConsts& g = c.f.g;
+  const float4 p = pos[t], v = vel[t];
+  const float tempIn = temp[t];
+  // cld_heatFromGround
+  const float heated = fminf(tempIn + ext_heat(p.y, g) * q.groundHeatCoeff * q.timeStep, 313.0f);
+  // cld_computeBuoyancy
+  const float envTemp = env_temp(p.y, g);
+  const float cd = cloud[t];
+  const float buoy = q.buoyancyCoeff * (heated - envTemp) / envTemp - q.gravCoeff * c.f.gravity * cd;
+  // cld_applyAdiabaticCooling
+  const float cooled = fmaxf(heated - q.adiabaticLapseRate * v.y * q.timeStep, 223.0f);
+  // cld_generateCloud
+  const float vd = vapor[t];
+  const float gen = q.phaseTransitionRate * (vd - svd(cooled));
+  // cld_applyPhaseTransition
+  cloud[t] = fmaxf(cd + gen * q.timeStep, 0.0f);
+  vapor[t] = fmaxf(vd - gen * q.timeStep, 0.0f);
+  // cld_applyLatentHeat
+  temp[t] = cooled + fmaxf(q.latentHeatCoeff * gen * q.timeStep, 0.0f);
+  buoyancy[t] = buoy;
+  // cld_predictPosition
+  const float4 predVel = add4(v, mul4s(make_float4(0.0f, buoy, 0.0f, 0.0f), q.timeStep));
+  totCorr[t] = mul4s(predVel, q.timeStep);
+  const float4 pp = mixed_boundary(add4(p, mul4s(predVel, q.timeStep)), g);
+  pred[t] = pp;
+  key[t] = cell1d(pp, g);
+}
+
+// The payloads the cell sort permutes besides p_pos, p_vel and p_predPos (k_reorder):
+// p_totCorrPos and the float buffers p_temp, p_buoyancy, p_vaporDens, p_cloudDens, p_partID
+// (Clouds.cpp:543), from [0] into [1] by the permutation, over [0, sortN).
+struct CloudGather {
+  const float4* tot[2];
+  float4* totOut;
+  const float* src[5];
+  float* dst[5];
+};
+__global__ __launch_bounds__(256) void k_cld_gather(const uint32_t* __restrict__ idx, uint32_t sortN,
+                                                    CloudGather a)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sortN)
+    return;
+  const uint32_t j = idx[i];
+  a.totOut[i] = a.tot[0][j];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    a.dst[k][i] = a.src[k][j];
+}
+
+// cld_computeLaplacianTemp (clouds.cl:508-569) and cld_computeConstraintFactorTemp
+// (:575-648) in one walk over p_pos: the two sums are independent and each keeps the
+// reference's order.
+__global__ __launch_bounds__(256) void k_cld_temp_lambda(const float4* __restrict__ posP,
+                                                         const float* __restrict__ tempP,
+                                                         const uint2* __restrict__ se,
+                                                         float* __restrict__ lap,
+                                                         float* __restrict__ lambdaT, uint32_t n,
+                                                         CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = posP[i];
+  const float temp = tempP[i];
+  float laplacian = 0.0f, sumGradCi = 0.0f, sumSqGradC = 0.0f;
+  const float rho0 = c.q.restDensity;
+  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e, float4 vec) {
+    const float4 grad = grad_spiky(vec, c.f);
+    const float vg = dot4(vec, grad), vv = dot4(vec, vec);
+    laplacian = laplacian + (temp - tempP[e]) * vg / (vv + kFloatEps);
+    const float derivativeTemp = vg / (vv * rho0 + kFloatEps);
+    sumGradCi = sumGradCi + derivativeTemp;
+    sumSqGradC = sumSqGradC + derivativeTemp * derivativeTemp;
+  });
+  const float l = laplacian / rho0;
+  lap[i] = l;
+  sumSqGradC = sumSqGradC + sumGradCi * sumGradCi;
+  lambdaT[i] = -l / (sumSqGradC + c.q.relaxCFM);
+}
+
+// cld_computeConstraintCorrectionTemp (clouds.cl:654-721) and cld_correctTemperature
+// (:931-935): temp += 0.3 * correction.
+__global__ __launch_bounds__(256) void k_cld_temp_correct(const float4* __restrict__ posP,
+                                                          const float* __restrict__ lambdaT,
+                                                          const uint2* __restrict__ se,
+                                                          float* __restrict__ temp, uint32_t n,
+                                                          CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = posP[i];
+  const float lambdaI = lambdaT[i];
+  const float rho0 = c.q.restDensity;
+  float corr = 0.0f;
+  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e, float4 vec) {
+    const float4 grad = grad_spiky(vec, c.f);
+    const float derivativeTemp = dot4(vec, grad) / (dot4(vec, vec) * rho0 + kFloatEps);
+    corr = corr + (lambdaI + lambdaT[e]) * derivativeTemp;
+  });
+  temp[i] = temp[i] + 0.3f * corr;
+}
+
+// cld_computeDensity (clouds.cl:305-359) + cld_computeConstraintFactor (:365-435).
+__global__ __launch_bounds__(256) void k_cld_density_lambda(const float4* __restrict__ pred,
+                                                            const uint2* __restrict__ se,
+                                                            float* __restrict__ density,
+                                                            float* __restrict__ lambda, uint32_t n,
+                                                            CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = pred[i];
+  // the density sum and the gradient sums are independent: one walk, each in its order
+  float fluidDensity = 0.0f;
+  float4 sumGradCi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float sumSqGradC = 0.0f;
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t, float4 vec) {
+    fluidDensity = fluidDensity + poly6(vec, c.f);
+    const float4 grad = grad_spiky(vec, c.f);
+    sumGradCi = add4(sumGradCi, grad);
+    sumSqGradC = sumSqGradC + dot4(grad, grad);
+  });
+  density[i] = fluidDensity;
+  const rtp_fluid_params& p = c.f.p;
+  const float densityC = fluidDensity / p.restDensity - 1.0f;
+  sumSqGradC = sumSqGradC + dot4(sumGradCi, sumGradCi);
+  sumSqGradC = sumSqGradC / (p.restDensity * p.restDensity);
+  lambda[i] = -densityC / (sumSqGradC + p.relaxCFM);
+}
+
+// cld_computeConstraintCorrection (clouds.cl:441-502), cld_correctPosition on p_predPos and
+// p_totCorrPos (:920-925) and cld_applyMixedBoundaryConditions (Clouds.cpp:573-586); on the
+// last iteration also cld_updateVel (clouds.cl:958-966).
+__global__ __launch_bounds__(256) void k_cld_correct(const float4* __restrict__ pred,
+                                                     const float* __restrict__ lambda,
+                                                     const uint2* __restrict__ se,
+                                                     float4* __restrict__ predOut,
+                                                     float4* __restrict__ totCorr,
+                                                     float4* __restrict__ vel, uint32_t n,
+                                                     CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = pred[i];
+  const float lambdaI = lambda[i];
+  float4 corr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
+    const float s = lambdaI + lambda[e] + art_pressure(vec, c.f);
+    corr = add4(corr, muls4(s, grad_spiky(vec, c.f)));
+  });
+  const float4 corrPos = div4s(corr, c.f.p.restDensity);
+  predOut[i] = mixed_boundary(add4(pos, corrPos), c.f.g);
+  const float4 tc = add4(totCorr[i], corrPos);
+  totCorr[i] = tc;
+  if (vel) {
+    const float m = c.f.maxVel;
+    const float4 u = div4s(tc, c.f.p.timeStep + kFloatEps);
+    vel[i] = make_float4(clampf(u.x, -m, m), clampf(u.y, -m, m), clampf(u.z, -m, m),
+                         clampf(u.w, -m, m));
+  }
+}
+
+// cld_updateVel alone (a step without Jacobi iterations).
+__global__ __launch_bounds__(256) void k_cld_update_vel(const float4* __restrict__ totCorr,
+                                                        float4* __restrict__ vel, uint32_t n,
+                                                        CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float m = c.f.maxVel;
+  const float4 u = div4s(totCorr[i], c.f.p.timeStep + kFloatEps);
+  vel[i] = make_float4(clampf(u.x, -m, m), clampf(u.y, -m, m), clampf(u.z, -m, m),
+                       clampf(u.w, -m, m));
+}
+
+// cld_computeVorticity (clouds.cl:727-784).
+__global__ __launch_bounds__(256) void k_cld_vorticity(const float4* __restrict__ pred,
+                                                       const uint2* __restrict__ se,
+                                                       const float4* __restrict__ vel,
+                                                       float4* __restrict__ vort, uint32_t n,
+                                                       CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = pred[i];
+  const float4 velocity = vel[i];
+  float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
+    w = add4(w, cross4(sub4(vel[e], velocity), grad_spiky(vec, c.f)));
+  });
+  vort[i] = w;
+}
+
+// cld_applyVorticityConfinement (clouds.cl:790-850), written to the copy XSPH reads
+// (p_velInViscosity, Clouds.cpp:597-599).
+__global__ __launch_bounds__(256) void k_cld_confinement(const float4* __restrict__ pred,
+                                                         const uint2* __restrict__ se,
+                                                         const float4* __restrict__ vort,
+                                                         const float4* __restrict__ vel,
+                                                         float4* __restrict__ velOut, uint32_t n,
+                                                         CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pos = pred[i];
+  const float4 vorticity = vort[i];
+  float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
+    nn = add4(nn, muls4(len4(vort[e]), grad_spiky(vec, c.f)));
+  });
+  const rtp_fluid_params& p = c.f.p;
+  velOut[i] = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
+}
+
+// cld_updatePosition (clouds.cl:942-952): the predicted position plus the wind.
+__device__ __forceinline__ float4 cld_update_pos(float4 pp, const CloudConsts& c)
+{
+  const float wy = c.f.g.absWall[1];
+  const rtp_cloud_params& q = c.q;
+  float4 o = pp;
+  o.x = o.x + (1 - exp32(-(pp.y + wy) * 0.2f)) * q.windCoeff * q.timeStep * (float)(q.dim - 2);
+  o.z = o.z + (1 - exp32(-(pp.y + wy) * 0.3f)) * 0.7f * q.windCoeff * q.timeStep;
+  return o;
+}
+
+// cld_applyXsphViscosityCorrection (clouds.cl:856-914) + cld_updatePosition.
+__global__ __launch_bounds__(256) void k_cld_xsph(const float4* __restrict__ pred,
+                                                  const uint2* __restrict__ se,
+                                                  const float4* __restrict__ velIn,
+                                                  float4* __restrict__ velOut,
+                                                  float4* __restrict__ pos, uint32_t n,
+                                                  CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float4 pp = pred[i];
+  const float4 velocity = velIn[i];
+  float4 viscosity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  cld_walk(pp, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
+    viscosity = add4(viscosity, mul4s(sub4(velIn[e], velocity), poly6(vec, c.f)));
+  });
+  velOut[i] = add4(velocity, muls4(c.f.p.xsphViscosityCoeff, viscosity));
+  pos[i] = cld_update_pos(pp, c);
+}
+
+__global__ __launch_bounds__(256) void k_cld_update_pos(const float4* __restrict__ pred,
+                                                        float4* __restrict__ pos, uint32_t n,
+                                                        CloudConsts c)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    pos[i] = cld_update_pos(pred[i], c);
+}
+
+__global__ __launch_bounds__(256) void k_cld_exp(const double* __restrict__ x64, double* __restrict__ y64,
+                                                 const float* __restrict__ x32, float* __restrict__ y32,
+                                                 float* __restrict__ s32, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  y64[i] = exp64(x64[i]);
+  y32[i] = exp32(x32[i]);
+  s32[i] = svd(x32[i]);
+}
+
+}  // namespace
+}  // namespace rtp
+
+using namespace rtp;
+
+// ------------------------------------------------------------------------------- the handle
+
+struct rtp_clouds {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  GridState grid;
+  CloudConsts c{};
+  uint32_t n = 0;
+  uint32_t iters = 2;  // "Nb Jacobi Iterations" of initCloudsJson (Clouds.cpp:68)
+  bool paused = false;
+  // state: ping-pong partners of the cell sort's gather (the partners hold the same data
+  // outside the sorted range); predicted positions also ping-pong between Jacobi iterations
+  float4* pos[2] = {};
+  float4* vel[2] = {};
+  float4* tot[2] = {};
+  float4* pred[2] = {};
+  float* temp[2] = {};
+  float* buoy[2] = {};
+  float* vapor[2] = {};
+  float* cloud[2] = {};
+  float* pid[2] = {};
+  float4* vort = nullptr;
+  float4* velTmp = nullptr;
+  float* density = nullptr;
+  float* lambda = nullptr;
+  float* lap = nullptr;
+  float* lambdaT = nullptr;
+  int cur = 0;  // state in [cur]
+  int pc = 0;   // predicted positions in pred[pc]
+  Profiler prof;
+
+  void release()
+  {
+    grid.release();
+    for (int k = 0; k < 2; ++k) {
+      dev_free(pos[k]);
+      dev_free(vel[k]);
+      dev_free(tot[k]);
+      dev_free(pred[k]);
+      dev_free(temp[k]);
+      dev_free(buoy[k]);
+      dev_free(vapor[k]);
+      dev_free(cloud[k]);
+      dev_free(pid[k]);
+    }
+    dev_free(vort);
+    dev_free(velTmp);
+    dev_free(density);
+    dev_free(lambda);
+    dev_free(lap);
+    dev_free(lambdaT);
+    prof.destroy();
+    if (stream)
+      (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  int step()
+  {
+    hipStream_t s = stream;
+    GridState& G = grid;
+    const uint32_t b256 = blocks_for(std::max(n, G.g.numCells), 256);
+    RTP_PROF(prof, "cld_thermo_predict", s,
+             hipLaunchKernelGGL(k_cld_thermo, dim3(b256), dim3(256), 0, s, pos[cur], vel[cur],
+                                temp[cur], vapor[cur], cloud[cur], buoy[cur], pred[pc], tot[cur],
+                                G.key[G.keyCur], G.startEnd, n, c));
+    int rc = G.sort(n, s, prof);
+    if (rc)
+      return rc;
+    ReorderArgs a{};
+    a.key = G.key[G.keyCur];
+    a.idx = G.idx[G.idxCur];
+    a.sortN = G.sortedN;
+    a.n = n;
+    a.maxN = G.maxN;
+    a.srcA = pos[cur];
+    a.dstA = pos[1 - cur];
+    a.srcB = vel[cur];
+    a.dstB = vel[1 - cur];
+    a.srcC = pred[pc];
+    a.dstC = pred[1 - pc];
+    a.clampC = 0;
+    a.startEnd = G.startEnd;
+    a.g = c.f.g;
+    RTP_PROF(prof, "reorder_cells", s, launch_reorder(a, s));
+    if (G.sortedN) {
+      CloudGather gth{};
+      gth.tot[0] = tot[cur];
+      gth.totOut = tot[1 - cur];
+      float* const* srcs[5] = { temp, buoy, vapor, cloud, pid };
+      for (int k = 0; k < 5; ++k) {
+        gth.src[k] = srcs[k][cur];
+        gth.dst[k] = srcs[k][1 - cur];
+      }
+      RTP_PROF(prof, "cld_gather", s,
+               hipLaunchKernelGGL(k_cld_gather, dim3(blocks_for(G.sortedN, 256)), dim3(256), 0, s,
+                                  G.idx[G.idxCur], G.sortedN, gth));
+    }
+    cur = 1 - cur;
+    pc = 1 - pc;
+    const int gathered = pc;
+    const uint32_t bn = blocks_for(std::max(n, 1u), 256);
+    if (n && c.q.isTempSmoothingEnabled) {
+      RTP_PROF(prof, "cld_temp_lambda", s,
+               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bn), dim3(256), 0, s, pos[cur], temp[cur],
+                                  G.startEnd, lap, lambdaT, n, c));
+      RTP_PROF(prof, "cld_temp_correct", s,
+               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bn), dim3(256), 0, s, pos[cur], lambdaT,
+                                  G.startEnd, temp[cur], n, c));
+    }
+    for (uint32_t k = 0; k < iters && n; ++k) {
+      RTP_PROF(prof, "cld_density_lambda", s,
+               hipLaunchKernelGGL(k_cld_density_lambda, dim3(bn), dim3(256), 0, s, pred[pc],
+                                  G.startEnd, density, lambda, n, c));
+      RTP_PROF(prof, "cld_correct", s,
+               hipLaunchKernelGGL(k_cld_correct, dim3(bn), dim3(256), 0, s, pred[pc], lambda,
+                                  G.startEnd, pred[1 - pc], tot[cur],
+                                  k + 1 == iters ? vel[cur] : nullptr, n, c));
+      pc = 1 - pc;
+    }
+    if (iters == 0 && n)
+      RTP_PROF(prof, "cld_update_vel", s,
+               hipLaunchKernelGGL(k_cld_update_vel, dim3(bn), dim3(256), 0, s, tot[cur], vel[cur], n, c));
+    // the sorted range's stale tail [n, sortedN) lives in the gather target
+    if (pc != gathered && G.sortedN > n)
+      launch_copy_f4(pred[pc], pred[gathered], n, G.sortedN, s);
+    if (!n)
+      return RTP_OK;
+    if (c.f.p.isVorticityConfEnabled) {
+      RTP_PROF(prof, "cld_vorticity", s,
+               hipLaunchKernelGGL(k_cld_vorticity, dim3(bn), dim3(256), 0, s, pred[pc], G.startEnd,
+                                  vel[cur], vort, n, c));
+      RTP_PROF(prof, "cld_confinement", s,
+               hipLaunchKernelGGL(k_cld_confinement, dim3(bn), dim3(256), 0, s, pred[pc],
+                                  G.startEnd, vort, vel[cur], velTmp, n, c));
+      RTP_PROF(prof, "cld_xsph", s,
+               hipLaunchKernelGGL(k_cld_xsph, dim3(bn), dim3(256), 0, s, pred[pc], G.startEnd,
+                                  velTmp, vel[cur], pos[cur], n, c));
+    } else {
+      RTP_PROF(prof, "cld_update_pos", s,
+               hipLaunchKernelGGL(k_cld_update_pos, dim3(bn), dim3(256), 0, s, pred[pc], pos[cur],
+                                  n, c));
+    }
+    return RTP_OK;
+  }
+};
+
+namespace {
+
+void refresh(rtp_clouds* h)
+{
+  refresh_pbf_consts(h->c.f);
+}
+
+// fld_* defaults are replaced by initCloudsJson's (Clouds.cpp:65-97) at the reference's
+// reset; these are CloudKernelInputs' own defaults (Clouds.hpp:14-48).
+rtp_cloud_params default_cloud_params()
+{
+  rtp_cloud_params q{};
+  q.dim = 3;
+  q.timeStep = 0.01f;
+  q.restDensity = 400.0f;
+  q.groundHeatCoeff = 10.0f;
+  q.buoyancyCoeff = 0.10f;
+  q.gravCoeff = 0.0005f;
+  q.adiabaticLapseRate = 5.0f;
+  q.phaseTransitionRate = 0.3485f;
+  q.latentHeatCoeff = 0.07f;
+  q.isTempSmoothingEnabled = 1;
+  q.relaxCFM = 600.0f;
+  q.initVaporDensityCoeff = 0.75f;
+  q.windCoeff = 1.0f;
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
+{
+  RTP_REQUIRE(d && out, "NULL argument");
+  *out = nullptr;
+  RTP_REQUIRE(d->maxNbParticles > 0, "maxNbParticles must be positive");
+  RTP_REQUIRE(d->currNbParticles <= d->maxNbParticles, "currNbParticles exceeds maxNbParticles");
+  int rc = check_device(d->device);
+  if (rc)
+    return rc;
+  auto* h = new rtp_clouds();
+  h->device = d->device;
+  if ((rc = make_grid(d, 100u, &h->grid.g))) {
+    delete h;
+    return rc;
+  }
+  init_pbf_consts(h->c.f, h->grid.g, d->boxSize[0], d->gridRes[0], d->gravity);
+  h->c.q = default_cloud_params();
+  refresh(h);
+  h->n = d->currNbParticles;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(RTP_ERR_HIP, "hipStreamCreate failed");
+  }
+  const uint32_t maxN = d->maxNbParticles;
+  rc = h->grid.alloc(maxN);
+  for (int k = 0; k < 2 && !rc; ++k) {
+    if ((rc = dev_alloc(&h->pos[k], maxN)) || (rc = dev_alloc(&h->vel[k], maxN)) ||
+        (rc = dev_alloc(&h->tot[k], maxN)) || (rc = dev_alloc(&h->pred[k], maxN)) ||
+        (rc = dev_alloc(&h->temp[k], maxN)) || (rc = dev_alloc(&h->buoy[k], maxN)) ||
+        (rc = dev_alloc(&h->vapor[k], maxN)) || (rc = dev_alloc(&h->cloud[k], maxN)) ||
+        (rc = dev_alloc(&h->pid[k], maxN)))
+      break;
+  }
+  if (!rc)
+    (void)((rc = dev_alloc(&h->vort, maxN)) || (rc = dev_alloc(&h->velTmp, maxN)) ||
+           (rc = dev_alloc(&h->density, maxN)) || (rc = dev_alloc(&h->lambda, maxN)) ||
+           (rc = dev_alloc(&h->lap, maxN)) || (rc = dev_alloc(&h->lambdaT, maxN)));
+  if (rc) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  std::vector<float> zeros(4, 0.0f);
+  if ((rc = rtp_clouds_load_state(h, zeros.data(), zeros.data(), 0))) {
+    h->release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return RTP_OK;
+}
+
+int rtp_clouds_destroy(rtp_clouds* h)
+{
+  if (!h)
+    return RTP_OK;
+  (void)hipSetDevice(h->device);
+  h->release();
+  delete h;
+  return RTP_OK;
+}
+
+int rtp_clouds_set_params(rtp_clouds* h, const rtp_fluid_params* p, const rtp_cloud_params* q)
+{
+  RTP_REQUIRE(h && (p || q), "NULL argument");
+  if (p) {
+    RTP_REQUIRE(p->restDensity != 0.0f, "restDensity must be non-zero");
+    RTP_REQUIRE(p->artPressureExp <= kMaxArtPressureExp, "artPressureExp must be at most 64");
+  }
+  if (q)
+    RTP_REQUIRE(q->restDensity != 0.0f, "cloud restDensity must be non-zero");
+  if (p)
+    h->c.f.p = *p;
+  if (q)
+    h->c.q = *q;
+  refresh(h);
+  return RTP_OK;
+}
+
+int rtp_clouds_get_params(rtp_clouds* h, rtp_fluid_params* p, rtp_cloud_params* q)
+{
+  RTP_REQUIRE(h && (p || q), "NULL argument");
+  if (p)
+    *p = h->c.f.p;
+  if (q)
+    *q = h->c.q;
+  return RTP_OK;
+}
+
+int rtp_clouds_set_nb_jacobi_iters(rtp_clouds* h, uint32_t iters)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(iters <= 1000, "iteration count out of range");
+  h->iters = iters;
+  return RTP_OK;
+}
+
+int rtp_clouds_get_nb_jacobi_iters(rtp_clouds* h, uint32_t* iters)
+{
+  RTP_REQUIRE(h && iters, "NULL argument");
+  *iters = h->iters;
+  return RTP_OK;
+}
+
+int rtp_clouds_set_nb_particles(rtp_clouds* h, uint32_t n)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(n <= h->grid.maxN, "nb particles exceeds maxNbParticles");
+  // keys in [n, old n) stay in the whole-buffer sort (RadixSort.cpp:134-161)
+  h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->n);
+  if (h->grid.dirtyEnd <= n)
+    h->grid.dirtyEnd = std::max(h->grid.dirtyEnd, h->grid.sortedN);
+  h->n = n;
+  return RTP_OK;
+}
+
+int rtp_clouds_load_state(rtp_clouds* h, const float* pos4, const float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_REQUIRE(count == 0 || (pos4 && vel4), "NULL state arrays");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  const float inf = INFINITY;
+  const uint32_t maxN = h->grid.maxN;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int rc = 0;
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = upload_f4(h->pos[k], pos4, count, maxN, make_float4(inf, inf, inf, 0.0f), h->stream)) ||
+        (rc = upload_f4(h->vel[k], vel4, count, maxN, zero4, h->stream)) ||
+        (rc = upload_f4(h->pred[k], nullptr, 0, maxN, zero4, h->stream)) ||
+        (rc = upload_f4(h->tot[k], nullptr, 0, maxN, zero4, h->stream)))
+      return rc;
+    // Clouds::initCloudsParticles (Clouds.cpp:485-499): cloud density 0, particle ids, then
+    // the temperature and vapour density over all maxN slots
+    hipLaunchKernelGGL(k_cld_init, dim3(blocks_for(maxN, 256)), dim3(256), 0, h->stream, h->pos[k],
+                       h->temp[k], h->vapor[k], h->cloud[k], h->pid[k], maxN, h->c);
+  }
+  RTP_HIP_TRY(hipMemsetAsync(h->buoy[0], 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->buoy[1], 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->density, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->lambda, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->lap, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->lambdaT, 0, sizeof(float) * maxN, h->stream));
+  RTP_HIP_TRY(hipMemsetAsync(h->vort, 0, sizeof(float4) * maxN, h->stream));
+  h->cur = 0;
+  h->pc = 0;
+  h->n = count;
+  h->grid.reset_keys(h->stream);  // resetCellIDs (Clouds.cpp:396)
+  RTP_HIP_TRY(hipGetLastError());
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_clouds_read_state(rtp_clouds* h, float* pos4, float* vel4, uint32_t count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (pos4)
+    RTP_HIP_TRY(hipMemcpyAsync(pos4, h->pos[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  if (vel4)
+    RTP_HIP_TRY(hipMemcpyAsync(vel4, h->vel[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_clouds_update(rtp_clouds* h, uint32_t nsteps)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  if (h->paused)
+    return RTP_OK;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const int rc = h->step();
+    if (rc)
+      return rc;
+  }
+  RTP_HIP_TRY(hipGetLastError());
+  return RTP_OK;
+}
+
+int rtp_clouds_pause(rtp_clouds* h, int pause)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->paused = pause != 0;
+  return RTP_OK;
+}
+
+int rtp_clouds_sync(rtp_clouds* h)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_clouds_read_grid(rtp_clouds* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  return h->grid.read(cellIDs, sortIdx, startEnd, h->stream);
+}
+
+int rtp_clouds_read_field(rtp_clouds* h, int field, float* out, uint32_t count)
+{
+  RTP_REQUIRE(h && out, "NULL argument");
+  RTP_REQUIRE(count <= h->grid.maxN, "count exceeds maxNbParticles");
+  RTP_HIP_TRY(hipSetDevice(h->device));
+  const void* src = nullptr;
+  size_t elem = sizeof(float);
+  const int c = h->cur;
+  switch (field) {
+  case RTP_FIELD_DENSITY: src = h->density; break;
+  case RTP_FIELD_LAMBDA: src = h->lambda; break;
+  case RTP_FIELD_PRED_POS: src = h->pred[h->pc]; elem = sizeof(float4); break;
+  case RTP_FIELD_VORTICITY: src = h->vort; elem = sizeof(float4); break;
+  case RTP_FIELD_TEMPERATURE: src = h->temp[c]; break;
+  case RTP_FIELD_VAPOR_DENSITY: src = h->vapor[c]; break;
+  case RTP_FIELD_CLOUD_DENSITY: src = h->cloud[c]; break;
+  case RTP_FIELD_BUOYANCY: src = h->buoy[c]; break;
+  case RTP_FIELD_PARTICLE_ID: src = h->pid[c]; break;
+  case RTP_FIELD_TOT_CORR_POS: src = h->tot[c]; elem = sizeof(float4); break;
+  case RTP_FIELD_LAPLACIAN_TEMP: src = h->lap; break;
+  case RTP_FIELD_LAMBDA_TEMP: src = h->lambdaT; break;
+  default: return fail(RTP_ERR_INVALID, "unknown field for clouds");
+  }
+  RTP_HIP_TRY(hipMemcpyAsync(out, src, elem * count, hipMemcpyDeviceToHost, h->stream));
+  RTP_HIP_TRY(hipStreamSynchronize(h->stream));
+  return RTP_OK;
+}
+
+int rtp_clouds_enable_profiling(rtp_clouds* h, int enable)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  h->prof.enabled = enable != 0;
+  return RTP_OK;
+}
+
+int rtp_clouds_profile(rtp_clouds* h, rtp_kernel_time* out, int capacity, int* count)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return h->prof.report(out, capacity, count);
+}
+
+int rtp_selfcheck_exp(int device, const double* x64, const float* x32, uint32_t n, double* y64,
+                      float* y32, float* svd32)
+{
+  RTP_REQUIRE(n == 0 || (x64 && x32 && y64 && y32 && svd32), "NULL argument");
+  if (n == 0)
+    return RTP_OK;
+  int rc = check_device(device);
+  if (rc)
+    return rc;
+  double *dx = nullptr, *dy = nullptr;
+  float *fx = nullptr, *fy = nullptr, *fs = nullptr;
+  if ((rc = dev_alloc(&dx, n)) || (rc = dev_alloc(&dy, n)) || (rc = dev_alloc(&fx, n)) ||
+      (rc = dev_alloc(&fy, n)) || (rc = dev_alloc(&fs, n))) {
+    dev_free(dx);
+    dev_free(dy);
+    dev_free(fx);
+    dev_free(fy);
+    dev_free(fs);
+    return rc;
+  }
+  if (hipMemcpy(dx, x64, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(fx, x32, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(RTP_ERR_HIP, "hipMemcpy inputs");
+  if (!rc) {
+    hipLaunchKernelGGL(k_cld_exp, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, dx, dy, fx, fy,
+                       fs, n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+      rc = fail(RTP_ERR_HIP, "exp self-check launch failed");
+  }
+  if (!rc && (hipMemcpy(y64, dy, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(y32, fy, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(svd32, fs, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(RTP_ERR_HIP, "hipMemcpy results");
+  dev_free(dx);
+  dev_free(dy);
+  dev_free(fx);
+  dev_free(fy);
+  dev_free(fs);
+  return rc;
+}
+
+}  // extern "C"

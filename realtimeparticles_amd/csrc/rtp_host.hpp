@@ -391,5 +391,100 @@ void refresh_pbf_consts(PbfConsts& c)
   }
 }
 
+// ------------------------------------------------------------------------- shared grid state
+
+// Buffers of the uniform grid that both models own: the reference's p_cellID,
+// RadixSortIndices and c_startEndPartID (plus the sort's ping-pong partners and counts).
+struct GridState {
+  GridConsts g{};
+  uint32_t maxN = 0;
+  uint32_t* key[2] = { nullptr, nullptr };
+  uint32_t* idx[2] = { nullptr, nullptr };
+  uint2* startEnd = nullptr;
+  uint32_t* counts = nullptr;
+  int keyCur = 0;   // key[keyCur] is p_cellID
+  int idxCur = 0;   // idx[idxCur] holds the last permutation over [0, sortedN)
+  uint32_t sortedN = 0;
+  uint32_t dirtyEnd = 0;  // keys in [n, dirtyEnd) are not the reset tail 2*NC+i
+  uint32_t permN = 0;     // range of the permutation in idx[idxCur] (cell or camera sort)
+  int sortBits = 0;
+
+  int alloc(uint32_t maxN_)
+  {
+    maxN = maxN_;
+    int rc = 0;
+    if ((rc = dev_alloc(&key[0], maxN)) || (rc = dev_alloc(&key[1], maxN)) ||
+        (rc = dev_alloc(&idx[0], maxN)) || (rc = dev_alloc(&idx[1], maxN)) ||
+        (rc = dev_alloc(&startEnd, g.numCells)) ||
+        (rc = dev_alloc(&counts, count_capacity(maxN))))
+      return rc;
+    return RTP_OK;
+  }
+  void release()
+  {
+    dev_free(key[0]);
+    dev_free(key[1]);
+    dev_free(idx[0]);
+    dev_free(idx[1]);
+    dev_free(startEnd);
+    dev_free(counts);
+  }
+  // resetCellIDs (grid.cl:65-71) into both key buffers so their tails always agree.
+  void reset_keys(hipStream_t s)
+  {
+    launch_reset_cell_ids(key[0], maxN, g.numCells, s);
+    launch_reset_cell_ids(key[1], maxN, g.numCells, s);
+    keyCur = 0;
+    idxCur = 0;
+    sortedN = 0;
+    dirtyEnd = 0;
+    permN = 0;
+  }
+  uint32_t sort_range(uint32_t n) const { return std::max(n, dirtyEnd); }
+  uint32_t max_key(uint32_t n) const
+  {
+    uint32_t m = max_active_key(g);
+    const uint32_t sn = sort_range(n);
+    if (sn > n) {  // stale keys may be anything fillCellIDs wrote or a reset tail value
+      const uint64_t tail = (uint64_t)g.numCells * 2 + sn - 1;
+      m = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(m, tail), 0xffffffffu);
+    }
+    return m;
+  }
+  // RadixSort::sort on p_cellID over [0, sort_range(n)); the rest is already in order.
+  int sort(uint32_t n, hipStream_t s, Profiler& prof)
+  {
+    const uint32_t sn = sort_range(n);
+    const SortPlan plan = make_sort_plan(sn, max_key(n), sortBits);
+    bool inB = false;
+    int rc = 0;
+    RTP_PROF(prof, "radix_sort", s,
+             rc = sort_pairs(plan, key[keyCur], key[1 - keyCur], idx[0], idx[1], counts, &inB, s));
+    if (rc)
+      return rc;
+    if (inB)
+      keyCur = 1 - keyCur;
+    idxCur = inB ? 1 : 0;
+    sortedN = sn;
+    permN = sn;
+    return RTP_OK;
+  }
+  // Copy p_cellID, the permutation (identity past the sorted range) and the cell table.
+  int read(uint32_t* keys, uint32_t* perm, uint32_t* se, hipStream_t s)
+  {
+    if (keys)
+      RTP_HIP_TRY(hipMemcpyAsync(keys, key[keyCur], sizeof(uint32_t) * maxN, hipMemcpyDeviceToHost, s));
+    if (perm && permN)
+      RTP_HIP_TRY(hipMemcpyAsync(perm, idx[idxCur], sizeof(uint32_t) * permN, hipMemcpyDeviceToHost, s));
+    if (se)
+      RTP_HIP_TRY(hipMemcpyAsync(se, startEnd, sizeof(uint2) * g.numCells, hipMemcpyDeviceToHost, s));
+    RTP_HIP_TRY(hipStreamSynchronize(s));
+    if (perm)
+      for (uint32_t i = permN; i < maxN; ++i)
+        perm[i] = i;
+    return RTP_OK;
+  }
+};
+
 }  // namespace
 }  // namespace rtp

@@ -187,3 +187,61 @@ def boids_flock(n: int = 8192, seed: int = 7, box: float = 2.0, res: int = 20,
     vel[:, :3] = (d * speed[:, None]).astype(np.float32)
     return BoidsScene(name=f"boids-{n}", box=(box, box, box), res=(res, res, res), n=n,
                       max_n=n + extra, pos=pos, vel=vel)
+
+
+@dataclasses.dataclass
+class CloudScene:
+    """A Clouds model case (physics/ocl/Clouds.cpp:401-500): particles at random positions in
+    a box region, velocity 0, with the parameter tree's defaults (Clouds.cpp:63-98)."""
+    name: str
+    box: tuple
+    res: tuple
+    n: int
+    max_n: int
+    pos: np.ndarray
+    vel: np.ndarray
+    iters: int = 2
+    gravity: float = 9.81
+
+    def params(self) -> dict:
+        """FluidKernelInputs of initCloudsJson's "Fluids" tree (Clouds.cpp:65-87)."""
+        return dict(restDensity=450.0, relaxCFM=600.0, timeStep=float(np.float32(0.010)), dim=3,
+                    isArtPressureEnabled=1, artPressureRadius=float(np.float32(0.006)),
+                    artPressureCoeff=float(np.float32(0.001)), artPressureExp=4,
+                    isVorticityConfEnabled=1, vorticityConfCoeff=float(np.float32(0.0004)),
+                    xsphViscosityCoeff=float(np.float32(0.0001)))
+
+    def cloud_params(self) -> dict:
+        """CloudKernelInputs after transferJsonInputsToModel (Clouds.cpp:283-303): the "Clouds"
+        tree (:88-97), rest density, time step and relaxation taken from the "Fluids" tree, and
+        the initial vapour coefficient of Clouds.hpp:45."""
+        f = lambda v: float(np.float32(v))  # noqa: E731
+        return dict(dim=3, timeStep=f(0.010), restDensity=450.0, groundHeatCoeff=10.0,
+                    buoyancyCoeff=f(0.10), gravCoeff=f(0.0005), adiabaticLapseRate=5.0,
+                    phaseTransitionRate=f(0.3485), latentHeatCoeff=f(0.07),
+                    isTempSmoothingEnabled=1, relaxCFM=600.0, initVaporDensityCoeff=0.75,
+                    windCoeff=1.0)
+
+
+CLOUD_SIZES = {
+    # name: (particles, box, res, fill region as fractions of the box: y from the floor)
+    # the app's cumulus case: 65,536 particles in the lowest quarter of a 10 x 20 x 10 box
+    # (Clouds.cpp:463-468 with ParticleSystemApp.cpp:300-304's doubled y)
+    "clouds-65k": (65536, (10.0, 20.0, 10.0), (30, 60, 30), 0.25),
+    # the homogeneous case: the whole box
+    "clouds-65k-homogeneous": (65536, (10.0, 20.0, 10.0), (30, 60, 30), 1.0),
+}
+
+
+def cloud_scene(name: str = "clouds-65k", seed: int = 17, max_n: int = 131072) -> CloudScene:
+    """Positions uniform in x, z over the box and in y over the fill region from the floor
+    (the reference draws them with rand(); a SplitMix64 stream here), velocity 0."""
+    n, box, res, frac = CLOUD_SIZES[name]
+    u = uniform24(seed, 0, 3 * n).reshape(n, 3).astype(np.float32)
+    lo = np.array([-box[0] / 2, -box[1] / 2, -box[2] / 2], np.float32)
+    hi = np.array([box[0] / 2, -box[1] / 2 + frac * box[1], box[2] / 2], np.float32)
+    vec = hi - lo
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = u * vec + lo  # Geometry.cpp:229-237: rand()/RAND_MAX * vec + start, in float
+    vel = np.zeros((n, 4), np.float32)
+    return CloudScene(name=name, box=box, res=res, n=n, max_n=max_n, pos=pos, vel=vel)
