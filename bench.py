@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "PBF particle-updates/s at 1M particles; learned vs plain at equal density error"
 BOIDS_STEPS = 200  # the 8,192-boid flock's run length
+CLOUDS_STEPS = 200  # the Clouds model's cumulus run
 
 
 def substep_bytes(name: str, n: int, nc: int, iters: int, vorticity: bool) -> float | None:
@@ -307,6 +308,41 @@ def run_boids(args):
     return out
 
 
+def run_clouds(args):
+    """The Clouds model's cumulus case as the app builds it (65,536 particles in the lowest
+    quarter of a 10 x 20 x 10 box, grid 30 x 60 x 30, initCloudsJson's parameters, 2 Jacobi
+    iterations): a run of CLOUDS_STEPS steps from the input (so the figure does not depend on
+    --steps), per-kernel times of the same run."""
+    from realtimeparticles_amd import capi, scenes
+    sc = scenes.cloud_scene("clouds-65k")
+    m = capi.Clouds(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(sc.params(), sc.cloud_params())
+    m.set_nb_jacobi_iters(sc.iters)
+
+    def run():
+        m.load_state(sc.pos, sc.vel)
+        m.sync()
+        t0 = time.perf_counter()
+        m.update(CLOUDS_STEPS)
+        m.sync()
+        return time.perf_counter() - t0
+
+    run()  # first launches
+    el = min(run() for _ in range(2))
+    m.enable_profiling(True)
+    run()
+    prof = m.profile()
+    temp = m.read_field(capi.FIELD_TEMPERATURE)
+    cloud = m.read_field(capi.FIELD_CLOUD_DENSITY)
+    m.close()
+    return {"particles": sc.n, "box": sc.box, "grid": sc.res, "steps": CLOUDS_STEPS,
+            "run": f"{CLOUDS_STEPS} steps from the input", "ms_per_step": round(el * 1e3 / CLOUDS_STEPS, 5),
+            "particle_updates_per_s": sc.n * CLOUDS_STEPS / el,
+            "kernel_ms_per_step": {k: round(v[1] / CLOUDS_STEPS, 5) for k, v in sorted(prof.items())},
+            "state_at_end": {"max_temperature": float(temp.max()),
+                             "max_cloud_density": float(cloud.max())}}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -569,6 +605,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-boids", action="store_true", help="skip the boids steps")
     ap.add_argument("--no-65k", action="store_true", help="skip the 65,536-particle 300-frame run")
+    ap.add_argument("--no-clouds", action="store_true", help="skip the Clouds model's cumulus run")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the substep with the reference's rendering steps on")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -637,6 +674,10 @@ def main():
     if not args.no_65k:
         dam_65k = run_dam_65k(args)
 
+    clouds = None
+    if not args.no_clouds:
+        clouds = run_clouds(args)
+
     learned_cmp = None
     if not args.no_learned:
         learned_cmp = run_learned(args, sc)
@@ -697,6 +738,7 @@ def main():
             "with_rendering_steps": render,
             "boids": boids,
             "pbf_65k_300_frames": dam_65k,
+            "clouds_cumulus": clouds,
             "learned_vs_plain": learned_cmp,
             "batched_scenes": batched,
         },

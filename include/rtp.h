@@ -161,6 +161,14 @@ typedef struct {
 int rtp_fluid_params_from_json(const char* json, rtp_fluid_params* p, uint32_t* nbJacobiIters);
 int rtp_fluid_params_to_json(const rtp_fluid_params* p, uint32_t nbJacobiIters, char* buf,
                              size_t cap, size_t* len);
+/* initCloudsJson (physics/ocl/Clouds.cpp:63-98): its "Fluids" tree as for the fluids, and
+ * the "Clouds" tree into CloudParams, whose rest density, time step and relaxation come from
+ * the "Fluids" tree (Clouds.cpp:283-302); dim and initVaporDensityCoeff are not in the tree
+ * and keep the values passed in. */
+int rtp_cloud_params_from_json(const char* json, rtp_fluid_params* p, rtp_cloud_params* q,
+                               uint32_t* nbJacobiIters);
+int rtp_cloud_params_to_json(const rtp_fluid_params* p, const rtp_cloud_params* q,
+                             uint32_t nbJacobiIters, char* buf, size_t cap, size_t* len);
 int rtp_boids_params_from_json(const char* json, rtp_boids_params* p, rtp_target_params* t);
 int rtp_boids_params_to_json(const rtp_boids_params* p, const rtp_target_params* t, char* buf,
                              size_t cap, size_t* len);
@@ -452,6 +460,12 @@ int rtp_clouds_sync(rtp_clouds* h);
 int rtp_clouds_read_grid(rtp_clouds* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd);
 /* RTP_FIELD_DENSITY, _LAMBDA, _PRED_POS, _VORTICITY and the Clouds fields 8..15. */
 int rtp_clouds_read_field(rtp_clouds* h, int field, float* out, uint32_t count);
+/* Device addresses of p_pos and p_vel (float4 x maxN), valid until the next update. */
+int rtp_clouds_device_state(rtp_clouds* h, void** pos4, void** vel4);
+/* updateModelWithInputJson / inputJson of the Clouds model (OclModel.hpp:48-54,
+ * Clouds.cpp:253-303). */
+int rtp_clouds_update_with_json(rtp_clouds* h, const char* json);
+int rtp_clouds_params_json(rtp_clouds* h, char* buf, size_t cap, size_t* len);
 int rtp_clouds_enable_profiling(rtp_clouds* h, int enable);
 int rtp_clouds_profile(rtp_clouds* h, rtp_kernel_time* out, int capacity, int* count);
 

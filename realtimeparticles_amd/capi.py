@@ -292,6 +292,13 @@ _SIGS = {
     "rtp_clouds_sync": ([_vp], _int),
     "rtp_clouds_read_grid": ([_vp, _vp, _vp, _vp], _int),
     "rtp_clouds_read_field": ([_vp, _int, _vp, _u32], _int),
+    "rtp_clouds_device_state": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
+    "rtp_clouds_update_with_json": ([_vp, ctypes.c_char_p], _int),
+    "rtp_clouds_params_json": ([_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
+    "rtp_cloud_params_from_json": ([ctypes.c_char_p, ctypes.POINTER(FluidParams),
+                                    ctypes.POINTER(CloudParams), ctypes.POINTER(_u32)], _int),
+    "rtp_cloud_params_to_json": ([ctypes.POINTER(FluidParams), ctypes.POINTER(CloudParams), _u32,
+                                  ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _int),
     "rtp_clouds_enable_profiling": ([_vp, _int], _int),
     "rtp_clouds_profile": ([_vp, ctypes.POINTER(KernelTime), _int, ctypes.POINTER(_int)], _int),
     "rtp_selfcheck_exp": ([_int, _vp, _vp, _u32, _vp, _vp, _vp], _int),
@@ -1043,6 +1050,27 @@ class Clouds:
         self._call("profile", buf, 64, ctypes.byref(cnt))
         return {buf[i].name.decode(): (buf[i].launches, buf[i].totalMs)
                 for i in range(min(cnt.value, 64))}
+
+    def update_with_json(self, text: str) -> None:
+        self._call("update_with_json", text.encode())
+
+    def params_json(self) -> str:
+        return _json_text(lib().rtp_clouds_params_json, self._h)
+
+
+def cloud_params_from_json(text: str, fluid=None, cloud=None):
+    """initCloudsJson's trees (Clouds.cpp:63-98) -> (FluidParams, CloudParams, iterations);
+    dim and initVaporDensityCoeff keep the values of `cloud` (defaults otherwise)."""
+    p = fluid or FluidParams(**FLUID_DEFAULTS)
+    q = cloud or CloudParams(**CLOUD_DEFAULTS)
+    it = _u32(0)
+    _check(lib().rtp_cloud_params_from_json(text.encode(), ctypes.byref(p), ctypes.byref(q),
+                                            ctypes.byref(it)), "rtp_cloud_params_from_json")
+    return p, q, int(it.value)
+
+
+def cloud_params_to_json(p, q, iters: int) -> str:
+    return _json_text(lib().rtp_cloud_params_to_json, ctypes.byref(p), ctypes.byref(q), int(iters))
 
 
 def selfcheck_sqrt(lo: int, hi: int, device: int = 0):

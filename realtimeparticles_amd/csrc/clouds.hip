@@ -141,16 +141,40 @@ __device__ __forceinline__ float4 mixed_boundary(float4 p, const GridConsts& g)
   return q;
 }
 
-// The 27-cell walk of clouds.cl:322-353: f(e, vec) for every candidate e of every visited
-// cell, vec = pos - p_e - absWall * sign (sign +-2 on a wrapped x or z).
-template <class F>
+// Candidates lo..hi (inclusive) in index order with one shift: four positions and four
+// neighbour data (ld(e)) in flight before their bodies run in order.
+template <class L, class F>
+__device__ __forceinline__ void cld_range(uint32_t lo, uint32_t hi, const float4 pos, const float4 shift,
+                                          const float4* __restrict__ src, L& ld, F& f)
+{
+  uint32_t e = lo;
+  for (; e + 3u <= hi; e += 4u) {
+    const float4 o0 = src[e], o1 = src[e + 1u], o2 = src[e + 2u], o3 = src[e + 3u];
+    const auto x0 = ld(e), x1 = ld(e + 1u), x2 = ld(e + 2u), x3 = ld(e + 3u);
+    f(e, x0, sub4(sub4(pos, o0), shift));
+    f(e + 1u, x1, sub4(sub4(pos, o1), shift));
+    f(e + 2u, x2, sub4(sub4(pos, o2), shift));
+    f(e + 3u, x3, sub4(sub4(pos, o3), shift));
+  }
+  for (; e <= hi; ++e)
+    f(e, ld(e), sub4(sub4(pos, src[e]), shift));
+}
+
+// The 27-cell walk of clouds.cl:322-353: f(e, ld(e), vec) for every candidate e of every
+// visited cell in the reference's order, vec = pos - p_e - absWall * sign (sign +-2 on a
+// wrapped x or z).  The three cells of a z-column are one index range when none of them
+// wraps in z and their non-empty ranges are back to back (cells are numbered z-minor), which
+// visits the same candidates in the same order.
+template <class L, class F>
 __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restrict__ src,
-                                         const uint2* __restrict__ se, const GridConsts& g, F&& f)
+                                         const uint2* __restrict__ se, const GridConsts& g, L&& ld,
+                                         F&& f)
 {
   uint32_t ux, uy, uz;
   cell3d(pos, g, ux, uy, uz);
   const int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
   const int32_t rx = g.res[0], ry = g.res[1], rz = g.res[2];
+  const bool zInside = cz - 1 >= 0 && cz + 1 < rz;
   for (int ix = -1; ix <= 1; ++ix) {
     const int32_t ax = cx + ix;
     const int32_t nx = (ax + rx) % rx;
@@ -160,18 +184,62 @@ __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restr
       if (ay >= ry || ay < 0)
         continue;
       const int32_t ny = (ay + ry) % ry;
+      const int32_t row = (nx * ry + ny) * rz;
+      if (zInside) {
+        const uint2 cells[3] = { se[row + cz - 1], se[row + cz], se[row + cz + 1] };
+        uint32_t lo = 1u, hi = 0u;
+        bool merged = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const uint2 r = cells[k];
+          if (r.x > r.y)
+            continue;
+          if (lo > hi) {
+            lo = r.x;
+            hi = r.y;
+          } else if (r.x == hi + 1u) {
+            hi = r.y;
+          } else {
+            merged = false;
+          }
+        }
+        const float4 shift =
+          make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * 0.0f, 0.0f * 0.0f);
+        if (merged) {
+          if (lo <= hi)
+            cld_range(lo, hi, pos, shift, src, ld, f);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            if (cells[k].x <= cells[k].y)
+              cld_range(cells[k].x, cells[k].y, pos, shift, src, ld, f);
+        }
+        continue;
+      }
       for (int iz = -1; iz <= 1; ++iz) {
         const int32_t az = cz + iz;
         const int32_t nz = (az + rz) % rz;
         const float sz = az >= rz ? 2.0f : (az < 0 ? -2.0f : 0.0f);
         const float4 shift =
           make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * sz, 0.0f * 0.0f);
-        const uint2 r = se[(nx * ry + ny) * rz + nz];
-        for (uint32_t e = r.x; e <= r.y; ++e)
-          f(e, sub4(sub4(pos, src[e]), shift));
+        const uint2 r = se[row + nz];
+        if (r.x <= r.y)
+          cld_range(r.x, r.y, pos, shift, src, ld, f);
       }
     }
   }
+}
+
+// A candidate at r >= h (d2 >= cullSq, so sqrt(d2) >= h) with a finite distance contributes
+// exactly +-0 to every sum here, as long as the neighbour data its term multiplies are finite:
+// its kernel weight and gradient are +-0 (the zero factor 1 - step(h, r) times finite
+// values), and a sum that starts at +0 never holds -0, so adding +-0 leaves it unchanged.
+// Such candidates are skipped; every other one (r < h, a NaN or huge distance, non-finite
+// data) runs the reference's full expression.
+__device__ __forceinline__ bool negligible(float4 vec, const PbfConsts& c)
+{
+  const float d2 = dot4(vec, vec);
+  return d2 >= c.cullSq && d2 < kFarD2;
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
@@ -286,10 +354,13 @@ __global__ __launch_bounds__(256) void k_cld_temp_lambda(const float4* __restric
   const float temp = tempP[i];
   float laplacian = 0.0f, sumGradCi = 0.0f, sumSqGradC = 0.0f;
   const float rho0 = c.q.restDensity;
-  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e, float4 vec) {
+  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e) { return tempP[e]; }, [&](uint32_t, float te, float4 vec) {
+    const float dTemp = temp - te;
+    if (negligible(vec, c.f) && __builtin_isfinite(dTemp))
+      return;
     const float4 grad = grad_spiky(vec, c.f);
     const float vg = dot4(vec, grad), vv = dot4(vec, vec);
-    laplacian = laplacian + (temp - tempP[e]) * vg / (vv + kFloatEps);
+    laplacian = laplacian + dTemp * vg / (vv + kFloatEps);
     const float derivativeTemp = vg / (vv * rho0 + kFloatEps);
     sumGradCi = sumGradCi + derivativeTemp;
     sumSqGradC = sumSqGradC + derivativeTemp * derivativeTemp;
@@ -315,10 +386,13 @@ __global__ __launch_bounds__(256) void k_cld_temp_correct(const float4* __restri
   const float lambdaI = lambdaT[i];
   const float rho0 = c.q.restDensity;
   float corr = 0.0f;
-  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e, float4 vec) {
+  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e) { return lambdaT[e]; }, [&](uint32_t, float le, float4 vec) {
+    const float l = lambdaI + le;
+    if (negligible(vec, c.f) && __builtin_isfinite(l))
+      return;
     const float4 grad = grad_spiky(vec, c.f);
     const float derivativeTemp = dot4(vec, grad) / (dot4(vec, vec) * rho0 + kFloatEps);
-    corr = corr + (lambdaI + lambdaT[e]) * derivativeTemp;
+    corr = corr + l * derivativeTemp;
   });
   temp[i] = temp[i] + 0.3f * corr;
 }
@@ -338,7 +412,9 @@ __global__ __launch_bounds__(256) void k_cld_density_lambda(const float4* __rest
   float fluidDensity = 0.0f;
   float4 sumGradCi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float sumSqGradC = 0.0f;
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t, float4 vec) {
+  cld_walk(pos, pred, se, c.f.g, [](uint32_t) { return 0; }, [&](uint32_t, int, float4 vec) {
+    if (negligible(vec, c.f))
+      return;
     fluidDensity = fluidDensity + poly6(vec, c.f);
     const float4 grad = grad_spiky(vec, c.f);
     sumGradCi = add4(sumGradCi, grad);
@@ -369,8 +445,10 @@ __global__ __launch_bounds__(256) void k_cld_correct(const float4* __restrict__ 
   const float4 pos = pred[i];
   const float lambdaI = lambda[i];
   float4 corr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
-    const float s = lambdaI + lambda[e] + art_pressure(vec, c.f);
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return lambda[e]; }, [&](uint32_t, float le, float4 vec) {
+    if (negligible(vec, c.f) && __builtin_isfinite(lambdaI + le))
+      return;
+    const float s = lambdaI + le + art_pressure(vec, c.f);
     corr = add4(corr, muls4(s, grad_spiky(vec, c.f)));
   });
   const float4 corrPos = div4s(corr, c.f.p.restDensity);
@@ -412,8 +490,12 @@ __global__ __launch_bounds__(256) void k_cld_vorticity(const float4* __restrict_
   const float4 pos = pred[i];
   const float4 velocity = vel[i];
   float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
-    w = add4(w, cross4(sub4(vel[e], velocity), grad_spiky(vec, c.f)));
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return vel[e]; }, [&](uint32_t, float4 ve, float4 vec) {
+    const float4 dv = sub4(ve, velocity);
+    if (negligible(vec, c.f) && __builtin_isfinite(dv.x) && __builtin_isfinite(dv.y) &&
+        __builtin_isfinite(dv.z) && __builtin_isfinite(dv.w))
+      return;
+    w = add4(w, cross4(dv, grad_spiky(vec, c.f)));
   });
   vort[i] = w;
 }
@@ -433,8 +515,11 @@ __global__ __launch_bounds__(256) void k_cld_confinement(const float4* __restric
   const float4 pos = pred[i];
   const float4 vorticity = vort[i];
   float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
-    nn = add4(nn, muls4(len4(vort[e]), grad_spiky(vec, c.f)));
+  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return vort[e]; }, [&](uint32_t, float4 we, float4 vec) {
+    const float lw = len4(we);
+    if (negligible(vec, c.f) && __builtin_isfinite(lw))
+      return;
+    nn = add4(nn, muls4(lw, grad_spiky(vec, c.f)));
   });
   const rtp_fluid_params& p = c.f.p;
   velOut[i] = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
@@ -465,8 +550,12 @@ __global__ __launch_bounds__(256) void k_cld_xsph(const float4* __restrict__ pre
   const float4 pp = pred[i];
   const float4 velocity = velIn[i];
   float4 viscosity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pp, pred, se, c.f.g, [&](uint32_t e, float4 vec) {
-    viscosity = add4(viscosity, mul4s(sub4(velIn[e], velocity), poly6(vec, c.f)));
+  cld_walk(pp, pred, se, c.f.g, [&](uint32_t e) { return velIn[e]; }, [&](uint32_t, float4 ve, float4 vec) {
+    const float4 dv = sub4(ve, velocity);
+    if (negligible(vec, c.f) && __builtin_isfinite(dv.x) && __builtin_isfinite(dv.y) &&
+        __builtin_isfinite(dv.z) && __builtin_isfinite(dv.w))
+      return;
+    viscosity = add4(viscosity, mul4s(dv, poly6(vec, c.f)));
   });
   velOut[i] = add4(velocity, muls4(c.f.p.xsphViscosityCoeff, viscosity));
   pos[i] = cld_update_pos(pp, c);
@@ -909,6 +998,16 @@ int rtp_clouds_read_field(rtp_clouds* h, int field, float* out, uint32_t count)
   return RTP_OK;
 }
 
+int rtp_clouds_device_state(rtp_clouds* h, void** pos4, void** vel4)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  if (pos4)
+    *pos4 = h->pos[h->cur];
+  if (vel4)
+    *vel4 = h->vel[h->cur];
+  return RTP_OK;
+}
+
 int rtp_clouds_enable_profiling(rtp_clouds* h, int enable)
 {
   RTP_REQUIRE(h, "NULL handle");
@@ -920,6 +1019,29 @@ int rtp_clouds_profile(rtp_clouds* h, rtp_kernel_time* out, int capacity, int* c
 {
   RTP_REQUIRE(h, "NULL handle");
   return h->prof.report(out, capacity, count);
+}
+
+int rtp_clouds_update_with_json(rtp_clouds* h, const char* json)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  rtp_fluid_params p = h->c.f.p;
+  rtp_cloud_params q = h->c.q;
+  uint32_t iters = 0;
+  int rc = rtp_cloud_params_from_json(json, &p, &q, &iters);
+  if (rc)
+    return rc;
+  // Clouds.cpp:260, 288: dim follows the model's dimension, not the tree
+  p.dim = h->c.f.p.dim;
+  q.dim = h->c.q.dim;
+  if ((rc = rtp_clouds_set_params(h, &p, &q)))
+    return rc;
+  return rtp_clouds_set_nb_jacobi_iters(h, iters > 1000u ? 1000u : iters);
+}
+
+int rtp_clouds_params_json(rtp_clouds* h, char* buf, size_t cap, size_t* len)
+{
+  RTP_REQUIRE(h, "NULL handle");
+  return rtp_cloud_params_to_json(&h->c.f.p, &h->c.q, h->iters, buf, cap, len);
 }
 
 int rtp_selfcheck_exp(int device, const double* x64, const float* x32, uint32_t n, double* y64,

@@ -2,6 +2,7 @@
 #include "Model.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 namespace {
@@ -22,6 +23,19 @@ const char* const kInitFluidsJson =
     "\"Radius\": [0.006, 0.001, 0.015], \"Exp\": [4, 1, 6]}, "
     "\"Vorticity Confinement\": {\"Enable##Vorticity\": true, \"Coefficient##Vorticity\": [0.0004, 0, 0.001], "
     "\"xSPH Viscosity Coefficient\": [0.0001, 0, 0.001]}}}";
+// initCloudsJson (physics/ocl/Clouds.cpp:63-98)
+const char* const kInitCloudsJson =
+    "{\"Fluids\": {\"Rest Density\": [450, 10, 1000], \"Relax CFM\": [600, 100, 1000], "
+    "\"Time Step\": [0.01, 0.0001, 0.02], \"Nb Jacobi Iterations\": [2, 1, 6], "
+    "\"Artificial Pressure\": {\"Enable##Pressure\": true, \"Coefficient##Pressure\": [0.001, 0, 0.001], "
+    "\"Radius\": [0.006, 0.001, 0.015], \"Exp\": [4, 1, 6]}, "
+    "\"Vorticity Confinement\": {\"Enable##Vorticity\": true, \"Coefficient##Vorticity\": [0.0004, 0, 0.001], "
+    "\"xSPH Viscosity Coefficient\": [0.0001, 0, 0.001]}}, "
+    "\"Clouds\": {\"Enable Temperature Smoothing\": true, \"Ground Heat Coefficient\": [10, 0, 1000], "
+    "\"Buoyancy Heat Coefficient\": [0.1, 0, 5], \"Gravity Coefficient\": [0.0005, 0, 0.1], "
+    "\"Adiabatic Lapse Rate\": [5, 0, 20], \"Phase Transition Rate\": [0.3485, 0, 20], "
+    "\"Latent Heat Coefficient\": [0.07, 0, 0.1], \"Wind Coefficient\": [1, 0, 1]}}";
+
 const char* const kInitBoidsJson =
     "{\"Boids\": {\"Velocity\": [0.5, 0.01, 5], "
     "\"Target\": {\"Enable##Target\": false, \"Show\": true, \"Radius\": [2, 1, 20], \"Attract\": true}, "
@@ -99,6 +113,23 @@ std::vector<F3> circle_grid_yz(std::array<int, 2> res, F3 s, F3 e)
   return v;
 }
 
+// utils/Geometry.cpp:123-132 / 229-238 with Distribution::Random: every vertex takes three
+// rand() draws, x then y then z, each rand() / RAND_MAX scaled into [s, e] in float.  The
+// layout is the reference's when the process's rand() sequence is at the same point.
+std::vector<F3> random_grid(size_t count, F3 s, F3 e)
+{
+  const F3 vec{ e.x - s.x, e.y - s.y, e.z - s.z };
+  std::vector<F3> v;
+  v.reserve(count);
+  for (size_t i = 0; i < count; ++i) {
+    const float x = (float)std::rand() / (float)RAND_MAX * vec.x + s.x;
+    const float y = (float)std::rand() / (float)RAND_MAX * vec.y + s.y;
+    const float z = (float)std::rand() / (float)RAND_MAX * vec.z + s.z;
+    v.push_back(F3{ x, y, z });
+  }
+  return v;
+}
+
 std::vector<float> to_float4(const std::vector<F3>& v)
 {
   std::vector<float> out(v.size() * 4, 0.0f);
@@ -169,6 +200,7 @@ std::unique_ptr<Model> CreateModel(ModelType type, ModelParams params)
   switch ((int)type) {
   case ModelType::BOIDS: return std::make_unique<HIP::Boids>(params);
   case ModelType::FLUIDS: return std::make_unique<HIP::Fluids>(params);
+  case ModelType::CLOUDS: return std::make_unique<HIP::Clouds>(params);
   default: return nullptr;
   }
 }
@@ -567,6 +599,206 @@ void Boids::applyPause() { check(rtp_boids_pause(m_h, m_pause ? 1 : 0), "rtp_boi
 void Boids::applyProfiling()
 {
   check(rtp_boids_enable_profiling(m_h, m_profiling ? 1 : 0), "rtp_boids_enable_profiling");
+}
+
+// ------------------------------------------------------------------------------ Clouds
+
+Clouds::Clouds(const ModelParams& params) : Model(params)
+{
+  const rtp_model_desc d = make_desc(params);
+  check(rtp_clouds_create(&d, &m_h), "rtp_clouds_create");
+  check(rtp_clouds_set_nb_jacobi_iters(m_h, m_nbJacobiIters), "rtp_clouds_set_nb_jacobi_iters");
+  resetInputJson(kInitCloudsJson);
+  m_init = true;
+  if (m_case != Utils::CASE_INVALID)
+    reset();
+}
+
+Clouds::~Clouds() { rtp_clouds_destroy(m_h); }
+
+void Clouds::update()
+{
+  check(rtp_clouds_update(m_h, 1), "rtp_clouds_update");
+  check(rtp_clouds_sync(m_h), "rtp_clouds_sync");  // each frame is synchronous (Context.cpp:746-747)
+}
+
+// Clouds::reset / initCloudsParticles (Clouds.cpp:379-500): the initial tree, then random
+// particles over the lower part of the box (cumulus) or the whole box (homogeneous).
+void Clouds::reset()
+{
+  if (!m_init)
+    return;
+  const bool d2 = m_dimension == Geometry::Dimension::dim2D;
+  rtp_fluid_params p{};
+  rtp_cloud_params q{};
+  check(rtp_clouds_get_params(m_h, &p, &q), "rtp_clouds_get_params");
+  p.dim = q.dim = d2 ? 2u : 3u;  // Clouds.cpp:260, 288
+  check(rtp_clouds_set_params(m_h, &p, &q), "rtp_clouds_set_params");
+  resetInputJson(kInitCloudsJson);
+  updateModelWithInputJson(getInputJson());
+  const F3 b{ m_boxSize.x, m_boxSize.y, m_boxSize.z };
+  std::vector<F3> verts;
+  if (d2) {
+    // Geometry::Generate2DGrid(Rectangle, YZ, subdiv2D(P8K), ..., Random)
+    const auto sub = Utils::GetNbParticlesSubdiv2D(Utils::P8K);
+    m_currNbParticles = Utils::P8K;
+    switch (m_case) {
+    case Utils::CLOUDS_CUMULUS:
+      verts = random_grid((size_t)sub[0] * sub[1], F3{ 0.0f, b.y / -2.0f, b.z / -2.0f },
+                          F3{ 0.0f, 0.0f, b.z / 2.0f });
+      break;
+    case Utils::CLOUDS_HOMOGENEOUS:
+      verts = random_grid((size_t)sub[0] * sub[1], F3{ 0.0f, b.y / -2.0f, b.z / -2.0f },
+                          F3{ 0.0f, b.y / 2.0f, b.z / 2.0f });
+      break;
+    default:
+      throw std::runtime_error("Clouds::reset: unknown case");
+    }
+  } else {
+    const auto sub = Utils::GetNbParticlesSubdiv3D(Utils::P65K);
+    m_currNbParticles = Utils::P65K;
+    switch (m_case) {
+    case Utils::CLOUDS_CUMULUS:
+      verts = random_grid((size_t)sub[0] * sub[1] * sub[2], F3{ b.x / -2.0f, b.y / -2.0f, b.z / -2.0f },
+                          F3{ b.x / 2.0f, b.y / -4.0f, b.z / 2.0f });
+      break;
+    case Utils::CLOUDS_HOMOGENEOUS:
+      verts = random_grid((size_t)sub[0] * sub[1] * sub[2], F3{ b.x / -2.0f, b.y / -2.0f, b.z / -2.0f },
+                          F3{ b.x / 2.0f, b.y / 2.0f, b.z / 2.0f });
+      break;
+    default:
+      throw std::runtime_error("Clouds::reset: unknown case");
+    }
+  }
+  if (m_currNbParticles > m_maxNbParticles)
+    throw std::runtime_error("Clouds::reset: case needs more particles than maxNbParticles");
+  const std::vector<float> pos = to_float4(verts);
+  const std::vector<float> vel(pos.size(), 0.0f);
+  check(rtp_clouds_load_state(m_h, pos.data(), vel.data(), (uint32_t)verts.size()), "rtp_clouds_load_state");
+}
+
+void Clouds::setFluidParams(const rtp_fluid_params& p)
+{
+  check(rtp_clouds_set_params(m_h, &p, nullptr), "rtp_clouds_set_params");
+}
+
+void Clouds::setCloudParams(const rtp_cloud_params& q)
+{
+  check(rtp_clouds_set_params(m_h, nullptr, &q), "rtp_clouds_set_params");
+}
+
+rtp_fluid_params Clouds::fluidParams() const
+{
+  rtp_fluid_params p{};
+  check(rtp_clouds_get_params(m_h, &p, nullptr), "rtp_clouds_get_params");
+  return p;
+}
+
+rtp_cloud_params Clouds::cloudParams() const
+{
+  rtp_cloud_params q{};
+  check(rtp_clouds_get_params(m_h, nullptr, &q), "rtp_clouds_get_params");
+  return q;
+}
+
+void Clouds::setNbJacobiIters(uint32_t iters)
+{
+  check(rtp_clouds_set_nb_jacobi_iters(m_h, iters), "rtp_clouds_set_nb_jacobi_iters");
+  m_nbJacobiIters = iters;
+}
+
+std::vector<float> Clouds::quantity(const std::string& name) const
+{
+  int field = -1;
+  if (name == "Particle ID")
+    field = RTP_FIELD_PARTICLE_ID;
+  else if (name == "Vapor Density")
+    field = RTP_FIELD_VAPOR_DENSITY;
+  else if (name == "Cloud Density")
+    field = RTP_FIELD_CLOUD_DENSITY;
+  else if (name == "Net Force")
+    field = RTP_FIELD_BUOYANCY;
+  else if (name == "Temperature")
+    field = RTP_FIELD_TEMPERATURE;
+  else
+    throw std::runtime_error("Clouds::quantity: unknown quantity " + name);
+  std::vector<float> out(m_currNbParticles);
+  check(rtp_clouds_read_field(m_h, field, out.data(), (uint32_t)out.size()), "rtp_clouds_read_field");
+  return out;
+}
+
+void Clouds::loadState(const float* pos4, const float* vel4, size_t count)
+{
+  check(rtp_clouds_load_state(m_h, pos4, vel4, (uint32_t)count), "rtp_clouds_load_state");
+  m_currNbParticles = count;
+}
+
+void Clouds::readState(float* pos4, float* vel4, size_t count) const
+{
+  check(rtp_clouds_read_state(m_h, pos4, vel4, (uint32_t)count), "rtp_clouds_read_state");
+}
+
+void* Clouds::devicePositions() const
+{
+  void* p = nullptr;
+  check(rtp_clouds_device_state(m_h, &p, nullptr), "rtp_clouds_device_state");
+  return p;
+}
+
+std::vector<rtp_kernel_time> Clouds::kernelTimes() const
+{
+  int count = 0;
+  check(rtp_clouds_profile(m_h, nullptr, 0, &count), "rtp_clouds_profile");
+  std::vector<rtp_kernel_time> out((size_t)count);
+  if (count)
+    check(rtp_clouds_profile(m_h, out.data(), count, &count), "rtp_clouds_profile");
+  return out;
+}
+
+void Clouds::enableRendering(bool enable)
+{
+  if (enable)
+    throw std::runtime_error("Clouds: the rendering steps are not provided for this model");
+}
+
+void Clouds::setCameraPos(float, float, float)
+{
+  throw std::runtime_error("Clouds: the rendering steps are not provided for this model");
+}
+
+void* Clouds::deviceColors() const
+{
+  throw std::runtime_error("Clouds: the rendering steps are not provided for this model");
+}
+
+void Clouds::updateModelWithInputJson(const std::string& json)
+{
+  check(rtp_clouds_update_with_json(m_h, json.c_str()), "rtp_clouds_update_with_json");
+  uint32_t iters = 0;
+  check(rtp_clouds_get_nb_jacobi_iters(m_h, &iters), "rtp_clouds_get_nb_jacobi_iters");
+  m_nbJacobiIters = iters;
+}
+
+std::string Clouds::inputJson() const
+{
+  size_t n = 0;
+  check(rtp_clouds_params_json(m_h, nullptr, 0, &n), "rtp_clouds_params_json");
+  std::string s(n + 1, '\0');
+  check(rtp_clouds_params_json(m_h, &s[0], s.size(), &n), "rtp_clouds_params_json");
+  s.resize(n);
+  return s;
+}
+
+void Clouds::applyNbParticles()
+{
+  check(rtp_clouds_set_nb_particles(m_h, (uint32_t)m_currNbParticles), "rtp_clouds_set_nb_particles");
+}
+
+void Clouds::applyPause() { check(rtp_clouds_pause(m_h, m_pause ? 1 : 0), "rtp_clouds_pause"); }
+
+void Clouds::applyProfiling()
+{
+  check(rtp_clouds_enable_profiling(m_h, m_profiling ? 1 : 0), "rtp_clouds_enable_profiling");
 }
 
 }  // namespace HIP

@@ -79,7 +79,7 @@ struct ModelParams {
 };
 
 class Model;
-// physics/Model.cpp:10-24.  CLOUDS is not provided by this backend (returns nullptr).
+// physics/Model.cpp:10-24.
 std::unique_ptr<Model> CreateModel(ModelType type, ModelParams params);
 
 // physics/Model.hpp:81-216 minus the displayable-quantity registry.
@@ -257,6 +257,50 @@ class Boids : public Model {
   bool m_targetActive = false;
   bool m_targetVisible = false;
   std::array<float, 3> m_targetPos{ 0.0f, 0.0f, 0.0f };
+};
+
+// The Clouds model (physics/ocl/Clouds.hpp:50-80) on librtp: position based fluids coupled
+// to a temperature, vapour and cloud density per particle.  The app builds it with the y
+// extent of the box and grid doubled (app/ParticleSystemApp.cpp:300-304); its cases are
+// CLOUDS_CUMULUS and CLOUDS_HOMOGENEOUS.  No rendering steps: enableRendering(true),
+// setCameraPos and deviceColors throw.
+class Clouds : public Model {
+ public:
+  explicit Clouds(const ModelParams& params);
+  ~Clouds() override;
+
+  void update() override;
+  void reset() override;
+
+  void setFluidParams(const rtp_fluid_params& p);
+  void setCloudParams(const rtp_cloud_params& q);
+  rtp_fluid_params fluidParams() const;
+  rtp_cloud_params cloudParams() const;
+  void setNbJacobiIters(uint32_t iters);
+  uint32_t nbJacobiIters() const { return m_nbJacobiIters; }
+  // The displayable quantities of Clouds.cpp:161-173 by name ("Particle ID", "Vapor Density",
+  // "Cloud Density", "Net Force", "Temperature"), one float per active particle.
+  std::vector<float> quantity(const std::string& name) const;
+
+  void loadState(const float* pos4, const float* vel4, size_t count) override;
+  void readState(float* pos4, float* vel4, size_t count) const override;
+  void* devicePositions() const override;
+  std::vector<rtp_kernel_time> kernelTimes() const override;
+  void enableRendering(bool enable) override;
+  void setCameraPos(float x, float y, float z) override;
+  void updateModelWithInputJson(const std::string& json) override;
+  std::string inputJson() const override;
+  void* deviceColors() const override;
+  rtp_clouds* handle() const { return m_h; }
+
+ protected:
+  void applyNbParticles() override;
+  void applyPause() override;
+  void applyProfiling() override;
+
+ private:
+  rtp_clouds* m_h = nullptr;
+  uint32_t m_nbJacobiIters = 1;  // Clouds.cpp:109, before reset() applies the tree (2)
 };
 
 }  // namespace HIP

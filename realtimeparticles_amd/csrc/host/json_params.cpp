@@ -497,6 +497,57 @@ void write(std::string& out, const JVal& v)
   }
 }
 
+// The "Fluids" tree of initFluidsJson (Fluids.cpp:52-74) and of initCloudsJson
+// (Clouds.cpp:65-87): FluidKernelInputs and the iteration count (Fluids.cpp:229-246).
+bool read_fluids(const JVal& root, rtp_fluid_params* p, uint32_t* iters)
+{
+  const Path f = Path{ &root, "" }.at("Fluids");
+  const Path ap = f.at("Artificial Pressure"), vc = f.at("Vorticity Confinement");
+  double it, rho, cfm, dt, apc, apr, ape, vcc, xs;
+  bool apOn, vcOn;
+  if (!first_number(f.at("Nb Jacobi Iterations"), &it) || !first_number(f.at("Rest Density"), &rho) ||
+      !first_number(f.at("Relax CFM"), &cfm) || !first_number(f.at("Time Step"), &dt) ||
+      !equals_true(ap.at("Enable##Pressure"), &apOn) ||
+      !first_number(ap.at("Coefficient##Pressure"), &apc) || !first_number(ap.at("Radius"), &apr) ||
+      !first_number(ap.at("Exp"), &ape) || !equals_true(vc.at("Enable##Vorticity"), &vcOn) ||
+      !first_number(vc.at("Coefficient##Vorticity"), &vcc) ||
+      !first_number(vc.at("xSPH Viscosity Coefficient"), &xs))
+    return false;
+  // A negative count runs no iteration: a deliberate deviation, since the reference stores
+  // it in a size_t (m_nbJacobiIters), where -1 wraps to SIZE_MAX and its iteration loop
+  // would not end.
+  const int32_t n = to_int(it);
+  *iters = n > 0 ? (uint32_t)n : 0u;
+  p->restDensity = (float)rho;
+  p->relaxCFM = (float)cfm;
+  p->timeStep = (float)dt;
+  p->isArtPressureEnabled = apOn ? 1u : 0u;
+  p->artPressureCoeff = (float)apc;
+  p->artPressureRadius = (float)apr;
+  p->artPressureExp = (uint32_t)to_int(ape);
+  p->isVorticityConfEnabled = vcOn ? 1u : 0u;
+  p->vorticityConfCoeff = (float)vcc;
+  p->xsphViscosityCoeff = (float)xs;
+  return true;
+}
+
+// "\"Fluids\": {...}" with the ranges a parameter widget shows (Fluids.cpp:52-74).
+std::string fluids_tree(const rtp_fluid_params& p, uint32_t iters)
+{
+  return std::string("\"Fluids\": {") +
+         "\"Rest Density\": " + range3(p.restDensity, 10.0f, 1000.0f) + ", " +
+         "\"Relax CFM\": " + range3(p.relaxCFM, 100.0f, 1000.0f) + ", " +
+         "\"Time Step\": " + range3(p.timeStep, 0.0001f, 0.020f) + ", " +
+         "\"Nb Jacobi Iterations\": " + range3(iters, 1, 6) + ", " +
+         "\"Artificial Pressure\": {\"Enable##Pressure\": " + (p.isArtPressureEnabled ? "true" : "false") +
+         ", \"Coefficient##Pressure\": " + range3(p.artPressureCoeff, 0.0f, 0.001f) +
+         ", \"Radius\": " + range3(p.artPressureRadius, 0.001f, 0.015f) +
+         ", \"Exp\": " + range3(p.artPressureExp, 1, 6) + "}, " +
+         "\"Vorticity Confinement\": {\"Enable##Vorticity\": " + (p.isVorticityConfEnabled ? "true" : "false") +
+         ", \"Coefficient##Vorticity\": " + range3(p.vorticityConfCoeff, 0.0f, 0.001f) +
+         ", \"xSPH Viscosity Coefficient\": " + range3(p.xsphViscosityCoeff, 0.0f, 0.001f) + "}}";
+}
+
 }  // namespace
 
 extern "C" {
@@ -510,34 +561,7 @@ int rtp_fluid_params_from_json(const char* json, rtp_fluid_params* p, uint32_t* 
   JVal root;
   if (!parse(json, root))
     return bad();
-  const Path f = Path{ &root, "" }.at("Fluids");
-  const Path ap = f.at("Artificial Pressure"), vc = f.at("Vorticity Confinement");
-  double it, rho, cfm, dt, apc, apr, ape, vcc, xs;
-  bool apOn, vcOn;
-  if (!first_number(f.at("Nb Jacobi Iterations"), &it) || !first_number(f.at("Rest Density"), &rho) ||
-      !first_number(f.at("Relax CFM"), &cfm) || !first_number(f.at("Time Step"), &dt) ||
-      !equals_true(ap.at("Enable##Pressure"), &apOn) ||
-      !first_number(ap.at("Coefficient##Pressure"), &apc) || !first_number(ap.at("Radius"), &apr) ||
-      !first_number(ap.at("Exp"), &ape) || !equals_true(vc.at("Enable##Vorticity"), &vcOn) ||
-      !first_number(vc.at("Coefficient##Vorticity"), &vcc) ||
-      !first_number(vc.at("xSPH Viscosity Coefficient"), &xs))
-    return bad();
-  // Fluids.cpp:229-246.  A negative count runs no iteration: a deliberate deviation, since
-  // the reference stores it in a size_t (m_nbJacobiIters), where -1 wraps to SIZE_MAX and
-  // its iteration loop would not end.
-  const int32_t n = to_int(it);
-  *iters = n > 0 ? (uint32_t)n : 0u;
-  p->restDensity = (float)rho;
-  p->relaxCFM = (float)cfm;
-  p->timeStep = (float)dt;
-  p->isArtPressureEnabled = apOn ? 1u : 0u;
-  p->artPressureCoeff = (float)apc;
-  p->artPressureRadius = (float)apr;
-  p->artPressureExp = (uint32_t)to_int(ape);
-  p->isVorticityConfEnabled = vcOn ? 1u : 0u;
-  p->vorticityConfCoeff = (float)vcc;
-  p->xsphViscosityCoeff = (float)xs;
-  return RTP_OK;
+  return read_fluids(root, p, iters) ? RTP_OK : bad();
 }
 
 int rtp_fluid_params_to_json(const rtp_fluid_params* p, uint32_t iters, char* buf, size_t cap,
@@ -547,20 +571,70 @@ int rtp_fluid_params_to_json(const rtp_fluid_params* p, uint32_t iters, char* bu
     g_jsonError = "NULL argument";
     return bad();
   }
-  // the ranges of initFluidsJson (Fluids.cpp:52-74), which a parameter widget shows
+  return put(buf, cap, len, "{" + fluids_tree(*p, iters) + "}");
+}
+
+int rtp_cloud_params_from_json(const char* json, rtp_fluid_params* p, rtp_cloud_params* q,
+                               uint32_t* iters)
+{
+  if (!p || !q || !iters) {
+    g_jsonError = "NULL argument";
+    return bad();
+  }
+  JVal root;
+  if (!parse(json, root))
+    return bad();
+  rtp_fluid_params fp = *p;
+  uint32_t it = 0;
+  if (!read_fluids(root, &fp, &it))
+    return bad();
+  const Path c = Path{ &root, "" }.at("Clouds");
+  bool smooth;
+  double gh, bc, gc, alr, ptr, lhc, wind;
+  if (!as_bool(c.at("Enable Temperature Smoothing"), &smooth) ||
+      !first_number(c.at("Ground Heat Coefficient"), &gh) ||
+      !first_number(c.at("Buoyancy Heat Coefficient"), &bc) ||
+      !first_number(c.at("Gravity Coefficient"), &gc) ||
+      !first_number(c.at("Adiabatic Lapse Rate"), &alr) ||
+      !first_number(c.at("Phase Transition Rate"), &ptr) ||
+      !first_number(c.at("Latent Heat Coefficient"), &lhc) ||
+      !first_number(c.at("Wind Coefficient"), &wind))
+    return bad();
+  // Clouds.cpp:283-302: rest density, time step and relaxation come from the "Fluids" tree
+  *p = fp;
+  *iters = it;
+  q->restDensity = fp.restDensity;
+  q->timeStep = fp.timeStep;
+  q->relaxCFM = fp.relaxCFM;
+  q->isTempSmoothingEnabled = smooth ? 1u : 0u;
+  q->groundHeatCoeff = (float)gh;
+  q->buoyancyCoeff = (float)bc;
+  q->gravCoeff = (float)gc;
+  q->adiabaticLapseRate = (float)alr;
+  q->phaseTransitionRate = (float)ptr;
+  q->latentHeatCoeff = (float)lhc;
+  q->windCoeff = (float)wind;
+  return RTP_OK;
+}
+
+int rtp_cloud_params_to_json(const rtp_fluid_params* p, const rtp_cloud_params* q, uint32_t iters,
+                             char* buf, size_t cap, size_t* len)
+{
+  if (!p || !q) {
+    g_jsonError = "NULL argument";
+    return bad();
+  }
+  // the ranges of initCloudsJson's "Clouds" tree (Clouds.cpp:88-97)
   const std::string s =
-      std::string("{\"Fluids\": {") +
-      "\"Rest Density\": " + range3(p->restDensity, 10.0f, 1000.0f) + ", " +
-      "\"Relax CFM\": " + range3(p->relaxCFM, 100.0f, 1000.0f) + ", " +
-      "\"Time Step\": " + range3(p->timeStep, 0.0001f, 0.020f) + ", " +
-      "\"Nb Jacobi Iterations\": " + range3(iters, 1, 6) + ", " +
-      "\"Artificial Pressure\": {\"Enable##Pressure\": " + (p->isArtPressureEnabled ? "true" : "false") +
-      ", \"Coefficient##Pressure\": " + range3(p->artPressureCoeff, 0.0f, 0.001f) +
-      ", \"Radius\": " + range3(p->artPressureRadius, 0.001f, 0.015f) +
-      ", \"Exp\": " + range3(p->artPressureExp, 1, 6) + "}, " +
-      "\"Vorticity Confinement\": {\"Enable##Vorticity\": " + (p->isVorticityConfEnabled ? "true" : "false") +
-      ", \"Coefficient##Vorticity\": " + range3(p->vorticityConfCoeff, 0.0f, 0.001f) +
-      ", \"xSPH Viscosity Coefficient\": " + range3(p->xsphViscosityCoeff, 0.0f, 0.001f) + "}}}";
+      "{" + fluids_tree(*p, iters) + ", \"Clouds\": {" +
+      "\"Enable Temperature Smoothing\": " + (q->isTempSmoothingEnabled ? "true" : "false") + ", " +
+      "\"Ground Heat Coefficient\": " + range3(q->groundHeatCoeff, 0.0f, 1000.0f) + ", " +
+      "\"Buoyancy Heat Coefficient\": " + range3(q->buoyancyCoeff, 0.0f, 5.0f) + ", " +
+      "\"Gravity Coefficient\": " + range3(q->gravCoeff, 0.0f, 0.1f) + ", " +
+      "\"Adiabatic Lapse Rate\": " + range3(q->adiabaticLapseRate, 0.0f, 20.0f) + ", " +
+      "\"Phase Transition Rate\": " + range3(q->phaseTransitionRate, 0.0f, 20.0f) + ", " +
+      "\"Latent Heat Coefficient\": " + range3(q->latentHeatCoeff, 0.0f, 0.100f) + ", " +
+      "\"Wind Coefficient\": " + range3(q->windCoeff, 0.0f, 1.0f) + "}}";
   return put(buf, cap, len, s);
 }
 

@@ -3,7 +3,7 @@
 // after `steps` updates so a test can replay the same steps on the CPU oracle.
 //
 //   model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget|fluidsjson|fluidsinput|boidsinput
-//              <out.bin> <steps>
+//              |clouds|cloudsinput <out.bin> <steps>
 //
 // The 2d kinds switch the model to 2-D (Model::setDimension, which resets it); boidscyclic
 // selects the cyclic wall (Model::setBoundary); boidstarget turns the target rule on with a
@@ -14,7 +14,10 @@
 // an unchanged tree is a no-op, a partial tree is merged (three iterations, vorticity off),
 // and reset() brings the initial tree back; the state dumped is after the edit.
 // boidsinput parks the target at (1, 0.5, 0) while it is off and then switches it on
-// through the tree (radius 2, attracting, shown).
+// through the tree (radius 2, attracting, shown).  clouds builds the Clouds model as the app
+// does (box and grid doubled in y, the cumulus case, app/ParticleSystemApp.cpp:300-304);
+// cloudsinput edits its tree (temperature smoothing off, three iterations) and checks the
+// typed parameters and inputJson().
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,10 +30,12 @@ static void put(FILE* f, const void* p, size_t n) { std::fwrite(p, 1, n, f); }
 int main(int argc, char** argv)
 {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget|fluidsjson|fluidsinput|boidsinput out.bin steps\n");
+    std::fprintf(stderr, "usage: model_demo fluids|boids|fluids2d|boids2d|boidscyclic|boidstarget|fluidsjson|fluidsinput|boidsinput|clouds|cloudsinput out.bin steps\n");
     return 2;
   }
   const bool fluids = std::strncmp(argv[1], "fluids", 6) == 0;
+  const bool clouds = std::strncmp(argv[1], "clouds", 6) == 0;
+  const bool cloudsInput = std::strcmp(argv[1], "cloudsinput") == 0;
   const bool dim2 = std::strstr(argv[1], "2d") != nullptr;
   const bool cyclic = std::strcmp(argv[1], "boidscyclic") == 0;
   const bool target = std::strcmp(argv[1], "boidstarget") == 0;
@@ -44,8 +49,14 @@ int main(int argc, char** argv)
   p.gridRes = Geometry::GRID_RES_3D;
   p.dimension = Geometry::Dimension::dim3D;
   p.pCase = fluids ? Utils::FLUIDS_DAM : Utils::BOIDS_LARGE;
+  if (clouds) {
+    p.pCase = Utils::CLOUDS_CUMULUS;
+    p.boxSize.y *= 2;
+    p.gridRes.y *= 2;
+  }
   try {
-    auto model = Physics::CreateModel(fluids ? Physics::FLUIDS : Physics::BOIDS, p);
+    auto model = Physics::CreateModel(
+        clouds ? Physics::CLOUDS : (fluids ? Physics::FLUIDS : Physics::BOIDS), p);
     if (dim2)
       model->setDimension(Geometry::Dimension::dim2D);  // app/ParticleSystemApp.cpp:428-431
     if (cyclic)
@@ -119,6 +130,27 @@ int main(int argc, char** argv)
       if (fl->nbJacobiIters() != 1000)
         return 18;
       fl->setNbJacobiIters(3);
+    }
+    if (cloudsInput) {
+      auto* cl = dynamic_cast<Physics::HIP::Clouds*>(model.get());
+      if (!cl)
+        return 30;
+      if (cl->nbJacobiIters() != 2 || cl->cloudParams().isTempSmoothingEnabled != 1 ||
+          cl->cloudParams().restDensity != 450.0f || cl->fluidParams().restDensity != 450.0f)
+        return 31;
+      model->updateInputJson(
+          "{\"Fluids\": {\"Nb Jacobi Iterations\": [3, 1, 6]},"
+          " \"Clouds\": {\"Enable Temperature Smoothing\": false, \"Wind Coefficient\": [0.5, 0, 1]}}");
+      const rtp_cloud_params q = cl->cloudParams();
+      if (cl->nbJacobiIters() != 3 || q.isTempSmoothingEnabled != 0 || q.windCoeff != 0.5f ||
+          q.groundHeatCoeff != 10.0f)
+        return 32;
+      const std::string back = model->inputJson();
+      if (back.find("\"Enable Temperature Smoothing\": false") == std::string::npos ||
+          back.find("\"Wind Coefficient\": [0.5, 0, 1]") == std::string::npos)
+        return 33;
+      if (cl->quantity("Temperature").size() != model->nbParticles())
+        return 34;
     }
     if (boidsInput) {
       auto* boids = dynamic_cast<Physics::HIP::Boids*>(model.get());

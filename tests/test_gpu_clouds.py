@@ -157,3 +157,36 @@ def test_cumulus_hundred_steps():
     log(f"clouds-65k after 100 steps: mean height {pos[:, 1].mean():.3f}, "
         f"max temperature {gpu.read_field(capi.FIELD_TEMPERATURE).max():.2f}")
     gpu.close()
+
+
+def test_capped_cell_and_wall_clusters_bit_exact():
+    """400 particles in one cell (the cap of 100 keeps 101 of them in its list, so its z-column
+    is not one contiguous range), clusters pressed against the periodic x and z walls and the
+    y walls (wrapped neighbours with +-2W shifts), among the cumulus particles."""
+    sc = scenes.cloud_scene("clouds-65k", seed=37)
+    rng = np.random.default_rng(11)
+    extra = []
+    # one interior cell of the 30 x 60 x 30 grid (cell size 1/3): x in [0, 1/3), y in [-8, -7.67)
+    c = np.zeros((400, 4), np.float32)
+    c[:, 0] = rng.uniform(0.01, 0.32, 400)
+    c[:, 1] = rng.uniform(-7.99, -7.68, 400)
+    c[:, 2] = rng.uniform(0.01, 0.32, 400)
+    extra.append(c)
+    for axis, side in ((0, 1), (0, -1), (2, 1), (2, -1), (1, -1)):
+        w = np.zeros((300, 4), np.float32)
+        w[:, :3] = rng.uniform([-4.9, -9.9, -4.9], [4.9, -5.1, 4.9], (300, 3))
+        half = sc.box[axis] / 2
+        w[:, axis] = side * rng.uniform(half - 0.3, half - 1e-4, 300)
+        extra.append(w)
+    pos = np.concatenate([sc.pos[:60000]] + extra)
+    sc.pos, sc.vel, sc.n = pos, np.zeros_like(pos), pos.shape[0]
+    gpu, orc = make_pair(sc)
+    for step in range(1, 5):
+        gpu.update(1)
+        orc.step(1)
+        log(f"capped cell and wall clusters, step {step}")
+        compare(gpu, orc, sc.n, f"step {step}", grid=True)
+        if step == 1:
+            _, _, se = gpu.read_grid()
+            assert (se[:, 1].astype(np.int64) - se[:, 0] == 100).any()  # a capped cell
+    gpu.close()

@@ -129,3 +129,27 @@ def test_cpp_boids_target_switched_on_by_the_tree(tmp_path):
     orc.step(3)
     _eq(pos1[:, :3], orc.pos[:n, :3], "pos")
     _eq(vel1[:, :3], orc.vel[:n, :3], "vel")
+
+
+@pytest.mark.parametrize("kind", ["clouds", "cloudsinput"])
+def test_cpp_clouds_cumulus_matches_oracle(tmp_path, kind):
+    """Clouds::reset's cumulus case (Clouds.cpp:463-468): 65,536 random particles in the lowest
+    quarter of the app's 10 x 20 x 10 box, then 3 updates, against the oracle from the dumped
+    initial state; cloudsinput first turns temperature smoothing off and sets three Jacobi
+    iterations and the wind coefficient 0.5 through the JSON tree."""
+    from realtimeparticles_amd import scenes
+    n, max_n, (pos0, vel0, pos1, vel1) = run_demo(kind, tmp_path, 3)
+    assert n == 65536 and max_n == 131072
+    assert np.all(pos0[:, 1] >= -10.0) and np.all(pos0[:, 1] <= -5.0)
+    assert np.all(np.abs(pos0[:, [0, 2]]) <= 5.0) and not vel0.any()
+    sc = scenes.cloud_scene("clouds-65k")
+    fl, cl, iters = sc.params(), sc.cloud_params(), 2
+    if kind == "cloudsinput":
+        cl.update(isTempSmoothingEnabled=0, windCoeff=0.5)
+        iters = 3
+    orc = O.OracleClouds((10, 20, 10), (30, 60, 30), max_n, n, O.fluid_params(**fl),
+                         O.cloud_params(**cl), iters, gravity=9.81)
+    orc.upload(pos0, vel0)
+    orc.step(3)
+    _eq(pos1, orc.pos[:n], "pos")
+    _eq(vel1, orc.vel[:n], "vel")

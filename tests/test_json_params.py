@@ -189,3 +189,40 @@ def test_json_equal_is_value_equality():
         eq("{", "{}")
     with pytest.raises(capi.RtpError, match="malformed"):
         capi.json_merge_patch("{}", "[1,")
+
+
+def test_clouds_tree_defaults_and_round_trip():
+    """initCloudsJson (Clouds.cpp:63-98) reads as CloudKernelInputs with the "Fluids" tree's
+    rest density, time step and relaxation (Clouds.cpp:283-302); a written tree reads back."""
+    tree = (
+        '{"Fluids": {"Rest Density": [450, 10, 1000], "Relax CFM": [600, 100, 1000], '
+        '"Time Step": [0.01, 0.0001, 0.02], "Nb Jacobi Iterations": [2, 1, 6], '
+        '"Artificial Pressure": {"Enable##Pressure": true, "Coefficient##Pressure": [0.001, 0, 0.001], '
+        '"Radius": [0.006, 0.001, 0.015], "Exp": [4, 1, 6]}, '
+        '"Vorticity Confinement": {"Enable##Vorticity": true, "Coefficient##Vorticity": [0.0004, 0, 0.001], '
+        '"xSPH Viscosity Coefficient": [0.0001, 0, 0.001]}}, '
+        '"Clouds": {"Enable Temperature Smoothing": true, "Ground Heat Coefficient": [10, 0, 1000], '
+        '"Buoyancy Heat Coefficient": [0.1, 0, 5], "Gravity Coefficient": [0.0005, 0, 0.1], '
+        '"Adiabatic Lapse Rate": [5, 0, 20], "Phase Transition Rate": [0.3485, 0, 20], '
+        '"Latent Heat Coefficient": [0.07, 0, 0.1], "Wind Coefficient": [1, 0, 1]}}')
+    p, q, it = capi.cloud_params_from_json(tree)
+    assert it == 2
+    assert q.restDensity == p.restDensity == 450.0
+    assert q.timeStep == p.timeStep == np.float32(0.01)
+    assert q.relaxCFM == 600.0 and q.isTempSmoothingEnabled == 1
+    assert q.groundHeatCoeff == 10.0 and q.buoyancyCoeff == np.float32(0.1)
+    assert q.gravCoeff == np.float32(0.0005) and q.adiabaticLapseRate == 5.0
+    assert q.phaseTransitionRate == np.float32(0.3485) and q.latentHeatCoeff == np.float32(0.07)
+    assert q.windCoeff == 1.0 and q.dim == 3 and q.initVaporDensityCoeff == 0.75
+    q.windCoeff = 0.25
+    q.isTempSmoothingEnabled = 0
+    back = capi.cloud_params_to_json(p, q, 3)
+    p2, q2, it2 = capi.cloud_params_from_json(back)
+    assert it2 == 3 and q2.windCoeff == 0.25 and q2.isTempSmoothingEnabled == 0
+    assert bytes(p2) == bytes(p)
+    # a numeric smoothing switch is a type error, as nlohmann::json's bool conversion
+    with pytest.raises(capi.RtpError, match="Enable Temperature Smoothing"):
+        capi.cloud_params_from_json(tree.replace('"Enable Temperature Smoothing": true',
+                                                 '"Enable Temperature Smoothing": 1'))
+    with pytest.raises(capi.RtpError, match="Wind Coefficient"):
+        capi.cloud_params_from_json(tree.replace('"Wind Coefficient": [1, 0, 1]', '"Wind": [1, 0, 1]'))
