@@ -551,9 +551,10 @@ def run_learned(args, sc):
                           "ms_per_step": round(v.ms_per_step, 4),
                           "density_error": {"max": v.residual_max, "mean": v.residual_mean,
                                             "max_compression": v.residual_compression}}
-        if v.fallback_set != (0, 0):
+        if any(v.fallback_set):
             variants[name]["fallback_set"] = {"seeds": v.fallback_set[0],
-                                              "particles": v.fallback_set[1]}
+                                              "particles": v.fallback_set[1],
+                                              "halo": v.fallback_set[2]}
         if name.startswith("learned"):
             entry = {}
             for key in ("residual_mean", "residual_max", "residual_compression"):

@@ -123,7 +123,10 @@ typedef enum {
   RTP_FIELD_PARTICLE_ID = 12,    /* p_partID (float) */
   RTP_FIELD_TOT_CORR_POS = 13,   /* p_totCorrPos (float4) */
   RTP_FIELD_LAPLACIAN_TEMP = 14, /* p_laplacianTemp */
-  RTP_FIELD_LAMBDA_TEMP = 15     /* p_constFactorTemp */
+  RTP_FIELD_LAMBDA_TEMP = 15,    /* p_constFactorTemp */
+  /* the learned warm-start (no reference counterpart), per sorted slot of the last update: */
+  RTP_FIELD_RESIDUAL_DENSITY = 16, /* post-correction density of the last residual pass (float) */
+  RTP_FIELD_FALLBACK_SET = 17      /* fallback set (uint32 words: 1 = set, 2 = halo, 0 = outside) */
 } rtp_field;
 
 /* physics/ocl/Clouds.hpp:14-48 CloudKernelInputs == clouds.cl:18-32 CloudParams (52 bytes). */
@@ -307,9 +310,10 @@ int rtp_mlp_train(rtp_mlp* h, const float* feat, const float* target, uint32_t n
  * substep equals one without fallback, bit for bit. */
 int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters,
                               uint32_t fallbackIters, float tau);
-/* Particles above tau (seeds) and in the fallback set (seeds and their neighbours) of the
- * last learned substep with fallback; waits for the handle's stream. */
-int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles);
+/* Particles above tau (seeds), in the fallback set (seeds and their neighbours within h) and
+ * in its halo (the other particles within h of the set; may be NULL) of the last learned
+ * substep with fallback; waits for the handle's stream. */
+int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles, uint32_t* halo);
 /* Run a post-correction density pass in every substep, plain or learned, after any
  * fallback iterations (diagnostics: the error the substep leaves). */
 int rtp_fluids_set_residual_pass(rtp_fluids* h, int enable);

@@ -368,14 +368,15 @@ class OracleFluids:
         - predict, cell keys, sort, permutation, cell ranges (:412-424);
         - learned_iters Jacobi iterations (:427-439) whose first takes its lambda from
           network(features) -- the 8 inputs of the first density pass (orc_fld_features);
-        - with fallback_iters > 0: the post-correction density, the fallback set
-          (orc_fallback_set at tau) and fallback_iters Jacobi iterations that correct only the
-          set's particles, with every other particle's lambda 0 (its density is kept; an
-          empty set changes nothing);
+        - with fallback_iters > 0: the post-correction density, the fallback set and its
+          halo (orc_fallback_set at tau) and fallback_iters Jacobi iterations that recompute
+          the density and lambda of the set and the halo and correct only the set's
+          particles, with every other particle's lambda 0 (its density is kept; an empty set
+          changes nothing);
         - the velocity update, vorticity confinement, XSPH and commit (:442-457).
 
-        Returns the fallback set's flags: 2 seed, 1 other member, 0 outside (all 0 without
-        fallback)."""
+        Returns the fallback set's flags: 2 seed, 1 other member, 3 halo, 0 outside (all 0
+        without fallback)."""
         L, c, p = lib(), ctypes.byref(self.c), ctypes.byref(self.p)
         n, max_n = self.n, self.max_n
         L.orc_fld_predict(ptr(self.pos), ptr(self.vel), p, ptr(self.pred), n, c)
@@ -407,15 +408,16 @@ class OracleFluids:
             L.orc_fld_density(ptr(self.pred), ptr(self.se), p, ptr(res), n, c)
             L.orc_fallback_set(ptr(self.pred), ptr(self.se), ptr(res), float(self.p.restDensity),
                                float(tau), ptr(act), n, c)
-            on = act > 0
+            on = (act == 1) | (act == 2)
+            known = act > 0  # the set and its halo
             dens = np.zeros(max_n, np.float32)
             lam = np.zeros(max_n, np.float32)
             for _ in range(fallback_iters if on.any() else 0):
                 L.orc_fld_boundary(ptr(self.pred), n, c)
                 L.orc_fld_density(ptr(self.pred), ptr(self.se), p, ptr(dens), n, c)
                 L.orc_fld_lambda(ptr(self.pred), ptr(dens), ptr(self.se), p, ptr(lam), n, c)
-                self.density[:n][on] = dens[:n][on]
-                self.lam[:n] = np.where(on, lam[:n], np.float32(0.0))
+                self.density[:n][known] = dens[:n][known]
+                self.lam[:n] = np.where(known, lam[:n], np.float32(0.0))
                 L.orc_fld_correction(ptr(self.lam), ptr(self.se), ptr(self.pred), p,
                                      ptr(self.corr), n, c)
                 moved = self.pred.copy()

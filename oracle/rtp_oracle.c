@@ -849,7 +849,9 @@ int orc_boids_update(orc_boids* s) { return boids_physics(s, NULL); }
 /* The learned substep's fallback set (no reference counterpart; the rule the library applies
  * after its learned iterations).  A particle whose |density/rho0 - 1| exceeds tau is a seed;
  * every particle a seed's 27-cell walk (fluids.cl:101-123) meets at length(p_seed - p_j) < h
- * joins the set.  act: n flags (2 = seed, 1 = other member, 0 = outside). */
+ * joins the set, and sorted element 0 (met by no walk) when its own walk meets a seed there.
+ * Every other particle within h of the set is its halo.  act: n flags (2 = seed, 1 = other
+ * member, 3 = halo, 0 = outside). */
 void orc_fallback_set(const orc_f4* pred, const orc_u2* se, const float* density, float rho0,
                       float tau, uint8_t* act, uint32_t n, const orc_consts* c)
 {
@@ -864,6 +866,35 @@ void orc_fallback_set(const orc_f4* pred, const orc_u2* se, const float* density
     {
       if (len4(sub4(pos, pred[e])) < h && act[e] == 0)
         act[e] = 1;
+    }
+  }
+  /* Sorted element 0 is in no cell list (grid.cl:95-108), so no seed's walk meets it: it
+   * joins when a seed lies within h on its own walk. */
+  if (n > 0 && act[0] == 0) {
+    const orc_f4 pos = pred[0];
+    ORC_FLD_FOR_NEIGHBOURS(c, pos, e)
+    {
+      if (len4(sub4(pos, pred[e])) < h && fabsf(density[e] / rho0 - 1.0f) > tau)
+        act[0] = 1;
+    }
+  }
+  /* The halo: every other particle within h of a set member (element 0 by its own walk). */
+  for (uint32_t i = 0; i < n; ++i) {
+    if (act[i] != 1 && act[i] != 2)
+      continue;
+    const orc_f4 pos = pred[i];
+    ORC_FLD_FOR_NEIGHBOURS(c, pos, e)
+    {
+      if (len4(sub4(pos, pred[e])) < h && act[e] == 0)
+        act[e] = 3;
+    }
+  }
+  if (n > 0 && act[0] == 0) {
+    const orc_f4 pos = pred[0];
+    ORC_FLD_FOR_NEIGHBOURS(c, pos, e)
+    {
+      if (len4(sub4(pos, pred[e])) < h && (act[e] == 1 || act[e] == 2))
+        act[0] = 3;
     }
   }
 }

@@ -128,8 +128,9 @@ void orc_fld_features(const orc_f4* pred, const float* density, const orc_u2* se
                       const float* lambda, const orc_f4* vel, const orc_fluid_params* p,
                       float* feat, uint32_t n, const orc_consts* c);
 /* The learned substep's fallback set: seeds are the particles with |density/rho0 - 1| > tau;
- * every particle a seed's 27-cell walk meets at length(p_seed - p_j) < h joins them.
- * act: n flags, 2 = seed, 1 = other member, 0 = outside. */
+ * every particle within h of a seed joins them (met by the seed's 27-cell walk, or, for
+ * sorted element 0, by its own).  The halo: every other particle within h of the set.
+ * act: n flags, 2 = seed, 1 = other member, 3 = halo, 0 = outside. */
 void orc_fallback_set(const orc_f4* pred, const orc_u2* se, const float* density, float rho0,
                       float tau, uint8_t* act, uint32_t n, const orc_consts* c);
 

@@ -24,6 +24,7 @@ FIELD_DENSITY, FIELD_LAMBDA, FIELD_PRED_POS, FIELD_VORTICITY, FIELD_ACC = range(
 FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
 (FIELD_TEMPERATURE, FIELD_VAPOR_DENSITY, FIELD_CLOUD_DENSITY, FIELD_BUOYANCY, FIELD_PARTICLE_ID,
  FIELD_TOT_CORR_POS, FIELD_LAPLACIAN_TEMP, FIELD_LAMBDA_TEMP) = range(8, 16)
+FIELD_RESIDUAL_DENSITY, FIELD_FALLBACK_SET = 16, 17
 BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
 
@@ -227,7 +228,8 @@ _SIGS = {
     "rtp_mlp_gradients": ([_vp, _vp, _vp, _u32, _vp, _vp], _int),
     "rtp_mlp_train": ([_vp, _vp, _vp, _u32, _u32, _u32, _f32, _u32, _vp], _int),
     "rtp_fluids_set_warm_start": ([_vp, _vp, _u32, _u32, _f32], _int),
-    "rtp_fluids_fallback_count": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
+    "rtp_fluids_fallback_count": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
+                                   ctypes.POINTER(_u32)], _int),
     "rtp_fluids_set_residual_pass": ([_vp, _int], _int),
     "rtp_fluids_residual_error": ([_vp, _vp], _int),
     "rtp_fluids_record_teacher": ([_vp, _u32, _vp, _vp], _int),
@@ -502,7 +504,7 @@ class Fluids(_RenderFields):
     def read_field(self, field: int, count: int | None = None) -> np.ndarray:
         count = self.n if count is None else int(count)
         shape = (count, 4) if field in (FIELD_PRED_POS, FIELD_VORTICITY) else (count,)
-        out = np.zeros(shape, np.float32)
+        out = np.zeros(shape, np.uint32 if field == FIELD_FALLBACK_SET else np.float32)
         _check(lib().rtp_fluids_read_field(self._h, int(field), out.ctypes.data, count),
                "rtp_fluids_read_field")
         return out
@@ -563,10 +565,15 @@ class Fluids(_RenderFields):
 
     def fallback_count(self):
         """(seeds, particles) of the last learned substep's fallback set."""
-        a, b = _u32(0), _u32(0)
-        _check(lib().rtp_fluids_fallback_count(self._h, ctypes.byref(a), ctypes.byref(b)),
-               "fallback_count")
-        return int(a.value), int(b.value)
+        return self.fallback_sizes()[:2]
+
+    def fallback_sizes(self):
+        """(seeds, particles in the set, particles in its halo) of the last learned
+        substep's fallback."""
+        a, b, c = _u32(0), _u32(0), _u32(0)
+        _check(lib().rtp_fluids_fallback_count(self._h, ctypes.byref(a), ctypes.byref(b),
+                                               ctypes.byref(c)), "fallback_count")
+        return int(a.value), int(b.value), int(c.value)
 
     def record_teacher(self, teacher_iters: int = 20):
         feat = np.zeros((self.n, 8), np.float32)

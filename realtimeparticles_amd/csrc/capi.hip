@@ -561,6 +561,7 @@ struct rtp_fluids {
     dev_free(fb.act);
     dev_free(fb.seeds);
     dev_free(fb.list);
+    dev_free(fb.halo);
     dev_free(fb.count);
     dev_free(lamAcc);
     dev_free(resDensity);
@@ -952,6 +953,12 @@ int rtp_fluids_read_field(rtp_fluids* h, int field, float* out, uint32_t count)
   case RTP_FIELD_LAMBDA: src = h->lambda; break;
   case RTP_FIELD_PRED_POS: src = h->pred[h->predFinal]; elem = sizeof(float4); break;
   case RTP_FIELD_VORTICITY: src = h->vort; elem = sizeof(float4); break;
+  case RTP_FIELD_RESIDUAL_DENSITY: src = h->resDensity; break;
+  case RTP_FIELD_FALLBACK_SET:
+    if (!h->fb.act)
+      return fail(RTP_ERR_INVALID, "no fallback set: the warm start has no fallback iterations");
+    src = h->fb.act;
+    break;
   case RTP_FIELD_COLOR:
   case RTP_FIELD_CAMERA_DIST:
   case RTP_FIELD_DETECTOR:
@@ -1792,10 +1799,12 @@ int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters
     FallbackSet& fb = h->fb;
     int rc = 0;
     if ((rc = dev_alloc(&fb.act, maxN)) || (rc = dev_alloc(&fb.seeds, maxN)) ||
-        (rc = dev_alloc(&fb.list, maxN)) || (rc = dev_alloc(&fb.count, 4))) {
+        (rc = dev_alloc(&fb.list, maxN)) || (rc = dev_alloc(&fb.halo, maxN)) ||
+        (rc = dev_alloc(&fb.count, 4))) {
       dev_free(fb.act);
       dev_free(fb.seeds);
       dev_free(fb.list);
+      dev_free(fb.halo);
       dev_free(fb.count);
       fb = FallbackSet();
       return rc;
@@ -1810,10 +1819,10 @@ int rtp_fluids_set_warm_start(rtp_fluids* h, rtp_mlp* mlp, uint32_t learnedIters
   return RTP_OK;
 }
 
-int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles)
+int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particles, uint32_t* halo)
 {
   RTP_REQUIRE(h && seeds && particles, "NULL argument");
-  uint32_t v[2] = { 0u, 0u };
+  uint32_t v[3] = { 0u, 0u, 0u };
   if (h->fb.count) {
     RTP_HIP_TRY(hipSetDevice(h->device));
     RTP_HIP_TRY(hipMemcpyAsync(v, h->fb.count, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -1821,6 +1830,8 @@ int rtp_fluids_fallback_count(rtp_fluids* h, uint32_t* seeds, uint32_t* particle
   }
   *seeds = v[0];
   *particles = v[1];
+  if (halo)
+    *halo = v[2];
   return RTP_OK;
 }
 

@@ -854,14 +854,19 @@ __global__ __launch_bounds__(256) void k_fb_mark(const float* __restrict__ resDe
 }
 
 // Every particle the walk of a seed meets within h (r = sqrt(d2) < h, d2 as the density
-// pass computes it) joins the set; the seed itself (r = 0) is already in.
+// pass computes it) joins the set; the seed itself (r = 0) is already in.  Sorted element 0
+// is in no cell list (fillStartCell skips it), so no seed's walk meets it: it joins when its
+// own walk meets a seed within h (the same distance, negated difference).
 __global__ __launch_bounds__(256) void k_fb_dilate(const float4* __restrict__ pred,
-                                                   const uint2* __restrict__ se, PbfConsts c,
+                                                   const uint2* __restrict__ se,
+                                                   const float* __restrict__ resDensity,
+                                                   uint32_t n, float tau, PbfConsts c,
                                                    uint32_t qdepth, FallbackSet fs)
 {
   RTP_PLAIN_CTX(qdepth)
   const uint32_t cnt = fs.count[0];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t k = t; k < cnt; k += gridDim.x * blockDim.x) {
     const uint32_t e = fs.seeds[k];
     walk27(pred[e], pred, se, c, [](uint32_t) { return 0; },
            [&](uint32_t j, int, float, float, float, float d2) {
@@ -869,13 +874,57 @@ __global__ __launch_bounds__(256) void k_fb_dilate(const float4* __restrict__ pr
                fs.act[j] = 1u;
            }, W);
   }
+  if (t == 0 && n > 0 && cnt > 0 && fs.act[0] == 0u) {
+    bool near = false;
+    const float rho0 = c.p.restDensity;
+    walk27(pred[0], pred, se, c, [](uint32_t) { return 0; },
+           [&](uint32_t j, int, float, float, float, float d2) {
+             if (__builtin_sqrtf(d2) < c.h && fabsf(resDensity[j] / rho0 - 1.0f) > tau)
+               near = true;
+           }, W);
+    if (near)
+      fs.act[0] = 1u;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_fb_compact(uint32_t n, FallbackSet fs)
+// The particles flagged `flag` in act, appended to out and counted in *count.
+__global__ __launch_bounds__(256) void k_fb_compact(uint32_t n, uint32_t flag,
+                                                    const uint32_t* __restrict__ act,
+                                                    uint32_t* __restrict__ out,
+                                                    uint32_t* __restrict__ count)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool on = t < n && fs.act[t] != 0u;
-  wave_append(on, t, fs.list, fs.count + 1);
+  const bool on = t < n && act[t] == flag;
+  wave_append(on, t, out, count);
+}
+
+// The halo: every particle outside the set that the walk of a set member meets within h,
+// and sorted element 0 (met by no walk) when its own walk meets a set member within h.
+__global__ __launch_bounds__(256) void k_fb_halo(const float4* __restrict__ pred,
+                                                 const uint2* __restrict__ se, uint32_t n,
+                                                 PbfConsts c, uint32_t qdepth, FallbackSet fs)
+{
+  RTP_PLAIN_CTX(qdepth)
+  const uint32_t cnt = fs.count[1];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t k = t; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t e = fs.list[k];
+    walk27(pred[e], pred, se, c, [](uint32_t) { return 0; },
+           [&](uint32_t j, int, float, float, float, float d2) {
+             if (__builtin_sqrtf(d2) < c.h && fs.act[j] == 0u)
+               fs.act[j] = 2u;
+           }, W);
+  }
+  if (t == 0 && n > 0 && cnt > 0 && fs.act[0] == 0u) {
+    bool near = false;
+    walk27(pred[0], pred, se, c, [](uint32_t) { return 0; },
+           [&](uint32_t j, int, float, float, float, float d2) {
+             if (__builtin_sqrtf(d2) < c.h && fs.act[j] == 1u)
+               near = true;
+           }, W);
+    if (near)
+      fs.act[0] = 2u;
+  }
 }
 
 // The start of the fallback iterations when the set is not empty: fld_applyBoundaryCondition
@@ -905,8 +954,8 @@ __global__ __launch_bounds__(256) void k_fb_scatter(float4* __restrict__ pred,
   }
 }
 
-// fld_computeDensity + fld_computeConstraintFactor over the set (list position k is the
-// thread's survivor-list slot).
+// fld_computeDensity + fld_computeConstraintFactor over the set and then the halo (position
+// k in that order is the thread's survivor-list slot; the set's slots are the correction's).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_density_lambda_local(const float4* __restrict__ pred,
                                                               const uint2* __restrict__ se,
@@ -918,10 +967,11 @@ __global__ __launch_bounds__(256) void k_density_lambda_local(const float4* __re
   RTP_PLAIN_CTX(qdepth)
   W.nb = nb;
   const LearnArgs la;
-  const uint32_t cnt = fs.count[1];
+  const uint32_t cs = fs.count[1], cnt = cs + fs.count[2];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
     W.slot = k;
-    density_lambda_body<MODE>(fs.list[k], pred, se, density, lambda, c, la, W);
+    density_lambda_body<MODE>(k < cs ? fs.list[k] : fs.halo[k - cs], pred, se, density, lambda,
+                              c, la, W);
   }
 }
 
@@ -1542,14 +1592,20 @@ void launch_fallback_set(const float4* pred, const uint2* startEnd, const float*
                          float tau, uint32_t n, const PbfConsts& c, const Launch& L,
                          const FallbackSet& fs, hipStream_t s)
 {
-  launch_fill_u32(fs.count, 2, 0u, s);
+  launch_fill_u32(fs.count, 3, 0u, s);
   if (n == 0)
     return;
+  const dim3 g(list_blocks(n, L.block)), b(L.block);
   hipLaunchKernelGGL(k_fb_mark, dim3(blocks_for(n, 256)), dim3(256), 0, s, resDensity, n,
                      c.p.restDensity, tau, fs);
-  hipLaunchKernelGGL(k_fb_dilate, dim3(list_blocks(n, L.block)), dim3(L.block), queue_bytes(L), s,
-                     pred, startEnd, c, (uint32_t)L.queue, fs);
-  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, fs);
+  hipLaunchKernelGGL(k_fb_dilate, g, b, queue_bytes(L), s, pred, startEnd, resDensity, n, tau, c,
+                     (uint32_t)L.queue, fs);
+  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, 1u, fs.act,
+                     fs.list, fs.count + 1);
+  hipLaunchKernelGGL(k_fb_halo, g, b, queue_bytes(L), s, pred, startEnd, n, c, (uint32_t)L.queue,
+                     fs);
+  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, 2u, fs.act,
+                     fs.halo, fs.count + 2);
 }
 
 void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,

@@ -274,12 +274,15 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
                         const PassOrder& po = PassOrder());
 // The learned substep's per-particle fallback (no reference counterpart).  The set: the
 // particles whose post-correction |rho/rho0 - 1| (resDensity) exceeds tau, and every
-// particle within h of one of them in its walk.  act, seeds and list hold maxN entries;
-// count[0] = seeds, count[1] = particles in the set.
+// particle within h of one of them.  The halo: every other particle within h of the set;
+// its density and lambda are recomputed each fallback iteration, but it does not move.
+// act (1 = set, 2 = halo), seeds, list (the set) and halo hold maxN entries; count[0] =
+// seeds, count[1] = particles in the set, count[2] = particles in the halo.
 struct FallbackSet {
   uint32_t* act = nullptr;
   uint32_t* seeds = nullptr;
   uint32_t* list = nullptr;
+  uint32_t* halo = nullptr;
   uint32_t* count = nullptr;
 };
 void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
@@ -287,9 +290,9 @@ void launch_fallback_set(const float4* pred, const uint2* startEnd, const float*
                          const FallbackSet& fs, hipStream_t s);
 // One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, in place in
 // pred: the first (when the set is not empty) clamps every particle to the walls and sets
-// the lambda of every particle outside the set to 0; the set's corrections are computed into
-// scratch (indexed by particle) and then written back.  nb: survivor lists recorded by the
-// density pass and replayed by the correction.
+// the lambda of every particle to 0; density and lambda are computed for the set and the
+// halo, and the set's corrections into scratch (indexed by particle), then written back.
+// nb: survivor lists recorded by the density pass and replayed by the correction.
 void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,
                                const uint2* startEnd, uint32_t n, bool first, bool last,
                                const PbfConsts& c, const Launch& L, const FallbackSet& fs,

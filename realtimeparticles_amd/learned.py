@@ -9,7 +9,9 @@ overshoot (every particle at the free surface keeps a lambda of the same sign wh
 displacement saturates), so the target stays at that count.  A learned substep runs
 `learned_iters` iterations whose first uses the network's lambda; with fallback, every
 particle whose post-correction |rho/rho0 - 1| exceeds tau, and every particle within h of
-one, then runs plain iterations (the others keep their positions).
+one, then runs plain iterations.  The particles within h of that set (its halo) recompute
+their density and lambda in each of those iterations, so the set yields to a compressed
+neighbour, but keep their positions, as every other particle does.
 
 Everything here runs on the GPU through the C ABI: teacher substeps, the training loop
 (forward, backward and Adam kernels) and the evaluation.
@@ -87,7 +89,7 @@ class Variant:
     residual_max: float
     residual_mean: float
     residual_compression: float
-    fallback_set: tuple = (0, 0)  # (seeds, particles) of the fallback set, learned runs
+    fallback_set: tuple = (0, 0, 0)  # (seeds, set, halo) of the fallback, learned runs
 
 
 def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup: int = 3,
@@ -129,7 +131,7 @@ def evaluate(sc: scenes.FluidScene, mapping: str, runs, steps: int = 20, warmup:
         m.update(1)
         r = m.residual_error()
         m.set_residual_pass(False)
-        fset = m.fallback_count() if (mlp is not None and fb > 0) else (0, 0)
+        fset = m.fallback_sizes() if (mlp is not None and fb > 0) else (0, 0, 0)
         out.append(Variant(name, sc.n * steps / el, el * 1e3 / steps, float(r[0]), float(r[1]),
                            float(r[2]), fset))
     m.set_warm_start(None)

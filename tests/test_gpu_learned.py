@@ -8,6 +8,8 @@
   the 1,048,576-particle dam break;
 * what the trained network gains over the same iteration budget without it.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -128,6 +130,18 @@ def _learned_run(sc, prm, mlp, learned_iters, fb, tau, tuning=None, residual=Fal
     return gpu, orc, act
 
 
+def _check_set(gpu, act, n):
+    """The GPU's fallback sizes and flags (1 set, 2 halo) against the oracle's (2 seed,
+    1 other member, 3 halo)."""
+    member = (act[:n] == 1) | (act[:n] == 2)
+    halo = act[:n] == 3
+    assert gpu.fallback_sizes() == (int((act == 2).sum()), int(member.sum()), int(halo.sum()))
+    flags = gpu.read_field(capi.FIELD_FALLBACK_SET, n)
+    _eq(flags == 1, member, "fallback set")
+    _eq(flags == 2, halo, "fallback halo")
+    return flags
+
+
 def _threshold_between(sc, prm, mlp):
     """A threshold between the mean and the max residual error of learned-1, so that some
     particles but not all seed the fallback set."""
@@ -155,10 +169,11 @@ def test_learned_substep_matches_oracle(gate, shape):
     tuning = (64, capi.LAUNCH_XCD_MAP | capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST, 0, 16) \
         if shape == "tuned" else None
     gpu, orc, act = _learned_run(sc, prm, mlp, 1, 3, tau, tuning)
-    log(f"gate {gate} (tau {tau:.4g}), {shape}: oracle set {int((act > 0).sum())} of {sc.n}, "
-        f"{int((act == 2).sum())} seeds; GPU {gpu.fallback_count()}")
+    log(f"gate {gate} (tau {tau:.4g}), {shape}: oracle set {int(((act == 1) | (act == 2)).sum())} "
+        f"of {sc.n}, {int((act == 2).sum())} seeds, halo {int((act == 3).sum())}; "
+        f"GPU {gpu.fallback_sizes()}")
     compare_pbf(gpu, orc, sc.n, 1)
-    assert gpu.fallback_count() == (int((act == 2).sum()), int((act > 0).sum()))
+    _check_set(gpu, act, sc.n)
     seeds, members = gpu.fallback_count()
     if gate == "none":
         assert members == 0
@@ -169,6 +184,73 @@ def test_learned_substep_matches_oracle(gate, shape):
         assert seeds == members == sc.n
     else:
         assert 0 < seeds < members < sc.n
+    gpu.close()
+    mlp.close()
+
+
+def _within_h(q, mask, h):
+    out = np.zeros(q.shape[0], bool)
+    for i in np.flatnonzero(mask):
+        d = q[i] - q
+        out |= np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) < h
+    return out
+
+
+@pytest.mark.parametrize("role", ["set", "halo"])
+def test_fallback_takes_sorted_element_0(role):
+    """Sorted element 0 is in no cell list (Q1), so no walk of another particle meets it.  A
+    clump moved near the dam block's corner particle (input 0, which sorts to slot 0) carries
+    larger residual errors than that corner.  "set": the clump at 0.7 h, tau between the
+    corner's error and the clump's: slot 0 is no seed but lies within h of one.  "halo": the
+    clump at 1.3 h, tau such that slot 0 is outside the set but within h of a member (found by
+    brute force on the probe's positions).  Either way slot 0 gets its flag by its own walk,
+    on the GPU as in the oracle, and the substep stays the oracle's bit for bit."""
+    base = small_scene(22)
+    pos = base.pos.copy()
+    rng = np.random.default_rng(0)
+    off, spread, k = ((0.07, 0.02, 0.02), 0.004, 8) if role == "set" else ((0.13, 0.03, 0.0), 0.01, 16)
+    pos[-k:, :3] = (pos[0, :3] + np.array(off, np.float32)
+                    + rng.uniform(-spread, spread, (k, 3)).astype(np.float32))
+    sc = dataclasses.replace(base, pos=pos)
+    prm = sc.params("stable")
+    mlp = _small_net(5)
+    probe = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    probe.set_params(**prm)
+    probe.load_state(sc.pos, sc.vel)
+    probe.set_warm_start(mlp, 1, 0, 0.0)
+    probe.set_residual_pass(True)
+    probe.update(1)
+    assert probe.read_grid()[1][0] == 0  # the corner particle is sorted element 0
+    rho0 = np.float32(prm["restDensity"])
+    err = np.abs(probe.read_field(capi.FIELD_RESIDUAL_DENSITY) / rho0 - np.float32(1.0))
+    q = probe.read_field(capi.FIELD_PRED_POS)[:, :3]
+    probe.close()
+    d = q[0] - q
+    r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    h = np.float32(sc.box[0]) / np.float32(sc.res[0])  # the cell size is the effect radius
+    if role == "set":
+        near = (r < np.float32(0.8) * h) & (np.arange(sc.n) > 0)
+        assert near.any() and float(err[near].max()) > float(err[0])
+        tau = float((float(err[near].max()) + float(err[0])) / 2)
+    else:
+        lo = max(float(err[0]), float(err[r < h].max()))
+        cands = np.unique(err[err > lo])
+        tau = None
+        for i in range(cands.size - 1, 0, -1):
+            t = float((cands[i] + cands[i - 1]) / 2)
+            member = _within_h(q, err > t, h) | (err > t)
+            if not member[0] and (member & (r < np.float32(0.9) * h)).any():
+                tau = t
+                break
+        assert tau is not None, "no tau leaves element 0 in the halo"
+    gpu, orc, act = _learned_run(sc, prm, mlp, 1, 3, tau)
+    log(f"element 0 ({role}): error {err[0]:.4g}, tau {tau:.4g}; sizes {gpu.fallback_sizes()}")
+    compare_pbf(gpu, orc, sc.n, 1)
+    flags = _check_set(gpu, act, sc.n)
+    if role == "set":
+        assert act[0] == 1 and flags[0] == 1
+    else:
+        assert act[0] == 3 and flags[0] == 2
     gpu.close()
     mlp.close()
 
@@ -223,11 +305,12 @@ def test_learned_substep_1m_matches_oracle():
     orc.upload(sc.pos, sc.vel)
     log("pbf-1m learned substep: oracle")
     act = orc.learned_step(network, 1, 3, tau)
-    log(f"pbf-1m learned substep: tau {tau:.4g}, fallback set {int((act > 0).sum())} particles, "
-        f"{int((act == 2).sum())} seeds; GPU {gpu.fallback_count()}")
+    log(f"pbf-1m learned substep: tau {tau:.4g}, fallback set "
+        f"{int(((act == 1) | (act == 2)).sum())} particles, {int((act == 2).sum())} seeds, halo "
+        f"{int((act == 3).sum())}; GPU {gpu.fallback_sizes()}")
     compare_pbf(gpu, orc, sc.n, 1)
-    assert gpu.fallback_count() == (int((act == 2).sum()), int((act > 0).sum()))
-    assert 0 < int((act == 2).sum()) < sc.n
+    _check_set(gpu, act, sc.n)
+    assert 0 < int((act == 2).sum()) < sc.n and (act == 3).any()
     p = mlp.params()
     ref, _ = MO.forward(p.astype(np.float64), captured["feat"].astype(np.float64),
                         np.zeros(8), np.ones(8), 1e-4, bf16_ops=True)

@@ -81,14 +81,14 @@ def test_empty_fallback_set_is_learned_alone(oracle):
     assert (b.pred[: sc.n, 0] < lo).any()  # a clamp would have moved these
 
 
-def test_fallback_set_matches_brute_force(oracle):
-    """Seeds |rho/rho0 - 1| > tau and every particle within h of a seed, on positions gridded
-    as they are (so the walk covers every particle within h)."""
-    sc = _scene(6)
+def _gridded(oracle, seed):
+    """A jittered dam break whose positions are gridded as they are (so the 27-cell walk
+    covers every particle within h), its density errors and the model."""
+    sc = _scene(seed)
     prm = sc.params("stable")
     f = O.OracleFluids(sc.box, sc.res, sc.max_n, sc.n, O.fluid_params(**prm), 1,
                        gravity=sc.gravity)
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(seed)
     pos = sc.pos.copy()
     pos[:, :3] += rng.uniform(-0.03, 0.03, (sc.n, 3)).astype(np.float32)
     f.upload(pos, sc.vel)
@@ -104,19 +104,86 @@ def test_fallback_set_matches_brute_force(oracle):
     L.orc_fld_density(pt(f.pred), pt(f.se), p, pt(dens), n, c)
     rho0 = np.float32(prm["restDensity"])
     err = np.abs(dens[:n] / rho0 - np.float32(1.0))
-    tau = float(np.quantile(err, 0.9))
+    return f, dens, rho0, err
+
+
+def _set_and_brute_force(f, dens, rho0, err, tau):
+    """The oracle's flags, the seeds, and the set and halo by brute force over all pairs."""
+    n = f.n
+    L, c, pt = O.lib(), ctypes.byref(f.c), O.ptr
     act = np.zeros(n, np.uint8)
     L.orc_fallback_set(pt(f.pred), pt(f.se), pt(dens), float(rho0), tau, pt(act), n, c)
     seeds = err > np.float32(tau)
-    assert np.array_equal(act == 2, seeds)
     q = f.pred[:n, :3].astype(np.float32)
-    want = seeds.copy()
     h = np.float32(f.c.effectRadius)
-    for i in np.flatnonzero(seeds):
+
+    def within_h_of(mask):
+        out = np.zeros(n, bool)
+        for i in np.flatnonzero(mask):
+            d = q[i] - q
+            out |= np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) < h
+        return out
+
+    want = seeds | within_h_of(seeds)
+    halo = within_h_of(want) & ~want
+    return act, seeds, want, halo
+
+
+def test_fallback_set_matches_brute_force(oracle):
+    """Seeds |rho/rho0 - 1| > tau and every particle within h of a seed."""
+    f, dens, rho0, err = _gridded(oracle, 6)
+    tau = float(np.quantile(err, 0.9))
+    act, seeds, want, halo = _set_and_brute_force(f, dens, rho0, err, tau)
+    assert np.array_equal(act == 2, seeds)
+    assert np.array_equal((act == 1) | (act == 2), want)
+    assert np.array_equal(act == 3, halo)
+    assert 0 < seeds.sum() < want.sum() < want.sum() + halo.sum() < f.n
+
+
+def test_fallback_set_takes_element_0_through_its_own_walk(oracle):
+    """Sorted element 0 is in no cell list (Q1), so no seed's walk meets it; with tau between
+    its own error and its worst neighbour's, it is no seed but lies within h of one, and
+    the set holds it."""
+    f, dens, rho0, err = _gridded(oracle, 7)
+    q = f.pred[: f.n, :3].astype(np.float32)
+    d = q[0] - q
+    r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    near = (r < np.float32(f.c.effectRadius)) & (np.arange(f.n) > 0)
+    worst = float(err[near].max())
+    assert worst > float(err[0])
+    tau = float(np.float32((worst + float(err[0])) / 2))
+    act, seeds, want, halo = _set_and_brute_force(f, dens, rho0, err, tau)
+    assert not seeds[0] and want[0]
+    assert act[0] == 1
+    assert np.array_equal((act == 1) | (act == 2), want)
+    assert np.array_equal(act == 3, halo)
+
+
+def test_fallback_halo_takes_element_0_through_its_own_walk(oracle):
+    """A single seed 1.2-1.8 h from element 0 (densities set by hand: every other particle at
+    rest density): element 0 is outside the set but within h of one of its members, and its
+    own walk puts it in the halo; the set and halo match brute force."""
+    f, dens, rho0, _ = _gridded(oracle, 7)
+    n = f.n
+    q = f.pred[:n, :3].astype(np.float32)
+    h = np.float32(f.c.effectRadius)
+
+    def dist(i):
         d = q[i] - q
-        r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
-        want |= r < h
-    first = 0  # sorted element 0 is in no cell list (Q1): no walk meets it; left out
-    got = act > 0
-    assert np.array_equal(np.delete(got, first), np.delete(want, first))
-    assert 0 < seeds.sum() < got.sum() < n
+        return np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+
+    r0 = dist(0)
+    for s_ in np.flatnonzero((r0 >= np.float32(1.2) * h) & (r0 < np.float32(1.8) * h)):
+        members = dist(s_) < h
+        if not members[0] and (members & (r0 < h)).any():
+            break
+    else:
+        pytest.fail("no particle at 1.2-1.8 h of element 0 shares a neighbour with it")
+    synth = np.full(f.max_n, rho0, np.float32)
+    synth[s_] = np.float32(2.0) * rho0
+    err = np.abs(synth[:n] / rho0 - np.float32(1.0))
+    act, seeds, want, halo = _set_and_brute_force(f, synth, rho0, err, 0.5)
+    assert seeds.sum() == 1 and not want[0] and halo[0]
+    assert act[0] == 3
+    assert np.array_equal((act == 1) | (act == 2), want)
+    assert np.array_equal(act == 3, halo)
