@@ -387,7 +387,7 @@ struct rtp_fluids {
     if (learned && fallbackIters > 0) {
       residual();
       RTP_PROF(prof, "pbf_fallback_set", s,
-               launch_fallback_set(pred[in], G.startEnd, resDensity, tau, n, c, L, fb, s));
+               launch_fallback_set(pred[in], G.startEnd, resDensity, tau, n, c, fb, s));
       // in place in pred[in]; the partner buffer holds the set's corrections in between
       for (uint32_t f = 0; f < fallbackIters; ++f)
         RTP_PROF(prof, "pbf_fallback", s,

@@ -853,77 +853,102 @@ __global__ __launch_bounds__(256) void k_fb_mark(const float* __restrict__ resDe
   wave_append(seed, t, fs.seeds, fs.count);
 }
 
-// Every particle the walk of a seed meets within h (r = sqrt(d2) < h, d2 as the density
-// pass computes it) joins the set; the seed itself (r = 0) is already in.  Sorted element 0
-// is in no cell list (fillStartCell skips it), so no seed's walk meets it: it joins when its
-// own walk meets a seed within h (the same distance, negated difference).
-__global__ __launch_bounds__(256) void k_fb_dilate(const float4* __restrict__ pred,
-                                                   const uint2* __restrict__ se,
-                                                   const float* __restrict__ resDensity,
-                                                   uint32_t n, float tau, PbfConsts c,
-                                                   uint32_t qdepth, FallbackSet fs)
+// Cell k (0..26, x outer and z inner, periodic wrap) of the 27 that walk_cells visits around
+// position p (fluids.cl:101-123).
+__device__ __forceinline__ uint32_t nb_cell(const float4 p, uint32_t k, const GridConsts& g)
 {
-  RTP_PLAIN_CTX(qdepth)
-  const uint32_t cnt = fs.count[0];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t k = t; k < cnt; k += gridDim.x * blockDim.x) {
-    const uint32_t e = fs.seeds[k];
-    walk27(pred[e], pred, se, c, [](uint32_t) { return 0; },
-           [&](uint32_t j, int, float, float, float, float d2) {
-             if (__builtin_sqrtf(d2) < c.h)
-               fs.act[j] = 1u;
-           }, W);
-  }
-  if (t == 0 && n > 0 && cnt > 0 && fs.act[0] == 0u) {
-    bool near = false;
-    const float rho0 = c.p.restDensity;
-    walk27(pred[0], pred, se, c, [](uint32_t) { return 0; },
-           [&](uint32_t j, int, float, float, float, float d2) {
-             if (__builtin_sqrtf(d2) < c.h && fabsf(resDensity[j] / rho0 - 1.0f) > tau)
-               near = true;
-           }, W);
-    if (near)
-      fs.act[0] = 1u;
+  uint32_t cx, cy, cz;
+  cell3d(p, g, cx, cy, cz);
+  const int32_t ix = (int32_t)(k / 9u) - 1, iy = (int32_t)((k / 3u) % 3u) - 1,
+                iz = (int32_t)(k % 3u) - 1;
+  return (uint32_t)((wrap((int32_t)cx + ix, g.res[0]) * g.res[1] + wrap((int32_t)cy + iy, g.res[1])) *
+                        g.res[2] +
+                    wrap((int32_t)cz + iz, g.res[2]));
+}
+
+// The candidates of one neighbour cell of p, in index order: f(j, sqrt(d2) < h), d2 as the
+// walks compute it, ((dx dx + dy dy) + dz dz).
+template <class F>
+__device__ __forceinline__ void scan_cell_within_h(const float4 p, uint2 r,
+                                                   const float4* __restrict__ pred,
+                                                   const PbfConsts& c, F&& f)
+{
+  for (uint32_t j = r.x; j <= r.y; ++j) {
+    const float4 o = pred[j];
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    f(j, __builtin_sqrtf(d2) < c.h);
   }
 }
 
-// The particles flagged `flag` in act, appended to out and counted in *count.
-__global__ __launch_bounds__(256) void k_fb_compact(uint32_t n, uint32_t flag,
-                                                    const uint32_t* __restrict__ act,
-                                                    uint32_t* __restrict__ out,
-                                                    uint32_t* __restrict__ count)
+// Grows the set (flag 1: from the seeds) or the halo (flag 2: from the set's members).  Item
+// (k, cell) scans neighbour cell `cell` of particle src[k]; every particle it meets within h
+// whose act is 0 takes the flag (plain stores of one value: concurrent writers agree).  The
+// last 27 items are sorted element 0's own cells: element 0 is in no cell list (fillStartCell
+// skips it), so no other particle's cells meet it; it takes the flag when it meets, within h,
+// a seed (flag 1: |resDensity / rho0 - 1| > tau) or a set member (flag 2).  The same cells
+// and distances as the walks, so the same particles join.
+__global__ __launch_bounds__(256) void k_fb_grow(const float4* __restrict__ pred,
+                                                 const uint2* __restrict__ se,
+                                                 const uint32_t* __restrict__ src,
+                                                 const uint32_t* __restrict__ srcCount,
+                                                 uint32_t flag, const float* __restrict__ resDensity,
+                                                 float tau, uint32_t n, PbfConsts c, uint32_t* act)
 {
+  const uint32_t items = *srcCount * 27u;
+  const uint32_t total = n > 0 ? items + 27u : 0u;
+  const float rho0 = c.p.restDensity;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += gridDim.x * blockDim.x) {
+    if (t < items) {
+      const float4 p = pred[src[t / 27u]];
+      scan_cell_within_h(p, se[nb_cell(p, t % 27u, c.g)], pred, c, [&](uint32_t j, bool near) {
+        if (near && act[j] == 0u)
+          act[j] = flag;
+      });
+    } else {
+      const float4 p = pred[0];
+      bool hit = false;
+      scan_cell_within_h(p, se[nb_cell(p, t - items, c.g)], pred, c, [&](uint32_t j, bool near) {
+        if (near && (flag == 1u ? fabsf(resDensity[j] / rho0 - 1.0f) > tau : act[j] == 1u))
+          hit = true;
+      });
+      if (hit && act[0] == 0u)
+        act[0] = flag;
+    }
+  }
+}
+
+// The particles flagged `flag` in act, appended to out and counted in *count: per workgroup
+// of 1024 one atomic on the counter (wave counts summed in LDS), so the atomics stay few.
+__global__ __launch_bounds__(1024) void k_fb_compact(uint32_t n, uint32_t flag,
+                                                     const uint32_t* __restrict__ act,
+                                                     uint32_t* __restrict__ out,
+                                                     uint32_t* __restrict__ count)
+{
+  __shared__ uint32_t waveBase[16];
+  __shared__ uint32_t blockBase;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = t < n && act[t] == flag;
-  wave_append(on, t, out, count);
-}
-
-// The halo: every particle outside the set that the walk of a set member meets within h,
-// and sorted element 0 (met by no walk) when its own walk meets a set member within h.
-__global__ __launch_bounds__(256) void k_fb_halo(const float4* __restrict__ pred,
-                                                 const uint2* __restrict__ se, uint32_t n,
-                                                 PbfConsts c, uint32_t qdepth, FallbackSet fs)
-{
-  RTP_PLAIN_CTX(qdepth)
-  const uint32_t cnt = fs.count[1];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t k = t; k < cnt; k += gridDim.x * blockDim.x) {
-    const uint32_t e = fs.list[k];
-    walk27(pred[e], pred, se, c, [](uint32_t) { return 0; },
-           [&](uint32_t j, int, float, float, float, float d2) {
-             if (__builtin_sqrtf(d2) < c.h && fs.act[j] == 0u)
-               fs.act[j] = 2u;
-           }, W);
+  const uint64_t m = __builtin_amdgcn_ballot_w64(on);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (lane == 0u)
+    waveBase[wave] = (uint32_t)__builtin_popcountll(m);
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    uint32_t sum = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+      const uint32_t k = waveBase[w];
+      waveBase[w] = sum;
+      sum += k;
+    }
+    blockBase = sum ? atomicAdd(count, sum) : 0u;
   }
-  if (t == 0 && n > 0 && cnt > 0 && fs.act[0] == 0u) {
-    bool near = false;
-    walk27(pred[0], pred, se, c, [](uint32_t) { return 0; },
-           [&](uint32_t j, int, float, float, float, float d2) {
-             if (__builtin_sqrtf(d2) < c.h && fs.act[j] == 1u)
-               near = true;
-           }, W);
-    if (near)
-      fs.act[0] = 2u;
+  __syncthreads();
+  if (on) {
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    out[blockBase + waveBase[wave] + below] = t;
   }
 }
 
@@ -1589,23 +1614,24 @@ static uint32_t list_blocks(uint32_t n, uint32_t block)
 }
 
 void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
-                         float tau, uint32_t n, const PbfConsts& c, const Launch& L,
-                         const FallbackSet& fs, hipStream_t s)
+                         float tau, uint32_t n, const PbfConsts& c, const FallbackSet& fs,
+                         hipStream_t s)
 {
   launch_fill_u32(fs.count, 3, 0u, s);
   if (n == 0)
     return;
-  const dim3 g(list_blocks(n, L.block)), b(L.block);
   hipLaunchKernelGGL(k_fb_mark, dim3(blocks_for(n, 256)), dim3(256), 0, s, resDensity, n,
                      c.p.restDensity, tau, fs);
-  hipLaunchKernelGGL(k_fb_dilate, g, b, queue_bytes(L), s, pred, startEnd, resDensity, n, tau, c,
-                     (uint32_t)L.queue, fs);
-  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, 1u, fs.act,
-                     fs.list, fs.count + 1);
-  hipLaunchKernelGGL(k_fb_halo, g, b, queue_bytes(L), s, pred, startEnd, n, c, (uint32_t)L.queue,
-                     fs);
-  hipLaunchKernelGGL(k_fb_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, 2u, fs.act,
-                     fs.halo, fs.count + 2);
+  // one item per (particle, neighbour cell); at most 2^18 threads loop over the items
+  const uint32_t cap = (1u << 18) / 256u;
+  const uint32_t b = blocks_for(n, 256) * 27u + 1u;
+  const dim3 g(b < cap ? b : cap), cg(blocks_for(n, 1024));
+  hipLaunchKernelGGL(k_fb_grow, g, dim3(256), 0, s, pred, startEnd, fs.seeds, fs.count, 1u,
+                     resDensity, tau, n, c, fs.act);
+  hipLaunchKernelGGL(k_fb_compact, cg, dim3(1024), 0, s, n, 1u, fs.act, fs.list, fs.count + 1);
+  hipLaunchKernelGGL(k_fb_grow, g, dim3(256), 0, s, pred, startEnd, fs.list, fs.count + 1, 2u,
+                     resDensity, tau, n, c, fs.act);
+  hipLaunchKernelGGL(k_fb_compact, cg, dim3(1024), 0, s, n, 2u, fs.act, fs.halo, fs.count + 2);
 }
 
 void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,

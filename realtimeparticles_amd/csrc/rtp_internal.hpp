@@ -286,8 +286,8 @@ struct FallbackSet {
   uint32_t* count = nullptr;
 };
 void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
-                         float tau, uint32_t n, const PbfConsts& c, const Launch& L,
-                         const FallbackSet& fs, hipStream_t s);
+                         float tau, uint32_t n, const PbfConsts& c, const FallbackSet& fs,
+                         hipStream_t s);
 // One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, in place in
 // pred: the first (when the set is not empty) clamps every particle to the walls and sets
 // the lambda of every particle to 0; density and lambda are computed for the set and the
