@@ -38,6 +38,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of the MI355X: 256 CUs at 2.4 GHz, each CU's four SIMDs issuing one wave64
+# VALU instruction per four cycles apiece (one per CU per cycle), in 1e9 wave-instructions/s
+VALU_PEAK_GIPS = 256 * 2.4
 METRIC = "PBF particle-updates/s at 1M particles; learned vs plain at equal density error"
 BOIDS_STEPS = 200  # the 8,192-boid flock's run length
 CLOUDS_STEPS = 200  # the Clouds model's cumulus run
@@ -118,6 +121,39 @@ def measured_traffic(workload: str, name: str):
     if not rows:
         return None
     return sum(r["traffic_bytes"] for r in rows) / len(rows)
+
+
+def measured_counters(workload: str, name: str):
+    """The SQ counters per launch (mean over the kernel's launch shapes) of the kernel behind
+    profiler entry `name`, from the committed rocprofv3 summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    base = KERNEL_OF.get(name)
+    if base is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rows = [v["counters"] for v in json.load(f)["kernels"].values()
+                if v.get("base") == base and v.get("counters")]
+    if not rows:
+        return None
+    keys = set.intersection(*(set(r) for r in rows))
+    return {k: sum(r[k] for r in rows) / len(rows) for k in keys}
+
+
+def valu_roofline(workload: str, name: str, avg_ms: float):
+    """VALU issue rate of the dominant kernel: its SQ_INSTS_VALU per launch (rocprofv3, the
+    committed summary) over its live mean launch time, against the issue peak."""
+    ctr = measured_counters(workload, name)
+    if not ctr or "SQ_INSTS_VALU" not in ctr:
+        return None
+    insts = ctr["SQ_INSTS_VALU"]
+    achieved = insts / (avg_ms * 1e-3) / 1e9
+    out = {"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_PEAK_GIPS,
+           "unit": "1e9 wave64 VALU instructions/s", "frac": round(achieved / VALU_PEAK_GIPS, 4),
+           "valu_insts_per_launch": round(insts), "kernel": name}
+    if "SQ_THREAD_CYCLES_VALU" in ctr and ctr.get("SQ_ACTIVE_INST_VALU"):
+        out["lane_efficiency"] = round(ctr["SQ_THREAD_CYCLES_VALU"] / 64.0 /
+                                       ctr["SQ_ACTIVE_INST_VALU"], 3)
+    return out
 
 
 def run_gpu(args, sc, mapping="literal", profile=True):
@@ -639,6 +675,7 @@ def main():
     launches_per_step = dom_launches / prof_steps
     nbytes = None if per_step is None else per_step / launches_per_step
     roofline = None
+    valu = valu_roofline(args.workload, dom_name, avg_ms)
     if nbytes is not None:
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
         traffic = measured_traffic(args.workload, dom_name)
@@ -744,6 +781,7 @@ def main():
             "batched_scenes": batched,
         },
         "roofline": roofline,
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

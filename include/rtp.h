@@ -103,6 +103,10 @@ typedef struct {
                                   replays it; likewise vorticity for confinement and XSPH
                                   (256 bytes of device memory per particle; ignored with
                                   RTP_LAUNCH_LDS_TILES) */
+#define RTP_LAUNCH_BRICKS 16   /* with RTP_LAUNCH_RESORT: the thread order runs along x through
+                                  pencils of 8 x 8 cells in (y, z), pencil after pencil, instead
+                                  of in cell index order, so an XCD's consecutive waves keep
+                                  their neighbours' data in its L2 on large grids */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {

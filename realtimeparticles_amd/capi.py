@@ -26,7 +26,7 @@ FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
  FIELD_TOT_CORR_POS, FIELD_LAPLACIAN_TEMP, FIELD_LAMBDA_TEMP) = range(8, 16)
 FIELD_RESIDUAL_DENSITY, FIELD_FALLBACK_SET = 16, 17
 BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
-LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST = 1, 2, 4, 8
+LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST, LAUNCH_BRICKS = 1, 2, 4, 8, 16
 
 
 class RtpError(RuntimeError):
@@ -583,7 +583,8 @@ class Fluids(_RenderFields):
         return feat, target
 
     def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0, queue: int = 0) -> None:
-        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT | LAUNCH_NB_LIST; queue:
+        """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT | LAUNCH_NB_LIST |
+        LAUNCH_BRICKS (with LAUNCH_RESORT); queue:
         survivor-queue depth (0 = 16)."""
         t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_fluids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")

@@ -338,8 +338,9 @@ struct rtp_fluids {
       po.nb = NbList{ nbList, nbCount, n };
       pc.nb = po.nb;
     }
-    const uint32_t m16 = order_cells(c.g);
+    const uint32_t m16 = L.bricks ? brick_order_cells(c.g) : order_cells(c.g);
     pc.keyOut = resort ? okey : nullptr;
+    pc.bricks = L.bricks;
     auto order_threads = [&]() -> int {
       if (!resort)
         return RTP_OK;
@@ -790,8 +791,8 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->lamAcc, maxN)) || (rc = dev_alloc(&h->resDensity, maxN)) ||
       (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey, maxN)) ||
       (rc = dev_alloc(&h->orank, maxN)) || (rc = dev_alloc(&h->oorder, maxN)) ||
-      (rc = dev_alloc(&h->ocount, order_cells(h->grid.g))) ||
-      (rc = dev_alloc(&h->opart, scan_partials_needed(order_cells(h->grid.g))))) {
+      (rc = dev_alloc(&h->ocount, order_count_words(h->grid.g))) ||
+      (rc = dev_alloc(&h->opart, scan_partials_needed(order_count_words(h->grid.g))))) {
     h->release();
     delete h;
     return rc;
@@ -1187,6 +1188,7 @@ int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
   h->L.tile = (t->flags & RTP_LAUNCH_LDS_TILES) ? 1 : 0;
   h->L.resort = (t->flags & RTP_LAUNCH_RESORT) ? 1 : 0;
   h->L.nbList = (t->flags & RTP_LAUNCH_NB_LIST) ? 1 : 0;
+  h->L.bricks = (t->flags & RTP_LAUNCH_BRICKS) ? 1 : 0;
   h->L.queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   h->grid.sortBits = t->sortBits;
   h->drop_graph();
@@ -1198,7 +1200,8 @@ rtp_tuning current_tuning(const rtp_fluids* h)
   rtp_tuning t{};
   t.blockSize = h->L.block;
   t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0) |
-            (h->L.resort ? RTP_LAUNCH_RESORT : 0) | (h->L.nbList ? RTP_LAUNCH_NB_LIST : 0);
+            (h->L.resort ? RTP_LAUNCH_RESORT : 0) | (h->L.nbList ? RTP_LAUNCH_NB_LIST : 0) |
+            (h->L.bricks ? RTP_LAUNCH_BRICKS : 0);
   t.sortBits = h->grid.sortBits;
   t.queueDepth = h->L.queue;
   return t;

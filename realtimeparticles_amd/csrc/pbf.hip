@@ -724,7 +724,7 @@ __device__ __forceinline__ void record_cell(uint32_t i, const float4 q, const Pb
 {
   if (!po.keyOut)
     return;
-  po.keyOut[i] = po.keyBase + cell1d(q, c.g);
+  po.keyOut[i] = po.keyBase + (po.bricks ? brick_key(q, c.g) : cell1d(q, c.g));
 }
 
 template <int MODE>

@@ -56,6 +56,18 @@ __device__ __forceinline__ uint32_t cell1d(float4 p, const GridConsts& g)
   return cx * rz * ry + cy * rz + cz;
 }
 
+// The thread-order key of the cell of q under RTP_LAUNCH_BRICKS: (y, z) tile of kBrick x kBrick
+// cells (cell coordinates 0..res), then x, then the cell's place in its tile, so that the
+// order runs along x through one pencil of tiles after another.
+__device__ __forceinline__ uint32_t brick_key(float4 q, const GridConsts& g)
+{
+  uint32_t cx, cy, cz;
+  cell3d(q, g, cx, cy, cz);
+  const uint32_t nTz = ((uint32_t)g.res[2] + kBrick) / kBrick;
+  const uint32_t tile = (cy / kBrick) * nTz + cz / kBrick;
+  return ((tile * ((uint32_t)g.res[0] + 1u) + cx) * kBrick + cy % kBrick) * kBrick + cz % kBrick;
+}
+
 // (c + o + res) % res for c in [0, res] and o in {-1, 0, 1} (fluids.cl:107).
 __device__ __forceinline__ int32_t wrap(int32_t v, int32_t res)
 {

@@ -79,6 +79,24 @@ uint32_t order_cells(const GridConsts& g)
   return (max_active_key(g) + 1u + 15u) & ~15u;
 }
 
+// Count-table entries for either key kind.
+uint32_t order_count_words(const GridConsts& g);
+
+// The same for the brick keys of RTP_LAUNCH_BRICKS (brick_key: cell coordinates up to res
+// in each dimension, (y, z) tiles of kBrick x kBrick).
+uint32_t brick_order_cells(const GridConsts& g)
+{
+  const uint64_t nTy = ((uint64_t)g.res[1] + kBrick) / kBrick;
+  const uint64_t nTz = ((uint64_t)g.res[2] + kBrick) / kBrick;
+  const uint64_t m = nTy * nTz * ((uint64_t)g.res[0] + 1u) * kBrick * kBrick;
+  return (uint32_t)std::min<uint64_t>((m + 15u) & ~(uint64_t)15u, 0xfffffff0u);
+}
+
+uint32_t order_count_words(const GridConsts& g)
+{
+  return std::max(order_cells(g), brick_order_cells(g));
+}
+
 // Smallest float >= v, largest float <= v.
 float float_above(double v)
 {
@@ -256,8 +274,8 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
 // ----------------------------------------------------------------------------- tuning
 
 // Launch flags a single model / a batch takes.
-constexpr int kFluidsLaunchFlags =
-    RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
+constexpr int kFluidsLaunchFlags = RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT |
+                                   RTP_LAUNCH_NB_LIST | RTP_LAUNCH_BRICKS;
 constexpr int kBatchLaunchFlags = RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
 
 // Validates a launch tuning; allowedFlags masks the RTP_LAUNCH_* bits the caller supports.
@@ -266,6 +284,8 @@ int check_tuning(const rtp_tuning& t, int allowedFlags)
   RTP_REQUIRE(t.blockSize >= 64 && t.blockSize <= 256 && t.blockSize % 64 == 0,
               "blockSize must be 64, 128, 192 or 256");
   RTP_REQUIRE((t.flags & ~allowedFlags) == 0, "launch flags not supported here");
+  RTP_REQUIRE(!(t.flags & RTP_LAUNCH_BRICKS) || (t.flags & RTP_LAUNCH_RESORT),
+              "RTP_LAUNCH_BRICKS orders the threads of RTP_LAUNCH_RESORT: set both");
   RTP_REQUIRE(t.sortBits == 0 || (t.sortBits >= 4 && t.sortBits <= 11), "sortBits must be 0 or 4..11");
   const int q = t.queueDepth ? t.queueDepth : kDefaultQueue;
   RTP_REQUIRE(q >= 8 && q <= 64, "queueDepth must be 0 or 8..64");

@@ -95,7 +95,11 @@ struct Launch {
                        // the cells their particles now sit in
   int nbList = 0;      // 1: the second pass of each pair over the same positions replays the
                        // first one's survivor lists
+  int bricks = 0;      // 1 (with resort): thread order by brick_key instead of the cell index
 };
+
+// Tile edge, in cells, of the (y, z) pencils of the brick thread order.
+constexpr uint32_t kBrick = 8;
 
 // ---------------------------------------------------------------------------- grid / sort
 // Stable LSD radix sort of (key, index) pairs on the device.
@@ -207,6 +211,7 @@ struct PassOrder {
   const uint32_t* order = nullptr;  // thread t handles particle order[t]; identity when null
   uint32_t* keyOut = nullptr;       // correction passes: cell key of each output position
   uint32_t keyBase = 0;             //   plus this (a batched scene's key base)
+  int bricks = 0;                   //   1: brick_key of the cell instead of its index
   NbList nb;
   __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
 };

@@ -1,5 +1,5 @@
 """Summarise rocprofv3 CSV output of bench.py runs into a short markdown table and a JSON
-file of HBM traffic per kernel launch.
+file of HBM traffic (and the SQ counters, when given) per kernel launch.
 
 Inputs are the output directories of up to four runs of the same command:
 
@@ -145,7 +145,7 @@ def main(argv=None):
         for k, d in acc.items():
             sq[k].update({c: d[c] / cnt[k][c] for c in d})
 
-    names = sorted(set(stats) | set(fetch) | set(write),
+    names = sorted(set(stats) | set(fetch) | set(write) | set(sq),
                    key=lambda k: -stats.get(k, {}).get("total_ns", 0.0))
     traffic = {}
     lines = [f"# {a.title}", "", "Commands (run from the repository root on one MI355X; "
@@ -163,10 +163,12 @@ def main(argv=None):
         fb = (sum(fv) / len(fv) * 1024.0 / ff) if fv and ff else None
         wb = (sum(wv) / len(wv) * 1024.0 / fw) if wv and fw else None
         m = meta.get(k, {})
-        if fb is not None or wb is not None:
+        if fb is not None or wb is not None or k in sq:
             traffic[k] = {"fetch_bytes": fb, "write_bytes": wb,
                           "traffic_bytes": (fb or 0.0) + (wb or 0.0), "base": base_name(k),
                           "avg_ns": st.get("avg_ns")}
+            if k in sq:
+                traffic[k]["counters"] = dict(sq[k])
         lines.append("| {} | {} | {} | {} | {} | {} | {} | {} | {} |".format(
             k, st.get("calls", ""), "" if not st else f"{st['avg_ns'] / 1e3:.2f}",
             "" if not st else f"{st['total_ns'] / 1e6:.3f}", "" if not st else f"{st['percent']:.1f}",

@@ -73,7 +73,8 @@ def test_pbf_4m_replay_and_graph_determinism(scene_4m, mapping, substeps):
     gpu.set_params(**sc.params(mapping))
     gpu.load_state(sc.pos, sc.vel)
     gpu.save_checkpoint()
-    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP | capi.LAUNCH_NB_LIST, 0, 16)
+    gpu.set_tuning(64, capi.LAUNCH_RESORT | capi.LAUNCH_XCD_MAP | capi.LAUNCH_NB_LIST
+                   | capi.LAUNCH_BRICKS, 0, 16)
     log(f"pbf-4m ({mapping}): {substeps} substeps by graph replay, tuned launches")
     gpu.update(substeps)
     a_pos, a_vel = gpu.read_state()

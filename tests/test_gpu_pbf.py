@@ -65,9 +65,10 @@ def test_pbf_parameter_variants(variant):
     gpu.close()
 
 
-@pytest.mark.parametrize("flags", [0, 13])
+@pytest.mark.parametrize("flags", [0, 13, 29])
 def test_graph_replay_equals_direct_launches(flags):
-    """flags 13: XCD blocks, thread order and survivor lists inside the captured graphs."""
+    """flags 13: XCD blocks, thread order and survivor lists inside the captured graphs; 29:
+    the same with the brick thread order."""
     sc = small_scene(seed=4)
     a, _ = make_pbf_pair(sc, iters=4)
     b, _ = make_pbf_pair(sc, iters=4)
@@ -92,7 +93,7 @@ def test_graph_replay_equals_direct_launches(flags):
     b.close()
 
 
-@pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11)])
+@pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11), (29, 9)])
 def test_nb_particles_change_keeps_reference_sort(flags, sort_bits):
     # Model::setNbParticles (Model.hpp:106) does not reset the grid: the keys of particles
     # that left the active range stay in the reference's whole-buffer sort.  flags 13: the
@@ -192,11 +193,12 @@ def test_checkpoint_restore_replays_identically():
     (256, 2, 16, 0), (128, 2, 24, 7), (64, 3, 48, 0), (192, 1, 64, 10), (256, 3, 16, 11),
     (256, 0, 64, 0), (128, 0, 8, 8), (64, 4, 64, 0), (128, 6, 32, 9), (256, 7, 16, 0),
     (192, 5, 40, 4), (64, 13, 16, 10), (256, 8, 8, 0), (128, 12, 24, 11), (64, 15, 12, 7),
-    (64, 13, 16, 5), (128, 13, 16, 6)])
+    (64, 13, 16, 5), (128, 13, 16, 6), (64, 29, 16, 10), (256, 20, 24, 0), (128, 31, 12, 8)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
 def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mapping):
-    """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell, the
-    survivor-queue depth, replayed survivor lists and the sort's digit width change where
+    """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell (in
+    cell-index or brick order), the survivor-queue depth, replayed survivor lists and the
+    sort's digit width change where
     candidates are read from, which thread handles a particle and when the neighbour bodies
     run -- never which candidates or in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
@@ -213,7 +215,7 @@ def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mappin
 
 
 @pytest.mark.parametrize("block,queue,flags", [(256, 8, 0), (64, 64, 0), (128, 48, 0),
-                                               (64, 16, 13), (256, 12, 8)])
+                                               (64, 16, 13), (256, 12, 8), (64, 16, 29)])
 def test_overfull_cell_keeps_bits(block, queue, flags):
     """20,000 particles in one cell (the cap iterates 101 of them) and 50 in the next cell
     of its column: the walks through them are no longer back-to-back ranges, survivor
@@ -364,7 +366,7 @@ def test_general_forms_outside_short_range(box):
     gpu.close()
 
 
-@pytest.mark.parametrize("flags", [0, 13])
+@pytest.mark.parametrize("flags", [0, 13, 29])
 def test_bad_state_matches_oracle(flags):
     """NaN and +-inf coordinates, a NaN velocity, particles outside the walls and exactly on
     +-W in a dam break: the hash aliases and clamps them (grid.cl:14-38), the first clamp
