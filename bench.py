@@ -110,8 +110,9 @@ def copy_bandwidth():
 
 
 def measured_traffic(workload: str, name: str):
-    """HBM bytes per launch of the kernel behind profiler entry `name`, from the committed
-    rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload (profiles/), or None."""
+    """HBM bytes per launch of the kernel behind profiler entry `name` (its most launched
+    shape), from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload
+    (profiles/), or None."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     base = KERNEL_OF.get(name)
     if base is None or not os.path.exists(path):
@@ -120,23 +121,24 @@ def measured_traffic(workload: str, name: str):
         rows = [v for v in json.load(f)["kernels"].values() if v.get("base") == base]
     if not rows:
         return None
-    return sum(r["traffic_bytes"] for r in rows) / len(rows)
+    # the launch shape of most launches: the profiled command's tuned shape
+    return max(rows, key=lambda r: r.get("calls") or 0)["traffic_bytes"]
 
 
 def measured_counters(workload: str, name: str):
-    """The SQ counters per launch (mean over the kernel's launch shapes) of the kernel behind
+    """The SQ counters per launch (of its most launched shape) of the kernel behind
     profiler entry `name`, from the committed rocprofv3 summary of this workload, or None."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     base = KERNEL_OF.get(name)
     if base is None or not os.path.exists(path):
         return None
     with open(path) as f:
-        rows = [v["counters"] for v in json.load(f)["kernels"].values()
+        rows = [v for v in json.load(f)["kernels"].values()
                 if v.get("base") == base and v.get("counters")]
     if not rows:
         return None
-    keys = set.intersection(*(set(r) for r in rows))
-    return {k: sum(r[k] for r in rows) / len(rows) for k in keys}
+    # the launch shape of most launches: the profiled command's tuned shape
+    return max(rows, key=lambda r: r.get("calls") or 0)["counters"]
 
 
 def valu_roofline(workload: str, name: str, avg_ms: float):
@@ -643,6 +645,9 @@ def main():
     ap.add_argument("--no-boids", action="store_true", help="skip the boids steps")
     ap.add_argument("--no-65k", action="store_true", help="skip the 65,536-particle 300-frame run")
     ap.add_argument("--no-clouds", action="store_true", help="skip the Clouds model's cumulus run")
+    ap.add_argument("--no-copy-probe", action="store_true",
+                    help="skip the device copy-bandwidth probe (a child process); for runs under "
+                         "rocprofv3, which initialises the GPU runtime before the fork")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the substep with the reference's rendering steps on")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -661,7 +666,9 @@ def main():
     from realtimeparticles_amd import scenes
     sc = scenes.pbf_scene(args.workload)
     nc = sc.res[0] * sc.res[1] * sc.res[2]
-    copy_gbs = copy_bandwidth()  # before this process touches the GPU
+    # before this process touches the GPU; not under a profiler, whose preloaded library
+    # initialises the HSA runtime at start, which a fork must not follow
+    copy_gbs = None if args.no_copy_probe else copy_bandwidth()
 
     g = run_gpu(args, sc)
     elapsed, prof, prof_steps = g["elapsed"], g["prof"], g["prof_steps"]

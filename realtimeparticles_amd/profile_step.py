@@ -1,0 +1,70 @@
+"""The fixed-input PBF substep of bench.py, for rocprofv3: the same scene, coefficients and
+launch shape, timed the same way, without the bench's other sections.
+
+One untimed substep at the default launch shape comes first; then the given shape runs
+`--warmup` untimed and `--steps` timed fixed-input substeps (restore the input, one
+substep), and a third window of the same substeps with per-kernel HIP events on the model's
+stream.  Prints one JSON line: ms per substep and each profiler entry's launches and mean
+launch time, to set beside the rocprofv3 trace of the same command.
+
+    rocprofv3 --kernel-trace --stats -d OUT -o run -- \\
+        python3 -m realtimeparticles_amd.profile_step --workload pbf-1m --block 64 --flags 13 \\
+        --queue 16 --sort-bits 10
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+from . import capi, scenes
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="pbf-1m")
+    ap.add_argument("--mapping", default="literal", choices=["literal", "stable"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--flags", type=int, default=13)
+    ap.add_argument("--queue", type=int, default=16)
+    ap.add_argument("--sort-bits", type=int, default=10)
+    a = ap.parse_args(argv)
+
+    sc = scenes.pbf_scene(a.workload)
+    m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(**sc.params(a.mapping))
+    m.set_nb_jacobi_iters(sc.iters)
+    m.load_state(sc.pos, sc.vel)
+    m.save_checkpoint()
+    m.update(1)  # the default launch shape
+    m.set_tuning(a.block, a.flags, a.sort_bits, a.queue)
+
+    def fixed_steps(k):
+        for _ in range(k):
+            m.restore_checkpoint()
+            m.update(1)
+
+    fixed_steps(a.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    fixed_steps(a.steps)
+    m.sync()
+    elapsed = time.perf_counter() - t0
+    m.enable_profiling(True)
+    m.reset_profile()
+    fixed_steps(a.steps)
+    m.sync()
+    prof = m.profile()
+    m.close()
+    print(json.dumps({
+        "workload": a.workload, "mapping": a.mapping, "particles": sc.n,
+        "tuning": {"block": a.block, "flags": a.flags, "queue": a.queue, "sort_bits": a.sort_bits},
+        "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
+        "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
+                    for k, v in sorted(prof.items())}}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

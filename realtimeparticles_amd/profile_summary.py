@@ -1,7 +1,8 @@
 """Summarise rocprofv3 CSV output of bench.py runs into a short markdown table and a JSON
 file of HBM traffic (and the SQ counters, when given) per kernel launch.
 
-Inputs are the output directories of up to four runs of the same command:
+Inputs are the output directories of up to four runs of the same command, each in CSV
+(--output-format csv) or rocprofv3's default rocpd database:
 
   --trace DIR   rocprofv3 --kernel-trace --stats           (run_kernel_stats.csv)
   --fetch DIR   rocprofv3 --pmc FETCH_SIZE                 (run_counter_collection.csv)
@@ -67,8 +68,45 @@ def find(dirpath: str, suffix: str) -> str:
     raise FileNotFoundError(f"no *{suffix} under {dirpath}")
 
 
+def find_db(dirpath: str):
+    """The rocpd database (rocprofv3's default output) under dirpath, or None."""
+    for root, _, files in os.walk(dirpath):
+        for fn in files:
+            if fn.endswith("_results.db"):
+                return os.path.join(root, fn)
+    return None
+
+
+def _db_rows(path: str, sql: str):
+    """Rows of `sql` over the library's own process: a profiled command may start child
+    processes (the bench's copy-bandwidth probe), whose dispatches are left out."""
+    import sqlite3
+    con = sqlite3.connect(path)
+    try:
+        cur = con.cursor()
+        top = cur.execute("SELECT pid, COUNT(*) FROM kernels WHERE name LIKE '%rtp::%' "
+                          "GROUP BY pid ORDER BY 2 DESC").fetchone()
+        if top is None:
+            return []
+        return cur.execute(sql, (top[0],)).fetchall()
+    finally:
+        con.close()
+
+
 def kernel_stats(trace_dir: str):
-    rows = read_csv(find(trace_dir, "kernel_stats.csv"))
+    db = None
+    try:
+        rows = read_csv(find(trace_dir, "kernel_stats.csv"))
+    except FileNotFoundError:
+        db = find_db(trace_dir)
+        if db is None:
+            raise
+        res = _db_rows(db, "SELECT name, COUNT(*), SUM(end - start), AVG(end - start), "
+                           "MIN(end - start), MAX(end - start) FROM kernels WHERE pid = ? "
+                           "GROUP BY name")
+        total = sum(r[2] for r in res) or 1.0
+        rows = [{"Name": r[0], "Calls": r[1], "TotalDurationNs": r[2], "AverageNs": r[3],
+                 "MinNs": r[4], "MaxNs": r[5], "Percentage": 100.0 * r[2] / total} for r in res]
     out = {}
     for r in rows:
         out[short_name(r["Name"])] = {"calls": int(r["Calls"]), "total_ns": float(r["TotalDurationNs"]),
@@ -77,9 +115,23 @@ def kernel_stats(trace_dir: str):
     return out
 
 
+def counter_rows(pmc_dir: str):
+    """Per-dispatch counter rows (CSV field names) of a --pmc run, CSV or rocpd output."""
+    try:
+        return read_csv(find(pmc_dir, "counter_collection.csv"))
+    except FileNotFoundError:
+        db = find_db(pmc_dir)
+        if db is None:
+            raise
+    res = _db_rows(db, "SELECT kernel_name, counter_name, value, vgpr_count, sgpr_count, "
+                       "lds_block_size, workgroup_size FROM counters_collection WHERE pid = ?")
+    return [{"Kernel_Name": r[0], "Counter_Name": r[1], "Counter_Value": r[2], "VGPR_Count": r[3],
+             "SGPR_Count": r[4], "LDS_Block_Size": r[5], "Workgroup_Size": r[6]} for r in res]
+
+
 def counters(pmc_dir: str, counter: str):
     """{short kernel name: [value per dispatch]} and the dispatch metadata."""
-    rows = read_csv(find(pmc_dir, "counter_collection.csv"))
+    rows = counter_rows(pmc_dir)
     vals = defaultdict(list)
     meta = {}
     for r in rows:
@@ -135,7 +187,7 @@ def main(argv=None):
     fw = copy_factor(write, a.max_n) if write else None
     sq = defaultdict(dict)
     for sqdir in a.sq:
-        rows = read_csv(find(sqdir, "counter_collection.csv"))
+        rows = counter_rows(sqdir)
         acc = defaultdict(lambda: defaultdict(float))
         cnt = defaultdict(lambda: defaultdict(int))
         for r in rows:
@@ -166,7 +218,7 @@ def main(argv=None):
         if fb is not None or wb is not None or k in sq:
             traffic[k] = {"fetch_bytes": fb, "write_bytes": wb,
                           "traffic_bytes": (fb or 0.0) + (wb or 0.0), "base": base_name(k),
-                          "avg_ns": st.get("avg_ns")}
+                          "avg_ns": st.get("avg_ns"), "calls": st.get("calls")}
             if k in sq:
                 traffic[k]["counters"] = dict(sq[k])
         lines.append("| {} | {} | {} | {} | {} | {} | {} | {} | {} |".format(
