@@ -297,8 +297,9 @@ def run_boids(args):
     --steps: the 8,192-boid flock (box 2, grid 20^3, dt 0.01) for its 200 steps, and the
     reference app's largest case, 131,072 boids in a 10^3 box on a 30^3 grid with the
     reference's default rules (Boids.cpp:44-69) and dt 0.1 (Boids.cpp:334), also for 200
-    steps.  Every launch shape (block size x XCD map) runs the same steps from the same
-    input; the fastest is reported beside the default one."""
+    steps.  Every launch shape (block size x XCD map, and for the flock the persistent
+    update, all 200 steps in one cooperative launch) runs the same steps from the same
+    input; the fastest is reported beside the default one and the best of each kind."""
     from realtimeparticles_amd import capi, scenes
     steps = BOIDS_STEPS
     cases = (("flock-8k", scenes.boids_flock(8192), None),
@@ -324,9 +325,15 @@ def run_boids(args):
             return (time.perf_counter() - t0) * 1e3 / steps
 
         timed(256, 0)  # first launches of the kernels
-        shapes = {(b, f): timed(b, f) for b in (64, 128, 192, 256)
-                  for f in (0, capi.LAUNCH_XCD_MAP)}
+        # the persistent update (every step in one cooperative launch) applies while the
+        # sort fits one workgroup's LDS
+        flag_sets = (0, capi.LAUNCH_XCD_MAP) + ((capi.LAUNCH_PERSISTENT,) if sc.n <= 8192 else ())
+        if sc.n <= 8192:
+            timed(256, capi.LAUNCH_PERSISTENT)  # its first launch
+        shapes = {(b, f): timed(b, f) for b in (64, 128, 192, 256) for f in flag_sets}
         (bb, bf), best_ms = min(shapes.items(), key=lambda kv: kv[1])
+        separate = min(v for (b, f), v in shapes.items() if not f & capi.LAUNCH_PERSISTENT)
+        persistent = [v for (b, f), v in shapes.items() if f & capi.LAUNCH_PERSISTENT]
         # per-kernel times of the same steps at the fastest shape
         m.set_tuning(bb, bf)
         m.load_state(sc.pos, sc.vel)
@@ -341,6 +348,8 @@ def run_boids(args):
                      "ms_per_step": round(best_ms, 5),
                      "launch": {"block": bb, "flags": bf},
                      "ms_per_step_default_launch": round(shapes[(256, 0)], 5),
+                     "ms_per_step_separate_launches": round(separate, 5),
+                     "ms_per_step_persistent": round(min(persistent), 5) if persistent else None,
                      "kernel_ms_per_step": {kk: round(v[1] / steps, 5)
                                             for kk, v in sorted(prof.items())}}
     return out

@@ -103,6 +103,10 @@ typedef struct {
                                   replays it; likewise vorticity for confinement and XSPH
                                   (256 bytes of device memory per particle; ignored with
                                   RTP_LAUNCH_LDS_TILES) */
+#define RTP_LAUNCH_PERSISTENT 32 /* boids: each update runs its steps in one cooperative launch
+                                    (hash, sort, reorder and rules as phases between grid
+                                    barriers; results unchanged) when the sort fits one
+                                    workgroup's LDS and the rendering steps are off */
 #define RTP_LAUNCH_BRICKS 16   /* with RTP_LAUNCH_RESORT: the thread order runs along x through
                                   pencils of 8 x 8 cells in (y, z), pencil after pencil, instead
                                   of in cell index order, so an XCD's consecutive waves keep

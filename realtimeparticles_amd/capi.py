@@ -27,6 +27,7 @@ FIELD_COLOR, FIELD_CAMERA_DIST, FIELD_DETECTOR = 5, 6, 7
 FIELD_RESIDUAL_DENSITY, FIELD_FALLBACK_SET = 16, 17
 BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST, LAUNCH_BRICKS = 1, 2, 4, 8, 16
+LAUNCH_PERSISTENT = 32  # boids: every step of an update in one cooperative launch
 
 
 class RtpError(RuntimeError):
@@ -824,7 +825,7 @@ class Boids(_RenderFields):
         self.n = int(n)
 
     def set_tuning(self, block: int = 256, flags: int = 0) -> None:
-        """Block size of the boids step; flags: LAUNCH_XCD_MAP or 0."""
+        """Block size of the boids step; flags: LAUNCH_XCD_MAP | LAUNCH_PERSISTENT, or 0."""
         t = _tuning(block, flags, 0, 0)
         _check(lib().rtp_boids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 

@@ -9,6 +9,7 @@
 // already-updated state.
 #include "rtp_crmath.hpp"
 #include "rtp_device.hpp"
+#include "rtp_grid_device.hpp"
 
 namespace rtp {
 
@@ -53,19 +54,17 @@ __global__ __launch_bounds__(256) void k_boids_unit_vel(const float4* __restrict
   unitVel[i] = make_float4(u.x, u.y, u.z, 0.0f);
 }
 
+// Boid i of the sorted state: the rules over its neighbours, the velocity update and the
+// wall (or periodic) boundary, written to the other pair of buffers.
 template <int DIM2>
-__global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
-                                                    const float4* __restrict__ velocity,
-                                                    const float4* __restrict__ unitVel,
-                                                    const uint2* __restrict__ se,
-                                                    float4* __restrict__ posOut,
-                                                    float4* __restrict__ velOut,
-                                                    float4* __restrict__ accOut, uint32_t n,
-                                                    BoidsConsts c, int remap)
+__device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __restrict__ position,
+                                                const float4* __restrict__ velocity,
+                                                const float4* __restrict__ unitVel,
+                                                const uint2* __restrict__ se,
+                                                float4* __restrict__ posOut,
+                                                float4* __restrict__ velOut,
+                                                float4* __restrict__ accOut, const BoidsConsts& c)
 {
-  const uint32_t i = logical_block(remap) * blockDim.x + threadIdx.x;
-  if (i >= n)
-    return;
   const float4 p = position[i];
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
@@ -193,6 +192,128 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
   velOut[i] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
 
+template <int DIM2>
+__global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
+                                                    const float4* __restrict__ velocity,
+                                                    const float4* __restrict__ unitVel,
+                                                    const uint2* __restrict__ se,
+                                                    float4* __restrict__ posOut,
+                                                    float4* __restrict__ velOut,
+                                                    float4* __restrict__ accOut, uint32_t n,
+                                                    BoidsConsts c, int remap)
+{
+  const uint32_t i = logical_block(remap) * blockDim.x + threadIdx.x;
+  if (i < n)
+    boids_step_body<DIM2>(i, position, velocity, unitVel, se, posOut, velOut, accOut, c);
+}
+
+// ----------------------------------------------------------------- persistent update
+// Every step of an update in one cooperative launch: the five launches of a step (hash,
+// sort, reorder, unit velocities, rules) become three phases of one loop separated by
+// grid-wide barriers, with the same per-element bodies, so the results are those of the
+// separate launches bit for bit.  Workgroup 0 runs the one-workgroup LDS sort; the element phases
+// hand wave w of workgroup b the 64 elements from 64 * (b + G * w), so the waves of a phase
+// spread over the workgroups.
+
+// Grid-wide barrier: each workgroup's first thread adds to bar[0]; the last to arrive resets
+// it and advances the generation bar[1], which the others poll (agent-scope release and
+// acquire, so the writes of every workgroup before the barrier are seen by every workgroup
+// after it).  The poll is bounded: after kBarrierPolls a workgroup sets bar[2] and goes on,
+// and every later barrier returns at once, so a launch always ends (the host reports it).
+constexpr uint32_t kBarrierPolls = 1u << 24;
+
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t& gen)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t arrived =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived + 1u == gridDim.x) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (uint32_t k = 0;; ++k) {
+        if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen)
+          break;
+        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+          break;
+        if (k == kBarrierPolls) {
+          __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  ++gen;
+  __syncthreads();
+}
+
+// Phases of one step, three barriers apart: (1) the cell table reset and, in workgroup 0,
+// the sort; (2) the gather with the cell table, and each boid's unit velocity from the
+// velocity its thread just gathered; (3) the rules, the velocity update and the wall, the
+// sorted tail carried over, and each boid's cell key for the next step from the position
+// its thread just wrote (not after the last step: the keys then stay the sorted ones, as
+// after separate launches).  The first step's keys are hashed before the loop.
+template <int DIM2>
+__global__ __launch_bounds__(512) void k_boids_persistent(BoidsPersistentArgs a, BoidsConsts c)
+{
+  __shared__ SmallSortLds lds;
+  uint32_t gen = 0;
+  const uint32_t t0 = (threadIdx.x & 63u) + 64u * (blockIdx.x + gridDim.x * (threadIdx.x >> 6));
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t n = a.n, sn = a.sortN, nc = c.g.numCells;
+  int kc = a.keyCur;
+  const int res = a.passes & 1;
+  float4* pos = a.pos[a.cur];
+  float4* vel = a.vel[a.cur];
+  float4* pos2 = a.pos[1 - a.cur];
+  float4* vel2 = a.vel[1 - a.cur];
+  for (uint32_t t = t0; t < n; t += stride)  // k_boids_hash of the first step
+    a.key[kc][t] = cell1d(pos[t], c.g);
+  grid_barrier(a.bar, gen);
+  for (uint32_t step = 0; step < a.steps; ++step) {
+    for (uint32_t t = t0; t < nc; t += stride)  // resetStartEndCell (k_boids_hash)
+      a.startEnd[t] = make_uint2(1u, 0u);
+    if (blockIdx.x == 0)  // k_sort_small: the sorted keys land in key[kc ^ res]
+      small_sort_block<16>(a.key[kc], a.key[1 - kc], a.idx[0], a.idx[1], sn, a.passes, lds);
+    kc = res ? 1 - kc : kc;
+    grid_barrier(a.bar, gen);
+    ReorderArgs r{};
+    r.key = a.key[kc];
+    r.idx = a.idx[res];
+    r.sortN = sn;
+    r.n = n;
+    r.maxN = a.maxN;
+    r.srcA = pos;
+    r.dstA = pos2;
+    r.srcB = vel;
+    r.dstB = vel2;
+    r.startEnd = a.startEnd;
+    r.g = c.g;
+    for (uint32_t t = t0; t < sn; t += stride) {  // k_reorder, k_boids_unit_vel
+      reorder_body(t, r, 0u, 0u);
+      const float4 v = vel2[t];
+      const V3 u = normalize3(V3{ v.x, v.y, v.z });
+      a.unitVel[t] = make_float4(u.x, u.y, u.z, 0.0f);
+    }
+    grid_barrier(a.bar, gen);
+    const bool more = step + 1u < a.steps;
+    for (uint32_t t = t0; t < sn; t += stride) {  // k_boids_step, the tail, the next hash
+      if (t < n) {
+        boids_step_body<DIM2>(t, pos2, vel2, a.unitVel, a.startEnd, pos, vel, a.acc, c);
+        if (more)
+          a.key[kc][t] = cell1d(pos[t], c.g);
+      } else {
+        pos[t] = pos2[t];
+        vel[t] = vel2[t];
+      }
+    }
+    grid_barrier(a.bar, gen);
+  }
+}
+
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
 }  // namespace
@@ -224,6 +345,31 @@ void launch_boids_step(const float4* pos, const float4* vel, float4* unitVel,
   else
     hipLaunchKernelGGL(k_boids_step<0>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
                        vel, unitVel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+}
+
+int launch_boids_persistent(const BoidsPersistentArgs& a, const BoidsConsts& c, uint32_t groups,
+                            hipStream_t s)
+{
+  const void* fn = c.dim2 ? reinterpret_cast<const void*>(&k_boids_persistent<1>)
+                          : reinterpret_cast<const void*>(&k_boids_persistent<0>);
+  BoidsPersistentArgs args = a;
+  BoidsConsts consts = c;
+  void* params[] = { &args, &consts };
+  if (hipLaunchCooperativeKernel(fn, dim3(groups), dim3(kSmallThreads), params, 0, s) != hipSuccess)
+    return fail(RTP_ERR_HIP, "cooperative launch of the persistent boids update failed");
+  return RTP_OK;
+}
+
+uint32_t boids_persistent_max_groups(int dim2)
+{
+  const void* fn = dim2 ? reinterpret_cast<const void*>(&k_boids_persistent<1>)
+                        : reinterpret_cast<const void*>(&k_boids_persistent<0>);
+  int perCu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, fn, (int)kSmallThreads, 0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return (uint32_t)(perCu > 0 && cus > 0 ? perCu * cus : 0);
 }
 
 }  // namespace rtp

@@ -599,6 +599,83 @@ struct rtp_boids {
   Profiler prof;
   RenderState render;  // rtp_boids_set_render
   bool targetVisible = false;  // the "Show" switch of the parameter tree (drawing only)
+  bool persistent = false;     // RTP_LAUNCH_PERSISTENT
+  uint32_t* bar = nullptr;      // the persistent launch's barrier words
+  uint32_t* barHost = nullptr;  // pinned copy of its timeout flag, read at the next sync
+  bool barPending = false;
+
+  // The steps of one update in one cooperative launch, when the configuration allows it
+  // (returns RTP_OK with *done = false otherwise).
+  int update_persistent(uint32_t nsteps, bool* done)
+  {
+    *done = false;
+    GridState& G = grid;
+    const uint32_t sn = G.sort_range(n);
+    const SortPlan plan = make_sort_plan(sn, G.max_key(n), G.sortBits);
+    const int passes = small_sort_pass_count(plan);
+    if (render.on || n == 0 || passes == 0)
+      return RTP_OK;
+    const uint32_t most = boids_persistent_max_groups(c.dim2);
+    uint32_t groups = (sn + (uint32_t)L.block - 1u) / (uint32_t)L.block;
+    groups = std::min(std::max(groups, 1u), most);
+    if (groups == 0)
+      return RTP_OK;
+    int rc = 0;
+    if (!bar) {
+      if ((rc = dev_alloc(&bar, 4)))
+        return rc;
+      if (hipHostMalloc(reinterpret_cast<void**>(&barHost), sizeof(uint32_t)) != hipSuccess) {
+        barHost = nullptr;
+        return fail(RTP_ERR_HIP, "hipHostMalloc failed");
+      }
+      *barHost = 0u;
+    }
+    launch_fill_u32(bar, 4, 0u, stream);
+    BoidsPersistentArgs a{};
+    for (int k = 0; k < 2; ++k) {
+      a.pos[k] = pos[k];
+      a.vel[k] = vel[k];
+      a.key[k] = G.key[k];
+      a.idx[k] = G.idx[k];
+    }
+    a.unitVel = unitVel;
+    a.acc = acc;
+    a.startEnd = G.startEnd;
+    a.bar = bar;
+    a.n = n;
+    a.sortN = sn;
+    a.maxN = G.maxN;
+    a.steps = nsteps;
+    a.cur = cur;
+    a.keyCur = G.keyCur;
+    a.passes = passes;
+    RTP_PROF(prof, "boids_persistent", stream, rc = launch_boids_persistent(a, c, groups, stream));
+    if (rc)
+      return rc;
+    RTP_HIP_TRY(hipMemcpyAsync(barHost, bar + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    barPending = true;
+    if ((passes & 1) && (nsteps & 1u))
+      G.keyCur = 1 - G.keyCur;
+    G.idxCur = passes & 1;
+    G.sortedN = sn;
+    G.permN = sn;
+    *done = true;
+    return RTP_OK;
+  }
+
+  // After a stream synchronisation: whether a persistent launch since the last check ended
+  // on a timed-out grid barrier.
+  int check_persistent()
+  {
+    if (!barPending)
+      return RTP_OK;
+    barPending = false;
+    if (barHost && *barHost != 0u) {
+      *barHost = 0u;
+      return fail(RTP_ERR_HIP, "persistent boids update: a grid barrier timed out (results invalid)");
+    }
+    return RTP_OK;
+  }
 
   int step()
   {
@@ -694,6 +771,10 @@ struct rtp_boids {
     dev_free(acc);
     dev_free(accAlt);
     dev_free(unitVel);
+    dev_free(bar);
+    if (barHost)
+      (void)hipHostFree(barHost);
+    barHost = nullptr;
     render.release();
     if (stream)
       (void)hipStreamDestroy(stream);
@@ -1364,7 +1445,7 @@ int rtp_boids_read_state(rtp_boids* h, float* pos4, float* vel4, uint32_t count)
   if (vel4)
     RTP_HIP_TRY(hipMemcpyAsync(vel4, h->vel[h->cur], sizeof(float4) * count, hipMemcpyDeviceToHost, h->stream));
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
-  return RTP_OK;
+  return h->check_persistent();
 }
 
 int rtp_boids_update(rtp_boids* h, uint32_t nsteps)
@@ -1385,7 +1466,13 @@ int rtp_boids_update(rtp_boids* h, uint32_t nsteps)
     return RTP_OK;
   }
   RTP_HIP_TRY(hipSetDevice(h->device));
-  for (uint32_t k = 0; k < nsteps; ++k) {
+  bool done = false;
+  if (h->persistent && nsteps > 0) {
+    const int rc = h->update_persistent(nsteps, &done);
+    if (rc)
+      return rc;
+  }
+  for (uint32_t k = 0; k < nsteps && !done; ++k) {
     const int rc = h->step();
     if (rc)
       return rc;
@@ -1405,7 +1492,7 @@ int rtp_boids_sync(rtp_boids* h)
 {
   RTP_REQUIRE(h, "NULL handle");
   RTP_HIP_TRY(hipStreamSynchronize(h->stream));
-  return RTP_OK;
+  return h->check_persistent();
 }
 
 int rtp_boids_read_grid(rtp_boids* h, uint32_t* cellIDs, uint32_t* sortIdx, uint32_t* startEnd)
@@ -1431,11 +1518,12 @@ int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
 int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP);
+  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_PERSISTENT);
   if (rc)
     return rc;
   h->L.block = t->blockSize;
   h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
+  h->persistent = (t->flags & RTP_LAUNCH_PERSISTENT) != 0;
   return RTP_OK;
 }
 

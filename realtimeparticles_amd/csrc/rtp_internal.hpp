@@ -78,6 +78,26 @@ struct BoidsConsts {
   int32_t shortDiv = 0;     // radiusSq <= 2^30: the pair divisions take the short form
 };
 
+// The persistent boids update (one cooperative launch for `steps` steps): the buffers of
+// rtp_boids and the sort's pass count.  bar: 3 words, zeroed before the launch (arrival
+// count, generation, timeout flag).
+struct BoidsPersistentArgs {
+  float4* pos[2];
+  float4* vel[2];
+  float4* unitVel;
+  float4* acc;
+  uint32_t* key[2];
+  uint32_t* idx[2];
+  uint2* startEnd;
+  uint32_t* bar;
+  uint32_t n, sortN, maxN, steps;
+  int cur, keyCur, passes;
+};
+int launch_boids_persistent(const BoidsPersistentArgs& a, const BoidsConsts& c, uint32_t groups,
+                            hipStream_t s);
+// Largest co-resident workgroup count of the persistent update (0 if unknown).
+uint32_t boids_persistent_max_groups(int dim2);
+
 // Largest artificial-pressure exponent a parameter set may carry: the kernels raise to it
 // with a left-to-right product, one multiplication per unit (the reference's range is 1-6).
 constexpr uint32_t kMaxArtPressureExp = 64;
@@ -133,6 +153,9 @@ int sort_pairs(const SortPlan& plan, uint32_t* keyA, uint32_t* keyB, uint32_t* i
 // Whether sort_pairs(plan, ...) on the current device leaves its result in the B buffers
 // (known before the launch, so a caller can put its own output buffers on that side).
 bool sort_result_in_b(const SortPlan& plan);
+// Passes of the one-workgroup LDS sort when it takes this plan (n <= kSmallSortMax and its
+// LDS fits the device), else 0.
+int small_sort_pass_count(const SortPlan& plan);
 
 void launch_fill_keys_and_reset_cells(const float4* pos, uint32_t n, uint32_t* key,
                                       uint2* startEnd, const GridConsts& g, hipStream_t s);
