@@ -77,30 +77,45 @@ struct Body {
   F& f;
 };
 
+// One survivor's body: the reference's d2 (unfused) and the pass's work.
+template <class B, class X>
+__device__ __forceinline__ void body_at(const float4 p, uint32_t e, const float4 o, const X& x,
+                                        B& b)
+{
+  const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  b.f(e, x, dx, dy, dz, d2);
+}
+
 // Process the lane's queued survivors in order, loading survivor k+1 while computing k.
+// Unrolled by two over two register sets, so no survivor's data is copied from one set to
+// the other between bodies.
 template <bool CL, class B>
 __device__ __forceinline__ void drain(LaneQueue& q, const float4 p, const float4* __restrict__ pred,
                                       const PbfConsts& c, B& b)
 {
   const uint32_t* rp = q.slot;
   if (rp != q.wp) {
-    uint32_t e = *rp;
-    float4 o = ldn<CL>(pred, e, c);
-    auto x = b.ld(e);
+    uint32_t e0 = *rp;
+    float4 o0 = ldn<CL>(pred, e0, c);
+    auto x0 = b.ld(e0);
     for (;;) {
       rp += q.stride;
-      const bool more = rp != q.wp;
-      const uint32_t en = *(more ? rp : q.slot);  // in bounds either way
-      const float4 on = ldn<CL>(pred, en, c);
-      const auto xn = b.ld(en);
-      const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-      const float d2 = dx * dx + dy * dy + dz * dz;
-      b.f(e, x, dx, dy, dz, d2);
+      bool more = rp != q.wp;
+      const uint32_t e1 = *(more ? rp : q.slot);  // in bounds either way
+      const float4 o1 = ldn<CL>(pred, e1, c);
+      const auto x1 = b.ld(e1);
+      body_at(p, e0, o0, x0, b);
       if (!more)
         break;
-      e = en;
-      o = on;
-      x = xn;
+      rp += q.stride;
+      more = rp != q.wp;
+      e0 = *(more ? rp : q.slot);
+      o0 = ldn<CL>(pred, e0, c);
+      x0 = b.ld(e0);
+      body_at(p, e1, o1, x1, b);
+      if (!more)
+        break;
     }
   }
   q.wp = q.slot;
@@ -399,7 +414,9 @@ __device__ __forceinline__ void walk_cells(const float4 p, const float4* __restr
 
 // Run a recorded survivor list: the same body calls, in the same order, as the walk that
 // recorded it.  Entry k + 1's granule, position and pass data are loaded while entry k is
-// computed (a new granule every fourth entry).
+// computed (a new granule every fourth entry).  Unrolled by two: odd entries are lanes y or
+// w of the current granule, even ones lanes x (of the next granule) or z, and the two
+// register sets alternate, so nothing is copied between bodies.
 template <bool CL, class B>
 __device__ __forceinline__ void replay(const uint4* __restrict__ gp, uint32_t stride, uint32_t cnt,
                                        const float4 p, const float4* __restrict__ pred,
@@ -408,27 +425,29 @@ __device__ __forceinline__ void replay(const uint4* __restrict__ gp, uint32_t st
   if (cnt == 0)
     return;
   uint4 v = *gp;
-  uint32_t e = v.x;
-  float4 o = ldn<CL>(pred, e, c);
-  auto x = b.ld(e);
-  for (uint32_t k = 1;; ++k) {
-    const bool more = k < cnt;
-    const uint32_t j = k & 3u;
-    if (more && j == 0u) {
+  uint32_t e0 = v.x;
+  float4 o0 = ldn<CL>(pred, e0, c);
+  auto x0 = b.ld(e0);
+  for (uint32_t k = 1;; k += 2u) {
+    bool more = k < cnt;
+    const uint32_t e1 = !more ? e0 : ((k & 2u) ? v.w : v.y);
+    const float4 o1 = ldn<CL>(pred, e1, c);
+    const auto x1 = b.ld(e1);
+    body_at(p, e0, o0, x0, b);
+    if (!more)
+      break;
+    const uint32_t k2 = k + 1u;
+    more = k2 < cnt;
+    if (more && (k2 & 3u) == 0u) {
       gp += stride;
       v = *gp;
     }
-    const uint32_t en = !more ? e : (j == 0u ? v.x : (j == 1u ? v.y : (j == 2u ? v.z : v.w)));
-    const float4 on = ldn<CL>(pred, en, c);
-    const auto xn = b.ld(en);
-    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    b.f(e, x, dx, dy, dz, d2);
+    e0 = !more ? e1 : ((k2 & 2u) ? v.z : v.x);
+    o0 = ldn<CL>(pred, e0, c);
+    x0 = b.ld(e0);
+    body_at(p, e1, o1, x1, b);
     if (!more)
       break;
-    e = en;
-    o = on;
-    x = xn;
   }
 }
 
