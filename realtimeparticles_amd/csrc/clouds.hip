@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "rtp_crmath.hpp"
 #include "rtp_device.hpp"
 #include "rtp_exp.hpp"
 #include "rtp_host.hpp"
@@ -64,7 +65,6 @@ __device__ __forceinline__ float dot4(float4 a, float4 b)
 {
   return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
 }
-__device__ __forceinline__ float len4(float4 v) { return __builtin_sqrtf(dot4(v, v)); }
 __device__ __forceinline__ float4 normalize4(float4 v)
 {
   if (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f && v.w == 0.0f)
@@ -77,17 +77,15 @@ __device__ __forceinline__ float4 cross4(float4 a, float4 b)
 }
 __device__ __forceinline__ float step1(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
 
-// sph.cl:10-14
-__device__ __forceinline__ float poly6(float4 vec, const PbfConsts& c)
+// sph.cl:10-14 given r = length(vec) = sqrtf(dot4(vec, vec))
+__device__ __forceinline__ float poly6_r(float r, const PbfConsts& c)
 {
-  const float r = len4(vec);
   return ((1.0f - step1(c.h, r)) * c.poly6) * pown(c.h * c.h - r * r, 3);
 }
 
-// sph.cl:26-34
-__device__ __forceinline__ float4 grad_spiky(float4 vec, const PbfConsts& c)
+// sph.cl:26-34 given r = length(vec)
+__device__ __forceinline__ float4 grad_spiky_r(float4 vec, float r, const PbfConsts& c)
 {
-  const float r = len4(vec);
   if (r <= kFloatEps)
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float4 g = mul4s(vec, 1.0f - step1(c.h, r));
@@ -97,12 +95,51 @@ __device__ __forceinline__ float4 grad_spiky(float4 vec, const PbfConsts& c)
   return div4s(g, r);
 }
 
-// clouds.cl:106-114 (poly6L(artPressureRadius * h) is folded into c.wdq)
-__device__ __forceinline__ float art_pressure(float4 vec, const PbfConsts& c)
+// length(vec) from d2 = dot4(vec, vec): sqrt_cr returns sqrtf's bits for every d2 >= 0.
+__device__ __forceinline__ float len_of(float d2) { return sqrt_cr(d2); }
+
+__device__ __forceinline__ bool tiny_numerator4(float a, float b, float c, float d)
+{
+  return tiny_numerator3(a, b, c) || num_key(d) < (kDivNumMin << 1) - 2u;
+}
+
+// grad_spiky_r with the short correctly rounded division where its operand ranges hold
+// (the shortMath bit: kFloatEps < r < h, every numerator +-0 or not tiny): the four
+// quotients by r share one refined reciprocal.  With t = 1 - step(h, r) = 1 the factor t
+// is an exact identity.  Other lanes take the general expression.
+__device__ __forceinline__ float4 grad_spiky_fast(float4 vec, float r, const PbfConsts& c)
+{
+  const bool in = (c.shortMath & kShortSqrtGrad) != 0 && r < c.h;
+  float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 n = g;
+  bool general = !in;
+  if (in) {
+    const float p2 = (c.h - r) * (c.h - r);
+    n = make_float4(((vec.x * c.spiky) * -3.0f) * p2, ((vec.y * c.spiky) * -3.0f) * p2,
+                    ((vec.z * c.spiky) * -3.0f) * p2, ((vec.w * c.spiky) * -3.0f) * p2);
+    const float yr = rcp_refined(r);
+    const bool on = r > kFloatEps;
+    if (on)
+      g = make_float4(div_short(n.x, r, yr), div_short(n.y, r, yr), div_short(n.z, r, yr),
+                      div_short(n.w, r, yr));
+    general = on && tiny_numerator4(n.x, n.y, n.z, n.w);
+  }
+  if (__builtin_amdgcn_ballot_w64(general) != 0) {
+    rare_path();
+    if (general)
+      g = in ? div4s(n, r) : grad_spiky_r(vec, r, c);
+  }
+  return g;
+}
+
+// clouds.cl:106-114 given w = poly6(vec) (poly6L(artPressureRadius * h) is folded into
+// c.wdq); ywdq = rcp_refined(c.wdq) when the short division applies (kShortArtDiv).
+__device__ __forceinline__ float art_pressure_w(float w, float ywdq, const PbfConsts& c)
 {
   if (c.p.isArtPressureEnabled == 0)
     return 0.0f;
-  return -c.p.artPressureCoeff * pown(poly6(vec, c) / c.wdq, c.p.artPressureExp);
+  const float q = (c.shortMath & kShortArtDiv) ? div_short(w, c.wdq, ywdq) : w / c.wdq;
+  return -c.p.artPressureCoeff * pown(q, c.p.artPressureExp);
 }
 
 // clouds.cl:67-70
@@ -141,44 +178,73 @@ __device__ __forceinline__ float4 mixed_boundary(float4 p, const GridConsts& g)
   return q;
 }
 
-// Candidates lo..hi (inclusive) in index order with one shift: four positions and four
-// neighbour data (ld(e)) in flight before their bodies run in order.
-template <class L, class F>
-__device__ __forceinline__ void cld_range(uint32_t lo, uint32_t hi, const float4 pos, const float4 shift,
-                                          const float4* __restrict__ src, L& ld, F& f)
+// ------------------------------------------------------------ the walk with survivor queues
+// Most candidates of a walk contribute exactly +-0 and are skipped (negligible, below).  A
+// wave whose lanes each ran the body in place would run it for almost every candidate step
+// (some lane nearly always keeps its candidate) with few lanes on, and lanes whose cells hold
+// different counts would leave the loops at different steps.  So the walk is split:
+//  - each lane lists its non-empty candidate ranges in walk order (cld_ranges, in LDS);
+//  - the wave steps through all lanes' lists together, kCldStep candidates per lane a step,
+//    and each lane queues the candidates it keeps (index and wrap-shift code) in its own LDS
+//    column;
+//  - when any lane's queue is nearly full, and at the end, every lane runs the bodies of its
+//    queued candidates, in order.
+// Each lane's bodies run in its walk's order, so every sum is the same.
+#ifndef RTP_CLD_STEP
+#define RTP_CLD_STEP 4
+#endif
+constexpr uint32_t kCldLanes = 64;   // threads per block of a walk kernel: one wave
+constexpr uint32_t kCldQueue = 32;   // queue entries per lane
+constexpr uint32_t kCldStep = RTP_CLD_STEP;  // candidates per lane per scan step
+constexpr uint32_t kCldRanges = 27;  // at most one range per visited cell
+constexpr uint32_t kCldIndexBits = 28;
+constexpr uint32_t kCldIndexMask = (1u << kCldIndexBits) - 1u;
+
+struct CldLds {
+  uint32_t queue[kCldQueue * kCldLanes];  // lane l's entries at l, l + 64, ...
+  uint2 ranges[kCldRanges * kCldLanes];   // lane l's ranges at l, l + 64, ...
+};
+
+struct CldQueue {
+  uint32_t* slot;   // this lane's first entry
+  uint32_t* wp;     // next free entry
+  uint32_t* room;   // last position of wp that leaves kCldStep free entries
+};
+
+// The shift of a wrapped cell, absWall * sign with sign +-2 on x and z (clouds.cl:334-343),
+// from its code: bit 0 / 1 x = +2 / -2, bit 2 / 3 z = +2 / -2.
+__device__ __forceinline__ float4 cld_shift(uint32_t code, const GridConsts& g)
 {
-  uint32_t e = lo;
-  for (; e + 3u <= hi; e += 4u) {
-    const float4 o0 = src[e], o1 = src[e + 1u], o2 = src[e + 2u], o3 = src[e + 3u];
-    const auto x0 = ld(e), x1 = ld(e + 1u), x2 = ld(e + 2u), x3 = ld(e + 3u);
-    f(e, x0, sub4(sub4(pos, o0), shift));
-    f(e + 1u, x1, sub4(sub4(pos, o1), shift));
-    f(e + 2u, x2, sub4(sub4(pos, o2), shift));
-    f(e + 3u, x3, sub4(sub4(pos, o3), shift));
-  }
-  for (; e <= hi; ++e)
-    f(e, ld(e), sub4(sub4(pos, src[e]), shift));
+  const float sx = (code & 1u) ? 2.0f : ((code & 2u) ? -2.0f : 0.0f);
+  const float sz = (code & 4u) ? 2.0f : ((code & 8u) ? -2.0f : 0.0f);
+  return make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * sz, 0.0f * 0.0f);
 }
 
-// The 27-cell walk of clouds.cl:322-353: f(e, ld(e), vec) for every candidate e of every
-// visited cell in the reference's order, vec = pos - p_e - absWall * sign (sign +-2 on a
-// wrapped x or z).  The three cells of a z-column are one index range when none of them
-// wraps in z and their non-empty ranges are back to back (cells are numbered z-minor), which
-// visits the same candidates in the same order.
-template <class L, class F>
-__device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restrict__ src,
-                                         const uint2* __restrict__ se, const GridConsts& g, L&& ld,
-                                         F&& f)
+// The non-empty candidate ranges of the 27-cell walk of clouds.cl:322-353 in its order (x
+// outer, z inner), as {first | shift code << 28, last}, every kCldLanes-th entry of out from
+// out[0]; returns their count.  A wrapped neighbour's position is shifted by the code's
+// +-2 absWall; y cells beyond the walls are skipped.  The three cells of a z-column are one
+// range when none of them wraps in z and their non-empty ranges are back to back (cells are
+// numbered z-minor), which lists the same candidates in the same order.
+__device__ __forceinline__ uint32_t cld_ranges(const float4 pos, const uint2* __restrict__ se,
+                                               const GridConsts& g, uint2* __restrict__ out)
 {
   uint32_t ux, uy, uz;
   cell3d(pos, g, ux, uy, uz);
   const int32_t cx = (int32_t)ux, cy = (int32_t)uy, cz = (int32_t)uz;
   const int32_t rx = g.res[0], ry = g.res[1], rz = g.res[2];
   const bool zInside = cz - 1 >= 0 && cz + 1 < rz;
+  uint32_t nr = 0;
+  const auto push = [&](uint32_t lo, uint32_t hi, uint32_t code) {
+    out[nr * kCldLanes] = make_uint2(lo | (code << kCldIndexBits), hi);
+    ++nr;
+  };
+#pragma unroll
   for (int ix = -1; ix <= 1; ++ix) {
     const int32_t ax = cx + ix;
     const int32_t nx = (ax + rx) % rx;
-    const float sx = ax >= rx ? 2.0f : (ax < 0 ? -2.0f : 0.0f);
+    const uint32_t xcode = ax >= rx ? 1u : (ax < 0 ? 2u : 0u);
+#pragma unroll
     for (int iy = -1; iy <= 1; ++iy) {
       const int32_t ay = cy + iy;
       if (ay >= ry || ay < 0)
@@ -203,31 +269,114 @@ __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restr
             merged = false;
           }
         }
-        const float4 shift =
-          make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * 0.0f, 0.0f * 0.0f);
         if (merged) {
           if (lo <= hi)
-            cld_range(lo, hi, pos, shift, src, ld, f);
+            push(lo, hi, xcode);
         } else {
 #pragma unroll
           for (int k = 0; k < 3; ++k)
             if (cells[k].x <= cells[k].y)
-              cld_range(cells[k].x, cells[k].y, pos, shift, src, ld, f);
+              push(cells[k].x, cells[k].y, xcode);
         }
         continue;
       }
+#pragma unroll
       for (int iz = -1; iz <= 1; ++iz) {
         const int32_t az = cz + iz;
         const int32_t nz = (az + rz) % rz;
-        const float sz = az >= rz ? 2.0f : (az < 0 ? -2.0f : 0.0f);
-        const float4 shift =
-          make_float4(g.absWall[0] * sx, g.absWall[1] * 0.0f, g.absWall[2] * sz, 0.0f * 0.0f);
         const uint2 r = se[row + nz];
         if (r.x <= r.y)
-          cld_range(r.x, r.y, pos, shift, src, ld, f);
+          push(r.x, r.y, xcode | (az >= rz ? 4u : (az < 0 ? 8u : 0u)));
       }
     }
   }
+  return nr;
+}
+
+// Run the lane's queued bodies in order (the next entry's data loaded while one computes).
+template <class L, class F>
+__device__ __forceinline__ void cld_drain(CldQueue& q, const float4 pos, const float4* __restrict__ src,
+                                          const GridConsts& g, L& ld, F& f)
+{
+  const uint32_t* rp = q.slot;
+  if (rp != q.wp) {
+    uint32_t v = *rp;
+    uint32_t e = v & kCldIndexMask;
+    float4 o = src[e];
+    auto x = ld(e);
+    for (;;) {
+      rp += kCldLanes;
+      const bool more = rp != q.wp;
+      const uint32_t vn = *(more ? rp : q.slot);  // in bounds either way
+      const uint32_t en = vn & kCldIndexMask;
+      const float4 on = src[en];
+      const auto xn = ld(en);
+      f(e, x, sub4(sub4(pos, o), cld_shift(v >> kCldIndexBits, g)));
+      if (!more)
+        break;
+      v = vn;
+      e = en;
+      o = on;
+      x = xn;
+    }
+  }
+  q.wp = q.slot;
+}
+
+// The walk: f(e, ld(e), vec) for every candidate e with keep(ld(e), vec), in the order of
+// clouds.cl:322-353, vec = pos - p_e - the wrap shift.  Called by every thread of a block of
+// kCldLanes threads that has a particle (the others return before).
+template <class L, class K, class F>
+__device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restrict__ src,
+                                         const uint2* __restrict__ se, const GridConsts& g,
+                                         CldLds& lds, L&& ld, K&& keep, F&& f)
+{
+  const uint32_t lane = threadIdx.x;
+  const uint2* rl = &lds.ranges[lane];
+  const uint32_t nr = cld_ranges(pos, se, g, &lds.ranges[lane]);
+  CldQueue q{ &lds.queue[lane], &lds.queue[lane],
+              &lds.queue[lane + (kCldQueue - kCldStep) * kCldLanes] };
+  uint32_t rk = 0, e = 1u, hi = 0u, tag = 0u;
+  float4 shift = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const auto next_range = [&]() {
+    if (rk < nr) {
+      const uint2 r = rl[rk * kCldLanes];
+      ++rk;
+      e = r.x & kCldIndexMask;
+      tag = r.x & ~kCldIndexMask;
+      hi = r.y;
+      shift = cld_shift(r.x >> kCldIndexBits, g);
+    }
+  };
+  next_range();
+  for (;;) {
+    const bool on = e <= hi;
+    if (__builtin_amdgcn_ballot_w64(on) == 0)
+      break;
+    if (on) {
+      // candidates e..e+kCldStep-1 that are <= hi; the loads of the others repeat hi's
+      float4 o[kCldStep];
+      decltype(ld(e)) x[kCldStep];
+#pragma unroll
+      for (uint32_t k = 0; k < kCldStep; ++k) {
+        const uint32_t ek = min(e + k, hi);
+        o[k] = src[ek];
+        x[k] = ld(ek);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kCldStep; ++k) {
+        const bool kept = keep(x[k], sub4(sub4(pos, o[k]), shift));
+        *q.wp = (e + k) | tag;
+        q.wp += (kept & (e + k <= hi)) ? kCldLanes : 0u;
+      }
+      e += kCldStep;
+      if (e > hi)
+        next_range();
+    }
+    if (__builtin_amdgcn_ballot_w64(q.wp > q.room) != 0)
+      cld_drain(q, pos, src, g, ld, f);
+  }
+  cld_drain(q, pos, src, g, ld, f);
 }
 
 // A candidate at r >= h (d2 >= cullSq, so sqrt(d2) >= h) with a finite distance contributes
@@ -239,7 +388,13 @@ __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restr
 __device__ __forceinline__ bool negligible(float4 vec, const PbfConsts& c)
 {
   const float d2 = dot4(vec, vec);
-  return d2 >= c.cullSq && d2 < kFarD2;
+  return (d2 >= c.cullSq) & (d2 < kFarD2);
+}
+
+__device__ __forceinline__ bool finite4(float4 v)
+{
+  return __builtin_isfinite(v.x) & __builtin_isfinite(v.y) & __builtin_isfinite(v.z) &
+         __builtin_isfinite(v.w);
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
@@ -340,26 +495,29 @@ __global__ __launch_bounds__(256) void k_cld_gather(const uint32_t* __restrict__
 // cld_computeLaplacianTemp (clouds.cl:508-569) and cld_computeConstraintFactorTemp
 // (:575-648) in one walk over p_pos: the two sums are independent and each keeps the
 // reference's order.
-__global__ __launch_bounds__(256) void k_cld_temp_lambda(const float4* __restrict__ posP,
+__global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda(const float4* __restrict__ posP,
                                                          const float* __restrict__ tempP,
                                                          const uint2* __restrict__ se,
                                                          float* __restrict__ lap,
                                                          float* __restrict__ lambdaT, uint32_t n,
                                                          CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = posP[i];
   const float temp = tempP[i];
   float laplacian = 0.0f, sumGradCi = 0.0f, sumSqGradC = 0.0f;
   const float rho0 = c.q.restDensity;
-  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e) { return tempP[e]; }, [&](uint32_t, float te, float4 vec) {
+  __shared__ CldLds lds;
+  const auto keep = [&](float te, float4 vec) {
+    return !(negligible(vec, c.f) & __builtin_isfinite(temp - te));
+  };
+  cld_walk(pos, posP, se, c.f.g, lds, [&](uint32_t e) { return tempP[e]; }, keep, [&](uint32_t, float te, float4 vec) {
     const float dTemp = temp - te;
-    if (negligible(vec, c.f) && __builtin_isfinite(dTemp))
-      return;
-    const float4 grad = grad_spiky(vec, c.f);
-    const float vg = dot4(vec, grad), vv = dot4(vec, vec);
+    const float vv = dot4(vec, vec);
+    const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
+    const float vg = dot4(vec, grad);
     laplacian = laplacian + dTemp * vg / (vv + kFloatEps);
     const float derivativeTemp = vg / (vv * rho0 + kFloatEps);
     sumGradCi = sumGradCi + derivativeTemp;
@@ -373,38 +531,41 @@ __global__ __launch_bounds__(256) void k_cld_temp_lambda(const float4* __restric
 
 // cld_computeConstraintCorrectionTemp (clouds.cl:654-721) and cld_correctTemperature
 // (:931-935): temp += 0.3 * correction.
-__global__ __launch_bounds__(256) void k_cld_temp_correct(const float4* __restrict__ posP,
+__global__ __launch_bounds__(kCldLanes) void k_cld_temp_correct(const float4* __restrict__ posP,
                                                           const float* __restrict__ lambdaT,
                                                           const uint2* __restrict__ se,
                                                           float* __restrict__ temp, uint32_t n,
                                                           CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = posP[i];
   const float lambdaI = lambdaT[i];
   const float rho0 = c.q.restDensity;
   float corr = 0.0f;
-  cld_walk(pos, posP, se, c.f.g, [&](uint32_t e) { return lambdaT[e]; }, [&](uint32_t, float le, float4 vec) {
+  __shared__ CldLds lds;
+  const auto keep = [&](float le, float4 vec) {
+    return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
+  };
+  cld_walk(pos, posP, se, c.f.g, lds, [&](uint32_t e) { return lambdaT[e]; }, keep, [&](uint32_t, float le, float4 vec) {
     const float l = lambdaI + le;
-    if (negligible(vec, c.f) && __builtin_isfinite(l))
-      return;
-    const float4 grad = grad_spiky(vec, c.f);
-    const float derivativeTemp = dot4(vec, grad) / (dot4(vec, vec) * rho0 + kFloatEps);
+    const float vv = dot4(vec, vec);
+    const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
+    const float derivativeTemp = dot4(vec, grad) / (vv * rho0 + kFloatEps);
     corr = corr + l * derivativeTemp;
   });
   temp[i] = temp[i] + 0.3f * corr;
 }
 
 // cld_computeDensity (clouds.cl:305-359) + cld_computeConstraintFactor (:365-435).
-__global__ __launch_bounds__(256) void k_cld_density_lambda(const float4* __restrict__ pred,
+__global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda(const float4* __restrict__ pred,
                                                             const uint2* __restrict__ se,
                                                             float* __restrict__ density,
                                                             float* __restrict__ lambda, uint32_t n,
                                                             CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = pred[i];
@@ -412,11 +573,12 @@ __global__ __launch_bounds__(256) void k_cld_density_lambda(const float4* __rest
   float fluidDensity = 0.0f;
   float4 sumGradCi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float sumSqGradC = 0.0f;
-  cld_walk(pos, pred, se, c.f.g, [](uint32_t) { return 0; }, [&](uint32_t, int, float4 vec) {
-    if (negligible(vec, c.f))
-      return;
-    fluidDensity = fluidDensity + poly6(vec, c.f);
-    const float4 grad = grad_spiky(vec, c.f);
+  __shared__ CldLds lds;
+  const auto keep = [&](int, float4 vec) { return !negligible(vec, c.f); };
+  cld_walk(pos, pred, se, c.f.g, lds, [](uint32_t) { return 0; }, keep, [&](uint32_t, int, float4 vec) {
+    const float r = len_of(dot4(vec, vec));
+    fluidDensity = fluidDensity + poly6_r(r, c.f);
+    const float4 grad = grad_spiky_fast(vec, r, c.f);
     sumGradCi = add4(sumGradCi, grad);
     sumSqGradC = sumSqGradC + dot4(grad, grad);
   });
@@ -431,7 +593,7 @@ __global__ __launch_bounds__(256) void k_cld_density_lambda(const float4* __rest
 // cld_computeConstraintCorrection (clouds.cl:441-502), cld_correctPosition on p_predPos and
 // p_totCorrPos (:920-925) and cld_applyMixedBoundaryConditions (Clouds.cpp:573-586); on the
 // last iteration also cld_updateVel (clouds.cl:958-966).
-__global__ __launch_bounds__(256) void k_cld_correct(const float4* __restrict__ pred,
+__global__ __launch_bounds__(kCldLanes) void k_cld_correct(const float4* __restrict__ pred,
                                                      const float* __restrict__ lambda,
                                                      const uint2* __restrict__ se,
                                                      float4* __restrict__ predOut,
@@ -439,17 +601,21 @@ __global__ __launch_bounds__(256) void k_cld_correct(const float4* __restrict__ 
                                                      float4* __restrict__ vel, uint32_t n,
                                                      CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = pred[i];
   const float lambdaI = lambda[i];
+  const float ywdq = (c.f.shortMath & kShortArtDiv) ? rcp_refined(c.f.wdq) : 0.0f;
   float4 corr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return lambda[e]; }, [&](uint32_t, float le, float4 vec) {
-    if (negligible(vec, c.f) && __builtin_isfinite(lambdaI + le))
-      return;
-    const float s = lambdaI + le + art_pressure(vec, c.f);
-    corr = add4(corr, muls4(s, grad_spiky(vec, c.f)));
+  __shared__ CldLds lds;
+  const auto keep = [&](float le, float4 vec) {
+    return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
+  };
+  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return lambda[e]; }, keep, [&](uint32_t, float le, float4 vec) {
+    const float r = len_of(dot4(vec, vec));
+    const float s = lambdaI + le + art_pressure_w(poly6_r(r, c.f), ywdq, c.f);
+    corr = add4(corr, muls4(s, grad_spiky_fast(vec, r, c.f)));
   });
   const float4 corrPos = div4s(corr, c.f.p.restDensity);
   predOut[i] = mixed_boundary(add4(pos, corrPos), c.f.g);
@@ -478,48 +644,50 @@ __global__ __launch_bounds__(256) void k_cld_update_vel(const float4* __restrict
 }
 
 // cld_computeVorticity (clouds.cl:727-784).
-__global__ __launch_bounds__(256) void k_cld_vorticity(const float4* __restrict__ pred,
+__global__ __launch_bounds__(kCldLanes) void k_cld_vorticity(const float4* __restrict__ pred,
                                                        const uint2* __restrict__ se,
                                                        const float4* __restrict__ vel,
                                                        float4* __restrict__ vort, uint32_t n,
                                                        CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = pred[i];
   const float4 velocity = vel[i];
   float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return vel[e]; }, [&](uint32_t, float4 ve, float4 vec) {
+  __shared__ CldLds lds;
+  const auto keep = [&](float4 ve, float4 vec) { return !(negligible(vec, c.f) & finite4(sub4(ve, velocity))); };
+  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return vel[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
     const float4 dv = sub4(ve, velocity);
-    if (negligible(vec, c.f) && __builtin_isfinite(dv.x) && __builtin_isfinite(dv.y) &&
-        __builtin_isfinite(dv.z) && __builtin_isfinite(dv.w))
-      return;
-    w = add4(w, cross4(dv, grad_spiky(vec, c.f)));
+    w = add4(w, cross4(dv, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
   });
   vort[i] = w;
 }
 
 // cld_applyVorticityConfinement (clouds.cl:790-850), written to the copy XSPH reads
 // (p_velInViscosity, Clouds.cpp:597-599).
-__global__ __launch_bounds__(256) void k_cld_confinement(const float4* __restrict__ pred,
+__global__ __launch_bounds__(kCldLanes) void k_cld_confinement(const float4* __restrict__ pred,
                                                          const uint2* __restrict__ se,
                                                          const float4* __restrict__ vort,
                                                          const float4* __restrict__ vel,
                                                          float4* __restrict__ velOut, uint32_t n,
                                                          CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pos = pred[i];
   const float4 vorticity = vort[i];
   float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pos, pred, se, c.f.g, [&](uint32_t e) { return vort[e]; }, [&](uint32_t, float4 we, float4 vec) {
-    const float lw = len4(we);
-    if (negligible(vec, c.f) && __builtin_isfinite(lw))
-      return;
-    nn = add4(nn, muls4(lw, grad_spiky(vec, c.f)));
+  __shared__ CldLds lds;
+  const auto keep = [&](float4 we, float4 vec) {
+    // sqrt keeps finiteness: isfinite(length(we)) == isfinite(dot4(we, we))
+    return !(negligible(vec, c.f) & __builtin_isfinite(dot4(we, we)));
+  };
+  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return vort[e]; }, keep, [&](uint32_t, float4 we, float4 vec) {
+    const float lw = len_of(dot4(we, we));
+    nn = add4(nn, muls4(lw, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
   });
   const rtp_fluid_params& p = c.f.p;
   velOut[i] = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
@@ -537,25 +705,24 @@ __device__ __forceinline__ float4 cld_update_pos(float4 pp, const CloudConsts& c
 }
 
 // cld_applyXsphViscosityCorrection (clouds.cl:856-914) + cld_updatePosition.
-__global__ __launch_bounds__(256) void k_cld_xsph(const float4* __restrict__ pred,
+__global__ __launch_bounds__(kCldLanes) void k_cld_xsph(const float4* __restrict__ pred,
                                                   const uint2* __restrict__ se,
                                                   const float4* __restrict__ velIn,
                                                   float4* __restrict__ velOut,
                                                   float4* __restrict__ pos, uint32_t n,
                                                   CloudConsts c)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
     return;
   const float4 pp = pred[i];
   const float4 velocity = velIn[i];
   float4 viscosity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  cld_walk(pp, pred, se, c.f.g, [&](uint32_t e) { return velIn[e]; }, [&](uint32_t, float4 ve, float4 vec) {
+  __shared__ CldLds lds;
+  const auto keep = [&](float4 ve, float4 vec) { return !(negligible(vec, c.f) & finite4(sub4(ve, velocity))); };
+  cld_walk(pp, pred, se, c.f.g, lds, [&](uint32_t e) { return velIn[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
     const float4 dv = sub4(ve, velocity);
-    if (negligible(vec, c.f) && __builtin_isfinite(dv.x) && __builtin_isfinite(dv.y) &&
-        __builtin_isfinite(dv.z) && __builtin_isfinite(dv.w))
-      return;
-    viscosity = add4(viscosity, mul4s(dv, poly6(vec, c.f)));
+    viscosity = add4(viscosity, mul4s(dv, poly6_r(len_of(dot4(vec, vec)), c.f)));
   });
   velOut[i] = add4(velocity, muls4(c.f.p.xsphViscosityCoeff, viscosity));
   pos[i] = cld_update_pos(pp, c);
@@ -689,20 +856,21 @@ struct rtp_clouds {
     pc = 1 - pc;
     const int gathered = pc;
     const uint32_t bn = blocks_for(std::max(n, 1u), 256);
+    const uint32_t bw = blocks_for(std::max(n, 1u), kCldLanes);  // the walk kernels
     if (n && c.q.isTempSmoothingEnabled) {
       RTP_PROF(prof, "cld_temp_lambda", s,
-               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bn), dim3(256), 0, s, pos[cur], temp[cur],
+               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], temp[cur],
                                   G.startEnd, lap, lambdaT, n, c));
       RTP_PROF(prof, "cld_temp_correct", s,
-               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bn), dim3(256), 0, s, pos[cur], lambdaT,
+               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], lambdaT,
                                   G.startEnd, temp[cur], n, c));
     }
     for (uint32_t k = 0; k < iters && n; ++k) {
       RTP_PROF(prof, "cld_density_lambda", s,
-               hipLaunchKernelGGL(k_cld_density_lambda, dim3(bn), dim3(256), 0, s, pred[pc],
+               hipLaunchKernelGGL(k_cld_density_lambda, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
                                   G.startEnd, density, lambda, n, c));
       RTP_PROF(prof, "cld_correct", s,
-               hipLaunchKernelGGL(k_cld_correct, dim3(bn), dim3(256), 0, s, pred[pc], lambda,
+               hipLaunchKernelGGL(k_cld_correct, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], lambda,
                                   G.startEnd, pred[1 - pc], tot[cur],
                                   k + 1 == iters ? vel[cur] : nullptr, n, c));
       pc = 1 - pc;
@@ -717,13 +885,13 @@ struct rtp_clouds {
       return RTP_OK;
     if (c.f.p.isVorticityConfEnabled) {
       RTP_PROF(prof, "cld_vorticity", s,
-               hipLaunchKernelGGL(k_cld_vorticity, dim3(bn), dim3(256), 0, s, pred[pc], G.startEnd,
+               hipLaunchKernelGGL(k_cld_vorticity, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
                                   vel[cur], vort, n, c));
       RTP_PROF(prof, "cld_confinement", s,
-               hipLaunchKernelGGL(k_cld_confinement, dim3(bn), dim3(256), 0, s, pred[pc],
+               hipLaunchKernelGGL(k_cld_confinement, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
                                   G.startEnd, vort, vel[cur], velTmp, n, c));
       RTP_PROF(prof, "cld_xsph", s,
-               hipLaunchKernelGGL(k_cld_xsph, dim3(bn), dim3(256), 0, s, pred[pc], G.startEnd,
+               hipLaunchKernelGGL(k_cld_xsph, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
                                   velTmp, vel[cur], pos[cur], n, c));
     } else {
       RTP_PROF(prof, "cld_update_pos", s,
@@ -772,6 +940,8 @@ int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
   *out = nullptr;
   RTP_REQUIRE(d->maxNbParticles > 0, "maxNbParticles must be positive");
   RTP_REQUIRE(d->currNbParticles <= d->maxNbParticles, "currNbParticles exceeds maxNbParticles");
+  RTP_REQUIRE(d->maxNbParticles < (1u << kCldIndexBits),
+              "maxNbParticles must be below 2^28 for the Clouds model");
   int rc = check_device(d->device);
   if (rc)
     return rc;

@@ -7,6 +7,10 @@ substep), and a third window of the same substeps with per-kernel HIP events on 
 stream.  Prints one JSON line: ms per substep and each profiler entry's launches and mean
 launch time, to set beside the rocprofv3 trace of the same command.
 
+A Clouds workload (clouds-65k: the cumulus case) runs the Clouds model's steps from the
+input instead, as bench.py's Clouds section does: `--warmup` untimed steps, then `--steps`
+timed steps from the input, then the same with per-kernel HIP events.
+
     rocprofv3 --kernel-trace --stats -d OUT -o run -- \\
         python3 -m realtimeparticles_amd.profile_step --workload pbf-1m --block 64 --flags 13 \\
         --queue 16 --sort-bits 10
@@ -31,6 +35,8 @@ def main(argv=None):
     ap.add_argument("--queue", type=int, default=16)
     ap.add_argument("--sort-bits", type=int, default=10)
     a = ap.parse_args(argv)
+    if a.workload.startswith("clouds"):
+        return clouds_main(a)
 
     sc = scenes.pbf_scene(a.workload)
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
@@ -61,6 +67,35 @@ def main(argv=None):
     print(json.dumps({
         "workload": a.workload, "mapping": a.mapping, "particles": sc.n,
         "tuning": {"block": a.block, "flags": a.flags, "queue": a.queue, "sort_bits": a.sort_bits},
+        "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
+        "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
+                    for k, v in sorted(prof.items())}}), flush=True)
+
+
+def clouds_main(a):
+    sc = scenes.cloud_scene(a.workload)
+    m = capi.Clouds(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(sc.params(), sc.cloud_params())
+    m.set_nb_jacobi_iters(sc.iters)
+
+    def from_input(k):
+        m.load_state(sc.pos, sc.vel)
+        m.update(k)
+        m.sync()
+
+    from_input(a.warmup)
+    m.load_state(sc.pos, sc.vel)
+    m.sync()
+    t0 = time.perf_counter()
+    m.update(a.steps)
+    m.sync()
+    elapsed = time.perf_counter() - t0
+    m.enable_profiling(True)
+    from_input(a.steps)
+    prof = m.profile()
+    m.close()
+    print(json.dumps({
+        "workload": a.workload, "particles": sc.n, "steps": a.steps,
         "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
         "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
                     for k, v in sorted(prof.items())}}), flush=True)
