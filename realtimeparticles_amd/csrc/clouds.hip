@@ -196,6 +196,10 @@ __device__ __forceinline__ float4 mixed_boundary(float4 p, const GridConsts& g)
 constexpr uint32_t kCldLanes = 64;   // threads per block of a walk kernel: one wave
 constexpr uint32_t kCldQueue = 32;   // queue entries per lane
 constexpr uint32_t kCldStep = RTP_CLD_STEP;  // candidates per lane per scan step
+#ifndef RTP_CLD_DRAIN
+#define RTP_CLD_DRAIN 4
+#endif
+constexpr uint32_t kCldDrain = RTP_CLD_DRAIN;  // queued bodies per drain group
 constexpr uint32_t kCldRanges = 27;  // at most one range per visited cell
 constexpr uint32_t kCldIndexBits = 28;
 constexpr uint32_t kCldIndexMask = (1u << kCldIndexBits) - 1u;
@@ -293,32 +297,27 @@ __device__ __forceinline__ uint32_t cld_ranges(const float4 pos, const uint2* __
   return nr;
 }
 
-// Run the lane's queued bodies in order (the next entry's data loaded while one computes).
+// Run the lane's queued bodies in order, kCldDrain entries at a time: their data loaded
+// together, then their bodies.
 template <class L, class F>
 __device__ __forceinline__ void cld_drain(CldQueue& q, const float4 pos, const float4* __restrict__ src,
                                           const GridConsts& g, L& ld, F& f)
 {
-  const uint32_t* rp = q.slot;
-  if (rp != q.wp) {
-    uint32_t v = *rp;
-    uint32_t e = v & kCldIndexMask;
-    float4 o = src[e];
-    auto x = ld(e);
-    for (;;) {
-      rp += kCldLanes;
-      const bool more = rp != q.wp;
-      const uint32_t vn = *(more ? rp : q.slot);  // in bounds either way
-      const uint32_t en = vn & kCldIndexMask;
-      const float4 on = src[en];
-      const auto xn = ld(en);
-      f(e, x, sub4(sub4(pos, o), cld_shift(v >> kCldIndexBits, g)));
-      if (!more)
-        break;
-      v = vn;
-      e = en;
-      o = on;
-      x = xn;
+  for (const uint32_t* rp = q.slot; rp < q.wp; rp += kCldDrain * kCldLanes) {
+    uint32_t v[kCldDrain];
+    float4 o[kCldDrain];
+    decltype(ld(0u)) x[kCldDrain];
+#pragma unroll
+    for (uint32_t k = 0; k < kCldDrain; ++k) {
+      const uint32_t* pk = rp + k * kCldLanes;
+      v[k] = *(pk < q.wp ? pk : q.slot);  // in bounds either way
+      o[k] = src[v[k] & kCldIndexMask];
+      x[k] = ld(v[k] & kCldIndexMask);
     }
+#pragma unroll
+    for (uint32_t k = 0; k < kCldDrain; ++k)
+      if (rp + k * kCldLanes < q.wp)
+        f(v[k] & kCldIndexMask, x[k], sub4(sub4(pos, o[k]), cld_shift(v[k] >> kCldIndexBits, g)));
   }
   q.wp = q.slot;
 }
