@@ -92,8 +92,9 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(const uint32_t* __restrict
       counts[(size_t)d * S + s] = hist[w][d];
 }
 
-// Exclusive scan of the digit-major count table (m words, m % 4 == 0) in three launches:
-// per-segment sums, a scan of the segment sums, and a rescan of every segment from its base.
+// Exclusive scan of the digit-major count table (m words, m % 4 == 0) in two launches:
+// per-segment sums, then a rescan of every segment from its base (the sum of the segment
+// sums before it).
 // A thread owns 16 consecutive words (four 16-byte loads), a block owns kScanSeg words.
 constexpr uint32_t kScanPer = 16;
 static_assert(kScanSeg == 256 * kScanPer, "a scan block owns 256 threads x kScanPer words");
@@ -152,31 +153,17 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     partial[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of nseg segment sums in place.
-__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t nseg)
-{
-  __shared__ uint32_t carryS;
-  if (threadIdx.x == 0)
-    carryS = 0;
-  __syncthreads();
-  for (uint32_t r0 = 0; r0 < nseg; r0 += 256) {
-    const uint32_t i = r0 + threadIdx.x;
-    const uint32_t v = i < nseg ? partial[i] : 0u;
-    uint32_t total = 0;
-    const uint32_t ex = block_exclusive_256(v, &total);
-    const uint32_t carry = carryS;
-    if (i < nseg)
-      partial[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      carryS = carry + total;
-    __syncthreads();
-  }
-}
-
+// Each block's base is the sum of the segment sums before it, summed here: the segments
+// are few (m / kScanSeg), so this costs less than a launch of its own.
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ data, uint32_t m,
                                                     const uint32_t* __restrict__ partial)
 {
+  uint32_t before = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 256u)
+    before += partial[j];
+  uint32_t segBase = 0;
+  (void)block_exclusive_256(before, &segBase);
+  __syncthreads();  // block_exclusive_256's LDS is used again below
   const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanPer;
   uint32_t v[kScanPer];
   load16(data, b, m, v);
@@ -185,7 +172,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ data,
   for (uint32_t k = 0; k < kScanPer; ++k)
     sum += v[k];
   uint32_t total = 0;
-  uint32_t run = partial[blockIdx.x] + block_exclusive_256(sum, &total);
+  uint32_t run = segBase + block_exclusive_256(sum, &total);
 #pragma unroll
   for (uint32_t k = 0; k < kScanPer; k += 4) {
     uint4 q;
@@ -309,7 +296,6 @@ int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* 
   const uint32_t nseg = (m + kScanSeg - 1u) / kScanSeg;
   uint32_t* partial = counts + (size_t)2048 * p.subtiles;  // past the largest table for n
   hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, counts, m, partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, partial, nseg);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, counts, m, partial);
   hipLaunchKernelGGL((k_sort_downsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, iin,
                      p.n, shift, p.subtiles, counts, kout, iout);
@@ -339,14 +325,44 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
     p[i] = v;
 }
 
-// Counting-sort ranks: each particle's place among the particles of its cell.
-__global__ __launch_bounds__(256) void k_count_ranks(const uint32_t* __restrict__ key, uint32_t n,
-                                                     uint32_t* __restrict__ count,
-                                                     uint32_t* __restrict__ rank)
+// Counting-sort ranks: each particle's place among the particles of its cell.  The
+// particles of a block are counted per cell in an LDS hash table first (open addressing
+// over twice as many slots as the block has threads, so a probe always finds the cell or
+// a free slot), and one device atomic per cell of the block reserves their places: far
+// fewer device atomics than one per particle, since a block's particles share few cells.
+constexpr uint32_t kRankThreads = 1024;
+constexpr uint32_t kRankSlots = 2 * kRankThreads;
+constexpr uint32_t kNoKey = 0xffffffffu;  // keys are cell indices below 2^32 - 16
+__global__ __launch_bounds__(kRankThreads) void k_count_ranks(const uint32_t* __restrict__ key,
+                                                              uint32_t n, uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ rank)
 {
+  __shared__ uint32_t tkey[kRankSlots];
+  __shared__ uint32_t tcnt[kRankSlots];
+  for (uint32_t s = threadIdx.x; s < kRankSlots; s += blockDim.x) {
+    tkey[s] = kNoKey;
+    tcnt[s] = 0u;
+  }
+  __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = kNoKey, slot = 0u, local = 0u;
+  if (i < n) {
+    k = key[i];
+    slot = (k * 2654435761u) >> 21;  // 11 bits: kRankSlots
+    for (;;) {
+      const uint32_t old = atomicCAS(&tkey[slot], kNoKey, k);
+      if (old == kNoKey || old == k)
+        break;
+      slot = (slot + 1u) & (kRankSlots - 1u);
+    }
+    local = atomicAdd(&tcnt[slot], 1u);
+  }
+  __syncthreads();
+  if (i < n && local == 0u)
+    tcnt[slot] = atomicAdd(count + k, tcnt[slot]);  // the cell's count in, its first place out
+  __syncthreads();
   if (i < n)
-    rank[i] = atomicAdd(count + key[i], 1u);
+    rank[i] = tcnt[slot] + local;
 }
 
 // Particle i goes to slot offset[key[i]] + rank[i] of the thread order.
@@ -377,9 +393,8 @@ void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, u
     return;
   const uint32_t nseg = (m16 + kScanSeg - 1u) / kScanSeg;
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(m16, 256)), dim3(256), 0, s, count, m16, 0u);
-  hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, n, count, rank);
+  hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, kRankThreads)), dim3(kRankThreads), 0, s, key, n, count, rank);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, partial, nseg);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, count, m16, partial);
   hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, rank, count, n,
                      order);
