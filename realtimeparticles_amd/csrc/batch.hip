@@ -53,7 +53,7 @@ struct rtp_batch {
   bool resort = false;  // order the neighbour passes' threads by current cell
   uint32_t* okey = nullptr;
   uint32_t* orank = nullptr;
-  uint32_t* oorder = nullptr;
+  uint32_t* oorder[2] = { nullptr, nullptr };  // thread orders, each counted in the other
   uint32_t* ocount = nullptr;
   uint32_t* opart = nullptr;
   // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled, for nbThreads launch
@@ -112,7 +112,8 @@ struct rtp_batch {
     dev_free(ckVel);
     dev_free(okey);
     dev_free(orank);
-    dev_free(oorder);
+    dev_free(oorder[0]);
+    dev_free(oorder[1]);
     dev_free(ocount);
     dev_free(opart);
     dev_free(nbList);
@@ -216,10 +217,11 @@ struct rtp_batch {
                                         rep));
       in = 1 - in;
       if (resort) {
+        uint32_t* next = nb.order == oorder[0] ? oorder[1] : oorder[0];
         RTP_PROF(prof, "thread_order", s,
-                 launch_counting_order(ocount, m16, okey, orank, total, opart, oorder, s));
-        nb.order = oorder;
-        cb.order = oorder;
+                 launch_counting_order(ocount, m16, okey, nb.order, orank, total, opart, next, s));
+        nb.order = next;
+        cb.order = next;
       }
     }
     if (iters == 0)
@@ -383,7 +385,8 @@ int rtp_batch_create(int device, const rtp_batch_scene* sc, uint32_t count, rtp_
       (rc = dev_alloc(&h->startEnd, h->keySpace)) ||
       (rc = dev_alloc(&h->dSlots, count)) || (rc = dev_alloc(&h->dConsts, count)) ||
       (rc = dev_alloc(&h->dBlocks, maxBlocks)) || (rc = dev_alloc(&h->okey, N)) ||
-      (rc = dev_alloc(&h->orank, N)) || (rc = dev_alloc(&h->oorder, N)) ||
+      (rc = dev_alloc(&h->orank, N)) || (rc = dev_alloc(&h->oorder[0], N)) ||
+      (rc = dev_alloc(&h->oorder[1], N)) ||
       (rc = dev_alloc(&h->ocount, (h->keySpace + 15u) & ~15u)) ||
       (rc = dev_alloc(&h->opart, scan_partials_needed((h->keySpace + 15u) & ~15u)))) {
     h->release();

@@ -242,7 +242,7 @@ struct rtp_fluids {
   // thread order of the neighbour passes (counting sort of the corrected positions' cells)
   uint32_t* okey = nullptr;
   uint32_t* orank = nullptr;
-  uint32_t* oorder = nullptr;
+  uint32_t* oorder[2] = { nullptr, nullptr };  // thread orders, each counted in the other
   uint32_t* ocount = nullptr;
   uint32_t* opart = nullptr;
   // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled
@@ -344,10 +344,11 @@ struct rtp_fluids {
     auto order_threads = [&]() -> int {
       if (!resort)
         return RTP_OK;
+      uint32_t* next = po.order == oorder[0] ? oorder[1] : oorder[0];
       RTP_PROF(prof, "thread_order", s,
-               launch_counting_order(ocount, m16, okey, orank, n, opart, oorder, s));
-      po.order = oorder;
-      pc.order = oorder;
+               launch_counting_order(ocount, m16, okey, po.order, orank, n, opart, next, s));
+      po.order = next;
+      pc.order = next;
       return RTP_OK;
     };
     for (uint32_t k = 0; k < iters; ++k) {
@@ -569,7 +570,8 @@ struct rtp_fluids {
     dev_free(resMetric);
     dev_free(okey);
     dev_free(orank);
-    dev_free(oorder);
+    dev_free(oorder[0]);
+    dev_free(oorder[1]);
     dev_free(ocount);
     dev_free(opart);
     dev_free(nbList);
@@ -871,7 +873,8 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->feat, (size_t)maxN * 8)) ||
       (rc = dev_alloc(&h->lamAcc, maxN)) || (rc = dev_alloc(&h->resDensity, maxN)) ||
       (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey, maxN)) ||
-      (rc = dev_alloc(&h->orank, maxN)) || (rc = dev_alloc(&h->oorder, maxN)) ||
+      (rc = dev_alloc(&h->orank, maxN)) || (rc = dev_alloc(&h->oorder[0], maxN)) ||
+      (rc = dev_alloc(&h->oorder[1], maxN)) ||
       (rc = dev_alloc(&h->ocount, order_count_words(h->grid.g))) ||
       (rc = dev_alloc(&h->opart, scan_partials_needed(order_count_words(h->grid.g))))) {
     h->release();

@@ -325,15 +325,18 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
     p[i] = v;
 }
 
-// Counting-sort ranks: each particle's place among the particles of its cell.  The
-// particles of a block are counted per cell in an LDS hash table first (open addressing
-// over twice as many slots as the block has threads, so a probe always finds the cell or
-// a free slot), and one device atomic per cell of the block reserves their places: far
-// fewer device atomics than one per particle, since a block's particles share few cells.
+// Counting-sort ranks: each particle's place among the particles of its cell.  Thread t
+// counts particle prev[t] (t without a previous order) and keeps its rank at rank[t]: in
+// the previous thread order a block's particles sat in few cells and mostly still do.
+// They are counted per cell in an LDS hash table first (open addressing over twice as many
+// slots as the block has threads, so a probe always finds the cell or a free slot), and
+// one device atomic per cell of the block reserves their places: far fewer device atomics
+// than one per particle.
 constexpr uint32_t kRankThreads = 1024;
 constexpr uint32_t kRankSlots = 2 * kRankThreads;
 constexpr uint32_t kNoKey = 0xffffffffu;  // keys are cell indices below 2^32 - 16
 __global__ __launch_bounds__(kRankThreads) void k_count_ranks(const uint32_t* __restrict__ key,
+                                                              const uint32_t* __restrict__ prev,
                                                               uint32_t n, uint32_t* __restrict__ count,
                                                               uint32_t* __restrict__ rank)
 {
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(kRankThreads) void k_count_ranks(const uint32_t* __
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t k = kNoKey, slot = 0u, local = 0u;
   if (i < n) {
-    k = key[i];
+    k = key[prev ? prev[i] : i];
     slot = (k * 2654435761u) >> 21;  // 11 bits: kRankSlots
     for (;;) {
       const uint32_t old = atomicCAS(&tkey[slot], kNoKey, k);
@@ -365,15 +368,19 @@ __global__ __launch_bounds__(kRankThreads) void k_count_ranks(const uint32_t* __
     rank[i] = tcnt[slot] + local;
 }
 
-// Particle i goes to slot offset[key[i]] + rank[i] of the thread order.
+// The particle i of thread t (i = prev[t]) goes to slot offset[key[i]] + rank[t] of the
+// new thread order.
 __global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ prev,
                                                        const uint32_t* __restrict__ rank,
                                                        const uint32_t* __restrict__ offset, uint32_t n,
                                                        uint32_t* __restrict__ order)
 {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    order[offset[key[i]] + rank[i]] = i;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const uint32_t i = prev ? prev[t] : t;
+    order[offset[key[i]] + rank[t]] = i;
+  }
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
@@ -386,18 +393,20 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
 }
 
-void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, uint32_t* rank,
-                           uint32_t n, uint32_t* partial, uint32_t* order, hipStream_t s)
+void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key,
+                           const uint32_t* prev, uint32_t* rank, uint32_t n, uint32_t* partial,
+                           uint32_t* order, hipStream_t s)
 {
   if (n == 0 || m16 == 0)
     return;
   const uint32_t nseg = (m16 + kScanSeg - 1u) / kScanSeg;
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(m16, 256)), dim3(256), 0, s, count, m16, 0u);
-  hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, kRankThreads)), dim3(kRankThreads), 0, s, key, n, count, rank);
+  hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, kRankThreads)), dim3(kRankThreads), 0, s,
+                     key, prev, n, count, rank);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, count, m16, partial);
-  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, rank, count, n,
-                     order);
+  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, prev, rank,
+                     count, n, order);
 }
 
 uint32_t scan_partials_needed(uint32_t m16) { return (m16 + kScanSeg - 1u) / kScanSeg; }

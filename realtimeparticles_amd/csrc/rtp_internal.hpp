@@ -239,12 +239,14 @@ struct PassOrder {
   __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
 };
 // Counting sort of particles by cell for the thread order: count[] (m16 entries, a
-// multiple of 16 covering every key) is cleared, counts the particles of each cell (ranks
-// into rank[]) and becomes its exclusive scan; then order[scan[key[i]] + rank[i]] = i.
-// partial holds scan_partials_needed(m16) words.
+// multiple of 16 covering every key) is cleared, counts the particles of each cell in the
+// previous thread order prev (the identity when null; ranks into rank[t]) and becomes its
+// exclusive scan; then order[scan[key[i]] + rank[t]] = i for i = prev[t].  order must not
+// be prev.  partial holds scan_partials_needed(m16) words.
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
-void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key, uint32_t* rank,
-                           uint32_t n, uint32_t* partial, uint32_t* order, hipStream_t s);
+void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key,
+                           const uint32_t* prev, uint32_t* rank, uint32_t n, uint32_t* partial,
+                           uint32_t* order, hipStream_t s);
 uint32_t scan_partials_needed(uint32_t m16);
 enum LastMode { kNextIter = 0, kLastVel = 1, kLastOnly = 2 };
 
