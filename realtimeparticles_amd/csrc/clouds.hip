@@ -297,11 +297,39 @@ __device__ __forceinline__ uint32_t cld_ranges(const float4 pos, const uint2* __
   return nr;
 }
 
+// Survivor lists: a walk over some positions records, per particle, the candidates it
+// kept in walk order; a later walk over the same positions and cell table replays them
+// instead of scanning (the candidates it would skip contribute exactly +-0; see negligible
+// below).  Entry j of particle i is entry[j * stride + i]; count[i] is the list's length, or
+// kCldListOverflow when the walk kept more than kCldListCap.
+constexpr uint32_t kCldListCap = 128;
+constexpr uint32_t kCldListOverflow = 0xffffffffu;
+struct CldList {
+  uint32_t* entry = nullptr;
+  uint32_t* count = nullptr;
+  uint32_t stride = 0;
+};
+
+struct CldNoRecord {
+  __device__ __forceinline__ void put(uint32_t) {}
+};
+struct CldRecord {
+  uint32_t* at;  // &entry[i]
+  uint32_t stride;
+  uint32_t n;
+  __device__ __forceinline__ void put(uint32_t v)
+  {
+    if (n < kCldListCap)
+      at[n * stride] = v;
+    ++n;
+  }
+};
+
 // Run the lane's queued bodies in order, kCldDrain entries at a time: their data loaded
-// together, then their bodies.
-template <class L, class F>
+// together, then their bodies; each entry run is also handed to rec.
+template <class L, class F, class R>
 __device__ __forceinline__ void cld_drain(CldQueue& q, const float4 pos, const float4* __restrict__ src,
-                                          const GridConsts& g, L& ld, F& f)
+                                          const GridConsts& g, L& ld, F& f, R& rec)
 {
   for (const uint32_t* rp = q.slot; rp < q.wp; rp += kCldDrain * kCldLanes) {
     uint32_t v[kCldDrain];
@@ -316,23 +344,26 @@ __device__ __forceinline__ void cld_drain(CldQueue& q, const float4 pos, const f
     }
 #pragma unroll
     for (uint32_t k = 0; k < kCldDrain; ++k)
-      if (rp + k * kCldLanes < q.wp)
+      if (rp + k * kCldLanes < q.wp) {
         f(v[k] & kCldIndexMask, x[k], sub4(sub4(pos, o[k]), cld_shift(v[k] >> kCldIndexBits, g)));
+        rec.put(v[k]);
+      }
   }
   q.wp = q.slot;
 }
 
 // The walk: f(e, ld(e), vec) for every candidate e with keep(ld(e), vec), in the order of
-// clouds.cl:322-353, vec = pos - p_e - the wrap shift.  Called by every thread of a block of
-// kCldLanes threads that has a particle (the others return before).
-template <class L, class K, class F>
+// clouds.cl:322-353, vec = pos - p_e - the wrap shift; each candidate run is handed to rec.
+// Called by every thread of a block of kCldLanes threads that has a particle (the others
+// return before); threads with active false walk no cells.
+template <class L, class K, class F, class R>
 __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restrict__ src,
                                          const uint2* __restrict__ se, const GridConsts& g,
-                                         CldLds& lds, L&& ld, K&& keep, F&& f)
+                                         CldLds& lds, bool active, L&& ld, K&& keep, F&& f, R& rec)
 {
   const uint32_t lane = threadIdx.x;
   const uint2* rl = &lds.ranges[lane];
-  const uint32_t nr = cld_ranges(pos, se, g, &lds.ranges[lane]);
+  const uint32_t nr = active ? cld_ranges(pos, se, g, &lds.ranges[lane]) : 0u;
   CldQueue q{ &lds.queue[lane], &lds.queue[lane],
               &lds.queue[lane + (kCldQueue - kCldStep) * kCldLanes] };
   uint32_t rk = 0, e = 1u, hi = 0u, tag = 0u;
@@ -373,10 +404,77 @@ __device__ __forceinline__ void cld_walk(const float4 pos, const float4* __restr
         next_range();
     }
     if (__builtin_amdgcn_ballot_w64(q.wp > q.room) != 0)
-      cld_drain(q, pos, src, g, ld, f);
+      cld_drain(q, pos, src, g, ld, f, rec);
   }
-  cld_drain(q, pos, src, g, ld, f);
+  cld_drain(q, pos, src, g, ld, f, rec);
 }
+
+// A recording walk: the walk, with the lane's list written to L.
+template <class L, class K, class F>
+__device__ __forceinline__ void cld_walk_record(uint32_t i, const float4 pos, const float4* __restrict__ src,
+                                                const uint2* __restrict__ se, const GridConsts& g,
+                                                CldLds& lds, const CldList& list, L&& ld, K&& keep, F&& f)
+{
+  CldRecord rec{ list.entry + i, list.stride, 0u };
+  cld_walk(pos, src, se, g, lds, true, ld, keep, f, rec);
+  list.count[i] = rec.n <= kCldListCap ? rec.n : kCldListOverflow;
+}
+
+// A replaying walk: f over the candidates of particle i's list, in order, kCldDrain at a
+// time.  Lanes whose list overflowed, and every lane when useList is false (some data the
+// bodies multiply may be non-finite or huge, see cld_flag), walk instead.
+template <class L, class K, class F>
+__device__ __forceinline__ void cld_walk_replay(uint32_t i, const float4 pos, const float4* __restrict__ src,
+                                                const uint2* __restrict__ se, const GridConsts& g,
+                                                CldLds& lds, const CldList& list, bool useList, L&& ld,
+                                                K&& keep, F&& f)
+{
+  const uint32_t cnt = useList ? list.count[i] : kCldListOverflow;
+  const bool walk = cnt == kCldListOverflow;
+  if (__builtin_amdgcn_ballot_w64(walk) != 0) {
+    CldNoRecord none;
+    cld_walk(pos, src, se, g, lds, walk, ld, keep, f, none);
+  }
+  if (walk)
+    return;
+  const uint32_t* at = list.entry + i;
+  for (uint32_t j = 0; j < cnt; j += kCldDrain) {
+    uint32_t v[kCldDrain];
+    float4 o[kCldDrain];
+    decltype(ld(0u)) x[kCldDrain];
+#pragma unroll
+    for (uint32_t k = 0; k < kCldDrain; ++k) {
+      v[k] = at[min(j + k, cnt - 1u) * list.stride];
+      o[k] = src[v[k] & kCldIndexMask];
+      x[k] = ld(v[k] & kCldIndexMask);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kCldDrain; ++k)
+      if (j + k < cnt)
+        f(v[k] & kCldIndexMask, x[k], sub4(sub4(pos, o[k]), cld_shift(v[k] >> kCldIndexBits, g)));
+  }
+}
+
+// The data a replaying walk's bodies multiply (lambdaT, lambda, vorticity, the confined
+// velocity) come from the pass before it.  That pass sets bit `bit` of *flags when any of
+// its outputs has a component that is not finite or is 2^60 or more in magnitude: then a
+// candidate left out of a list may not contribute +-0 (for finite data below 2^60 the sums
+// and differences the skip tests evaluate are finite, and so is the length of a 4-vector),
+// and the replaying pass walks.  The flags are cleared at the start of every step.
+__device__ __forceinline__ bool cld_safe(float x) { return fabsf(x) < 0x1p60f; }
+__device__ __forceinline__ bool cld_safe(float4 v)
+{
+  return cld_safe(v.x) & cld_safe(v.y) & cld_safe(v.z) & cld_safe(v.w);
+}
+__device__ __forceinline__ void cld_flag(bool safe, uint32_t* flags, uint32_t bit)
+{
+  const uint64_t bad = __builtin_amdgcn_ballot_w64(!safe);
+  if (bad != 0 && (threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(bad))
+    atomicOr(flags, bit);
+}
+constexpr uint32_t kFlagLambdaT = 1u, kFlagVort = 2u, kFlagVelTmp = 4u;
+constexpr uint32_t kFlagLambda0 = 8u;   // iteration k: kFlagLambda0 << k for k < kFlagIters
+constexpr uint32_t kFlagIters = 29u;
 
 // A candidate at r >= h (d2 >= cullSq, so sqrt(d2) >= h) with a finite distance contributes
 // exactly +-0 to every sum here, as long as the neighbour data its term multiplies are finite:
@@ -433,11 +531,13 @@ __global__ __launch_bounds__(256) void k_cld_thermo(const float4* __restrict__ p
                                                     float4* __restrict__ totCorr,
                                                     uint32_t* __restrict__ key,
                                                     uint2* __restrict__ startEnd, uint32_t n,
-                                                    CloudConsts c)
+                                                    CloudConsts c, uint32_t* __restrict__ flags)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < c.f.g.numCells)
     startEnd[t] = make_uint2(1u, 0u);
+  if (t == 0)
+    *flags = 0u;  // the survivor-list flags of this step's passes
   if (t >= n)
     return;
   const rtp_cloud_params& q = c.q;
@@ -499,7 +599,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda(const float4* __r
                                                          const uint2* __restrict__ se,
                                                          float* __restrict__ lap,
                                                          float* __restrict__ lambdaT, uint32_t n,
-                                                         CloudConsts c)
+                                                         CloudConsts c, CldList list,
+                                                         uint32_t* __restrict__ flags)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -512,7 +613,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda(const float4* __r
   const auto keep = [&](float te, float4 vec) {
     return !(negligible(vec, c.f) & __builtin_isfinite(temp - te));
   };
-  cld_walk(pos, posP, se, c.f.g, lds, [&](uint32_t e) { return tempP[e]; }, keep, [&](uint32_t, float te, float4 vec) {
+  cld_walk_record(i, pos, posP, se, c.f.g, lds, list, [&](uint32_t e) { return tempP[e]; }, keep, [&](uint32_t, float te, float4 vec) {
     const float dTemp = temp - te;
     const float vv = dot4(vec, vec);
     const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
@@ -525,7 +626,9 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda(const float4* __r
   const float l = laplacian / rho0;
   lap[i] = l;
   sumSqGradC = sumSqGradC + sumGradCi * sumGradCi;
-  lambdaT[i] = -l / (sumSqGradC + c.q.relaxCFM);
+  const float lt = -l / (sumSqGradC + c.q.relaxCFM);
+  lambdaT[i] = lt;
+  cld_flag(cld_safe(lt), flags, kFlagLambdaT);
 }
 
 // cld_computeConstraintCorrectionTemp (clouds.cl:654-721) and cld_correctTemperature
@@ -534,7 +637,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_correct(const float4* __
                                                           const float* __restrict__ lambdaT,
                                                           const uint2* __restrict__ se,
                                                           float* __restrict__ temp, uint32_t n,
-                                                          CloudConsts c)
+                                                          CloudConsts c, CldList list,
+                                                          const uint32_t* __restrict__ flags, int useList)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -547,7 +651,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_correct(const float4* __
   const auto keep = [&](float le, float4 vec) {
     return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
   };
-  cld_walk(pos, posP, se, c.f.g, lds, [&](uint32_t e) { return lambdaT[e]; }, keep, [&](uint32_t, float le, float4 vec) {
+  cld_walk_replay(i, pos, posP, se, c.f.g, lds, list, useList && !(*flags & kFlagLambdaT),
+                  [&](uint32_t e) { return lambdaT[e]; }, keep, [&](uint32_t, float le, float4 vec) {
     const float l = lambdaI + le;
     const float vv = dot4(vec, vec);
     const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
@@ -562,7 +667,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda(const float4* 
                                                             const uint2* __restrict__ se,
                                                             float* __restrict__ density,
                                                             float* __restrict__ lambda, uint32_t n,
-                                                            CloudConsts c)
+                                                            CloudConsts c, CldList list,
+                                                            uint32_t* __restrict__ flags, uint32_t bit)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -574,7 +680,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda(const float4* 
   float sumSqGradC = 0.0f;
   __shared__ CldLds lds;
   const auto keep = [&](int, float4 vec) { return !negligible(vec, c.f); };
-  cld_walk(pos, pred, se, c.f.g, lds, [](uint32_t) { return 0; }, keep, [&](uint32_t, int, float4 vec) {
+  cld_walk_record(i, pos, pred, se, c.f.g, lds, list, [](uint32_t) { return 0; }, keep, [&](uint32_t, int, float4 vec) {
     const float r = len_of(dot4(vec, vec));
     fluidDensity = fluidDensity + poly6_r(r, c.f);
     const float4 grad = grad_spiky_fast(vec, r, c.f);
@@ -586,7 +692,10 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda(const float4* 
   const float densityC = fluidDensity / p.restDensity - 1.0f;
   sumSqGradC = sumSqGradC + dot4(sumGradCi, sumGradCi);
   sumSqGradC = sumSqGradC / (p.restDensity * p.restDensity);
-  lambda[i] = -densityC / (sumSqGradC + p.relaxCFM);
+  const float li = -densityC / (sumSqGradC + p.relaxCFM);
+  lambda[i] = li;
+  if (bit)
+    cld_flag(cld_safe(li), flags, bit);
 }
 
 // cld_computeConstraintCorrection (clouds.cl:441-502), cld_correctPosition on p_predPos and
@@ -598,7 +707,9 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_correct(const float4* __restr
                                                      float4* __restrict__ predOut,
                                                      float4* __restrict__ totCorr,
                                                      float4* __restrict__ vel, uint32_t n,
-                                                     CloudConsts c)
+                                                     CloudConsts c, CldList list,
+                                                     const uint32_t* __restrict__ flags, uint32_t bit,
+                                                     int useList)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -611,7 +722,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_correct(const float4* __restr
   const auto keep = [&](float le, float4 vec) {
     return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
   };
-  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return lambda[e]; }, keep, [&](uint32_t, float le, float4 vec) {
+  cld_walk_replay(i, pos, pred, se, c.f.g, lds, list, useList && bit && !(*flags & bit),
+                  [&](uint32_t e) { return lambda[e]; }, keep, [&](uint32_t, float le, float4 vec) {
     const float r = len_of(dot4(vec, vec));
     const float s = lambdaI + le + art_pressure_w(poly6_r(r, c.f), ywdq, c.f);
     corr = add4(corr, muls4(s, grad_spiky_fast(vec, r, c.f)));
@@ -647,7 +759,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_vorticity(const float4* __res
                                                        const uint2* __restrict__ se,
                                                        const float4* __restrict__ vel,
                                                        float4* __restrict__ vort, uint32_t n,
-                                                       CloudConsts c)
+                                                       CloudConsts c, CldList list,
+                                                       uint32_t* __restrict__ flags)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -657,11 +770,12 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_vorticity(const float4* __res
   float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   __shared__ CldLds lds;
   const auto keep = [&](float4 ve, float4 vec) { return !(negligible(vec, c.f) & finite4(sub4(ve, velocity))); };
-  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return vel[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
+  cld_walk_record(i, pos, pred, se, c.f.g, lds, list, [&](uint32_t e) { return vel[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
     const float4 dv = sub4(ve, velocity);
     w = add4(w, cross4(dv, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
   });
   vort[i] = w;
+  cld_flag(cld_safe(w), flags, kFlagVort);
 }
 
 // cld_applyVorticityConfinement (clouds.cl:790-850), written to the copy XSPH reads
@@ -671,7 +785,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_confinement(const float4* __r
                                                          const float4* __restrict__ vort,
                                                          const float4* __restrict__ vel,
                                                          float4* __restrict__ velOut, uint32_t n,
-                                                         CloudConsts c)
+                                                         CloudConsts c, CldList list,
+                                                         uint32_t* __restrict__ flags, int useList)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -684,12 +799,15 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_confinement(const float4* __r
     // sqrt keeps finiteness: isfinite(length(we)) == isfinite(dot4(we, we))
     return !(negligible(vec, c.f) & __builtin_isfinite(dot4(we, we)));
   };
-  cld_walk(pos, pred, se, c.f.g, lds, [&](uint32_t e) { return vort[e]; }, keep, [&](uint32_t, float4 we, float4 vec) {
+  cld_walk_replay(i, pos, pred, se, c.f.g, lds, list, useList && !(*flags & kFlagVort),
+                  [&](uint32_t e) { return vort[e]; }, keep, [&](uint32_t, float4 we, float4 vec) {
     const float lw = len_of(dot4(we, we));
     nn = add4(nn, muls4(lw, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
   });
   const rtp_fluid_params& p = c.f.p;
-  velOut[i] = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
+  const float4 vo = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
+  velOut[i] = vo;
+  cld_flag(cld_safe(vo), flags, kFlagVelTmp);
 }
 
 // cld_updatePosition (clouds.cl:942-952): the predicted position plus the wind.
@@ -709,7 +827,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_xsph(const float4* __restrict
                                                   const float4* __restrict__ velIn,
                                                   float4* __restrict__ velOut,
                                                   float4* __restrict__ pos, uint32_t n,
-                                                  CloudConsts c)
+                                                  CloudConsts c, CldList list,
+                                                  const uint32_t* __restrict__ flags, int useList)
 {
   const uint32_t i = logical_block(1) * kCldLanes + threadIdx.x;
   if (i >= n)
@@ -719,7 +838,8 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_xsph(const float4* __restrict
   float4 viscosity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   __shared__ CldLds lds;
   const auto keep = [&](float4 ve, float4 vec) { return !(negligible(vec, c.f) & finite4(sub4(ve, velocity))); };
-  cld_walk(pp, pred, se, c.f.g, lds, [&](uint32_t e) { return velIn[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
+  cld_walk_replay(i, pp, pred, se, c.f.g, lds, list, useList && !(*flags & kFlagVelTmp),
+                  [&](uint32_t e) { return velIn[e]; }, keep, [&](uint32_t, float4 ve, float4 vec) {
     const float4 dv = sub4(ve, velocity);
     viscosity = add4(viscosity, mul4s(dv, poly6_r(len_of(dot4(vec, vec)), c.f)));
   });
@@ -780,6 +900,8 @@ struct rtp_clouds {
   float* lambda = nullptr;
   float* lap = nullptr;
   float* lambdaT = nullptr;
+  CldList list;              // survivor lists of the recording passes (kCldListCap per slot)
+  uint32_t* flags = nullptr;  // cld_flag bits of the step
   int cur = 0;  // state in [cur]
   int pc = 0;   // predicted positions in pred[pc]
   Profiler prof;
@@ -804,6 +926,9 @@ struct rtp_clouds {
     dev_free(lambda);
     dev_free(lap);
     dev_free(lambdaT);
+    dev_free(list.entry);
+    dev_free(list.count);
+    dev_free(flags);
     prof.destroy();
     if (stream)
       (void)hipStreamDestroy(stream);
@@ -818,7 +943,7 @@ struct rtp_clouds {
     RTP_PROF(prof, "cld_thermo_predict", s,
              hipLaunchKernelGGL(k_cld_thermo, dim3(b256), dim3(256), 0, s, pos[cur], vel[cur],
                                 temp[cur], vapor[cur], cloud[cur], buoy[cur], pred[pc], tot[cur],
-                                G.key[G.keyCur], G.startEnd, n, c));
+                                G.key[G.keyCur], G.startEnd, n, c, flags));
     int rc = G.sort(n, s, prof);
     if (rc)
       return rc;
@@ -856,22 +981,27 @@ struct rtp_clouds {
     const int gathered = pc;
     const uint32_t bn = blocks_for(std::max(n, 1u), 256);
     const uint32_t bw = blocks_for(std::max(n, 1u), kCldLanes);  // the walk kernels
+    // Replays need every candidate a walk may visit to have data from this step: not so
+    // when the sorted range holds stale particles past n (after the count shrank).
+    const int useList = G.sortedN <= n;
     if (n && c.q.isTempSmoothingEnabled) {
       RTP_PROF(prof, "cld_temp_lambda", s,
                hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], temp[cur],
-                                  G.startEnd, lap, lambdaT, n, c));
+                                  G.startEnd, lap, lambdaT, n, c, list, flags));
       RTP_PROF(prof, "cld_temp_correct", s,
                hipLaunchKernelGGL(k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], lambdaT,
-                                  G.startEnd, temp[cur], n, c));
+                                  G.startEnd, temp[cur], n, c, list, flags, useList));
     }
     for (uint32_t k = 0; k < iters && n; ++k) {
+      const uint32_t bit = k < kFlagIters ? kFlagLambda0 << k : 0u;
       RTP_PROF(prof, "cld_density_lambda", s,
                hipLaunchKernelGGL(k_cld_density_lambda, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
-                                  G.startEnd, density, lambda, n, c));
+                                  G.startEnd, density, lambda, n, c, list, flags, bit));
       RTP_PROF(prof, "cld_correct", s,
                hipLaunchKernelGGL(k_cld_correct, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], lambda,
                                   G.startEnd, pred[1 - pc], tot[cur],
-                                  k + 1 == iters ? vel[cur] : nullptr, n, c));
+                                  k + 1 == iters ? vel[cur] : nullptr, n, c, list, flags, bit,
+                                  useList));
       pc = 1 - pc;
     }
     if (iters == 0 && n)
@@ -885,13 +1015,13 @@ struct rtp_clouds {
     if (c.f.p.isVorticityConfEnabled) {
       RTP_PROF(prof, "cld_vorticity", s,
                hipLaunchKernelGGL(k_cld_vorticity, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
-                                  vel[cur], vort, n, c));
+                                  vel[cur], vort, n, c, list, flags));
       RTP_PROF(prof, "cld_confinement", s,
                hipLaunchKernelGGL(k_cld_confinement, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
-                                  G.startEnd, vort, vel[cur], velTmp, n, c));
+                                  G.startEnd, vort, vel[cur], velTmp, n, c, list, flags, useList));
       RTP_PROF(prof, "cld_xsph", s,
                hipLaunchKernelGGL(k_cld_xsph, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
-                                  velTmp, vel[cur], pos[cur], n, c));
+                                  velTmp, vel[cur], pos[cur], n, c, list, flags, useList));
     } else {
       RTP_PROF(prof, "cld_update_pos", s,
                hipLaunchKernelGGL(k_cld_update_pos, dim3(bn), dim3(256), 0, s, pred[pc], pos[cur],
@@ -971,7 +1101,10 @@ int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
   if (!rc)
     (void)((rc = dev_alloc(&h->vort, maxN)) || (rc = dev_alloc(&h->velTmp, maxN)) ||
            (rc = dev_alloc(&h->density, maxN)) || (rc = dev_alloc(&h->lambda, maxN)) ||
-           (rc = dev_alloc(&h->lap, maxN)) || (rc = dev_alloc(&h->lambdaT, maxN)));
+           (rc = dev_alloc(&h->lap, maxN)) || (rc = dev_alloc(&h->lambdaT, maxN)) ||
+           (rc = dev_alloc(&h->list.entry, (size_t)kCldListCap * maxN)) ||
+           (rc = dev_alloc(&h->list.count, maxN)) || (rc = dev_alloc(&h->flags, 1)));
+  h->list.stride = maxN;
   if (rc) {
     h->release();
     delete h;

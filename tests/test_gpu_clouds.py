@@ -190,3 +190,45 @@ def test_capped_cell_and_wall_clusters_bit_exact():
             _, _, se = gpu.read_grid()
             assert (se[:, 1].astype(np.int64) - se[:, 0] == 100).any()  # a capped cell
     gpu.close()
+
+
+def test_dense_ball_overflows_the_survivor_lists_bit_exact():
+    """600 particles in a ball of radius 0.2 around a cell corner (about 75 per cell, under
+    the cap of 100): each keeps several hundred neighbours, more than a survivor list holds
+    (128), so the passes that replay lists walk for those particles."""
+    sc = scenes.cloud_scene("clouds-65k", seed=41)
+    rng = np.random.default_rng(13)
+    d = rng.normal(size=(600, 3))
+    d *= (0.2 * rng.uniform(0.0, 1.0, (600, 1)) ** (1.0 / 3.0)) / np.linalg.norm(d, axis=1, keepdims=True)
+    ball = np.zeros((600, 4), np.float32)
+    ball[:, :3] = np.array([1.0, -6.0, 1.0]) + d  # (1, -6, 1) is a corner of cells of size 1/3
+    pos = np.concatenate([sc.pos[:64000], ball])
+    sc.pos, sc.vel, sc.n = pos, np.zeros_like(pos), pos.shape[0]
+    gpu, orc = make_pair(sc)
+    for step in range(1, 4):
+        gpu.update(1)
+        orc.step(1)
+        log(f"dense ball, step {step}")
+        compare(gpu, orc, sc.n, f"step {step}", grid=(step == 3))
+    _, _, se = gpu.read_grid()
+    assert (se[:, 1].astype(np.int64) - se[:, 0] + 1).max() <= 100  # no cell was capped
+    gpu.close()
+
+
+def test_infinite_velocity_spreads_nan_bit_exact():
+    """One particle with an infinite velocity: its predicted position is infinite, so the
+    gradient terms of its neighbours turn NaN and so do the lambda, vorticity and confined
+    velocity buffers the replaying passes multiply.  Those passes then walk instead of
+    replaying their lists (a candidate left out of a list need not contribute +-0 with
+    non-finite data), and every buffer still equals the oracle's, NaN for NaN."""
+    sc = scenes.cloud_scene("clouds-65k", seed=43)
+    sc.vel = np.zeros_like(sc.pos)
+    sc.vel[1234, 0] = np.inf
+    gpu, orc = make_pair(sc)
+    for step in range(1, 3):
+        gpu.update(1)
+        orc.step(1)
+        log(f"infinite velocity, step {step}")
+        compare(gpu, orc, sc.n, f"step {step}", grid=(step == 2))
+    assert np.isnan(gpu.read_field(capi.FIELD_LAMBDA)).any()
+    gpu.close()
