@@ -57,6 +57,20 @@ def test_argument_checks_do_not_need_a_device():
     assert L.rtp_boids_update(None, 1) == -1
 
 
+def test_clouds_slot_limit_is_checked_before_the_device():
+    """A Clouds survivor-list entry packs the particle index into 28 bits: the model refuses
+    2^28 slots or more with an argument error, before it looks for a device."""
+    d = capi.ModelDesc()
+    d.maxNbParticles = 1 << 28
+    d.currNbParticles = 0
+    for k in range(3):
+        d.boxSize[k] = 10.0
+        d.gridRes[k] = 30
+    h = ctypes.c_void_p()
+    assert capi.lib().rtp_clouds_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"2^28" in capi.lib().rtp_last_error()
+
+
 @pytest.mark.skipif(capi.device_count() > 0, reason="a GPU is visible")
 def test_no_cpu_fallback_without_gpu():
     with pytest.raises(capi.RtpError, match="no HIP device"):
