@@ -54,63 +54,67 @@ __global__ __launch_bounds__(256) void k_boids_unit_vel(const float4* __restrict
   unitVel[i] = make_float4(u.x, u.y, u.z, 0.0f);
 }
 
-// Boid i of the sorted state: the rules over its neighbours, the velocity update and the
-// wall (or periodic) boundary, written to the other pair of buffers.
-template <int DIM2>
-__device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __restrict__ position,
-                                                const float4* __restrict__ velocity,
-                                                const float4* __restrict__ unitVel,
-                                                const uint2* __restrict__ se,
-                                                float4* __restrict__ posOut,
-                                                float4* __restrict__ velOut,
-                                                float4* __restrict__ accOut, const BoidsConsts& c)
+// The neighbour sums of boids.cl:96-113, in the walk's order.
+struct BoidsSums {
+  int count = 0;
+  V3 avgPos{ 0.0f, 0.0f, 0.0f }, avgVel{ 0.0f, 0.0f, 0.0f }, rep{ 0.0f, 0.0f, 0.0f };
+};
+
+// boids.cl:100-103: the squared distance of a candidate, and whether it is a neighbour.
+__device__ __forceinline__ float boids_sq(const float4 p, const float4 q)
 {
-  const float4 p = position[i];
+  const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
+  return vx * vx + vy * vy + vz * vz;
+}
+__device__ __forceinline__ bool boids_near(float sq, const BoidsConsts& c)
+{
+  return (sq < c.radiusSq) & (sq > kFloatEps);
+}
+
+// boids.cl:104-112 for a neighbour at q with unit velocity nv.
+__device__ __forceinline__ void boids_pair(const float4 p, const float4 q, const float4 nv,
+                                           BoidsSums& S, const BoidsConsts& c)
+{
+  const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
+  const float sq = vx * vx + vy * vy + vz * vz;
+  S.avgPos.x = S.avgPos.x + q.x;
+  S.avgPos.y = S.avgPos.y + q.y;
+  S.avgPos.z = S.avgPos.z + q.z;
+  S.avgVel.x = S.avgVel.x + nv.x;
+  S.avgVel.y = S.avgVel.y + nv.y;
+  S.avgVel.z = S.avgVel.z + nv.z;
+  // vec / squaredDist: sq lies in (1e-8, radiusSq] with radiusSq <= 2^30 (checked on the
+  // host), so the short division applies unless a numerator is a nonzero value under 2^-90
+  const float y = rcp_refined(sq);
+  float qx = div_short(vx, sq, y), qy = div_short(vy, sq, y), qz = div_short(vz, sq, y);
+  const bool general = !c.shortDiv || tiny_numerator3(vx, vy, vz);
+  if (__builtin_amdgcn_ballot_w64(general) != 0) {
+    rare_path();
+    if (general) {
+      qx = vx / sq;
+      qy = vy / sq;
+      qz = vz / sq;
+    }
+  }
+  S.rep.x = S.rep.x + qx;
+  S.rep.y = S.rep.y + qy;
+  S.rep.z = S.rep.z + qz;
+  ++S.count;
+}
+
+// The cells of boids.cl:64-95 (3-D: the 27 cells inside the walls, x outer, z inner) or
+// :170-182 (2-D: the YZ neighbours in the x layer RX/2, every coordinate checked against RX
+// and the index formed as (RX/2 * RX + y) * RY + z; on the cubic grids the reference runs
+// in 2-D these are the grid's own bounds and index; an index past the table, a non-cubic
+// grid where the reference reads out of bounds, is skipped), as visit(cell, z-column, k):
+// column `col` holds up to three cells k = 0..2 of consecutive indices.
+template <int DIM2, class V>
+__device__ __forceinline__ void boids_cells(const float4 p, const BoidsConsts& c, V&& visit)
+{
   uint32_t cx, cy, cz;
   cell3d(p, c.g, cx, cy, cz);
   const int32_t rx = c.g.res[0], ry = c.g.res[1], rz = c.g.res[2];
-  int count = 0;
-  V3 avgPos{ 0.0f, 0.0f, 0.0f }, avgVel{ 0.0f, 0.0f, 0.0f }, rep{ 0.0f, 0.0f, 0.0f };
-  auto visit = [&](uint32_t cell) {
-    const uint2 r = se[cell];
-    for (uint32_t e = r.x; e <= r.y; ++e) {
-      const float4 q = position[e];
-      const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
-      const float sq = vx * vx + vy * vy + vz * vz;
-      if (sq < c.radiusSq && sq > kFloatEps) {  // boids.cl:102-103
-        avgPos.x = avgPos.x + q.x;
-        avgPos.y = avgPos.y + q.y;
-        avgPos.z = avgPos.z + q.z;
-        const float4 nv = unitVel[e];
-        avgVel.x = avgVel.x + nv.x;
-        avgVel.y = avgVel.y + nv.y;
-        avgVel.z = avgVel.z + nv.z;
-        // vec / squaredDist: sq lies in (1e-8, radiusSq] with radiusSq <= 2^30 (checked on
-        // the host), so the short division applies unless a numerator is a nonzero value
-        // under 2^-90
-        const float y = rcp_refined(sq);
-        float qx = div_short(vx, sq, y), qy = div_short(vy, sq, y), qz = div_short(vz, sq, y);
-        const bool general = !c.shortDiv || tiny_numerator3(vx, vy, vz);
-        if (__builtin_amdgcn_ballot_w64(general) != 0) {
-          rare_path();
-          if (general) {
-            qx = vx / sq;
-            qy = vy / sq;
-            qz = vz / sq;
-          }
-        }
-        rep.x = rep.x + qx;
-        rep.y = rep.y + qy;
-        rep.z = rep.z + qz;
-        ++count;
-      }
-    }
-  };
   if (DIM2) {
-    // boids.cl:170-182: the YZ neighbours in the x layer RX/2, every coordinate checked
-    // against RX and the index formed as (RX/2 * RX + y) * RY + z.  On the cubic grids the
-    // reference runs in 2-D these are the grid's own bounds and index; an index past the
-    // table (a non-cubic grid, where the reference reads out of bounds) is skipped.
     const int32_t nx = (int32_t)cx;
     for (int iy = -1; iy <= 1; ++iy)
       for (int iz = -1; iz <= 1; ++iz) {
@@ -119,7 +123,7 @@ __device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __rest
           continue;
         const uint32_t cell = (uint32_t)((rx / 2 * rx + ny) * ry + nz);
         if (cell < c.g.numCells)
-          visit(cell);
+          visit(cell, iy + 1, iz + 1);
       }
   } else {
     for (int ix = -1; ix <= 1; ++ix) {
@@ -134,11 +138,146 @@ __device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __rest
           const int32_t nz = (int32_t)cz + iz;
           if (nz < 0 || nz >= rz)
             continue;
-          visit((uint32_t)((nx * ry + ny) * rz + nz));
+          visit((uint32_t)((nx * ry + ny) * rz + nz), (ix + 1) * 3 + iy + 1, iz + 1);
         }
       }
     }
   }
+}
+
+// The neighbour sums with each pair's body run in place (the persistent update's walk).
+template <int DIM2>
+__device__ __forceinline__ void boids_sums_in_place(const float4 p, const float4* __restrict__ position,
+                                                    const float4* __restrict__ unitVel,
+                                                    const uint2* __restrict__ se,
+                                                    const BoidsConsts& c, BoidsSums& S)
+{
+  boids_cells<DIM2>(p, c, [&](uint32_t cell, int, int) {
+    const uint2 r = se[cell];
+    for (uint32_t e = r.x; e <= r.y; ++e) {
+      const float4 q = position[e];
+      if (boids_near(boids_sq(p, q), c))
+        boids_pair(p, q, unitVel[e], S, c);
+    }
+  });
+}
+
+// ---------------------------------------------------------- the walk with survivor queues
+// A wave whose lanes ran each pair's body in place would run it for almost every candidate
+// step with few lanes on (a boid keeps about one candidate in six).  Instead each lane lists
+// its candidate ranges in walk order (the cells of a z-column merged into one range when
+// their non-empty ranges are back to back, which visits the same candidates in the same
+// order), the wave steps through every lane's list together, kBoidsStep candidates per lane
+// a step, each lane queueing the neighbours it finds in its LDS column, and when any lane's
+// queue is nearly full, and at the end, every lane runs its queued bodies in order.  Each
+// lane's sums therefore add the same terms in the same order.
+constexpr uint32_t kBoidsQueue = 32;   // queue entries per lane
+constexpr uint32_t kBoidsRanges = 27;  // at most one range per cell
+constexpr uint32_t kBoidsStep = 4;     // candidates per lane per scan step
+constexpr uint32_t kBoidsDrain = 4;    // queued bodies per drain group
+
+// Dynamic LDS of k_boids_step: lane l's queue entries at l, l + B, ... (B threads per block),
+// then its ranges likewise.
+inline size_t boids_step_lds(uint32_t block)
+{
+  return (size_t)block * (kBoidsQueue * sizeof(uint32_t) + kBoidsRanges * sizeof(uint2));
+}
+
+template <int DIM2>
+__device__ __forceinline__ void boids_sums_queued(const float4 p, const float4* __restrict__ position,
+                                                  const float4* __restrict__ unitVel,
+                                                  const uint2* __restrict__ se,
+                                                  const BoidsConsts& c, BoidsSums& S)
+{
+  extern __shared__ uint32_t boidsLds[];
+  const uint32_t B = blockDim.x;
+  uint32_t* const qslot = boidsLds + threadIdx.x;
+  uint2* const rl = reinterpret_cast<uint2*>(boidsLds + kBoidsQueue * B) + threadIdx.x;
+  // the lane's ranges
+  uint32_t nr = 0, lo = 1u, hi = 0u;
+  int lastCol = -1;
+  boids_cells<DIM2>(p, c, [&](uint32_t cell, int col, int) {
+    const uint2 r = se[cell];
+    if (r.x > r.y)
+      return;
+    if (col == lastCol && lo <= hi && r.x == hi + 1u) {
+      hi = r.y;  // back to back with the column's range so far
+      return;
+    }
+    if (lo <= hi)
+      rl[(nr++) * B] = make_uint2(lo, hi);
+    lo = r.x;
+    hi = r.y;
+    lastCol = col;
+  });
+  if (lo <= hi)
+    rl[(nr++) * B] = make_uint2(lo, hi);
+  // the scan
+  uint32_t* wp = qslot;
+  uint32_t* const room = qslot + (kBoidsQueue - kBoidsStep) * B;
+  uint32_t rk = 0, e = 1u;
+  hi = 0u;
+  auto next_range = [&]() {
+    if (rk < nr) {
+      const uint2 r = rl[(rk++) * B];
+      e = r.x;
+      hi = r.y;
+    }
+  };
+  auto drain = [&]() {
+    for (const uint32_t* rp = qslot; rp < wp; rp += kBoidsDrain * B) {
+      uint32_t v[kBoidsDrain];
+      float4 q[kBoidsDrain], nv[kBoidsDrain];
+#pragma unroll
+      for (uint32_t k = 0; k < kBoidsDrain; ++k) {
+        const uint32_t* pk = rp + k * B;
+        v[k] = *(pk < wp ? pk : qslot);  // in bounds either way
+        q[k] = position[v[k]];
+        nv[k] = unitVel[v[k]];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kBoidsDrain; ++k)
+        if (rp + k * B < wp)
+          boids_pair(p, q[k], nv[k], S, c);
+    }
+    wp = qslot;
+  };
+  next_range();
+  for (;;) {
+    const bool on = e <= hi;
+    if (__builtin_amdgcn_ballot_w64(on) == 0)
+      break;
+    if (on) {
+      float4 q[kBoidsStep];
+#pragma unroll
+      for (uint32_t k = 0; k < kBoidsStep; ++k)
+        q[k] = position[min(e + k, hi)];  // the loads past hi repeat hi's
+#pragma unroll
+      for (uint32_t k = 0; k < kBoidsStep; ++k) {
+        *wp = e + k;
+        wp += (boids_near(boids_sq(p, q[k]), c) & (e + k <= hi)) ? B : 0u;
+      }
+      e += kBoidsStep;
+      if (e > hi)
+        next_range();
+    }
+    if (__builtin_amdgcn_ballot_w64(wp > room) != 0)
+      drain();
+  }
+  drain();
+}
+
+// Boid i of the sorted state: the rules over its neighbours (sums S), the velocity update
+// and the wall (or periodic) boundary, written to the other pair of buffers.
+template <int DIM2>
+__device__ __forceinline__ void boids_finish(uint32_t i, const float4 p, BoidsSums S,
+                                             const float4* __restrict__ velocity,
+                                             float4* __restrict__ posOut,
+                                             float4* __restrict__ velOut,
+                                             float4* __restrict__ accOut, const BoidsConsts& c)
+{
+  const int count = S.count;
+  V3 avgPos = S.avgPos, avgVel = S.avgVel, rep = S.rep;
   V3 acc{ 0.0f, 0.0f, 0.0f };
   if (count != 0) {  // boids.cl:115-129
     const float fc = (float)count;
@@ -192,7 +331,25 @@ __device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __rest
   velOut[i] = make_float4(vel.x, vel.y, vel.z, 0.0f);
 }
 
+// The persistent update's per-boid step: sums with bodies in place, then the update.
 template <int DIM2>
+__device__ __forceinline__ void boids_step_body(uint32_t i, const float4* __restrict__ position,
+                                                const float4* __restrict__ velocity,
+                                                const float4* __restrict__ unitVel,
+                                                const uint2* __restrict__ se,
+                                                float4* __restrict__ posOut,
+                                                float4* __restrict__ velOut,
+                                                float4* __restrict__ accOut, const BoidsConsts& c)
+{
+  const float4 p = position[i];
+  BoidsSums S;
+  boids_sums_in_place<DIM2>(p, position, unitVel, se, c, S);
+  boids_finish<DIM2>(i, p, S, velocity, posOut, velOut, accOut, c);
+}
+
+// QUEUED: the walk with survivor queues (its LDS is boids_step_lds(blockDim.x)), else
+// the bodies in place.
+template <int DIM2, int QUEUED>
 __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ position,
                                                     const float4* __restrict__ velocity,
                                                     const float4* __restrict__ unitVel,
@@ -203,8 +360,32 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
                                                     BoidsConsts c, int remap)
 {
   const uint32_t i = logical_block(remap) * blockDim.x + threadIdx.x;
-  if (i < n)
-    boids_step_body<DIM2>(i, position, velocity, unitVel, se, posOut, velOut, accOut, c);
+  if (i >= n)
+    return;
+  const float4 p = position[i];
+  BoidsSums S;
+  if (QUEUED)
+    boids_sums_queued<DIM2>(p, position, unitVel, se, c, S);
+  else
+    boids_sums_in_place<DIM2>(p, position, unitVel, se, c, S);
+  boids_finish<DIM2>(i, p, S, velocity, posOut, velOut, accOut, c);
+}
+
+// Whether the current device gives one workgroup `bytes` of LDS (the limit is read once per
+// device).
+bool lds_fits(size_t bytes)
+{
+  static int limit[64] = {};  // 0 unknown
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return false;
+  if (limit[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = -1;
+    limit[dev] = v;
+  }
+  return limit[dev] > 0 && bytes <= (size_t)limit[dev];
 }
 
 // ----------------------------------------------------------------- persistent update
@@ -339,12 +520,25 @@ void launch_boids_step(const float4* pos, const float4* vel, float4* unitVel,
   // active ones
   hipLaunchKernelGGL(k_boids_unit_vel, dim3(blocks_for(sortN, 256)), dim3(256), 0, s, vel,
                      unitVel, sortN);
-  if (c.dim2)
-    hipLaunchKernelGGL(k_boids_step<1>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
-                       vel, unitVel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
-  else
-    hipLaunchKernelGGL(k_boids_step<0>, dim3(blocks_for(n, L.block)), dim3(L.block), 0, s, pos,
-                       vel, unitVel, startEnd, posOut, velOut, acc, n, c, L.xcdRemap);
+  // the survivor-queue walk where its LDS fits in a workgroup, else bodies in place
+  const size_t lds = boids_step_lds(L.block);
+  const bool queued = lds_fits(lds);
+  const dim3 grid(blocks_for(n, L.block)), block(L.block);
+  if (c.dim2) {
+    if (queued)
+      hipLaunchKernelGGL((k_boids_step<1, 1>), grid, block, lds, s, pos, vel, unitVel, startEnd,
+                         posOut, velOut, acc, n, c, L.xcdRemap);
+    else
+      hipLaunchKernelGGL((k_boids_step<1, 0>), grid, block, 0, s, pos, vel, unitVel, startEnd,
+                         posOut, velOut, acc, n, c, L.xcdRemap);
+  } else {
+    if (queued)
+      hipLaunchKernelGGL((k_boids_step<0, 1>), grid, block, lds, s, pos, vel, unitVel, startEnd,
+                         posOut, velOut, acc, n, c, L.xcdRemap);
+    else
+      hipLaunchKernelGGL((k_boids_step<0, 0>), grid, block, 0, s, pos, vel, unitVel, startEnd,
+                         posOut, velOut, acc, n, c, L.xcdRemap);
+  }
 }
 
 int launch_boids_persistent(const BoidsPersistentArgs& a, const BoidsConsts& c, uint32_t groups,
