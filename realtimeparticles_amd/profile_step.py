@@ -9,7 +9,9 @@ launch time, to set beside the rocprofv3 trace of the same command.
 
 A Clouds workload (clouds-65k: the cumulus case) runs the Clouds model's steps from the
 input instead, as bench.py's Clouds section does: `--warmup` untimed steps, then `--steps`
-timed steps from the input, then the same with per-kernel HIP events.
+timed steps from the input, then the same with per-kernel HIP events.  The boids workloads
+(boids-8k: the 8,192-boid flock; boids-131k: the reference app's 131,072 boids with its
+default rules and dt 0.1) do the same for bench.py's boids runs, at `--block` and `--flags`.
 
     rocprofv3 --kernel-trace --stats -d OUT -o run -- \\
         python3 -m realtimeparticles_amd.profile_step --workload pbf-1m --block 64 --flags 13 \\
@@ -37,6 +39,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.workload.startswith("clouds"):
         return clouds_main(a)
+    if a.workload.startswith("boids"):
+        return boids_main(a)
 
     sc = scenes.pbf_scene(a.workload)
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
@@ -96,6 +100,46 @@ def clouds_main(a):
     m.close()
     print(json.dumps({
         "workload": a.workload, "particles": sc.n, "steps": a.steps,
+        "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
+        "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
+                    for k, v in sorted(prof.items())}}), flush=True)
+
+
+def boids_main(a):
+    if a.workload == "boids-8k":
+        sc, prm = scenes.boids_flock(8192), None
+    elif a.workload == "boids-131k":
+        sc, prm = scenes.boids_flock(131072, seed=3, box=10.0, res=30), (0.5, 1.6, 1.6, 1.45, 0.1)
+    else:
+        raise SystemExit(f"unknown workload {a.workload}")
+    m = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
+    if prm is None:
+        m.set_params(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
+        m.set_time_step(sc.time_step)
+    else:
+        m.set_params(*prm[:4])
+        m.set_time_step(prm[4])
+    m.set_tuning(a.block, a.flags)
+
+    def from_input(k):
+        m.load_state(sc.pos, sc.vel)
+        m.update(k)
+        m.sync()
+
+    from_input(a.warmup)
+    m.load_state(sc.pos, sc.vel)
+    m.sync()
+    t0 = time.perf_counter()
+    m.update(a.steps)
+    m.sync()
+    elapsed = time.perf_counter() - t0
+    m.enable_profiling(True)
+    from_input(a.steps)
+    prof = m.profile()
+    m.close()
+    print(json.dumps({
+        "workload": a.workload, "boids": sc.n, "steps": a.steps,
+        "tuning": {"block": a.block, "flags": a.flags},
         "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
         "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
                     for k, v in sorted(prof.items())}}), flush=True)
