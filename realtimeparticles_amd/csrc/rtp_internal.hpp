@@ -219,8 +219,9 @@ enum PassMode {
 // granule list[(k / 4) * stride + t], so each thread stores four at a time and a wave's
 // stores and loads of one granule row are contiguous -- and their number to count[t].  The
 // second runs its body over the recorded indices instead of walking the cells, or walks
-// when count[t] > kNbCap.
-constexpr uint32_t kNbCap = 64;
+// when count[t] > kNbCap.  256 entries (1 KiB per thread slot) hold the survivors of the
+// clumps the dam-break coefficients form, whose cells fill to the cap.
+constexpr uint32_t kNbCap = 256;
 struct NbList {
   uint4* list = nullptr;      // kNbCap / 4 x stride granules
   uint32_t* count = nullptr;  // stride words

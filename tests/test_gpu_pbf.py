@@ -218,9 +218,10 @@ def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mappin
                                                (64, 16, 13), (256, 12, 8), (64, 16, 29)])
 def test_overfull_cell_keeps_bits(block, queue, flags):
     """20,000 particles in one cell (the cap iterates 101 of them) and 50 in the next cell
-    of its column: the walks through them are no longer back-to-back ranges, survivor
-    queues of every depth drain in the reference's order, and particles with more
-    survivors than a recorded list holds walk again instead of replaying."""
+    of its column: the walks through them are no longer back-to-back ranges, and survivor
+    queues of every depth drain in the reference's order.  A clump around a corner of eight
+    capped cells gives particles more survivors than a recorded list holds: they walk again
+    instead of replaying."""
     sc = scenes.dam_break((16, 16, 16), (4, 2, 2), (40, 20, 20), seed=8, extra=0)
     rng = np.random.default_rng(5)
     k = 20000
@@ -234,7 +235,14 @@ def test_overfull_cell_keeps_bits(block, queue, flags):
     nxt = np.zeros((50, 4), np.float32)
     nxt[:, :3] = dense[:50, :3]
     nxt[:, 2] += np.float32(0.1)
-    sc.pos = np.concatenate([sc.pos, dense, nxt])
+    # and 4,000 within 0.02 of the corner (0, 0.4, 0.4) that cells (19..20, 13..14, 13..14)
+    # share: all eight are capped, so each of these particles has over 800 survivors, more
+    # than a recorded list holds
+    ball = np.zeros((4000, 4), np.float32)
+    d = rng.normal(size=(4000, 3))
+    d *= 0.02 * rng.random((4000, 1)) ** (1.0 / 3.0) / np.linalg.norm(d, axis=1, keepdims=True)
+    ball[:, :3] = np.array([0.0, 0.4, 0.4]) + d
+    sc.pos = np.concatenate([sc.pos, dense, nxt, ball])
     sc.vel = np.zeros_like(sc.pos)
     sc.n = sc.pos.shape[0]
     sc.max_n = (sc.n + 511) // 512 * 512 + 512
