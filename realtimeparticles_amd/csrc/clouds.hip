@@ -878,6 +878,10 @@ using namespace rtp;
 struct rtp_clouds {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the temperature passes run on a second stream beside the density iterations: neither
+  // reads what the other writes, until XSPH rewrites the positions the temperature walks
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   GridState grid;
   CloudConsts c{};
   uint32_t n = 0;
@@ -901,6 +905,7 @@ struct rtp_clouds {
   float* lap = nullptr;
   float* lambdaT = nullptr;
   CldList list;              // survivor lists of the recording passes (kCldListCap per slot)
+  CldList listT;             // ... of the temperature passes, which run beside the others
   uint32_t* flags = nullptr;  // cld_flag bits of the step
   int cur = 0;  // state in [cur]
   int pc = 0;   // predicted positions in pred[pc]
@@ -928,8 +933,18 @@ struct rtp_clouds {
     dev_free(lambdaT);
     dev_free(list.entry);
     dev_free(list.count);
+    dev_free(listT.entry);
+    dev_free(listT.count);
     dev_free(flags);
     prof.destroy();
+    if (fork)
+      (void)hipEventDestroy(fork);
+    if (join)
+      (void)hipEventDestroy(join);
+    fork = join = nullptr;
+    if (side)
+      (void)hipStreamDestroy(side);
+    side = nullptr;
     if (stream)
       (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -984,13 +999,18 @@ struct rtp_clouds {
     // Replays need every candidate a walk may visit to have data from this step: not so
     // when the sorted range holds stale particles past n (after the count shrank).
     const int useList = G.sortedN <= n;
-    if (n && c.q.isTempSmoothingEnabled) {
-      RTP_PROF(prof, "cld_temp_lambda", s,
-               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], temp[cur],
-                                  G.startEnd, lap, lambdaT, n, c, list, flags));
-      RTP_PROF(prof, "cld_temp_correct", s,
-               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, s, pos[cur], lambdaT,
-                                  G.startEnd, temp[cur], n, c, list, flags, useList));
+    const bool tempOn = n && c.q.isTempSmoothingEnabled;
+    if (tempOn) {
+      hipStream_t t = side;
+      RTP_HIP_TRY(hipEventRecord(fork, s));
+      RTP_HIP_TRY(hipStreamWaitEvent(t, fork, 0));
+      RTP_PROF(prof, "cld_temp_lambda", t,
+               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], temp[cur],
+                                  G.startEnd, lap, lambdaT, n, c, listT, flags));
+      RTP_PROF(prof, "cld_temp_correct", t,
+               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], lambdaT,
+                                  G.startEnd, temp[cur], n, c, listT, flags, useList));
+      RTP_HIP_TRY(hipEventRecord(join, t));
     }
     for (uint32_t k = 0; k < iters && n; ++k) {
       const uint32_t bit = k < kFlagIters ? kFlagLambda0 << k : 0u;
@@ -1010,6 +1030,8 @@ struct rtp_clouds {
     // the sorted range's stale tail [n, sortedN) lives in the gather target
     if (pc != gathered && G.sortedN > n)
       launch_copy_f4(pred[pc], pred[gathered], n, G.sortedN, s);
+    if (tempOn)  // the temperature walks read p_pos, which XSPH or the position update rewrites
+      RTP_HIP_TRY(hipStreamWaitEvent(s, join, 0));
     if (!n)
       return RTP_OK;
     if (c.f.p.isVorticityConfEnabled) {
@@ -1103,8 +1125,15 @@ int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
            (rc = dev_alloc(&h->density, maxN)) || (rc = dev_alloc(&h->lambda, maxN)) ||
            (rc = dev_alloc(&h->lap, maxN)) || (rc = dev_alloc(&h->lambdaT, maxN)) ||
            (rc = dev_alloc(&h->list.entry, (size_t)kCldListCap * maxN)) ||
-           (rc = dev_alloc(&h->list.count, maxN)) || (rc = dev_alloc(&h->flags, 1)));
+           (rc = dev_alloc(&h->list.count, maxN)) || (rc = dev_alloc(&h->flags, 1)) ||
+           (rc = dev_alloc(&h->listT.entry, (size_t)kCldListCap * maxN)) ||
+           (rc = dev_alloc(&h->listT.count, maxN)));
   h->list.stride = maxN;
+  h->listT.stride = maxN;
+  if (!rc && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess))
+    rc = fail(RTP_ERR_HIP, "stream or event creation failed");
   if (rc) {
     h->release();
     delete h;
