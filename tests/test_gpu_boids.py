@@ -138,8 +138,12 @@ def test_boids_variants_bit_exact(variant):
     gpu.close()
 
 
-@pytest.mark.parametrize("block,flags", [(64, capi.LAUNCH_XCD_MAP), (128, 0), (192, capi.LAUNCH_XCD_MAP)])
+@pytest.mark.parametrize("block,flags", [(64, capi.LAUNCH_XCD_MAP), (128, 0), (192, capi.LAUNCH_XCD_MAP),
+                                         (256, 0)])
 def test_boids_launch_shapes_keep_bits(block, flags):
+    """Block sizes and block order: the step kernel's walk with survivor queues (344 B of LDS
+    per thread), or its bodies in place where a workgroup's share exceeds the device's
+    per-workgroup LDS, give the same bits."""
     sc = scenes.boids_flock(8192, seed=31)
     gpu, orc = make_pair(sc)
     gpu.set_tuning(block, flags)

@@ -129,6 +129,21 @@ def test_cloud_variants_bit_exact(variant):
     gpu.close()
 
 
+def test_thirty_one_iterations_bit_exact():
+    """31 Jacobi iterations: the lambda of iterations 29 and 30 has no flag bit left, so their
+    corrections walk instead of replaying the density passes' survivor lists."""
+    sc = scenes.cloud_scene("clouds-65k", seed=47)
+    sc.pos, sc.n = sc.pos[:8192].copy(), 8192
+    sc.vel = np.zeros_like(sc.pos)
+    gpu, orc = make_pair(sc, iters=31)
+    for step in range(1, 3):
+        gpu.update(1)
+        orc.step(1)
+        log(f"31 iterations, step {step}")
+        compare(gpu, orc, sc.n, f"step {step}", grid=(step == 2))
+    gpu.close()
+
+
 def test_cumulus_particle_count_change():
     """setNbParticles keeps the whole-buffer sort of the stale keys, as for the fluids."""
     sc = scenes.cloud_scene("clouds-65k", seed=29)
