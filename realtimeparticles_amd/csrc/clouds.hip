@@ -12,6 +12,10 @@
 //   iters x {k_cld_density_lambda, k_cld_correct}   (the last also updates the velocity)
 //   k_cld_vorticity, k_cld_confinement, k_cld_xsph (+ cld_updatePosition), or
 //   k_cld_update_pos without vorticity.
+// The two temperature passes run on a second stream beside the density iterations (neither
+// reads what the other writes); the step joins them before the vorticity pass.  The first
+// walk over each set of positions (temperature, each density iteration, vorticity) records
+// its survivors, and the passes after it over the same positions replay them.
 // The neighbour walks visit the reference's 27 cells in its order (x outer, z inner), with x
 // and z periodic -- a wrapped neighbour's position is shifted by +-2 ABS_WALL -- and y cells
 // beyond the walls skipped (clouds.cl:322-353).  Every expression follows clouds.cl's order
