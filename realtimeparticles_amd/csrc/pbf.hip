@@ -998,46 +998,359 @@ __global__ __launch_bounds__(256) void k_fb_scatter(float4* __restrict__ pred,
   }
 }
 
-// fld_computeDensity + fld_computeConstraintFactor over the set and then the halo (position
-// k in that order is the thread's survivor-list slot; the set's slots are the correction's).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_density_lambda_local(const float4* __restrict__ pred,
-                                                              const uint2* __restrict__ se,
-                                                              float* __restrict__ density,
-                                                              float* __restrict__ lambda,
-                                                              PbfConsts c, uint32_t qdepth,
-                                                              FallbackSet fs, NbList nb)
+// ------------------------------------------------------- quad walks of the fallback passes
+// A fallback pass covers a few thousand particles: one lane per particle would leave most
+// SIMDs idle while each wave waits out its own long chain of loads.  Here a particle has a
+// quad of four lanes.  The quad scans its candidates sixteen at a time (lane u takes
+// candidates u, u + 4, u + 8 and u + 12 of each step) into a survivor queue in LDS, in walk
+// order, and then runs the bodies of four survivors at once, one per lane; the four terms
+// enter every sum in walk order (a DPP broadcast within the quad and one add each), so each
+// sum adds the same terms in the same order as walk_cells' one-lane walk.  Lanes without a
+// survivor add +0, which leaves every sum (started at +0) unchanged.
+constexpr uint32_t kQuadBlock = 64;                // threads per block: one wave, 16 particles
+constexpr uint32_t kQuadGroups = kQuadBlock / 4u;
+constexpr uint32_t kQuadQueue = 64;                // queued survivors per particle
+constexpr uint32_t kQuadRanges = 27;               // index ranges per particle (<= one per cell)
+
+struct QuadLds {
+  uint32_t queue[kQuadGroups][kQuadQueue];
+  uint2 ranges[kQuadGroups][kQuadRanges];
+};
+
+// Orders the wave's LDS accesses on both sides (a quad's lanes read what the others wrote).
+__device__ __forceinline__ void wave_sync()
 {
-  RTP_PLAIN_CTX(qdepth)
-  W.nb = nb;
-  const LearnArgs la;
-  const uint32_t cs = fs.count[1], cnt = cs + fs.count[2];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-    W.slot = k;
-    density_lambda_body<MODE>(k < cs ? fs.list[k] : fs.halo[k - cs], pred, se, density, lambda,
-                              c, la, W);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v of lane K of the quad.
+template <int K>
+__device__ __forceinline__ float quad_lane(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               K * 0x55, 0xF, 0xF, false));
+}
+
+// ((((s + t0) + t1) + t2) + t3) for t_k the t of quad lane k.
+__device__ __forceinline__ float quad_add(float s, float t)
+{
+  s = s + quad_lane<0>(t);
+  s = s + quad_lane<1>(t);
+  s = s + quad_lane<2>(t);
+  return s + quad_lane<3>(t);
+}
+
+// The quad's four bits of a wave ballot, lane 0 lowest.
+__device__ __forceinline__ uint32_t quad_bits(bool x)
+{
+  return (uint32_t)(__builtin_amdgcn_ballot_w64(x) >> (threadIdx.x & 60u)) & 0xFu;
+}
+
+// Survivor-list entry k of slot s: word k % 4 of the slot's granule k / 4.
+__device__ __forceinline__ uint32_t* nb_word(const NbList& nb, uint32_t s, uint32_t k)
+{
+  return reinterpret_cast<uint32_t*>(nb.list + s + (size_t)(k >> 2) * nb.stride) + (k & 3u);
+}
+
+// walk_cells' index ranges of position p in its order, empty ones left out: per (x, y)
+// column one range when its non-empty cells are back to back, else one per non-empty cell.
+// Returns their number (at most 27).
+__device__ __forceinline__ uint32_t quad_ranges(const float4 p, const uint2* __restrict__ se,
+                                                const GridConsts& g, uint2* out)
+{
+  uint32_t cx, cy, cz;
+  cell3d(p, g, cx, cy, cz);
+  const int32_t rx = g.res[0], ry = g.res[1], rz = g.res[2];
+  const int32_t z0 = wrap((int32_t)cz - 1, rz), z1 = wrap((int32_t)cz, rz),
+                z2 = wrap((int32_t)cz + 1, rz);
+  uint32_t nr = 0;
+#pragma unroll
+  for (int ix = -1; ix <= 1; ++ix) {
+    const int32_t nx = wrap((int32_t)cx + ix, rx);
+#pragma unroll
+    for (int iy = -1; iy <= 1; ++iy) {
+      const int32_t row = (nx * ry + wrap((int32_t)cy + iy, ry)) * rz;
+      const uint2 cells[3] = { se[row + z0], se[row + z1], se[row + z2] };
+      uint32_t lo = 1u, hi = 0u;
+      bool ok = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const uint2 r = cells[k];
+        if (r.x > r.y)
+          continue;
+        if (lo > hi) {
+          lo = r.x;
+          hi = r.y;
+        } else if (r.x == hi + 1u) {
+          hi = r.y;
+        } else {
+          ok = false;
+        }
+      }
+      if (ok) {
+        if (lo <= hi)
+          out[nr++] = make_uint2(lo, hi);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (cells[k].x <= cells[k].y)
+            out[nr++] = cells[k];
+      }
+    }
+  }
+  return nr;
+}
+
+// Runs the body over survivors 0..cnt-1 of the quad (src(k): survivor k), four at a time
+// (lane u: survivors u, u + 4, ...), loading each round's data during the previous round's
+// bodies; rec(v, k, e) sees every round first.  The wave loops until its longest quad is
+// done; empty lanes run the body on the particle itself and add +0.
+template <class S, class R, class B>
+__device__ __forceinline__ void quad_run(uint32_t cnt, uint32_t self, const float4 p,
+                                         const float4* __restrict__ pred, const PbfConsts& c,
+                                         S&& src, R&& rec, B& b)
+{
+  const uint32_t u = threadIdx.x & 3u;
+  bool v = u < cnt;
+  uint32_t e = v ? src(u) : self;
+  float4 o = ldn<false>(pred, e, c);
+  auto x = b.ld(e);
+  for (uint32_t k = u; __builtin_amdgcn_ballot_w64(k - u < cnt) != 0; k += 4u) {
+    const uint32_t k1 = k + 4u;
+    const bool v1 = k1 < cnt;
+    const uint32_t e1 = v1 ? src(k1) : self;
+    const float4 o1 = ldn<false>(pred, e1, c);
+    const auto x1 = b.ld(e1);
+    rec(v, k, e);
+    b.add(v, p, o, x);
+    v = v1;
+    e = e1;
+    o = o1;
+    x = x1;
   }
 }
 
-// fld_computeConstraintCorrection + fld_correctPosition over the set (and, unless `last`,
-// the next iteration's boundary condition).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_correct_local(const float4* __restrict__ pred,
-                                                       const float* __restrict__ lambda,
-                                                       const uint2* __restrict__ se,
-                                                       float4* __restrict__ predOut, int last,
-                                                       PbfConsts c, uint32_t qdepth,
-                                                       FallbackSet fs, NbList nb)
+// The quad's walk over its nr ranges rl (in LDS): candidates are queued (the test of offer)
+// and the queues drained by quad_run whenever one of the wave's quads has fewer than 16
+// free entries, and at the end.  With REC the survivors go to the quad's survivor list
+// (slot `slot`, the first kNbCap of them) and their number to its count.
+template <bool REC, class B>
+__device__ __forceinline__ void quad_walk(uint32_t nr, const uint2* rl, uint32_t* q,
+                                          uint32_t self, const float4 p,
+                                          const float4* __restrict__ pred, const PbfConsts& c,
+                                          B& b, const NbList& nb, uint32_t slot, bool record)
 {
-  RTP_PLAIN_CTX(qdepth)
-  W.nb = nb;
-  const LearnArgs la;
-  const PassOrder po;
+  const uint32_t u = threadIdx.x & 3u, below = (1u << u) - 1u;
+  uint32_t ri = 0, cur = 1u, hi = 0u, qn = 0, tot = 0;
+  if (nr > 0) {
+    const uint2 r = rl[0];
+    cur = r.x;
+    hi = r.y;
+  }
+  auto drain = [&]() {
+    wave_sync();
+    quad_run(qn, self, p, pred, c, [&](uint32_t k) { return q[k]; },
+             [&](bool v, uint32_t k, uint32_t e) {
+               if (REC && v & record & (tot + k < kNbCap))
+                 *nb_word(nb, slot, tot + k) = e;
+             }, b);
+    tot += qn;
+    qn = 0;
+    wave_sync();
+  };
+  while (__builtin_amdgcn_ballot_w64(ri < nr) != 0) {
+    const bool act = ri < nr;
+    uint32_t e[4];
+    bool keep[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      e[j] = cur + u + 4u * (uint32_t)j;
+      const bool valid = act & (e[j] <= hi);
+      const float4 o = ldn<false>(pred, valid ? e[j] : self, c);
+      const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+      const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      keep[j] = valid & !((d2 >= c.cullFused) & (d2 < c.farFused));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t m = quad_bits(keep[j]);
+      if (keep[j])
+        q[qn + (uint32_t)__builtin_popcount(m & below)] = e[j];
+      qn += (uint32_t)__builtin_popcount(m);
+    }
+    cur += 16u;
+    if (act & (cur > hi)) {
+      ++ri;
+      if (ri < nr) {
+        const uint2 r = rl[ri];
+        cur = r.x;
+        hi = r.y;
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(qn > kQuadQueue - 16u) != 0)
+      drain();
+  }
+  drain();
+  if (REC && record && u == 0u)
+    nb.count[slot] = tot;
+}
+
+// density_lambda_body's sums (MODE 0) as quad terms.
+struct QuadDensity {
+  const PbfConsts& c;
+  bool fast;
+  float dens = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, ssq = 0.0f;
+  __device__ __forceinline__ int ld(uint32_t) const { return 0; }
+  __device__ __forceinline__ void add(bool v, const float4 p, const float4 o, int)
+  {
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    float w, gx = 0.0f, gy = 0.0f, gz = 0.0f, sq = 0.0f;
+    if (fast && r < c.h) {
+      w = poly6_in(r, c);
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const bool on = r > kFloatEps;
+      gx = on ? g.x : 0.0f;
+      gy = on ? g.y : 0.0f;
+      gz = on ? g.z : 0.0f;
+      sq = gx * gx + gy * gy + gz * gz;
+    } else {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      w = poly6(r, t, c);
+      if (r > kFloatEps) {  // else the reference adds no gradient: +0 here
+        const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+        gx = g.x;
+        gy = g.y;
+        gz = g.z;
+        sq = gx * gx + gy * gy + gz * gz;
+      }
+    }
+    dens = quad_add(dens, v ? w : 0.0f);
+    sgx = quad_add(sgx, v ? gx : 0.0f);
+    sgy = quad_add(sgy, v ? gy : 0.0f);
+    sgz = quad_add(sgz, v ? gz : 0.0f);
+    ssq = quad_add(ssq, v ? sq : 0.0f);
+  }
+};
+
+// correct_body's sums as quad terms.
+struct QuadCorrect {
+  const PbfConsts& c;
+  const float* __restrict__ lambda;
+  bool fast;
+  float li, ywdq;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  __device__ __forceinline__ float ld(uint32_t e) const { return lambda[e]; }
+  __device__ __forceinline__ void add(bool v, const float4 p, const float4 o, float le)
+  {
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    float tx, ty, tz;
+    if (fast && r < c.h) {
+      const float s = li + le + art_pressure_in(poly6_in(r, c), ywdq, c);
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const bool on = r > kFloatEps;
+      tx = s * (on ? g.x : 0.0f);
+      ty = s * (on ? g.y : 0.0f);
+      tz = s * (on ? g.z : 0.0f);
+    } else {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      const float s = li + le + art_pressure(poly6(r, t, c), c);
+      if (r > kFloatEps) {
+        const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+        tx = s * g.x;
+        ty = s * g.y;
+        tz = s * g.z;
+      } else {  // gradSpiky returns the zero vector
+        tx = s * 0.0f;
+        ty = s * 0.0f;
+        tz = s * 0.0f;
+      }
+    }
+    cx = quad_add(cx, v ? tx : 0.0f);
+    cy = quad_add(cy, v ? ty : 0.0f);
+    cz = quad_add(cz, v ? tz : 0.0f);
+  }
+};
+
+// fld_computeDensity + fld_computeConstraintFactor over the set and then the halo, a quad
+// per particle.  With REC the set's survivors go to survivor-list slot k (the position in
+// the set) for the correction.
+template <bool REC>
+__global__ __launch_bounds__(kQuadBlock) void k_density_lambda_quad(
+    const float4* __restrict__ pred, const uint2* __restrict__ se, float* __restrict__ density,
+    float* __restrict__ lambda, PbfConsts c, FallbackSet fs, NbList nb)
+{
+  __shared__ QuadLds lds;
+  const uint32_t cs = fs.count[1], cnt = cs + fs.count[2];
+  const uint32_t g = threadIdx.x >> 2;
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  for (uint32_t base = blockIdx.x * kQuadGroups; base < cnt; base += gridDim.x * kQuadGroups) {
+    const uint32_t k = base + g;
+    const bool on = k < cnt;
+    const uint32_t i = on ? (k < cs ? fs.list[k] : fs.halo[k - cs]) : 0u;
+    const float4 p = ldp<false>(pred, i, c);
+    const uint32_t nr = on ? quad_ranges(p, se, c.g, lds.ranges[g]) : 0u;
+    QuadDensity b{ c, fast };
+    quad_walk<REC>(nr, lds.ranges[g], lds.queue[g], i, p, pred, c, b, nb, k, on & (k < cs));
+    if (on && (threadIdx.x & 3u) == 0u) {
+      density[i] = b.dens;
+      const float C = b.dens / c.p.restDensity - 1.0f;
+      float sq = b.ssq + (b.sgx * b.sgx + b.sgy * b.sgy + b.sgz * b.sgz);
+      sq = sq / c.rho0sq;
+      lambda[i] = -C / (sq + c.p.relaxCFM);
+    }
+  }
+}
+
+// fld_computeConstraintCorrection + fld_correctPosition over the set, a quad per particle
+// (and, unless `last`, the next iteration's boundary condition).  With REPLAY a particle
+// whose survivor list holds all its survivors runs the list; the others walk.
+template <bool REPLAY>
+__global__ __launch_bounds__(kQuadBlock) void k_correct_quad(
+    const float4* __restrict__ pred, const float* __restrict__ lambda,
+    const uint2* __restrict__ se, float4* __restrict__ predOut, int last, PbfConsts c,
+    FallbackSet fs, NbList nb)
+{
+  __shared__ QuadLds lds;
   const uint32_t cnt = fs.count[1];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-    W.slot = k;
-    correct_body<MODE>(fs.list[k], pred, lambda, se, predOut, nullptr, nullptr, nullptr, last, c,
-                       la, W, po);
+  const uint32_t g = threadIdx.x >> 2;
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+  const float ywdq = (c.shortMath & kShortArtDiv) ? rcp_refined(c.wdq) : 0.0f;
+  auto none = [](bool, uint32_t, uint32_t) {};
+  for (uint32_t base = blockIdx.x * kQuadGroups; base < cnt; base += gridDim.x * kQuadGroups) {
+    const uint32_t k = base + g;
+    const bool on = k < cnt;
+    const uint32_t i = on ? fs.list[k] : 0u;
+    const float4 p = ldp<false>(pred, i, c);
+    QuadCorrect b{ c, lambda, fast, lambda[i], ywdq };
+    bool listed = false;
+    if (REPLAY) {
+      const uint32_t lc = on ? nb.count[k] : 0u;
+      listed = on & (lc <= kNbCap);
+      quad_run(listed ? lc : 0u, i, p, pred, c, [&](uint32_t m) { return *nb_word(nb, k, m); },
+               none, b);
+    }
+    const uint32_t nr = (on & !listed) ? quad_ranges(p, se, c.g, lds.ranges[g]) : 0u;
+    if (__builtin_amdgcn_ballot_w64(nr > 0u) != 0)
+      quad_walk<false>(nr, lds.ranges[g], lds.queue[g], i, p, pred, c, b, nb, k, false);
+    if (on && (threadIdx.x & 3u) == 0u) {
+      float4 q;
+      q.x = p.x + b.cx / c.p.restDensity;
+      q.y = p.y + b.cy / c.p.restDensity;
+      q.z = p.z + b.cz / c.p.restDensity;
+      q.w = 0.0f;
+      if (last == kNextIter) {
+        q.x = clampf(q.x, c.lo[0], c.hi[0]);
+        q.y = clampf(q.y, c.lo[1], c.hi[1]);
+        q.z = clampf(q.z, c.lo[2], c.hi[2]);
+      }
+      predOut[i] = q;
+    }
   }
 }
 
@@ -1663,23 +1976,25 @@ void launch_fallback_iteration(float4* pred, float4* scratch, float* density, fl
   if (first)
     hipLaunchKernelGGL(k_fb_begin, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, lambda, n, c,
                        fs);
-  const dim3 g(list_blocks(n, L.block)), b(L.block);
-  const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
+  (void)L;
+  // a quad per particle, at most 2^16 particles per round of the grid-stride loop
+  const uint32_t qb = blocks_for(n, kQuadGroups), qcap = (1u << 16) / kQuadGroups;
+  const dim3 g(qb < qcap ? qb : qcap), b(kQuadBlock);
   const NbList lists = nb ? *nb : NbList();
   const int lastMode = last ? kLastOnly : kNextIter;
   // the correction replays the density pass's lists: the same positions
   if (nb) {
-    hipLaunchKernelGGL(k_density_lambda_local<kNbRecord>, g, b, lds, s, pred, startEnd, density,
-                       lambda, c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<kNbReplay>, g, b, lds, s, pred, lambda, startEnd, scratch,
-                       lastMode, c, q, fs, lists);
+    hipLaunchKernelGGL(k_density_lambda_quad<true>, g, b, 0, s, pred, startEnd, density, lambda,
+                       c, fs, lists);
+    hipLaunchKernelGGL(k_correct_quad<true>, g, b, 0, s, pred, lambda, startEnd, scratch,
+                       lastMode, c, fs, lists);
   } else {
-    hipLaunchKernelGGL(k_density_lambda_local<0>, g, b, lds, s, pred, startEnd, density, lambda,
-                       c, q, fs, lists);
-    hipLaunchKernelGGL(k_correct_local<0>, g, b, lds, s, pred, lambda, startEnd, scratch, lastMode,
-                       c, q, fs, lists);
+    hipLaunchKernelGGL(k_density_lambda_quad<false>, g, b, 0, s, pred, startEnd, density, lambda,
+                       c, fs, lists);
+    hipLaunchKernelGGL(k_correct_quad<false>, g, b, 0, s, pred, lambda, startEnd, scratch,
+                       lastMode, c, fs, lists);
   }
-  hipLaunchKernelGGL(k_fb_scatter, g, dim3(L.block), 0, s, pred, scratch, fs);
+  hipLaunchKernelGGL(k_fb_scatter, dim3(list_blocks(n, 256)), dim3(256), 0, s, pred, scratch, fs);
 }
 
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,
