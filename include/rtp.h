@@ -99,9 +99,9 @@ typedef struct {
 #define RTP_LAUNCH_RESORT 4    /* after every correction, order the next passes' threads by the
                                   cells their particles now sit in (a counting sort) */
 #define RTP_LAUNCH_NB_LIST 8   /* the density pass of each Jacobi iteration records every
-                                  particle's neighbour list (up to 64) and the correction pass
+                                  particle's neighbour list (up to 256) and the correction pass
                                   replays it; likewise vorticity for confinement and XSPH
-                                  (256 bytes of device memory per particle; ignored with
+                                  (1 KiB of device memory per particle; ignored with
                                   RTP_LAUNCH_LDS_TILES) */
 #define RTP_LAUNCH_PERSISTENT 32 /* boids: each update runs its steps in one cooperative launch
                                     (hash, sort, reorder and rules as phases between grid
@@ -111,6 +111,11 @@ typedef struct {
                                   pencils of 8 x 8 cells in (y, z), pencil after pencil, instead
                                   of in cell index order, so an XCD's consecutive waves keep
                                   their neighbours' data in its L2 on large grids */
+#define RTP_LAUNCH_QUAD 64     /* fluids: the plain solver's neighbour passes give each particle
+                                  four lanes (sixteen particles per 64-lane workgroup) that
+                                  split its walk and add the terms in the same order; for
+                                  scenes too small to fill the GPU one lane per particle
+                                  (results unchanged; not with RTP_LAUNCH_LDS_TILES) */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {
@@ -375,7 +380,7 @@ int rtp_batch_save_checkpoint(rtp_batch* h);
 int rtp_batch_restore_checkpoint(rtp_batch* h);
 int rtp_batch_use_graph(rtp_batch* h, int enable);
 /* As rtp_fluids_set_tuning; of the flags RTP_LAUNCH_RESORT and RTP_LAUNCH_NB_LIST apply to a
- * batch (the lists take 256 bytes per particle plus 64 KiB per scene). */
+ * batch (the lists take 1 KiB per particle plus 255 KiB per scene). */
 int rtp_batch_set_tuning(rtp_batch* h, const rtp_tuning* t);
 int rtp_batch_autotune(rtp_batch* h, uint32_t trialSteps, rtp_tuning* best);
 int rtp_batch_enable_profiling(rtp_batch* h, int enable);

@@ -28,6 +28,7 @@ FIELD_RESIDUAL_DENSITY, FIELD_FALLBACK_SET = 16, 17
 BOUNDARY_BOUNCING, BOUNDARY_CYCLIC = 0, 1
 LAUNCH_XCD_MAP, LAUNCH_LDS_TILES, LAUNCH_RESORT, LAUNCH_NB_LIST, LAUNCH_BRICKS = 1, 2, 4, 8, 16
 LAUNCH_PERSISTENT = 32  # boids: every step of an update in one cooperative launch
+LAUNCH_QUAD = 64  # fluids: four lanes per particle in the plain solver's neighbour passes
 
 
 class RtpError(RuntimeError):
@@ -585,7 +586,7 @@ class Fluids(_RenderFields):
 
     def set_tuning(self, block: int = 256, flags: int = 0, sort_bits: int = 0, queue: int = 0) -> None:
         """flags: LAUNCH_XCD_MAP | LAUNCH_LDS_TILES | LAUNCH_RESORT | LAUNCH_NB_LIST |
-        LAUNCH_BRICKS (with LAUNCH_RESORT); queue:
+        LAUNCH_BRICKS (with LAUNCH_RESORT) | LAUNCH_QUAD (not with LAUNCH_LDS_TILES); queue:
         survivor-queue depth (0 = 16)."""
         t = _tuning(block, flags, sort_bits, queue)
         _check(lib().rtp_fluids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")

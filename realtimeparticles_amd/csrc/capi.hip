@@ -1273,6 +1273,7 @@ int rtp_fluids_set_tuning(rtp_fluids* h, const rtp_tuning* t)
   h->L.resort = (t->flags & RTP_LAUNCH_RESORT) ? 1 : 0;
   h->L.nbList = (t->flags & RTP_LAUNCH_NB_LIST) ? 1 : 0;
   h->L.bricks = (t->flags & RTP_LAUNCH_BRICKS) ? 1 : 0;
+  h->L.quad = (t->flags & RTP_LAUNCH_QUAD) ? 1 : 0;
   h->L.queue = t->queueDepth ? t->queueDepth : kDefaultQueue;
   h->grid.sortBits = t->sortBits;
   h->drop_graph();
@@ -1285,7 +1286,7 @@ rtp_tuning current_tuning(const rtp_fluids* h)
   t.blockSize = h->L.block;
   t.flags = (h->L.xcdRemap ? RTP_LAUNCH_XCD_MAP : 0) | (h->L.tile ? RTP_LAUNCH_LDS_TILES : 0) |
             (h->L.resort ? RTP_LAUNCH_RESORT : 0) | (h->L.nbList ? RTP_LAUNCH_NB_LIST : 0) |
-            (h->L.bricks ? RTP_LAUNCH_BRICKS : 0);
+            (h->L.bricks ? RTP_LAUNCH_BRICKS : 0) | (h->L.quad ? RTP_LAUNCH_QUAD : 0);
   t.sortBits = h->grid.sortBits;
   t.queueDepth = h->L.queue;
   return t;

@@ -746,6 +746,41 @@ __device__ __forceinline__ void record_cell(uint32_t i, const float4 q, const Pb
   po.keyOut[i] = po.keyBase + (po.bricks ? brick_key(q, c.g) : cell1d(q, c.g));
 }
 
+// The corrected position of particle i (position p, correction sum (cx, cy, cz)) and what
+// `last` adds to it.
+__device__ __forceinline__ void correct_out(uint32_t i, const float4 p, float cx, float cy,
+                                            float cz, float4* __restrict__ predOut,
+                                            const float4* __restrict__ pos,
+                                            float4* __restrict__ vel, float4* __restrict__ posOut,
+                                            int last, const PbfConsts& c, const PassOrder& po)
+{
+  float4 q;
+  q.x = p.x + cx / c.p.restDensity;
+  q.y = p.y + cy / c.p.restDensity;
+  q.z = p.z + cz / c.p.restDensity;
+  q.w = 0.0f;
+  if (last == kNextIter) {
+    q.x = clampf(q.x, c.lo[0], c.hi[0]);
+    q.y = clampf(q.y, c.lo[1], c.hi[1]);
+    q.z = clampf(q.z, c.lo[2], c.hi[2]);
+  }
+  predOut[i] = q;
+  record_cell(i, q, c, po);
+  if (last == kNextIter)
+    return;
+  if (last == kLastOnly)
+    return;
+  const float4 o = pos[i];
+  float4 v;
+  v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
+  v.y = clampf((q.y - o.y) / c.dtEps, -c.maxVel, c.maxVel);
+  v.z = clampf((q.z - o.z) / c.dtEps, -c.maxVel, c.maxVel);
+  v.w = 0.0f;
+  vel[i] = v;
+  if (posOut)
+    posOut[i] = q;
+}
+
 template <int MODE>
 __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restrict__ pred,
                                              const float* __restrict__ lambda,
@@ -789,31 +824,7 @@ __device__ __forceinline__ void correct_body(uint32_t i, const float4* __restric
       cz = cz + s * 0.0f;
     }
   }, T);
-  float4 q;
-  q.x = p.x + cx / c.p.restDensity;
-  q.y = p.y + cy / c.p.restDensity;
-  q.z = p.z + cz / c.p.restDensity;
-  q.w = 0.0f;
-  if (last == kNextIter) {
-    q.x = clampf(q.x, c.lo[0], c.hi[0]);
-    q.y = clampf(q.y, c.lo[1], c.hi[1]);
-    q.z = clampf(q.z, c.lo[2], c.hi[2]);
-  }
-  predOut[i] = q;
-  record_cell(i, q, c, po);
-  if (last == kNextIter)
-    return;
-  if (last == kLastOnly)
-    return;
-  const float4 o = pos[i];
-  float4 v;
-  v.x = clampf((q.x - o.x) / c.dtEps, -c.maxVel, c.maxVel);
-  v.y = clampf((q.y - o.y) / c.dtEps, -c.maxVel, c.maxVel);
-  v.z = clampf((q.z - o.z) / c.dtEps, -c.maxVel, c.maxVel);
-  v.w = 0.0f;
-  vel[i] = v;
-  if (posOut)
-    posOut[i] = q;
+  correct_out(i, p, cx, cy, cz, predOut, pos, vel, posOut, last, c, po);
 }
 
 template <int MODE>
@@ -1277,6 +1288,42 @@ struct QuadCorrect {
   }
 };
 
+// density_lambda_body's outputs (MODE 0) from a quad's sums.
+__device__ __forceinline__ void density_lambda_out(uint32_t i, const QuadDensity& b,
+                                                   float* __restrict__ density,
+                                                   float* __restrict__ lambda, const PbfConsts& c)
+{
+  density[i] = b.dens;
+  const float C = b.dens / c.p.restDensity - 1.0f;
+  float sq = b.ssq + (b.sgx * b.sgx + b.sgy * b.sgy + b.sgz * b.sgz);
+  sq = sq / c.rho0sq;
+  lambda[i] = -C / (sq + c.p.relaxCFM);
+}
+
+// The quad's survivors of particle i, into the body b: with NB == kNbReplay its survivor list
+// (slot `slot`) when that holds all of them, else its walk, recorded (slot `slot`) with
+// NB == kNbRecord when `record`.
+template <int NB, class B>
+__device__ __forceinline__ void quad_neighbours(bool on, uint32_t i, uint32_t slot, bool record,
+                                                const float4 p, const float4* __restrict__ pred,
+                                                const uint2* __restrict__ se, const PbfConsts& c,
+                                                const NbList& nb, QuadLds& lds, B& b)
+{
+  const uint32_t g = threadIdx.x >> 2;
+  bool listed = false;
+  if constexpr (NB == kNbReplay) {
+    const uint32_t lc = on ? nb.count[slot] : 0u;
+    listed = on & (lc <= kNbCap);
+    quad_run(listed ? lc : 0u, i, p, pred, c, [&](uint32_t m) { return *nb_word(nb, slot, m); },
+             [](bool, uint32_t, uint32_t) {}, b);
+  }
+  const uint32_t nr = (on & !listed) ? quad_ranges(p, se, c.g, lds.ranges[g]) : 0u;
+  // a recording walk writes every recorded count, also of a particle without candidates
+  if (NB == kNbRecord || __builtin_amdgcn_ballot_w64(nr > 0u) != 0)
+    quad_walk<NB == kNbRecord>(nr, lds.ranges[g], lds.queue[g], i, p, pred, c, b, nb, slot,
+                               record);
+}
+
 // fld_computeDensity + fld_computeConstraintFactor over the set and then the halo, a quad
 // per particle.  With REC the set's survivors go to survivor-list slot k (the position in
 // the set) for the correction.
@@ -1294,16 +1341,10 @@ __global__ __launch_bounds__(kQuadBlock) void k_density_lambda_quad(
     const bool on = k < cnt;
     const uint32_t i = on ? (k < cs ? fs.list[k] : fs.halo[k - cs]) : 0u;
     const float4 p = ldp<false>(pred, i, c);
-    const uint32_t nr = on ? quad_ranges(p, se, c.g, lds.ranges[g]) : 0u;
     QuadDensity b{ c, fast };
-    quad_walk<REC>(nr, lds.ranges[g], lds.queue[g], i, p, pred, c, b, nb, k, on & (k < cs));
-    if (on && (threadIdx.x & 3u) == 0u) {
-      density[i] = b.dens;
-      const float C = b.dens / c.p.restDensity - 1.0f;
-      float sq = b.ssq + (b.sgx * b.sgx + b.sgy * b.sgy + b.sgz * b.sgz);
-      sq = sq / c.rho0sq;
-      lambda[i] = -C / (sq + c.p.relaxCFM);
-    }
+    quad_neighbours<REC ? kNbRecord : 0>(on, i, k, on & (k < cs), p, pred, se, c, nb, lds, b);
+    if (on && (threadIdx.x & 3u) == 0u)
+      density_lambda_out(i, b, density, lambda, c);
   }
 }
 
@@ -1321,36 +1362,16 @@ __global__ __launch_bounds__(kQuadBlock) void k_correct_quad(
   const uint32_t g = threadIdx.x >> 2;
   const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
   const float ywdq = (c.shortMath & kShortArtDiv) ? rcp_refined(c.wdq) : 0.0f;
-  auto none = [](bool, uint32_t, uint32_t) {};
+  const PassOrder po;  // no cell keys to record
   for (uint32_t base = blockIdx.x * kQuadGroups; base < cnt; base += gridDim.x * kQuadGroups) {
     const uint32_t k = base + g;
     const bool on = k < cnt;
     const uint32_t i = on ? fs.list[k] : 0u;
     const float4 p = ldp<false>(pred, i, c);
     QuadCorrect b{ c, lambda, fast, lambda[i], ywdq };
-    bool listed = false;
-    if (REPLAY) {
-      const uint32_t lc = on ? nb.count[k] : 0u;
-      listed = on & (lc <= kNbCap);
-      quad_run(listed ? lc : 0u, i, p, pred, c, [&](uint32_t m) { return *nb_word(nb, k, m); },
-               none, b);
-    }
-    const uint32_t nr = (on & !listed) ? quad_ranges(p, se, c.g, lds.ranges[g]) : 0u;
-    if (__builtin_amdgcn_ballot_w64(nr > 0u) != 0)
-      quad_walk<false>(nr, lds.ranges[g], lds.queue[g], i, p, pred, c, b, nb, k, false);
-    if (on && (threadIdx.x & 3u) == 0u) {
-      float4 q;
-      q.x = p.x + b.cx / c.p.restDensity;
-      q.y = p.y + b.cy / c.p.restDensity;
-      q.z = p.z + b.cz / c.p.restDensity;
-      q.w = 0.0f;
-      if (last == kNextIter) {
-        q.x = clampf(q.x, c.lo[0], c.hi[0]);
-        q.y = clampf(q.y, c.lo[1], c.hi[1]);
-        q.z = clampf(q.z, c.lo[2], c.hi[2]);
-      }
-      predOut[i] = q;
-    }
+    quad_neighbours<REPLAY ? kNbReplay : 0>(on, i, k, false, p, pred, se, c, nb, lds, b);
+    if (on && (threadIdx.x & 3u) == 0u)
+      correct_out(i, p, b.cx, b.cy, b.cz, predOut, nullptr, nullptr, nullptr, last, c, po);
   }
 }
 
@@ -1435,6 +1456,28 @@ __global__ __launch_bounds__(256) void k_vorticity(const float4* __restrict__ pr
     vorticity_body<NB>(po.particle(t), pred, se, vel, vort, c, W);
 }
 
+// The confined velocity of particle i from its sum (nx, ny, nz) of |vort_j| gradW.
+__device__ __forceinline__ void confinement_out(uint32_t i, float nx, float ny, float nz,
+                                                const float4* __restrict__ vort,
+                                                const float4* __restrict__ vel,
+                                                float4* __restrict__ velOut, const PbfConsts& c)
+{
+  // normalize(n), 0 for the zero vector
+  if (!(nx == 0.0f && ny == 0.0f && nz == 0.0f)) {
+    const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx / len;
+    ny = ny / len;
+    nz = nz / len;
+  }
+  const float4 w = vort[i];
+  const float crx = ny * w.z - nz * w.y;
+  const float cry = nz * w.x - nx * w.z;
+  const float crz = nx * w.y - ny * w.x;
+  const float k = c.p.vorticityConfCoeff, dt = c.p.timeStep;
+  const float4 v = vel[i];
+  velOut[i] = make_float4(v.x + (k * crx) * dt, v.y + (k * cry) * dt, v.z + (k * crz) * dt, 0.0f);
+}
+
 // fld_applyVorticityConfinement (fluids.cl:329-377), writing the confined velocity to
 // velOut (the copy fld_applyXsphViscosityCorrection reads, Fluids.cpp:451).
 template <int NB = 0>
@@ -1468,20 +1511,7 @@ __device__ __forceinline__ void confinement_body(uint32_t i, const float4* __res
       nz = nz + l * g.z;
     }
   }, T);
-  // normalize(n), 0 for the zero vector
-  if (!(nx == 0.0f && ny == 0.0f && nz == 0.0f)) {
-    const float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
-    nx = nx / len;
-    ny = ny / len;
-    nz = nz / len;
-  }
-  const float4 w = vort[i];
-  const float crx = ny * w.z - nz * w.y;
-  const float cry = nz * w.x - nx * w.z;
-  const float crz = nx * w.y - ny * w.x;
-  const float k = c.p.vorticityConfCoeff, dt = c.p.timeStep;
-  const float4 v = vel[i];
-  velOut[i] = make_float4(v.x + (k * crx) * dt, v.y + (k * cry) * dt, v.z + (k * crz) * dt, 0.0f);
+  confinement_out(i, nx, ny, nz, vort, vel, velOut, c);
 }
 
 template <int NB>
@@ -1541,6 +1571,203 @@ __global__ __launch_bounds__(256) void k_xsph(const float4* __restrict__ pred,
   if (t < n)
     xsph_body<NB>(po.particle(t), pred, se, velIn, velOut, posOut, c, W);
 }
+
+// ------------------------------------------------------- quad passes of the plain solver
+// With Launch::quad the neighbour passes of the plain solver run a quad of lanes per
+// particle (the fallback passes' walks above): thread order t, survivor-list slot t, the
+// same sums in the same order.  For scenes too small to give every SIMD several waves with
+// one lane per particle.
+
+// vorticity_body's sums as quad terms.
+struct QuadVorticity {
+  const PbfConsts& c;
+  const float4* __restrict__ vel;
+  bool fast;
+  float4 vi;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  __device__ __forceinline__ float4 ld(uint32_t e) const { return vel[e]; }
+  __device__ __forceinline__ void add(bool v, const float4 p, const float4 o, const float4 ve)
+  {
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    if (fast && r < c.h) {
+      // nothing is added for r <= kFloatEps
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const float ax = ve.x - vi.x, ay = ve.y - vi.y, az = ve.z - vi.z;
+      const bool on = r > kFloatEps;
+      tx = on ? ay * g.z - az * g.y : 0.0f;
+      ty = on ? az * g.x - ax * g.z : 0.0f;
+      tz = on ? ax * g.y - ay * g.x : 0.0f;
+    } else if (r > kFloatEps) {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      const float ax = ve.x - vi.x, ay = ve.y - vi.y, az = ve.z - vi.z;
+      tx = ay * g.z - az * g.y;
+      ty = az * g.x - ax * g.z;
+      tz = ax * g.y - ay * g.x;
+    }
+    wx = quad_add(wx, v ? tx : 0.0f);
+    wy = quad_add(wy, v ? ty : 0.0f);
+    wz = quad_add(wz, v ? tz : 0.0f);
+  }
+};
+
+// confinement_body's sums as quad terms.
+struct QuadConfinement {
+  const PbfConsts& c;
+  const float4* __restrict__ vort;
+  bool fast;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  __device__ __forceinline__ float ld(uint32_t e) const { return vort[e].w; }
+  __device__ __forceinline__ void add(bool v, const float4 p, const float4 o, float l)
+  {
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    if (fast && r < c.h) {
+      // nothing is added for r <= kFloatEps
+      const Grad g = grad_spiky_in(dx, dy, dz, r, rcp_refined(r), c);
+      const bool on = r > kFloatEps;
+      tx = on ? l * g.x : 0.0f;
+      ty = on ? l * g.y : 0.0f;
+      tz = on ? l * g.z : 0.0f;
+    } else if (r > kFloatEps) {
+      const float t = r < c.h ? 1.0f : 0.0f;
+      const Grad g = grad_spiky(dx, dy, dz, r, t, c);
+      tx = l * g.x;
+      ty = l * g.y;
+      tz = l * g.z;
+    }
+    nx = quad_add(nx, v ? tx : 0.0f);
+    ny = quad_add(ny, v ? ty : 0.0f);
+    nz = quad_add(nz, v ? tz : 0.0f);
+  }
+};
+
+// xsph_body's sums as quad terms.
+struct QuadXsph {
+  const PbfConsts& c;
+  const float4* __restrict__ velIn;
+  bool fast;
+  float4 vi;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  __device__ __forceinline__ float4 ld(uint32_t e) const { return velIn[e]; }
+  __device__ __forceinline__ void add(bool v, const float4 p, const float4 o, const float4 ve)
+  {
+    const float dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float r = fast ? sqrt_cr(d2) : __builtin_sqrtf(d2);
+    const float t = r < c.h ? 1.0f : 0.0f;
+    const float w = (fast && r < c.h) ? poly6_in(r, c) : poly6(r, t, c);
+    sx = quad_add(sx, v ? (ve.x - vi.x) * w : 0.0f);
+    sy = quad_add(sy, v ? (ve.y - vi.y) * w : 0.0f);
+    sz = quad_add(sz, v ? (ve.z - vi.z) * w : 0.0f);
+  }
+};
+
+// The quad's thread t, its particle and whether it has one (quads past n stand in for
+// particle 0 and write nothing).
+#define RTP_QUAD_PROLOGUE                                                     \
+  __shared__ QuadLds lds;                                                     \
+  const uint32_t t = blockIdx.x * kQuadGroups + (threadIdx.x >> 2);           \
+  const bool on = t < n;                                                      \
+  const uint32_t i = on ? po.particle(t) : 0u;                                \
+  const bool lead = on && (threadIdx.x & 3u) == 0u;                           \
+  const bool fast = (c.shortMath & kShortSqrtGrad) != 0;
+
+// k_density_lambda (MODE 0 or kNbRecord) with a quad per particle.
+template <int NB>
+__global__ __launch_bounds__(kQuadBlock) void k_density_lambda_q(const float4* __restrict__ pred,
+                                                                 const uint2* __restrict__ se,
+                                                                 float* __restrict__ density,
+                                                                 float* __restrict__ lambda,
+                                                                 uint32_t n, PbfConsts c,
+                                                                 PassOrder po)
+{
+  RTP_QUAD_PROLOGUE
+  const float4 p = ldp<false>(pred, i, c);
+  QuadDensity b{ c, fast };
+  quad_neighbours<NB>(on, i, t, on, p, pred, se, c, po.nb, lds, b);
+  if (lead)
+    density_lambda_out(i, b, density, lambda, c);
+}
+
+// k_correct (MODE 0 or kNbReplay) with a quad per particle.
+template <int NB>
+__global__ __launch_bounds__(kQuadBlock) void k_correct_q(const float4* __restrict__ pred,
+                                                          const float* __restrict__ lambda,
+                                                          const uint2* __restrict__ se,
+                                                          float4* __restrict__ predOut,
+                                                          const float4* __restrict__ pos,
+                                                          float4* __restrict__ vel,
+                                                          float4* __restrict__ posOut, uint32_t n,
+                                                          int last, PbfConsts c, PassOrder po)
+{
+  RTP_QUAD_PROLOGUE
+  const float4 p = ldp<false>(pred, i, c);
+  const float ywdq = (c.shortMath & kShortArtDiv) ? rcp_refined(c.wdq) : 0.0f;
+  QuadCorrect b{ c, lambda, fast, lambda[i], ywdq };
+  quad_neighbours<NB>(on, i, t, false, p, pred, se, c, po.nb, lds, b);
+  if (lead)
+    correct_out(i, p, b.cx, b.cy, b.cz, predOut, pos, vel, posOut, last, c, po);
+}
+
+// k_vorticity with a quad per particle.
+template <int NB>
+__global__ __launch_bounds__(kQuadBlock) void k_vorticity_q(const float4* __restrict__ pred,
+                                                            const uint2* __restrict__ se,
+                                                            const float4* __restrict__ vel,
+                                                            float4* __restrict__ vort, uint32_t n,
+                                                            PbfConsts c, PassOrder po)
+{
+  RTP_QUAD_PROLOGUE
+  const float4 p = pred[i];
+  QuadVorticity b{ c, vel, fast, vel[i] };
+  quad_neighbours<NB>(on, i, t, on, p, pred, se, c, po.nb, lds, b);
+  if (lead)
+    vort[i] = make_float4(b.wx, b.wy, b.wz, __builtin_sqrtf(b.wx * b.wx + b.wy * b.wy + b.wz * b.wz));
+}
+
+// k_confinement with a quad per particle.
+template <int NB>
+__global__ __launch_bounds__(kQuadBlock) void k_confinement_q(const float4* __restrict__ pred,
+                                                              const uint2* __restrict__ se,
+                                                              const float4* __restrict__ vort,
+                                                              const float4* __restrict__ vel,
+                                                              float4* __restrict__ velOut,
+                                                              uint32_t n, PbfConsts c, PassOrder po)
+{
+  RTP_QUAD_PROLOGUE
+  const float4 p = pred[i];
+  QuadConfinement b{ c, vort, fast };
+  quad_neighbours<NB>(on, i, t, false, p, pred, se, c, po.nb, lds, b);
+  if (lead)
+    confinement_out(i, b.nx, b.ny, b.nz, vort, vel, velOut, c);
+}
+
+// k_xsph with a quad per particle.
+template <int NB>
+__global__ __launch_bounds__(kQuadBlock) void k_xsph_q(const float4* __restrict__ pred,
+                                                       const uint2* __restrict__ se,
+                                                       const float4* __restrict__ velIn,
+                                                       float4* __restrict__ velOut,
+                                                       float4* __restrict__ posOut, uint32_t n,
+                                                       PbfConsts c, PassOrder po)
+{
+  RTP_QUAD_PROLOGUE
+  const float4 p = pred[i];
+  QuadXsph b{ c, velIn, fast, velIn[i] };
+  quad_neighbours<NB>(on, i, t, false, p, pred, se, c, po.nb, lds, b);
+  if (lead) {
+    const float k = c.p.xsphViscosityCoeff;
+    velOut[i] = make_float4(b.vi.x + k * b.sx, b.vi.y + k * b.sy, b.vi.z + k * b.sz, 0.0f);
+    posOut[i] = make_float4(p.x, p.y, p.z, 0.0f);
+  }
+}
+#undef RTP_QUAD_PROLOGUE
 
 // --------------------------------------------------------------- LDS-tiled wrappers
 // Same passes with the wave tiles staged first.  Every lane takes part in staging (lanes
@@ -1875,6 +2102,12 @@ void launch_pbf_predict(const float4* pos, const float4* vel, float4* pred, uint
                      startEnd, n, c);
 }
 
+// A quad pass's grid: sixteen particles per 64-lane workgroup.
+static dim3 quad_grid(uint32_t n)
+{
+  return dim3(blocks_for(n, kQuadGroups));
+}
+
 void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float* density,
                                float* lambda, uint32_t n, const PbfConsts& c, const Launch& L,
                                hipStream_t s, int mode, const LearnArgs& la, const PassOrder& po)
@@ -1883,6 +2116,12 @@ void launch_pbf_density_lambda(const float4* pred, const uint2* startEnd, float*
     return;
   if (L.tile)  // survivor lists are kept by the untiled kernels only
     mode &= ~(kNbRecord | kNbReplay);
+  if (L.quad && !L.tile && (mode & ~kNbRecord) == 0) {
+    hipLaunchKernelGGL(mode ? k_density_lambda_q<kNbRecord> : k_density_lambda_q<0>,
+                       quad_grid(n), dim3(kQuadBlock), 0, s, pred, startEnd, density, lambda, n,
+                       c, po);
+    return;
+  }
   const dim3 g(blocks_for(n, L.block)), b(L.block);
   const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_DL(M)                                                                              \
@@ -1916,6 +2155,12 @@ void launch_pbf_correct(const float4* pred, const float* lambda, const uint2* st
     return;
   if (L.tile)
     mode &= ~(kNbRecord | kNbReplay);
+  if (L.quad && !L.tile && (mode & ~kNbReplay) == 0) {
+    hipLaunchKernelGGL(mode ? k_correct_q<kNbReplay> : k_correct_q<0>, quad_grid(n),
+                       dim3(kQuadBlock), 0, s, pred, lambda, startEnd, predOut, pos, vel, posOut,
+                       n, last, c, po);
+    return;
+  }
   const dim3 g(blocks_for(n, L.block)), b(L.block);
   const uint32_t q = (uint32_t)L.queue, lds = queue_bytes(L);
 #define RTP_CO(M)                                                                              \
@@ -2018,6 +2263,11 @@ void launch_pbf_vorticity(const float4* pred, const uint2* startEnd, const float
 {
   if (n == 0)
     return;
+  if (L.quad && !L.tile) {
+    hipLaunchKernelGGL(RTP_BY_NB(k_vorticity_q)(nb), quad_grid(n), dim3(kQuadBlock), 0, s, pred,
+                       startEnd, vel, vort, n, c, po);
+    return;
+  }
   hipLaunchKernelGGL(L.tile ? k_vorticity_t : RTP_BY_NB(k_vorticity)(nb),
                      dim3(blocks_for(n, L.block)), dim3(L.block), queue_bytes(L), s, pred,
                      startEnd, vel, vort, n, c, L.xcdRemap, (uint32_t)L.queue, po);
@@ -2029,6 +2279,11 @@ void launch_pbf_confinement(const float4* pred, const uint2* startEnd, const flo
 {
   if (n == 0)
     return;
+  if (L.quad && !L.tile) {
+    hipLaunchKernelGGL(RTP_BY_NB(k_confinement_q)(nb), quad_grid(n), dim3(kQuadBlock), 0, s,
+                       pred, startEnd, vort, vel, velOut, n, c, po);
+    return;
+  }
   hipLaunchKernelGGL(L.tile ? k_confinement_t : RTP_BY_NB(k_confinement)(nb),
                      dim3(blocks_for(n, L.block)), dim3(L.block), queue_bytes(L), s, pred,
                      startEnd, vort, vel, velOut, n, c, L.xcdRemap, (uint32_t)L.queue, po);
@@ -2040,6 +2295,11 @@ void launch_pbf_xsph(const float4* pred, const uint2* startEnd, const float4* ve
 {
   if (n == 0)
     return;
+  if (L.quad && !L.tile) {
+    hipLaunchKernelGGL(RTP_BY_NB(k_xsph_q)(nb), quad_grid(n), dim3(kQuadBlock), 0, s, pred,
+                       startEnd, velIn, velOut, posOut, n, c, po);
+    return;
+  }
   hipLaunchKernelGGL(L.tile ? k_xsph_t : RTP_BY_NB(k_xsph)(nb), dim3(blocks_for(n, L.block)),
                      dim3(L.block), queue_bytes(L), s, pred, startEnd, velIn, velOut, posOut, n,
                      c, L.xcdRemap, (uint32_t)L.queue, po);

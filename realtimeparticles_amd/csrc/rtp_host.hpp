@@ -275,7 +275,7 @@ int upload_f4(float4* dst, const float* src, uint32_t count, uint32_t maxN, floa
 
 // Launch flags a single model / a batch takes.
 constexpr int kFluidsLaunchFlags = RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES | RTP_LAUNCH_RESORT |
-                                   RTP_LAUNCH_NB_LIST | RTP_LAUNCH_BRICKS;
+                                   RTP_LAUNCH_NB_LIST | RTP_LAUNCH_BRICKS | RTP_LAUNCH_QUAD;
 constexpr int kBatchLaunchFlags = RTP_LAUNCH_RESORT | RTP_LAUNCH_NB_LIST;
 
 // Validates a launch tuning; allowedFlags masks the RTP_LAUNCH_* bits the caller supports.
@@ -286,6 +286,8 @@ int check_tuning(const rtp_tuning& t, int allowedFlags)
   RTP_REQUIRE((t.flags & ~allowedFlags) == 0, "launch flags not supported here");
   RTP_REQUIRE(!(t.flags & RTP_LAUNCH_BRICKS) || (t.flags & RTP_LAUNCH_RESORT),
               "RTP_LAUNCH_BRICKS orders the threads of RTP_LAUNCH_RESORT: set both");
+  RTP_REQUIRE(!(t.flags & RTP_LAUNCH_QUAD) || !(t.flags & RTP_LAUNCH_LDS_TILES),
+              "RTP_LAUNCH_QUAD and RTP_LAUNCH_LDS_TILES exclude each other");
   RTP_REQUIRE(t.sortBits == 0 || (t.sortBits >= 4 && t.sortBits <= 11), "sortBits must be 0 or 4..11");
   const int q = t.queueDepth ? t.queueDepth : kDefaultQueue;
   RTP_REQUIRE(q >= 8 && q <= 64, "queueDepth must be 0 or 8..64");
@@ -297,7 +299,9 @@ int check_tuning(const rtp_tuning& t, int allowedFlags)
 // Times fixed-input substeps for candidate tunings and returns the fastest in *best.
 // set(t) applies a tuning, steps(k) runs k fixed-input substeps.  The sort's digit width
 // only affects the sort, so it is chosen first; then every valid combination of block size,
-// launch flags (single models) and queue depth is timed with it.
+// launch flags (single models) and queue depth is timed with it.  The quad passes of a
+// single model ignore the block size and queue depth, so their flag combinations are timed
+// last, once each, with the best lane-per-particle block size and depth.
 template <class Set, class Steps>
 int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, bool singleModel,
                 Set set, Steps steps, rtp_tuning* best)
@@ -334,26 +338,40 @@ int tune_search(hipStream_t s, uint32_t trialSteps, const rtp_tuning& start, boo
       top.msPerStep = ms;
     }
   }
+  auto consider = [&](const rtp_tuning& t) -> int {
+    if (check_tuning(t, singleModel ? kFluidsLaunchFlags : kBatchLaunchFlags))
+      return RTP_OK;
+    float ms = 0.0f;
+    const int r = time_one(t, &ms);
+    if (!r && ms < top.msPerStep) {
+      top = t;
+      top.msPerStep = ms;
+    }
+    return r;
+  };
   // every combination of the flags the caller takes
-  const int flagChoices = singleModel ? kFluidsLaunchFlags + 1 : 4;
+  const int laneFlags = singleModel ? kFluidsLaunchFlags & ~RTP_LAUNCH_QUAD : 3;
   for (int b = 64; b <= 256 && !rc; b += 64)
-    for (int fi = 0; fi < flagChoices && !rc; ++fi)
+    for (int fi = 0; fi <= laneFlags && !rc; ++fi)
       for (int q : { 8, 12, 16, 24, 32 }) {
         const int f = singleModel ? fi : ((fi & 1) ? RTP_LAUNCH_RESORT : 0) | ((fi & 2) ? RTP_LAUNCH_NB_LIST : 0);
         rtp_tuning t = top;
         t.blockSize = b;
         t.flags = f;
         t.queueDepth = q;
-        if (check_tuning(t, singleModel ? kFluidsLaunchFlags : kBatchLaunchFlags))
-          continue;
-        float ms = 0.0f;
-        if ((rc = time_one(t, &ms)))
+        if ((rc = consider(t)))
           break;
-        if (ms < top.msPerStep) {
-          top = t;
-          top.msPerStep = ms;
-        }
       }
+  if (singleModel) {
+    const rtp_tuning lane = top;
+    for (int fi = 0; fi <= laneFlags && !rc; ++fi) {
+      if (fi & (RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_LDS_TILES))  // no effect on / not with quads
+        continue;
+      rtp_tuning t = lane;
+      t.flags = fi | RTP_LAUNCH_QUAD;
+      rc = consider(t);
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc)

@@ -116,6 +116,7 @@ struct Launch {
   int nbList = 0;      // 1: the second pass of each pair over the same positions replays the
                        // first one's survivor lists
   int bricks = 0;      // 1 (with resort): thread order by brick_key instead of the cell index
+  int quad = 0;        // 1: the plain solver's neighbour passes run a quad of lanes per particle
 };
 
 // Tile edge, in cells, of the (y, z) pencils of the brick thread order.

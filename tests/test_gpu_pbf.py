@@ -65,10 +65,11 @@ def test_pbf_parameter_variants(variant):
     gpu.close()
 
 
-@pytest.mark.parametrize("flags", [0, 13, 29])
+@pytest.mark.parametrize("flags", [0, 13, 29, 64, 76])
 def test_graph_replay_equals_direct_launches(flags):
     """flags 13: XCD blocks, thread order and survivor lists inside the captured graphs; 29:
-    the same with the brick thread order."""
+    the same with the brick thread order; 64 and 76: the quad passes, alone and with thread
+    order and lists."""
     sc = small_scene(seed=4)
     a, _ = make_pbf_pair(sc, iters=4)
     b, _ = make_pbf_pair(sc, iters=4)
@@ -93,7 +94,8 @@ def test_graph_replay_equals_direct_launches(flags):
     b.close()
 
 
-@pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11), (29, 9)])
+@pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11), (29, 9),
+                                             (76, 0), (92, 9)])
 def test_nb_particles_change_keeps_reference_sort(flags, sort_bits):
     # Model::setNbParticles (Model.hpp:106) does not reset the grid: the keys of particles
     # that left the active range stay in the reference's whole-buffer sort.  flags 13: the
@@ -193,14 +195,16 @@ def test_checkpoint_restore_replays_identically():
     (256, 2, 16, 0), (128, 2, 24, 7), (64, 3, 48, 0), (192, 1, 64, 10), (256, 3, 16, 11),
     (256, 0, 64, 0), (128, 0, 8, 8), (64, 4, 64, 0), (128, 6, 32, 9), (256, 7, 16, 0),
     (192, 5, 40, 4), (64, 13, 16, 10), (256, 8, 8, 0), (128, 12, 24, 11), (64, 15, 12, 7),
-    (64, 13, 16, 5), (128, 13, 16, 6), (64, 29, 16, 10), (256, 20, 24, 0), (128, 31, 12, 8)])
+    (64, 13, 16, 5), (128, 13, 16, 6), (64, 29, 16, 10), (256, 20, 24, 0), (128, 31, 12, 8),
+    (64, 64, 16, 0), (128, 68, 16, 10), (256, 72, 8, 0), (64, 76, 16, 9), (64, 92, 16, 10),
+    (128, 84, 24, 7)])
 @pytest.mark.parametrize("mapping", ["literal", "stable"])
 def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mapping):
     """XCD-contiguous block order, LDS wave tiles, thread re-ordering by current cell (in
-    cell-index or brick order), the survivor-queue depth, replayed survivor lists and the
-    sort's digit width change where
-    candidates are read from, which thread handles a particle and when the neighbour bodies
-    run -- never which candidates or in which order."""
+    cell-index or brick order), the survivor-queue depth, replayed survivor lists, the quad
+    passes (four lanes per particle, terms added in walk order) and the sort's digit width
+    change where candidates are read from, which thread handles a particle and when the
+    neighbour bodies run -- never which candidates or in which order."""
     sc = scenes.dam_break((48, 24, 24), (4, 2, 2), (40, 20, 20), seed=6)
     prm = sc.params(mapping)
     gpu, orc = make_pbf_pair(sc, iters=3, **{k: v for k, v in prm.items()})
@@ -215,7 +219,8 @@ def test_launch_shapes_do_not_change_bits(block, flags, queue, sort_bits, mappin
 
 
 @pytest.mark.parametrize("block,queue,flags", [(256, 8, 0), (64, 64, 0), (128, 48, 0),
-                                               (64, 16, 13), (256, 12, 8), (64, 16, 29)])
+                                               (64, 16, 13), (256, 12, 8), (64, 16, 29),
+                                               (64, 16, 64), (64, 16, 72), (64, 16, 92)])
 def test_overfull_cell_keeps_bits(block, queue, flags):
     """20,000 particles in one cell (the cap iterates 101 of them) and 50 in the next cell
     of its column: the walks through them are no longer back-to-back ranges, and survivor
@@ -331,6 +336,30 @@ def test_pbf_65k_long_run(mapping):
     gpu.close()
 
 
+@pytest.mark.parametrize("flags", [capi.LAUNCH_QUAD,
+                                   capi.LAUNCH_QUAD | capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST])
+@pytest.mark.parametrize("mapping", ["literal", "stable"])
+def test_pbf_65k_quad_passes_follow_oracle_trace(mapping, flags):
+    """The 64x32x32 dam break with the quad passes (graph replay, 600 substeps): the SHA-256
+    of positions and velocities against the oracle's run every 30 substeps."""
+    trace = _trace()[mapping]
+    sc = scenes.pbf_scene("pbf-65k")
+    prm = sc.params(mapping)
+    gpu, _ = make_pbf_pair(sc, iters=4, **prm)
+    gpu.set_tuning(64, flags, 0, 16)
+    hashes = {h["substep"]: h for h in trace["state_sha256"]}
+    done = 0
+    while done < 600:
+        gpu.update(30)
+        done += 30
+        log(f"pbf-65k quad {mapping} flags {flags} substep {done}")
+        pos, vel = gpu.read_state()
+        want = hashes[done]
+        assert _digest(pos) == want["pos"] and _digest(vel) == want["vel"], \
+            f"{mapping}: state after substep {done} differs from the oracle's run"
+    gpu.close()
+
+
 def test_short_math_general_lanes_keep_bits():
     """Neighbour pairs that send single lanes down the general correctly rounded forms inside
     the short ones: squared distances in (0, 2^-96) (pairs 1e-16 and 1e-20 apart, next to
@@ -374,7 +403,7 @@ def test_general_forms_outside_short_range(box):
     gpu.close()
 
 
-@pytest.mark.parametrize("flags", [0, 13, 29])
+@pytest.mark.parametrize("flags", [0, 13, 29, 76])
 def test_bad_state_matches_oracle(flags):
     """NaN and +-inf coordinates, a NaN velocity, particles outside the walls and exactly on
     +-W in a dam break: the hash aliases and clamps them (grid.cl:14-38), the first clamp
