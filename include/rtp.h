@@ -111,11 +111,11 @@ typedef struct {
                                   pencils of 8 x 8 cells in (y, z), pencil after pencil, instead
                                   of in cell index order, so an XCD's consecutive waves keep
                                   their neighbours' data in its L2 on large grids */
-#define RTP_LAUNCH_QUAD 64     /* fluids: the plain solver's neighbour passes give each particle
-                                  four lanes (sixteen particles per 64-lane workgroup) that
-                                  split its walk and add the terms in the same order; for
-                                  scenes too small to fill the GPU one lane per particle
-                                  (results unchanged; not with RTP_LAUNCH_LDS_TILES) */
+#define RTP_LAUNCH_QUAD 64     /* fluids and clouds: the neighbour passes (of the plain solver)
+                                  give each particle four lanes (sixteen particles per 64-lane
+                                  workgroup) that split its walk and add the terms in the same
+                                  order; for scenes too small to fill the GPU one lane per
+                                  particle (results unchanged; not with RTP_LAUNCH_LDS_TILES) */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {
@@ -459,6 +459,10 @@ int rtp_clouds_destroy(rtp_clouds* h);
 /* transferJsonInputsToModel's typed result (Clouds.cpp:253-303); either pointer may be NULL. */
 int rtp_clouds_set_params(rtp_clouds* h, const rtp_fluid_params* p, const rtp_cloud_params* q);
 int rtp_clouds_get_params(rtp_clouds* h, rtp_fluid_params* p, rtp_cloud_params* q);
+/* Launch choice of the walk passes (no reference counterpart): t->flags RTP_LAUNCH_QUAD gives
+ * each particle a quad of four lanes (the default when maxNbParticles <= 262,144), 0 one lane;
+ * the other fields are ignored.  Results are the same either way. */
+int rtp_clouds_set_tuning(rtp_clouds* h, const rtp_tuning* t);
 /* m_nbJacobiIters (Clouds.hpp:71) */
 int rtp_clouds_set_nb_jacobi_iters(rtp_clouds* h, uint32_t iters);
 int rtp_clouds_get_nb_jacobi_iters(rtp_clouds* h, uint32_t* iters);

@@ -286,6 +286,7 @@ _SIGS = {
     "rtp_clouds_destroy": ([_vp], _int),
     "rtp_clouds_set_params": ([_vp, _vp, _vp], _int),
     "rtp_clouds_get_params": ([_vp, _vp, _vp], _int),
+    "rtp_clouds_set_tuning": ([_vp, ctypes.POINTER(Tuning)], _int),
     "rtp_clouds_set_nb_jacobi_iters": ([_vp, _u32], _int),
     "rtp_clouds_get_nb_jacobi_iters": ([_vp, ctypes.POINTER(_u32)], _int),
     "rtp_clouds_set_nb_particles": ([_vp, _u32], _int),
@@ -1010,6 +1011,11 @@ class Clouds:
 
     def set_nb_jacobi_iters(self, iters: int) -> None:
         self._call("set_nb_jacobi_iters", int(iters))
+
+    def set_quad(self, on: bool) -> None:
+        """Walk passes with a quad of lanes per particle (LAUNCH_QUAD) or one lane."""
+        t = _tuning(64, LAUNCH_QUAD if on else 0, 0, 0)
+        self._call("set_tuning", ctypes.byref(t))
 
     def set_nb_particles(self, n: int) -> None:
         self._call("set_nb_particles", int(n))

@@ -229,11 +229,12 @@ __device__ __forceinline__ float4 cld_shift(uint32_t code, const GridConsts& g)
 }
 
 // The non-empty candidate ranges of the 27-cell walk of clouds.cl:322-353 in its order (x
-// outer, z inner), as {first | shift code << 28, last}, every kCldLanes-th entry of out from
+// outer, z inner), as {first | shift code << 28, last}, every STRIDE-th entry of out from
 // out[0]; returns their count.  A wrapped neighbour's position is shifted by the code's
 // +-2 absWall; y cells beyond the walls are skipped.  The three cells of a z-column are one
 // range when none of them wraps in z and their non-empty ranges are back to back (cells are
 // numbered z-minor), which lists the same candidates in the same order.
+template <uint32_t STRIDE = kCldLanes>
 __device__ __forceinline__ uint32_t cld_ranges(const float4 pos, const uint2* __restrict__ se,
                                                const GridConsts& g, uint2* __restrict__ out)
 {
@@ -244,7 +245,7 @@ __device__ __forceinline__ uint32_t cld_ranges(const float4 pos, const uint2* __
   const bool zInside = cz - 1 >= 0 && cz + 1 < rz;
   uint32_t nr = 0;
   const auto push = [&](uint32_t lo, uint32_t hi, uint32_t code) {
-    out[nr * kCldLanes] = make_uint2(lo | (code << kCldIndexBits), hi);
+    out[nr * STRIDE] = make_uint2(lo | (code << kCldIndexBits), hi);
     ++nr;
   };
 #pragma unroll
@@ -500,6 +501,144 @@ __device__ __forceinline__ bool finite4(float4 v)
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
+// ------------------------------------------------------------------------ quad walks
+// With quads (the default for scenes up to kCldQuadMaxN particles) each walk kernel gives a
+// particle four lanes of a 64-lane workgroup.  The quad builds the particle's ranges
+// (cld_ranges) in LDS and scans them sixteen candidates per step, lane u taking candidates
+// u, u + 4, u + 8 and u + 12 with the pass's keep test; the kept ones go to a queue of
+// kCqQueue entries per particle in LDS, in walk order.  A drain runs four queued bodies at
+// once, one per lane, and adds the four terms into every sum in walk order (quad_add), so
+// each sum adds the same terms in the same order as cld_walk's one-lane walk; lanes
+// without an entry add +0, which leaves a sum started at +0 unchanged.  Survivor lists keep
+// cld_walk's layout and codes, so either kind of walk replays the other's lists.
+constexpr uint32_t kCqGroups = kCldLanes / 4u;  // particles per workgroup
+constexpr uint32_t kCqQueue = 64;               // queued survivors per particle
+constexpr uint32_t kCldQuadMaxN = 1u << 18;     // above this, one lane per particle fills the GPU
+
+struct CqLds {
+  uint32_t queue[kCqGroups][kCqQueue];
+  uint2 ranges[kCqGroups][kCldRanges];
+};
+
+// The body over entries 0..cnt-1 of the quad (entry(k): entry k, an index with its shift
+// code), four at a time, loading each round's data during the previous round's bodies;
+// rec(v, k, code) sees every round first.  The wave loops until its longest quad is done;
+// lanes without an entry run the body on the particle itself and add +0.
+template <class S, class R, class B>
+__device__ __forceinline__ void cq_run(uint32_t cnt, uint32_t self, const float4 pos,
+                                       const float4* __restrict__ src, const GridConsts& g,
+                                       S&& entry, R&& rec, B& b)
+{
+  const uint32_t u = threadIdx.x & 3u;
+  bool v = u < cnt;
+  uint32_t code = v ? entry(u) : self;
+  float4 o = src[code & kCldIndexMask];
+  auto x = b.ld(code & kCldIndexMask);
+  for (uint32_t k = u; __builtin_amdgcn_ballot_w64(k - u < cnt) != 0; k += 4u) {
+    const uint32_t k1 = k + 4u;
+    const bool v1 = k1 < cnt;
+    const uint32_t code1 = v1 ? entry(k1) : self;
+    const float4 o1 = src[code1 & kCldIndexMask];
+    const auto x1 = b.ld(code1 & kCldIndexMask);
+    rec(v, k, code);
+    b.add(v, x, sub4(sub4(pos, o), cld_shift(code >> kCldIndexBits, g)));
+    v = v1;
+    code = code1;
+    o = o1;
+    x = x1;
+  }
+}
+
+// The quad's walk of particle `self` over its nr ranges rl (in LDS), queued in q; with REC
+// (and `record`) the kept candidates go to the particle's survivor list.
+template <bool REC, class B>
+__device__ __forceinline__ void cq_walk(uint32_t nr, const uint2* rl, uint32_t* q, uint32_t self,
+                                        const float4 pos, const float4* __restrict__ src,
+                                        const GridConsts& g, B& b, const CldList& list,
+                                        bool record)
+{
+  const uint32_t u = threadIdx.x & 3u, below = (1u << u) - 1u;
+  uint32_t ri = 0, e = 1u, hi = 0u, tag = 0u, qn = 0, tot = 0;
+  float4 shift = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const auto range = [&](uint32_t k) {
+    const uint2 r = rl[k];
+    e = r.x & kCldIndexMask;
+    tag = r.x & ~kCldIndexMask;
+    hi = r.y;
+    shift = cld_shift(r.x >> kCldIndexBits, g);
+  };
+  if (nr > 0)
+    range(0);
+  const auto drain = [&]() {
+    wave_sync();
+    cq_run(qn, self, pos, src, g, [&](uint32_t k) { return q[k]; },
+           [&](bool v, uint32_t k, uint32_t code) {
+             if (REC && v & record & (tot + k < kCldListCap))
+               list.entry[(tot + k) * list.stride + self] = code;
+           }, b);
+    tot += qn;
+    qn = 0;
+    wave_sync();
+  };
+  while (__builtin_amdgcn_ballot_w64(ri < nr) != 0) {
+    const bool act = ri < nr;
+    uint32_t code[4];
+    bool kept[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t ej = e + u + 4u * (uint32_t)j;
+      const bool valid = act & (ej <= hi);
+      const uint32_t at = valid ? ej : self;
+      const float4 o = src[at];
+      kept[j] = valid & b.keep(b.ld(at), sub4(sub4(pos, o), shift));
+      code[j] = ej | tag;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t m = quad_bits(kept[j]);
+      if (kept[j])
+        q[qn + (uint32_t)__builtin_popcount(m & below)] = code[j];
+      qn += (uint32_t)__builtin_popcount(m);
+    }
+    e += 16u;
+    if (act & (e > hi)) {
+      ++ri;
+      if (ri < nr)
+        range(ri);
+    }
+    if (__builtin_amdgcn_ballot_w64(qn > kCqQueue - 16u) != 0)
+      drain();
+  }
+  drain();
+  if (REC && record && u == 0u)
+    list.count[self] = tot <= kCldListCap ? tot : kCldListOverflow;
+}
+
+// The candidates of particle i into the body b: a recording walk (MODE kCqRecord), a
+// replay of its list when useList and the list is complete, else a walk (kCqReplay), or a
+// walk (kCqWalk).
+constexpr int kCqWalk = 0, kCqRecord = 1, kCqReplay = 2;
+template <int MODE, class B>
+__device__ __forceinline__ void cq_neighbours(bool on, uint32_t i, bool useList, const float4 pos,
+                                              const float4* __restrict__ src,
+                                              const uint2* __restrict__ se, const GridConsts& g,
+                                              const CldList& list, CqLds& lds, B& b)
+{
+  const uint32_t gq = threadIdx.x >> 2;
+  bool listed = false;
+  if constexpr (MODE == kCqReplay) {
+    const uint32_t cnt = (on & useList) ? list.count[i] : kCldListOverflow;
+    listed = cnt != kCldListOverflow;
+    cq_run(listed ? cnt : 0u, i, pos, src, g,
+           [&](uint32_t m) { return list.entry[m * list.stride + i]; },
+           [](bool, uint32_t, uint32_t) {}, b);
+  }
+  const uint32_t nr = (on & !listed) ? cld_ranges<1>(pos, se, g, lds.ranges[gq]) : 0u;
+  // a recording walk writes every recorded count, also of a particle without candidates
+  if (MODE == kCqRecord || __builtin_amdgcn_ballot_w64(nr > 0u) != 0)
+    cq_walk<MODE == kCqRecord>(nr, lds.ranges[gq], lds.queue[gq], i, pos, src, g, b, list, on);
+}
+
 // --------------------------------------------------------------------------------- kernels
 
 // cld_initTemperature (clouds.cl:116-121), cld_initVaporDensity (:127-133), and the host's
@@ -595,6 +734,66 @@ __global__ __launch_bounds__(256) void k_cld_gather(const uint32_t* __restrict__
     a.dst[k][i] = a.src[k][j];
 }
 
+// The outputs of the walk kernels from their sums (shared by the one-lane and quad kernels);
+// those the next pass multiplies are returned for cld_flag.
+__device__ __forceinline__ float cld_temp_lambda_out(uint32_t i, float laplacian, float sumGradCi,
+                                                     float sumSqGradC, float* __restrict__ lap,
+                                                     float* __restrict__ lambdaT,
+                                                     const CloudConsts& c)
+{
+  const float l = laplacian / c.q.restDensity;
+  lap[i] = l;
+  sumSqGradC = sumSqGradC + sumGradCi * sumGradCi;
+  const float lt = -l / (sumSqGradC + c.q.relaxCFM);
+  lambdaT[i] = lt;
+  return lt;
+}
+
+__device__ __forceinline__ float cld_density_lambda_out(uint32_t i, float fluidDensity,
+                                                        float4 sumGradCi, float sumSqGradC,
+                                                        float* __restrict__ density,
+                                                        float* __restrict__ lambda,
+                                                        const CloudConsts& c)
+{
+  density[i] = fluidDensity;
+  const rtp_fluid_params& p = c.f.p;
+  const float densityC = fluidDensity / p.restDensity - 1.0f;
+  sumSqGradC = sumSqGradC + dot4(sumGradCi, sumGradCi);
+  sumSqGradC = sumSqGradC / (p.restDensity * p.restDensity);
+  const float li = -densityC / (sumSqGradC + p.relaxCFM);
+  lambda[i] = li;
+  return li;
+}
+
+__device__ __forceinline__ void cld_correct_out(uint32_t i, const float4 pos, const float4 corr,
+                                                float4* __restrict__ predOut,
+                                                float4* __restrict__ totCorr,
+                                                float4* __restrict__ vel, const CloudConsts& c)
+{
+  const float4 corrPos = div4s(corr, c.f.p.restDensity);
+  predOut[i] = mixed_boundary(add4(pos, corrPos), c.f.g);
+  const float4 tc = add4(totCorr[i], corrPos);
+  totCorr[i] = tc;
+  if (vel) {
+    const float m = c.f.maxVel;
+    const float4 u = div4s(tc, c.f.p.timeStep + kFloatEps);
+    vel[i] = make_float4(clampf(u.x, -m, m), clampf(u.y, -m, m), clampf(u.z, -m, m),
+                         clampf(u.w, -m, m));
+  }
+}
+
+__device__ __forceinline__ float4 cld_confinement_out(uint32_t i, const float4 nn,
+                                                      const float4 vorticity,
+                                                      const float4* __restrict__ vel,
+                                                      float4* __restrict__ velOut,
+                                                      const CloudConsts& c)
+{
+  const rtp_fluid_params& p = c.f.p;
+  const float4 vo = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
+  velOut[i] = vo;
+  return vo;
+}
+
 // cld_computeLaplacianTemp (clouds.cl:508-569) and cld_computeConstraintFactorTemp
 // (:575-648) in one walk over p_pos: the two sums are independent and each keeps the
 // reference's order.
@@ -627,11 +826,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda(const float4* __r
     sumGradCi = sumGradCi + derivativeTemp;
     sumSqGradC = sumSqGradC + derivativeTemp * derivativeTemp;
   });
-  const float l = laplacian / rho0;
-  lap[i] = l;
-  sumSqGradC = sumSqGradC + sumGradCi * sumGradCi;
-  const float lt = -l / (sumSqGradC + c.q.relaxCFM);
-  lambdaT[i] = lt;
+  const float lt = cld_temp_lambda_out(i, laplacian, sumGradCi, sumSqGradC, lap, lambdaT, c);
   cld_flag(cld_safe(lt), flags, kFlagLambdaT);
 }
 
@@ -691,13 +886,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda(const float4* 
     sumGradCi = add4(sumGradCi, grad);
     sumSqGradC = sumSqGradC + dot4(grad, grad);
   });
-  density[i] = fluidDensity;
-  const rtp_fluid_params& p = c.f.p;
-  const float densityC = fluidDensity / p.restDensity - 1.0f;
-  sumSqGradC = sumSqGradC + dot4(sumGradCi, sumGradCi);
-  sumSqGradC = sumSqGradC / (p.restDensity * p.restDensity);
-  const float li = -densityC / (sumSqGradC + p.relaxCFM);
-  lambda[i] = li;
+  const float li = cld_density_lambda_out(i, fluidDensity, sumGradCi, sumSqGradC, density, lambda, c);
   if (bit)
     cld_flag(cld_safe(li), flags, bit);
 }
@@ -732,16 +921,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_correct(const float4* __restr
     const float s = lambdaI + le + art_pressure_w(poly6_r(r, c.f), ywdq, c.f);
     corr = add4(corr, muls4(s, grad_spiky_fast(vec, r, c.f)));
   });
-  const float4 corrPos = div4s(corr, c.f.p.restDensity);
-  predOut[i] = mixed_boundary(add4(pos, corrPos), c.f.g);
-  const float4 tc = add4(totCorr[i], corrPos);
-  totCorr[i] = tc;
-  if (vel) {
-    const float m = c.f.maxVel;
-    const float4 u = div4s(tc, c.f.p.timeStep + kFloatEps);
-    vel[i] = make_float4(clampf(u.x, -m, m), clampf(u.y, -m, m), clampf(u.z, -m, m),
-                         clampf(u.w, -m, m));
-  }
+  cld_correct_out(i, pos, corr, predOut, totCorr, vel, c);
 }
 
 // cld_updateVel alone (a step without Jacobi iterations).
@@ -808,9 +988,7 @@ __global__ __launch_bounds__(kCldLanes) void k_cld_confinement(const float4* __r
     const float lw = len_of(dot4(we, we));
     nn = add4(nn, muls4(lw, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
   });
-  const rtp_fluid_params& p = c.f.p;
-  const float4 vo = add4(vel[i], mul4s(muls4(p.vorticityConfCoeff, cross4(normalize4(nn), vorticity)), p.timeStep));
-  velOut[i] = vo;
+  const float4 vo = cld_confinement_out(i, nn, vorticity, vel, velOut, c);
   cld_flag(cld_safe(vo), flags, kFlagVelTmp);
 }
 
@@ -859,6 +1037,291 @@ __global__ __launch_bounds__(256) void k_cld_update_pos(const float4* __restrict
   if (i < n)
     pos[i] = cld_update_pos(pred[i], c);
 }
+
+// --------------------------------------------------------------- quad walk kernels
+// The walk kernels above with a quad of lanes per particle (cq_neighbours): the same sums in
+// the same order, the same survivor lists and flags.
+
+// s + t component by component in quad order (t counts only when v).
+__device__ __forceinline__ float4 quad_add4(float4 s, bool v, const float4 t)
+{
+  return make_float4(quad_add(s.x, v ? t.x : 0.0f), quad_add(s.y, v ? t.y : 0.0f),
+                     quad_add(s.z, v ? t.z : 0.0f), quad_add(s.w, v ? t.w : 0.0f));
+}
+
+struct CqTempLambda {
+  const CloudConsts& c;
+  const float* __restrict__ tempP;
+  float temp, rho0;
+  float laplacian = 0.0f, sumGradCi = 0.0f, sumSqGradC = 0.0f;
+  __device__ __forceinline__ float ld(uint32_t e) const { return tempP[e]; }
+  __device__ __forceinline__ bool keep(float te, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & __builtin_isfinite(temp - te));
+  }
+  __device__ __forceinline__ void add(bool v, float te, float4 vec)
+  {
+    const float dTemp = temp - te;
+    const float vv = dot4(vec, vec);
+    const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
+    const float vg = dot4(vec, grad);
+    const float lapTerm = dTemp * vg / (vv + kFloatEps);
+    const float derivativeTemp = vg / (vv * rho0 + kFloatEps);
+    laplacian = quad_add(laplacian, v ? lapTerm : 0.0f);
+    sumGradCi = quad_add(sumGradCi, v ? derivativeTemp : 0.0f);
+    sumSqGradC = quad_add(sumSqGradC, v ? derivativeTemp * derivativeTemp : 0.0f);
+  }
+};
+
+struct CqTempCorrect {
+  const CloudConsts& c;
+  const float* __restrict__ lambdaT;
+  float lambdaI, rho0;
+  float corr = 0.0f;
+  __device__ __forceinline__ float ld(uint32_t e) const { return lambdaT[e]; }
+  __device__ __forceinline__ bool keep(float le, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
+  }
+  __device__ __forceinline__ void add(bool v, float le, float4 vec)
+  {
+    const float l = lambdaI + le;
+    const float vv = dot4(vec, vec);
+    const float4 grad = grad_spiky_fast(vec, len_of(vv), c.f);
+    const float derivativeTemp = dot4(vec, grad) / (vv * rho0 + kFloatEps);
+    corr = quad_add(corr, v ? l * derivativeTemp : 0.0f);
+  }
+};
+
+struct CqDensity {
+  const CloudConsts& c;
+  float fluidDensity = 0.0f;
+  float4 sumGradCi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float sumSqGradC = 0.0f;
+  __device__ __forceinline__ int ld(uint32_t) const { return 0; }
+  __device__ __forceinline__ bool keep(int, float4 vec) const { return !negligible(vec, c.f); }
+  __device__ __forceinline__ void add(bool v, int, float4 vec)
+  {
+    const float r = len_of(dot4(vec, vec));
+    const float w = poly6_r(r, c.f);
+    const float4 grad = grad_spiky_fast(vec, r, c.f);
+    const float sq = dot4(grad, grad);
+    fluidDensity = quad_add(fluidDensity, v ? w : 0.0f);
+    sumGradCi = quad_add4(sumGradCi, v, grad);
+    sumSqGradC = quad_add(sumSqGradC, v ? sq : 0.0f);
+  }
+};
+
+struct CqCorrect {
+  const CloudConsts& c;
+  const float* __restrict__ lambda;
+  float lambdaI, ywdq;
+  float4 corr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __device__ __forceinline__ float ld(uint32_t e) const { return lambda[e]; }
+  __device__ __forceinline__ bool keep(float le, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & __builtin_isfinite(lambdaI + le));
+  }
+  __device__ __forceinline__ void add(bool v, float le, float4 vec)
+  {
+    const float r = len_of(dot4(vec, vec));
+    const float s = lambdaI + le + art_pressure_w(poly6_r(r, c.f), ywdq, c.f);
+    corr = quad_add4(corr, v, muls4(s, grad_spiky_fast(vec, r, c.f)));
+  }
+};
+
+struct CqVorticity {
+  const CloudConsts& c;
+  const float4* __restrict__ vel;
+  float4 velocity;
+  float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __device__ __forceinline__ float4 ld(uint32_t e) const { return vel[e]; }
+  __device__ __forceinline__ bool keep(float4 ve, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & finite4(sub4(ve, velocity)));
+  }
+  __device__ __forceinline__ void add(bool v, float4 ve, float4 vec)
+  {
+    const float4 dv = sub4(ve, velocity);
+    w = quad_add4(w, v, cross4(dv, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
+  }
+};
+
+struct CqConfinement {
+  const CloudConsts& c;
+  const float4* __restrict__ vort;
+  float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __device__ __forceinline__ float4 ld(uint32_t e) const { return vort[e]; }
+  __device__ __forceinline__ bool keep(float4 we, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & __builtin_isfinite(dot4(we, we)));
+  }
+  __device__ __forceinline__ void add(bool v, float4 we, float4 vec)
+  {
+    const float lw = len_of(dot4(we, we));
+    nn = quad_add4(nn, v, muls4(lw, grad_spiky_fast(vec, len_of(dot4(vec, vec)), c.f)));
+  }
+};
+
+struct CqXsph {
+  const CloudConsts& c;
+  const float4* __restrict__ velIn;
+  float4 velocity;
+  float4 viscosity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __device__ __forceinline__ float4 ld(uint32_t e) const { return velIn[e]; }
+  __device__ __forceinline__ bool keep(float4 ve, float4 vec) const
+  {
+    return !(negligible(vec, c.f) & finite4(sub4(ve, velocity)));
+  }
+  __device__ __forceinline__ void add(bool v, float4 ve, float4 vec)
+  {
+    const float4 dv = sub4(ve, velocity);
+    viscosity = quad_add4(viscosity, v, mul4s(dv, poly6_r(len_of(dot4(vec, vec)), c.f)));
+  }
+};
+
+// The quad's particle (quads past n stand in for particle 0 and write nothing).
+#define RTP_CQ_PROLOGUE                                                      \
+  __shared__ CqLds lds;                                                      \
+  const uint32_t q0 = logical_block(1) * kCqGroups + (threadIdx.x >> 2);     \
+  const bool on = q0 < n;                                                    \
+  const uint32_t i = on ? q0 : 0u;                                           \
+  const bool lead = on && (threadIdx.x & 3u) == 0u;
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_temp_lambda_q(const float4* __restrict__ posP,
+                                                                 const float* __restrict__ tempP,
+                                                                 const uint2* __restrict__ se,
+                                                                 float* __restrict__ lap,
+                                                                 float* __restrict__ lambdaT,
+                                                                 uint32_t n, CloudConsts c,
+                                                                 CldList list,
+                                                                 uint32_t* __restrict__ flags)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = posP[i];
+  CqTempLambda b{ c, tempP, tempP[i], c.q.restDensity };
+  cq_neighbours<kCqRecord>(on, i, true, pos, posP, se, c.f.g, list, lds, b);
+  float lt = 0.0f;
+  if (lead)
+    lt = cld_temp_lambda_out(i, b.laplacian, b.sumGradCi, b.sumSqGradC, lap, lambdaT, c);
+  cld_flag(!lead | cld_safe(lt), flags, kFlagLambdaT);
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_temp_correct_q(const float4* __restrict__ posP,
+                                                                  const float* __restrict__ lambdaT,
+                                                                  const uint2* __restrict__ se,
+                                                                  float* __restrict__ temp,
+                                                                  uint32_t n, CloudConsts c,
+                                                                  CldList list,
+                                                                  const uint32_t* __restrict__ flags,
+                                                                  int useList)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = posP[i];
+  CqTempCorrect b{ c, lambdaT, lambdaT[i], c.q.restDensity };
+  cq_neighbours<kCqReplay>(on, i, useList && !(*flags & kFlagLambdaT), pos, posP, se, c.f.g,
+                           list, lds, b);
+  if (lead)
+    temp[i] = temp[i] + 0.3f * b.corr;
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_density_lambda_q(const float4* __restrict__ pred,
+                                                                    const uint2* __restrict__ se,
+                                                                    float* __restrict__ density,
+                                                                    float* __restrict__ lambda,
+                                                                    uint32_t n, CloudConsts c,
+                                                                    CldList list,
+                                                                    uint32_t* __restrict__ flags,
+                                                                    uint32_t bit)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = pred[i];
+  CqDensity b{ c };
+  cq_neighbours<kCqRecord>(on, i, true, pos, pred, se, c.f.g, list, lds, b);
+  float li = 0.0f;
+  if (lead)
+    li = cld_density_lambda_out(i, b.fluidDensity, b.sumGradCi, b.sumSqGradC, density, lambda, c);
+  if (bit)
+    cld_flag(!lead | cld_safe(li), flags, bit);
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_correct_q(const float4* __restrict__ pred,
+                                                             const float* __restrict__ lambda,
+                                                             const uint2* __restrict__ se,
+                                                             float4* __restrict__ predOut,
+                                                             float4* __restrict__ totCorr,
+                                                             float4* __restrict__ vel, uint32_t n,
+                                                             CloudConsts c, CldList list,
+                                                             const uint32_t* __restrict__ flags,
+                                                             uint32_t bit, int useList)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = pred[i];
+  const float ywdq = (c.f.shortMath & kShortArtDiv) ? rcp_refined(c.f.wdq) : 0.0f;
+  CqCorrect b{ c, lambda, lambda[i], ywdq };
+  cq_neighbours<kCqReplay>(on, i, useList && bit && !(*flags & bit), pos, pred, se, c.f.g, list,
+                           lds, b);
+  if (lead)
+    cld_correct_out(i, pos, b.corr, predOut, totCorr, vel, c);
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_vorticity_q(const float4* __restrict__ pred,
+                                                               const uint2* __restrict__ se,
+                                                               const float4* __restrict__ vel,
+                                                               float4* __restrict__ vort, uint32_t n,
+                                                               CloudConsts c, CldList list,
+                                                               uint32_t* __restrict__ flags)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = pred[i];
+  CqVorticity b{ c, vel, vel[i] };
+  cq_neighbours<kCqRecord>(on, i, true, pos, pred, se, c.f.g, list, lds, b);
+  if (lead)
+    vort[i] = b.w;
+  cld_flag(!lead | cld_safe(b.w), flags, kFlagVort);
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_confinement_q(const float4* __restrict__ pred,
+                                                                 const uint2* __restrict__ se,
+                                                                 const float4* __restrict__ vort,
+                                                                 const float4* __restrict__ vel,
+                                                                 float4* __restrict__ velOut,
+                                                                 uint32_t n, CloudConsts c,
+                                                                 CldList list,
+                                                                 uint32_t* __restrict__ flags,
+                                                                 int useList)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pos = pred[i];
+  CqConfinement b{ c, vort };
+  cq_neighbours<kCqReplay>(on, i, useList && !(*flags & kFlagVort), pos, pred, se, c.f.g, list,
+                           lds, b);
+  float4 vo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (lead)
+    vo = cld_confinement_out(i, b.nn, vort[i], vel, velOut, c);
+  cld_flag(!lead | cld_safe(vo), flags, kFlagVelTmp);
+}
+
+__global__ __launch_bounds__(kCldLanes) void k_cld_xsph_q(const float4* __restrict__ pred,
+                                                          const uint2* __restrict__ se,
+                                                          const float4* __restrict__ velIn,
+                                                          float4* __restrict__ velOut,
+                                                          float4* __restrict__ pos, uint32_t n,
+                                                          CloudConsts c, CldList list,
+                                                          const uint32_t* __restrict__ flags,
+                                                          int useList)
+{
+  RTP_CQ_PROLOGUE
+  const float4 pp = pred[i];
+  CqXsph b{ c, velIn, velIn[i] };
+  cq_neighbours<kCqReplay>(on, i, useList && !(*flags & kFlagVelTmp), pp, pred, se, c.f.g, list,
+                           lds, b);
+  if (lead) {
+    velOut[i] = add4(b.velocity, muls4(c.f.p.xsphViscosityCoeff, b.viscosity));
+    pos[i] = cld_update_pos(pp, c);
+  }
+}
+#undef RTP_CQ_PROLOGUE
 
 __global__ __launch_bounds__(256) void k_cld_exp(const double* __restrict__ x64, double* __restrict__ y64,
                                                  const float* __restrict__ x32, float* __restrict__ y32,
@@ -911,6 +1374,7 @@ struct rtp_clouds {
   CldList list;              // survivor lists of the recording passes (kCldListCap per slot)
   CldList listT;             // ... of the temperature passes, which run beside the others
   uint32_t* flags = nullptr;  // cld_flag bits of the step
+  bool quad = true;           // walk kernels: a quad of lanes per particle (cq_neighbours)
   int cur = 0;  // state in [cur]
   int pc = 0;   // predicted positions in pred[pc]
   Profiler prof;
@@ -999,7 +1463,9 @@ struct rtp_clouds {
     pc = 1 - pc;
     const int gathered = pc;
     const uint32_t bn = blocks_for(std::max(n, 1u), 256);
-    const uint32_t bw = blocks_for(std::max(n, 1u), kCldLanes);  // the walk kernels
+    // the walk kernels: 64 particles per workgroup, or 16 with quads
+    const uint32_t bw = blocks_for(std::max(n, 1u), quad ? kCqGroups : kCldLanes);
+    const bool qd = quad;
     // Replays need every candidate a walk may visit to have data from this step: not so
     // when the sorted range holds stale particles past n (after the count shrank).
     const int useList = G.sortedN <= n;
@@ -1009,20 +1475,20 @@ struct rtp_clouds {
       RTP_HIP_TRY(hipEventRecord(fork, s));
       RTP_HIP_TRY(hipStreamWaitEvent(t, fork, 0));
       RTP_PROF(prof, "cld_temp_lambda", t,
-               hipLaunchKernelGGL(k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], temp[cur],
+               hipLaunchKernelGGL(qd ? k_cld_temp_lambda_q : k_cld_temp_lambda, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], temp[cur],
                                   G.startEnd, lap, lambdaT, n, c, listT, flags));
       RTP_PROF(prof, "cld_temp_correct", t,
-               hipLaunchKernelGGL(k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], lambdaT,
+               hipLaunchKernelGGL(qd ? k_cld_temp_correct_q : k_cld_temp_correct, dim3(bw), dim3(kCldLanes), 0, t, pos[cur], lambdaT,
                                   G.startEnd, temp[cur], n, c, listT, flags, useList));
       RTP_HIP_TRY(hipEventRecord(join, t));
     }
     for (uint32_t k = 0; k < iters && n; ++k) {
       const uint32_t bit = k < kFlagIters ? kFlagLambda0 << k : 0u;
       RTP_PROF(prof, "cld_density_lambda", s,
-               hipLaunchKernelGGL(k_cld_density_lambda, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
+               hipLaunchKernelGGL(qd ? k_cld_density_lambda_q : k_cld_density_lambda, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
                                   G.startEnd, density, lambda, n, c, list, flags, bit));
       RTP_PROF(prof, "cld_correct", s,
-               hipLaunchKernelGGL(k_cld_correct, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], lambda,
+               hipLaunchKernelGGL(qd ? k_cld_correct_q : k_cld_correct, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], lambda,
                                   G.startEnd, pred[1 - pc], tot[cur],
                                   k + 1 == iters ? vel[cur] : nullptr, n, c, list, flags, bit,
                                   useList));
@@ -1040,13 +1506,13 @@ struct rtp_clouds {
       return RTP_OK;
     if (c.f.p.isVorticityConfEnabled) {
       RTP_PROF(prof, "cld_vorticity", s,
-               hipLaunchKernelGGL(k_cld_vorticity, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
+               hipLaunchKernelGGL(qd ? k_cld_vorticity_q : k_cld_vorticity, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
                                   vel[cur], vort, n, c, list, flags));
       RTP_PROF(prof, "cld_confinement", s,
-               hipLaunchKernelGGL(k_cld_confinement, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
+               hipLaunchKernelGGL(qd ? k_cld_confinement_q : k_cld_confinement, dim3(bw), dim3(kCldLanes), 0, s, pred[pc],
                                   G.startEnd, vort, vel[cur], velTmp, n, c, list, flags, useList));
       RTP_PROF(prof, "cld_xsph", s,
-               hipLaunchKernelGGL(k_cld_xsph, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
+               hipLaunchKernelGGL(qd ? k_cld_xsph_q : k_cld_xsph, dim3(bw), dim3(kCldLanes), 0, s, pred[pc], G.startEnd,
                                   velTmp, vel[cur], pos[cur], n, c, list, flags, useList));
     } else {
       RTP_PROF(prof, "cld_update_pos", s,
@@ -1134,6 +1600,7 @@ int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
            (rc = dev_alloc(&h->listT.count, maxN)));
   h->list.stride = maxN;
   h->listT.stride = maxN;
+  h->quad = maxN <= kCldQuadMaxN;
   if (!rc && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
               hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess))
@@ -1187,6 +1654,14 @@ int rtp_clouds_get_params(rtp_clouds* h, rtp_fluid_params* p, rtp_cloud_params* 
     *p = h->c.f.p;
   if (q)
     *q = h->c.q;
+  return RTP_OK;
+}
+
+int rtp_clouds_set_tuning(rtp_clouds* h, const rtp_tuning* t)
+{
+  RTP_REQUIRE(h && t, "NULL argument");
+  RTP_REQUIRE((t->flags & ~RTP_LAUNCH_QUAD) == 0, "launch flags not supported here");
+  h->quad = (t->flags & RTP_LAUNCH_QUAD) != 0;
   return RTP_OK;
 }
 

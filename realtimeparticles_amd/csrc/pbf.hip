@@ -1028,37 +1028,6 @@ struct QuadLds {
   uint2 ranges[kQuadGroups][kQuadRanges];
 };
 
-// Orders the wave's LDS accesses on both sides (a quad's lanes read what the others wrote).
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// v of lane K of the quad.
-template <int K>
-__device__ __forceinline__ float quad_lane(float v)
-{
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                               K * 0x55, 0xF, 0xF, false));
-}
-
-// ((((s + t0) + t1) + t2) + t3) for t_k the t of quad lane k.
-__device__ __forceinline__ float quad_add(float s, float t)
-{
-  s = s + quad_lane<0>(t);
-  s = s + quad_lane<1>(t);
-  s = s + quad_lane<2>(t);
-  return s + quad_lane<3>(t);
-}
-
-// The quad's four bits of a wave ballot, lane 0 lowest.
-__device__ __forceinline__ uint32_t quad_bits(bool x)
-{
-  return (uint32_t)(__builtin_amdgcn_ballot_w64(x) >> (threadIdx.x & 60u)) & 0xFu;
-}
-
 // Survivor-list entry k of slot s: word k % 4 of the slot's granule k / 4.
 __device__ __forceinline__ uint32_t* nb_word(const NbList& nb, uint32_t s, uint32_t k)
 {

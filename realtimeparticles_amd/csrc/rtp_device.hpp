@@ -91,4 +91,40 @@ __device__ __forceinline__ uint32_t logical_block(int remap)
 
 __device__ __forceinline__ float4 ld4(const float4* p) { return *p; }
 
+// ------------------------------------------------------------------- quads of lanes
+// Walks that give a particle a quad of four lanes (lanes 4q .. 4q + 3 of a wave) exchange
+// their terms within the quad and queue survivors in LDS that the quad's other lanes read.
+
+// Orders the wave's LDS accesses on both sides (a quad's lanes read what the others wrote).
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v of lane K of the quad.
+template <int K>
+__device__ __forceinline__ float quad_lane(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               K * 0x55, 0xF, 0xF, false));
+}
+
+// ((((s + t0) + t1) + t2) + t3) for t_k the t of quad lane k: four terms added to a sum
+// in the order of the quad's lanes.
+__device__ __forceinline__ float quad_add(float s, float t)
+{
+  s = s + quad_lane<0>(t);
+  s = s + quad_lane<1>(t);
+  s = s + quad_lane<2>(t);
+  return s + quad_lane<3>(t);
+}
+
+// The quad's four bits of a wave ballot, lane 0 lowest.
+__device__ __forceinline__ uint32_t quad_bits(bool x)
+{
+  return (uint32_t)(__builtin_amdgcn_ballot_w64(x) >> (threadIdx.x & 60u)) & 0xFu;
+}
+
 }  // namespace rtp

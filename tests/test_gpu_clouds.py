@@ -34,12 +34,16 @@ def test_pinned_exp_matches_oracle():
     _eq(s32, rs, "saturation vapour density")
 
 
-def make_pair(sc, iters=None, fluid=None, cloud=None, gravity=None):
+def make_pair(sc, iters=None, fluid=None, cloud=None, gravity=None, quad=None):
+    """quad: the walk passes with a quad of lanes per particle (True), one lane (False), or
+    the model's default for the scene's size (None: quads at these sizes)."""
     fl = sc.params() if fluid is None else fluid
     cl = sc.cloud_params() if cloud is None else cloud
     it = sc.iters if iters is None else iters
     g = sc.gravity if gravity is None else gravity
     gpu = capi.Clouds(sc.max_n, sc.n, sc.box, sc.res, gravity=g)
+    if quad is not None:
+        gpu.set_quad(quad)
     gpu.set_params(fl, cl)
     gpu.set_nb_jacobi_iters(it)
     gpu.load_state(sc.pos, sc.vel)
@@ -83,11 +87,13 @@ def test_initial_state_matches_oracle():
     gpu.close()
 
 
-def test_cumulus_ten_steps_bit_exact():
+@pytest.mark.parametrize("quad", [True, False])
+def test_cumulus_ten_steps_bit_exact(quad):
     """The app's cumulus case (65,536 particles in the lowest quarter of a 10 x 20 x 10 box,
-    grid 30 x 60 x 30) with initCloudsJson's parameters, 2 Jacobi iterations."""
+    grid 30 x 60 x 30) with initCloudsJson's parameters, 2 Jacobi iterations; the walk passes
+    with a quad of lanes per particle and with one lane."""
     sc = scenes.cloud_scene("clouds-65k")
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     for step in range(1, 11):
         gpu.update(1)
         orc.step(1)
@@ -129,13 +135,14 @@ def test_cloud_variants_bit_exact(variant):
     gpu.close()
 
 
-def test_thirty_one_iterations_bit_exact():
+@pytest.mark.parametrize("quad", [True, False])
+def test_thirty_one_iterations_bit_exact(quad):
     """31 Jacobi iterations: the lambda of iterations 29 and 30 has no flag bit left, so their
     corrections walk instead of replaying the density passes' survivor lists."""
     sc = scenes.cloud_scene("clouds-65k", seed=47)
     sc.pos, sc.n = sc.pos[:8192].copy(), 8192
     sc.vel = np.zeros_like(sc.pos)
-    gpu, orc = make_pair(sc, iters=31)
+    gpu, orc = make_pair(sc, iters=31, quad=quad)
     for step in range(1, 3):
         gpu.update(1)
         orc.step(1)
@@ -144,10 +151,11 @@ def test_thirty_one_iterations_bit_exact():
     gpu.close()
 
 
-def test_cumulus_particle_count_change():
+@pytest.mark.parametrize("quad", [True, False])
+def test_cumulus_particle_count_change(quad):
     """setNbParticles keeps the whole-buffer sort of the stale keys, as for the fluids."""
     sc = scenes.cloud_scene("clouds-65k", seed=29)
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     gpu.update(2)
     orc.step(2)
     n2 = sc.n - 3000
@@ -160,10 +168,11 @@ def test_cumulus_particle_count_change():
     gpu.close()
 
 
-def test_cumulus_hundred_steps():
+@pytest.mark.parametrize("quad", [True, False])
+def test_cumulus_hundred_steps(quad):
     """100 steps: the state stays the oracle's, bit for bit, after the clouds start rising."""
     sc = scenes.cloud_scene("clouds-65k", seed=31)
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     gpu.update(100)
     orc.step(100)
     compare(gpu, orc, sc.n, "step 100")
@@ -174,7 +183,8 @@ def test_cumulus_hundred_steps():
     gpu.close()
 
 
-def test_capped_cell_and_wall_clusters_bit_exact():
+@pytest.mark.parametrize("quad", [True, False])
+def test_capped_cell_and_wall_clusters_bit_exact(quad):
     """400 particles in one cell (the cap of 100 keeps 101 of them in its list, so its z-column
     is not one contiguous range), clusters pressed against the periodic x and z walls and the
     y walls (wrapped neighbours with +-2W shifts), among the cumulus particles."""
@@ -195,7 +205,7 @@ def test_capped_cell_and_wall_clusters_bit_exact():
         extra.append(w)
     pos = np.concatenate([sc.pos[:60000]] + extra)
     sc.pos, sc.vel, sc.n = pos, np.zeros_like(pos), pos.shape[0]
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     for step in range(1, 5):
         gpu.update(1)
         orc.step(1)
@@ -207,7 +217,8 @@ def test_capped_cell_and_wall_clusters_bit_exact():
     gpu.close()
 
 
-def test_dense_ball_overflows_the_survivor_lists_bit_exact():
+@pytest.mark.parametrize("quad", [True, False])
+def test_dense_ball_overflows_the_survivor_lists_bit_exact(quad):
     """600 particles in a ball of radius 0.2 around a cell corner (about 75 per cell, under
     the cap of 100): each keeps several hundred neighbours, more than a survivor list holds
     (128), so the passes that replay lists walk for those particles."""
@@ -219,7 +230,7 @@ def test_dense_ball_overflows_the_survivor_lists_bit_exact():
     ball[:, :3] = np.array([1.0, -6.0, 1.0]) + d  # (1, -6, 1) is a corner of cells of size 1/3
     pos = np.concatenate([sc.pos[:64000], ball])
     sc.pos, sc.vel, sc.n = pos, np.zeros_like(pos), pos.shape[0]
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     for step in range(1, 4):
         gpu.update(1)
         orc.step(1)
@@ -230,7 +241,8 @@ def test_dense_ball_overflows_the_survivor_lists_bit_exact():
     gpu.close()
 
 
-def test_infinite_velocity_spreads_nan_bit_exact():
+@pytest.mark.parametrize("quad", [True, False])
+def test_infinite_velocity_spreads_nan_bit_exact(quad):
     """One particle with an infinite velocity: its predicted position is infinite, so the
     gradient terms of its neighbours turn NaN and so do the lambda, vorticity and confined
     velocity buffers the replaying passes multiply.  Those passes then walk instead of
@@ -239,7 +251,7 @@ def test_infinite_velocity_spreads_nan_bit_exact():
     sc = scenes.cloud_scene("clouds-65k", seed=43)
     sc.vel = np.zeros_like(sc.pos)
     sc.vel[1234, 0] = np.inf
-    gpu, orc = make_pair(sc)
+    gpu, orc = make_pair(sc, quad=quad)
     for step in range(1, 3):
         gpu.update(1)
         orc.step(1)
