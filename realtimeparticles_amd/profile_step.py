@@ -9,7 +9,8 @@ launch time, to set beside the rocprofv3 trace of the same command.
 
 A Clouds workload (clouds-65k: the cumulus case) runs the Clouds model's steps from the
 input instead, as bench.py's Clouds section does: `--warmup` untimed steps, then `--steps`
-timed steps from the input, then the same with per-kernel HIP events.  The boids workloads
+timed steps from the input, then the same with per-kernel HIP events; `--flags` (if given)
+picks its walk: LAUNCH_QUAD (64) a quad of lanes per particle, 0 one lane.  The boids workloads
 (boids-8k: the 8,192-boid flock; boids-131k: the reference app's 131,072 boids with its
 default rules and dt 0.1) do the same for bench.py's boids runs, at `--block` and `--flags`.
 
@@ -33,7 +34,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--block", type=int, default=64)
-    ap.add_argument("--flags", type=int, default=13)
+    ap.add_argument("--flags", type=int, default=None,
+                    help="RTP_LAUNCH_* bits (default 13 for the fluids, the model's own for "
+                         "the Clouds model)")
     ap.add_argument("--queue", type=int, default=16)
     ap.add_argument("--sort-bits", type=int, default=10)
     a = ap.parse_args(argv)
@@ -41,6 +44,8 @@ def main(argv=None):
         return clouds_main(a)
     if a.workload.startswith("boids"):
         return boids_main(a)
+    if a.flags is None:
+        a.flags = 13
 
     sc = scenes.pbf_scene(a.workload)
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
@@ -79,6 +84,8 @@ def main(argv=None):
 def clouds_main(a):
     sc = scenes.cloud_scene(a.workload)
     m = capi.Clouds(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    if a.flags is not None:
+        m.set_quad(bool(a.flags & capi.LAUNCH_QUAD))
     m.set_params(sc.params(), sc.cloud_params())
     m.set_nb_jacobi_iters(sc.iters)
 
@@ -119,7 +126,7 @@ def boids_main(a):
     else:
         m.set_params(*prm[:4])
         m.set_time_step(prm[4])
-    m.set_tuning(a.block, a.flags)
+    m.set_tuning(a.block, a.flags if a.flags is not None else 0)
 
     def from_input(k):
         m.load_state(sc.pos, sc.vel)
