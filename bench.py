@@ -331,6 +331,10 @@ def run_boids(args):
         if sc.n <= 8192:
             timed(256, capi.LAUNCH_PERSISTENT)  # its first launch
         shapes = {(b, f): timed(b, f) for b in (64, 128, 192, 256) for f in flag_sets}
+        # the quad walk (four lanes per boid) ignores the block size
+        timed(64, capi.LAUNCH_QUAD)  # its first launch
+        for f in (capi.LAUNCH_QUAD, capi.LAUNCH_QUAD | capi.LAUNCH_XCD_MAP):
+            shapes[(64, f)] = timed(64, f)
         (bb, bf), best_ms = min(shapes.items(), key=lambda kv: kv[1])
         separate = min(v for (b, f), v in shapes.items() if not f & capi.LAUNCH_PERSISTENT)
         persistent = [v for (b, f), v in shapes.items() if f & capi.LAUNCH_PERSISTENT]
@@ -350,6 +354,8 @@ def run_boids(args):
                      "ms_per_step_default_launch": round(shapes[(256, 0)], 5),
                      "ms_per_step_separate_launches": round(separate, 5),
                      "ms_per_step_persistent": round(min(persistent), 5) if persistent else None,
+                     "ms_per_step_quad": round(min(v for (b, f), v in shapes.items()
+                                                   if f & capi.LAUNCH_QUAD), 5),
                      "kernel_ms_per_step": {kk: round(v[1] / steps, 5)
                                             for kk, v in sorted(prof.items())}}
     return out

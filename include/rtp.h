@@ -111,7 +111,7 @@ typedef struct {
                                   pencils of 8 x 8 cells in (y, z), pencil after pencil, instead
                                   of in cell index order, so an XCD's consecutive waves keep
                                   their neighbours' data in its L2 on large grids */
-#define RTP_LAUNCH_QUAD 64     /* fluids and clouds: the neighbour passes (of the plain solver)
+#define RTP_LAUNCH_QUAD 64     /* fluids, clouds, boids: the neighbour passes (of the plain solver)
                                   give each particle four lanes (sixteen particles per 64-lane
                                   workgroup) that split its walk and add the terms in the same
                                   order; for scenes too small to fill the GPU one lane per
@@ -414,7 +414,9 @@ int rtp_boids_device_state(rtp_boids* h, void** pos4, void** vel4);
  * GRID_RES_X / 2, Boids.cpp:346-349), 3 (the default) the 3-D rules. */
 int rtp_boids_set_dimension(rtp_boids* h, int dim);
 /* Launch shape of the boids step (no reference counterpart): the block size and, of the
- * flags, RTP_LAUNCH_XCD_MAP.  Results do not depend on it.  Default 256 threads. */
+ * flags, RTP_LAUNCH_XCD_MAP, RTP_LAUNCH_PERSISTENT and RTP_LAUNCH_QUAD (the step launch with
+ * a quad of four lanes per boid, sixteen boids per 64-lane workgroup, which ignores the block
+ * size).  Results do not depend on it.  Default 256 threads. */
 int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t);
 /* Model::setBoundary (physics/Model.hpp:116-119, Boundary :38-44): the wall of bd_updatePosAndApplyWallBC
  * (bouncing, the default) or of bd_updatePosAndApplyPeriodicBC (cyclic, boids.cl:289-315),

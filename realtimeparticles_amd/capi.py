@@ -827,7 +827,8 @@ class Boids(_RenderFields):
         self.n = int(n)
 
     def set_tuning(self, block: int = 256, flags: int = 0) -> None:
-        """Block size of the boids step; flags: LAUNCH_XCD_MAP | LAUNCH_PERSISTENT, or 0."""
+        """Block size of the boids step; flags: LAUNCH_XCD_MAP | LAUNCH_PERSISTENT |
+        LAUNCH_QUAD (a quad of lanes per boid; ignores the block size), or 0."""
         t = _tuning(block, flags, 0, 0)
         _check(lib().rtp_boids_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 

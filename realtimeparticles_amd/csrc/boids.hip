@@ -371,6 +371,161 @@ __global__ __launch_bounds__(256) void k_boids_step(const float4* __restrict__ p
   boids_finish<DIM2>(i, p, S, velocity, posOut, velOut, accOut, c);
 }
 
+// ------------------------------------------------------------------- the quad walk
+// For flocks too small to fill the GPU with one lane per boid (RTP_LAUNCH_QUAD), each boid
+// gets a quad of four lanes: the quad lists the boid's ranges as boids_sums_queued does (in
+// LDS), scans them sixteen candidates per step (lane u: candidates u, u + 4, u + 8, u + 12),
+// queues the neighbours in walk order, and runs four queued bodies at once, one per lane,
+// adding their terms to every sum in walk order (quad_add).  The sums therefore add the same
+// terms in the same order as one lane's walk.
+constexpr uint32_t kBoidsQuadGroups = 16;  // boids per 64-lane workgroup
+constexpr uint32_t kBoidsQuadQueue = 64;   // queued neighbours per boid
+
+struct BoidsQuadLds {
+  uint32_t queue[kBoidsQuadGroups][kBoidsQuadQueue];
+  uint2 ranges[kBoidsQuadGroups][kBoidsRanges];
+};
+
+// boids_pair's terms for the quad's four neighbours (v: this lane has one), added in order.
+__device__ __forceinline__ void boids_pair_quad(bool v, const float4 p, const float4 q,
+                                                const float4 nv, BoidsSums& S,
+                                                const BoidsConsts& c)
+{
+  const float vx = p.x - q.x, vy = p.y - q.y, vz = p.z - q.z;
+  const float sq = vx * vx + vy * vy + vz * vz;
+  const float y = rcp_refined(sq);
+  float qx = div_short(vx, sq, y), qy = div_short(vy, sq, y), qz = div_short(vz, sq, y);
+  const bool general = !c.shortDiv || tiny_numerator3(vx, vy, vz);
+  if (__builtin_amdgcn_ballot_w64(general) != 0) {
+    rare_path();
+    if (general) {
+      qx = vx / sq;
+      qy = vy / sq;
+      qz = vz / sq;
+    }
+  }
+  S.avgPos.x = quad_add(S.avgPos.x, v ? q.x : 0.0f);
+  S.avgPos.y = quad_add(S.avgPos.y, v ? q.y : 0.0f);
+  S.avgPos.z = quad_add(S.avgPos.z, v ? q.z : 0.0f);
+  S.avgVel.x = quad_add(S.avgVel.x, v ? nv.x : 0.0f);
+  S.avgVel.y = quad_add(S.avgVel.y, v ? nv.y : 0.0f);
+  S.avgVel.z = quad_add(S.avgVel.z, v ? nv.z : 0.0f);
+  S.rep.x = quad_add(S.rep.x, v ? qx : 0.0f);
+  S.rep.y = quad_add(S.rep.y, v ? qy : 0.0f);
+  S.rep.z = quad_add(S.rep.z, v ? qz : 0.0f);
+  S.count += __builtin_popcount(quad_bits(v));
+}
+
+template <int DIM2>
+__device__ __forceinline__ void boids_sums_quad(bool on, uint32_t self, const float4 p,
+                                                const float4* __restrict__ position,
+                                                const float4* __restrict__ unitVel,
+                                                const uint2* __restrict__ se,
+                                                const BoidsConsts& c, BoidsQuadLds& lds,
+                                                BoidsSums& S)
+{
+  const uint32_t gq = threadIdx.x >> 2, u = threadIdx.x & 3u, below = (1u << u) - 1u;
+  uint2* const rl = lds.ranges[gq];
+  uint32_t* const q = lds.queue[gq];
+  // the boid's ranges (every lane of the quad lists the same ones)
+  uint32_t nr = 0, e = 1u, hi = 0u;
+  if (on) {
+    uint32_t lo = 1u;
+    int lastCol = -1;
+    boids_cells<DIM2>(p, c, [&](uint32_t cell, int col, int) {
+      const uint2 r = se[cell];
+      if (r.x > r.y)
+        return;
+      if (col == lastCol && lo <= hi && r.x == hi + 1u) {
+        hi = r.y;
+        return;
+      }
+      if (lo <= hi)
+        rl[nr++] = make_uint2(lo, hi);
+      lo = r.x;
+      hi = r.y;
+      lastCol = col;
+    });
+    if (lo <= hi)
+      rl[nr++] = make_uint2(lo, hi);
+    hi = 0u;
+  }
+  uint32_t ri = 0, qn = 0;
+  if (nr > 0) {
+    e = rl[0].x;
+    hi = rl[0].y;
+  }
+  const auto drain = [&]() {
+    wave_sync();
+    bool v = u < qn;
+    uint32_t ev = v ? q[u] : self;
+    float4 qq = position[ev], nv = unitVel[ev];
+    for (uint32_t k = u; __builtin_amdgcn_ballot_w64(k - u < qn) != 0; k += 4u) {
+      const uint32_t k1 = k + 4u;
+      const bool v1 = k1 < qn;
+      const uint32_t e1 = v1 ? q[k1] : self;
+      const float4 q1 = position[e1], nv1 = unitVel[e1];
+      boids_pair_quad(v, p, qq, nv, S, c);
+      v = v1;
+      qq = q1;
+      nv = nv1;
+    }
+    qn = 0;
+    wave_sync();
+  };
+  while (__builtin_amdgcn_ballot_w64(ri < nr) != 0) {
+    const bool act = ri < nr;
+    uint32_t ej[4];
+    bool kept[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ej[j] = e + u + 4u * (uint32_t)j;
+      const bool valid = act & (ej[j] <= hi);
+      kept[j] = valid & boids_near(boids_sq(p, position[valid ? ej[j] : self]), c);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t m = quad_bits(kept[j]);
+      if (kept[j])
+        q[qn + (uint32_t)__builtin_popcount(m & below)] = ej[j];
+      qn += (uint32_t)__builtin_popcount(m);
+    }
+    e += 16u;
+    if (act & (e > hi)) {
+      ++ri;
+      if (ri < nr) {
+        e = rl[ri].x;
+        hi = rl[ri].y;
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(qn > kBoidsQuadQueue - 16u) != 0)
+      drain();
+  }
+  drain();
+}
+
+// k_boids_step with a quad of lanes per boid.
+template <int DIM2>
+__global__ __launch_bounds__(64) void k_boids_step_q(const float4* __restrict__ position,
+                                                     const float4* __restrict__ velocity,
+                                                     const float4* __restrict__ unitVel,
+                                                     const uint2* __restrict__ se,
+                                                     float4* __restrict__ posOut,
+                                                     float4* __restrict__ velOut,
+                                                     float4* __restrict__ accOut, uint32_t n,
+                                                     BoidsConsts c, int remap)
+{
+  __shared__ BoidsQuadLds lds;
+  const uint32_t t = logical_block(remap) * kBoidsQuadGroups + (threadIdx.x >> 2);
+  const bool on = t < n;
+  const uint32_t i = on ? t : 0u;
+  const float4 p = position[i];
+  BoidsSums S;
+  boids_sums_quad<DIM2>(on, i, p, position, unitVel, se, c, lds, S);
+  if (on && (threadIdx.x & 3u) == 0u)
+    boids_finish<DIM2>(i, p, S, velocity, posOut, velOut, accOut, c);
+}
+
 // Whether the current device gives one workgroup `bytes` of LDS (the limit is read once per
 // device).
 bool lds_fits(size_t bytes)
@@ -520,6 +675,16 @@ void launch_boids_step(const float4* pos, const float4* vel, float4* unitVel,
   // active ones
   hipLaunchKernelGGL(k_boids_unit_vel, dim3(blocks_for(sortN, 256)), dim3(256), 0, s, vel,
                      unitVel, sortN);
+  if (L.quad) {
+    const dim3 grid(blocks_for(n, kBoidsQuadGroups)), block(64);
+    if (c.dim2)
+      hipLaunchKernelGGL(k_boids_step_q<1>, grid, block, 0, s, pos, vel, unitVel, startEnd, posOut,
+                         velOut, acc, n, c, L.xcdRemap);
+    else
+      hipLaunchKernelGGL(k_boids_step_q<0>, grid, block, 0, s, pos, vel, unitVel, startEnd, posOut,
+                         velOut, acc, n, c, L.xcdRemap);
+    return;
+  }
   // the survivor-queue walk where its LDS fits in a workgroup, else bodies in place
   const size_t lds = boids_step_lds(L.block);
   const bool queued = lds_fits(lds);

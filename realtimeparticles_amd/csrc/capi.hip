@@ -1522,11 +1522,12 @@ int rtp_boids_read_field(rtp_boids* h, int field, float* out, uint32_t count)
 int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t)
 {
   RTP_REQUIRE(h && t, "NULL argument");
-  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_PERSISTENT);
+  const int rc = check_tuning(*t, RTP_LAUNCH_XCD_MAP | RTP_LAUNCH_PERSISTENT | RTP_LAUNCH_QUAD);
   if (rc)
     return rc;
   h->L.block = t->blockSize;
   h->L.xcdRemap = t->flags & RTP_LAUNCH_XCD_MAP;
+  h->L.quad = (t->flags & RTP_LAUNCH_QUAD) ? 1 : 0;
   h->persistent = (t->flags & RTP_LAUNCH_PERSISTENT) != 0;
   return RTP_OK;
 }

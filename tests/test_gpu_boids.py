@@ -88,10 +88,12 @@ def _target_path(step, planar):
     return (0.0 if planar else 0.6 * np.cos(a), 0.4 * np.sin(a), 0.3 * np.sin(2 * a))
 
 
+@pytest.mark.parametrize("quad", [False, True])
 @pytest.mark.parametrize("variant", ["2d", "cyclic", "target", "2d-cyclic-target"])
-def test_boids_variants_bit_exact(variant):
+def test_boids_variants_bit_exact(variant, quad):
     """The 2-D rules (boids.cl:137-221), the cyclic wall (:289-315) and the target rule
-    (:226-241), alone and together, over 120 steps against the oracle."""
+    (:226-241), alone and together, over 120 steps against the oracle; one lane or a quad
+    of lanes per boid."""
     sc = scenes.boids_flock(8192, seed=21)
     pos, vel = sc.pos.copy(), sc.vel.copy()
     if "2d" in variant:  # a flock in the YZ plane, as the reference's 2-D cases lay it out
@@ -100,6 +102,8 @@ def test_boids_variants_bit_exact(variant):
     gpu = capi.Boids(sc.max_n, sc.n, sc.box, sc.res)
     gpu.set_params(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
     gpu.set_time_step(sc.time_step)
+    if quad:
+        gpu.set_tuning(64, capi.LAUNCH_QUAD)
     gpu.load_state(pos, vel)
     p = O.BoidsParams(sc.velocity_scale, sc.alignment, sc.separation, sc.cohesion)
     orc = O.OracleBoids(sc.box, sc.res, sc.max_n, sc.n, p, time_step=sc.time_step)
@@ -139,11 +143,12 @@ def test_boids_variants_bit_exact(variant):
 
 
 @pytest.mark.parametrize("block,flags", [(64, capi.LAUNCH_XCD_MAP), (128, 0), (192, capi.LAUNCH_XCD_MAP),
-                                         (256, 0)])
+                                         (256, 0), (64, capi.LAUNCH_QUAD),
+                                         (256, capi.LAUNCH_QUAD | capi.LAUNCH_XCD_MAP)])
 def test_boids_launch_shapes_keep_bits(block, flags):
     """Block sizes and block order: the step kernel's walk with survivor queues (344 B of LDS
     per thread), or its bodies in place where a workgroup's share exceeds the device's
-    per-workgroup LDS, give the same bits."""
+    per-workgroup LDS, or the quad walk (four lanes per boid), give the same bits."""
     sc = scenes.boids_flock(8192, seed=31)
     gpu, orc = make_pair(sc)
     gpu.set_tuning(block, flags)
