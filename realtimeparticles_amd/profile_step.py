@@ -13,6 +13,11 @@ timed steps from the input, then the same with per-kernel HIP events; `--flags` 
 picks its walk: LAUNCH_QUAD (64) a quad of lanes per particle, 0 one lane.  The boids workloads
 (boids-8k: the 8,192-boid flock; boids-131k: the reference app's 131,072 boids with its
 default rules and dt 0.1) do the same for bench.py's boids runs, at `--block` and `--flags`.
+The learned workload (learned-1m) trains bench.py's network (2M teacher samples from seeds 101
+and 102 of the 1M dam break, stable coefficients, 200 epochs), sets the fallback threshold to
+`--tau-fraction` of the plain 4-iteration substep's post-correction max |rho/rho0 - 1| on the
+1M input, and times the learned substep (one learned iteration, `--fallback-iters` fallback
+iterations) the same fixed-input way.
 
     rocprofv3 --kernel-trace --stats -d OUT -o run -- \\
         python3 -m realtimeparticles_amd.profile_step --workload pbf-1m --block 64 --flags 13 \\
@@ -39,6 +44,8 @@ def main(argv=None):
                          "the Clouds model)")
     ap.add_argument("--queue", type=int, default=16)
     ap.add_argument("--sort-bits", type=int, default=10)
+    ap.add_argument("--tau-fraction", type=float, default=0.8)
+    ap.add_argument("--fallback-iters", type=int, default=4)
     a = ap.parse_args(argv)
     if a.workload.startswith("clouds"):
         return clouds_main(a)
@@ -46,6 +53,8 @@ def main(argv=None):
         return boids_main(a)
     if a.flags is None:
         a.flags = 13
+    if a.workload == "learned-1m":
+        return learned_main(a)
 
     sc = scenes.pbf_scene(a.workload)
     m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
@@ -75,6 +84,56 @@ def main(argv=None):
     m.close()
     print(json.dumps({
         "workload": a.workload, "mapping": a.mapping, "particles": sc.n,
+        "tuning": {"block": a.block, "flags": a.flags, "queue": a.queue, "sort_bits": a.sort_bits},
+        "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
+        "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
+                    for k, v in sorted(prof.items())}}), flush=True)
+
+
+def learned_main(a):
+    from . import learned
+    feat, target = learned.teacher_data(lambda s: scenes.pbf_scene("pbf-1m", seed=s), [101, 102],
+                                        mapping="stable", teacher_iters=4, substeps=1)
+    mlp, _, _ = learned.train(feat, target, epochs=200, batch=65536, lr=2e-3, seed=1)
+    sc = scenes.pbf_scene("pbf-1m")
+    m = capi.Fluids(sc.max_n, sc.n, sc.box, sc.res, gravity=sc.gravity)
+    m.set_params(**sc.params("stable"))
+    m.set_nb_jacobi_iters(sc.iters)
+    m.load_state(sc.pos, sc.vel)
+    m.save_checkpoint()
+    m.set_tuning(a.block, a.flags, a.sort_bits, a.queue)
+    # the threshold: a fraction of the plain substep's post-correction max error here
+    m.set_residual_pass(True)
+    m.restore_checkpoint()
+    m.update(1)
+    plain_max = float(m.residual_error()[0])
+    m.set_residual_pass(False)
+    tau = a.tau_fraction * plain_max
+    m.set_warm_start(mlp, 1, a.fallback_iters, tau)
+
+    def fixed_steps(k):
+        for _ in range(k):
+            m.restore_checkpoint()
+            m.update(1)
+
+    fixed_steps(a.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    fixed_steps(a.steps)
+    m.sync()
+    elapsed = time.perf_counter() - t0
+    sizes = m.fallback_sizes()
+    m.enable_profiling(True)
+    m.reset_profile()
+    fixed_steps(a.steps)
+    m.sync()
+    prof = m.profile()
+    m.set_warm_start(None)
+    m.close()
+    mlp.close()
+    print(json.dumps({
+        "workload": a.workload, "particles": sc.n, "tau": tau, "plain4_max_error": plain_max,
+        "fallback_iters": a.fallback_iters, "fallback_sizes": list(sizes),
         "tuning": {"block": a.block, "flags": a.flags, "queue": a.queue, "sort_bits": a.sort_bits},
         "ms_per_step": round(elapsed * 1e3 / a.steps, 5),
         "kernels": {k: {"launches": v[0], "mean_launch_ms": round(v[1] / max(v[0], 1), 5)}
