@@ -122,7 +122,8 @@ struct RenderState {
 // (its name on the host is a stand-in, not a device address); it passes the address
 // hipGetSymbolAddress returns on each device.
 constexpr uint32_t kNoAllocMax = RTP_NOALLOC_MAX_PARTICLES;
-constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocMax + count_capacity(kNoAllocMax);
+// (the sort here keeps subtiles of kSortItems keys per lane at every size)
+constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocMax + count_words(kNoAllocMax, kSortItems);
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
 constexpr int kMaxDevices = 64;
@@ -2094,7 +2095,7 @@ int rtp_grid_build_noalloc(const rtp_model_desc* d, const float* pos4, uint32_t*
   // the caller's keys and idx.  No scratch word needs to start at zero: every count a sort
   // pass reads is written earlier in that pass, and the partners are read only where a pass
   // has written them.
-  const SortPlan plan = make_sort_plan(n, max_active_key(g), 0);
+  const SortPlan plan = make_sort_plan(n, max_active_key(g), 0, kSortItems);
   const bool inB = sort_result_in_b(plan);
   uint32_t* const keyA = inB ? keyScratch : keys;
   uint32_t* const keyB = inB ? keys : keyScratch;

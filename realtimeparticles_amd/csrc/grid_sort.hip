@@ -285,36 +285,48 @@ __global__ __launch_bounds__(512) void k_sort_small(uint32_t* __restrict__ keyA,
   small_sort_block<CH>(keyA, keyB, idxA, idxB, n, passes, lds);
 }
 
-template <int BITS>
+template <int BITS, int ITEMS>
 int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
              uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
 {
   const uint32_t blocks = (p.subtiles + 3u) / 4u;
-  hipLaunchKernelGGL((k_sort_upsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, p.n,
+  hipLaunchKernelGGL((k_sort_upsweep<BITS, ITEMS>), dim3(blocks), dim3(256), 0, s, kin, p.n,
                      shift, p.subtiles, counts);
   const uint32_t m = (uint32_t)((size_t)(1u << BITS) * p.subtiles);
   const uint32_t nseg = (m + kScanSeg - 1u) / kScanSeg;
   uint32_t* partial = counts + (size_t)2048 * p.subtiles;  // past the largest table for n
   hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, counts, m, partial);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, counts, m, partial);
-  hipLaunchKernelGGL((k_sort_downsweep<BITS, kSortItems>), dim3(blocks), dim3(256), 0, s, kin, iin,
+  hipLaunchKernelGGL((k_sort_downsweep<BITS, ITEMS>), dim3(blocks), dim3(256), 0, s, kin, iin,
                      p.n, shift, p.subtiles, counts, kout, iout);
   return RTP_OK;
+}
+
+template <int ITEMS>
+int dispatch_bits(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
+                  uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
+{
+  switch (p.bits) {
+  case 4: return run_pass<4, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 5: return run_pass<5, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 6: return run_pass<6, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 7: return run_pass<7, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 8: return run_pass<8, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 9: return run_pass<9, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 10: return run_pass<10, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  case 11: return run_pass<11, ITEMS>(p, shift, kin, iin, kout, iout, counts, s);
+  default: return fail(RTP_ERR_INVALID, "sort digit width out of range");
+  }
 }
 
 int dispatch_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* iin,
                   uint32_t* kout, uint32_t* iout, uint32_t* counts, hipStream_t s)
 {
-  switch (p.bits) {
-  case 4: return run_pass<4>(p, shift, kin, iin, kout, iout, counts, s);
-  case 5: return run_pass<5>(p, shift, kin, iin, kout, iout, counts, s);
-  case 6: return run_pass<6>(p, shift, kin, iin, kout, iout, counts, s);
-  case 7: return run_pass<7>(p, shift, kin, iin, kout, iout, counts, s);
-  case 8: return run_pass<8>(p, shift, kin, iin, kout, iout, counts, s);
-  case 9: return run_pass<9>(p, shift, kin, iin, kout, iout, counts, s);
-  case 10: return run_pass<10>(p, shift, kin, iin, kout, iout, counts, s);
-  case 11: return run_pass<11>(p, shift, kin, iin, kout, iout, counts, s);
-  default: return fail(RTP_ERR_INVALID, "sort digit width out of range");
+  switch (p.items) {
+  case 4: return dispatch_bits<4>(p, shift, kin, iin, kout, iout, counts, s);
+  case 8: return dispatch_bits<8>(p, shift, kin, iin, kout, iout, counts, s);
+  case kSortItems: return dispatch_bits<kSortItems>(p, shift, kin, iin, kout, iout, counts, s);
+  default: return fail(RTP_ERR_INVALID, "sort subtile size out of range");
   }
 }
 
@@ -411,7 +423,7 @@ void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key,
 
 uint32_t scan_partials_needed(uint32_t m16) { return (m16 + kScanSeg - 1u) / kScanSeg; }
 
-SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
+SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride, int fixedItems)
 {
   SortPlan p;
   p.n = n;
@@ -425,7 +437,8 @@ SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride)
   p.bits = (keyBits + p.passes - 1) / p.passes;
   if (p.bits < 4)  // the count table scan works on whole 16-byte groups
     p.bits = 4;
-  p.subtiles = (n + (uint32_t)(kSortItems * 64) - 1u) / (uint32_t)(kSortItems * 64);
+  p.items = fixedItems ? fixedItems : sort_items(n);
+  p.subtiles = (n + (uint32_t)(p.items * 64) - 1u) / (uint32_t)(p.items * 64);
   p.countWords = ((size_t)1u << p.bits) * p.subtiles;
   p.keyBits = keyBits;
   return p;

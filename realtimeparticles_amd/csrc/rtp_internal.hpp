@@ -128,25 +128,44 @@ struct SortPlan {
   uint32_t n = 0;          // keys to sort
   int passes = 0;
   int bits = 0;            // digit width
-  uint32_t subtiles = 0;   // wave-owned subtiles of kSortItems*64 keys
+  int items = 32;          // keys per lane of a subtile (sort_items)
+  uint32_t subtiles = 0;   // wave-owned subtiles of items*64 keys
   size_t countWords = 0;   // radix * subtiles
   int keyBits = 0;         // significant bits of the largest key
 };
 // Sorts of at most this many keys run in one workgroup, in LDS (all passes in one launch).
 constexpr uint32_t kSmallSortMax = 8192;
-constexpr int kSortItems = 32;  // keys per lane in one subtile
+constexpr int kSortItems = 32;  // keys per lane of a subtile, for 2^20 keys and more
+// Keys per lane of a subtile for a sort of n keys: smaller subtiles below 2^20 keys, so a
+// pass has about a thousand waves or more instead of n / 2048.
+constexpr int sort_items(uint32_t n)
+{
+  return n < (1u << 18) ? 4 : (n < (1u << 20) ? 8 : kSortItems);
+}
 // Words of the count table one block of the multi-block scan owns (256 threads x 16 words).
 constexpr uint32_t kScanSeg = 4096;
-// Words of the counts buffer a sort of up to maxN keys needs (any digit width): the
+// Words of the counts buffer for n keys in subtiles of items*64 (any digit width): the
 // digit-major count table (at most 2048 digits x subtiles) followed by its scan's segment sums.
-constexpr size_t count_capacity(uint32_t maxN)
+constexpr size_t count_words(uint32_t n, int items)
 {
-  const size_t per = (size_t)kSortItems * 64;
-  const size_t subtiles = maxN ? (maxN + per - 1) / per : 1;
+  const size_t per = (size_t)items * 64;
+  const size_t subtiles = n ? (n + per - 1) / per : 1;
   const size_t table = (size_t)2048 * subtiles;
   return table + table / kScanSeg + 64;
 }
-SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride);
+// ... for a sort of up to maxN keys with the subtiles sort_items picks: the largest table
+// of each size class up to maxN.
+constexpr size_t count_capacity(uint32_t maxN)
+{
+  size_t w = count_words(maxN, sort_items(maxN));
+  const uint32_t a = maxN < (1u << 18) ? maxN : (1u << 18) - 1u;
+  const uint32_t b = maxN < (1u << 20) ? maxN : (1u << 20) - 1u;
+  w = w > count_words(a, sort_items(a)) ? w : count_words(a, sort_items(a));
+  w = w > count_words(b, sort_items(b)) ? w : count_words(b, sort_items(b));
+  return w;
+}
+// fixedItems: 0 for sort_items(n), else the keys per lane of every subtile (4, 8 or 32).
+SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride, int fixedItems = 0);
 // keys/idx ping-pong through (keyA, idxA) <-> (keyB, idxB); returns whether the sorted
 // result ended in the B buffers.  idxIn == nullptr means the identity permutation.
 int sort_pairs(const SortPlan& plan, uint32_t* keyA, uint32_t* keyB, uint32_t* idxA,
