@@ -325,6 +325,7 @@ int dispatch_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint3
   switch (p.items) {
   case 4: return dispatch_bits<4>(p, shift, kin, iin, kout, iout, counts, s);
   case 8: return dispatch_bits<8>(p, shift, kin, iin, kout, iout, counts, s);
+  case 16: return dispatch_bits<16>(p, shift, kin, iin, kout, iout, counts, s);
   case kSortItems: return dispatch_bits<kSortItems>(p, shift, kin, iin, kout, iout, counts, s);
   default: return fail(RTP_ERR_INVALID, "sort subtile size out of range");
   }
@@ -437,7 +438,7 @@ SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride, int fixed
   p.bits = (keyBits + p.passes - 1) / p.passes;
   if (p.bits < 4)  // the count table scan works on whole 16-byte groups
     p.bits = 4;
-  p.items = fixedItems ? fixedItems : sort_items(n);
+  p.items = fixedItems ? fixedItems : sort_items(n);  // 4, 8, 16 or 32
   p.subtiles = (n + (uint32_t)(p.items * 64) - 1u) / (uint32_t)(p.items * 64);
   p.countWords = ((size_t)1u << p.bits) * p.subtiles;
   p.keyBits = keyBits;

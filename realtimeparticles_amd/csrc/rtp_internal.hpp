@@ -135,12 +135,12 @@ struct SortPlan {
 };
 // Sorts of at most this many keys run in one workgroup, in LDS (all passes in one launch).
 constexpr uint32_t kSmallSortMax = 8192;
-constexpr int kSortItems = 32;  // keys per lane of a subtile, for 2^20 keys and more
-// Keys per lane of a subtile for a sort of n keys: smaller subtiles below 2^20 keys, so a
+constexpr int kSortItems = 32;  // keys per lane of a subtile, for 2^22 keys and more
+// Keys per lane of a subtile for a sort of n keys: smaller subtiles below 2^22 keys, so a
 // pass has about a thousand waves or more instead of n / 2048.
 constexpr int sort_items(uint32_t n)
 {
-  return n < (1u << 18) ? 4 : (n < (1u << 20) ? 8 : kSortItems);
+  return n < (1u << 18) ? 4 : (n < (1u << 20) ? 8 : (n < (1u << 22) ? 16 : kSortItems));
 }
 // Words of the count table one block of the multi-block scan owns (256 threads x 16 words).
 constexpr uint32_t kScanSeg = 4096;
@@ -160,11 +160,13 @@ constexpr size_t count_capacity(uint32_t maxN)
   size_t w = count_words(maxN, sort_items(maxN));
   const uint32_t a = maxN < (1u << 18) ? maxN : (1u << 18) - 1u;
   const uint32_t b = maxN < (1u << 20) ? maxN : (1u << 20) - 1u;
+  const uint32_t d = maxN < (1u << 22) ? maxN : (1u << 22) - 1u;
   w = w > count_words(a, sort_items(a)) ? w : count_words(a, sort_items(a));
   w = w > count_words(b, sort_items(b)) ? w : count_words(b, sort_items(b));
+  w = w > count_words(d, sort_items(d)) ? w : count_words(d, sort_items(d));
   return w;
 }
-// fixedItems: 0 for sort_items(n), else the keys per lane of every subtile (4, 8 or 32).
+// fixedItems: 0 for sort_items(n), else the keys per lane of every subtile (4, 8, 16 or 32).
 SortPlan make_sort_plan(uint32_t n, uint32_t maxKey, int bitsOverride, int fixedItems = 0);
 // keys/idx ping-pong through (keyA, idxA) <-> (keyB, idxB); returns whether the sorted
 // result ended in the B buffers.  idxIn == nullptr means the identity permutation.

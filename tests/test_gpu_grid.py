@@ -9,9 +9,10 @@ pytestmark = pytest.mark.gpu
 
 
 # 4096 / 4097 and 8192 / 8193 straddle the one-workgroup sort's chunk sizes and its limit;
-# 2^18 and 2^20 the multi-launch sort's subtile sizes (4, 8 and 32 keys per lane)
+# 2^18, 2^20 and 2^22 the multi-launch sort's subtile sizes (4, 8, 16 and 32 keys per lane)
 @pytest.mark.parametrize("n", [1, 63, 64, 2047, 2048, 2049, 4096, 4097, 8191, 8192, 8193, 65536,
-                               100003, (1 << 18) - 1, 1 << 18, (1 << 20) - 1, 1 << 20])
+                               100003, (1 << 18) - 1, 1 << 18, (1 << 20) - 1, 1 << 20,
+                               (1 << 22) - 1, 1 << 22])
 @pytest.mark.parametrize("hi", [2, 1 << 9, 1 << 18, (1 << 32) - 1])
 def test_sort_pairs_is_stable_sort(n, hi):
     log(f"sort_pairs n={n} hi={hi}")
