@@ -133,7 +133,7 @@ struct SortPlan {
   size_t countWords = 0;   // radix * subtiles
   int keyBits = 0;         // significant bits of the largest key
 };
-// Sorts of at most this many keys run in one workgroup, in LDS (all passes in one launch).
+// The one-workgroup LDS sort (all passes in one launch) takes at most this many keys.
 constexpr uint32_t kSmallSortMax = 8192;
 constexpr int kSortItems = 32;  // keys per lane of a subtile, for 2^22 keys and more
 // Keys per lane of a subtile for a sort of n keys: smaller subtiles below 2^22 keys, so a
