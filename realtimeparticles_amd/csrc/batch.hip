@@ -54,7 +54,7 @@ struct rtp_batch {
   uint32_t* okey = nullptr;
   uint32_t* orank = nullptr;
   uint32_t* oorder[2] = { nullptr, nullptr };  // thread orders, each counted in the other
-  uint32_t* ocount = nullptr;
+  CountTables ocount;  // the thread order's count tables
   uint32_t* opart = nullptr;
   // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled, for nbThreads launch
   // threads (every block size's launch fits)
@@ -74,6 +74,7 @@ struct rtp_batch {
   bool useGraph = true;
   bool graphValid = false;
   int graphCur = -1;
+  int graphCount = -1;  // ocount.cur at the capture
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
 
@@ -114,7 +115,8 @@ struct rtp_batch {
     dev_free(orank);
     dev_free(oorder[0]);
     dev_free(oorder[1]);
-    dev_free(ocount);
+    dev_free(ocount.table[0]);
+    dev_free(ocount.table[1]);
     dev_free(opart);
     dev_free(nbList);
     dev_free(nbCount);
@@ -218,8 +220,11 @@ struct rtp_batch {
       in = 1 - in;
       if (resort) {
         uint32_t* next = nb.order == oorder[0] ? oorder[1] : oorder[0];
+        bool ran = false;
         RTP_PROF(prof, "thread_order", s,
-                 launch_counting_order(ocount, m16, okey, nb.order, orank, total, opart, next, s));
+                 ran = launch_counting_order(ocount, m16, okey, nb.order, orank, total, opart, next, s));
+        if (ran)
+          ocount.cur = 1 - ocount.cur;
         nb.order = next;
         cb.order = next;
       }
@@ -244,6 +249,7 @@ struct rtp_batch {
   {
     drop_graph();
     graphCur = cur;
+    graphCount = ocount.cur;
     RTP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = substep();
     if (!rc)
@@ -271,7 +277,7 @@ struct rtp_batch {
       return rc;
     uint32_t done = 0;
     if (useGraph && !prof.enabled && nsteps >= 2) {
-      if (!graphValid || graphCur != cur) {
+      if (!graphValid || graphCur != cur || graphCount != ocount.cur) {
         if ((rc = capture_pair()))
           return rc;
       }
@@ -387,11 +393,19 @@ int rtp_batch_create(int device, const rtp_batch_scene* sc, uint32_t count, rtp_
       (rc = dev_alloc(&h->dBlocks, maxBlocks)) || (rc = dev_alloc(&h->okey, N)) ||
       (rc = dev_alloc(&h->orank, N)) || (rc = dev_alloc(&h->oorder[0], N)) ||
       (rc = dev_alloc(&h->oorder[1], N)) ||
-      (rc = dev_alloc(&h->ocount, (h->keySpace + 15u) & ~15u)) ||
+      (rc = dev_alloc(&h->ocount.table[0], (h->keySpace + 15u) & ~15u)) ||
+      (rc = dev_alloc(&h->ocount.table[1], (h->keySpace + 15u) & ~15u)) ||
       (rc = dev_alloc(&h->opart, scan_partials_needed((h->keySpace + 15u) & ~15u)))) {
     h->release();
     delete h;
     return rc;
+  }
+  h->ocount.words = (h->keySpace + 15u) & ~15u;
+  if (hipMemset(h->ocount.table[0], 0, sizeof(uint32_t) * h->ocount.words) != hipSuccess ||
+      hipMemset(h->ocount.table[1], 0, sizeof(uint32_t) * h->ocount.words) != hipSuccess) {
+    h->release();
+    delete h;
+    return fail(RTP_ERR_HIP, "hipMemset failed");
   }
   std::vector<SceneSlot> slots(count);
   for (uint32_t s = 0; s < count; ++s)

@@ -234,6 +234,7 @@ struct rtp_fluids {
   bool useGraph = true;
   bool graphValid = false;
   int graphCur = -1, graphKeyCur = -1, graphPred = -1, graphCol = -1, graphCam = -1;
+  int graphCount = -1;  // ocount.cur at the capture
   RenderState render;  // rtp_fluids_set_render
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
@@ -244,7 +245,7 @@ struct rtp_fluids {
   uint32_t* okey = nullptr;
   uint32_t* orank = nullptr;
   uint32_t* oorder[2] = { nullptr, nullptr };  // thread orders, each counted in the other
-  uint32_t* ocount = nullptr;
+  CountTables ocount;  // the thread order's count tables
   uint32_t* opart = nullptr;
   // survivor lists (RTP_LAUNCH_NB_LIST), allocated when first enabled
   uint4* nbList = nullptr;      // kNbCap / 4 x maxN granules
@@ -346,8 +347,11 @@ struct rtp_fluids {
       if (!resort)
         return RTP_OK;
       uint32_t* next = po.order == oorder[0] ? oorder[1] : oorder[0];
+      bool ran = false;
       RTP_PROF(prof, "thread_order", s,
-               launch_counting_order(ocount, m16, okey, po.order, orank, n, opart, next, s));
+               ran = launch_counting_order(ocount, m16, okey, po.order, orank, n, opart, next, s));
+      if (ran)
+        ocount.cur = 1 - ocount.cur;
       po.order = next;
       pc.order = next;
       return RTP_OK;
@@ -475,6 +479,7 @@ struct rtp_fluids {
     graphPred = predFinal;
     graphCol = render.colCur;
     graphCam = render.keyCur;
+    graphCount = ocount.cur;
     RTP_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int rc = substep();
     if (!rc)
@@ -516,7 +521,7 @@ struct rtp_fluids {
     uint32_t done = 0;
     if (useGraph && !prof.enabled && nsteps >= 2 && n > 0) {
       if (!graphValid || graphCur != cur || graphKeyCur != grid.keyCur || graphPred != predFinal ||
-          graphCol != render.colCur || graphCam != render.keyCur) {
+          graphCol != render.colCur || graphCam != render.keyCur || graphCount != ocount.cur) {
         const int rc = capture_pair();
         if (rc)
           return rc;
@@ -573,7 +578,8 @@ struct rtp_fluids {
     dev_free(orank);
     dev_free(oorder[0]);
     dev_free(oorder[1]);
-    dev_free(ocount);
+    dev_free(ocount.table[0]);
+    dev_free(ocount.table[1]);
     dev_free(opart);
     dev_free(nbList);
     dev_free(nbCount);
@@ -876,11 +882,19 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
       (rc = dev_alloc(&h->resMetric, kMetricFloats)) || (rc = dev_alloc(&h->okey, maxN)) ||
       (rc = dev_alloc(&h->orank, maxN)) || (rc = dev_alloc(&h->oorder[0], maxN)) ||
       (rc = dev_alloc(&h->oorder[1], maxN)) ||
-      (rc = dev_alloc(&h->ocount, order_count_words(h->grid.g))) ||
+      (rc = dev_alloc(&h->ocount.table[0], order_count_words(h->grid.g))) ||
+      (rc = dev_alloc(&h->ocount.table[1], order_count_words(h->grid.g))) ||
       (rc = dev_alloc(&h->opart, scan_partials_needed(order_count_words(h->grid.g))))) {
     h->release();
     delete h;
     return rc;
+  }
+  h->ocount.words = order_count_words(h->grid.g);
+  if (hipMemset(h->ocount.table[0], 0, sizeof(uint32_t) * h->ocount.words) != hipSuccess ||
+      hipMemset(h->ocount.table[1], 0, sizeof(uint32_t) * h->ocount.words) != hipSuccess) {
+    h->release();
+    delete h;
+    return fail(RTP_ERR_HIP, "hipMemset failed");
   }
   h->refresh_consts();
   // Zero state and the reset key tail, like the constructor's reset() (Fluids.cpp:91).

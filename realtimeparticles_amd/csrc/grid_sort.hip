@@ -383,17 +383,22 @@ __global__ __launch_bounds__(kRankThreads) void k_count_ranks(const uint32_t* __
 
 // The particle i of thread t (i = prev[t]) goes to slot offset[key[i]] + rank[t] of the
 // new thread order.
+// ... and the next call's count table cleared (clearWords words of clearNext).
 __global__ __launch_bounds__(256) void k_order_scatter(const uint32_t* __restrict__ key,
                                                        const uint32_t* __restrict__ prev,
                                                        const uint32_t* __restrict__ rank,
                                                        const uint32_t* __restrict__ offset, uint32_t n,
-                                                       uint32_t* __restrict__ order)
+                                                       uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ clearNext,
+                                                       uint32_t clearWords)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
     const uint32_t i = prev ? prev[t] : t;
     order[offset[key[i]] + rank[t]] = i;
   }
+  if (t < clearWords)
+    clearNext[t] = 0u;
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
@@ -406,20 +411,21 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
 }
 
-void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key,
+bool launch_counting_order(const CountTables& ct, uint32_t m16, const uint32_t* key,
                            const uint32_t* prev, uint32_t* rank, uint32_t n, uint32_t* partial,
                            uint32_t* order, hipStream_t s)
 {
   if (n == 0 || m16 == 0)
-    return;
+    return false;
+  uint32_t* count = ct.table[ct.cur];
   const uint32_t nseg = (m16 + kScanSeg - 1u) / kScanSeg;
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(m16, 256)), dim3(256), 0, s, count, m16, 0u);
   hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, kRankThreads)), dim3(kRankThreads), 0, s,
                      key, prev, n, count, rank);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, count, m16, partial);
-  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, key, prev, rank,
-                     count, n, order);
+  hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(std::max(n, ct.words), 256)), dim3(256), 0, s,
+                     key, prev, rank, count, n, order, ct.table[1 - ct.cur], ct.words);
+  return true;
 }
 
 uint32_t scan_partials_needed(uint32_t m16) { return (m16 + kScanSeg - 1u) / kScanSeg; }

@@ -261,13 +261,23 @@ struct PassOrder {
   NbList nb;
   __device__ uint32_t particle(uint32_t t) const { return order ? order[t] : t; }
 };
-// Counting sort of particles by cell for the thread order: count[] (m16 entries, a
-// multiple of 16 covering every key) is cleared, counts the particles of each cell in the
+// The thread order's two count tables of `words` words each, both all zero when allocated.
+// A call counts in table[cur] (all zero on entry) and clears table[1 - cur] for the next
+// call; the caller flips cur after every call that ran (a hipGraph replays its calls from
+// the cur it was captured at, so a caller recaptures when cur differs).
+struct CountTables {
+  uint32_t* table[2] = { nullptr, nullptr };
+  uint32_t words = 0;
+  int cur = 0;
+};
+// Counting sort of particles by cell for the thread order: the count table (m16 <= words
+// entries, a multiple of 16 covering every key) counts the particles of each cell in the
 // previous thread order prev (the identity when null; ranks into rank[t]) and becomes its
 // exclusive scan; then order[scan[key[i]] + rank[t]] = i for i = prev[t].  order must not
-// be prev.  partial holds scan_partials_needed(m16) words.
+// be prev.  partial holds scan_partials_needed(m16) words.  Returns whether it launched
+// (n and m16 nonzero): then the caller flips ct.cur.
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
-void launch_counting_order(uint32_t* count, uint32_t m16, const uint32_t* key,
+bool launch_counting_order(const CountTables& ct, uint32_t m16, const uint32_t* key,
                            const uint32_t* prev, uint32_t* rank, uint32_t n, uint32_t* partial,
                            uint32_t* order, hipStream_t s);
 uint32_t scan_partials_needed(uint32_t m16);

@@ -118,7 +118,17 @@ def test_batch_graph_replay_and_tuning_do_not_change_bits():
             p, v = b.read_state(k)
             _eq(p, ref[k][0], f"scene {k} pos (block {block}, bits {bits})")
             _eq(v, ref[k][1], f"scene {k} vel (block {block}, bits {bits})")
+    # graph replays between direct substeps with the thread order on: three iterations order
+    # the threads three times a substep, so a direct substep turns the count tables round and
+    # the graph is captured again
     b.use_graph(True)
+    b.restore_checkpoint()
+    b.set_tuning(64, 0, 16, capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST)
+    for k in (1, 2, 1):
+        log(f"thread order, graph and direct: update({k})")
+        b.update(k)
+    for k in range(len(scs)):
+        _eq(b.read_state(k)[0], ref[k][0], f"scene {k} pos, graph and direct with thread order")
     b.restore_checkpoint()
     best = b.autotune(trial_steps=2)
     log(f"autotuned: {best}")

@@ -94,6 +94,31 @@ def test_graph_replay_equals_direct_launches(flags):
     b.close()
 
 
+@pytest.mark.parametrize("flags", [13, 29, 76])
+def test_graph_replay_with_odd_thread_order_calls(flags):
+    """Three Jacobi iterations: a substep orders the threads three times, so a direct substep
+    leaves the thread order's count tables the other way round from the graph's capture; the
+    graph must be captured again before its next replay.  Graph and direct runs alternate and
+    every state equals a direct-only run's."""
+    sc = small_scene(seed=9)
+    a, _ = make_pbf_pair(sc, iters=3)
+    b, _ = make_pbf_pair(sc, iters=3)
+    a.set_tuning(64, flags, 0, 16)
+    b.set_tuning(64, flags, 0, 16)
+    b.use_graph(False)
+    for rep, k in enumerate((2, 1, 2, 3, 4, 1, 1, 2)):
+        log(f"odd thread-order calls, flags {flags}, update({k})")
+        a.update(k)
+        for _ in range(k):
+            b.update(1)
+        pa, va = a.read_state()
+        pb, vb = b.read_state()
+        _eq(pa, pb, f"pos after update({k}) #{rep}")
+        _eq(va, vb, f"vel after update({k}) #{rep}")
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("flags,sort_bits", [(0, 0), (13, 0), (0, 7), (13, 10), (13, 11), (29, 9),
                                              (76, 0), (92, 9)])
 def test_nb_particles_change_keeps_reference_sort(flags, sort_bits):
