@@ -154,7 +154,8 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 }
 
 // Each block's base is the sum of the segment sums before it, summed here: the segments
-// are few (m / kScanSeg), so this costs less than a launch of its own.
+// are few (m / kScanSeg), so this costs less than a launch of its own.  Block 0 reads none,
+// so a table of one segment needs no k_scan_reduce.
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ data, uint32_t m,
                                                     const uint32_t* __restrict__ partial)
 {
@@ -295,7 +296,8 @@ int run_pass(const SortPlan& p, int shift, const uint32_t* kin, const uint32_t* 
   const uint32_t m = (uint32_t)((size_t)(1u << BITS) * p.subtiles);
   const uint32_t nseg = (m + kScanSeg - 1u) / kScanSeg;
   uint32_t* partial = counts + (size_t)2048 * p.subtiles;  // past the largest table for n
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, counts, m, partial);
+  if (nseg > 1)  // one segment: its apply block reads no segment sums
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, counts, m, partial);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, counts, m, partial);
   hipLaunchKernelGGL((k_sort_downsweep<BITS, ITEMS>), dim3(blocks), dim3(256), 0, s, kin, iin,
                      p.n, shift, p.subtiles, counts, kout, iout);
@@ -421,7 +423,8 @@ bool launch_counting_order(const CountTables& ct, uint32_t m16, const uint32_t* 
   const uint32_t nseg = (m16 + kScanSeg - 1u) / kScanSeg;
   hipLaunchKernelGGL(k_count_ranks, dim3(blocks_for(n, kRankThreads)), dim3(kRankThreads), 0, s,
                      key, prev, n, count, rank);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
+  if (nseg > 1)  // one segment: its apply block reads no segment sums
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(256), 0, s, count, m16, partial);
   hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(256), 0, s, count, m16, partial);
   hipLaunchKernelGGL(k_order_scatter, dim3(blocks_for(std::max(n, ct.words), 256)), dim3(256), 0, s,
                      key, prev, rank, count, n, order, ct.table[1 - ct.cur], ct.words);
