@@ -395,11 +395,11 @@ struct rtp_fluids {
       residual();
       RTP_PROF(prof, "pbf_fallback_set", s,
                launch_fallback_set(pred[in], G.startEnd, resDensity, tau, n, c, fb, s));
-      // in place in pred[in]; the partner buffer holds the set's corrections in between
+      // ending in pred[in]; the iterations alternate between it and its partner buffer
       for (uint32_t f = 0; f < fallbackIters; ++f)
         RTP_PROF(prof, "pbf_fallback", s,
                  launch_fallback_iteration(pred[in], pred[1 - in], density, lambda, G.startEnd, n,
-                                           f == 0, f + 1 == fallbackIters, c, L, fb,
+                                           f, f + 1 == fallbackIters, c, L, fb,
                                            lists ? &po.nb : nullptr, s));
     }
     if (residualPass)  // after any fallback iterations: the error the substep leaves

@@ -983,21 +983,26 @@ __global__ __launch_bounds__(1024) void k_fb_compact(uint32_t n, uint32_t flag,
 }
 
 // The start of the fallback iterations when the set is not empty: fld_applyBoundaryCondition
-// on every particle, in place (later iterations keep the outsiders' positions, and clamp the
-// set's own corrections), and lambda 0 outside the set (the set's own is written by the
-// density pass).  An empty set leaves the substep as it was.
+// on every particle, into pred and its partner (the iterations alternate between the two:
+// later ones keep the outsiders' positions, and clamp the set's own corrections), and lambda
+// 0 outside the set (the set's own is written by the density pass).  An empty set leaves the
+// substep as it was.
 __global__ __launch_bounds__(256) void k_fb_begin(float4* __restrict__ pred,
+                                                  float4* __restrict__ partner,
                                                   float* __restrict__ lambda, uint32_t n,
                                                   PbfConsts c, FallbackSet fs)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || fs.count[1] == 0u)
     return;
-  pred[t] = clamp_pos(pred[t], c);
+  const float4 q = clamp_pos(pred[t], c);
+  pred[t] = q;
+  partner[t] = q;
   lambda[t] = 0.0f;
 }
 
-// The set's corrected positions, computed into the partner buffer, back into place.
+// The set's corrected positions, left in the partner buffer by an odd number of iterations,
+// back into place.
 __global__ __launch_bounds__(256) void k_fb_scatter(float4* __restrict__ pred,
                                                     const float4* __restrict__ corrected,
                                                     FallbackSet fs)
@@ -2180,17 +2185,21 @@ void launch_fallback_set(const float4* pred, const uint2* startEnd, const float*
   hipLaunchKernelGGL(k_fb_compact, cg, dim3(1024), 0, s, n, 2u, fs.act, fs.halo, fs.count + 2);
 }
 
-void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,
-                               const uint2* startEnd, uint32_t n, bool first, bool last,
+void launch_fallback_iteration(float4* pred, float4* partner, float* density, float* lambda,
+                               const uint2* startEnd, uint32_t n, uint32_t f, bool last,
                                const PbfConsts& c, const Launch& L, const FallbackSet& fs,
                                const NbList* nb, hipStream_t s)
 {
   if (n == 0)
     return;
-  if (first)
-    hipLaunchKernelGGL(k_fb_begin, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, lambda, n, c,
-                       fs);
+  if (f == 0)
+    hipLaunchKernelGGL(k_fb_begin, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, partner,
+                       lambda, n, c, fs);
   (void)L;
+  // iteration f reads one buffer and writes the set's corrections into the other: pred for
+  // odd f, the partner for even f; both hold the same positions outside the set
+  const float4* src = (f & 1u) ? partner : pred;
+  float4* dst = (f & 1u) ? pred : partner;
   // a quad per particle, at most 2^16 particles per round of the grid-stride loop
   const uint32_t qb = blocks_for(n, kQuadGroups), qcap = (1u << 16) / kQuadGroups;
   const dim3 g(qb < qcap ? qb : qcap), b(kQuadBlock);
@@ -2198,17 +2207,19 @@ void launch_fallback_iteration(float4* pred, float4* scratch, float* density, fl
   const int lastMode = last ? kLastOnly : kNextIter;
   // the correction replays the density pass's lists: the same positions
   if (nb) {
-    hipLaunchKernelGGL(k_density_lambda_quad<true>, g, b, 0, s, pred, startEnd, density, lambda,
+    hipLaunchKernelGGL(k_density_lambda_quad<true>, g, b, 0, s, src, startEnd, density, lambda,
                        c, fs, lists);
-    hipLaunchKernelGGL(k_correct_quad<true>, g, b, 0, s, pred, lambda, startEnd, scratch,
-                       lastMode, c, fs, lists);
+    hipLaunchKernelGGL(k_correct_quad<true>, g, b, 0, s, src, lambda, startEnd, dst, lastMode, c,
+                       fs, lists);
   } else {
-    hipLaunchKernelGGL(k_density_lambda_quad<false>, g, b, 0, s, pred, startEnd, density, lambda,
+    hipLaunchKernelGGL(k_density_lambda_quad<false>, g, b, 0, s, src, startEnd, density, lambda,
                        c, fs, lists);
-    hipLaunchKernelGGL(k_correct_quad<false>, g, b, 0, s, pred, lambda, startEnd, scratch,
-                       lastMode, c, fs, lists);
+    hipLaunchKernelGGL(k_correct_quad<false>, g, b, 0, s, src, lambda, startEnd, dst, lastMode,
+                       c, fs, lists);
   }
-  hipLaunchKernelGGL(k_fb_scatter, dim3(list_blocks(n, 256)), dim3(256), 0, s, pred, scratch, fs);
+  if (last && dst == partner)  // an odd number of iterations: the result is in the partner
+    hipLaunchKernelGGL(k_fb_scatter, dim3(list_blocks(n, 256)), dim3(256), 0, s, pred, partner,
+                       fs);
 }
 
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,

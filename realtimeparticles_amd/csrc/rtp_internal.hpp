@@ -351,13 +351,15 @@ struct FallbackSet {
 void launch_fallback_set(const float4* pred, const uint2* startEnd, const float* resDensity,
                          float tau, uint32_t n, const PbfConsts& c, const FallbackSet& fs,
                          hipStream_t s);
-// One plain Jacobi iteration (fluids.cl:80-258) for the particles of the set, in place in
-// pred: the first (when the set is not empty) clamps every particle to the walls and sets
-// the lambda of every particle to 0; density and lambda are computed for the set and the
-// halo, and the set's corrections into scratch (indexed by particle), then written back.
-// nb: survivor lists recorded by the density pass and replayed by the correction.
-void launch_fallback_iteration(float4* pred, float4* scratch, float* density, float* lambda,
-                               const uint2* startEnd, uint32_t n, bool first, bool last,
+// Plain Jacobi iteration f (fluids.cl:80-258) for the particles of the set, ending in pred:
+// the first (when the set is not empty) clamps every particle to the walls, into pred and
+// partner, and sets the lambda of every particle to 0; density and lambda are computed for
+// the set and the halo, and the set's corrections written into the other buffer (iterations
+// alternate between pred and partner); the last one of an odd count copies the set's final
+// positions back into pred.  nb: survivor lists recorded by the density pass and replayed
+// by the correction.
+void launch_fallback_iteration(float4* pred, float4* partner, float* density, float* lambda,
+                               const uint2* startEnd, uint32_t n, uint32_t f, bool last,
                                const PbfConsts& c, const Launch& L, const FallbackSet& fs,
                                const NbList* nb, hipStream_t s);
 void launch_pbf_update_vel(const float4* pred, const float4* pos, float4* vel, float4* posOut,

@@ -156,19 +156,21 @@ def _threshold_between(sc, prm, mlp):
     return float(r[1] + 0.5 * (r[0] - r[1]))
 
 
+@pytest.mark.parametrize("fb", [3, 4])
 @pytest.mark.parametrize("shape", ["default", "tuned"])
 @pytest.mark.parametrize("gate", ["none", "some", "all"])
-def test_learned_substep_matches_oracle(gate, shape):
-    """Learned-1 with 3 fallback iterations against the oracle's learned substep, bit for
-    bit on every field: no particle above tau (the substep equals one without fallback),
-    some, and every particle (the fallback is plain iterations)."""
+def test_learned_substep_matches_oracle(gate, shape, fb):
+    """Learned-1 with 3 or 4 fallback iterations (the iterations alternate between two
+    position buffers; an odd count copies the set back) against the oracle's learned
+    substep, bit for bit on every field: no particle above tau (the substep equals one
+    without fallback), some, and every particle (the fallback is plain iterations)."""
     sc = small_scene(22)
     prm = sc.params("stable")
     mlp = _small_net(5)
     tau = {"none": 1e30, "all": -1.0}.get(gate) or _threshold_between(sc, prm, mlp)
     tuning = (64, capi.LAUNCH_XCD_MAP | capi.LAUNCH_RESORT | capi.LAUNCH_NB_LIST, 0, 16) \
         if shape == "tuned" else None
-    gpu, orc, act = _learned_run(sc, prm, mlp, 1, 3, tau, tuning)
+    gpu, orc, act = _learned_run(sc, prm, mlp, 1, fb, tau, tuning)
     log(f"gate {gate} (tau {tau:.4g}), {shape}: oracle set {int(((act == 1) | (act == 2)).sum())} "
         f"of {sc.n}, {int((act == 2).sum())} seeds, halo {int((act == 3).sum())}; "
         f"GPU {gpu.fallback_sizes()}")
