@@ -115,7 +115,9 @@ typedef struct {
                                   give each particle four lanes (sixteen particles per 64-lane
                                   workgroup) that split its walk and add the terms in the same
                                   order; for scenes too small to fill the GPU one lane per
-                                  particle (results unchanged; not with RTP_LAUNCH_LDS_TILES) */
+                                  particle (results unchanged; not with RTP_LAUNCH_LDS_TILES).
+                                  The default shape of a model of at most 262,144 particles
+                                  (maxNbParticles) has it; set_tuning / autotune replace it */
 
 /* Which buffers rtp_fluids_read_field copies. */
 typedef enum {
@@ -416,7 +418,8 @@ int rtp_boids_set_dimension(rtp_boids* h, int dim);
 /* Launch shape of the boids step (no reference counterpart): the block size and, of the
  * flags, RTP_LAUNCH_XCD_MAP, RTP_LAUNCH_PERSISTENT and RTP_LAUNCH_QUAD (the step launch with
  * a quad of four lanes per boid, sixteen boids per 64-lane workgroup, which ignores the block
- * size).  Results do not depend on it.  Default 256 threads. */
+ * size).  Results do not depend on it.  Default 256 threads, with RTP_LAUNCH_QUAD for at most
+ * 262,144 boids (maxNbParticles). */
 int rtp_boids_set_tuning(rtp_boids* h, const rtp_tuning* t);
 /* Model::setBoundary (physics/Model.hpp:116-119, Boundary :38-44): the wall of bd_updatePosAndApplyWallBC
  * (bouncing, the default) or of bd_updatePosAndApplyPeriodicBC (cyclic, boids.cl:289-315),

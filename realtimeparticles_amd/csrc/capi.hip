@@ -866,6 +866,7 @@ int rtp_fluids_create(const rtp_model_desc* d, rtp_fluids** out)
   }
   init_pbf_consts(h->c, h->grid.g, d->boxSize[0], d->gridRes[0], d->gravity);
   h->n = d->currNbParticles;
+  h->L.quad = d->maxNbParticles <= kQuadDefaultMaxN ? 1 : 0;  // the default shape's walks
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(RTP_ERR_HIP, "hipStreamCreate failed");
@@ -1370,6 +1371,7 @@ int rtp_boids_create(const rtp_model_desc* d, rtp_boids** out)
   h->c.shortDiv = h->c.radiusSq <= 0x1p30f ? 1 : 0;
   h->c.vmin = 0.2f * h->c.p.velocityScale;
   h->n = d->currNbParticles;
+  h->L.quad = d->maxNbParticles <= kQuadDefaultMaxN ? 1 : 0;  // the default shape's walk
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(RTP_ERR_HIP, "hipStreamCreate failed");

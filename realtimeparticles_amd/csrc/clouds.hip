@@ -502,7 +502,7 @@ __device__ __forceinline__ bool finite4(float4 v)
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
 // ------------------------------------------------------------------------ quad walks
-// With quads (the default for scenes up to kCldQuadMaxN particles) each walk kernel gives a
+// With quads (the default for scenes up to kQuadDefaultMaxN particles) each walk kernel gives a
 // particle four lanes of a 64-lane workgroup.  The quad builds the particle's ranges
 // (cld_ranges) in LDS and scans them sixteen candidates per step, lane u taking candidates
 // u, u + 4, u + 8 and u + 12 with the pass's keep test; the kept ones go to a queue of
@@ -513,7 +513,6 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 // cld_walk's layout and codes, so either kind of walk replays the other's lists.
 constexpr uint32_t kCqGroups = kCldLanes / 4u;  // particles per workgroup
 constexpr uint32_t kCqQueue = 64;               // queued survivors per particle
-constexpr uint32_t kCldQuadMaxN = 1u << 18;     // above this, one lane per particle fills the GPU
 
 struct CqLds {
   uint32_t queue[kCqGroups][kCqQueue];
@@ -1600,7 +1599,7 @@ int rtp_clouds_create(const rtp_model_desc* d, rtp_clouds** out)
            (rc = dev_alloc(&h->listT.count, maxN)));
   h->list.stride = maxN;
   h->listT.stride = maxN;
-  h->quad = maxN <= kCldQuadMaxN;
+  h->quad = maxN <= kQuadDefaultMaxN;
   if (!rc && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
               hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess))

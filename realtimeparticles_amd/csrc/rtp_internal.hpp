@@ -118,6 +118,9 @@ struct Launch {
   int bricks = 0;      // 1 (with resort): thread order by brick_key instead of the cell index
   int quad = 0;        // 1: the plain solver's neighbour passes run a quad of lanes per particle
 };
+// Models of at most this many particles start with quads (Launch::quad): below it one lane
+// per particle leaves most SIMDs with a single wave or none.
+constexpr uint32_t kQuadDefaultMaxN = 1u << 18;
 
 // Tile edge, in cells, of the (y, z) pencils of the brick thread order.
 constexpr uint32_t kBrick = 8;
